@@ -1,0 +1,146 @@
+"""Benchmark of the Feynman–Kac hot path on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload point|rows]
+
+One step = one pass of the hot path over the whole workload with inputs resident
+in HBM.  Prints one JSON line (metric path-steps/s) with a roofline object for
+the path kernel (HIP events on the kernel's stream, measured inside the library)
+and the CPU restatement timed on the host cores over a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from spac_amd import abi  # noqa: E402
+from spac_amd.fk import FKProblem, SolveBatchArrays, problem_desc  # noqa: E402
+from spac_amd.flops import FLOPS_PER_STEP, PEAK_FP64_TFLOPS  # noqa: E402
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def config1_points():
+    t = -1.0 + 2.0 * (np.arange(16) + 1) / 17
+    X, Y = np.meshgrid(t, t, indexing="ij")
+    return X.ravel().copy(), Y.ravel().copy()
+
+
+def cpu_baseline_points(x, y, paths, h, seed, budget_s=20.0):
+    """The CPU restatement (OpenMP over paths) on a bounded sample of the workload."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as o
+
+    threads = min(16, os.cpu_count() or 1)
+    desc = problem_desc()
+    # calibrate on 2 points x 500 paths, then size the sample to ~budget_s
+    idx = np.arange(0, x.size, max(1, x.size // 2))[:2]
+    b = SolveBatchArrays(x[idx], y[idx], 500, h, idx)
+    t0 = time.perf_counter()
+    rec = o.simulate(desc, b, seed=seed, nthreads=threads)
+    dt = time.perf_counter() - t0
+    rate = rec["steps"].astype(np.int64).sum() / dt
+    per_point = rate and (rec["steps"].mean() * paths) / rate
+    npts = int(max(1, min(x.size, budget_s / max(per_point, 1e-9))))
+    idx = np.linspace(0, x.size - 1, npts).astype(np.int64)
+    b = SolveBatchArrays(x[idx], y[idx], paths, h, idx)
+    t0 = time.perf_counter()
+    rec = o.simulate(desc, b, seed=seed, nthreads=threads)
+    dt = time.perf_counter() - t0
+    steps = int(rec["steps"].astype(np.int64).sum())
+    return {"value": steps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{npts} of the 256 grid points x {paths} paths, h={h} ({steps} path-steps, {dt:.1f} s)"}
+
+
+def bench_points(args):
+    import torch
+
+    x, y = config1_points()
+    n, paths, h, seed = x.size, 10000, 1e-3, 0
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    dy = torch.from_numpy(y).to(dev)
+    dstats = torch.zeros(n * abi.C.sizeof(abi.PointStats), dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    prob = FKProblem(problem_desc())
+    lib = prob.lib
+
+    def step():
+        abi.check(lib.spac_fk_solve_points_device(prob.h, dx.data_ptr(), dy.data_ptr(), n, paths, h, seed,
+                                                  dstats.data_ptr()), lib)
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    kern_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        kern_ms.append(prob.timing().path_kernel_ms)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s")
+    host = dstats.cpu().numpy().view(np.uint8)
+    stats = (abi.PointStats * n).from_buffer_copy(host.tobytes())
+    path_steps = int(sum(s.path_steps for s in stats))
+    assert all(s.status == 0 for s in stats)
+    value = path_steps * args.steps / elapsed
+    avg_ms = float(np.mean(kern_ms))
+    fps = FLOPS_PER_STEP["franke_point_fp64"]
+    achieved = path_steps * fps / (avg_ms * 1e-3) / 1e12
+    out = {
+        "metric": "path-steps/s",
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain Feynman-Kac FP64, 2D Poisson with Franke data on [-1,1]^2, 16x16 interior grid, "
+                               "10000 paths/point, h=1e-3, sigma=sqrt(2) I, Philox seed 0",
+                   "points": n, "paths_per_point": paths, "h": h, "path_steps_per_step": path_steps},
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "fk_path_kernel_fp64<CoefFranke, SUB_NONE>", "kernel_avg_ms": avg_ms,
+                     "flops_per_path_step": fps},
+    }
+    prob.close()
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline_points(x, y, paths, h, seed, budget_s=args.cpu_seconds)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="points", choices=["points"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by node batches, "
+              "see the sharding note in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    out = bench_points(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,241 @@
+/*
+ * spac_fk.h — C ABI of the MI355X Feynman–Kac path solver and interface-row
+ * assembly.  Plain C types only (no STL, no Eigen, no torch): host pointers,
+ * sizes and SoA arrays.  The library owns device buffers and one HIP stream per
+ * handle; the caller owns every host buffer passed in or out.
+ *
+ * Reference interfaces replaced (SpAc, paths relative to the reference root):
+ *   FeynmanKacSolver::Solve_OMP_Analytic(Vector2d X0, unsigned N, double h,
+ *       double rho, BVP)                        Solvers/FeynmanKacSolver.hpp:95-96
+ *   FeynmanKacSolver::Solve_OMP_Analytic(Nudo&, unsigned N, double h,
+ *       double rho, BVP, Interfaz)              Solvers/FeynmanKacSolver.hpp:97-98
+ *   public estimator fields phi, phiphi, phi_VR, phiphi_VR, xi, xixi, xiphi,
+ *       APL, RNGC                               Solvers/FeynmanKacSolver.hpp:80-83
+ *   per-path exit data X_tau, Y_tau, Z_tau, tau, xi_tau, outcome_tau
+ *                                               Solvers/FeynmanKacSolver.hpp:46-49
+ *   Interfaz::Update_G (G row weights)          Mesh/Interfaz.cpp:219-264
+ *
+ * Errors: every call returns spac_status (0 = OK, negative = error) and never
+ * prints or exits; spac_last_error() gives a thread-local message.
+ * Threading: calls on one handle are serialised by the caller; different
+ * handles may be used from different threads.
+ */
+#ifndef SPAC_FK_H
+#define SPAC_FK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPAC_ABI_VERSION 1
+
+typedef int32_t spac_status;
+enum {
+    SPAC_OK = 0,
+    SPAC_E_INVALID_ARG = -1,
+    SPAC_E_UNREGISTERED_PLUGIN = -2, /* coefficient/boundary with no device twin */
+    SPAC_E_MAX_STEPS = -3,           /* at least one path hit max_steps */
+    SPAC_E_NAN = -4,                 /* a NaN reached an estimator */
+    SPAC_E_HIP = -5,                 /* HIP runtime error */
+    SPAC_E_PATTERN = -6,             /* G column outside the CSR pattern */
+    SPAC_E_BAD_INTERFACE = -7,       /* unknown interface code (Mesh/Interfaz.cpp:248-250) */
+    SPAC_E_NO_DEVICE = -8,
+    SPAC_E_OUT_OF_MEMORY = -9
+};
+
+/* Coefficient sets with a device implementation.  Each is the fused device
+ * twin of a set of host plugin functions (u, g, c, f, psi, varphi, b, gradu,
+ * F, mu, sigma) registered by the C++ plugin surface. */
+enum {
+    SPAC_COEF_FRANKE_POISSON = 1, /* BVPs/Franke_Poisson.hpp: sigma = 1.41421356237 I, b = c = 0 */
+    SPAC_COEF_VARCOEF_FRANKE = 2  /* sigma = (1 + 0.5 sin(pi x) cos(pi y)) I, b = 0.3 (y, -x),
+                                     c = -1 - x^2, u = Franke, f = -(L u) */
+};
+
+/* Boundary distance plugins (Dominios/Rectangulo.hpp:7-229, Dominios/Circulo.hpp:7-22). */
+enum {
+    SPAC_BOUNDARY_NONE = 0,      /* no subdomain: distance -inf, never the nearest */
+    SPAC_BOUNDARY_RECTANGLE = 1, /* params: xmin, ymin, xmax, ymax */
+    SPAC_BOUNDARY_CIRCLE = 2     /* params: R, cx, cy */
+};
+
+enum { SPAC_FP64 = 0, SPAC_FP32 = 1 };
+
+/* Path outcome (Solvers/FeynmanKacSolver.hpp:41) plus the step cap. */
+enum {
+    SPAC_OUT_INSIDE = 0,
+    SPAC_OUT_DOMAIN = 1,    /* parada_dominio */
+    SPAC_OUT_SUBDOMAIN = 2, /* parada_subdominio */
+    SPAC_OUT_REFLECTED = 3,
+    SPAC_OUT_MAX_STEPS = 4
+};
+
+/* Interpolation of the G row over a circle's nodes (Mesh/Interfaz.cpp:206-264). */
+enum {
+    SPAC_RBF_IMQ_GLOBAL = 0,    /* reference: IMQ over all arc nodes, iPsi = Psi^-1 */
+    SPAC_RBF_GAUSS_STENCIL = 1  /* Gaussian, local stencils of stencil_size nodes */
+};
+
+typedef struct {
+    int32_t coef_kind;
+    int32_t domain_kind;
+    double domain_params[4];
+    int32_t subdomain_kind;     /* used by the point solve; node solves use their circle */
+    int32_t precision;          /* SPAC_FP64 or SPAC_FP32 stepping */
+    double subdomain_params[4];
+    double gobet_constant;      /* -0.5826 in the reference (Solvers/FeynmanKacSolver.cpp:268,283) */
+    int32_t max_steps;          /* cap on steps per path; the reference has none */
+    int32_t reserved;
+} spac_problem_desc;
+
+typedef struct spac_problem spac_problem;
+
+/* A batch of independent solves: solve k starts paths[k] paths at (x0[k], y0[k])
+ * with step h[k]; RNG stream id stream[k]; stopped by the problem's domain and
+ * either the problem's subdomain (circle == NULL) or circle (R, cx, cy) at
+ * circle[3k..3k+2]. */
+typedef struct {
+    int32_t n;
+    int32_t reserved;
+    const double* x0;
+    const double* y0;
+    const uint32_t* paths;
+    const double* h;
+    const uint32_t* stream;
+    const double* circle;
+} spac_solve_batch;
+
+/* Per-path exit data, SoA over sum(paths) entries in solve order
+ * (Solvers/FeynmanKacSolver.cpp:285-291).  Any pointer may be NULL. */
+typedef struct {
+    double* x_tau;   /* projected exit point (Npro) */
+    double* y_tau;
+    double* Y;
+    double* Z;
+    double* xi;
+    double* tau;
+    double* gval;    /* g(X_tau) for domain exits, u(X_tau) for subdomain exits */
+    int32_t* steps;
+    int8_t* outcome;
+} spac_path_records;
+
+/* Node estimator, the reference's public fields after Update()
+ * (Solvers/FeynmanKacSolver.cpp:294-323).  xi/xixi are the true moments (the
+ * reference's sums[8] aliasing is not reproduced); RNGC counts normals drawn. */
+typedef struct {
+    double phi, phiphi, phi_VR, phiphi_VR, xi, xixi, xiphi, APL, RNGC;
+    double B;             /* node mode only */
+    int64_t paths;
+    int64_t path_steps;   /* executed Paso_Absorbente + Dentro pairs */
+    int64_t n_max_steps;  /* paths stopped by the cap */
+    int32_t status;
+    int32_t n_domain_exits;
+} spac_point_stats;
+
+/* Interfaces (circles) and their nodes, the device copy of Interfaz
+ * (Mesh/Interfaz.hpp:8-30).  Node lists are CSR: interface k owns nodes
+ * node_offset[k] .. node_offset[k+1]-1. */
+typedef struct {
+    int32_t n_interfaces;
+    int32_t rbf_kind;           /* SPAC_RBF_* */
+    const double* cx;
+    const double* cy;
+    const double* radius;
+    const int32_t* code;        /* interior code: -11,-22,-33,-44,-1,-2,-3,-4 or 1 */
+    const double* c2;           /* IMQ shape per interface */
+    const int32_t* node_offset; /* [n_interfaces + 1] */
+    const double* node_theta;   /* angular position of each circle node */
+    const int32_t* node_global; /* global index = matrix column */
+    const int64_t* ipsi_offset; /* [n_interfaces]: start of the M*M row-major iPsi block */
+    const double* ipsi;         /* IMQ mode: iPsi blocks; stencil mode: stencil inverses */
+    double gauss_eps;           /* stencil mode: Gaussian shape on stencil-normalised distance */
+    int32_t stencil_size;       /* stencil mode: nodes per stencil (odd) */
+    int32_t reserved;
+} spac_interface_table;
+
+/* Rows to assemble: node k lies inside interface iface[k] and needs its G/B
+ * row (Solvers/FeynmanKacSolver.cpp:344-355).  frontera[k] != 0 gives B = g(x)
+ * with no paths (:346-347). */
+typedef struct {
+    int32_t n;
+    int32_t reserved;
+    const double* x;
+    const double* y;
+    const int32_t* iface;
+    const int32_t* frontera;
+    const uint32_t* paths;
+    const double* h;
+    const uint32_t* stream;
+} spac_node_batch;
+
+/* Output rows: B[k] and G weights of node k over its interface's nodes at
+ * G[G_offset[k] .. G_offset[k+1]), columns in G_col (caller sizes
+ * G_offset[n] = sum over rows of the interface's node count). */
+typedef struct {
+    double* B;
+    double* G;
+    int32_t* G_col;
+    int64_t* G_offset;
+    spac_point_stats* stats;    /* optional */
+} spac_row_block;
+
+/* ---------------------------------------------------------------- API -- */
+const char* spac_last_error(void);
+int32_t spac_abi_version(void);
+/* Number of visible devices (0 on a machine without a GPU; never fails). */
+int32_t spac_device_count(void);
+
+spac_status spac_fk_problem_create(const spac_problem_desc* desc, spac_problem** out);
+spac_status spac_fk_problem_destroy(spac_problem* p);
+
+/* Point solve for n starting points, paths each: Solve_OMP_Analytic(X0, N, h,
+ * rho, BVP) (Solvers/FeynmanKacSolver.cpp:335-342) batched over points.
+ * stats[k] receives point k's estimator. */
+spac_status spac_fk_solve_points(spac_problem* p, const double* x0, const double* y0, int32_t n,
+                                 uint32_t paths, double h, uint64_t seed, spac_point_stats* stats);
+
+/* Same on device-resident inputs and outputs (d_x0, d_y0 [n], d_stats [n]);
+ * the timed form for throughput measurements.  Synchronous. */
+spac_status spac_fk_solve_points_device(spac_problem* p, const double* d_x0, const double* d_y0,
+                                        int32_t n, uint32_t paths, double h, uint64_t seed,
+                                        spac_point_stats* d_stats);
+
+/* General batch of solves with per-path exit data (Simulacion_OMP,
+ * Solvers/FeynmanKacSolver.cpp:137-197) and, optionally, per-solve estimators. */
+spac_status spac_fk_simulate(spac_problem* p, const spac_solve_batch* batch, uint64_t seed,
+                             spac_path_records* records, spac_point_stats* stats);
+
+/* Interface rows: Solve_OMP_Analytic(Nudo&, ...) + Reduce_Analytic(..., Nudo&,
+ * Interfaz) + Update(Nudo&) (Solvers/FeynmanKacSolver.cpp:233-292, 324-355)
+ * for a batch of nodes in one launch. */
+spac_status spac_fk_assemble_rows(spac_problem* p, const spac_interface_table* ifaces,
+                                  const spac_node_batch* nodes, uint64_t seed, spac_row_block* out);
+
+/* Timing of the last call's kernels, measured with HIP events on the handle's
+ * stream: path kernel, reduction kernels (ms), and launches counted. */
+typedef struct {
+    double path_kernel_ms;
+    double reduce_kernel_ms;
+    double total_ms;
+    int64_t path_steps;
+    int32_t path_kernel_launches;
+    int32_t grid_blocks;
+} spac_timing;
+spac_status spac_fk_last_timing(spac_problem* p, spac_timing* out);
+
+/* Launch shape of the persistent path kernel (0 = library default). */
+typedef struct {
+    int32_t block_size;       /* threads per block (multiple of 64) */
+    int32_t blocks_per_cu;    /* persistent blocks per compute unit */
+    int32_t chunk_paths;      /* paths per work-queue item */
+    int32_t reserved;
+} spac_launch_config;
+spac_status spac_fk_set_launch(spac_problem* p, const spac_launch_config* cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPAC_FK_H */

@@ -1,0 +1,539 @@
+/*
+ * fk_oracle.c — CPU restatement of SpAc's Feynman–Kac hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP kernels in
+ * spac_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.  The product never links or calls it.
+ *
+ * It keeps the reference's structure on purpose: every coefficient and
+ * boundary is a plugin reached through a function pointer and evaluated once
+ * per use, exactly as Paso_Absorbente and Dentro call them
+ * (Solvers/FeynmanKacSolver.cpp:7-18, 68-106), so that it doubles as the
+ * "reference OpenMP CPU path" timed beside the GPU.  Differences from the
+ * reference, all deliberate and listed in DESIGN.md:
+ *   - Brownian increments come from Philox4x32-10 + Box–Muller keyed by
+ *     (seed, stream) and counted by (step, path) instead of one boost::mt19937
+ *     per OpenMP thread (Solvers/FeynmanKacSolver.cpp:2-6, 107-136), so a path
+ *     is the same on any thread count and on the GPU;
+ *   - exp/log/sin/cos are the project's own FP64 functions (spac_math.h) so
+ *     that CPU and GPU bits agree;
+ *   - the estimator sums follow one fixed compensated reduction order
+ *     (tiles of 64 paths folded pairwise, tiles added in order) instead of
+ *     per-thread partial sums merged under omp critical (:217-228, 276-290);
+ *   - xi and xixi are the true moments (the reference's enum alias puts
+ *     both into sums[8], Solvers/FeynmanKacSolver.hpp:16-23);
+ *   - RNGC counts the normals actually drawn (2 per step; the reference
+ *     counts 4, :111 and :275);
+ *   - a step cap (max_steps) ends a path that never stops.
+ *
+ * Parity status of each function is stated beside it ("pinned by" = the test
+ * that checks it independently of this file).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/spac_fk.h"
+#include "../spac_amd/csrc/spac_math.h"
+
+typedef struct { double x, y; } vec2;
+typedef struct { double a00, a01, a10, a11; } mat2;
+
+typedef double (*pfescalar)(vec2);
+typedef double (*pfescalarN)(vec2, vec2);
+typedef vec2 (*pfvector)(vec2);
+typedef mat2 (*pfmatriz)(vec2);
+typedef double (*pfdistancia)(const double* params, vec2 x, vec2* exitpoint, vec2* normal);
+
+typedef struct {
+    pfdistancia distancia;
+    double params[4];
+} frontera_t;
+
+/* BVP container, BVP/BVP.hpp:17-28 */
+typedef struct {
+    pfescalar u, g, c, f;
+    pfescalarN psi, varphi;
+    pfvector b, gradiente_u, F, mu;
+    pfmatriz sigma;
+    frontera_t frontera_dominio, frontera_subdominio;
+} bvp_t;
+
+/* ------------------------------------------------------------ defaults --
+ * BVP/Coeficiente_Escalar.hpp:7-9, Coeficiente_EscalarN.hpp:7-9,
+ * Coeficiente_Vectorial.hpp:6-8, Coeficiente_Matricial.hpp:6-8 */
+static double escalar_defecto(vec2 x) { (void)x; return 0.0; }
+static double escalarN_defecto(vec2 x, vec2 n) { (void)x; (void)n; return 0.0; }
+static vec2 vector_defecto(vec2 x) { vec2 r = {x.x * 0.0, x.y * 0.0}; return r; }
+
+/* ------------------------------------------------------------- Franke --
+ * BVPs/Franke_Poisson.hpp:4-73.  pow(a, 2) is written a*a (pow(a,2) is
+ * exactly a*a under correct rounding).  Pinned by: tests/test_oracle.py
+ * (values and derivatives against finite differences of u; f = -lap u). */
+static double fr1(vec2 X) { double a = 9.0 * X.x - 2.0, b = 9.0 * X.y - 2.0; return spac_exp(-0.25 * (a * a + b * b)); }
+static double fr2(vec2 X) { double a = 9.0 * X.x + 1.0; return spac_exp((-1.0 / 49.0) * (a * a) - 0.1 * (9.0 * X.y + 1.0)); }
+static double fr3(vec2 X) { double a = 9.0 * X.x - 7.0, b = 9.0 * X.y - 3.0; return spac_exp(-0.25 * (a * a + b * b)); }
+static double fr4(vec2 X) { double a = 9.0 * X.x - 4.0, b = 9.0 * X.y - 7.0; return spac_exp(-1.0 * (a * a + b * b)); }
+
+static double franke_u(vec2 X) {
+    return 0.75 * fr1(X) + 0.75 * fr2(X) + 0.5 * fr3(X) - 0.2 * fr4(X);
+}
+static double franke_dudx(vec2 X) {
+    return -3.375 * (9.0 * X.x - 2.0) * fr1(X) - (27.0 / 98.0) * (9.0 * X.x + 1.0) * fr2(X)
+           - 2.25 * (9.0 * X.x - 7.0) * fr3(X) + 3.6 * (9.0 * X.x - 4.0) * fr4(X);
+}
+static double franke_dudy(vec2 X) {
+    return -3.375 * (9.0 * X.y - 2.0) * fr1(X) - 0.675 * fr2(X)
+           - 2.25 * (9.0 * X.y - 3.0) * fr3(X) + 3.6 * (9.0 * X.y - 7.0) * fr4(X);
+}
+static double franke_d2udx2(vec2 X) {
+    double a = 9.0 * X.x - 2.0, b = 9.0 * X.x + 1.0, c = 9.0 * X.x - 7.0, d = 9.0 * X.x - 4.0;
+    return -3.375 * (9.0 - 4.5 * (a * a)) * fr1(X)
+           - (27.0 / 98.0) * (9.0 - (18.0 / 49.0) * (b * b)) * fr2(X)
+           - 2.25 * (9.0 - 4.5 * (c * c)) * fr3(X)
+           + 3.6 * (9.0 - 18.0 * (d * d)) * fr4(X);
+}
+static double franke_d2udy2(vec2 X) {
+    double a = 9.0 * X.y - 2.0, c = 9.0 * X.y - 3.0, d = 9.0 * X.y - 7.0;
+    return -3.375 * (9.0 - 4.5 * (a * a)) * fr1(X)
+           + (243.0 / 400.0) * fr2(X) - 2.25 * (9.0 - 4.5 * (c * c)) * fr3(X)
+           + 3.6 * (9.0 - 18.0 * (d * d)) * fr4(X);
+}
+static double franke_c(vec2 X) { (void)X; return -0.0; }
+/* Franke_b builds an uninitialised Vector2d times 0.0 (BVPs/Franke_Poisson.hpp:51-54);
+ * restated as the intended zero vector. */
+static vec2 franke_b(vec2 X) { (void)X; vec2 r = {0.0, 0.0}; return r; }
+static mat2 franke_sigma(vec2 X) {
+    (void)X;
+    const double s = 1.41421356237; /* literal of BVPs/Franke_Poisson.hpp:56 */
+    mat2 m = {1.0 * s, 0.0 * s, 0.0 * s, 1.0 * s};
+    return m;
+}
+static double franke_f(vec2 X) { return -franke_d2udx2(X) - franke_d2udy2(X); }
+static vec2 franke_gradu(vec2 X) { vec2 r = {franke_dudx(X), franke_dudy(X)}; return r; }
+/* F = -sigma^T grad u (BVPs/Franke_Poisson.hpp:67-73) */
+static vec2 franke_F(vec2 X) {
+    vec2 g = {franke_dudx(X), franke_dudy(X)};
+    mat2 s = franke_sigma(X);
+    vec2 r = {-(s.a00 * g.x + s.a10 * g.y), -(s.a01 * g.x + s.a11 * g.y)};
+    return r;
+}
+
+/* ----------------------------------------- variable-coefficient Franke --
+ * Operator L = 1/2 tr(sigma sigma^T D2) + b.grad + c with sigma = s(x) I,
+ * s = 1 + 0.5 sin(pi x) cos(pi y), b = 0.3 (y, -x), c = -1 - x^2; the
+ * manufactured solution is Franke u, so f = -(L u) and g = u (the sign
+ * convention L u + f = 0 of BVPs/Franke_Poisson.hpp:58-60,
+ * BVPs/Monegros_Revamped.hpp:58).  Pinned by: tests/test_oracle.py (L u + f = 0
+ * by finite differences). */
+static double vc_s(vec2 X) { return 1.0 + 0.5 * spac_sinpi(X.x) * spac_cospi(X.y); }
+static mat2 vc_sigma(vec2 X) {
+    double s = vc_s(X);
+    mat2 m = {1.0 * s, 0.0 * s, 0.0 * s, 1.0 * s};
+    return m;
+}
+static vec2 vc_b(vec2 X) { vec2 r = {0.3 * X.y, -0.3 * X.x}; return r; }
+static double vc_c(vec2 X) { return -1.0 - X.x * X.x; }
+static double vc_f(vec2 X) {
+    double s = vc_s(X);
+    double lap = franke_d2udx2(X) + franke_d2udy2(X);
+    vec2 b = vc_b(X);
+    double adv = b.x * franke_dudx(X) + b.y * franke_dudy(X);
+    double reac = vc_c(X) * franke_u(X);
+    double diff = (0.5 * (s * s)) * lap;
+    return -((diff + adv) + reac);
+}
+static vec2 vc_F(vec2 X) {
+    vec2 g = {franke_dudx(X), franke_dudy(X)};
+    mat2 s = vc_sigma(X);
+    vec2 r = {-(s.a00 * g.x + s.a10 * g.y), -(s.a01 * g.x + s.a11 * g.y)};
+    return r;
+}
+
+/* ----------------------------------------------------------- boundaries --
+ * Rectangulo: Dominios/Rectangulo.hpp:7-229, the quadrant tree restated
+ * branch for branch (including the float literal 0.70710678118f of the corner
+ * normals, :49-50,100-101,153-154,205-206).  Pinned by: tests/test_oracle.py
+ * (brute-force signed distance to the rectangle outline). */
+static double rectangulo(const double* params, vec2 position, vec2* exitpoint, vec2* normal) {
+    const double K = (double)0.70710678118f;
+    double center[2] = {0.5 * (params[0] + params[2]), 0.5 * (params[1] + params[3])};
+    double halfside[2] = {params[2] - center[0], params[3] - center[1]};
+    double distances[2] = {fabs(position.x - center[0]), fabs(position.y - center[1])};
+    double r;
+    *exitpoint = position;
+    int inside = (distances[0] <= halfside[0]) && (distances[1] <= halfside[1]);
+    int right = position.x > center[0];
+    int top = position.y > center[1];
+    double sx = right ? 1.0 : -1.0;
+    double sy = top ? 1.0 : -1.0;
+    if (inside) {
+        if (distances[0] - halfside[0] > distances[1] - halfside[1]) {
+            exitpoint->x = sx * halfside[0] + center[0];
+            normal->x = sx; normal->y = 0.0;
+            return distances[0] - halfside[0];
+        }
+        exitpoint->y = sy * halfside[1] + center[1];
+        normal->x = 0.0; normal->y = sy;
+        return distances[1] - halfside[1];
+    }
+    if (distances[0] > halfside[0]) {
+        if (distances[1] > halfside[1]) {
+            r = sqrt((distances[0] - halfside[0]) * (distances[0] - halfside[0]) +
+                     (distances[1] - halfside[1]) * (distances[1] - halfside[1]));
+            exitpoint->x = sx * halfside[0] + center[0];
+            exitpoint->y = sy * halfside[1] + center[1];
+            normal->x = sx * K; normal->y = sy * K;
+            return r;
+        }
+        exitpoint->x = sx * halfside[0] + center[0];
+        normal->x = sx; normal->y = 0.0;
+        return distances[0] - halfside[0];
+    }
+    exitpoint->y = sy * halfside[1] + center[1];
+    normal->x = 0.0; normal->y = sy;
+    return distances[1] - halfside[1];
+}
+
+/* Circulo: Dominios/Circulo.hpp:7-22 (params R, cx, cy; radius clamp 5e-4). */
+static double circulo(const double* p, vec2 posicion, vec2* exitpoint, vec2* normal) {
+    normal->x = posicion.x - p[1];
+    normal->y = posicion.y - p[2];
+    double r = sqrt(normal->x * normal->x + normal->y * normal->y);
+    r = (5E-4 < r) ? r : 5E-4; /* std::max(5E-4, r) */
+    normal->x = normal->x / r;
+    normal->y = normal->y / r;
+    exitpoint->x = normal->x * p[0] + p[1];
+    exitpoint->y = normal->y * p[0] + p[2];
+    return r - p[0];
+}
+
+/* No subdomain: never the nearest boundary (the reference's default frontier
+ * returns distance 0 and would stop every path after one step,
+ * BVP/Frontera.hpp:12-16). */
+static double ninguna(const double* p, vec2 x, vec2* exitpoint, vec2* normal) {
+    (void)p;
+    *exitpoint = x;
+    normal->x = 0.0; normal->y = 0.0;
+    return -INFINITY;
+}
+
+static void frontera_init(frontera_t* f, int kind, const double* params) {
+    switch (kind) {
+    case SPAC_BOUNDARY_RECTANGLE: f->distancia = rectangulo; break;
+    case SPAC_BOUNDARY_CIRCLE: f->distancia = circulo; break;
+    default: f->distancia = ninguna; break;
+    }
+    for (int i = 0; i < 4; ++i) f->params[i] = params ? params[i] : 0.0;
+}
+
+static int bvp_init(bvp_t* bvp, const spac_problem_desc* d) {
+    bvp->psi = escalarN_defecto;
+    bvp->varphi = escalarN_defecto;
+    bvp->mu = vector_defecto;
+    switch (d->coef_kind) {
+    case SPAC_COEF_FRANKE_POISSON:
+        bvp->u = franke_u; bvp->g = franke_u; bvp->c = franke_c; bvp->f = franke_f;
+        bvp->b = franke_b; bvp->gradiente_u = franke_gradu; bvp->F = franke_F;
+        bvp->sigma = franke_sigma;
+        break;
+    case SPAC_COEF_VARCOEF_FRANKE:
+        bvp->u = franke_u; bvp->g = franke_u; bvp->c = vc_c; bvp->f = vc_f;
+        bvp->b = vc_b; bvp->gradiente_u = franke_gradu; bvp->F = vc_F;
+        bvp->sigma = vc_sigma;
+        break;
+    default:
+        return SPAC_E_UNREGISTERED_PLUGIN;
+    }
+    frontera_init(&bvp->frontera_dominio, d->domain_kind, d->domain_params);
+    frontera_init(&bvp->frontera_subdominio, d->subdomain_kind, d->subdomain_params);
+    (void)escalar_defecto;
+    return SPAC_OK;
+}
+
+/* ----------------------------------------------------------- Dentro --
+ * Solvers/FeynmanKacSolver.cpp:68-106 (absorbing boundaries: Stopping
+ * returns true for every shipped frontier, Dominios/Rectangulo.hpp:230-232). */
+static int dentro(const bvp_t* bvp, vec2 X, double sqrth, double gobet, vec2* Npro, vec2* N,
+                  double* distancia) {
+    vec2 Npro_aux, N_aux;
+    double distancia_aux = bvp->frontera_dominio.distancia(bvp->frontera_dominio.params, X, &Npro_aux, &N_aux);
+    double d = bvp->frontera_subdominio.distancia(bvp->frontera_subdominio.params, X, Npro, N);
+    mat2 s;
+    int status;
+    if (distancia_aux >= d) {
+        d = distancia_aux;
+        *Npro = Npro_aux;
+        *N = N_aux;
+        s = (d < 0.0) ? bvp->sigma(X) : bvp->sigma(*Npro);
+        double r0 = N->x * s.a00 + N->y * s.a10;
+        double r1 = N->x * s.a01 + N->y * s.a11;
+        double nrm = sqrt(r0 * r0 + r1 * r1);
+        status = (d < gobet * nrm * sqrth) ? SPAC_OUT_INSIDE : SPAC_OUT_DOMAIN;
+    } else {
+        s = (d < 0.0) ? bvp->sigma(X) : bvp->sigma(*Npro);
+        double r0 = N->x * s.a00 + N->y * s.a10;
+        double r1 = N->x * s.a01 + N->y * s.a11;
+        double nrm = sqrt(r0 * r0 + r1 * r1);
+        status = (d < gobet * nrm * sqrth) ? SPAC_OUT_INSIDE : SPAC_OUT_SUBDOMAIN;
+    }
+    *distancia = d;
+    return status;
+}
+
+/* --------------------------------------------------- Paso_Absorbente --
+ * Solvers/FeynmanKacSolver.cpp:7-18, update order kept: Z, xi, Y, X, t.
+ * ji_t is 0 for absorbing boundaries (:101-103). */
+typedef struct { vec2 X, N; double Y, Z, xi, t; } path_state;
+
+static void paso_absorbente(const bvp_t* bvp, path_state* st, double h, double sqrth, uint64_t seed,
+                            uint32_t stream, uint32_t path, uint32_t step) {
+    const double ji_t = 0.0;
+    mat2 sig = bvp->sigma(st->X);
+    double n0, n1;
+    spac_normal_pair(seed, stream, path, step, &n0, &n1);
+    vec2 dW = {sqrth * n0, sqrth * n1};
+    double Y = st->Y;
+    st->Z = st->Z + (bvp->f(st->X) * Y * h + bvp->psi(st->X, st->N) * Y * ji_t);
+    vec2 F = bvp->F(st->X);
+    st->xi = st->xi + Y * (F.x * dW.x + F.y * dW.y);
+    vec2 mu = bvp->mu(st->X);
+    st->Y = Y + (bvp->c(st->X) * Y * h + Y * (mu.x * dW.x + mu.y * dW.y) +
+                 bvp->varphi(st->X, st->N) * Y * ji_t);
+    vec2 b = bvp->b(st->X);
+    vec2 mu2 = bvp->mu(st->X);
+    double smx = sig.a00 * mu2.x + sig.a01 * mu2.y;
+    double smy = sig.a10 * mu2.x + sig.a11 * mu2.y;
+    double sdx = sig.a00 * dW.x + sig.a01 * dW.y;
+    double sdy = sig.a10 * dW.x + sig.a11 * dW.y;
+    st->X.x = st->X.x + ((b.x - smx) * h + sdx);
+    st->X.y = st->X.y + ((b.y - smy) * h + sdy);
+    st->t = st->t + h;
+}
+
+/* ----------------------------------------------------- Simulacion_OMP --
+ * Solvers/FeynmanKacSolver.cpp:137-197: initial Dentro, then do { step;
+ * Dentro } while not stopped — at least one step even when X0 starts in the
+ * stop band.  Records X_tau = Npro (the projected point). */
+static void simulate_one(const bvp_t* bvp, vec2 X0, double h, double sqrth, double gobet, int max_steps,
+                         uint64_t seed, uint32_t stream, uint32_t path, double* xt, double* yt,
+                         double* Yo, double* Zo, double* xio, double* tauo, double* gvalo,
+                         int32_t* stepso, int8_t* outo) {
+    path_state st;
+    st.X = X0; st.Y = 1.0; st.Z = 0.0; st.xi = 0.0; st.t = 0.0;
+    vec2 Npro, N;
+    double dist;
+    int estado = dentro(bvp, st.X, sqrth, gobet, &Npro, &N, &dist);
+    st.N = N;
+    int steps = 0;
+    do {
+        paso_absorbente(bvp, &st, h, sqrth, seed, stream, path, (uint32_t)steps);
+        steps++;
+        estado = dentro(bvp, st.X, sqrth, gobet, &Npro, &N, &dist);
+        st.N = N;
+        if (estado == SPAC_OUT_INSIDE && steps >= max_steps) { estado = SPAC_OUT_MAX_STEPS; break; }
+    } while (estado != SPAC_OUT_DOMAIN && estado != SPAC_OUT_SUBDOMAIN);
+    if (xt) *xt = Npro.x;
+    if (yt) *yt = Npro.y;
+    if (Yo) *Yo = st.Y;
+    if (Zo) *Zo = st.Z;
+    if (xio) *xio = st.xi;
+    if (tauo) *tauo = st.t;
+    if (gvalo) *gvalo = (estado == SPAC_OUT_SUBDOMAIN) ? bvp->u(Npro) : bvp->g(Npro);
+    if (stepso) *stepso = steps;
+    if (outo) *outo = (int8_t)estado;
+}
+
+/* Simulate a batch of solves; records are laid out solve after solve. */
+int oracle_simulate(const spac_problem_desc* desc, const spac_solve_batch* batch, uint64_t seed,
+                    spac_path_records* rec, int nthreads) {
+    bvp_t base;
+    int rc = bvp_init(&base, desc);
+    if (rc) return rc;
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(batch->n + 1));
+    off[0] = 0;
+    for (int k = 0; k < batch->n; ++k) off[k + 1] = off[k] + batch->paths[k];
+    int max_steps = desc->max_steps > 0 ? desc->max_steps : 0x7fffffff;
+    for (int k = 0; k < batch->n; ++k) {
+        bvp_t bvp = base;
+        if (batch->circle) {
+            double cp[4] = {batch->circle[3 * k], batch->circle[3 * k + 1], batch->circle[3 * k + 2], 0.0};
+            frontera_init(&bvp.frontera_subdominio, SPAC_BOUNDARY_CIRCLE, cp);
+        }
+        vec2 X0 = {batch->x0[k], batch->y0[k]};
+        double h = batch->h[k];
+        double sqrth = sqrt(h);
+        int64_t np = batch->paths[k];
+        int64_t o = off[k];
+        uint32_t stream = batch->stream[k];
+#ifdef _OPENMP
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+        for (int64_t p = 0; p < np; ++p) {
+            int64_t g = o + p;
+            simulate_one(&bvp, X0, h, sqrth, desc->gobet_constant, max_steps, seed, stream, (uint32_t)p,
+                         rec->x_tau ? rec->x_tau + g : NULL, rec->y_tau ? rec->y_tau + g : NULL,
+                         rec->Y ? rec->Y + g : NULL, rec->Z ? rec->Z + g : NULL,
+                         rec->xi ? rec->xi + g : NULL, rec->tau ? rec->tau + g : NULL,
+                         rec->gval ? rec->gval + g : NULL, rec->steps ? rec->steps + g : NULL,
+                         rec->outcome ? rec->outcome + g : NULL);
+        }
+    }
+    free(off);
+    (void)nthreads;
+    return SPAC_OK;
+}
+
+/* ------------------------------------------------------ Reduce_Analytic --
+ * Solvers/FeynmanKacSolver.cpp:198-231 (point) and 233-292 (node) with the
+ * fixed compensated order: tiles of 64 consecutive paths, each folded
+ * pairwise (lane i += lane i+off for off = 32, 16, ..., 1) with TwoSum
+ * error terms, tiles accumulated in order.  Update(): :294-334. */
+#define NSUMS 10
+enum { S_SCORE, S_SCORE2, S_XI, S_XI2, S_VR, S_VR2, S_XISCORE, S_RNG, S_TAU, S_B };
+
+void oracle_path_terms(int node_mode, double Y, double Z, double xi, double tau, double gval,
+                       int32_t steps, int8_t outcome, double* v) {
+    double score = Z + Y * gval;
+    double vr = score + xi;
+    v[S_SCORE] = score;
+    v[S_SCORE2] = score * score;
+    v[S_XI] = xi;
+    v[S_XI2] = xi * xi;
+    v[S_VR] = vr;
+    v[S_VR2] = vr * vr;
+    v[S_XISCORE] = xi * score;
+    v[S_RNG] = 2.0 * (double)steps;
+    v[S_TAU] = tau;
+    if (node_mode) v[S_B] = (outcome == SPAC_OUT_SUBDOMAIN) ? (Z + xi) : vr;
+    else v[S_B] = 0.0;
+}
+
+static void reduce_one(int node_mode, int64_t np, const double* Y, const double* Z, const double* xi,
+                       const double* tau, const double* gval, const int32_t* steps, const int8_t* outcome,
+                       spac_point_stats* st) {
+    spac_ksum tot[NSUMS];
+    for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_zero();
+    int64_t ntile = (np + 63) / 64;
+    int64_t total_steps = 0, nmax = 0, ndom = 0;
+    int has_nan = 0;
+    for (int64_t t = 0; t < ntile; ++t) {
+        spac_ksum lane[NSUMS][64];
+        for (int l = 0; l < 64; ++l) {
+            int64_t p = t * 64 + l;
+            double v[NSUMS];
+            if (p < np) {
+                oracle_path_terms(node_mode, Y[p], Z[p], xi[p], tau[p], gval[p], steps[p], outcome[p], v);
+                total_steps += steps[p];
+                if (outcome[p] == SPAC_OUT_MAX_STEPS) nmax++;
+                if (outcome[p] != SPAC_OUT_SUBDOMAIN) ndom++;
+            } else {
+                for (int k = 0; k < NSUMS; ++k) v[k] = 0.0;
+            }
+            for (int k = 0; k < NSUMS; ++k) { lane[k][l].s = v[k]; lane[k][l].c = 0.0; }
+        }
+        for (int off = 32; off >= 1; off >>= 1)
+            for (int l = 0; l < off; ++l)
+                for (int k = 0; k < NSUMS; ++k) lane[k][l] = spac_ksum_merge(lane[k][l], lane[k][l + off]);
+        for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_merge(tot[k], lane[k][0]);
+    }
+    double N = (double)np;
+    double s[NSUMS];
+    for (int k = 0; k < NSUMS; ++k) { s[k] = spac_ksum_value(tot[k]); if (s[k] != s[k]) has_nan = 1; }
+    st->phi = s[S_SCORE] / N;
+    st->phiphi = s[S_SCORE2] / N;
+    st->xi = s[S_XI] / N;
+    st->xixi = s[S_XI2] / N;
+    st->phi_VR = s[S_VR] / N;
+    st->phiphi_VR = s[S_VR2] / N;
+    st->xiphi = s[S_XISCORE] / N;
+    st->RNGC = s[S_RNG];
+    st->APL = s[S_TAU] / N;
+    st->B = node_mode ? s[S_B] / N : 0.0;
+    st->paths = np;
+    st->path_steps = total_steps;
+    st->n_max_steps = nmax;
+    st->n_domain_exits = (int32_t)ndom;
+    st->status = has_nan ? SPAC_E_NAN : (nmax ? SPAC_E_MAX_STEPS : SPAC_OK);
+}
+
+int oracle_reduce(int node_mode, const spac_solve_batch* batch, const spac_path_records* rec,
+                  spac_point_stats* stats) {
+    int64_t o = 0;
+    for (int k = 0; k < batch->n; ++k) {
+        int64_t np = batch->paths[k];
+        reduce_one(node_mode, np, rec->Y + o, rec->Z + o, rec->xi + o, rec->tau + o, rec->gval + o,
+                   rec->steps + o, rec->outcome + o, &stats[k]);
+        o += np;
+    }
+    return SPAC_OK;
+}
+
+/* ------------------------------------------------------- math exports -- */
+void oracle_math_eval(int fn, const double* x, const double* y, double* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (fn) {
+        case 0: out[i] = spac_exp(x[i]); break;
+        case 1: out[i] = spac_log(x[i]); break;
+        case 2: out[i] = spac_sinpi(x[i]); break;
+        case 3: out[i] = spac_cospi(x[i]); break;
+        case 4: out[i] = spac_atan2(x[i], y[i]); break;
+        case 5: out[i] = sqrt(x[i]); break;
+        case 6: out[i] = x[i] / y[i]; break;
+        default: out[i] = 0.0; break;
+        }
+    }
+}
+
+void oracle_philox(int rounds, const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    spac_u32x4 r = spac_philox4x32_R(rounds, ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+
+void oracle_normals(uint64_t seed, uint32_t stream, uint32_t path, int32_t nsteps, double* out) {
+    for (int32_t s = 0; s < nsteps; ++s) spac_normal_pair(seed, stream, path, (uint32_t)s, &out[2 * s], &out[2 * s + 1]);
+}
+
+/* Coefficient probes for the plugin pins: which = 0 u, 1 f, 2 F.x, 3 F.y,
+ * 4 sigma00, 5 b.x, 6 b.y, 7 c, 8 dudx, 9 dudy, 10 d2udx2 + d2udy2 */
+int oracle_coef_eval(int coef_kind, int which, const double* x, const double* y, double* out, int64_t n) {
+    spac_problem_desc d;
+    memset(&d, 0, sizeof d);
+    d.coef_kind = coef_kind;
+    bvp_t bvp;
+    if (bvp_init(&bvp, &d)) return SPAC_E_UNREGISTERED_PLUGIN;
+    for (int64_t i = 0; i < n; ++i) {
+        vec2 X = {x[i], y[i]};
+        switch (which) {
+        case 0: out[i] = bvp.u(X); break;
+        case 1: out[i] = bvp.f(X); break;
+        case 2: out[i] = bvp.F(X).x; break;
+        case 3: out[i] = bvp.F(X).y; break;
+        case 4: out[i] = bvp.sigma(X).a00; break;
+        case 5: out[i] = bvp.b(X).x; break;
+        case 6: out[i] = bvp.b(X).y; break;
+        case 7: out[i] = bvp.c(X); break;
+        case 8: out[i] = franke_dudx(X); break;
+        case 9: out[i] = franke_dudy(X); break;
+        case 10: out[i] = franke_d2udx2(X) + franke_d2udy2(X); break;
+        default: out[i] = 0.0; break;
+        }
+    }
+    return SPAC_OK;
+}
+
+/* Boundary probes: kind SPAC_BOUNDARY_*; out = [dist, ex, ey, nx, ny] per point. */
+void oracle_boundary_eval(int kind, const double* params, const double* x, const double* y, double* out, int64_t n) {
+    frontera_t f;
+    frontera_init(&f, kind, params);
+    for (int64_t i = 0; i < n; ++i) {
+        vec2 X = {x[i], y[i]}, e, nn;
+        double d = f.distancia(f.params, X, &e, &nn);
+        out[5 * i] = d; out[5 * i + 1] = e.x; out[5 * i + 2] = e.y; out[5 * i + 3] = nn.x; out[5 * i + 4] = nn.y;
+    }
+}

@@ -1,0 +1,73 @@
+/*
+ * fk_internal.h — library-private state behind the opaque spac_problem handle.
+ */
+#ifndef SPAC_FK_INTERNAL_H
+#define SPAC_FK_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+
+#include "../../include/spac_fk.h"
+
+namespace spac {
+
+spac_status set_error(spac_status code, const char* fmt, ...);
+
+/* Grow-only device allocation owned by a handle. */
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    spac_status ensure(size_t bytes);
+    void release();
+};
+
+/* Device-side description of a batch of solves.  Arrays are device pointers;
+ * a null per-solve array means "uniform", taken from the scalar beside it. */
+struct SolveSet {
+    int32_t n = 0;
+    const double* x0 = nullptr;
+    const double* y0 = nullptr;
+    const double* h = nullptr;
+    double h_uniform = 0.0;
+    const uint32_t* stream = nullptr;
+    const uint32_t* paths = nullptr;
+    uint32_t paths_uniform = 0;
+    const int64_t* rec_off = nullptr;
+    const double* circle = nullptr;
+    const int4* items = nullptr;
+    int32_t n_items = 0;
+    int32_t chunks_per_solve = 0;
+    int64_t total_paths = 0;
+};
+
+struct Problem {
+    spac_problem_desc desc{};
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    spac_launch_config launch{};
+    spac_timing last{};
+    /* per-path records (SoA) */
+    DeviceBuffer rec_xt, rec_yt, rec_Y, rec_Z, rec_xi, rec_tau, rec_gval, rec_steps, rec_outcome;
+    /* solve table, work queue, outputs */
+    DeviceBuffer sx, sy, sh, sstream, spaths, soff, scircle, sitems, queue, stats;
+    /* interface tables and row outputs */
+    DeviceBuffer ifc, ifnode, ifipsi, ifoff, rows_iface, rows_goff, rows_G, rows_part, rows_col;
+
+    spac_status ensure_records(int64_t total_paths);
+    spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode);
+    void release_all() {
+        DeviceBuffer* all[] = {&rec_xt, &rec_yt, &rec_Y, &rec_Z, &rec_xi, &rec_tau, &rec_gval, &rec_steps,
+                               &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
+                               &queue, &stats, &ifc, &ifnode, &ifipsi, &ifoff, &rows_iface, &rows_goff,
+                               &rows_G, &rows_part, &rows_col};
+        for (DeviceBuffer* b : all) b->release();
+    }
+};
+
+} // namespace spac
+
+#endif

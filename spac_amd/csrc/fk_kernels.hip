@@ -1,0 +1,670 @@
+/*
+ * fk_kernels.hip — MI355X (gfx950) Feynman–Kac path kernels and the C ABI of
+ * include/spac_fk.h.
+ *
+ * Path kernel (Simulacion_OMP, Solvers/FeynmanKacSolver.cpp:137-197):
+ *   - one wavefront lane per path, all path state in VGPRs;
+ *   - persistent grid: each wave takes work items (solve, path range) from a
+ *     global queue and refills lanes individually as soon as their path
+ *     stops, so heavy-tailed exit times do not idle the other 63 lanes;
+ *   - counter-based Philox stream keyed by (seed, stream) and counted by
+ *     (step, path): a path's bits do not depend on which lane, wave or launch
+ *     shape ran it;
+ *   - per-path exit data written to fixed SoA slots (solve offset + path).
+ * Reduction kernel (Reduce_Analytic + Update, :198-334): one wave per solve,
+ * tiles of 64 paths folded pairwise with TwoSum error terms, tiles added in
+ * order — a fixed order, so estimators are bit-stable across launches and
+ * equal to the CPU restatement's.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/spac_fk.h"
+#include "fk_device.h"
+#include "fk_internal.h"
+
+namespace spac {
+
+/* ------------------------------------------------------------- errors -- */
+static thread_local std::string g_last_error;
+
+spac_status set_error(spac_status code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define SPAC_HIP_CHECK(expr)                                                                   \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return set_error(SPAC_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                             __FILE__, __LINE__);                                              \
+    } while (0)
+
+/* ------------------------------------------------------ kernel args -- */
+enum { SUB_NONE = 0, SUB_CIRCLE = 2 };
+
+struct PathArgs {
+    double dom[4];
+    double gobet;
+    uint64_t seed;
+    int32_t max_steps;
+    int32_t n_items;
+    /* solves */
+    const double* x0;
+    const double* y0;
+    const double* h;        /* per solve, or nullptr: h_uniform */
+    double h_uniform;
+    const uint32_t* stream; /* per solve, or nullptr: solve index */
+    const double* circle;   /* 3 per solve (R, cx, cy), or nullptr: circle_uniform */
+    double circle_uniform[3];
+    const int64_t* rec_off; /* per solve, or nullptr: solve * paths_uniform */
+    uint32_t paths_uniform;
+    uint32_t chunk;
+    /* work queue: items[i] = (solve, p0, p1, 0) or nullptr: uniform chunks */
+    const int4* items;
+    int32_t chunks_per_solve;
+    int32_t* queue;
+    /* records */
+    double* xt;
+    double* yt;
+    double* Y;
+    double* Z;
+    double* xi;
+    double* tau;
+    double* gval;
+    int32_t* steps;
+    int8_t* outcome;
+};
+
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__global__ void zero_counter_kernel(int32_t* q) {
+    if (threadIdx.x == 0) q[0] = 0;
+}
+
+template <class Coef, int SUB>
+__global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    bool active = false;
+    /* lane path state */
+    int32_t solve = 0;
+    uint32_t path = 0;
+    int32_t nstep = 0;
+    double X = 0.0, Yc = 0.0, Yw = 0.0, Z = 0.0, xi = 0.0, t = 0.0;
+    double h = 0.0, sqrth = 0.0, R = 0.0, cx = 0.0, cy = 0.0;
+    uint32_t stream = 0;
+    int64_t rec = 0;
+    /* wave-uniform pool */
+    int32_t pool_solve = 0;
+    uint32_t pool_next = 0, pool_end = 0;
+    bool drained = false;
+
+    while (true) {
+        unsigned long long idle = __ballot(!active);
+        while (idle != 0ull && !drained) {
+            if (pool_next >= pool_end) {
+                int32_t item = 0;
+                if (lane == 0) item = atomicAdd(a.queue, 1);
+                item = __shfl(item, 0);
+                if (item >= a.n_items) { drained = true; break; }
+                if (a.items) {
+                    int4 it = a.items[item];
+                    pool_solve = it.x;
+                    pool_next = (uint32_t)it.y;
+                    pool_end = (uint32_t)it.z;
+                } else {
+                    pool_solve = item / a.chunks_per_solve;
+                    uint32_t c = (uint32_t)(item % a.chunks_per_solve);
+                    pool_next = c * a.chunk;
+                    pool_end = min(pool_next + a.chunk, a.paths_uniform);
+                }
+            }
+            uint32_t avail = pool_end - pool_next;
+            uint32_t nidle = (uint32_t)__popcll(idle);
+            uint32_t rank = lane_rank(idle);
+            if (!active && rank < avail) {
+                solve = pool_solve;
+                path = pool_next + rank;
+                h = a.h ? a.h[solve] : a.h_uniform;
+                sqrth = __builtin_sqrt(h);
+                stream = a.stream ? a.stream[solve] : (uint32_t)solve;
+                rec = a.rec_off ? a.rec_off[solve] : (int64_t)solve * (int64_t)a.paths_uniform;
+                if (SUB == SUB_CIRCLE) {
+                    if (a.circle) {
+                        R = a.circle[3 * solve]; cx = a.circle[3 * solve + 1]; cy = a.circle[3 * solve + 2];
+                    } else {
+                        R = a.circle_uniform[0]; cx = a.circle_uniform[1]; cy = a.circle_uniform[2];
+                    }
+                }
+                X = a.x0[solve];
+                Yc = a.y0[solve];
+                Yw = 1.0; Z = 0.0; xi = 0.0; t = 0.0;
+                nstep = 0;
+                active = true;
+            }
+            uint32_t take = min(avail, nidle);
+            pool_next += take;
+            idle = __ballot(!active);
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            /* Paso_Absorbente (Solvers/FeynmanKacSolver.cpp:7-18) */
+            StepCoef cf = Coef::step(X, Yc);
+            double n0, n1;
+            spac_normal_pair(a.seed, stream, path, (uint32_t)nstep, &n0, &n1);
+            double dWx = sqrth * n0, dWy = sqrth * n1;
+            Z = Z + (cf.f * Yw) * h;
+            xi = xi + Yw * (cf.Fx * dWx + cf.Fy * dWy);
+            if (Coef::kHasKilling) Yw = Yw + (cf.c * Yw) * h;
+            if (Coef::kHasDrift) {
+                X = X + (cf.bx * h + cf.s * dWx);
+                Yc = Yc + (cf.by * h + cf.s * dWy);
+            } else {
+                X = X + cf.s * dWx;
+                Yc = Yc + cf.s * dWy;
+            }
+            t = t + h;
+            nstep++;
+            /* Dentro (Solvers/FeynmanKacSolver.cpp:68-106) */
+            BoundaryHit hit = rectangle_distance(a.dom, X, Yc);
+            bool use_dom = true;
+            if (SUB == SUB_CIRCLE) {
+                BoundaryHit sub = circle_distance(R, cx, cy, X, Yc);
+                use_dom = hit.d >= sub.d;
+                if (!use_dom) hit = sub;
+            } else {
+                if (!(hit.d >= -__builtin_inf())) {
+                    use_dom = false;
+                    hit.d = -__builtin_inf(); hit.ex = X; hit.ey = Yc; hit.nx = 0.0; hit.ny = 0.0;
+                }
+            }
+            double s = (hit.d < 0.0) ? Coef::sigma(X, Yc) : Coef::sigma(hit.ex, hit.ey);
+            double r0 = hit.nx * s, r1 = hit.ny * s;
+            double nrm = __builtin_sqrt(r0 * r0 + r1 * r1);
+            bool inside = hit.d < a.gobet * nrm * sqrth;
+            int8_t out = inside ? (int8_t)SPAC_OUT_INSIDE : (use_dom ? (int8_t)SPAC_OUT_DOMAIN : (int8_t)SPAC_OUT_SUBDOMAIN);
+            if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
+            if (out != SPAC_OUT_INSIDE) {
+                int64_t g = rec + (int64_t)path;
+                a.xt[g] = hit.ex;
+                a.yt[g] = hit.ey;
+                a.Y[g] = Yw;
+                a.Z[g] = Z;
+                a.xi[g] = xi;
+                a.tau[g] = t;
+                a.gval[g] = (out == SPAC_OUT_SUBDOMAIN) ? Coef::u(hit.ex, hit.ey) : Coef::g(hit.ex, hit.ey);
+                a.steps[g] = nstep;
+                a.outcome[g] = out;
+                active = false;
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------- reduction -- */
+struct ReduceArgs {
+    int32_t n_solves;
+    int32_t node_mode;
+    const uint32_t* paths; /* per solve or nullptr: paths_uniform */
+    uint32_t paths_uniform;
+    const int64_t* rec_off;
+    const double* Y;
+    const double* Z;
+    const double* xi;
+    const double* tau;
+    const double* gval;
+    const int32_t* steps;
+    const int8_t* outcome;
+    spac_point_stats* stats;
+};
+
+#define NSUMS 10
+enum { S_SCORE, S_SCORE2, S_XI, S_XI2, S_VR, S_VR2, S_XISCORE, S_RNG, S_TAU, S_B };
+
+__device__ __forceinline__ spac_ksum shfl_down_ksum(spac_ksum v, int off) {
+    spac_ksum r;
+    r.s = __shfl_down(v.s, off, 64);
+    r.c = __shfl_down(v.c, off, 64);
+    return r;
+}
+
+__global__ void __launch_bounds__(64) fk_reduce_kernel(ReduceArgs a) {
+    const int solve = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (solve >= a.n_solves) return;
+    int64_t np = a.paths ? (int64_t)a.paths[solve] : (int64_t)a.paths_uniform;
+    int64_t off = a.rec_off ? a.rec_off[solve] : (int64_t)solve * (int64_t)a.paths_uniform;
+    spac_ksum tot[NSUMS];
+#pragma unroll
+    for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_zero();
+    int64_t my_steps = 0, my_max = 0, my_dom = 0;
+    int64_t ntile = (np + 63) / 64;
+    for (int64_t tl = 0; tl < ntile; ++tl) {
+        int64_t p = tl * 64 + lane;
+        double v[NSUMS];
+        if (p < np) {
+            int64_t g = off + p;
+            double Y = a.Y[g], Z = a.Z[g], xi = a.xi[g], gv = a.gval[g];
+            int32_t st = a.steps[g];
+            int8_t oc = a.outcome[g];
+            double score = Z + Y * gv;
+            double vr = score + xi;
+            v[S_SCORE] = score;
+            v[S_SCORE2] = score * score;
+            v[S_XI] = xi;
+            v[S_XI2] = xi * xi;
+            v[S_VR] = vr;
+            v[S_VR2] = vr * vr;
+            v[S_XISCORE] = xi * score;
+            v[S_RNG] = 2.0 * (double)st;
+            v[S_TAU] = a.tau[g];
+            v[S_B] = a.node_mode ? ((oc == SPAC_OUT_SUBDOMAIN) ? (Z + xi) : vr) : 0.0;
+            my_steps += st;
+            my_max += (oc == SPAC_OUT_MAX_STEPS);
+            my_dom += (oc != SPAC_OUT_SUBDOMAIN);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NSUMS; ++k) v[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < NSUMS; ++k) {
+            spac_ksum acc;
+            acc.s = v[k];
+            acc.c = 0.0;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) acc = spac_ksum_merge(acc, shfl_down_ksum(acc, o));
+            tot[k] = spac_ksum_merge(tot[k], acc);
+        }
+    }
+    /* exact integer totals */
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        my_steps += __shfl_xor(my_steps, o, 64);
+        my_max += __shfl_xor(my_max, o, 64);
+        my_dom += __shfl_xor(my_dom, o, 64);
+    }
+    if (lane == 0) {
+        double N = (double)np;
+        double s[NSUMS];
+        bool has_nan = false;
+#pragma unroll
+        for (int k = 0; k < NSUMS; ++k) {
+            s[k] = spac_ksum_value(tot[k]);
+            if (s[k] != s[k]) has_nan = true;
+        }
+        spac_point_stats st;
+        st.phi = s[S_SCORE] / N;
+        st.phiphi = s[S_SCORE2] / N;
+        st.xi = s[S_XI] / N;
+        st.xixi = s[S_XI2] / N;
+        st.phi_VR = s[S_VR] / N;
+        st.phiphi_VR = s[S_VR2] / N;
+        st.xiphi = s[S_XISCORE] / N;
+        st.RNGC = s[S_RNG];
+        st.APL = s[S_TAU] / N;
+        st.B = a.node_mode ? s[S_B] / N : 0.0;
+        st.paths = np;
+        st.path_steps = my_steps;
+        st.n_max_steps = my_max;
+        st.n_domain_exits = (int32_t)my_dom;
+        st.status = has_nan ? SPAC_E_NAN : (my_max ? SPAC_E_MAX_STEPS : SPAC_OK);
+        a.stats[solve] = st;
+    }
+}
+
+/* ------------------------------------------------------ host side -- */
+template <class Coef>
+static void launch_paths_coef(int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
+    if (sub == SUB_CIRCLE)
+        hipLaunchKernelGGL((fk_path_kernel_fp64<Coef, SUB_CIRCLE>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((fk_path_kernel_fp64<Coef, SUB_NONE>), grid, block, 0, s, a);
+}
+
+static void launch_paths(int coef, int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
+    if (coef == SPAC_COEF_FRANKE_POISSON) launch_paths_coef<CoefFranke>(sub, grid, block, s, a);
+    else launch_paths_coef<CoefVarFranke>(sub, grid, block, s, a);
+}
+
+spac_status DeviceBuffer::ensure(size_t bytes) {
+    if (bytes <= cap) return SPAC_OK;
+    if (ptr) { (void)hipFree(ptr); ptr = nullptr; cap = 0; }
+    size_t want = std::max(bytes, (size_t)256);
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return set_error(SPAC_E_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    cap = want;
+    return SPAC_OK;
+}
+void DeviceBuffer::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+spac_status Problem::ensure_records(int64_t total_paths) {
+    spac_status rc;
+    size_t n = (size_t)std::max<int64_t>(total_paths, 1);
+    if ((rc = rec_xt.ensure(n * 8))) return rc;
+    if ((rc = rec_yt.ensure(n * 8))) return rc;
+    if ((rc = rec_Y.ensure(n * 8))) return rc;
+    if ((rc = rec_Z.ensure(n * 8))) return rc;
+    if ((rc = rec_xi.ensure(n * 8))) return rc;
+    if ((rc = rec_tau.ensure(n * 8))) return rc;
+    if ((rc = rec_gval.ensure(n * 8))) return rc;
+    if ((rc = rec_steps.ensure(n * 4))) return rc;
+    if ((rc = rec_outcome.ensure(n))) return rc;
+    return SPAC_OK;
+}
+
+static spac_status check_problem(const spac_problem_desc* d) {
+    if (d->coef_kind != SPAC_COEF_FRANKE_POISSON && d->coef_kind != SPAC_COEF_VARCOEF_FRANKE)
+        return set_error(SPAC_E_UNREGISTERED_PLUGIN, "coefficient set %d has no device implementation", d->coef_kind);
+    if (d->domain_kind != SPAC_BOUNDARY_RECTANGLE)
+        return set_error(SPAC_E_UNREGISTERED_PLUGIN, "domain boundary kind %d has no device implementation", d->domain_kind);
+    if (d->subdomain_kind != SPAC_BOUNDARY_NONE && d->subdomain_kind != SPAC_BOUNDARY_CIRCLE)
+        return set_error(SPAC_E_UNREGISTERED_PLUGIN, "subdomain boundary kind %d has no device implementation",
+                         d->subdomain_kind);
+    if (d->precision != SPAC_FP64 && d->precision != SPAC_FP32)
+        return set_error(SPAC_E_INVALID_ARG, "precision %d unknown", d->precision);
+    if (!(d->domain_params[2] > d->domain_params[0]) || !(d->domain_params[3] > d->domain_params[1]))
+        return set_error(SPAC_E_INVALID_ARG, "rectangle domain must have xmax > xmin and ymax > ymin");
+    if (d->subdomain_kind == SPAC_BOUNDARY_CIRCLE && !(d->subdomain_params[0] > 0.0))
+        return set_error(SPAC_E_INVALID_ARG, "circle subdomain radius must be > 0");
+    return SPAC_OK;
+}
+
+/* Launch the persistent path kernel and, optionally, the reduction. */
+spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode) {
+    spac_status rc;
+    if ((rc = ensure_records(ss.total_paths))) return rc;
+    if ((rc = queue.ensure(64))) return rc;
+    PathArgs a;
+    memset(&a, 0, sizeof a);
+    for (int i = 0; i < 4; ++i) a.dom[i] = desc.domain_params[i];
+    a.gobet = desc.gobet_constant;
+    a.seed = seed;
+    a.max_steps = desc.max_steps > 0 ? desc.max_steps : 0x7fffffff;
+    a.x0 = ss.x0; a.y0 = ss.y0;
+    a.h = ss.h; a.h_uniform = ss.h_uniform;
+    a.stream = ss.stream;
+    a.circle = ss.circle;
+    for (int i = 0; i < 3; ++i) a.circle_uniform[i] = desc.subdomain_params[i];
+    a.rec_off = ss.rec_off;
+    a.paths_uniform = ss.paths_uniform;
+    a.chunk = (uint32_t)launch.chunk_paths;
+    a.items = ss.items;
+    a.n_items = ss.n_items;
+    a.chunks_per_solve = ss.chunks_per_solve;
+    a.queue = (int32_t*)queue.ptr;
+    a.xt = (double*)rec_xt.ptr; a.yt = (double*)rec_yt.ptr;
+    a.Y = (double*)rec_Y.ptr; a.Z = (double*)rec_Z.ptr; a.xi = (double*)rec_xi.ptr;
+    a.tau = (double*)rec_tau.ptr; a.gval = (double*)rec_gval.ptr;
+    a.steps = (int32_t*)rec_steps.ptr; a.outcome = (int8_t*)rec_outcome.ptr;
+
+    int sub = (ss.circle || desc.subdomain_kind == SPAC_BOUNDARY_CIRCLE) ? SUB_CIRCLE : SUB_NONE;
+    int block = launch.block_size;
+    int blocks = num_cus * launch.blocks_per_cu;
+    int64_t waves_needed = (ss.n_items + 0);
+    int64_t max_blocks = std::max<int64_t>(1, (waves_needed + (block / 64) - 1) / (block / 64));
+    if (blocks > max_blocks) blocks = (int)max_blocks;
+
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
+    SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
+    launch_paths(desc.coef_kind, sub, dim3(blocks), dim3(block), stream, a);
+    SPAC_HIP_CHECK(hipGetLastError());
+    SPAC_HIP_CHECK(hipEventRecord(ev[1], stream));
+    if (d_stats) {
+        ReduceArgs r;
+        memset(&r, 0, sizeof r);
+        r.n_solves = ss.n;
+        r.node_mode = node_mode;
+        r.paths = ss.paths;
+        r.paths_uniform = ss.paths_uniform;
+        r.rec_off = ss.rec_off;
+        r.Y = a.Y; r.Z = a.Z; r.xi = a.xi; r.tau = a.tau; r.gval = a.gval;
+        r.steps = a.steps; r.outcome = a.outcome;
+        r.stats = d_stats;
+        hipLaunchKernelGGL(fk_reduce_kernel, dim3(ss.n), dim3(64), 0, stream, r);
+        SPAC_HIP_CHECK(hipGetLastError());
+    }
+    SPAC_HIP_CHECK(hipEventRecord(ev[2], stream));
+    SPAC_HIP_CHECK(hipStreamSynchronize(stream));
+    float ms_path = 0.f, ms_red = 0.f;
+    SPAC_HIP_CHECK(hipEventElapsedTime(&ms_path, ev[0], ev[1]));
+    SPAC_HIP_CHECK(hipEventElapsedTime(&ms_red, ev[1], ev[2]));
+    last.path_kernel_ms = ms_path;
+    last.reduce_kernel_ms = ms_red;
+    last.total_ms = (double)ms_path + (double)ms_red;
+    last.path_kernel_launches = 1;
+    last.grid_blocks = blocks;
+    last.path_steps = 0;
+    return SPAC_OK;
+}
+
+} // namespace spac
+
+using namespace spac;
+
+/* ================================================================ C ABI == */
+extern "C" {
+
+const char* spac_last_error(void) { return g_last_error.c_str(); }
+int32_t spac_abi_version(void) { return SPAC_ABI_VERSION; }
+
+int32_t spac_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+spac_status spac_fk_problem_create(const spac_problem_desc* desc, spac_problem** out) {
+    if (!desc || !out) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    *out = nullptr;
+    spac_status rc = check_problem(desc);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_error(SPAC_E_NO_DEVICE, "no HIP device visible");
+    Problem* p = new Problem();
+    p->desc = *desc;
+    SPAC_HIP_CHECK(hipGetDevice(&p->device));
+    hipDeviceProp_t prop;
+    SPAC_HIP_CHECK(hipGetDeviceProperties(&prop, p->device));
+    p->num_cus = prop.multiProcessorCount;
+    SPAC_HIP_CHECK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    for (int i = 0; i < 4; ++i) SPAC_HIP_CHECK(hipEventCreate(&p->ev[i]));
+    p->launch.block_size = 256;
+    p->launch.blocks_per_cu = 8;
+    p->launch.chunk_paths = 256;
+    *out = reinterpret_cast<spac_problem*>(p);
+    return SPAC_OK;
+}
+
+spac_status spac_fk_problem_destroy(spac_problem* hp) {
+    if (!hp) return SPAC_OK;
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    (void)hipStreamSynchronize(p->stream);
+    p->release_all();
+    for (int i = 0; i < 4; ++i) (void)hipEventDestroy(p->ev[i]);
+    (void)hipStreamDestroy(p->stream);
+    delete p;
+    return SPAC_OK;
+}
+
+spac_status spac_fk_set_launch(spac_problem* hp, const spac_launch_config* cfg) {
+    if (!hp || !cfg) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    if (cfg->block_size) {
+        if (cfg->block_size % 64 || cfg->block_size > 256 || cfg->block_size < 64)
+            return set_error(SPAC_E_INVALID_ARG, "block_size must be 64, 128, 192 or 256");
+        p->launch.block_size = cfg->block_size;
+    }
+    if (cfg->blocks_per_cu) {
+        if (cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64)
+            return set_error(SPAC_E_INVALID_ARG, "blocks_per_cu must be in [1, 64]");
+        p->launch.blocks_per_cu = cfg->blocks_per_cu;
+    }
+    if (cfg->chunk_paths) {
+        if (cfg->chunk_paths < 1) return set_error(SPAC_E_INVALID_ARG, "chunk_paths must be >= 1");
+        p->launch.chunk_paths = cfg->chunk_paths;
+    }
+    return SPAC_OK;
+}
+
+spac_status spac_fk_last_timing(spac_problem* hp, spac_timing* out) {
+    if (!hp || !out) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    *out = reinterpret_cast<Problem*>(hp)->last;
+    return SPAC_OK;
+}
+
+static spac_status solve_points_impl(Problem* p, const double* d_x0, const double* d_y0, int32_t n, uint32_t paths,
+                                     double h, uint64_t seed, spac_point_stats* d_stats) {
+    SolveSet ss;
+    ss.n = n;
+    ss.x0 = d_x0;
+    ss.y0 = d_y0;
+    ss.h_uniform = h;
+    ss.paths_uniform = paths;
+    ss.total_paths = (int64_t)n * (int64_t)paths;
+    ss.chunks_per_solve = (int32_t)((paths + (uint32_t)p->launch.chunk_paths - 1) / (uint32_t)p->launch.chunk_paths);
+    int64_t items = (int64_t)ss.chunks_per_solve * n;
+    if (items > 0x7fffffff) return set_error(SPAC_E_INVALID_ARG, "too many work items (%lld)", (long long)items);
+    ss.n_items = (int32_t)items;
+    spac_status rc = p->run(ss, seed, d_stats, 0);
+    if (rc) return rc;
+    p->last.path_steps = -1;
+    return SPAC_OK;
+}
+
+static spac_status check_points_args(const void* x, const void* y, int32_t n, uint32_t paths, double h, const void* st) {
+    if (!x || !y || !st) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (n < 1) return set_error(SPAC_E_INVALID_ARG, "n must be >= 1");
+    if (paths < 1) return set_error(SPAC_E_INVALID_ARG, "paths must be >= 1");
+    if (!(h > 0.0) || !(h < 1e30)) return set_error(SPAC_E_INVALID_ARG, "h must be positive and finite");
+    return SPAC_OK;
+}
+
+spac_status spac_fk_solve_points_device(spac_problem* hp, const double* d_x0, const double* d_y0, int32_t n,
+                                        uint32_t paths, double h, uint64_t seed, spac_point_stats* d_stats) {
+    if (!hp) return set_error(SPAC_E_INVALID_ARG, "null problem");
+    spac_status rc = check_points_args(d_x0, d_y0, n, paths, h, d_stats);
+    if (rc) return rc;
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    if (p->desc.precision != SPAC_FP64 && p->desc.precision != SPAC_FP32)
+        return set_error(SPAC_E_INVALID_ARG, "bad precision");
+    return solve_points_impl(p, d_x0, d_y0, n, paths, h, seed, d_stats);
+}
+
+spac_status spac_fk_solve_points(spac_problem* hp, const double* x0, const double* y0, int32_t n, uint32_t paths,
+                                 double h, uint64_t seed, spac_point_stats* stats) {
+    if (!hp) return set_error(SPAC_E_INVALID_ARG, "null problem");
+    spac_status rc = check_points_args(x0, y0, n, paths, h, stats);
+    if (rc) return rc;
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    if ((rc = p->sx.ensure((size_t)n * 8))) return rc;
+    if ((rc = p->sy.ensure((size_t)n * 8))) return rc;
+    if ((rc = p->stats.ensure((size_t)n * sizeof(spac_point_stats)))) return rc;
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->sx.ptr, x0, (size_t)n * 8, hipMemcpyHostToDevice, p->stream));
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->sy.ptr, y0, (size_t)n * 8, hipMemcpyHostToDevice, p->stream));
+    rc = solve_points_impl(p, (const double*)p->sx.ptr, (const double*)p->sy.ptr, n, paths, h, seed,
+                           (spac_point_stats*)p->stats.ptr);
+    if (rc) return rc;
+    SPAC_HIP_CHECK(hipMemcpy(stats, p->stats.ptr, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
+    spac_status worst = SPAC_OK;
+    for (int i = 0; i < n; ++i)
+        if (stats[i].status < worst) worst = stats[i].status;
+    if (worst == SPAC_E_NAN) return set_error(SPAC_E_NAN, "NaN reached an estimator");
+    if (worst == SPAC_E_MAX_STEPS) return set_error(SPAC_E_MAX_STEPS, "paths stopped by max_steps");
+    return SPAC_OK;
+}
+
+spac_status spac_fk_simulate(spac_problem* hp, const spac_solve_batch* b, uint64_t seed, spac_path_records* rec,
+                             spac_point_stats* stats) {
+    if (!hp || !b) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (b->n < 1 || !b->x0 || !b->y0 || !b->paths || !b->h || !b->stream)
+        return set_error(SPAC_E_INVALID_ARG, "batch needs n >= 1 and x0, y0, paths, h, stream");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    const int32_t n = b->n;
+    std::vector<int64_t> off(n + 1, 0);
+    std::vector<int4> items;
+    const uint32_t chunk = (uint32_t)p->launch.chunk_paths;
+    for (int k = 0; k < n; ++k) {
+        if (b->paths[k] < 1) return set_error(SPAC_E_INVALID_ARG, "solve %d has no paths", k);
+        if (!(b->h[k] > 0.0) || !(b->h[k] < 1e30)) return set_error(SPAC_E_INVALID_ARG, "solve %d has bad h", k);
+        if (b->circle && !(b->circle[3 * k] > 0.0))
+            return set_error(SPAC_E_INVALID_ARG, "solve %d has a circle of radius <= 0", k);
+        off[k + 1] = off[k] + b->paths[k];
+        for (uint32_t p0 = 0; p0 < b->paths[k]; p0 += chunk) {
+            uint32_t p1 = std::min(p0 + chunk, b->paths[k]);
+            items.push_back(make_int4(k, (int)p0, (int)p1, 0));
+        }
+    }
+    if (items.size() > 0x7fffffffu) return set_error(SPAC_E_INVALID_ARG, "too many work items");
+    spac_status rc;
+    size_t nb = (size_t)n;
+    if ((rc = p->sx.ensure(nb * 8))) return rc;
+    if ((rc = p->sy.ensure(nb * 8))) return rc;
+    if ((rc = p->sh.ensure(nb * 8))) return rc;
+    if ((rc = p->sstream.ensure(nb * 4))) return rc;
+    if ((rc = p->spaths.ensure(nb * 4))) return rc;
+    if ((rc = p->soff.ensure((nb + 1) * 8))) return rc;
+    if ((rc = p->scircle.ensure(nb * 24))) return rc;
+    if ((rc = p->sitems.ensure(items.size() * sizeof(int4)))) return rc;
+    if ((rc = p->stats.ensure(nb * sizeof(spac_point_stats)))) return rc;
+    hipStream_t s = p->stream;
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->sx.ptr, b->x0, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->sy.ptr, b->y0, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->sh.ptr, b->h, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->sstream.ptr, b->stream, nb * 4, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->spaths.ptr, b->paths, nb * 4, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->soff.ptr, off.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
+    if (b->circle) SPAC_HIP_CHECK(hipMemcpyAsync(p->scircle.ptr, b->circle, nb * 24, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->sitems.ptr, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    SolveSet ss;
+    ss.n = n;
+    ss.x0 = (const double*)p->sx.ptr;
+    ss.y0 = (const double*)p->sy.ptr;
+    ss.h = (const double*)p->sh.ptr;
+    ss.stream = (const uint32_t*)p->sstream.ptr;
+    ss.paths = (const uint32_t*)p->spaths.ptr;
+    ss.rec_off = (const int64_t*)p->soff.ptr;
+    ss.circle = b->circle ? (const double*)p->scircle.ptr : nullptr;
+    ss.items = (const int4*)p->sitems.ptr;
+    ss.n_items = (int32_t)items.size();
+    ss.total_paths = off[n];
+    rc = p->run(ss, seed, stats ? (spac_point_stats*)p->stats.ptr : nullptr, b->circle ? 1 : 0);
+    if (rc) return rc;
+    size_t tp = (size_t)off[n];
+    if (rec) {
+        if (rec->x_tau) SPAC_HIP_CHECK(hipMemcpy(rec->x_tau, p->rec_xt.ptr, tp * 8, hipMemcpyDeviceToHost));
+        if (rec->y_tau) SPAC_HIP_CHECK(hipMemcpy(rec->y_tau, p->rec_yt.ptr, tp * 8, hipMemcpyDeviceToHost));
+        if (rec->Y) SPAC_HIP_CHECK(hipMemcpy(rec->Y, p->rec_Y.ptr, tp * 8, hipMemcpyDeviceToHost));
+        if (rec->Z) SPAC_HIP_CHECK(hipMemcpy(rec->Z, p->rec_Z.ptr, tp * 8, hipMemcpyDeviceToHost));
+        if (rec->xi) SPAC_HIP_CHECK(hipMemcpy(rec->xi, p->rec_xi.ptr, tp * 8, hipMemcpyDeviceToHost));
+        if (rec->tau) SPAC_HIP_CHECK(hipMemcpy(rec->tau, p->rec_tau.ptr, tp * 8, hipMemcpyDeviceToHost));
+        if (rec->gval) SPAC_HIP_CHECK(hipMemcpy(rec->gval, p->rec_gval.ptr, tp * 8, hipMemcpyDeviceToHost));
+        if (rec->steps) SPAC_HIP_CHECK(hipMemcpy(rec->steps, p->rec_steps.ptr, tp * 4, hipMemcpyDeviceToHost));
+        if (rec->outcome) SPAC_HIP_CHECK(hipMemcpy(rec->outcome, p->rec_outcome.ptr, tp, hipMemcpyDeviceToHost));
+    }
+    if (stats) SPAC_HIP_CHECK(hipMemcpy(stats, p->stats.ptr, nb * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
+    return SPAC_OK;
+}
+
+} /* extern "C" */

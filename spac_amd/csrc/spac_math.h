@@ -1,0 +1,301 @@
+/*
+ * spac_math.h — FP64 elementary functions, Philox4x32-10 and Box–Muller normals,
+ * written once and compiled both as C (gcc, CPU) and as HIP (device, gfx950).
+ *
+ * Why this file exists: the reference draws its Brownian increments from
+ * boost::mt19937 + normal_distribution and evaluates exp/pow through glibc
+ * (Solvers/FeynmanKacSolver.cpp:2-6, BVPs/Franke_Poisson.hpp:4-17).  Neither
+ * stream nor libm bits can be reproduced on a GPU, so this project defines its
+ * own: every function here uses only IEEE-754 operations that are correctly
+ * rounded on both targets (+, -, *, /, fma, sqrt, rint) and plain integer
+ * arithmetic, with floating-point contraction disabled at build time
+ * (-ffp-contract=off).  Given the same inputs a CPU build and a GPU build
+ * therefore produce the same bits, which is what lets per-path exit data be
+ * compared exactly between the HIP kernels and the CPU restatement.
+ *
+ * Methods (all from the mathematics; constants derived with exact rational
+ * arithmetic):
+ *   exp   : Cody–Waite reduction x = k ln2 + r with a 32-bit ln2 head,
+ *           Taylor polynomial of degree 13 on |r| <= ln2/2, scaling by 2^k.
+ *   log   : x = 2^e m, m in [sqrt(2)/2, sqrt(2)), f = (m-1)/(m+1),
+ *           log m = 2 atanh f as an odd series in f (12 terms).
+ *   sincospi : exact reduction a = q/2 + r, |r| <= 1/4, Taylor series of
+ *           sin(pi r) and cos(pi r), quadrant rotation by q mod 4.
+ *   atan2 : octant reduction to t in [0,1], t > tan(pi/8) mapped by
+ *           atan t = pi/4 + atan((t-1)/(t+1)), one half-angle step
+ *           atan u = 2 atan(u / (1 + sqrt(1+u^2))), odd Taylor series.
+ *   Philox4x32-10 : Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ *           easy as 1, 2, 3", SC'11 (multipliers, Weyl key increments, 10 rounds).
+ *   Box–Muller : two 53-bit uniforms from one Philox block give two N(0,1).
+ */
+#ifndef SPAC_MATH_H
+#define SPAC_MATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define SPAC_HD __host__ __device__ __forceinline__
+#define SPAC_RINT(x) __builtin_rint(x)
+#define SPAC_FMA(a, b, c) __builtin_fma((a), (b), (c))
+#define SPAC_SQRT(x) __builtin_sqrt(x)
+#define SPAC_FABS(x) __builtin_fabs(x)
+#define SPAC_COPYSIGN(a, b) __builtin_copysign((a), (b))
+#else
+#include <math.h>
+#define SPAC_HD static inline
+#define SPAC_RINT(x) rint(x)
+#define SPAC_FMA(a, b, c) fma((a), (b), (c))
+#define SPAC_SQRT(x) sqrt(x)
+#define SPAC_FABS(x) fabs(x)
+#define SPAC_COPYSIGN(a, b) copysign((a), (b))
+#endif
+
+#define SPAC_PI 3.141592653589793
+#define SPAC_PI_2 1.5707963267948966
+#define SPAC_PI_4 0.7853981633974483
+
+/* ---------------------------------------------------------------- bits -- */
+SPAC_HD double spac_u64_as_f64(uint64_t u) {
+    union { uint64_t u; double d; } v;
+    v.u = u;
+    return v.d;
+}
+SPAC_HD uint64_t spac_f64_as_u64(double d) {
+    union { uint64_t u; double d; } v;
+    v.d = d;
+    return v.u;
+}
+/* 2^k for k in [-1022, 1023] */
+SPAC_HD double spac_pow2i(int k) {
+    return spac_u64_as_f64((uint64_t)(k + 1023) << 52);
+}
+
+/* ----------------------------------------------------------------- exp -- */
+SPAC_HD double spac_exp(double x) {
+    const double LOG2E = 1.4426950408889634;
+    const double LN2_HI = 0.6931471803691238;     /* 32 significant bits */
+    const double LN2_LO = 1.9082149292705877e-10; /* ln2 - LN2_HI */
+    if (!(x < 709.782712893384)) return (x != x) ? x : spac_u64_as_f64(0x7ff0000000000000ULL);
+    if (x < -745.1332191019412) return 0.0;
+    double kd = SPAC_RINT(x * LOG2E);
+    double r = SPAC_FMA(-kd, LN2_HI, x); /* exact: |k| <= 1075 needs 11 bits */
+    r = SPAC_FMA(-kd, LN2_LO, r);
+    /* e^r = 1 + r + r^2/2! + ... + r^13/13!, |r| <= 0.3466 */
+    double p = 1.6059043836821613e-10;
+    p = SPAC_FMA(p, r, 2.08767569878681e-09);
+    p = SPAC_FMA(p, r, 2.505210838544172e-08);
+    p = SPAC_FMA(p, r, 2.755731922398589e-07);
+    p = SPAC_FMA(p, r, 2.7557319223985893e-06);
+    p = SPAC_FMA(p, r, 2.48015873015873e-05);
+    p = SPAC_FMA(p, r, 0.0001984126984126984);
+    p = SPAC_FMA(p, r, 0.001388888888888889);
+    p = SPAC_FMA(p, r, 0.008333333333333333);
+    p = SPAC_FMA(p, r, 0.041666666666666664);
+    p = SPAC_FMA(p, r, 0.16666666666666666);
+    p = SPAC_FMA(p, r, 0.5);
+    p = SPAC_FMA(p, r, 1.0);
+    p = SPAC_FMA(p, r, 1.0);
+    int k = (int)kd;
+    if (k > 1023) return (p * 2.0) * spac_pow2i(k - 1);
+    if (k < -1021) return (p * spac_pow2i(k + 54)) * spac_pow2i(-54);
+    return p * spac_pow2i(k);
+}
+
+/* ----------------------------------------------------------------- log -- */
+/* Natural log for finite x > 0 (the only domain the solver uses). */
+SPAC_HD double spac_log(double x) {
+    const double LN2_HI = 0.6931471803691238;
+    const double LN2_LO = 1.9082149292705877e-10;
+    if (!(x > 0.0)) return (x == 0.0) ? -spac_u64_as_f64(0x7ff0000000000000ULL)
+                                      : spac_u64_as_f64(0x7ff8000000000000ULL);
+    int scale = 0;
+    if (x < 2.2250738585072014e-308) { x *= 18014398509481984.0; scale = -54; } /* 2^54 */
+    uint64_t bits = spac_f64_as_u64(x);
+    int e = (int)((bits >> 52) & 0x7ff) - 1023;
+    double m = spac_u64_as_f64((bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL); /* [1,2) */
+    if (m > 1.4142135623730951) { m *= 0.5; e += 1; }
+    e += scale;
+    double f = (m - 1.0) / (m + 1.0);
+    double s = f * f;
+    /* log m = 2f (1 + s/3 + s^2/5 + ... + s^11/23) */
+    double p = 0.08695652173913043;
+    p = SPAC_FMA(p, s, 0.09523809523809523);
+    p = SPAC_FMA(p, s, 0.10526315789473684);
+    p = SPAC_FMA(p, s, 0.11764705882352941);
+    p = SPAC_FMA(p, s, 0.13333333333333333);
+    p = SPAC_FMA(p, s, 0.15384615384615385);
+    p = SPAC_FMA(p, s, 0.18181818181818182);
+    p = SPAC_FMA(p, s, 0.2222222222222222);
+    p = SPAC_FMA(p, s, 0.2857142857142857);
+    p = SPAC_FMA(p, s, 0.4);
+    p = SPAC_FMA(p, s, 0.6666666666666666);
+    double logm = SPAC_FMA(f * s, p, 2.0 * f);
+    double ed = (double)e;
+    return SPAC_FMA(ed, LN2_HI, SPAC_FMA(ed, LN2_LO, logm));
+}
+
+/* ------------------------------------------------------------ sincospi -- */
+/* sin(pi a), cos(pi a) for |a| < 2^51. */
+SPAC_HD void spac_sincospi(double a, double* s_out, double* c_out) {
+    double q = SPAC_RINT(a * 2.0);
+    double r = a - 0.5 * q; /* exact, |r| <= 1/4 */
+    double r2 = r * r;
+    double sp = -2.2948428997269873e-08;
+    sp = SPAC_FMA(sp, r2, 7.952054001475513e-07);
+    sp = SPAC_FMA(sp, r2, -2.1915353447830217e-05);
+    sp = SPAC_FMA(sp, r2, 0.00046630280576761255);
+    sp = SPAC_FMA(sp, r2, -0.0073704309457143504);
+    sp = SPAC_FMA(sp, r2, 0.08214588661112823);
+    sp = SPAC_FMA(sp, r2, -0.5992645293207921);
+    sp = SPAC_FMA(sp, r2, 2.5501640398773455);
+    sp = SPAC_FMA(sp, r2, -5.16771278004997);
+    double sr = SPAC_FMA(r * r2, sp, r * 3.141592653589793);
+    double cp = -1.3878952462213771e-07;
+    cp = SPAC_FMA(cp, r2, 4.303069587032947e-06);
+    cp = SPAC_FMA(cp, r2, -0.0001046381049248457);
+    cp = SPAC_FMA(cp, r2, 0.0019295743094039231);
+    cp = SPAC_FMA(cp, r2, -0.02580689139001406);
+    cp = SPAC_FMA(cp, r2, 0.2353306303588932);
+    cp = SPAC_FMA(cp, r2, -1.3352627688545895);
+    cp = SPAC_FMA(cp, r2, 4.0587121264167685);
+    cp = SPAC_FMA(cp, r2, -4.934802200544679);
+    double cr = SPAC_FMA(cp, r2, 1.0);
+    int64_t qi = (int64_t)q;
+    int quad = (int)(qi & 3);
+    double s, c;
+    switch (quad) {
+    case 0: s = sr; c = cr; break;
+    case 1: s = cr; c = -sr; break;
+    case 2: s = -sr; c = -cr; break;
+    default: s = -cr; c = sr; break;
+    }
+    *s_out = s;
+    *c_out = c;
+}
+SPAC_HD double spac_sinpi(double a) { double s, c; spac_sincospi(a, &s, &c); return s; }
+SPAC_HD double spac_cospi(double a) { double s, c; spac_sincospi(a, &s, &c); return c; }
+
+/* --------------------------------------------------------------- atan2 -- */
+/* atan(u) for |u| <= 0.4143 */
+SPAC_HD double spac_atan_small(double u) {
+    /* half-angle step: atan u = 2 atan(v), v = u / (1 + sqrt(1 + u^2)), |v| <= 0.199 */
+    double v = u / (1.0 + SPAC_SQRT(SPAC_FMA(u, u, 1.0)));
+    double v2 = v * v;
+    double p = -0.043478260869565216;
+    p = SPAC_FMA(p, v2, 0.047619047619047616);
+    p = SPAC_FMA(p, v2, -0.05263157894736842);
+    p = SPAC_FMA(p, v2, 0.058823529411764705);
+    p = SPAC_FMA(p, v2, -0.06666666666666667);
+    p = SPAC_FMA(p, v2, 0.07692307692307693);
+    p = SPAC_FMA(p, v2, -0.09090909090909091);
+    p = SPAC_FMA(p, v2, 0.1111111111111111);
+    p = SPAC_FMA(p, v2, -0.14285714285714285);
+    p = SPAC_FMA(p, v2, 0.2);
+    p = SPAC_FMA(p, v2, -0.3333333333333333);
+    double at = SPAC_FMA(v * v2, p, v);
+    return 2.0 * at;
+}
+SPAC_HD double spac_atan2(double y, double x) {
+    double ax = SPAC_FABS(x), ay = SPAC_FABS(y);
+    if (ax == 0.0 && ay == 0.0) {
+        double base = (spac_f64_as_u64(x) >> 63) ? SPAC_PI : 0.0;
+        return SPAC_COPYSIGN(base, y);
+    }
+    int swap = ay > ax;
+    double t = swap ? (ax / ay) : (ay / ax); /* [0, 1] */
+    double a;
+    if (t > 0.41421356237309503) {
+        a = SPAC_PI_4 + spac_atan_small((t - 1.0) / (t + 1.0));
+    } else {
+        a = spac_atan_small(t);
+    }
+    if (swap) a = SPAC_PI_2 - a;
+    if (spac_f64_as_u64(x) >> 63) a = SPAC_PI - a;
+    return SPAC_COPYSIGN(a, y);
+}
+
+/* -------------------------------------------------------------- Philox -- */
+typedef struct { uint32_t v[4]; } spac_u32x4;
+
+SPAC_HD void spac_mulhilo32(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo) {
+    uint64_t p = (uint64_t)a * (uint64_t)b;
+    *hi = (uint32_t)(p >> 32);
+    *lo = (uint32_t)p;
+}
+
+/* Philox4x32 with R rounds (R = 10 is the standard generator). */
+SPAC_HD spac_u32x4 spac_philox4x32_R(int R, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                      uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < R; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t hi0, lo0, hi1, lo1;
+        spac_mulhilo32(0xD2511F53u, c0, &hi0, &lo0);
+        spac_mulhilo32(0xCD9E8D57u, c2, &hi1, &lo1);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    spac_u32x4 out;
+    out.v[0] = c0; out.v[1] = c1; out.v[2] = c2; out.v[3] = c3;
+    return out;
+}
+SPAC_HD spac_u32x4 spac_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1) {
+    return spac_philox4x32_R(10, c0, c1, c2, c3, k0, k1);
+}
+
+/* 53-bit uniform in [0, 1) from two 32-bit words (a supplies the top 27 bits). */
+SPAC_HD double spac_u01_53(uint32_t a, uint32_t b) {
+    uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+    return (double)m * 1.1102230246251565e-16; /* 2^-53 */
+}
+
+/*
+ * Stream layout shared by every kernel and the CPU restatement:
+ *   key     = (low 32 bits of the seed, stream id = solve/node index)
+ *   counter = (step index, path index, high 32 bits of the seed, purpose tag)
+ * One Philox block per path-step gives the two Brownian normals of that step.
+ */
+#define SPAC_TAG_BROWNIAN 0u
+#define SPAC_TAG_WORKLOAD 7u
+
+SPAC_HD void spac_normal_pair(uint64_t seed, uint32_t stream, uint32_t path, uint32_t step,
+                              double* n0, double* n1) {
+    spac_u32x4 r = spac_philox4x32_10(step, path, (uint32_t)(seed >> 32), SPAC_TAG_BROWNIAN,
+                                      (uint32_t)seed, stream);
+    double u1 = 1.0 - spac_u01_53(r.v[0], r.v[1]); /* (0, 1] */
+    double u2 = spac_u01_53(r.v[2], r.v[3]);       /* [0, 1) */
+    double rad = SPAC_SQRT(-2.0 * spac_log(u1));
+    double s, c;
+    spac_sincospi(2.0 * u2, &s, &c);
+    *n0 = rad * c;
+    *n1 = rad * s;
+}
+
+/* -------------------------------------------- compensated accumulation -- */
+/* Knuth TwoSum: s + e == a + b exactly. */
+typedef struct { double s, c; } spac_ksum;
+
+SPAC_HD spac_ksum spac_ksum_zero(void) { spac_ksum k; k.s = 0.0; k.c = 0.0; return k; }
+SPAC_HD spac_ksum spac_ksum_add(spac_ksum a, double x) {
+    double t = a.s + x;
+    double bp = t - a.s;
+    double err = (a.s - (t - bp)) + (x - bp);
+    spac_ksum r;
+    r.s = t;
+    r.c = a.c + err;
+    return r;
+}
+SPAC_HD spac_ksum spac_ksum_merge(spac_ksum a, spac_ksum b) {
+    double t = a.s + b.s;
+    double bp = t - a.s;
+    double err = (a.s - (t - bp)) + (b.s - bp);
+    spac_ksum r;
+    r.s = t;
+    r.c = (a.c + b.c) + err;
+    return r;
+}
+SPAC_HD double spac_ksum_value(spac_ksum a) { return a.s + a.c; }
+
+#endif /* SPAC_MATH_H */

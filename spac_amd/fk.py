@@ -1,0 +1,140 @@
+"""Python handle over the C ABI (tests, bench and smoke use it; no compute here).
+
+Mirrors the reference's FeynmanKacSolver call pattern
+(Solvers/FeynmanKacSolver.hpp:95-98): a problem (BVP + frontiers) is created
+once, then point solves or batches of solves run on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+def problem_desc(coef=abi.SPAC_COEF_FRANKE_POISSON, domain=(-1.0, -1.0, 1.0, 1.0),
+                 subdomain_kind=abi.SPAC_BOUNDARY_NONE, subdomain=(0.0, 0.0, 0.0),
+                 gobet=abi.GOBET_CONSTANT, max_steps=0, precision=abi.SPAC_FP64) -> abi.ProblemDesc:
+    d = abi.ProblemDesc()
+    d.coef_kind = coef
+    d.domain_kind = abi.SPAC_BOUNDARY_RECTANGLE
+    for i, v in enumerate(domain):
+        d.domain_params[i] = v
+    d.subdomain_kind = subdomain_kind
+    for i, v in enumerate(subdomain):
+        d.subdomain_params[i] = v
+    d.gobet_constant = gobet
+    d.max_steps = max_steps
+    d.precision = precision
+    return d
+
+
+RECORD_FIELDS = (("x_tau", np.float64), ("y_tau", np.float64), ("Y", np.float64), ("Z", np.float64),
+                 ("xi", np.float64), ("tau", np.float64), ("gval", np.float64), ("steps", np.int32),
+                 ("outcome", np.int8))
+STATS_FIELDS = ("phi", "phiphi", "phi_VR", "phiphi_VR", "xi", "xixi", "xiphi", "APL", "RNGC", "B",
+                "paths", "path_steps", "n_max_steps", "status", "n_domain_exits")
+
+
+def alloc_records(total: int) -> dict:
+    return {name: np.zeros(total, dtype=dt) for name, dt in RECORD_FIELDS}
+
+
+def records_struct(rec: dict) -> abi.PathRecords:
+    r = abi.PathRecords()
+    ctmap = {np.float64: C.c_double, np.int32: C.c_int32, np.int8: C.c_int8}
+    for name, dt in RECORD_FIELDS:
+        setattr(r, name, abi.ptr(rec[name], ctmap[dt]))
+    return r
+
+
+def stats_to_dicts(arr) -> list[dict]:
+    return [{f: getattr(s, f) for f in STATS_FIELDS} for s in arr]
+
+
+class SolveBatchArrays:
+    """Owns the numpy arrays behind a spac_solve_batch."""
+
+    def __init__(self, x0, y0, paths, h, stream, circle=None):
+        self.x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        self.y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        n = self.x0.size
+        self.paths = np.ascontiguousarray(np.broadcast_to(paths, (n,)), dtype=np.uint32)
+        self.h = np.ascontiguousarray(np.broadcast_to(h, (n,)), dtype=np.float64)
+        self.stream = np.ascontiguousarray(np.broadcast_to(stream, (n,)) if np.ndim(stream) == 0 else stream,
+                                           dtype=np.uint32)
+        self.circle = None if circle is None else np.ascontiguousarray(circle, dtype=np.float64).reshape(-1)
+        self.n = n
+
+    @property
+    def total_paths(self) -> int:
+        return int(self.paths.astype(np.int64).sum())
+
+    def struct(self) -> abi.SolveBatch:
+        b = abi.SolveBatch()
+        b.n = self.n
+        b.x0 = abi.ptr(self.x0, C.c_double)
+        b.y0 = abi.ptr(self.y0, C.c_double)
+        b.paths = abi.ptr(self.paths, C.c_uint32)
+        b.h = abi.ptr(self.h, C.c_double)
+        b.stream = abi.ptr(self.stream, C.c_uint32)
+        b.circle = abi.ptr(self.circle, C.c_double)
+        return b
+
+
+class FKProblem:
+    """Owns one spac_problem handle (one HIP stream)."""
+
+    def __init__(self, desc: abi.ProblemDesc | None = None, **kw):
+        self.lib = abi.load()
+        self.desc = desc if desc is not None else problem_desc(**kw)
+        h = C.c_void_p()
+        abi.check(self.lib.spac_fk_problem_create(C.byref(self.desc), C.byref(h)), self.lib)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.spac_fk_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_launch(self, block_size=0, blocks_per_cu=0, chunk_paths=0):
+        cfg = abi.LaunchConfig(block_size, blocks_per_cu, chunk_paths, 0)
+        abi.check(self.lib.spac_fk_set_launch(self.h, C.byref(cfg)), self.lib)
+
+    def timing(self) -> abi.Timing:
+        t = abi.Timing()
+        abi.check(self.lib.spac_fk_last_timing(self.h, C.byref(t)), self.lib)
+        return t
+
+    def solve_points(self, x0, y0, paths, h, seed=0, allow_status=False):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        n = x0.size
+        stats = (abi.PointStats * n)()
+        rc = self.lib.spac_fk_solve_points(self.h, abi.ptr(x0, C.c_double), abi.ptr(y0, C.c_double), n, paths, h,
+                                           seed, stats)
+        if not allow_status:
+            abi.check(rc, self.lib)
+        return stats_to_dicts(stats), rc
+
+    def simulate(self, batch: SolveBatchArrays, seed=0, want_records=True, want_stats=True):
+        rec = alloc_records(batch.total_paths) if want_records else None
+        stats = (abi.PointStats * batch.n)() if want_stats else None
+        b = batch.struct()
+        rc = self.lib.spac_fk_simulate(self.h, C.byref(b), seed, C.byref(records_struct(rec)) if rec else None,
+                                       stats)
+        abi.check(rc, self.lib)
+        return rec, (stats_to_dicts(stats) if stats is not None else None)

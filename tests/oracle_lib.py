@@ -1,0 +1,119 @@
+"""ctypes loader for the CPU restatement (oracle/_build/libfk_oracle.so).
+
+Test infrastructure: only tests, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it, always as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from spac_amd import abi
+from spac_amd.fk import SolveBatchArrays, alloc_records, records_struct, stats_to_dicts
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_PATH = ROOT / "oracle" / "_build" / "libfk_oracle.so"
+
+_LIB = None
+
+
+def load() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not ORACLE_PATH.exists():
+        raise RuntimeError(f"{ORACLE_PATH} missing: run __graft_entry__.build()")
+    lib = C.CDLL(str(ORACLE_PATH))
+    dp = C.POINTER(C.c_double)
+    lib.oracle_simulate.argtypes = [C.POINTER(abi.ProblemDesc), C.POINTER(abi.SolveBatch), C.c_uint64,
+                                    C.POINTER(abi.PathRecords), C.c_int]
+    lib.oracle_simulate.restype = C.c_int
+    lib.oracle_reduce.argtypes = [C.c_int, C.POINTER(abi.SolveBatch), C.POINTER(abi.PathRecords),
+                                  C.POINTER(abi.PointStats)]
+    lib.oracle_reduce.restype = C.c_int
+    lib.oracle_math_eval.argtypes = [C.c_int, dp, dp, dp, C.c_int64]
+    lib.oracle_math_eval.restype = None
+    lib.oracle_philox.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.oracle_philox.restype = None
+    lib.oracle_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, dp]
+    lib.oracle_normals.restype = None
+    lib.oracle_coef_eval.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int64]
+    lib.oracle_coef_eval.restype = C.c_int
+    lib.oracle_boundary_eval.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int64]
+    lib.oracle_boundary_eval.restype = None
+    lib.oracle_path_terms.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_int32, C.c_int8, dp]
+    lib.oracle_path_terms.restype = None
+    _LIB = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def math_eval(fn: int, x, y=None) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64)
+    out = np.empty_like(x)
+    load().oracle_math_eval(fn, _p(x), _p(y), _p(out), x.size)
+    return out
+
+
+def philox(rounds: int, ctr, key) -> list[int]:
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    load().oracle_philox(rounds, c, k, o)
+    return list(o)
+
+
+def normals(seed: int, stream: int, path: int, nsteps: int) -> np.ndarray:
+    out = np.empty(2 * nsteps, dtype=np.float64)
+    load().oracle_normals(seed, stream, path, nsteps, _p(out))
+    return out
+
+
+def coef_eval(coef_kind: int, which: int, x, y) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(x)
+    rc = load().oracle_coef_eval(coef_kind, which, _p(x), _p(y), _p(out), x.size)
+    assert rc == 0
+    return out
+
+
+def boundary_eval(kind: int, params, x, y) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    pr = np.zeros(4, dtype=np.float64)
+    pr[: len(params)] = params
+    out = np.empty((x.size, 5), dtype=np.float64)
+    load().oracle_boundary_eval(kind, _p(pr), _p(x), _p(y), _p(out), x.size)
+    return out
+
+
+def simulate(desc: abi.ProblemDesc, batch: SolveBatchArrays, seed: int = 0, nthreads: int = 0) -> dict:
+    rec = alloc_records(batch.total_paths)
+    b = batch.struct()
+    rs = records_struct(rec)
+    rc = load().oracle_simulate(C.byref(desc), C.byref(b), seed, C.byref(rs), nthreads)
+    assert rc == 0, rc
+    return rec
+
+
+def reduce(batch: SolveBatchArrays, rec: dict, node_mode: bool = False) -> list[dict]:
+    stats = (abi.PointStats * batch.n)()
+    b = batch.struct()
+    rs = records_struct(rec)
+    rc = load().oracle_reduce(1 if node_mode else 0, C.byref(b), C.byref(rs), stats)
+    assert rc == 0
+    return stats_to_dicts(stats)
+
+
+def path_terms(node_mode, Y, Z, xi, tau, gval, steps, outcome) -> np.ndarray:
+    out = np.empty(10, dtype=np.float64)
+    load().oracle_path_terms(1 if node_mode else 0, Y, Z, xi, tau, gval, steps, outcome, _p(out))
+    return out

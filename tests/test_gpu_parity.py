@@ -1,0 +1,189 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU restatement.
+
+Bar: per-path exit data and estimators are bit-identical (FP64 plain mode,
+same Philox stream, same IEEE operation order), and independent of the launch
+shape.  Full-size properties (config 1) are checked against the closed forms.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as o
+from spac_amd import abi
+from spac_amd.fk import FKProblem, SolveBatchArrays, problem_desc
+
+pytestmark = pytest.mark.gpu
+
+FR = abi.SPAC_COEF_FRANKE_POISSON
+VC = abi.SPAC_COEF_VARCOEF_FRANKE
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _debug_math(fn, x, y):
+    lib = abi.load()
+    import ctypes as C
+
+    lib.spac_debug_math.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.c_int64]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(x)
+    abi.check(lib.spac_debug_math(fn, abi.ptr(x, C.c_double), abi.ptr(y, C.c_double), abi.ptr(out, C.c_double),
+                                  x.size), lib)
+    return out
+
+
+@pytest.mark.parametrize("fn,lo,hi", [(0, -745, 709), (1, 1e-300, 10.0), (2, -4, 4), (3, -4, 4), (4, -3, 3),
+                                      (5, 0, 1e6), (6, -5, 5)])
+def test_device_math_bits_equal_cpu(gpu, fn, lo, hi):
+    say(f"device math fn={fn}")
+    rng = np.random.default_rng(fn)
+    x = rng.uniform(lo, hi, 200000)
+    y = rng.uniform(0.1, 3, 200000) * np.where(rng.uniform(size=200000) < 0.5, -1, 1)
+    cpu = o.math_eval(fn, x, y)
+    dev = _debug_math(fn, x, y)
+    assert np.array_equal(cpu.view(np.uint64), dev.view(np.uint64))
+
+
+def test_device_normals_and_franke_bits(gpu):
+    say("device normals / Franke u")
+    paths = np.repeat(np.arange(50), 40).astype(np.float64)
+    steps = np.tile(np.arange(40), 50).astype(np.float64)
+    dev = _debug_math(8, paths, steps)
+    cpu = np.concatenate([o.normals(0, 0, p, 40)[0::2] for p in range(50)])
+    assert np.array_equal(cpu.view(np.uint64), dev.view(np.uint64))
+    rng = np.random.default_rng(7)
+    x, y = rng.uniform(-1, 1.2, 5000), rng.uniform(-1, 1.2, 5000)
+    assert np.array_equal(_debug_math(7, x, y).view(np.uint64), o.coef_eval(FR, 0, x, y).view(np.uint64))
+
+
+def _assert_records_equal(rd, ro):
+    for k in ro:
+        a, b = rd[k], ro[k]
+        if a.dtype.kind == "f":
+            same = np.array_equal(a.view(np.uint64), b.view(np.uint64))
+            if not same:
+                bad = np.nonzero(a.view(np.uint64) != b.view(np.uint64))[0]
+                rel = np.abs(a[bad] - b[bad]) / np.maximum(np.abs(b[bad]), 1e-300)
+                raise AssertionError(f"{k}: {bad.size} paths differ, first {bad[:5]}, max rel {rel.max():.3e}")
+        else:
+            assert np.array_equal(a, b), k
+
+
+def _assert_stats_equal(sd, so):
+    for a, b in zip(sd, so):
+        for k in a:
+            assert a[k] == b[k] or (isinstance(a[k], float) and math.isnan(a[k]) and math.isnan(b[k])), (k, a[k], b[k])
+
+
+CASES = {
+    "franke_no_subdomain": dict(coef=FR, domain=(-1.0, -1.0, 1.0, 1.0), circle=None, h=1e-3,
+                                x0=[0.0, 0.5, -0.9, 0.97], y0=[0.0, -0.3, 0.9, 0.0], paths=[700, 64, 1, 300]),
+    "varcoef_circles": dict(coef=VC, domain=(0.0, 0.0, 1.0, 1.0),
+                            circle=[[0.283, 0.0, 0.333], [0.283, 0.333, 0.333], [0.283, 1.0, 1.0]], h=5e-4,
+                            x0=[0.2, 0.45, 0.85], y0=[0.4, 0.2, 0.9], paths=[500, 257, 130]),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_records_and_stats_bit_identical(gpu, case):
+    c = CASES[case]
+    say(f"records parity {case}")
+    desc = problem_desc(coef=c["coef"], domain=c["domain"])
+    n = len(c["x0"])
+    b = SolveBatchArrays(c["x0"], c["y0"], c["paths"], c["h"], np.arange(n) + 40,
+                         circle=None if c["circle"] is None else np.array(c["circle"]))
+    node_mode = c["circle"] is not None
+    ro = o.simulate(desc, b, seed=2024)
+    so = o.reduce(b, ro, node_mode=node_mode)
+    with FKProblem(desc) as p:
+        rd, sd = p.simulate(b, seed=2024)
+    _assert_records_equal(rd, ro)
+    _assert_stats_equal(sd, so)
+    if node_mode:
+        assert set(np.unique(ro["outcome"])) <= {abi.SPAC_OUT_DOMAIN, abi.SPAC_OUT_SUBDOMAIN}
+        assert (ro["outcome"] == abi.SPAC_OUT_SUBDOMAIN).any()
+
+
+@pytest.mark.parametrize("launch", [(64, 1, 1), (128, 3, 7), (256, 2, 1000), (192, 8, 64)])
+def test_launch_shape_independence(gpu, launch):
+    say(f"launch shape {launch}")
+    desc = problem_desc()
+    b = SolveBatchArrays([0.1, -0.4, 0.6], [0.2, 0.7, -0.5], [333, 1000, 77], 1e-3, [1, 2, 3])
+    with FKProblem(desc) as p:
+        ref, sref = p.simulate(b, seed=7)
+        p.set_launch(*launch)
+        got, sgot = p.simulate(b, seed=7)
+    _assert_records_equal(got, ref)
+    _assert_stats_equal(sgot, sref)
+
+
+def test_point_solve_matches_batch_and_oracle(gpu):
+    say("solve_points vs oracle")
+    x = np.array([0.25, -0.6, 0.0])
+    y = np.array([0.3, 0.1, -0.75])
+    desc = problem_desc()
+    with FKProblem(desc) as p:
+        st, rc = p.solve_points(x, y, 2000, 1e-3, seed=11)
+    b = SolveBatchArrays(x, y, 2000, 1e-3, np.arange(3))
+    so = o.reduce(b, o.simulate(desc, b, seed=11))
+    _assert_stats_equal(st, so)
+
+
+def test_max_steps_and_status(gpu):
+    say("max_steps cap")
+    desc = problem_desc(max_steps=5)
+    b = SolveBatchArrays([0.0], [0.0], 128, 1e-5, [0])
+    ro = o.simulate(desc, b, seed=1)
+    with FKProblem(desc) as p:
+        rd, sd = p.simulate(b, seed=1)
+        st, rc = p.solve_points(np.array([0.0]), np.array([0.0]), 128, 1e-5, seed=1, allow_status=True)
+    _assert_records_equal(rd, ro)
+    assert sd[0]["status"] == abi.SPAC_E_MAX_STEPS and sd[0]["n_max_steps"] == 128
+    assert rc == abi.SPAC_E_MAX_STEPS
+
+
+def test_invalid_arguments_rejected(gpu):
+    say("argument checks")
+    with FKProblem(problem_desc()) as p:
+        _, rc = p.solve_points(np.array([0.0]), np.array([0.0]), 0, 1e-3, allow_status=True)
+        assert rc == abi.SPAC_E_INVALID_ARG
+        _, rc = p.solve_points(np.array([0.0]), np.array([0.0]), 10, -1.0, allow_status=True)
+        assert rc == abi.SPAC_E_INVALID_ARG
+        with pytest.raises(abi.SpacError):
+            p.set_launch(block_size=100)
+
+
+def test_config1_full_size_properties(gpu):
+    """256 grid points x 10,000 paths, h = 1e-3 on [-1,1]^2 (the plain FP64 workload)."""
+    say("config 1 full size")
+    t = -1.0 + 2.0 * (np.arange(16) + 1) / 17
+    X, Y = np.meshgrid(t, t, indexing="ij")
+    x, y = X.ravel(), Y.ravel()
+    with FKProblem(problem_desc()) as p:
+        st, rc = p.solve_points(x, y, 10000, 1e-3, seed=0)
+        st2, _ = p.solve_points(x, y, 10000, 1e-3, seed=0)
+    assert rc == 0
+    from test_oracle import franke_published, tau_closed_form
+
+    u = franke_published(x, y)
+    phi = np.array([s["phi"] for s in st])
+    se = np.sqrt(np.array([s["phiphi"] - s["phi"] ** 2 for s in st]) / 10000)
+    z = (phi - u) / se
+    assert np.abs(z).max() < 5.0 and abs(z.mean()) < 0.5
+    apl = np.array([s["APL"] for s in st])
+    assert abs(apl.mean() / tau_closed_form(x, y).mean() - 1) < 0.01
+    steps = sum(s["path_steps"] for s in st)
+    assert 3.8e8 < steps < 4.3e8
+    # bit-stable across launches
+    _assert_stats_equal(st, st2)
+    # spot-check one point against the oracle bit for bit
+    b = SolveBatchArrays(x[100:101], y[100:101], 10000, 1e-3, [100])
+    so = o.reduce(b, o.simulate(desc := problem_desc(), b, seed=0))
+    _assert_stats_equal([st[100]], so)

@@ -1,0 +1,103 @@
+"""Pins for the shared FP64 primitives (spac_amd/csrc/spac_math.h) on the CPU.
+
+- exp/log/sinpi/cospi/atan2 against libm/numpy on sweeps (ulp bounds);
+- Philox4x32-10 against the published Random123 known-answer vectors;
+- Box–Muller normals: moments and a Kolmogorov–Smirnov test.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy import stats as sst
+
+import oracle_lib as o
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def ulps(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    sp = np.spacing(np.abs(b))
+    sp[sp == 0] = np.spacing(0.0)
+    return np.abs(a - b) / sp
+
+
+def test_exp_ulp():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-745, 709, 100000), rng.uniform(-60, 1, 100000), rng.uniform(-1e-3, 1e-3, 5000)])
+    assert ulps(o.math_eval(0, x), np.exp(x)).max() <= 1.0
+    sp = o.math_eval(0, np.array([0.0, 710.0, -746.0, -745.0]))
+    assert sp[0] == 1.0 and np.isinf(sp[1]) and sp[2] == 0.0 and sp[3] > 0.0
+
+
+def test_log_ulp():
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(1e-300, 1, 100000), 10 ** rng.uniform(-307, 300, 50000),
+                        np.array([5e-324, 1e-310, 1.0, 2.0, 0.5])])
+    assert ulps(o.math_eval(1, x), np.log(x)).max() <= 2.0
+    assert o.math_eval(1, np.array([1.0]))[0] == 0.0
+
+
+def _sincospi_ref(x):
+    pi = np.longdouble("3.14159265358979323846264338327950288")
+    xl = x.astype(np.longdouble)
+    q = np.rint(xl * 2)
+    r = xl - q / 2
+    s, c = np.sin(pi * r), np.cos(pi * r)
+    qm = q % 4
+    sin = np.select([qm == 0, qm == 1, qm == 2], [s, c, -s], -c)
+    cos = np.select([qm == 0, qm == 1, qm == 2], [c, -s, -c], s)
+    return sin.astype(np.float64), cos.astype(np.float64)
+
+
+def test_sincospi_ulp():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-4, 4, 100000), rng.uniform(0, 2, 100000), np.arange(-16, 16) * 0.25])
+    rs, rc = _sincospi_ref(x)
+    assert ulps(o.math_eval(2, x), rs).max() <= 2.0
+    assert ulps(o.math_eval(3, x), rc).max() <= 2.0
+    # exact values at quarter points
+    q = np.array([0.0, 0.5, 1.0, 1.5])
+    assert np.array_equal(o.math_eval(2, q), [0.0, 1.0, 0.0, -1.0])
+
+
+def test_atan2_ulp():
+    rng = np.random.default_rng(4)
+    y = rng.uniform(-3, 3, 200000)
+    x = rng.uniform(-3, 3, 200000)
+    assert ulps(o.math_eval(4, y, x), np.arctan2(y, x)).max() <= 4.0
+    ys = np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0])
+    xs = np.array([1.0, 1.0, -1.0, -1.0, 0.0, 0.0])
+    got = o.math_eval(4, ys, xs)
+    ref = np.arctan2(ys, xs)
+    assert np.all(ulps(got, ref) <= 1.0) and np.array_equal(np.signbit(got), np.signbit(ref))
+
+
+def test_philox_known_answers():
+    kat = json.loads((GOLDEN / "philox4x32_10_kat.json").read_text())
+    assert len(kat["vectors"]) >= 3
+    for v in kat["vectors"]:
+        out = o.philox(v["rounds"], [int(c, 16) for c in v["ctr"]], [int(k, 16) for k in v["key"]])
+        assert [f"{w:08x}" for w in out] == v["out"]
+
+
+def test_box_muller_normals():
+    # 4 streams x 8 paths x 4000 steps = 256k normals
+    z = np.concatenate([o.normals(0, s, p, 4000) for s in range(4) for p in range(8)])
+    n = z.size
+    assert abs(z.mean()) < 4.0 / np.sqrt(n)
+    assert abs(z.var() - 1.0) < 4.0 * np.sqrt(2.0 / n)
+    assert abs(sst.skew(z)) < 4.0 * np.sqrt(6.0 / n)
+    assert sst.kstest(z, "norm").pvalue > 1e-3
+    # distinct streams / paths / seeds give distinct sequences
+    a, b, c = o.normals(0, 0, 0, 16), o.normals(0, 1, 0, 16), o.normals(1, 0, 0, 16)
+    assert not np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("seed", [0, 123456789, 2**40 + 7])
+def test_normals_pairs_independent(seed):
+    z = o.normals(seed, 5, 9, 20000).reshape(-1, 2)
+    r = np.corrcoef(z[:, 0], z[:, 1])[0, 1]
+    assert abs(r) < 4.0 / np.sqrt(z.shape[0])
