@@ -160,30 +160,68 @@ def test_invalid_arguments_rejected(gpu):
             p.set_launch(block_size=100)
 
 
+def _grid16():
+    t = -1.0 + 2.0 * (np.arange(16) + 1) / 17
+    X, Y = np.meshgrid(t, t, indexing="ij")
+    return X.ravel(), Y.ravel()
+
+
+def _est(st, key):
+    v = np.array([s[key] for s in st])
+    sq = np.array([s["phiphi" if key == "phi" else "phiphi_VR"] for s in st])
+    n = np.array([s["paths"] for s in st])
+    return v, np.sqrt(np.maximum(sq - v * v, 0) / n)
+
+
 def test_config1_full_size_properties(gpu):
     """256 grid points x 10,000 paths, h = 1e-3 on [-1,1]^2 (the plain FP64 workload)."""
     say("config 1 full size")
-    t = -1.0 + 2.0 * (np.arange(16) + 1) / 17
-    X, Y = np.meshgrid(t, t, indexing="ij")
-    x, y = X.ravel(), Y.ravel()
+    x, y = _grid16()
     with FKProblem(problem_desc()) as p:
         st, rc = p.solve_points(x, y, 10000, 1e-3, seed=0)
         st2, _ = p.solve_points(x, y, 10000, 1e-3, seed=0)
+        st_b, _ = p.solve_points(x, y, 10000, 1e-3, seed=1)
     assert rc == 0
     from test_oracle import franke_published, tau_closed_form
 
     u = franke_published(x, y)
-    phi = np.array([s["phi"] for s in st])
-    se = np.sqrt(np.array([s["phiphi"] - s["phi"] ** 2 for s in st]) / 10000)
-    z = (phi - u) / se
-    assert np.abs(z).max() < 5.0 and abs(z.mean()) < 0.5
+    # Euler-Maruyama + Gobet shift has weak order 1; the largest bias at h = 1e-3 is
+    # ~31.5 h at the sharp negative Gaussian of Franke's function (measured, and it
+    # halves with h: test_config1_bias_is_first_order)
+    for key in ("phi", "phi_VR"):
+        v, se = _est(st, key)
+        assert np.all(np.abs(v - u) < 4 * se + 40 * 1e-3), key
+        # statistical parity with an independent stream of the same scheme
+        vb, seb = _est(st_b, key)
+        z = (v - vb) / np.sqrt(se ** 2 + seb ** 2)
+        assert np.abs(z).max() < 4.5 and 0.75 < np.sqrt((z ** 2).mean()) < 1.25, key
     apl = np.array([s["APL"] for s in st])
     assert abs(apl.mean() / tau_closed_form(x, y).mean() - 1) < 0.01
     steps = sum(s["path_steps"] for s in st)
     assert 3.8e8 < steps < 4.3e8
     # bit-stable across launches
     _assert_stats_equal(st, st2)
-    # spot-check one point against the oracle bit for bit
+    # one point against the CPU restatement, bit for bit
     b = SolveBatchArrays(x[100:101], y[100:101], 10000, 1e-3, [100])
-    so = o.reduce(b, o.simulate(desc := problem_desc(), b, seed=0))
+    so = o.reduce(b, o.simulate(problem_desc(), b, seed=0))
     _assert_stats_equal([st[100]], so)
+
+
+def test_config1_bias_is_first_order(gpu):
+    """Halving h twice cuts the largest discretisation errors by ~4x (weak order 1)."""
+    say("config 1 h-refinement")
+    x, y = _grid16()
+    from test_oracle import franke_published
+
+    u = franke_published(x, y)
+    errs = {}
+    with FKProblem(problem_desc()) as p:
+        for h in (1e-3, 2.5e-4):
+            st, _ = p.solve_points(x, y, 40000, h, seed=0)
+            v, se = _est(st, "phi_VR")
+            errs[h] = (v - u, se)
+    e1, s1 = errs[1e-3]
+    e4, s4 = errs[2.5e-4]
+    top = np.argsort(-np.abs(e1))[:8]
+    assert np.all(np.abs(e4[top]) < 0.4 * np.abs(e1[top]) + 4 * s4[top])
+    assert np.abs(e4).max() < 12 * 2.5e-4 + 4 * s4.max()
