@@ -22,6 +22,7 @@ sys.path.insert(0, str(ROOT))
 from spac_amd import abi  # noqa: E402
 from spac_amd.fk import FKProblem, SolveBatchArrays, problem_desc  # noqa: E402
 from spac_amd.flops import FLOPS_PER_STEP, PEAK_FP64_TFLOPS  # noqa: E402
+from spac_amd.mesh import Mesh, gmres, mesh_desc  # noqa: E402
 
 
 def log(msg):
@@ -125,12 +126,102 @@ def bench_points(args):
     return out
 
 
+def cpu_baseline_rows(mesh, desc, paths, h, seed, budget_s=20.0):
+    """The CPU restatement's row assembly (OpenMP over paths) on a bounded sample of the rows."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as o
+
+    threads = min(16, os.cpu_count() or 1)
+    live = np.nonzero(mesh.job_frontera == 0)[0]
+    probe = live[:: max(1, live.size // 4)][:4]
+    t0 = time.perf_counter()
+    _, _, st = o.assemble_rows(desc, mesh, 1000, h, seed=seed, jobs=probe, nthreads=threads)
+    dt = time.perf_counter() - t0
+    per_job = dt / probe.size * (paths / 1000.0)
+    njobs = int(max(1, min(live.size, budget_s / max(per_job, 1e-9))))
+    jobs = live[np.linspace(0, live.size - 1, njobs).astype(np.int64)]
+    t0 = time.perf_counter()
+    _, _, st = o.assemble_rows(desc, mesh, paths, h, seed=seed, jobs=jobs, nthreads=threads)
+    dt = time.perf_counter() - t0
+    steps = int(sum(x["path_steps"] for x in st))
+    return {"value": steps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{njobs} of the {mesh.info.n_jobs} rows x {paths} paths, h={h}, G and B included "
+                      f"({steps} path-steps, {dt:.1f} s)"}
+
+
+def bench_rows(args):
+    """PDDSparse interface assembly: 4x4 circles on the unit square, 1016 nodes, 1952 rows."""
+    paths, h, seed = 20000, 5e-4, 0
+    mesh = Mesh(mesh_desc())
+    desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=(0.0, 0.0, 1.0, 1.0))
+    prob = FKProblem(desc)
+    for i in range(args.warmup):
+        prob.assemble_rows(mesh, paths, h, seed=seed)
+        log(f"warmup {i + 1}/{args.warmup} done")
+    dev_ms, path_ms, wall = [], [], []
+    rows = None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        rows = prob.assemble_rows(mesh, paths, h, seed=seed)
+        wall.append(time.perf_counter() - t0)
+        t = prob.timing()
+        dev_ms.append(t.total_ms)
+        path_ms.append(t.path_kernel_ms)
+    log(f"timed {args.steps} steps")
+    path_steps = int(sum(x["path_steps"] for x in rows["stats"]))
+    mesh.accumulate(rows)
+    Gi, Gj, Gv, B = mesh.csr()
+    t0 = time.perf_counter()
+    u, info = gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=5000, restart=200)
+    t_gmres = time.perf_counter() - t0
+    sys.path.insert(0, str(ROOT / "tests"))
+    from test_oracle import franke_published
+
+    err = u - franke_published(mesh.node_x, mesh.node_y)
+    dev_s = float(np.mean(dev_ms)) * 1e-3
+    avg_path_ms = float(np.mean(path_ms))
+    fps = FLOPS_PER_STEP["varcoef_circle_fp64"]
+    achieved = path_steps * fps / (avg_path_ms * 1e-3) / 1e12
+    out = {
+        "metric": "path-steps/s",
+        "value": path_steps / dev_s,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dev_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "PDDSparse interface assembly FP64: unit square, 4x4 overlapping circles (overlap 1.2), "
+                               "1016 nodes / 1952 rows, 20000 paths/row, h=5e-4, sigma=(1+0.5 sin(pi x) cos(pi y)) I, "
+                               "b=0.3(y,-x), c=-1-x^2, manufactured Franke u, Gaussian RBF eps=3 with 15-node stencils",
+                   "rows": int(mesh.info.n_jobs), "nodes": int(mesh.info.n_nodes), "paths_per_row": paths, "h": h,
+                   "path_steps_per_step": path_steps},
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "fk_path_kernel_fp64<CoefVarFranke, SUB_CIRCLE>", "kernel_avg_ms": avg_path_ms,
+                     "flops_per_path_step": fps},
+        "pcie_inclusive_path_steps_per_s": path_steps / float(np.mean(wall)),
+        "gmres": {"tol": 1e-10, "iterations": int(info.iterations), "residual": float(info.residual),
+                  "seconds_host": t_gmres, "rms_err_vs_exact": float(np.sqrt(np.mean(err ** 2))),
+                  "max_err_vs_exact": float(np.abs(err).max())},
+    }
+    prob.close()
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline_rows(mesh, desc, paths, h, seed, budget_s=args.cpu_seconds)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="points", choices=["points"])
+    ap.add_argument("--workload", default="rows", choices=["rows", "points"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -138,7 +229,7 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by node batches, "
               "see the sharding note in DESIGN.md", file=sys.stderr)
         sys.exit(2)
-    out = bench_points(args)
+    out = bench_rows(args) if args.workload == "rows" else bench_points(args)
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,96 @@
+/*
+ * spac_mesh.h — C ABI of the PDDSparse mesh, interface tables and the
+ * assembled interface system G u = B.
+ *
+ * Reference interfaces replaced:
+ *   Malla::Construye_Cuadrado(params, Nl, superposicion, theta_0, Nt, BVP, c2)
+ *                                               Mesh/Malla.cpp:12-83
+ *   Interfaz::Inicia_Cuadrado / Calcula_Psi     Mesh/Interfaz.cpp:2-217
+ *   SistemaLineal::Actualiza_GB(Nudo&)          Mesh/SistemaLineal.cpp:68-88
+ *   SistemaLineal::Escribe_GB_COO / Escribe_GB_CSR / Lee_GB_COO
+ *                                               Mesh/SistemaLineal.cpp:96-164
+ *   Malla::Escribe_Posiciones                   Mesh/Malla.cpp:99-112
+ *   SistemaLinealSolverGMRES::Resuelve          Solvers/SistemaLinealSolverGMRES_OMP.hpp:159-616
+ * The mesh lives in host memory; arrays returned by pointer stay valid until
+ * spac_mesh_destroy.
+ */
+#ifndef SPAC_MESH_H
+#define SPAC_MESH_H
+
+#include <stdint.h>
+
+#include "spac_fk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double domain[4];           /* xmin, ymin, xmax, ymax */
+    int32_t n_side;             /* Nl: circles per side */
+    int32_t nodes_per_circle;   /* Nt */
+    double overlap;             /* superposicion, in [1, 2] */
+    double theta0;              /* angulo_inicial */
+    double c2_corner;           /* IMQ shape of corner arcs ("esquina") */
+    double c2_side;             /* IMQ shape of side arcs ("lado") */
+    int32_t rbf_kind;           /* SPAC_RBF_IMQ_GLOBAL or SPAC_RBF_GAUSS_STENCIL */
+    int32_t stencil_size;       /* Gaussian stencils: nodes per stencil */
+    double gauss_eps;           /* Gaussian stencils: shape on stencil-normalised distance */
+    int32_t fk_all_circles;     /* 1: rows of full interior circles by FK too; 0: perimeter only */
+    int32_t reserved;
+} spac_mesh_desc;
+
+typedef struct spac_mesh spac_mesh;
+
+typedef struct {
+    int32_t n_nodes;            /* rows of G */
+    int32_t n_interfaces;
+    int32_t n_jobs;             /* FK row jobs (node inside a circle) */
+    int32_t nnz;                /* CSR pattern size */
+    int64_t total_G;            /* sum over jobs of the job interface's node count */
+    double radius;
+    double max_cond;            /* largest |iPsi|_F |Psi|_F over the interpolation blocks */
+} spac_mesh_info;
+
+spac_status spac_mesh_build_square(const spac_mesh_desc* desc, spac_mesh** out);
+spac_status spac_mesh_destroy(spac_mesh* m);
+spac_status spac_mesh_get_info(const spac_mesh* m, spac_mesh_info* out);
+/* Interface table for spac_fk_assemble_rows (pointers owned by the mesh). */
+spac_status spac_mesh_interfaces(const spac_mesh* m, spac_interface_table* out);
+/* Per job: global row, position, interface, frontera flag; arrays of n_jobs. */
+spac_status spac_mesh_jobs(const spac_mesh* m, int32_t* row, double* x, double* y, int32_t* iface, int32_t* frontera);
+/* Per node (global index order): position and frontera flag; arrays of n_nodes. */
+spac_status spac_mesh_nodes(const spac_mesh* m, double* x, double* y, int32_t* frontera);
+/* Calcula_Psi's return value |iPsi|_F |Psi|_F for interface k (IMQ tables). */
+spac_status spac_mesh_interface_cond(const spac_mesh* m, int32_t k, double* cond);
+/* Reset B and G to the initial pattern values (diagonal = containing-circle count, off-diagonal 1e-23). */
+spac_status spac_mesh_reset_system(spac_mesh* m);
+/* Actualiza_GB for every job of rows (as produced by spac_fk_assemble_rows). */
+spac_status spac_mesh_accumulate(spac_mesh* m, const spac_row_block* rows);
+/* CSR export: G_i[n+1], G_j[nnz], G_ij[nnz], B[n] (any pointer may be NULL). */
+spac_status spac_mesh_csr(const spac_mesh* m, int32_t* G_i, int32_t* G_j, double* G_ij, double* B);
+spac_status spac_mesh_write_coo(const spac_mesh* m, const char* G_path, const char* B_path);
+spac_status spac_mesh_write_csr(const spac_mesh* m, const char* dir);
+/* Escribe_Posiciones with u = the registered coefficient set's exact solution. */
+spac_status spac_mesh_write_positions(const spac_mesh* m, int32_t coef_kind, const char* path);
+
+typedef struct {
+    int32_t iterations;
+    int32_t restarts;
+    double residual;            /* final ||B - G u|| / ||B|| */
+    int32_t converged;
+    int32_t reserved;
+} spac_gmres_info;
+
+/* Restarted FP64 GMRES (modified Gram–Schmidt, Givens rotations) with a
+ * Jacobi preconditioner on the CSR system; u holds the initial guess on entry. */
+spac_status spac_gmres_solve(int32_t n, const int32_t* G_i, const int32_t* G_j, const double* G_ij, const double* B,
+                             double* u, double tol, int32_t max_iter, int32_t restart, spac_gmres_info* info);
+/* Read a COO file ("nnz n" then "i j v") into CSR (Lee_GB_COO). Call with
+ * G_i == NULL to get n and nnz first. */
+spac_status spac_read_coo(const char* path, int32_t* n, int32_t* nnz, int32_t* G_i, int32_t* G_j, double* G_ij);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -39,6 +39,7 @@
 
 #include "../include/spac_fk.h"
 #include "../spac_amd/csrc/spac_math.h"
+#include "../spac_amd/csrc/spac_rbf.h"
 
 typedef struct { double x, y; } vec2;
 typedef struct { double a00, a01, a10, a11; } mat2;
@@ -417,6 +418,7 @@ void oracle_path_terms(int node_mode, double Y, double Z, double xi, double tau,
 static void reduce_one(int node_mode, int64_t np, const double* Y, const double* Z, const double* xi,
                        const double* tau, const double* gval, const int32_t* steps, const int8_t* outcome,
                        spac_point_stats* st) {
+    if (np == 0) { memset(st, 0, sizeof *st); return; }
     spac_ksum tot[NSUMS];
     for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_zero();
     int64_t ntile = (np + 63) / 64;
@@ -536,4 +538,151 @@ void oracle_boundary_eval(int kind, const double* params, const double* x, const
         double d = f.distancia(f.params, X, &e, &nn);
         out[5 * i] = d; out[5 * i + 1] = e.x; out[5 * i + 2] = e.y; out[5 * i + 3] = nn.x; out[5 * i + 4] = nn.y;
     }
+}
+
+/* ------------------------------------------------------ interface rows --
+ * Solve_OMP_Analytic(Nudo&, ...) for a batch of nodes
+ * (Solvers/FeynmanKacSolver.cpp:233-292, 324-355).  G per node, column by
+ * column, as a sequential TwoSum-compensated sum over the node's paths in
+ * path order:
+ *   IMQ (moment form):  S_i = sum_p Y_p phi(theta_p, theta_i),
+ *                       G_j = -(sum_i iPsi(i,j) S_i) / N
+ *   IMQ (direct form, direct != 0, Mesh/Interfaz.cpp:253-261):
+ *                       G_j = (sum_p sum_i -Y_p iPsi(i,j) phi(theta_p, theta_i)) / N
+ *   Gaussian stencils:  G_j = -(sum_p Y_p H_j(theta_p)) / N over the paths whose
+ *                       stencil holds column j.
+ * Pinned by: tests/test_rows.py (cardinality of the interpolation tables,
+ * consistency sum_j u(x_j) H_j(theta) ~ u(X) on the arcs, moment form = direct
+ * form within cond(Psi) eps, G rows summing to -E[Y 1{circle exit}]). */
+int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_table* t, const spac_node_batch* nb,
+                         uint64_t seed, int nthreads, int direct, double* B, double* G, spac_point_stats* stats) {
+    int n = nb->n;
+    uint32_t* paths = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    double* circle = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    off[0] = 0;
+    for (int k = 0; k < n; ++k) {
+        int f = nb->iface[k];
+        paths[k] = nb->frontera[k] ? 0u : nb->paths[k];
+        circle[3 * k] = t->radius[f]; circle[3 * k + 1] = t->cx[f]; circle[3 * k + 2] = t->cy[f];
+        off[k + 1] = off[k] + paths[k];
+    }
+    int64_t tp = off[n] > 0 ? off[n] : 1;
+    spac_path_records rec;
+    rec.x_tau = (double*)malloc(8 * (size_t)tp); rec.y_tau = (double*)malloc(8 * (size_t)tp);
+    rec.Y = (double*)malloc(8 * (size_t)tp); rec.Z = (double*)malloc(8 * (size_t)tp);
+    rec.xi = (double*)malloc(8 * (size_t)tp); rec.tau = (double*)malloc(8 * (size_t)tp);
+    rec.gval = (double*)malloc(8 * (size_t)tp); rec.steps = (int32_t*)malloc(4 * (size_t)tp);
+    rec.outcome = (int8_t*)malloc((size_t)tp);
+    spac_solve_batch b;
+    memset(&b, 0, sizeof b);
+    b.n = n; b.x0 = nb->x; b.y0 = nb->y; b.paths = paths; b.h = nb->h; b.stream = nb->stream; b.circle = circle;
+    int rc = oracle_simulate(desc, &b, seed, &rec, nthreads);
+    if (rc == 0) rc = oracle_reduce(1, &b, &rec, stats);
+    bvp_t bvp;
+    if (rc == 0) rc = bvp_init(&bvp, desc);
+    int64_t goff = 0;
+    for (int k = 0; k < n && rc == 0; ++k) {
+        int f = nb->iface[k];
+        int nbase = t->node_offset[f], M = t->node_offset[f + 1] - nbase;
+        const double* th = t->node_theta + nbase;
+        double* Gk = G + goff;
+        goff += M;
+        if (nb->frontera[k]) {
+            vec2 X = {nb->x[k], nb->y[k]};
+            B[k] = bvp.g(X);
+            for (int j = 0; j < M; ++j) Gk[j] = 0.0;
+            continue;
+        }
+        B[k] = stats[k].B;
+        int code = t->code[f];
+        double cx = t->cx[f], cy = t->cy[f], c2 = t->c2[f];
+        const double* ipsi = t->ipsi + (t->ipsi_offset[f] >= 0 ? t->ipsi_offset[f] : 0);
+        int64_t np = paths[k];
+        int stencil = t->rbf_kind == SPAC_RBF_GAUSS_STENCIL;
+        int S = stencil ? (t->stencil_size < M ? t->stencil_size : M) : 0;
+        spac_ksum* acc = (spac_ksum*)malloc(sizeof(spac_ksum) * (size_t)M);
+        double* dsum = (double*)calloc((size_t)M, sizeof(double));
+        for (int j = 0; j < M; ++j) acc[j] = spac_ksum_zero();
+        for (int64_t p = 0; p < np; ++p) {
+            int64_t g = off[k] + p;
+            if (rec.outcome[g] != SPAC_OUT_SUBDOMAIN) continue;
+            double theta = 0.0;
+            spac_exit_angle(code, rec.x_tau[g] - cx, rec.y_tau[g] - cy, th[0], &theta);
+            double Yp = rec.Y[g];
+            if (stencil) {
+                double phi[32];
+                int s0 = (code == 1) ? spac_circle_stencil(M, S, t->gauss_eps, theta, phi)
+                                     : spac_arc_stencil(th, M, S, t->gauss_eps, theta, phi);
+                const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s0 * S * S);
+                for (int j = 0; j < M; ++j) {
+                    int rel = j - s0;
+                    if (code == 1 && rel < 0) rel += M;
+                    if (rel < 0 || rel >= S) continue;
+                    double w = 0.0;
+                    for (int i = 0; i < S; ++i) w = w + blk[i * S + rel] * phi[i];
+                    acc[j] = spac_ksum_add(acc[j], Yp * w);
+                }
+            } else if (direct) {
+                for (int j = 0; j < M; ++j)
+                    for (int i = 0; i < M; ++i) dsum[j] += -Yp * ipsi[(int64_t)i * M + j] * spac_imq(theta, th[i], c2);
+            } else {
+                for (int i = 0; i < M; ++i) acc[i] = spac_ksum_add(acc[i], Yp * spac_imq(theta, th[i], c2));
+            }
+        }
+        double N = (double)np;
+        if (stencil) {
+            for (int j = 0; j < M; ++j) Gk[j] = -spac_ksum_value(acc[j]) / N;
+        } else if (direct) {
+            for (int j = 0; j < M; ++j) Gk[j] = dsum[j] / N;
+        } else {
+            for (int j = 0; j < M; ++j) {
+                double w = 0.0;
+                for (int i = 0; i < M; ++i) w = w + ipsi[(int64_t)i * M + j] * spac_ksum_value(acc[i]);
+                Gk[j] = -w / N;
+            }
+        }
+        free(acc);
+        free(dsum);
+    }
+    free(rec.x_tau); free(rec.y_tau); free(rec.Y); free(rec.Z); free(rec.xi); free(rec.tau); free(rec.gval);
+    free(rec.steps); free(rec.outcome);
+    free(paths); free(circle); free(off);
+    return rc;
+}
+
+/* exit angle probe (spac_exit_angle) */
+int oracle_exit_angle(int code, double dx, double dy, double first, double* theta) {
+    return spac_exit_angle(code, dx, dy, first, theta);
+}
+
+/* Row weights H_j(theta) of interface f for an exit angle theta (the table
+ * the G rows apply per path); out has the interface's node count. */
+int oracle_row_weights(const spac_interface_table* t, int f, double theta, double* out) {
+    int nbase = t->node_offset[f], M = t->node_offset[f + 1] - nbase;
+    const double* th = t->node_theta + nbase;
+    const double* ipsi = t->ipsi + (t->ipsi_offset[f] >= 0 ? t->ipsi_offset[f] : 0);
+    int code = t->code[f];
+    for (int j = 0; j < M; ++j) out[j] = 0.0;
+    if (t->rbf_kind == SPAC_RBF_GAUSS_STENCIL) {
+        int S = t->stencil_size < M ? t->stencil_size : M;
+        double phi[32];
+        if (S > 32) return SPAC_E_INVALID_ARG;
+        int s0 = (code == 1) ? spac_circle_stencil(M, S, t->gauss_eps, theta, phi)
+                             : spac_arc_stencil(th, M, S, t->gauss_eps, theta, phi);
+        const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s0 * S * S);
+        for (int r = 0; r < S; ++r) {
+            double w = 0.0;
+            for (int i = 0; i < S; ++i) w = w + blk[i * S + r] * phi[i];
+            out[(s0 + r) % M] = w;
+        }
+        return SPAC_OK;
+    }
+    if (t->ipsi_offset[f] < 0) return SPAC_E_BAD_INTERFACE;
+    for (int j = 0; j < M; ++j) {
+        double w = 0.0;
+        for (int i = 0; i < M; ++i) w = w + ipsi[(int64_t)i * M + j] * spac_imq(theta, th[i], t->c2[f]);
+        out[j] = w;
+    }
+    return SPAC_OK;
 }

@@ -11,9 +11,9 @@
 
 #include "../../include/spac_fk.h"
 
-namespace spac {
+#include "spac_error.h"
 
-spac_status set_error(spac_status code, const char* fmt, ...);
+namespace spac {
 
 /* Grow-only device allocation owned by a handle. */
 struct DeviceBuffer {
@@ -58,6 +58,8 @@ struct Problem {
     DeviceBuffer ifc, ifnode, ifipsi, ifoff, rows_iface, rows_goff, rows_G, rows_part, rows_col;
 
     spac_status ensure_records(int64_t total_paths);
+    spac_status upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths, const double* h,
+                              const uint32_t* strm, const double* circle, SolveSet* out);
     spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode);
     void release_all() {
         DeviceBuffer* all[] = {&rec_xt, &rec_yt, &rec_Y, &rec_Z, &rec_xi, &rec_tau, &rec_gval, &rec_steps,

@@ -297,6 +297,12 @@ __global__ void __launch_bounds__(64) fk_reduce_kernel(ReduceArgs a) {
         my_max += __shfl_xor(my_max, o, 64);
         my_dom += __shfl_xor(my_dom, o, 64);
     }
+    if (lane == 0 && np == 0) {
+        spac_point_stats st;
+        memset(&st, 0, sizeof st);
+        a.stats[solve] = st;
+        return;
+    }
     if (lane == 0) {
         double N = (double)np;
         double s[NSUMS];
@@ -373,6 +379,62 @@ spac_status Problem::ensure_records(int64_t total_paths) {
     return SPAC_OK;
 }
 
+/* Upload a host batch of solves into the handle's solve table and build the
+ * work queue (solves with zero paths get no work items). */
+spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths,
+                                   const double* h, const uint32_t* strm, const double* circle, SolveSet* out) {
+    std::vector<int64_t> off(n + 1, 0);
+    std::vector<int4> items;
+    const uint32_t chunk = (uint32_t)launch.chunk_paths;
+    for (int k = 0; k < n; ++k) {
+        if (!(h[k] > 0.0) || !(h[k] < 1e30)) return set_error(SPAC_E_INVALID_ARG, "solve %d has bad h", k);
+        if (circle && !(circle[3 * k] > 0.0))
+            return set_error(SPAC_E_INVALID_ARG, "solve %d has a circle of radius <= 0", k);
+        off[k + 1] = off[k] + paths[k];
+        for (uint32_t p0 = 0; p0 < paths[k]; p0 += chunk) {
+            uint32_t p1 = std::min(p0 + chunk, paths[k]);
+            items.push_back(make_int4(k, (int)p0, (int)p1, 0));
+        }
+    }
+    if (items.size() > 0x7fffffffu) return set_error(SPAC_E_INVALID_ARG, "too many work items");
+    spac_status rc;
+    size_t nb = (size_t)n;
+    if ((rc = sx.ensure(nb * 8))) return rc;
+    if ((rc = sy.ensure(nb * 8))) return rc;
+    if ((rc = sh.ensure(nb * 8))) return rc;
+    if ((rc = sstream.ensure(nb * 4))) return rc;
+    if ((rc = spaths.ensure(nb * 4))) return rc;
+    if ((rc = soff.ensure((nb + 1) * 8))) return rc;
+    if ((rc = scircle.ensure(nb * 24))) return rc;
+    if ((rc = sitems.ensure(std::max<size_t>(items.size(), 1) * sizeof(int4)))) return rc;
+    hipStream_t s = stream;
+    SPAC_HIP_CHECK(hipMemcpyAsync(sx.ptr, x0, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(sy.ptr, y0, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(sh.ptr, h, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(sstream.ptr, strm, nb * 4, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(spaths.ptr, paths, nb * 4, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(soff.ptr, off.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
+    if (circle) SPAC_HIP_CHECK(hipMemcpyAsync(scircle.ptr, circle, nb * 24, hipMemcpyHostToDevice, s));
+    if (!items.empty())
+        SPAC_HIP_CHECK(hipMemcpyAsync(sitems.ptr, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    /* the host vectors die at return: wait for the copies */
+    SPAC_HIP_CHECK(hipStreamSynchronize(s));
+    SolveSet ss;
+    ss.n = n;
+    ss.x0 = (const double*)sx.ptr;
+    ss.y0 = (const double*)sy.ptr;
+    ss.h = (const double*)sh.ptr;
+    ss.stream = (const uint32_t*)sstream.ptr;
+    ss.paths = (const uint32_t*)spaths.ptr;
+    ss.rec_off = (const int64_t*)soff.ptr;
+    ss.circle = circle ? (const double*)scircle.ptr : nullptr;
+    ss.items = (const int4*)sitems.ptr;
+    ss.n_items = (int32_t)items.size();
+    ss.total_paths = off[n];
+    *out = ss;
+    return SPAC_OK;
+}
+
 static spac_status check_problem(const spac_problem_desc* d) {
     if (d->coef_kind != SPAC_COEF_FRANKE_POISSON && d->coef_kind != SPAC_COEF_VARCOEF_FRANKE)
         return set_error(SPAC_E_UNREGISTERED_PLUGIN, "coefficient set %d has no device implementation", d->coef_kind);
@@ -425,8 +487,8 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
     int64_t max_blocks = std::max<int64_t>(1, (waves_needed + (block / 64) - 1) / (block / 64));
     if (blocks > max_blocks) blocks = (int)max_blocks;
 
-    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
     SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
     launch_paths(desc.coef_kind, sub, dim3(blocks), dim3(block), stream, a);
     SPAC_HIP_CHECK(hipGetLastError());
     SPAC_HIP_CHECK(hipEventRecord(ev[1], stream));
@@ -602,56 +664,16 @@ spac_status spac_fk_simulate(spac_problem* hp, const spac_solve_batch* b, uint64
         return set_error(SPAC_E_INVALID_ARG, "batch needs n >= 1 and x0, y0, paths, h, stream");
     Problem* p = reinterpret_cast<Problem*>(hp);
     const int32_t n = b->n;
-    std::vector<int64_t> off(n + 1, 0);
-    std::vector<int4> items;
-    const uint32_t chunk = (uint32_t)p->launch.chunk_paths;
-    for (int k = 0; k < n; ++k) {
+    for (int k = 0; k < n; ++k)
         if (b->paths[k] < 1) return set_error(SPAC_E_INVALID_ARG, "solve %d has no paths", k);
-        if (!(b->h[k] > 0.0) || !(b->h[k] < 1e30)) return set_error(SPAC_E_INVALID_ARG, "solve %d has bad h", k);
-        if (b->circle && !(b->circle[3 * k] > 0.0))
-            return set_error(SPAC_E_INVALID_ARG, "solve %d has a circle of radius <= 0", k);
-        off[k + 1] = off[k] + b->paths[k];
-        for (uint32_t p0 = 0; p0 < b->paths[k]; p0 += chunk) {
-            uint32_t p1 = std::min(p0 + chunk, b->paths[k]);
-            items.push_back(make_int4(k, (int)p0, (int)p1, 0));
-        }
-    }
-    if (items.size() > 0x7fffffffu) return set_error(SPAC_E_INVALID_ARG, "too many work items");
-    spac_status rc;
-    size_t nb = (size_t)n;
-    if ((rc = p->sx.ensure(nb * 8))) return rc;
-    if ((rc = p->sy.ensure(nb * 8))) return rc;
-    if ((rc = p->sh.ensure(nb * 8))) return rc;
-    if ((rc = p->sstream.ensure(nb * 4))) return rc;
-    if ((rc = p->spaths.ensure(nb * 4))) return rc;
-    if ((rc = p->soff.ensure((nb + 1) * 8))) return rc;
-    if ((rc = p->scircle.ensure(nb * 24))) return rc;
-    if ((rc = p->sitems.ensure(items.size() * sizeof(int4)))) return rc;
-    if ((rc = p->stats.ensure(nb * sizeof(spac_point_stats)))) return rc;
-    hipStream_t s = p->stream;
-    SPAC_HIP_CHECK(hipMemcpyAsync(p->sx.ptr, b->x0, nb * 8, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(p->sy.ptr, b->y0, nb * 8, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(p->sh.ptr, b->h, nb * 8, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(p->sstream.ptr, b->stream, nb * 4, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(p->spaths.ptr, b->paths, nb * 4, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(p->soff.ptr, off.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
-    if (b->circle) SPAC_HIP_CHECK(hipMemcpyAsync(p->scircle.ptr, b->circle, nb * 24, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(p->sitems.ptr, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s));
     SolveSet ss;
-    ss.n = n;
-    ss.x0 = (const double*)p->sx.ptr;
-    ss.y0 = (const double*)p->sy.ptr;
-    ss.h = (const double*)p->sh.ptr;
-    ss.stream = (const uint32_t*)p->sstream.ptr;
-    ss.paths = (const uint32_t*)p->spaths.ptr;
-    ss.rec_off = (const int64_t*)p->soff.ptr;
-    ss.circle = b->circle ? (const double*)p->scircle.ptr : nullptr;
-    ss.items = (const int4*)p->sitems.ptr;
-    ss.n_items = (int32_t)items.size();
-    ss.total_paths = off[n];
+    spac_status rc = p->upload_solves(n, b->x0, b->y0, b->paths, b->h, b->stream, b->circle, &ss);
+    if (rc) return rc;
+    size_t nb = (size_t)n;
+    if ((rc = p->stats.ensure(nb * sizeof(spac_point_stats)))) return rc;
     rc = p->run(ss, seed, stats ? (spac_point_stats*)p->stats.ptr : nullptr, b->circle ? 1 : 0);
     if (rc) return rc;
-    size_t tp = (size_t)off[n];
+    size_t tp = (size_t)ss.total_paths;
     if (rec) {
         if (rec->x_tau) SPAC_HIP_CHECK(hipMemcpy(rec->x_tau, p->rec_xt.ptr, tp * 8, hipMemcpyDeviceToHost));
         if (rec->y_tau) SPAC_HIP_CHECK(hipMemcpy(rec->y_tau, p->rec_yt.ptr, tp * 8, hipMemcpyDeviceToHost));

@@ -1,13 +1,344 @@
 /*
  * fk_rows.hip — interface-row (G, B) assembly on the GPU.
+ *
+ * One call assembles the rows of a whole batch of nodes
+ * (Solve_OMP_Analytic(Nudo&, ...) + Reduce_Analytic(..., Nudo&, Interfaz) +
+ * Update(Nudo&), Solvers/FeynmanKacSolver.cpp:233-292, 324-355):
+ *   1. the persistent path kernel runs every node's paths, stopped by the
+ *      domain rectangle or the node's interface circle;
+ *   2. the estimator reduction gives B = mean of (Z + xi + Y g) over domain
+ *      exits and (Z + xi) over circle exits, and the node statistics;
+ *   3. boundary nodes (frontera) get B = g(x) with no paths (:346-347);
+ *   4. the G kernel (one block per node) accumulates the row weights over the
+ *      circle-exit paths.  Global IMQ interpolation uses the moment form
+ *      S_i = sum_p Y_p phi(theta_p, theta_i), G_j = -sum_i iPsi(i,j) S_i / N;
+ *      Gaussian stencils accumulate G_j += Y_p H_j(theta_p) directly for the
+ *      stencil_size columns of each path's stencil.  Every column is a
+ *      sequential TwoSum-compensated sum over the node's paths in path order,
+ *      so the result does not depend on the launch and equals the CPU
+ *      restatement's.
  */
 #include <hip/hip_runtime.h>
 
-#include "../../include/spac_fk.h"
-#include "fk_internal.h"
+#include <algorithm>
+#include <cstring>
+#include <vector>
 
-extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interface_table* ifaces,
-                                             const spac_node_batch* nodes, uint64_t seed, spac_row_block* out) {
-    (void)hp; (void)ifaces; (void)nodes; (void)seed; (void)out;
-    return spac::set_error(SPAC_E_INVALID_ARG, "spac_fk_assemble_rows: not available in this build");
+#include "../../include/spac_fk.h"
+#include "fk_device.h"
+#include "fk_internal.h"
+#include "spac_rbf.h"
+
+namespace spac {
+
+#define ROWS_HIP_CHECK(expr)                                                                   \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return set_error(SPAC_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                             __FILE__, __LINE__);                                              \
+    } while (0)
+
+constexpr int kRowsBlock = 256;
+constexpr int kMaxStencil = 16;
+constexpr int kMaxColsPerThread = 4; /* interfaces up to 1024 nodes */
+
+struct RowsArgs {
+    int32_t n_jobs;
+    int32_t rbf_kind;
+    int32_t stencil_size;
+    double gauss_eps;
+    const int32_t* job_iface;
+    const uint32_t* paths;
+    const int64_t* rec_off;
+    const int64_t* g_off;
+    const double* cx;
+    const double* cy;
+    const int32_t* code;
+    const double* c2;
+    const int32_t* node_offset;
+    const double* theta;
+    const int64_t* ipsi_offset;
+    const double* ipsi;
+    const double* xt;
+    const double* yt;
+    const double* Y;
+    const int8_t* outcome;
+    double* G;
+};
+
+__global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
+    const int job = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int64_t np = a.paths[job];
+    const int iface = a.job_iface[job];
+    const int nb = a.node_offset[iface];
+    const int M = a.node_offset[iface + 1] - nb;
+    const double* th = a.theta + nb;
+    const int code = a.code[iface];
+    const double cx = a.cx[iface], cy = a.cy[iface], c2 = a.c2[iface];
+    const bool stencil = a.rbf_kind == SPAC_RBF_GAUSS_STENCIL;
+    const int S = stencil ? min(a.stencil_size, M) : 0;
+    const double* ipsi = a.ipsi + (a.ipsi_offset[iface] >= 0 ? a.ipsi_offset[iface] : 0);
+    double* G = a.G + a.g_off[job];
+
+    __shared__ double s_th[kMaxColsPerThread * kRowsBlock];
+    __shared__ double s_Y[kRowsBlock];
+    __shared__ double s_t[kRowsBlock];           /* exit angle (IMQ) */
+    __shared__ int s_s[kRowsBlock];              /* stencil start, -1: not a circle exit */
+    __shared__ double s_phi[kRowsBlock * kMaxStencil];
+    __shared__ double s_S[kMaxColsPerThread * kRowsBlock];
+
+    for (int i = tid; i < M; i += kRowsBlock) s_th[i] = th[i];
+    __syncthreads();
+    if (np == 0) {
+        for (int j = tid; j < M; j += kRowsBlock) G[j] = 0.0;
+        return;
+    }
+    spac_ksum acc[kMaxColsPerThread];
+#pragma unroll
+    for (int q = 0; q < kMaxColsPerThread; ++q) acc[q] = spac_ksum_zero();
+    const int64_t off = a.rec_off[job];
+    for (int64_t base = 0; base < np; base += kRowsBlock) {
+        const int cnt = (np - base < (int64_t)kRowsBlock) ? (int)(np - base) : kRowsBlock;
+        if (tid < cnt) {
+            const int64_t g = off + base + tid;
+            int s = -1;
+            if (a.outcome[g] == SPAC_OUT_SUBDOMAIN) {
+                double theta = 0.0;
+                spac_exit_angle(code, a.xt[g] - cx, a.yt[g] - cy, s_th[0], &theta);
+                s_Y[tid] = a.Y[g];
+                s_t[tid] = theta;
+                if (stencil) {
+                    double* phi = &s_phi[tid * kMaxStencil];
+                    s = (code == 1) ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
+                                    : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
+                } else {
+                    s = 0;
+                }
+            }
+            s_s[tid] = s;
+        }
+        __syncthreads();
+        for (int p = 0; p < cnt; ++p) {
+            const int s = s_s[p];
+            if (s < 0) continue; /* uniform across the block */
+            const double Yp = s_Y[p];
+#pragma unroll
+            for (int q = 0; q < kMaxColsPerThread; ++q) {
+                const int j = tid + q * kRowsBlock;
+                if (j >= M) break;
+                if (stencil) {
+                    int rel = j - s;
+                    if (code == 1) { if (rel < 0) rel += M; }
+                    if (rel < 0 || rel >= S) continue;
+                    const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s * S * S);
+                    const double* phi = &s_phi[p * kMaxStencil];
+                    double w = 0.0;
+                    for (int i = 0; i < S; ++i) w = w + blk[i * S + rel] * phi[i];
+                    acc[q] = spac_ksum_add(acc[q], Yp * w);
+                } else {
+                    acc[q] = spac_ksum_add(acc[q], Yp * spac_imq(s_t[p], s_th[j], c2));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const double N = (double)np;
+    if (stencil) {
+#pragma unroll
+        for (int q = 0; q < kMaxColsPerThread; ++q) {
+            const int j = tid + q * kRowsBlock;
+            if (j < M) G[j] = -spac_ksum_value(acc[q]) / N;
+        }
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxColsPerThread; ++q) {
+        const int j = tid + q * kRowsBlock;
+        if (j < M) s_S[j] = spac_ksum_value(acc[q]);
+    }
+    __syncthreads();
+    for (int j = tid; j < M; j += kRowsBlock) {
+        double w = 0.0;
+        for (int i = 0; i < M; ++i) w = w + ipsi[(int64_t)i * M + j] * s_S[i];
+        G[j] = -w / N;
+    }
+}
+
+/* B for boundary nodes (g at the node) and the copy of every node's B. */
+template <class Coef>
+__global__ void fk_rows_b_kernel(int32_t n, const int32_t* frontera, const double* x, const double* y,
+                                 const spac_point_stats* stats, double* B) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    B[k] = frontera[k] ? Coef::g(x[k], y[k]) : stats[k].B;
+}
+
+static spac_status validate(const spac_interface_table* t, const spac_node_batch* nb) {
+    if (!t || !nb) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (t->n_interfaces < 1 || !t->cx || !t->cy || !t->radius || !t->code || !t->node_offset || !t->node_theta ||
+        !t->node_global || !t->ipsi_offset || !t->ipsi || !t->c2)
+        return set_error(SPAC_E_INVALID_ARG, "interface table is incomplete");
+    if (t->rbf_kind != SPAC_RBF_IMQ_GLOBAL && t->rbf_kind != SPAC_RBF_GAUSS_STENCIL)
+        return set_error(SPAC_E_INVALID_ARG, "unknown rbf_kind %d", t->rbf_kind);
+    if (t->rbf_kind == SPAC_RBF_GAUSS_STENCIL &&
+        (t->stencil_size < 1 || t->stencil_size > kMaxStencil || !(t->gauss_eps > 0.0)))
+        return set_error(SPAC_E_INVALID_ARG, "stencil_size must be in [1, %d] and gauss_eps > 0", kMaxStencil);
+    for (int k = 0; k < t->n_interfaces; ++k) {
+        int M = t->node_offset[k + 1] - t->node_offset[k];
+        if (M < 1 || M > kMaxColsPerThread * kRowsBlock)
+            return set_error(SPAC_E_INVALID_ARG, "interface %d has %d nodes (1..%d supported)", k, M,
+                             kMaxColsPerThread * kRowsBlock);
+        int c = t->code[k];
+        bool known = c == 1 || c == -1 || c == -2 || c == -3 || c == -4 || c == -11 || c == -22 || c == -33 || c == -44;
+        if (!known) return set_error(SPAC_E_BAD_INTERFACE, "interface %d has unknown code %d", k, c);
+        if (!(t->radius[k] > 0.0)) return set_error(SPAC_E_INVALID_ARG, "interface %d has radius <= 0", k);
+    }
+    if (nb->n < 1 || !nb->x || !nb->y || !nb->iface || !nb->frontera || !nb->paths || !nb->h || !nb->stream)
+        return set_error(SPAC_E_INVALID_ARG, "node batch is incomplete");
+    for (int k = 0; k < nb->n; ++k) {
+        int f = nb->iface[k];
+        if (f < 0 || f >= t->n_interfaces) return set_error(SPAC_E_INVALID_ARG, "node %d: interface %d out of range", k, f);
+        if (nb->frontera[k]) continue;
+        if (nb->paths[k] < 1) return set_error(SPAC_E_INVALID_ARG, "node %d has no paths", k);
+        if (t->rbf_kind == SPAC_RBF_IMQ_GLOBAL && (t->code[f] == 1 || t->ipsi_offset[f] < 0))
+            return set_error(SPAC_E_BAD_INTERFACE,
+                             "node %d: interface %d is a full circle; global IMQ rows exist for arcs only "
+                             "(Mesh/Interfaz.cpp:248-250)", k, f);
+    }
+    return SPAC_OK;
+}
+
+} // namespace spac
+
+using namespace spac;
+
+extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interface_table* t,
+                                             const spac_node_batch* nb, uint64_t seed, spac_row_block* out) {
+    if (!hp || !out || !out->B || !out->G || !out->G_col || !out->G_offset)
+        return set_error(SPAC_E_INVALID_ARG, "null argument");
+    spac_status rc = validate(t, nb);
+    if (rc) return rc;
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    const int n = nb->n;
+    /* host-side job tables */
+    std::vector<uint32_t> paths(n);
+    std::vector<double> circle(3 * (size_t)n);
+    std::vector<int64_t> goff(n + 1, 0);
+    for (int k = 0; k < n; ++k) {
+        int f = nb->iface[k];
+        paths[k] = nb->frontera[k] ? 0u : nb->paths[k];
+        circle[3 * k] = t->radius[f];
+        circle[3 * k + 1] = t->cx[f];
+        circle[3 * k + 2] = t->cy[f];
+        int M = t->node_offset[f + 1] - t->node_offset[f];
+        goff[k + 1] = goff[k] + M;
+        for (int j = 0; j < M; ++j) out->G_col[goff[k] + j] = t->node_global[t->node_offset[f] + j];
+    }
+    for (int k = 0; k <= n; ++k) out->G_offset[k] = goff[k];
+    SolveSet ss;
+    if ((rc = p->upload_solves(n, nb->x, nb->y, paths.data(), nb->h, nb->stream, circle.data(), &ss))) return rc;
+    /* interface table */
+    const int K = t->n_interfaces;
+    const int nodes = t->node_offset[K];
+    int64_t nipsi = 0;
+    for (int k = 0; k < K; ++k) {
+        if (t->ipsi_offset[k] < 0) continue;
+        int M = t->node_offset[k + 1] - t->node_offset[k];
+        int64_t sz;
+        if (t->rbf_kind == SPAC_RBF_IMQ_GLOBAL) sz = (int64_t)M * M;
+        else {
+            int S = std::min(t->stencil_size, M);
+            sz = (t->code[k] == 1 ? 1 : (M - S + 1)) * (int64_t)S * S;
+        }
+        nipsi = std::max(nipsi, t->ipsi_offset[k] + sz);
+    }
+    const size_t szK = (size_t)K;
+    if ((rc = p->ifc.ensure(szK * (3 * 8 + 4)))) return rc;           /* cx, cy, c2, code */
+    if ((rc = p->ifnode.ensure((size_t)nodes * 8))) return rc;
+    if ((rc = p->ifipsi.ensure((size_t)std::max<int64_t>(nipsi, 1) * 8))) return rc;
+    if ((rc = p->rows_iface.ensure((size_t)n * 4 + (size_t)n * 4))) return rc; /* job_iface, frontera */
+    if ((rc = p->rows_goff.ensure((size_t)(n + 1) * 8))) return rc;
+    if ((rc = p->rows_G.ensure((size_t)goff[n] * 8))) return rc;
+    if ((rc = p->rows_part.ensure((size_t)n * 8))) return rc;            /* B */
+    if ((rc = p->stats.ensure((size_t)n * sizeof(spac_point_stats)))) return rc;
+    char* ifc = (char*)p->ifc.ptr;
+    double* d_cx = (double*)ifc;
+    double* d_cy = d_cx + K;
+    double* d_c2 = d_cy + K;
+    int32_t* d_code = (int32_t*)(d_c2 + K);
+    const size_t noff_bytes = (((szK + 1) * 4 + 7) / 8) * 8; /* node_offset, then 8-aligned ipsi_offset */
+    if ((rc = p->ifoff.ensure(noff_bytes + szK * 8))) return rc;
+    int32_t* d_noff = (int32_t*)p->ifoff.ptr;
+    int64_t* d_ioff = (int64_t*)((char*)p->ifoff.ptr + noff_bytes);
+    int32_t* d_iface = (int32_t*)p->rows_iface.ptr;
+    int32_t* d_front = d_iface + n;
+    hipStream_t s = p->stream;
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_cx, t->cx, szK * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_cy, t->cy, szK * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_c2, t->c2, szK * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_code, t->code, szK * 4, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_noff, t->node_offset, (szK + 1) * 4, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_ioff, t->ipsi_offset, szK * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(p->ifnode.ptr, t->node_theta, (size_t)nodes * 8, hipMemcpyHostToDevice, s));
+    if (nipsi > 0) ROWS_HIP_CHECK(hipMemcpyAsync(p->ifipsi.ptr, t->ipsi, (size_t)nipsi * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_iface, nb->iface, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_front, nb->frontera, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(p->rows_goff.ptr, goff.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipStreamSynchronize(s));
+    /* 1 + 2: paths and node estimators (node mode) */
+    spac_point_stats* d_stats = (spac_point_stats*)p->stats.ptr;
+    if ((rc = p->run(ss, seed, d_stats, 1))) return rc;
+    /* 3: B, including boundary nodes */
+    double* d_B = (double*)p->rows_part.ptr;
+    int nblk = (n + 255) / 256;
+    if (p->desc.coef_kind == SPAC_COEF_FRANKE_POISSON)
+        hipLaunchKernelGGL(fk_rows_b_kernel<CoefFranke>, dim3(nblk), dim3(256), 0, s, n, d_front, ss.x0, ss.y0, d_stats, d_B);
+    else
+        hipLaunchKernelGGL(fk_rows_b_kernel<CoefVarFranke>, dim3(nblk), dim3(256), 0, s, n, d_front, ss.x0, ss.y0, d_stats,
+                           d_B);
+    ROWS_HIP_CHECK(hipGetLastError());
+    /* 4: G */
+    RowsArgs ra;
+    std::memset(&ra, 0, sizeof ra);
+    ra.n_jobs = n;
+    ra.rbf_kind = t->rbf_kind;
+    ra.stencil_size = t->stencil_size;
+    ra.gauss_eps = t->gauss_eps;
+    ra.job_iface = d_iface;
+    ra.paths = ss.paths;
+    ra.rec_off = ss.rec_off;
+    ra.g_off = (const int64_t*)p->rows_goff.ptr;
+    ra.cx = d_cx; ra.cy = d_cy; ra.code = d_code; ra.c2 = d_c2;
+    ra.node_offset = d_noff;
+    ra.theta = (const double*)p->ifnode.ptr;
+    ra.ipsi_offset = d_ioff;
+    ra.ipsi = (const double*)p->ifipsi.ptr;
+    ra.xt = (const double*)p->rec_xt.ptr;
+    ra.yt = (const double*)p->rec_yt.ptr;
+    ra.Y = (const double*)p->rec_Y.ptr;
+    ra.outcome = (const int8_t*)p->rec_outcome.ptr;
+    ra.G = (double*)p->rows_G.ptr;
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[2], s));
+    hipLaunchKernelGGL(fk_rows_g_kernel, dim3(n), dim3(kRowsBlock), 0, s, ra);
+    ROWS_HIP_CHECK(hipGetLastError());
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[3], s));
+    ROWS_HIP_CHECK(hipStreamSynchronize(s));
+    float ms_g = 0.f, ms_all = 0.f;
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_g, p->ev[2], p->ev[3]));
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_all, p->ev[0], p->ev[3]));
+    p->last.reduce_kernel_ms += ms_g;
+    p->last.total_ms = ms_all; /* all device work of the call, first kernel to last */
+    ROWS_HIP_CHECK(hipMemcpy(out->B, d_B, (size_t)n * 8, hipMemcpyDeviceToHost));
+    ROWS_HIP_CHECK(hipMemcpy(out->G, p->rows_G.ptr, (size_t)goff[n] * 8, hipMemcpyDeviceToHost));
+    std::vector<spac_point_stats> st(n);
+    ROWS_HIP_CHECK(hipMemcpy(st.data(), d_stats, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
+    if (out->stats) std::memcpy(out->stats, st.data(), (size_t)n * sizeof(spac_point_stats));
+    spac_status worst = SPAC_OK;
+    for (int k = 0; k < n; ++k)
+        if (st[k].status < worst) worst = st[k].status;
+    if (worst == SPAC_E_NAN) return set_error(SPAC_E_NAN, "NaN reached a row estimator");
+    if (worst == SPAC_E_MAX_STEPS) return set_error(SPAC_E_MAX_STEPS, "paths stopped by max_steps");
+    return SPAC_OK;
 }

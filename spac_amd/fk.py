@@ -138,3 +138,15 @@ class FKProblem:
                                        stats)
         abi.check(rc, self.lib)
         return rec, (stats_to_dicts(stats) if stats is not None else None)
+
+    def assemble_rows(self, mesh, paths, h, seed=0, jobs=None, allow_status=False):
+        """spac_fk_assemble_rows over a mesh's jobs (or a subset): B, G, G_col, G_offset, stats."""
+        nb, keep = mesh.node_batch(paths, h, jobs)
+        rows = mesh.alloc_rows(jobs)
+        rb = mesh.row_block(rows)
+        rc = self.lib.spac_fk_assemble_rows(self.h, C.byref(mesh.table), C.byref(nb), seed, C.byref(rb))
+        if not allow_status:
+            abi.check(rc, self.lib)
+        rows["stats"] = stats_to_dicts(rows["stats"])
+        rows["rc"] = rc
+        return rows

@@ -117,3 +117,67 @@ def path_terms(node_mode, Y, Z, xi, tau, gval, steps, outcome) -> np.ndarray:
     out = np.empty(10, dtype=np.float64)
     load().oracle_path_terms(1 if node_mode else 0, Y, Z, xi, tau, gval, steps, outcome, _p(out))
     return out
+
+
+def assemble_rows(desc: abi.ProblemDesc, mesh, paths, h, seed=0, jobs=None, direct=False, nthreads=0):
+    """CPU restatement of spac_fk_assemble_rows over a mesh's jobs (or a subset)."""
+    lib = load()
+    lib.oracle_assemble_rows.argtypes = [C.POINTER(abi.ProblemDesc), C.POINTER(abi.InterfaceTable),
+                                         C.POINTER(abi.NodeBatch), C.c_uint64, C.c_int, C.c_int,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(abi.PointStats)]
+    lib.oracle_assemble_rows.restype = C.c_int
+    nb, keep = node_batch(mesh, paths, h, jobs)
+    n = nb.n
+    a = mesh.iface_arrays()
+    M = a["node_offset"][1:] - a["node_offset"][:-1]
+    total = int(M[keep["iface"]].sum())
+    B = np.zeros(n)
+    G = np.zeros(total)
+    st = (abi.PointStats * n)()
+    rc = lib.oracle_assemble_rows(C.byref(desc), C.byref(mesh.table), C.byref(nb), seed, nthreads, 1 if direct else 0,
+                                  _p(B), _p(G), st)
+    assert rc == 0, rc
+    return B, G, stats_to_dicts(st)
+
+
+def node_batch(mesh, paths, h, jobs=None):
+    """A spac_node_batch over all of a mesh's jobs or the selected ones (stream = job index)."""
+    idx = np.arange(mesh.info.n_jobs) if jobs is None else np.asarray(jobs)
+    n = idx.size
+    keep = dict(x=np.ascontiguousarray(mesh.job_x[idx]), y=np.ascontiguousarray(mesh.job_y[idx]),
+                iface=np.ascontiguousarray(mesh.job_iface[idx]), frontera=np.ascontiguousarray(mesh.job_frontera[idx]),
+                paths=np.ascontiguousarray(np.broadcast_to(paths, (n,)), dtype=np.uint32),
+                h=np.ascontiguousarray(np.broadcast_to(h, (n,)), dtype=np.float64),
+                stream=np.ascontiguousarray(idx, dtype=np.uint32))
+    nb = abi.NodeBatch()
+    nb.n = n
+    nb.x = abi.ptr(keep["x"], C.c_double)
+    nb.y = abi.ptr(keep["y"], C.c_double)
+    nb.iface = abi.ptr(keep["iface"], C.c_int32)
+    nb.frontera = abi.ptr(keep["frontera"], C.c_int32)
+    nb.paths = abi.ptr(keep["paths"], C.c_uint32)
+    nb.h = abi.ptr(keep["h"], C.c_double)
+    nb.stream = abi.ptr(keep["stream"], C.c_uint32)
+    nb._keep = keep
+    return nb, keep
+
+
+def row_weights(mesh, f: int, theta: float) -> np.ndarray:
+    lib = load()
+    lib.oracle_row_weights.argtypes = [C.POINTER(abi.InterfaceTable), C.c_int, C.c_double, C.POINTER(C.c_double)]
+    lib.oracle_row_weights.restype = C.c_int
+    a = mesh.iface_arrays()
+    M = int(a["node_offset"][f + 1] - a["node_offset"][f])
+    out = np.zeros(M)
+    rc = lib.oracle_row_weights(C.byref(mesh.table), f, theta, _p(out))
+    assert rc == 0, rc
+    return out
+
+
+def exit_angle(code: int, dx: float, dy: float, first: float) -> float:
+    lib = load()
+    lib.oracle_exit_angle.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    lib.oracle_exit_angle.restype = C.c_int
+    t = C.c_double()
+    assert lib.oracle_exit_angle(code, dx, dy, first, C.byref(t)) == 1
+    return t.value

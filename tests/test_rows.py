@@ -1,0 +1,144 @@
+"""Interface rows (G, B): CPU properties and GPU parity.
+
+CPU pins of the restatement's rows:
+- boundary nodes: B = g(x) exactly (Solvers/FeynmanKacSolver.cpp:346-347);
+- with Y = 1 (c = 0), sum_j G_j = -(fraction of circle exits) up to the
+  interpolation's reproduction error of constants;
+- global IMQ: the moment form equals the reference's direct form
+  (Mesh/Interfaz.cpp:253-261) within cond(Psi) * eps.
+GPU: B, G and the node statistics are bit-identical to the restatement; the
+full PDDSparse workload (config 2) solves to the manufactured solution.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as o
+from spac_amd import abi
+from spac_amd.fk import FKProblem, problem_desc
+from spac_amd.mesh import Mesh, gmres, mesh_desc
+from test_oracle import franke_published
+
+FR = abi.SPAC_COEF_FRANKE_POISSON
+VC = abi.SPAC_COEF_VARCOEF_FRANKE
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def serial_mesh():
+    return Mesh(mesh_desc(domain=(0.0, 0.0, 1.25, 1.25), n_side=4, nodes_per_circle=44, overlap=1.0, theta0=0.1428,
+                          c2_corner=1.5, c2_side=1.1, rbf_kind=abi.SPAC_RBF_IMQ_GLOBAL, fk_all_circles=False))
+
+
+def _offsets(mesh, jobs):
+    a = mesh.iface_arrays()
+    M = a["node_offset"][1:] - a["node_offset"][:-1]
+    return np.concatenate([[0], np.cumsum(M[mesh.job_iface[jobs]])])
+
+
+def test_boundary_rows_and_row_sums():
+    m = Mesh(mesh_desc())
+    jobs = np.arange(0, m.info.n_jobs, 23)
+    assert m.job_frontera[jobs].any()
+    desc = problem_desc(coef=FR, domain=(0.0, 0.0, 1.0, 1.0))
+    B, G, st = o.assemble_rows(desc, m, 600, 5e-4, seed=3, jobs=jobs)
+    off = _offsets(m, jobs)
+    for k, j in enumerate(jobs):
+        Gk = G[off[k]:off[k + 1]]
+        if m.job_frontera[j]:
+            assert B[k] == o.coef_eval(FR, 0, m.job_x[j:j + 1], m.job_y[j:j + 1])[0]
+            assert not Gk.any() and st[k]["paths"] == 0
+        else:
+            frac = 1.0 - st[k]["n_domain_exits"] / st[k]["paths"]
+            assert abs(Gk.sum() + frac) < 2e-3, (j, Gk.sum(), frac)
+
+
+def test_imq_moment_form_equals_direct_form():
+    m = serial_mesh()
+    jobs = np.arange(0, m.info.n_jobs, 5)
+    desc = problem_desc(coef=FR, domain=(0.0, 0.0, 1.25, 1.25))
+    B1, G1, _ = o.assemble_rows(desc, m, 400, 1e-3, seed=1, jobs=jobs)
+    B2, G2, _ = o.assemble_rows(desc, m, 400, 1e-3, seed=1, jobs=jobs, direct=True)
+    assert np.array_equal(B1, B2)
+    # |iPsi| ~ cond(Psi) ~ 1e10: reassociating the sum moves G by <= cond * eps relative
+    assert np.abs(G1 - G2).max() <= 1e-6 * max(1.0, np.abs(G2).max())
+
+
+def _rows_equal(gpu, B, G, st, jobs, mesh):
+    assert np.array_equal(gpu["B"].view(np.uint64), B.view(np.uint64))
+    bad = np.nonzero(gpu["G"].view(np.uint64) != G.view(np.uint64))[0]
+    assert bad.size == 0, (bad[:5], gpu["G"][bad[:5]], G[bad[:5]])
+    off = _offsets(mesh, jobs)
+    assert np.array_equal(gpu["G_offset"], off)
+    a = mesh.iface_arrays()
+    for k, j in enumerate(jobs):
+        f = mesh.job_iface[j]
+        assert np.array_equal(gpu["G_col"][off[k]:off[k + 1]], a["node_global"][a["node_offset"][f]:a["node_offset"][f + 1]])
+    for s1, s2 in zip(gpu["stats"], st):
+        for key in s1:
+            assert s1[key] == s2[key], key
+
+
+@pytest.mark.gpu
+def test_gpu_rows_bit_identical_stencil(gpu):
+    say("rows parity: Gaussian stencils, variable coefficients")
+    m = Mesh(mesh_desc())
+    jobs = np.concatenate([np.arange(0, m.info.n_jobs, 29), np.nonzero(m.job_frontera)[0][:3]])
+    desc = problem_desc(coef=VC, domain=(0.0, 0.0, 1.0, 1.0))
+    B, G, st = o.assemble_rows(desc, m, 700, 5e-4, seed=11, jobs=jobs)
+    with FKProblem(desc) as p:
+        rows = p.assemble_rows(m, 700, 5e-4, seed=11, jobs=jobs)
+        p.set_launch(128, 3, 37)
+        rows2 = p.assemble_rows(m, 700, 5e-4, seed=11, jobs=jobs)
+    _rows_equal(rows, B, G, st, jobs, m)
+    _rows_equal(rows2, B, G, st, jobs, m)
+
+
+@pytest.mark.gpu
+def test_gpu_rows_bit_identical_imq(gpu):
+    say("rows parity: global IMQ (reference mode), Franke")
+    m = serial_mesh()
+    jobs = np.arange(0, m.info.n_jobs, 3)
+    desc = problem_desc(coef=FR, domain=(0.0, 0.0, 1.25, 1.25))
+    B, G, st = o.assemble_rows(desc, m, 500, 1e-3, seed=5, jobs=jobs)
+    with FKProblem(desc) as p:
+        rows = p.assemble_rows(m, 500, 1e-3, seed=5, jobs=jobs)
+    _rows_equal(rows, B, G, st, jobs, m)
+
+
+@pytest.mark.gpu
+def test_gpu_rows_reject_bad_input(gpu):
+    say("rows: argument checks")
+    m = serial_mesh()
+    desc = problem_desc(coef=FR, domain=(0.0, 0.0, 1.25, 1.25))
+    with FKProblem(desc) as p:
+        rows = p.assemble_rows(m, 0, 1e-3, jobs=np.nonzero(m.job_frontera == 0)[0][:2], allow_status=True)
+        assert rows["rc"] == abi.SPAC_E_INVALID_ARG
+
+
+@pytest.mark.gpu
+def test_config2_pddsparse_solve(gpu):
+    """Unit square, 4x4 circles, 1016 nodes (1952 rows), 20k paths/row, h = 5e-4,
+    variable-coefficient operator, Gaussian 15-point stencils; GMRES to 1e-10."""
+    say("config 2: assemble + GMRES")
+    m = Mesh(mesh_desc())
+    desc = problem_desc(coef=VC, domain=(0.0, 0.0, 1.0, 1.0))
+    with FKProblem(desc) as p:
+        rows = p.assemble_rows(m, 20000, 5e-4, seed=0)
+        rows_b = p.assemble_rows(m, 20000, 5e-4, seed=0)
+    assert np.array_equal(rows["G"], rows_b["G"]) and np.array_equal(rows["B"], rows_b["B"])
+    m.accumulate(rows)
+    Gi, Gj, Gv, B = m.csr()
+    u, info = gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=5000, restart=200)
+    assert info.converged and info.residual < 1e-10
+    exact = franke_published(m.node_x, m.node_y)
+    err = u - exact
+    rms, mx = np.sqrt(np.mean(err ** 2)), np.abs(err).max()
+    say(f"config 2: GMRES iters {info.iterations}, rms err {rms:.3e}, max err {mx:.3e}")
+    assert rms < 1e-2 and mx < 5e-2
+    steps = sum(s["path_steps"] for s in rows["stats"])
+    say(f"config 2: path-steps {steps}")
