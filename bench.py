@@ -174,10 +174,9 @@ def bench_rows(args):
     t0 = time.perf_counter()
     u, info = gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=5000, restart=200)
     t_gmres = time.perf_counter() - t0
-    sys.path.insert(0, str(ROOT / "tests"))
-    from test_oracle import franke_published
+    from spac_amd.problems import franke_u
 
-    err = u - franke_published(mesh.node_x, mesh.node_y)
+    err = u - franke_u(mesh.node_x, mesh.node_y)
     dev_s = float(np.mean(dev_ms)) * 1e-3
     avg_path_ms = float(np.mean(path_ms))
     fps = FLOPS_PER_STEP["varcoef_circle_fp64"]
