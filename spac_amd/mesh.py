@@ -189,7 +189,8 @@ class Mesh:
         r.G = abi.ptr(rows["G"], C.c_double)
         r.G_col = abi.ptr(rows["G_col"], C.c_int32)
         r.G_offset = abi.ptr(rows["G_offset"], C.c_int64)
-        r.stats = rows["stats"]
+        st = rows.get("stats")
+        r.stats = st if isinstance(st, C.Array) else None
         return r
 
     def accumulate(self, rows: dict, reset=True):
