@@ -42,6 +42,7 @@ namespace spac {
 constexpr int kRowsBlock = 256;
 constexpr int kMaxStencil = 16;
 constexpr int kMaxColsPerThread = 4; /* interfaces up to 1024 nodes */
+constexpr int kWStride = kMaxStencil + 1; /* odd stride: conflict-free LDS rows */
 
 struct RowsArgs {
     int32_t n_jobs;
@@ -86,7 +87,7 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
     __shared__ double s_Y[kRowsBlock];
     __shared__ double s_t[kRowsBlock];           /* exit angle (IMQ) */
     __shared__ int s_s[kRowsBlock];              /* stencil start, -1: not a circle exit */
-    __shared__ double s_phi[kRowsBlock * kMaxStencil];
+    __shared__ double s_w[kRowsBlock * kWStride]; /* per-path stencil weights H_{s+r}(theta) */
     __shared__ double s_S[kMaxColsPerThread * kRowsBlock];
 
     for (int i = tid; i < M; i += kRowsBlock) s_th[i] = th[i];
@@ -110,9 +111,16 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
                 s_Y[tid] = a.Y[g];
                 s_t[tid] = theta;
                 if (stencil) {
-                    double* phi = &s_phi[tid * kMaxStencil];
+                    double phi[kMaxStencil];
                     s = (code == 1) ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
                                     : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
+                    /* H_{s+r} = sum_i iPsiS(i, r) phi_i, i in order (the per-column formula) */
+                    const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s * S * S);
+                    for (int r = 0; r < S; ++r) {
+                        double w = 0.0;
+                        for (int i = 0; i < S; ++i) w = w + blk[i * S + r] * phi[i];
+                        s_w[tid * kWStride + r] = w;
+                    }
                 } else {
                     s = 0;
                 }
@@ -132,11 +140,7 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
                     int rel = j - s;
                     if (code == 1) { if (rel < 0) rel += M; }
                     if (rel < 0 || rel >= S) continue;
-                    const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s * S * S);
-                    const double* phi = &s_phi[p * kMaxStencil];
-                    double w = 0.0;
-                    for (int i = 0; i < S; ++i) w = w + blk[i * S + rel] * phi[i];
-                    acc[q] = spac_ksum_add(acc[q], Yp * w);
+                    acc[q] = spac_ksum_add(acc[q], Yp * s_w[p * kWStride + rel]);
                 } else {
                     acc[q] = spac_ksum_add(acc[q], Yp * spac_imq(s_t[p], s_th[j], c2));
                 }

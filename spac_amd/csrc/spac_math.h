@@ -76,10 +76,10 @@ SPAC_HD double spac_exp(double x) {
     const double LOG2E = 1.4426950408889634;
     const double LN2_HI = 0.6931471803691238;     /* 32 significant bits */
     const double LN2_LO = 1.9082149292705877e-10; /* ln2 - LN2_HI */
-    if (!(x < 709.782712893384)) return (x != x) ? x : spac_u64_as_f64(0x7ff0000000000000ULL);
-    if (x < -745.1332191019412) return 0.0;
-    double kd = SPAC_RINT(x * LOG2E);
-    double r = SPAC_FMA(-kd, LN2_HI, x); /* exact: |k| <= 1075 needs 11 bits */
+    /* clamp keeps k in [-1076, 1025]; out-of-range inputs are fixed up at the end */
+    double xc = (x < -746.0) ? -746.0 : ((x > 710.0) ? 710.0 : x);
+    double kd = SPAC_RINT(xc * LOG2E);
+    double r = SPAC_FMA(-kd, LN2_HI, xc); /* exact: |k| <= 1076 needs 11 bits */
     r = SPAC_FMA(-kd, LN2_LO, r);
     /* e^r = 1 + r + r^2/2! + ... + r^13/13!, |r| <= 0.3466 */
     double p = 1.6059043836821613e-10;
@@ -96,10 +96,14 @@ SPAC_HD double spac_exp(double x) {
     p = SPAC_FMA(p, r, 0.5);
     p = SPAC_FMA(p, r, 1.0);
     p = SPAC_FMA(p, r, 1.0);
+    /* p 2^k as (p 2^k1) 2^k2, k1 = floor(k/2): the first product is exact, the
+     * second rounds once, also into the subnormal range */
     int k = (int)kd;
-    if (k > 1023) return (p * 2.0) * spac_pow2i(k - 1);
-    if (k < -1021) return (p * spac_pow2i(k + 54)) * spac_pow2i(-54);
-    return p * spac_pow2i(k);
+    int k1 = k >> 1;
+    double res = (p * spac_pow2i(k1)) * spac_pow2i(k - k1);
+    if (!(x < 709.782712893384)) res = (x != x) ? x : spac_u64_as_f64(0x7ff0000000000000ULL);
+    if (x < -745.1332191019412) res = 0.0;
+    return res;
 }
 
 /* ----------------------------------------------------------------- log -- */
@@ -163,13 +167,11 @@ SPAC_HD void spac_sincospi(double a, double* s_out, double* c_out) {
     double cr = SPAC_FMA(cp, r2, 1.0);
     int64_t qi = (int64_t)q;
     int quad = (int)(qi & 3);
-    double s, c;
-    switch (quad) {
-    case 0: s = sr; c = cr; break;
-    case 1: s = cr; c = -sr; break;
-    case 2: s = -sr; c = -cr; break;
-    default: s = -cr; c = sr; break;
-    }
+    /* quadrant rotation: q = 0 (sr, cr), 1 (cr, -sr), 2 (-sr, -cr), 3 (-cr, sr) */
+    double s = (quad & 1) ? cr : sr;
+    double c = (quad & 1) ? -sr : cr;
+    s = (quad & 2) ? -s : s;
+    c = (quad & 2) ? -c : c;
     *s_out = s;
     *c_out = c;
 }
