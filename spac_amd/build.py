@@ -87,9 +87,21 @@ def build_oracle() -> Path:
     return out
 
 
+def build_surface_driver() -> Path:
+    """The C++ driver of tests/cpp (links the library like a reference caller would)."""
+    out_dir = ROOT / "tests" / "cpp" / "_bin"
+    out_dir.mkdir(parents=True, exist_ok=True)
+    out = out_dir / "surface_driver"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{CPP}", f"-I{CSRC}", f"-I{ROOT / 'include'}",
+          ROOT / "tests" / "cpp" / "surface_driver.cpp", f"-L{LIBDIR}", "-lspac_fk", f"-Wl,-rpath,{LIBDIR}",
+          "-Wl,-rpath,$ORIGIN/../../../spac_amd/lib", "-o", out])
+    return out
+
+
 def build_all() -> None:
     build_oracle()
     build_library()
+    build_surface_driver()
     from oracle import build_ref  # noqa: F401  (oracle/_ref: reference sources, when present)
     build_ref.build_if_present()
 
@@ -101,5 +113,7 @@ if __name__ == "__main__":
         build_oracle()
     if "lib" in what:
         build_library()
+    if "driver" in what:
+        build_surface_driver()
     if "all" in what:
         build_all()

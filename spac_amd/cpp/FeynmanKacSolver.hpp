@@ -1,0 +1,74 @@
+/*
+ * FeynmanKacSolver.hpp — drop-in for the reference's solver class
+ * (Solvers/FeynmanKacSolver.hpp:43-99): same public estimator fields and the
+ * same two Solve_OMP_Analytic overloads, implemented over the C ABI of
+ * include/spac_fk.h, so the paths run on the GPU.
+ *
+ * Differences a caller can see:
+ *   - the BVP and Interfaz are taken by const reference (the reference copies
+ *     both per call, :335-355);
+ *   - every solve returns a spac_status (0 = OK) instead of printing ERROR;
+ *   - RNG: one Philox stream per call (seed = MPI rank, stream = call count),
+ *     so a run is reproducible for any thread count;
+ *   - xi/xixi are the true moments and RNGC counts normals drawn;
+ *   - a default (unset) subdomain frontier means "no subdomain", not a
+ *     distance-0 boundary that stops every path after one step;
+ *   - rho is accepted and ignored: the reflecting step is never reached with
+ *     absorbing (Stopping) frontiers, the only kind the shipped drivers use.
+ */
+#ifndef SPAC_FEYNMANKACSOLVER_HPP
+#define SPAC_FEYNMANKACSOLVER_HPP
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/spac_fk.h"
+#include "BVP.hpp"
+#include "Interfaz.hpp"
+#include "Nudo.hpp"
+
+namespace spac {
+
+class FeynmanKacSolver {
+public:
+    double h = 0.0, sqrth = 0.0;
+    unsigned int N = 0;
+    double eps = 0.0;
+    double phi = 0, phiphi = 0, phi_sublinear = 0, phiphi_sublinear = 0, phi_VR = 0, phiphi_VR = 0,
+           phi_sublinearVR = 0, phiphi_sublinearVR = 0, xi = 0, xixi = 0, xi_sublinear = 0, xixi_sublinear = 0,
+           xiphi = 0, xiphi_sublinear = 0, phi_num = 0, phiphi_num = 0, phi_sublinear_num = 0,
+           phiphi_sublinear_num = 0, phi_VR_num = 0, phiphi_VR_num = 0, phi_sublinearVR_num = 0,
+           phiphi_sublinearVR_num = 0, xi_num = 0, xixi_num = 0, xi_sublinear_num = 0, xixi_sublinear_num = 0,
+           xiphi_num = 0, xiphi_sublinear_num = 0, APL = 0, RNGC = 0;
+    int64_t path_steps = 0;     /* executed Paso_Absorbente + Dentro pairs of the last solve */
+    uint32_t max_steps = 0;     /* cap per path (0: none) */
+
+    FeynmanKacSolver();
+    explicit FeynmanKacSolver(unsigned int MPIrank);
+    ~FeynmanKacSolver();
+    FeynmanKacSolver(const FeynmanKacSolver&) = delete;
+    FeynmanKacSolver& operator=(const FeynmanKacSolver&) = delete;
+
+    /* Solvers/FeynmanKacSolver.cpp:335-342 */
+    int Solve_OMP_Analytic(Vec2 X0, unsigned int numero_trayectorias, double discretizacion_temporal, double rho,
+                           const BVP& Problema);
+    /* Solvers/FeynmanKacSolver.cpp:344-355: fills nudo.G (over the interface's
+     * nodes) and nudo.B */
+    int Solve_OMP_Analytic(Nudo& nudo, unsigned int numero_trayectorias, double discretizacion_temporal, double rho,
+                           const BVP& Problema, const Interfaz& interfaz);
+    /* Batched form: every (node, interface) pair in one GPU launch. */
+    int Solve_Nodes(std::vector<Nudo*>& nudos, const std::vector<const Interfaz*>& interfaces,
+                    unsigned int numero_trayectorias, double discretizacion_temporal, const BVP& Problema);
+
+private:
+    spac_problem* prob_ = nullptr;
+    spac_problem_desc desc_{};
+    uint64_t seed_ = 0;
+    uint32_t calls_ = 0;
+    int ensure_problem(const BVP& P, bool with_subdomain);
+    void copy_stats(const spac_point_stats& s);
+};
+
+} // namespace spac
+
+#endif

@@ -119,14 +119,14 @@ def path_terms(node_mode, Y, Z, xi, tau, gval, steps, outcome) -> np.ndarray:
     return out
 
 
-def assemble_rows(desc: abi.ProblemDesc, mesh, paths, h, seed=0, jobs=None, direct=False, nthreads=0):
+def assemble_rows(desc: abi.ProblemDesc, mesh, paths, h, seed=0, jobs=None, direct=False, nthreads=0, streams=None):
     """CPU restatement of spac_fk_assemble_rows over a mesh's jobs (or a subset)."""
     lib = load()
     lib.oracle_assemble_rows.argtypes = [C.POINTER(abi.ProblemDesc), C.POINTER(abi.InterfaceTable),
                                          C.POINTER(abi.NodeBatch), C.c_uint64, C.c_int, C.c_int,
                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(abi.PointStats)]
     lib.oracle_assemble_rows.restype = C.c_int
-    nb, keep = node_batch(mesh, paths, h, jobs)
+    nb, keep = node_batch(mesh, paths, h, jobs, streams)
     n = nb.n
     a = mesh.iface_arrays()
     M = a["node_offset"][1:] - a["node_offset"][:-1]
@@ -140,7 +140,7 @@ def assemble_rows(desc: abi.ProblemDesc, mesh, paths, h, seed=0, jobs=None, dire
     return B, G, stats_to_dicts(st)
 
 
-def node_batch(mesh, paths, h, jobs=None):
+def node_batch(mesh, paths, h, jobs=None, streams=None):
     """A spac_node_batch over all of a mesh's jobs or the selected ones (stream = job index)."""
     idx = np.arange(mesh.info.n_jobs) if jobs is None else np.asarray(jobs)
     n = idx.size
@@ -148,7 +148,7 @@ def node_batch(mesh, paths, h, jobs=None):
                 iface=np.ascontiguousarray(mesh.job_iface[idx]), frontera=np.ascontiguousarray(mesh.job_frontera[idx]),
                 paths=np.ascontiguousarray(np.broadcast_to(paths, (n,)), dtype=np.uint32),
                 h=np.ascontiguousarray(np.broadcast_to(h, (n,)), dtype=np.float64),
-                stream=np.ascontiguousarray(idx, dtype=np.uint32))
+                stream=np.ascontiguousarray(idx if streams is None else streams, dtype=np.uint32))
     nb = abi.NodeBatch()
     nb.n = n
     nb.x = abi.ptr(keep["x"], C.c_double)

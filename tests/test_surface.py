@@ -1,0 +1,68 @@
+"""The drop-in C++ surface (BVP / Coeficiente_* / Frontera / FeynmanKacSolver /
+Malla / SistemaLineal) driven the way the reference's serial driver drives it
+(main_test_serial.cpp:11-83), from a separately compiled C++ program.
+"""
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as o
+from spac_amd import abi
+from spac_amd.fk import SolveBatchArrays, problem_desc
+from spac_amd.mesh import Mesh, mesh_desc
+
+DRIVER = Path(__file__).resolve().parent / "cpp" / "_bin" / "surface_driver"
+
+
+def run_driver(mode):
+    assert DRIVER.exists(), "build() compiles tests/cpp/surface_driver.cpp"
+    out = subprocess.run([str(DRIVER), mode], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    return json.loads(out.stdout)
+
+
+def test_surface_cpu():
+    r = run_driver("cpu")
+    assert r["mesh_rc"] == 0 and r["nodes"] == 408
+    assert r["coef_rc"] == 0 and r["coef_kind"] == abi.SPAC_COEF_FRANKE_POISSON
+    assert r["dom_kind"] == abi.SPAC_BOUNDARY_RECTANGLE and r["sub_kind"] == abi.SPAC_BOUNDARY_CIRCLE
+    assert r["bad_rc"] == abi.SPAC_E_UNREGISTERED_PLUGIN
+    # Update_G (host, direct form): the IMQ cardinal functions sum to ~1 on the arc
+    assert r["update_g_rc"] == 0 and abs(r["update_g_sum"] + 1.0) < 1e-4
+    m = Mesh(mesh_desc(domain=(0.0, 0.0, 1.25, 1.25), n_side=4, nodes_per_circle=44, overlap=1.0, theta0=0.1428,
+                       rbf_kind=abi.SPAC_RBF_IMQ_GLOBAL, fk_all_circles=False))
+    assert r["nnz"] == m.info.nnz
+
+
+@pytest.mark.gpu
+def test_surface_gpu_rows_match_restatement(gpu):
+    print("surface driver gpu", flush=True)
+    r = run_driver("gpu")
+    assert r["worst_rc"] == 0
+    m = Mesh(mesh_desc(domain=(0.0, 0.0, 1.25, 1.25), n_side=4, nodes_per_circle=44, overlap=1.0, theta0=0.1428,
+                       rbf_kind=abi.SPAC_RBF_IMQ_GLOBAL, fk_all_circles=False))
+    jobs = np.arange(m.info.n_jobs)
+    assert np.array_equal(m.job_row[jobs], r["rows"])
+    live = m.job_frontera == 0
+    streams = np.cumsum(live) - 1  # one Philox stream per FK call, in call order
+    desc = problem_desc(coef=abi.SPAC_COEF_FRANKE_POISSON, domain=(0.0, 0.0, 1.25, 1.25))
+    B, G, st = o.assemble_rows(desc, m, 400, 1e-3, seed=0, jobs=jobs, streams=np.maximum(streams, 0))
+    a = m.iface_arrays()
+    M = a["node_offset"][1:] - a["node_offset"][:-1]
+    off = np.concatenate([[0], np.cumsum(M[m.job_iface])])
+    Bd, Gs = np.array(r["B"]), np.array(r["Gsum"])
+    assert r["calls"] == int(live.sum())
+    for k in range(jobs.size):
+        if live[k]:
+            assert Bd[k] == B[k]
+            assert Gs[k] == pytest.approx(G[off[k]:off[k + 1]].sum(), rel=1e-12, abs=1e-14)
+        else:  # B = g(x) with the host plugin (libm exp) vs the device twin: within 2 ulp
+            assert Bd[k] == pytest.approx(B[k], rel=5e-16, abs=0) and Gs[k] == 0.0
+    # point solve through Solve_OMP_Analytic(X0, ...)
+    b = SolveBatchArrays([0.6], [0.7], 2000, 1e-3, [0])
+    so = o.reduce(b, o.simulate(problem_desc(domain=(0.0, 0.0, 1.25, 1.25)), b, seed=0))[0]
+    assert r["point_rc"] == 0 and r["point_phi"] == so["phi"] and r["point_phi_VR"] == so["phi_VR"]
+    assert r["point_steps"] == so["path_steps"]
