@@ -178,6 +178,126 @@ __device__ __forceinline__ BoundaryHit circle_distance(double R, double cx, doub
     return r;
 }
 
+/* ----------------------------------------------------- FP32 stepping --
+ * Mixed-precision twins: coefficients, increments and the per-step stopping
+ * test in FP32 with the gfx950 transcendental instructions (v_exp_f32 = 2^x,
+ * v_log_f32 = log2 x, v_sin_f32 / v_cos_f32 with the argument in revolutions);
+ * the weight Y and the sums Z, xi stay FP64, and exits are projected in FP64. */
+struct StepCoef32 {
+    float s, bx, by, c, f, Fx, Fy;
+};
+
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+/* sin(pi x), cos(pi x) */
+__device__ __forceinline__ float fsinpi(float x) { return __builtin_amdgcn_sinf(0.5f * x); }
+__device__ __forceinline__ float fcospi(float x) { return __builtin_amdgcn_cosf(0.5f * x); }
+
+struct FrankeTerms32 {
+    float a1, b1, a2, a3, b3, a4, b4;
+    float F1, F2, F3, F4;
+};
+__device__ __forceinline__ FrankeTerms32 franke_terms32(float x, float y) {
+    FrankeTerms32 t;
+    t.a1 = 9.0f * x - 2.0f; t.b1 = 9.0f * y - 2.0f;
+    t.a2 = 9.0f * x + 1.0f;
+    t.a3 = 9.0f * x - 7.0f; t.b3 = 9.0f * y - 3.0f;
+    t.a4 = 9.0f * x - 4.0f; t.b4 = 9.0f * y - 7.0f;
+    t.F1 = fexp(-0.25f * (t.a1 * t.a1 + t.b1 * t.b1));
+    t.F2 = fexp((-1.0f / 49.0f) * (t.a2 * t.a2) - 0.1f * (9.0f * y + 1.0f));
+    t.F3 = fexp(-0.25f * (t.a3 * t.a3 + t.b3 * t.b3));
+    t.F4 = fexp(-(t.a4 * t.a4 + t.b4 * t.b4));
+    return t;
+}
+__device__ __forceinline__ float franke_ux32(const FrankeTerms32& t) {
+    return -3.375f * t.a1 * t.F1 - (27.0f / 98.0f) * t.a2 * t.F2 - 2.25f * t.a3 * t.F3 + 3.6f * t.a4 * t.F4;
+}
+__device__ __forceinline__ float franke_uy32(const FrankeTerms32& t) {
+    return -3.375f * t.b1 * t.F1 - 0.675f * t.F2 - 2.25f * t.b3 * t.F3 + 3.6f * t.b4 * t.F4;
+}
+__device__ __forceinline__ float franke_lap32(const FrankeTerms32& t) {
+    float uxx = -3.375f * (9.0f - 4.5f * (t.a1 * t.a1)) * t.F1 -
+                (27.0f / 98.0f) * (9.0f - (18.0f / 49.0f) * (t.a2 * t.a2)) * t.F2 -
+                2.25f * (9.0f - 4.5f * (t.a3 * t.a3)) * t.F3 + 3.6f * (9.0f - 18.0f * (t.a4 * t.a4)) * t.F4;
+    float uyy = -3.375f * (9.0f - 4.5f * (t.b1 * t.b1)) * t.F1 + (243.0f / 400.0f) * t.F2 -
+                2.25f * (9.0f - 4.5f * (t.b3 * t.b3)) * t.F3 + 3.6f * (9.0f - 18.0f * (t.b4 * t.b4)) * t.F4;
+    return uxx + uyy;
+}
+__device__ __forceinline__ float franke_u32(const FrankeTerms32& t) {
+    return 0.75f * t.F1 + 0.75f * t.F2 + 0.5f * t.F3 - 0.2f * t.F4;
+}
+
+struct CoefFranke32 {
+    static constexpr bool kHasDrift = false;
+    static constexpr bool kHasKilling = false;
+    using F64 = CoefFranke;
+    __device__ __forceinline__ static float sigma(float, float) { return 1.41421356237f; }
+    __device__ __forceinline__ static StepCoef32 step(float x, float y) {
+        FrankeTerms32 t = franke_terms32(x, y);
+        StepCoef32 c;
+        c.s = 1.41421356237f;
+        c.bx = c.by = c.c = 0.0f;
+        c.f = -franke_lap32(t);
+        c.Fx = -(c.s * franke_ux32(t));
+        c.Fy = -(c.s * franke_uy32(t));
+        return c;
+    }
+};
+
+struct CoefVarFranke32 {
+    static constexpr bool kHasDrift = true;
+    static constexpr bool kHasKilling = true;
+    using F64 = CoefVarFranke;
+    __device__ __forceinline__ static float sigma(float x, float y) { return 1.0f + 0.5f * fsinpi(x) * fcospi(y); }
+    __device__ __forceinline__ static StepCoef32 step(float x, float y) {
+        FrankeTerms32 t = franke_terms32(x, y);
+        StepCoef32 c;
+        float s = sigma(x, y);
+        c.s = s;
+        c.bx = 0.3f * y;
+        c.by = -0.3f * x;
+        c.c = -1.0f - x * x;
+        float ux = franke_ux32(t), uy = franke_uy32(t);
+        c.f = -(((0.5f * (s * s)) * franke_lap32(t) + (c.bx * ux + c.by * uy)) + c.c * franke_u32(t));
+        c.Fx = -(s * ux);
+        c.Fy = -(s * uy);
+        return c;
+    }
+};
+
+/* Box–Muller in FP32 from the same Philox block as the FP64 path (24-bit uniforms). */
+__device__ __forceinline__ void normal_pair32(uint64_t seed, uint32_t stream, uint32_t path, uint32_t step, float* n0,
+                                              float* n1) {
+    spac_u32x4 r = spac_philox4x32_10(step, path, (uint32_t)(seed >> 32), SPAC_TAG_BROWNIAN, (uint32_t)seed, stream);
+    float u1 = 1.0f - (float)(r.v[0] >> 8) * 5.9604644775390625e-08f; /* (0, 1] */
+    float u2 = (float)(r.v[2] >> 8) * 5.9604644775390625e-08f;        /* [0, 1) */
+    float rad = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1)); /* -2 ln u1 = -2 ln2 log2 u1 */
+    *n0 = rad * __builtin_amdgcn_cosf(u2);
+    *n1 = rad * __builtin_amdgcn_sinf(u2);
+}
+
+__device__ __forceinline__ float rect_distance32(const float* p, float x, float y, float* nx, float* ny, float* ex,
+                                                 float* ey) {
+    const float K = 0.70710678118f;
+    float c0 = 0.5f * (p[0] + p[2]), c1 = 0.5f * (p[1] + p[3]);
+    float h0 = p[2] - c0, h1 = p[3] - c1;
+    float d0 = __builtin_fabsf(x - c0), d1 = __builtin_fabsf(y - c1);
+    float sx = (x > c0) ? 1.0f : -1.0f, sy = (y > c1) ? 1.0f : -1.0f;
+    *ex = x; *ey = y;
+    if (d0 <= h0 && d1 <= h1) {
+        if (d0 - h0 > d1 - h1) { *ex = sx * h0 + c0; *nx = sx; *ny = 0.0f; return d0 - h0; }
+        *ey = sy * h1 + c1; *nx = 0.0f; *ny = sy; return d1 - h1;
+    }
+    if (d0 > h0) {
+        if (d1 > h1) {
+            float e0 = d0 - h0, e1 = d1 - h1;
+            *ex = sx * h0 + c0; *ey = sy * h1 + c1; *nx = sx * K; *ny = sy * K;
+            return __builtin_sqrtf(e0 * e0 + e1 * e1);
+        }
+        *ex = sx * h0 + c0; *nx = sx; *ny = 0.0f; return d0 - h0;
+    }
+    *ey = sy * h1 + c1; *nx = 0.0f; *ny = sy; return d1 - h1;
+}
+
 } // namespace spac
 
 #endif

@@ -215,6 +215,131 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     }
 }
 
+/* Mixed-precision path kernel: FP32 stepping and stopping test (hardware
+ * transcendentals), FP64 weight Y and sums Z, xi, FP64 exit projection.  Same
+ * work queue, lane refill and record layout as the FP64 kernel. */
+template <class Coef32, int SUB>
+__global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
+    using Coef = typename Coef32::F64;
+    const uint32_t lane = threadIdx.x & 63u;
+    bool active = false;
+    int32_t solve = 0;
+    uint32_t path = 0;
+    int32_t nstep = 0;
+    float xf = 0.f, yf = 0.f, hf = 0.f, sqrthf = 0.f, Rf = 0.f, cxf = 0.f, cyf = 0.f;
+    double Yw = 0.0, Z = 0.0, xi = 0.0, h = 0.0, R = 0.0, cx = 0.0, cy = 0.0;
+    uint32_t stream = 0;
+    int64_t rec = 0;
+    int32_t pool_solve = 0;
+    uint32_t pool_next = 0, pool_end = 0;
+    bool drained = false;
+    float domf[4] = {(float)a.dom[0], (float)a.dom[1], (float)a.dom[2], (float)a.dom[3]};
+    const float gobetf = (float)a.gobet;
+
+    while (true) {
+        unsigned long long idle = __ballot(!active);
+        while (idle != 0ull && !drained) {
+            if (pool_next >= pool_end) {
+                int32_t item = 0;
+                if (lane == 0) item = atomicAdd(a.queue, 1);
+                item = __shfl(item, 0);
+                if (item >= a.n_items) { drained = true; break; }
+                if (a.items) {
+                    int4 it = a.items[item];
+                    pool_solve = it.x;
+                    pool_next = (uint32_t)it.y;
+                    pool_end = (uint32_t)it.z;
+                } else {
+                    pool_solve = item / a.chunks_per_solve;
+                    uint32_t c = (uint32_t)(item % a.chunks_per_solve);
+                    pool_next = c * a.chunk;
+                    pool_end = min(pool_next + a.chunk, a.paths_uniform);
+                }
+            }
+            uint32_t avail = pool_end - pool_next;
+            uint32_t nidle = (uint32_t)__popcll(idle);
+            uint32_t rank = lane_rank(idle);
+            if (!active && rank < avail) {
+                solve = pool_solve;
+                path = pool_next + rank;
+                h = a.h ? a.h[solve] : a.h_uniform;
+                hf = (float)h;
+                sqrthf = __builtin_sqrtf(hf);
+                stream = a.stream ? a.stream[solve] : (uint32_t)solve;
+                rec = a.rec_off ? a.rec_off[solve] : (int64_t)solve * (int64_t)a.paths_uniform;
+                if (SUB == SUB_CIRCLE) {
+                    if (a.circle) {
+                        R = a.circle[3 * solve]; cx = a.circle[3 * solve + 1]; cy = a.circle[3 * solve + 2];
+                    } else {
+                        R = a.circle_uniform[0]; cx = a.circle_uniform[1]; cy = a.circle_uniform[2];
+                    }
+                    Rf = (float)R; cxf = (float)cx; cyf = (float)cy;
+                }
+                xf = (float)a.x0[solve];
+                yf = (float)a.y0[solve];
+                Yw = 1.0; Z = 0.0; xi = 0.0;
+                nstep = 0;
+                active = true;
+            }
+            uint32_t take = min(avail, nidle);
+            pool_next += take;
+            idle = __ballot(!active);
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            StepCoef32 cf = Coef32::step(xf, yf);
+            float n0, n1;
+            normal_pair32(a.seed, stream, path, (uint32_t)nstep, &n0, &n1);
+            const float dWx = sqrthf * n0, dWy = sqrthf * n1;
+            const float Ywf = (float)Yw;
+            Z += (double)((cf.f * Ywf) * hf);
+            xi += (double)(Ywf * (cf.Fx * dWx + cf.Fy * dWy));
+            if (Coef32::kHasKilling) Yw += (double)(cf.c * hf) * Yw;
+            if (Coef32::kHasDrift) {
+                xf += cf.bx * hf + cf.s * dWx;
+                yf += cf.by * hf + cf.s * dWy;
+            } else {
+                xf += cf.s * dWx;
+                yf += cf.s * dWy;
+            }
+            nstep++;
+            float nx, ny, ex, ey;
+            float d = rect_distance32(domf, xf, yf, &nx, &ny, &ex, &ey);
+            bool use_dom = true;
+            if (SUB == SUB_CIRCLE) {
+                float dx = xf - cxf, dy = yf - cyf;
+                float rr = __builtin_sqrtf(dx * dx + dy * dy);
+                rr = (5E-4f < rr) ? rr : 5E-4f;
+                float ir = __builtin_amdgcn_rcpf(rr);
+                float ds = rr - Rf;
+                use_dom = d >= ds;
+                if (!use_dom) { d = ds; nx = dx * ir; ny = dy * ir; ex = nx * Rf + cxf; ey = ny * Rf + cyf; }
+            }
+            float sg = (d < 0.0f) ? Coef32::sigma(xf, yf) : Coef32::sigma(ex, ey);
+            float r0 = nx * sg, r1 = ny * sg;
+            bool inside = d < gobetf * __builtin_sqrtf(r0 * r0 + r1 * r1) * sqrthf;
+            int8_t out = inside ? (int8_t)SPAC_OUT_INSIDE : (use_dom ? (int8_t)SPAC_OUT_DOMAIN : (int8_t)SPAC_OUT_SUBDOMAIN);
+            if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
+            if (out != SPAC_OUT_INSIDE) {
+                /* exit projection in FP64 onto the boundary the FP32 test chose */
+                BoundaryHit hit = (out == SPAC_OUT_SUBDOMAIN) ? circle_distance(R, cx, cy, (double)xf, (double)yf)
+                                                              : rectangle_distance(a.dom, (double)xf, (double)yf);
+                int64_t g = rec + (int64_t)path;
+                a.xt[g] = hit.ex;
+                a.yt[g] = hit.ey;
+                a.Y[g] = Yw;
+                a.Z[g] = Z;
+                a.xi[g] = xi;
+                a.tau[g] = (double)nstep * h;
+                a.gval[g] = (out == SPAC_OUT_SUBDOMAIN) ? Coef::u(hit.ex, hit.ey) : Coef::g(hit.ex, hit.ey);
+                a.steps[g] = nstep;
+                a.outcome[g] = out;
+                active = false;
+            }
+        }
+    }
+}
+
 /* ---------------------------------------------------------- reduction -- */
 struct ReduceArgs {
     int32_t n_solves;
@@ -341,7 +466,20 @@ static void launch_paths_coef(int sub, dim3 grid, dim3 block, hipStream_t s, con
         hipLaunchKernelGGL((fk_path_kernel_fp64<Coef, SUB_NONE>), grid, block, 0, s, a);
 }
 
-static void launch_paths(int coef, int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
+template <class Coef32>
+static void launch_paths_coef32(int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
+    if (sub == SUB_CIRCLE)
+        hipLaunchKernelGGL((fk_path_kernel_fp32<Coef32, SUB_CIRCLE>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((fk_path_kernel_fp32<Coef32, SUB_NONE>), grid, block, 0, s, a);
+}
+
+static void launch_paths(int coef, int precision, int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
+    if (precision == SPAC_FP32) {
+        if (coef == SPAC_COEF_FRANKE_POISSON) launch_paths_coef32<CoefFranke32>(sub, grid, block, s, a);
+        else launch_paths_coef32<CoefVarFranke32>(sub, grid, block, s, a);
+        return;
+    }
     if (coef == SPAC_COEF_FRANKE_POISSON) launch_paths_coef<CoefFranke>(sub, grid, block, s, a);
     else launch_paths_coef<CoefVarFranke>(sub, grid, block, s, a);
 }
@@ -489,7 +627,7 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
 
     SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
-    launch_paths(desc.coef_kind, sub, dim3(blocks), dim3(block), stream, a);
+    launch_paths(desc.coef_kind, desc.precision, sub, dim3(blocks), dim3(block), stream, a);
     SPAC_HIP_CHECK(hipGetLastError());
     SPAC_HIP_CHECK(hipEventRecord(ev[1], stream));
     if (d_stats) {
