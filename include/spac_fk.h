@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SPAC_ABI_VERSION 1
+#define SPAC_ABI_VERSION 2
 
 typedef int32_t spac_status;
 enum {
@@ -133,6 +133,8 @@ typedef struct {
     int64_t n_max_steps;  /* paths stopped by the cap */
     int32_t status;
     int32_t n_domain_exits;
+    int32_t cv_fallback;  /* 1: Var(score + xi) > max_ratio Var(score), xi left out of B */
+    int32_t reserved;
 } spac_point_stats;
 
 /* Interfaces (circles) and their nodes, the device copy of Interfaz
@@ -234,6 +236,30 @@ typedef struct {
     int32_t reserved;
 } spac_launch_config;
 spac_status spac_fk_set_launch(spac_problem* p, const spac_launch_config* cfg);
+
+/* Control variate F in xi += Y F(X).dW (Solvers/FeynmanKacSolver.cpp:13). */
+enum {
+    SPAC_CV_ANALYTIC = 0, /* the coefficient set's F plugin (reference behaviour; default) */
+    SPAC_CV_NONE = 1,     /* F = 0: plain Monte Carlo, xi = 0 */
+    SPAC_CV_TABLE = 2     /* F = -sigma(X) grad u_hat(X), grad u_hat bilinear in a T x T table */
+};
+
+/* A tabulated u_hat (the device form of a TablaDeValores + Interpolador,
+ * BVP/Interpolador.hpp:15-19): table[(j T + i) 4 + c] holds, at
+ * x = xmin + i (xmax - xmin) / (T - 1) and y = ymin + j (ymax - ymin) / (T - 1),
+ * c = 0: u_hat, 1: d u_hat / dx, 2: d u_hat / dy, 3: unused.  Points outside
+ * the table take the value at the nearest edge.  In table mode u_hat(X_tau)
+ * replaces u(X_tau) in the subdomain-exit score of the statistics. */
+typedef struct {
+    int32_t mode;            /* SPAC_CV_* */
+    int32_t fallback;        /* 1: per solve, drop xi where Var(score + xi) > max_ratio Var(score) */
+    double max_ratio;        /* 1.0 = fall back wherever the control variate adds variance */
+    int32_t table_size;      /* T >= 2 (table mode) */
+    int32_t reserved;
+    double table_domain[4];  /* xmin, ymin, xmax, ymax */
+    const float* table;      /* host, T * T * 4 floats; copied */
+} spac_cv_config;
+spac_status spac_fk_set_control_variate(spac_problem* p, const spac_cv_config* cfg);
 
 #ifdef __cplusplus
 }

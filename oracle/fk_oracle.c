@@ -40,6 +40,7 @@
 #include "../include/spac_fk.h"
 #include "../spac_amd/csrc/spac_math.h"
 #include "../spac_amd/csrc/spac_rbf.h"
+#include "../spac_amd/csrc/spac_cv.h"
 
 typedef struct { double x, y; } vec2;
 typedef struct { double a00, a01, a10, a11; } mat2;
@@ -286,6 +287,37 @@ static int dentro(const bvp_t* bvp, vec2 X, double sqrth, double gobet, vec2* Np
     return status;
 }
 
+/* ------------------------------------------------- control variate --
+ * The F of xi += Y F(X).dW (Solvers/FeynmanKacSolver.cpp:13): the plugin
+ * (reference behaviour), none (plain Monte Carlo), or -sigma grad u_hat read
+ * from a table with spac_cv.h (scalar diffusion sigma = s I).  In table mode
+ * u_hat also replaces u in the subdomain-exit score, and the reduction may
+ * drop xi from B where Var(score + xi) > max_ratio Var(score).
+ * Pinned by: tests/test_oracle.py (table of the exact Franke u and gradient
+ * reproduces the plugin's xi to interpolation accuracy; a zero table equals
+ * plain Monte Carlo; the fallback rule on constructed records). */
+static struct {
+    int mode, fallback;
+    double max_ratio;
+    spac_cv_table tab;
+} g_cv = {SPAC_CV_ANALYTIC, 0, 1.0, {NULL, 0, 0.0, 0.0, 0.0, 0.0}};
+
+int oracle_set_cv(int mode, int fallback, double max_ratio, const float* table, int t, const double* dom) {
+    if (mode == SPAC_CV_TABLE && (!table || t < 2 || !(dom[2] > dom[0]) || !(dom[3] > dom[1]))) return SPAC_E_INVALID_ARG;
+    g_cv.mode = mode;
+    g_cv.fallback = fallback;
+    g_cv.max_ratio = max_ratio;
+    if (mode == SPAC_CV_TABLE) {
+        g_cv.tab.table = table;
+        g_cv.tab.t = t;
+        g_cv.tab.lox = dom[0];
+        g_cv.tab.loy = dom[1];
+        g_cv.tab.ix = (double)(t - 1) / (dom[2] - dom[0]);
+        g_cv.tab.iy = (double)(t - 1) / (dom[3] - dom[1]);
+    }
+    return SPAC_OK;
+}
+
 /* --------------------------------------------------- Paso_Absorbente --
  * Solvers/FeynmanKacSolver.cpp:7-18, update order kept: Z, xi, Y, X, t.
  * ji_t is 0 for absorbing boundaries (:101-103). */
@@ -300,8 +332,17 @@ static void paso_absorbente(const bvp_t* bvp, path_state* st, double h, double s
     vec2 dW = {sqrth * n0, sqrth * n1};
     double Y = st->Y;
     st->Z = st->Z + (bvp->f(st->X) * Y * h + bvp->psi(st->X, st->N) * Y * ji_t);
-    vec2 F = bvp->F(st->X);
-    st->xi = st->xi + Y * (F.x * dW.x + F.y * dW.y);
+    if (g_cv.mode == SPAC_CV_ANALYTIC) {
+        vec2 F = bvp->F(st->X);
+        st->xi = st->xi + Y * (F.x * dW.x + F.y * dW.y);
+    } else if (g_cv.mode == SPAC_CV_TABLE) {
+        int32_t k00;
+        double ax, ay;
+        spac_cv_cell(&g_cv.tab, st->X.x, st->X.y, &k00, &ax, &ay);
+        double Fx = -(sig.a00 * spac_cv_eval(&g_cv.tab, k00, ax, ay, 1));
+        double Fy = -(sig.a00 * spac_cv_eval(&g_cv.tab, k00, ax, ay, 2));
+        st->xi = st->xi + Y * (Fx * dW.x + Fy * dW.y);
+    }
     vec2 mu = bvp->mu(st->X);
     st->Y = Y + (bvp->c(st->X) * Y * h + Y * (mu.x * dW.x + mu.y * dW.y) +
                  bvp->varphi(st->X, st->N) * Y * ji_t);
@@ -344,7 +385,18 @@ static void simulate_one(const bvp_t* bvp, vec2 X0, double h, double sqrth, doub
     if (Zo) *Zo = st.Z;
     if (xio) *xio = st.xi;
     if (tauo) *tauo = st.t;
-    if (gvalo) *gvalo = (estado == SPAC_OUT_SUBDOMAIN) ? bvp->u(Npro) : bvp->g(Npro);
+    if (gvalo) {
+        if (estado != SPAC_OUT_SUBDOMAIN) {
+            *gvalo = bvp->g(Npro);
+        } else if (g_cv.mode == SPAC_CV_TABLE) {
+            int32_t k00;
+            double ax, ay;
+            spac_cv_cell(&g_cv.tab, Npro.x, Npro.y, &k00, &ax, &ay);
+            *gvalo = spac_cv_eval(&g_cv.tab, k00, ax, ay, 0);
+        } else {
+            *gvalo = bvp->u(Npro);
+        }
+    }
     if (stepso) *stepso = steps;
     if (outo) *outo = (int8_t)estado;
 }
@@ -462,6 +514,16 @@ static void reduce_one(int node_mode, int64_t np, const double* Y, const double*
     st->n_max_steps = nmax;
     st->n_domain_exits = (int32_t)ndom;
     st->status = has_nan ? SPAC_E_NAN : (nmax ? SPAC_E_MAX_STEPS : SPAC_OK);
+    st->cv_fallback = 0;
+    st->reserved = 0;
+    if (g_cv.fallback && g_cv.mode == SPAC_CV_TABLE) {
+        double var = st->phiphi - st->phi * st->phi;
+        double var_vr = st->phiphi_VR - st->phi_VR * st->phi_VR;
+        if (var_vr > g_cv.max_ratio * var) {
+            st->cv_fallback = 1;
+            if (node_mode) st->B = (s[S_B] - s[S_XI]) / N;
+        }
+    }
 }
 
 int oracle_reduce(int node_mode, const spac_solve_batch* batch, const spac_path_records* rec,

@@ -39,6 +39,10 @@ SPAC_OUT_REFLECTED = 3
 SPAC_OUT_MAX_STEPS = 4
 SPAC_RBF_IMQ_GLOBAL = 0
 SPAC_RBF_GAUSS_STENCIL = 1
+SPAC_CV_ANALYTIC = 0
+SPAC_CV_NONE = 1
+SPAC_CV_TABLE = 2
+MLP_N_PARAMS = 8707
 
 GOBET_CONSTANT = -0.5826  # Solvers/FeynmanKacSolver.cpp:268,283
 
@@ -107,6 +111,8 @@ class PointStats(C.Structure):
         ("n_max_steps", C.c_int64),
         ("status", C.c_int32),
         ("n_domain_exits", C.c_int32),
+        ("cv_fallback", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -174,6 +180,41 @@ class LaunchConfig(C.Structure):
     ]
 
 
+class CvConfig(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("fallback", C.c_int32),
+        ("max_ratio", C.c_double),
+        ("table_size", C.c_int32),
+        ("reserved", C.c_int32),
+        ("table_domain", C.c_double * 4),
+        ("table", C.POINTER(C.c_float)),
+    ]
+
+
+class MlpDesc(C.Structure):
+    _fields_ = [("domain", C.c_double * 4), ("seed", C.c_uint64)]
+
+
+class MlpTrainCfg(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32),
+        ("reserved", C.c_int32),
+        ("lr", C.c_double),
+        ("lr_final", C.c_double),
+        ("grad_weight", C.c_double),
+    ]
+
+
+class MlpTrainReport(C.Structure):
+    _fields_ = [
+        ("final_loss", C.c_double),
+        ("train_ms", C.c_double),
+        ("steps", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
 _LIB = None
 
 
@@ -209,6 +250,23 @@ def load(path: Path | None = None) -> C.CDLL:
                                           C.POINTER(RowBlock)]
     lib.spac_fk_last_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
     lib.spac_fk_set_launch.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
+    lib.spac_fk_set_control_variate.argtypes = [C.c_void_p, C.POINTER(CvConfig)]
+    fp = C.POINTER(C.c_float)
+    lib.spac_mlp_create.argtypes = [C.POINTER(MlpDesc), C.POINTER(C.c_void_p)]
+    lib.spac_mlp_destroy.argtypes = [C.c_void_p]
+    lib.spac_mlp_get_params.argtypes = [C.c_void_p, fp]
+    lib.spac_mlp_set_params.argtypes = [C.c_void_p, fp]
+    lib.spac_mlp_loss_grad.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32, C.c_double, fp, dp]
+    lib.spac_mlp_train.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32, C.POINTER(MlpTrainCfg),
+                                   C.POINTER(MlpTrainReport)]
+    lib.spac_mlp_forward_fp32.argtypes = [C.c_void_p, dp, dp, C.c_int32, fp]
+    lib.spac_mlp_infer.argtypes = [C.c_void_p, dp, dp, C.c_int32, fp]
+    lib.spac_mlp_build_table.argtypes = [C.c_void_p, C.c_int32, dp]
+    lib.spac_mlp_read_table.argtypes = [C.c_void_p, fp]
+    lib.spac_mlp_attach.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double]
+    for header in sorted(HEADER.parent.glob("*.h")):
+        for name in declared_symbols(header):
+            getattr(lib, name).restype = C.c_int32 if name != "spac_last_error" else C.c_char_p
     for name in declared_symbols():
         getattr(lib, name).restype = C.c_int32 if name != "spac_last_error" else C.c_char_p
     if path is None:

@@ -56,6 +56,13 @@ struct Problem {
     DeviceBuffer sx, sy, sh, sstream, spaths, soff, scircle, sitems, queue, stats;
     /* interface tables and row outputs */
     DeviceBuffer ifc, ifnode, ifipsi, ifoff, rows_iface, rows_goff, rows_G, rows_part, rows_col;
+    /* control variate (SPAC_CV_*) and its table (T x T x float4) */
+    DeviceBuffer cv_table;
+    int32_t cv_mode = SPAC_CV_ANALYTIC;
+    int32_t cv_fallback = 0;
+    double cv_max_ratio = 1.0;
+    int32_t cv_t = 0;
+    double cv_dom[4] = {0.0, 0.0, 1.0, 1.0};
 
     spac_status ensure_records(int64_t total_paths);
     spac_status upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths, const double* h,
@@ -65,10 +72,14 @@ struct Problem {
         DeviceBuffer* all[] = {&rec_xt, &rec_yt, &rec_Y, &rec_Z, &rec_xi, &rec_tau, &rec_gval, &rec_steps,
                                &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
                                &queue, &stats, &ifc, &ifnode, &ifipsi, &ifoff, &rows_iface, &rows_goff,
-                               &rows_G, &rows_part, &rows_col};
+                               &rows_G, &rows_part, &rows_col, &cv_table};
         for (DeviceBuffer* b : all) b->release();
     }
 };
+
+/* Install a control variate; the table (cfg->table ignored) is copied from
+ * src, a host pointer or (src_on_device) a device pointer. */
+spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* src, bool src_on_device);
 
 } // namespace spac
 

@@ -54,6 +54,7 @@ spac_status set_error(spac_status code, const char* fmt, ...) {
 
 /* ------------------------------------------------------ kernel args -- */
 enum { SUB_NONE = 0, SUB_CIRCLE = 2 };
+enum { CV_ANALYTIC = SPAC_CV_ANALYTIC, CV_NONE = SPAC_CV_NONE, CV_TABLE = SPAC_CV_TABLE };
 
 struct PathArgs {
     double dom[4];
@@ -86,6 +87,9 @@ struct PathArgs {
     double* gval;
     int32_t* steps;
     int8_t* outcome;
+    /* tabulated control variate (CV_TABLE): FP64 and FP32 views of one table */
+    spac_cv_table cv;
+    CvTable32 cv32;
 };
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
@@ -96,7 +100,20 @@ __global__ void zero_counter_kernel(int32_t* q) {
     if (threadIdx.x == 0) q[0] = 0;
 }
 
-template <class Coef, int SUB>
+/* u at a subdomain exit for the statistics' score (Solvers/FeynmanKacSolver.cpp:254):
+ * the coefficient set's u, or u_hat from the control-variate table. */
+template <class Coef, int CV>
+__device__ __forceinline__ double exit_u(const PathArgs& a, double x, double y) {
+    if (CV == CV_TABLE) {
+        int32_t k00;
+        double ax, ay;
+        spac_cv_cell(&a.cv, x, y, &k00, &ax, &ay);
+        return spac_cv_eval(&a.cv, k00, ax, ay, 0);
+    }
+    return Coef::u(x, y);
+}
+
+template <class Coef, int SUB, int CV>
 __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     bool active = false;
@@ -168,7 +185,17 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             spac_normal_pair(a.seed, stream, path, (uint32_t)nstep, &n0, &n1);
             double dWx = sqrth * n0, dWy = sqrth * n1;
             Z = Z + (cf.f * Yw) * h;
-            xi = xi + Yw * (cf.Fx * dWx + cf.Fy * dWy);
+            if (CV == CV_ANALYTIC) {
+                xi = xi + Yw * (cf.Fx * dWx + cf.Fy * dWy);
+            } else if (CV == CV_TABLE) {
+                /* F = -sigma grad u_hat with sigma = s I */
+                int32_t k00;
+                double ax, ay;
+                spac_cv_cell(&a.cv, X, Yc, &k00, &ax, &ay);
+                double Fx = -(cf.s * spac_cv_eval(&a.cv, k00, ax, ay, 1));
+                double Fy = -(cf.s * spac_cv_eval(&a.cv, k00, ax, ay, 2));
+                xi = xi + Yw * (Fx * dWx + Fy * dWy);
+            }
             if (Coef::kHasKilling) Yw = Yw + (cf.c * Yw) * h;
             if (Coef::kHasDrift) {
                 X = X + (cf.bx * h + cf.s * dWx);
@@ -206,7 +233,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                 a.Z[g] = Z;
                 a.xi[g] = xi;
                 a.tau[g] = t;
-                a.gval[g] = (out == SPAC_OUT_SUBDOMAIN) ? Coef::u(hit.ex, hit.ey) : Coef::g(hit.ex, hit.ey);
+                a.gval[g] = (out == SPAC_OUT_SUBDOMAIN) ? exit_u<Coef, CV>(a, hit.ex, hit.ey) : Coef::g(hit.ex, hit.ey);
                 a.steps[g] = nstep;
                 a.outcome[g] = out;
                 active = false;
@@ -218,7 +245,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
 /* Mixed-precision path kernel: FP32 stepping and stopping test (hardware
  * transcendentals), FP64 weight Y and sums Z, xi, FP64 exit projection.  Same
  * work queue, lane refill and record layout as the FP64 kernel. */
-template <class Coef32, int SUB>
+template <class Coef32, int SUB, int CV>
 __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
     using Coef = typename Coef32::F64;
     const uint32_t lane = threadIdx.x & 63u;
@@ -293,7 +320,13 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
             const float dWx = sqrthf * n0, dWy = sqrthf * n1;
             const float Ywf = (float)Yw;
             Z += (double)((cf.f * Ywf) * hf);
-            xi += (double)(Ywf * (cf.Fx * dWx + cf.Fy * dWy));
+            if (CV == CV_ANALYTIC) {
+                xi += (double)(Ywf * (cf.Fx * dWx + cf.Fy * dWy));
+            } else if (CV == CV_TABLE) {
+                float gx, gy;
+                cv_grad32(a.cv32, xf, yf, &gx, &gy);
+                xi += (double)(Ywf * (-cf.s) * (gx * dWx + gy * dWy));
+            }
             if (Coef32::kHasKilling) Yw += (double)(cf.c * hf) * Yw;
             if (Coef32::kHasDrift) {
                 xf += cf.bx * hf + cf.s * dWx;
@@ -331,7 +364,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
                 a.Z[g] = Z;
                 a.xi[g] = xi;
                 a.tau[g] = (double)nstep * h;
-                a.gval[g] = (out == SPAC_OUT_SUBDOMAIN) ? Coef::u(hit.ex, hit.ey) : Coef::g(hit.ex, hit.ey);
+                a.gval[g] = (out == SPAC_OUT_SUBDOMAIN) ? exit_u<Coef, CV>(a, hit.ex, hit.ey) : Coef::g(hit.ex, hit.ey);
                 a.steps[g] = nstep;
                 a.outcome[g] = out;
                 active = false;
@@ -355,6 +388,8 @@ struct ReduceArgs {
     const int32_t* steps;
     const int8_t* outcome;
     spac_point_stats* stats;
+    int32_t cv_fallback;
+    double cv_max_ratio;
 };
 
 #define NSUMS 10
@@ -453,35 +488,49 @@ __global__ void __launch_bounds__(64) fk_reduce_kernel(ReduceArgs a) {
         st.n_max_steps = my_max;
         st.n_domain_exits = (int32_t)my_dom;
         st.status = has_nan ? SPAC_E_NAN : (my_max ? SPAC_E_MAX_STEPS : SPAC_OK);
+        st.cv_fallback = 0;
+        st.reserved = 0;
+        if (a.cv_fallback) {
+            /* keep the control variate only where it does not add variance */
+            double var = st.phiphi - st.phi * st.phi;
+            double var_vr = st.phiphi_VR - st.phi_VR * st.phi_VR;
+            if (var_vr > a.cv_max_ratio * var) {
+                st.cv_fallback = 1;
+                if (a.node_mode) st.B = (s[S_B] - s[S_XI]) / N;
+            }
+        }
         a.stats[solve] = st;
     }
 }
 
 /* ------------------------------------------------------ host side -- */
-template <class Coef>
-static void launch_paths_coef(int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
-    if (sub == SUB_CIRCLE)
-        hipLaunchKernelGGL((fk_path_kernel_fp64<Coef, SUB_CIRCLE>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((fk_path_kernel_fp64<Coef, SUB_NONE>), grid, block, 0, s, a);
-}
-
-template <class Coef32>
-static void launch_paths_coef32(int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
-    if (sub == SUB_CIRCLE)
-        hipLaunchKernelGGL((fk_path_kernel_fp32<Coef32, SUB_CIRCLE>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((fk_path_kernel_fp32<Coef32, SUB_NONE>), grid, block, 0, s, a);
-}
-
-static void launch_paths(int coef, int precision, int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
+template <int CV>
+static void launch_paths_cv(int coef, int precision, int sub, dim3 grid, dim3 block, hipStream_t s,
+                            const PathArgs& a) {
     if (precision == SPAC_FP32) {
-        if (coef == SPAC_COEF_FRANKE_POISSON) launch_paths_coef32<CoefFranke32>(sub, grid, block, s, a);
-        else launch_paths_coef32<CoefVarFranke32>(sub, grid, block, s, a);
+        if (coef == SPAC_COEF_FRANKE_POISSON) {
+            if (sub == SUB_CIRCLE) hipLaunchKernelGGL((fk_path_kernel_fp32<CoefFranke32, SUB_CIRCLE, CV>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((fk_path_kernel_fp32<CoefFranke32, SUB_NONE, CV>), grid, block, 0, s, a);
+        } else {
+            if (sub == SUB_CIRCLE) hipLaunchKernelGGL((fk_path_kernel_fp32<CoefVarFranke32, SUB_CIRCLE, CV>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((fk_path_kernel_fp32<CoefVarFranke32, SUB_NONE, CV>), grid, block, 0, s, a);
+        }
         return;
     }
-    if (coef == SPAC_COEF_FRANKE_POISSON) launch_paths_coef<CoefFranke>(sub, grid, block, s, a);
-    else launch_paths_coef<CoefVarFranke>(sub, grid, block, s, a);
+    if (coef == SPAC_COEF_FRANKE_POISSON) {
+        if (sub == SUB_CIRCLE) hipLaunchKernelGGL((fk_path_kernel_fp64<CoefFranke, SUB_CIRCLE, CV>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fk_path_kernel_fp64<CoefFranke, SUB_NONE, CV>), grid, block, 0, s, a);
+    } else {
+        if (sub == SUB_CIRCLE) hipLaunchKernelGGL((fk_path_kernel_fp64<CoefVarFranke, SUB_CIRCLE, CV>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fk_path_kernel_fp64<CoefVarFranke, SUB_NONE, CV>), grid, block, 0, s, a);
+    }
+}
+
+static void launch_paths(int coef, int precision, int sub, int cv, dim3 grid, dim3 block, hipStream_t s,
+                         const PathArgs& a) {
+    if (cv == CV_NONE) launch_paths_cv<CV_NONE>(coef, precision, sub, grid, block, s, a);
+    else if (cv == CV_TABLE) launch_paths_cv<CV_TABLE>(coef, precision, sub, grid, block, s, a);
+    else launch_paths_cv<CV_ANALYTIC>(coef, precision, sub, grid, block, s, a);
 }
 
 spac_status DeviceBuffer::ensure(size_t bytes) {
@@ -617,6 +666,21 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
     a.Y = (double*)rec_Y.ptr; a.Z = (double*)rec_Z.ptr; a.xi = (double*)rec_xi.ptr;
     a.tau = (double*)rec_tau.ptr; a.gval = (double*)rec_gval.ptr;
     a.steps = (int32_t*)rec_steps.ptr; a.outcome = (int8_t*)rec_outcome.ptr;
+    if (cv_mode == SPAC_CV_TABLE) {
+        a.cv.table = (const float*)cv_table.ptr;
+        a.cv.t = cv_t;
+        a.cv.lox = cv_dom[0];
+        a.cv.loy = cv_dom[1];
+        a.cv.ix = (double)(cv_t - 1) / (cv_dom[2] - cv_dom[0]);
+        a.cv.iy = (double)(cv_t - 1) / (cv_dom[3] - cv_dom[1]);
+        a.cv32.table = (const float4*)cv_table.ptr;
+        a.cv32.t = cv_t;
+        a.cv32.lox = (float)a.cv.lox;
+        a.cv32.loy = (float)a.cv.loy;
+        a.cv32.ix = (float)a.cv.ix;
+        a.cv32.iy = (float)a.cv.iy;
+        a.cv32.top = (float)(cv_t - 1);
+    }
 
     int sub = (ss.circle || desc.subdomain_kind == SPAC_BOUNDARY_CIRCLE) ? SUB_CIRCLE : SUB_NONE;
     int block = launch.block_size;
@@ -627,7 +691,7 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
 
     SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
-    launch_paths(desc.coef_kind, desc.precision, sub, dim3(blocks), dim3(block), stream, a);
+    launch_paths(desc.coef_kind, desc.precision, sub, cv_mode, dim3(blocks), dim3(block), stream, a);
     SPAC_HIP_CHECK(hipGetLastError());
     SPAC_HIP_CHECK(hipEventRecord(ev[1], stream));
     if (d_stats) {
@@ -641,6 +705,8 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
         r.Y = a.Y; r.Z = a.Z; r.xi = a.xi; r.tau = a.tau; r.gval = a.gval;
         r.steps = a.steps; r.outcome = a.outcome;
         r.stats = d_stats;
+        r.cv_fallback = (cv_mode == SPAC_CV_TABLE) ? cv_fallback : 0;
+        r.cv_max_ratio = cv_max_ratio;
         hipLaunchKernelGGL(fk_reduce_kernel, dim3(ss.n), dim3(64), 0, stream, r);
         SPAC_HIP_CHECK(hipGetLastError());
     }
@@ -655,6 +721,35 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
     last.path_kernel_launches = 1;
     last.grid_blocks = blocks;
     last.path_steps = 0;
+    return SPAC_OK;
+}
+
+spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* src, bool src_on_device) {
+    if (cfg->mode != SPAC_CV_ANALYTIC && cfg->mode != SPAC_CV_NONE && cfg->mode != SPAC_CV_TABLE)
+        return set_error(SPAC_E_INVALID_ARG, "unknown control-variate mode %d", cfg->mode);
+    if (cfg->fallback && !(cfg->max_ratio > 0.0))
+        return set_error(SPAC_E_INVALID_ARG, "max_ratio must be positive");
+    if (cfg->mode == SPAC_CV_TABLE) {
+        if (!src) return set_error(SPAC_E_INVALID_ARG, "table mode needs a table");
+        if (cfg->table_size < 2 || cfg->table_size > 8192)
+            return set_error(SPAC_E_INVALID_ARG, "table_size must be in [2, 8192]");
+        const double* d = cfg->table_domain;
+        if (!(d[2] > d[0]) || !(d[3] > d[1]))
+            return set_error(SPAC_E_INVALID_ARG, "table domain must have xmax > xmin and ymax > ymin");
+        if (p->desc.coef_kind != SPAC_COEF_FRANKE_POISSON && p->desc.coef_kind != SPAC_COEF_VARCOEF_FRANKE)
+            return set_error(SPAC_E_INVALID_ARG, "table mode needs a scalar-diffusion coefficient set");
+        size_t bytes = (size_t)cfg->table_size * (size_t)cfg->table_size * 16;
+        spac_status rc;
+        if ((rc = p->cv_table.ensure(bytes))) return rc;
+        SPAC_HIP_CHECK(hipMemcpyAsync(p->cv_table.ptr, src, bytes,
+                                      src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
+        SPAC_HIP_CHECK(hipStreamSynchronize(p->stream));
+        p->cv_t = cfg->table_size;
+        for (int i = 0; i < 4; ++i) p->cv_dom[i] = d[i];
+    }
+    p->cv_mode = cfg->mode;
+    p->cv_fallback = cfg->fallback ? 1 : 0;
+    p->cv_max_ratio = cfg->max_ratio > 0.0 ? cfg->max_ratio : 1.0;
     return SPAC_OK;
 }
 
@@ -726,6 +821,11 @@ spac_status spac_fk_set_launch(spac_problem* hp, const spac_launch_config* cfg) 
         p->launch.chunk_paths = cfg->chunk_paths;
     }
     return SPAC_OK;
+}
+
+spac_status spac_fk_set_control_variate(spac_problem* hp, const spac_cv_config* cfg) {
+    if (!hp || !cfg) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    return problem_set_cv(reinterpret_cast<Problem*>(hp), cfg, cfg->table, false);
 }
 
 spac_status spac_fk_last_timing(spac_problem* hp, spac_timing* out) {

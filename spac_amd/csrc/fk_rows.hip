@@ -128,54 +128,39 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
             s_s[tid] = s;
         }
         __syncthreads();
-        /* paths in order; LDS reads batched 4 paths at a time to hide their latency */
-        int p = 0;
-        for (; p + 4 <= cnt; p += 4) {
-            const int4 s4 = *reinterpret_cast<const int4*>(&s_s[p]);
-            if ((s4.x & s4.y & s4.z & s4.w) < 0) continue; /* none of the 4 is a circle exit */
-            const int sv[4] = {s4.x, s4.y, s4.z, s4.w};
-            double yv[4], tv[4];
+        /* Paths in order, 4 at a time, without branches: a path that does not
+         * reach column j adds +0.0.  TwoSum with +0.0 leaves s unchanged and can
+         * only turn a -0.0 error term into +0.0, which never changes s + c, so the
+         * column values equal those of skipping the path (the CPU restatement). */
+        if ((tid & ~63) < M) {
+            for (int p = 0; p < cnt; p += 4) {
+                const int4 s4 = *reinterpret_cast<const int4*>(&s_s[p]);
+                int sv[4] = {s4.x, s4.y, s4.z, s4.w};
+                double yv[4], tv[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) { yv[k] = s_Y[p + k]; tv[k] = s_t[p + k]; }
-#pragma unroll
-            for (int q = 0; q < kMaxColsPerThread; ++q) {
-                const int j = tid + q * kRowsBlock;
-                if (j >= M) break;
-                if (stencil) {
-                    double wv[4];
-                    bool ok[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        int rel = j - sv[k];
-                        if (code == 1 && rel < 0) rel += M;
-                        ok[k] = sv[k] >= 0 && rel >= 0 && rel < S;
-                        wv[k] = ok[k] ? s_w[(p + k) * kWStride + rel] : 0.0;
-                    }
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        if (ok[k]) acc[q] = spac_ksum_add(acc[q], yv[k] * wv[k]);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        if (sv[k] >= 0) acc[q] = spac_ksum_add(acc[q], yv[k] * spac_imq(tv[k], s_th[j], c2));
+                for (int k = 0; k < 4; ++k) {
+                    if (p + k >= cnt) sv[k] = -1;
+                    yv[k] = s_Y[p + k];
+                    tv[k] = s_t[p + k];
                 }
-            }
-        }
-        for (; p < cnt; ++p) {
-            const int s = s_s[p];
-            if (s < 0) continue;
-            const double Yp = s_Y[p];
 #pragma unroll
-            for (int q = 0; q < kMaxColsPerThread; ++q) {
-                const int j = tid + q * kRowsBlock;
-                if (j >= M) break;
-                if (stencil) {
-                    int rel = j - s;
-                    if (code == 1 && rel < 0) rel += M;
-                    if (rel < 0 || rel >= S) continue;
-                    acc[q] = spac_ksum_add(acc[q], Yp * s_w[p * kWStride + rel]);
-                } else {
-                    acc[q] = spac_ksum_add(acc[q], Yp * spac_imq(s_t[p], s_th[j], c2));
+                for (int q = 0; q < kMaxColsPerThread; ++q) {
+                    const int j = tid + q * kRowsBlock;
+                    if (q * kRowsBlock >= M) break; /* uniform */
+                    if (stencil) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            int rel = j - sv[k];
+                            if (code == 1 && rel < 0) rel += M;
+                            const bool ok = sv[k] >= 0 && j < M && rel >= 0 && rel < S;
+                            const double w = s_w[(p + k) * kWStride + (ok ? rel : 0)];
+                            acc[q] = spac_ksum_add(acc[q], ok ? yv[k] * w : 0.0);
+                        }
+                    } else if (j < M) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (sv[k] >= 0) acc[q] = spac_ksum_add(acc[q], yv[k] * spac_imq(tv[k], s_th[j], c2));
+                    }
                 }
             }
         }

@@ -34,7 +34,7 @@ RECORD_FIELDS = (("x_tau", np.float64), ("y_tau", np.float64), ("Y", np.float64)
                  ("xi", np.float64), ("tau", np.float64), ("gval", np.float64), ("steps", np.int32),
                  ("outcome", np.int8))
 STATS_FIELDS = ("phi", "phiphi", "phi_VR", "phiphi_VR", "xi", "xixi", "xiphi", "APL", "RNGC", "B",
-                "paths", "path_steps", "n_max_steps", "status", "n_domain_exits")
+                "paths", "path_steps", "n_max_steps", "status", "n_domain_exits", "cv_fallback")
 
 
 def alloc_records(total: int) -> dict:
@@ -113,6 +113,23 @@ class FKProblem:
     def set_launch(self, block_size=0, blocks_per_cu=0, chunk_paths=0):
         cfg = abi.LaunchConfig(block_size, blocks_per_cu, chunk_paths, 0)
         abi.check(self.lib.spac_fk_set_launch(self.h, C.byref(cfg)), self.lib)
+
+    def set_control_variate(self, mode, table=None, domain=None, fallback=False, max_ratio=1.0):
+        """spac_fk_set_control_variate: mode SPAC_CV_*; table (T, T, 4) float32 for SPAC_CV_TABLE,
+        indexed [j, i] = (u_hat, du/dx, du/dy, 0) at (xmin + i dx, ymin + j dy)."""
+        cfg = abi.CvConfig()
+        cfg.mode = mode
+        cfg.fallback = 1 if fallback else 0
+        cfg.max_ratio = max_ratio
+        keep = None
+        if table is not None:
+            keep = np.ascontiguousarray(table, dtype=np.float32)
+            assert keep.ndim == 3 and keep.shape[0] == keep.shape[1] and keep.shape[2] == 4
+            cfg.table_size = keep.shape[0]
+            for i, v in enumerate(domain):
+                cfg.table_domain[i] = v
+            cfg.table = abi.ptr(keep, C.c_float)
+        abi.check(self.lib.spac_fk_set_control_variate(self.h, C.byref(cfg)), self.lib)
 
     def timing(self) -> abi.Timing:
         t = abi.Timing()

@@ -1,0 +1,105 @@
+"""Python handle over the MLP control-variate C ABI (include/spac_mlp.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+dp = C.c_double
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class MLP:
+    """Owns one spac_mlp handle: parameters, Adam state, training buffers and the table."""
+
+    def __init__(self, domain, seed=0):
+        self.lib = abi.load()
+        d = abi.MlpDesc()
+        for i, v in enumerate(domain):
+            d.domain[i] = v
+        d.seed = seed
+        h = C.c_void_p()
+        abi.check(self.lib.spac_mlp_create(C.byref(d), C.byref(h)), self.lib)
+        self.h = h
+        self.domain = tuple(domain)
+        self.table_size = 0
+
+    def close(self):
+        if self.h:
+            self.lib.spac_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(abi.MLP_N_PARAMS, dtype=np.float32)
+        abi.check(self.lib.spac_mlp_get_params(self.h, abi.ptr(out, C.c_float)), self.lib)
+        return out
+
+    @params.setter
+    def params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        assert t.size == abi.MLP_N_PARAMS
+        abi.check(self.lib.spac_mlp_set_params(self.h, abi.ptr(t, C.c_float)), self.lib)
+
+    def loss_grad(self, x, y, u, grad_weight):
+        x, y, u = _d(x), _d(y), _d(u)
+        g = np.zeros(abi.MLP_N_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        abi.check(self.lib.spac_mlp_loss_grad(self.h, abi.ptr(x, dp), abi.ptr(y, dp), abi.ptr(u, dp), x.size,
+                                              grad_weight, abi.ptr(g, C.c_float), C.byref(loss)), self.lib)
+        return loss.value, g
+
+    def train(self, x, y, u, steps, lr=3e-3, lr_final=1e-4, grad_weight=0.1):
+        x, y, u = _d(x), _d(y), _d(u)
+        cfg = abi.MlpTrainCfg(steps, 0, lr, lr_final, grad_weight)
+        rep = abi.MlpTrainReport()
+        abi.check(self.lib.spac_mlp_train(self.h, abi.ptr(x, dp), abi.ptr(y, dp), abi.ptr(u, dp), x.size,
+                                          C.byref(cfg), C.byref(rep)), self.lib)
+        return {"final_loss": rep.final_loss, "train_ms": rep.train_ms, "steps": rep.steps}
+
+    def forward_fp32(self, x, y):
+        x, y = _d(x), _d(y)
+        out = np.zeros((3, x.size), dtype=np.float32)
+        abi.check(self.lib.spac_mlp_forward_fp32(self.h, abi.ptr(x, dp), abi.ptr(y, dp), x.size,
+                                                 abi.ptr(out, C.c_float)), self.lib)
+        return out
+
+    def infer(self, x, y):
+        """BF16-MFMA inference: (n, 4) = (u, gx, gy, 0)."""
+        x, y = _d(x), _d(y)
+        out = np.zeros((x.size, 4), dtype=np.float32)
+        abi.check(self.lib.spac_mlp_infer(self.h, abi.ptr(x, dp), abi.ptr(y, dp), x.size,
+                                          abi.ptr(out, C.c_float)), self.lib)
+        return out
+
+    def build_table(self, t=1024) -> float:
+        ms = C.c_double()
+        abi.check(self.lib.spac_mlp_build_table(self.h, t, C.byref(ms)), self.lib)
+        self.table_size = t
+        return ms.value
+
+    def read_table(self) -> np.ndarray:
+        t = self.table_size
+        out = np.zeros((t, t, 4), dtype=np.float32)
+        abi.check(self.lib.spac_mlp_read_table(self.h, abi.ptr(out, C.c_float)), self.lib)
+        return out
+
+    def attach(self, problem, fallback=True, max_ratio=1.0):
+        abi.check(self.lib.spac_mlp_attach(self.h, problem.h, 1 if fallback else 0, max_ratio), self.lib)

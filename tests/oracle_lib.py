@@ -181,3 +181,36 @@ def exit_angle(code: int, dx: float, dy: float, first: float) -> float:
     t = C.c_double()
     assert lib.oracle_exit_angle(code, dx, dy, first, C.byref(t)) == 1
     return t.value
+
+
+_CV_TABLE = None
+
+
+def set_cv(mode=abi.SPAC_CV_ANALYTIC, fallback=False, max_ratio=1.0, table=None, domain=None):
+    """oracle_set_cv: the control variate of later simulate/reduce/assemble calls
+    (process-wide; reset with set_cv() when done)."""
+    global _CV_TABLE
+    lib = load()
+    lib.oracle_set_cv.argtypes = [C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_double)]
+    lib.oracle_set_cv.restype = C.c_int
+    t = 0
+    tp = C.cast(None, C.POINTER(C.c_float))
+    dom = np.zeros(4)
+    if table is not None:
+        _CV_TABLE = np.ascontiguousarray(table, dtype=np.float32)
+        t = _CV_TABLE.shape[0]
+        tp = _CV_TABLE.ctypes.data_as(C.POINTER(C.c_float))
+        dom = np.asarray(domain, dtype=np.float64)
+    rc = lib.oracle_set_cv(mode, 1 if fallback else 0, max_ratio, tp, t, _p(dom))
+    assert rc == 0, rc
+
+
+def tabulate(fn, t, domain):
+    """(t, t, 4) float32 table of fn(x, y) -> (u, ux, uy) on the grid of spac_cv_config."""
+    xs = domain[0] + np.arange(t) * ((domain[2] - domain[0]) / (t - 1))
+    ys = domain[1] + np.arange(t) * ((domain[3] - domain[1]) / (t - 1))
+    X, Y = np.meshgrid(xs, ys, indexing="xy")  # [j, i] = (x_i, y_j)
+    u, ux, uy = fn(X, Y)
+    out = np.zeros((t, t, 4), dtype=np.float32)
+    out[..., 0], out[..., 1], out[..., 2] = u, ux, uy
+    return out
