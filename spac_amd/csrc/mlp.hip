@@ -45,6 +45,10 @@ constexpr int oW1 = 0, ob1 = oW1 + kH * 2, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, 
 static_assert(kNParams == SPAC_MLP_N_PARAMS, "parameter count");
 
 /* ------------------------------------------------------------ training -- */
+constexpr int kGroup = 16; /* outputs per thread in the dense kernels */
+constexpr int kChunk = 32; /* samples per weight-gradient partial sum */
+
+/* z = 2 (x - lo) / (hi - lo) - 1, once per batch */
 __global__ void k_input(const double* x, const double* y, int n, float lox, float loy, float sx, float sy,
                         float* z) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -53,35 +57,58 @@ __global__ void k_input(const double* x, const double* y, int n, float lox, floa
     z[n + s] = 2.0f * ((float)y[s] - loy) * sy - 1.0f;
 }
 
-/* out[o][s] = tanh(sum_i W[o][i] in[i][s] + b[o]) */
-__global__ void k_dense_tanh(const float* W, const float* b, const float* in, int in_dim, float* out, int out_dim,
-                             int n) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int o = blockIdx.y;
-    if (s >= n || o >= out_dim) return;
-    float a = 0.0f;
-    for (int i = 0; i < in_dim; ++i) a = fmaf(W[o * in_dim + i], in[(size_t)i * n + s], a);
-    out[(size_t)o * n + s] = tanhf(a + b[o]);
-}
-
-/* tangent: tout[o][s] = (1 - h[o][s]^2) sum_i W[o][i] tin[i][s]; layer 1: tin = scale e_d */
-__global__ void k_tangent(const float* W, const float* tin, int in_dim, const float* h, float* tout, int out_dim,
-                          int n, int first_dir, float first_scale) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int o = blockIdx.y;
-    if (s >= n || o >= out_dim) return;
-    float a;
-    if (first_dir >= 0) {
-        a = W[o * in_dim + first_dir] * first_scale;
-    } else {
-        a = 0.0f;
-        for (int i = 0; i < in_dim; ++i) a = fmaf(W[o * in_dim + i], tin[(size_t)i * n + s], a);
+/* One hidden layer with both forward-mode tangents (d/dx, d/dy):
+ *   h = tanh(W hin + b), t = (1 - h^2) (W tin)   (layer 1: tin = 2 e_d / span_d).
+ * Thread = one sample x kGroup outputs; activations are [feature][sample]. */
+template <int IN>
+__global__ void __launch_bounds__(256) k_layer_fwd(const float* __restrict__ W, const float* __restrict__ b,
+                                                   const float* __restrict__ hin, const float* __restrict__ txin,
+                                                   const float* __restrict__ tyin, float* __restrict__ hout,
+                                                   float* __restrict__ txout, float* __restrict__ tyout, int n,
+                                                   float scx, float scy) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int o0 = blockIdx.y * kGroup;
+    if (s >= n) return;
+    float a[kGroup], ax[kGroup], ay[kGroup];
+#pragma unroll
+    for (int o = 0; o < kGroup; ++o) a[o] = ax[o] = ay[o] = 0.0f;
+    for (int i = 0; i < IN; ++i) {
+        const float hv = hin[(size_t)i * n + s];
+        float tx = 0.0f, ty = 0.0f;
+        if (IN != 2) {
+            tx = txin[(size_t)i * n + s];
+            ty = tyin[(size_t)i * n + s];
+        }
+#pragma unroll
+        for (int o = 0; o < kGroup; ++o) {
+            const float w = W[(o0 + o) * IN + i];
+            a[o] = fmaf(w, hv, a[o]);
+            if (IN != 2) {
+                ax[o] = fmaf(w, tx, ax[o]);
+                ay[o] = fmaf(w, ty, ay[o]);
+            }
+        }
     }
-    float hv = h[(size_t)o * n + s];
-    tout[(size_t)o * n + s] = (1.0f - hv * hv) * a;
+#pragma unroll
+    for (int o = 0; o < kGroup; ++o) {
+        const float h = tanhf(a[o] + b[o0 + o]);
+        const float d = 1.0f - h * h;
+        float tx, ty;
+        if (IN == 2) {
+            tx = d * (W[(o0 + o) * 2] * scx);
+            ty = d * (W[(o0 + o) * 2 + 1] * scy);
+        } else {
+            tx = d * ax[o];
+            ty = d * ay[o];
+        }
+        const size_t k = (size_t)(o0 + o) * n + s;
+        hout[k] = h;
+        txout[k] = tx;
+        tyout[k] = ty;
+    }
 }
 
-/* outputs, forward-mode gradient of o0, residuals and dL/do */
+/* outputs, forward-mode gradient of o0, residuals and dL/do (thread = sample) */
 __global__ void k_output(const float* W4, const float* b4, const float* h3, const float* t3x, const float* t3y,
                          const double* u, int n, float lam, float* out3, float* dO, float* loss_s) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,8 +118,10 @@ __global__ void k_output(const float* W4, const float* b4, const float* h3, cons
     for (int i = 0; i < kH; ++i) {
         float hv = h3[(size_t)i * n + s];
         for (int c = 0; c < 3; ++c) o[c] = fmaf(W4[c * kH + i], hv, o[c]);
-        gx = fmaf(W4[i], t3x[(size_t)i * n + s], gx);
-        gy = fmaf(W4[i], t3y[(size_t)i * n + s], gy);
+        if (u) {
+            gx = fmaf(W4[i], t3x[(size_t)i * n + s], gx);
+            gy = fmaf(W4[i], t3y[(size_t)i * n + s], gy);
+        }
     }
     for (int c = 0; c < 3; ++c) o[c] += b4[c];
     for (int c = 0; c < 3; ++c) out3[(size_t)c * n + s] = o[c];
@@ -105,44 +134,115 @@ __global__ void k_output(const float* W4, const float* b4, const float* h3, cons
     loss_s[s] = r0 * r0 + lam * (r1 * r1 + r2 * r2);
 }
 
-/* dout[i][s] = (1 - h[i][s]^2) sum_o W[o][i] dn[o][s] */
-__global__ void k_backprop(const float* W, const float* dn, int out_dim, const float* h, float* dout, int in_dim,
-                           int n) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
-    if (s >= n || i >= in_dim) return;
-    float a = 0.0f;
-    for (int o = 0; o < out_dim; ++o) a = fmaf(W[o * in_dim + i], dn[(size_t)o * n + s], a);
-    float hv = h[(size_t)i * n + s];
-    dout[(size_t)i * n + s] = (1.0f - hv * hv) * a;
-}
-
-/* dW[o][i] = sum_s d[o][s] x[i][s]; db[o] = sum_s d[o][s] (i == in_dim) */
-__global__ void k_weight_grad(const float* d, const float* xin, int out_dim, int in_dim, int n, float* dW, float* db) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int per = in_dim + 1;
-    if (t >= out_dim * per) return;
-    int o = t / per, i = t % per;
-    float a = 0.0f;
-    if (i < in_dim) {
-        for (int s = 0; s < n; ++s) a = fmaf(d[(size_t)o * n + s], xin[(size_t)i * n + s], a);
-        dW[o * in_dim + i] = a;
-    } else {
-        for (int s = 0; s < n; ++s) a += d[(size_t)o * n + s];
-        db[o] = a;
+/* dout[i][s] = (1 - h[i][s]^2) sum_o W[o][i] dn[o][s]; thread = sample x kGroup inputs */
+template <int OUT>
+__global__ void __launch_bounds__(256) k_backprop(const float* __restrict__ W, const float* __restrict__ dn,
+                                                  const float* __restrict__ h, float* __restrict__ dout, int n) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = blockIdx.y * kGroup;
+    if (s >= n) return;
+    float a[kGroup];
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) a[i] = 0.0f;
+    for (int o = 0; o < OUT; ++o) {
+        const float dv = dn[(size_t)o * n + s];
+#pragma unroll
+        for (int i = 0; i < kGroup; ++i) a[i] = fmaf(W[o * kH + i0 + i], dv, a[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) {
+        const float hv = h[(size_t)(i0 + i) * n + s];
+        dout[(size_t)(i0 + i) * n + s] = (1.0f - hv * hv) * a[i];
     }
 }
 
-__global__ void k_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
-                       float c1, float c2) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    float gk = g[k];
-    float mk = b1 * m[k] + (1.0f - b1) * gk;
-    float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
-    m[k] = mk;
-    v[k] = vk;
-    p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+/* Weight and bias gradients of all four layers, split over chunks of kChunk
+ * samples: part[chunk][param] = sum over the chunk's samples (sequential), both
+ * operands staged in LDS.  grid = (chunks, 4 layers). */
+struct WgradArgs {
+    const float* d[4];   /* layer deltas [OUT][n] */
+    const float* x[4];   /* layer inputs [IN][n] */
+    float* part;         /* [chunks][kNParams] */
+    int n;
+};
+__global__ void __launch_bounds__(256) k_wgrad_partial(WgradArgs a) {
+    constexpr int kPad = kChunk + 1;
+    __shared__ float dL[kH * kPad];
+    __shared__ float xL[kH * kPad];
+    const int L = blockIdx.y, chunk = blockIdx.x, tid = threadIdx.x;
+    const int OUT = (L == 3) ? 3 : kH;
+    const int IN = (L == 0) ? 2 : kH;
+    const int offW = (L == 0) ? oW1 : (L == 1) ? oW2 : (L == 2) ? oW3 : oW4;
+    const int offb = (L == 0) ? ob1 : (L == 1) ? ob2 : (L == 2) ? ob3 : ob4;
+    const float* d = a.d[L];
+    const float* x = a.x[L];
+    const int s0 = chunk * kChunk;
+    for (int k = tid; k < OUT * kChunk; k += 256) {
+        int o = k / kChunk, c = k % kChunk, s = s0 + c;
+        dL[o * kPad + c] = (s < a.n) ? d[(size_t)o * a.n + s] : 0.0f;
+    }
+    for (int k = tid; k < IN * kChunk; k += 256) {
+        int i = k / kChunk, c = k % kChunk, s = s0 + c;
+        xL[i * kPad + c] = (s < a.n) ? x[(size_t)i * a.n + s] : 0.0f;
+    }
+    __syncthreads();
+    float* out = a.part + (size_t)chunk * kNParams;
+    for (int k = tid; k < OUT * (IN + 1); k += 256) {
+        const int o = k / (IN + 1), i = k % (IN + 1);
+        const float* dr = dL + o * kPad;
+        float acc = 0.0f;
+        if (i < IN) {
+            const float* xr = xL + i * kPad;
+#pragma unroll 8
+            for (int c = 0; c < kChunk; ++c) acc = fmaf(dr[c], xr[c], acc);
+            out[offW + o * IN + i] = acc;
+        } else {
+#pragma unroll 8
+            for (int c = 0; c < kChunk; ++c) acc += dr[c];
+            out[offb + o] = acc;
+        }
+    }
+}
+
+/* grad[k] = sum over chunks (in order) of part[chunk][k]; then, in training,
+ * one Adam step with a geometric learning-rate decay.  The step counter lives
+ * in device memory so a captured graph of steps replays unchanged. */
+struct AdamArgs {
+    float* p;
+    float* g;
+    float* m;
+    float* v;
+    const float* part;
+    int chunks;
+    const int* ctr; /* Adam steps taken so far */
+    int t0;         /* counter value at the first step of this schedule */
+    int steps;
+    float lr0, lr1; /* lr1 <= 0: constant lr0 */
+    int adam;
+};
+__global__ void k_reduce_adam(AdamArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kNParams) return;
+    float g = 0.0f;
+    for (int c = 0; c < a.chunks; ++c) g += a.part[(size_t)c * kNParams + k];
+    a.g[k] = g;
+    if (!a.adam) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const int tp = a.ctr[0];
+    const float t = (float)(tp + 1);
+    float lr = a.lr0;
+    if (a.lr1 > 0.0f && a.steps > 1)
+        lr = a.lr0 * powf(a.lr1 / a.lr0, (float)(tp - a.t0) / (float)(a.steps - 1));
+    const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
+    const float mk = b1 * a.m[k] + (1.0f - b1) * g;
+    const float vk = b2 * a.v[k] + (1.0f - b2) * g * g;
+    a.m[k] = mk;
+    a.v[k] = vk;
+    a.p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+__global__ void k_tick(int* ctr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] += 1;
 }
 
 /* ----------------------------------------------------- MFMA inference -- */
@@ -255,89 +355,120 @@ __global__ void k_to_bf16(const float* in, short* out, int n) {
 struct Mlp {
     spac_mlp_desc desc{};
     hipStream_t stream = nullptr;
-    DeviceBuffer params, grads, am, av, wbf, act, pts, tgt, table;
-    int64_t cap_n = 0;
+    DeviceBuffer params, grads, am, av, wbf, act, pts, tgt, table, part, ctr;
+    int n = 0; /* prepared batch size */
     int adam_t = 0;
     int table_t = 0;
     float lox = 0, loy = 0, sx = 1, sy = 1;
 };
 
+/* Activation buffer layout (floats, [feature][sample]): z(2) h1 h2 h3 tx1 ty1
+ * tx2 ty2 tx3 ty3 d3 d2 d1 (64 each) out3(3) dO(3) loss(1). */
+struct ActView {
+    float *z, *h[3], *tx[3], *ty[3], *d3, *d2, *d1, *out3, *dO, *loss;
+};
+static size_t act_floats(size_t N) { return N * (2 + 13 * kH + 3 + 3 + 1); }
+static ActView act_view(float* A, size_t N) {
+    ActView v;
+    v.z = A;
+    float* p = A + 2 * N;
+    for (int l = 0; l < 3; ++l) { v.h[l] = p; p += kH * N; }
+    for (int l = 0; l < 3; ++l) { v.tx[l] = p; p += kH * N; v.ty[l] = p; p += kH * N; }
+    v.d3 = p; p += kH * N;
+    v.d2 = p; p += kH * N;
+    v.d1 = p; p += kH * N;
+    v.out3 = p; p += 3 * N;
+    v.dO = p; p += 3 * N;
+    v.loss = p;
+    return v;
+}
+static int n_chunks(int n) { return (n + kChunk - 1) / kChunk; }
+
 static spac_status mlp_prepare_batch(Mlp* m, const double* x, const double* y, const double* u, int n) {
     spac_status rc;
-    size_t N = (size_t)n;
-    /* act: z(2) h1 h2 h3 (3x64) t1..t3 x 2 dirs (6x64) d3 d2 d1 (3x64) out3 dO(3) loss(1) */
-    size_t floats = N * (2 + 3 * kH + 6 * kH + 3 * kH + 3 + 3 + 1);
-    if ((rc = m->act.ensure(floats * 4))) return rc;
+    const size_t N = (size_t)n;
+    if ((rc = m->act.ensure(act_floats(N) * 4))) return rc;
     if ((rc = m->pts.ensure(N * 16))) return rc;
     if ((rc = m->tgt.ensure(N * 8))) return rc;
+    if ((rc = m->part.ensure((size_t)n_chunks(n) * kNParams * 4))) return rc;
     MLP_HIP_CHECK(hipMemcpyAsync(m->pts.ptr, x, N * 8, hipMemcpyHostToDevice, m->stream));
     MLP_HIP_CHECK(hipMemcpyAsync((char*)m->pts.ptr + N * 8, y, N * 8, hipMemcpyHostToDevice, m->stream));
     if (u) MLP_HIP_CHECK(hipMemcpyAsync(m->tgt.ptr, u, N * 8, hipMemcpyHostToDevice, m->stream));
-    return SPAC_OK;
-}
-
-/* forward + backward over the prepared batch: grads and per-sample loss on device */
-static spac_status mlp_fwd_bwd(Mlp* m, int n, float lam, bool want_grad) {
-    hipStream_t s = m->stream;
-    const size_t N = (size_t)n;
-    float* A = (float*)m->act.ptr;
-    float* z = A;
-    float* h1 = z + 2 * N;
-    float* h2 = h1 + kH * N;
-    float* h3 = h2 + kH * N;
-    float* t1[2] = {h3 + kH * N, h3 + 2 * kH * N};
-    float* t2[2] = {h3 + 3 * kH * N, h3 + 4 * kH * N};
-    float* t3[2] = {h3 + 5 * kH * N, h3 + 6 * kH * N};
-    float* d3 = h3 + 7 * kH * N;
-    float* d2 = d3 + kH * N;
-    float* d1 = d2 + kH * N;
-    float* out3 = d1 + kH * N;
-    float* dO = out3 + 3 * N;
-    float* loss_s = dO + 3 * N;
-    const float* P = (const float*)m->params.ptr;
-    float* Gd = (float*)m->grads.ptr;
-    const double* x = (const double*)m->pts.ptr;
-    const double* y = x + N;
-    dim3 blk(256), g1((n + 255) / 256), gH((n + 255) / 256, kH);
-    hipLaunchKernelGGL(k_input, g1, blk, 0, s, x, y, n, m->lox, m->loy, m->sx, m->sy, z);
-    hipLaunchKernelGGL(k_dense_tanh, gH, blk, 0, s, P + oW1, P + ob1, z, 2, h1, kH, n);
-    hipLaunchKernelGGL(k_dense_tanh, gH, blk, 0, s, P + oW2, P + ob2, h1, kH, h2, kH, n);
-    hipLaunchKernelGGL(k_dense_tanh, gH, blk, 0, s, P + oW3, P + ob3, h2, kH, h3, kH, n);
-    const float sc[2] = {2.0f * m->sx, 2.0f * m->sy};
-    for (int d = 0; d < 2; ++d) {
-        hipLaunchKernelGGL(k_tangent, gH, blk, 0, s, P + oW1, (const float*)nullptr, 2, h1, t1[d], kH, n, d, sc[d]);
-        hipLaunchKernelGGL(k_tangent, gH, blk, 0, s, P + oW2, t1[d], kH, h2, t2[d], kH, n, -1, 0.0f);
-        hipLaunchKernelGGL(k_tangent, gH, blk, 0, s, P + oW3, t2[d], kH, h3, t3[d], kH, n, -1, 0.0f);
-    }
-    hipLaunchKernelGGL(k_output, g1, blk, 0, s, P + oW4, P + ob4, h3, t3[0], t3[1], (const double*)m->tgt.ptr, n, lam,
-                       out3, dO, loss_s);
-    if (!want_grad) return SPAC_OK;
-    auto wgrad = [&](const float* d, const float* xin, int od, int id, int oW, int ob) {
-        int threads = od * (id + 1);
-        hipLaunchKernelGGL(k_weight_grad, dim3((threads + 127) / 128), dim3(128), 0, s, d, xin, od, id, n, Gd + oW,
-                           Gd + ob);
-    };
-    wgrad(dO, h3, 3, kH, oW4, ob4);
-    hipLaunchKernelGGL(k_backprop, gH, blk, 0, s, P + oW4, dO, 3, h3, d3, kH, n);
-    wgrad(d3, h2, kH, kH, oW3, ob3);
-    hipLaunchKernelGGL(k_backprop, gH, blk, 0, s, P + oW3, d3, kH, h2, d2, kH, n);
-    wgrad(d2, h1, kH, kH, oW2, ob2);
-    hipLaunchKernelGGL(k_backprop, gH, blk, 0, s, P + oW2, d2, kH, h1, d1, kH, n);
-    wgrad(d1, z, kH, 2, oW1, ob1);
+    ActView v = act_view((float*)m->act.ptr, N);
+    const double* dx = (const double*)m->pts.ptr;
+    hipLaunchKernelGGL(k_input, dim3((n + 255) / 256), dim3(256), 0, m->stream, dx, dx + N, n, m->lox, m->loy, m->sx,
+                       m->sy, v.z);
     MLP_HIP_CHECK(hipGetLastError());
+    m->n = n;
     return SPAC_OK;
 }
 
-static spac_status mlp_loss_host(Mlp* m, int n, double* loss) {
+/* Forward (with tangents) and outputs over the prepared batch; with targets,
+ * also the loss terms and dL/do. */
+static void enqueue_forward(Mlp* m, bool with_targets, float lam) {
+    const int n = m->n;
     const size_t N = (size_t)n;
+    hipStream_t s = m->stream;
+    ActView v = act_view((float*)m->act.ptr, N);
+    const float* P = (const float*)m->params.ptr;
+    dim3 blk(256), gH((n + 255) / 256, kH / kGroup), g1((n + 255) / 256);
+    hipLaunchKernelGGL(k_layer_fwd<2>, gH, blk, 0, s, P + oW1, P + ob1, v.z, (const float*)nullptr,
+                       (const float*)nullptr, v.h[0], v.tx[0], v.ty[0], n, 2.0f * m->sx, 2.0f * m->sy);
+    hipLaunchKernelGGL(k_layer_fwd<kH>, gH, blk, 0, s, P + oW2, P + ob2, v.h[0], v.tx[0], v.ty[0], v.h[1], v.tx[1],
+                       v.ty[1], n, 0.0f, 0.0f);
+    hipLaunchKernelGGL(k_layer_fwd<kH>, gH, blk, 0, s, P + oW3, P + ob3, v.h[1], v.tx[1], v.ty[1], v.h[2], v.tx[2],
+                       v.ty[2], n, 0.0f, 0.0f);
+    hipLaunchKernelGGL(k_output, g1, blk, 0, s, P + oW4, P + ob4, v.h[2], v.tx[2], v.ty[2],
+                       with_targets ? (const double*)m->tgt.ptr : (const double*)nullptr, n, lam, v.out3, v.dO,
+                       v.loss);
+}
+
+/* One full step: forward, backward, gradient reduction and (adam) the update. */
+static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float lr0, float lr1) {
+    const int n = m->n;
+    const size_t N = (size_t)n;
+    hipStream_t s = m->stream;
+    ActView v = act_view((float*)m->act.ptr, N);
+    const float* P = (const float*)m->params.ptr;
+    enqueue_forward(m, true, lam);
+    dim3 blk(256), gH((n + 255) / 256, kH / kGroup);
+    hipLaunchKernelGGL(k_backprop<3>, gH, blk, 0, s, P + oW4, v.dO, v.h[2], v.d3, n);
+    hipLaunchKernelGGL(k_backprop<kH>, gH, blk, 0, s, P + oW3, v.d3, v.h[1], v.d2, n);
+    hipLaunchKernelGGL(k_backprop<kH>, gH, blk, 0, s, P + oW2, v.d2, v.h[0], v.d1, n);
+    WgradArgs w;
+    w.d[0] = v.d1; w.x[0] = v.z;
+    w.d[1] = v.d2; w.x[1] = v.h[0];
+    w.d[2] = v.d3; w.x[2] = v.h[1];
+    w.d[3] = v.dO; w.x[3] = v.h[2];
+    w.part = (float*)m->part.ptr;
+    w.n = n;
+    hipLaunchKernelGGL(k_wgrad_partial, dim3(n_chunks(n), 4), dim3(256), 0, s, w);
+    AdamArgs a;
+    a.p = (float*)m->params.ptr;
+    a.g = (float*)m->grads.ptr;
+    a.m = (float*)m->am.ptr;
+    a.v = (float*)m->av.ptr;
+    a.part = (const float*)m->part.ptr;
+    a.chunks = n_chunks(n);
+    a.ctr = (const int*)m->ctr.ptr;
+    a.t0 = t0;
+    a.steps = steps;
+    a.lr0 = lr0;
+    a.lr1 = lr1;
+    a.adam = adam ? 1 : 0;
+    hipLaunchKernelGGL(k_reduce_adam, dim3((kNParams + 255) / 256), dim3(256), 0, s, a);
+    if (adam) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, (int*)m->ctr.ptr);
+}
+
+static spac_status mlp_loss_host(Mlp* m, double* loss) {
+    const size_t N = (size_t)m->n;
     std::vector<float> ls(N);
-    float* A = (float*)m->act.ptr;
-    float* loss_s = A + N * (2 + 3 * kH + 6 * kH + 3 * kH + 3 + 3);
-    MLP_HIP_CHECK(hipMemcpyAsync(ls.data(), loss_s, N * 4, hipMemcpyDeviceToHost, m->stream));
+    ActView v = act_view((float*)m->act.ptr, N);
+    MLP_HIP_CHECK(hipMemcpyAsync(ls.data(), v.loss, N * 4, hipMemcpyDeviceToHost, m->stream));
     MLP_HIP_CHECK(hipStreamSynchronize(m->stream));
     double acc = 0.0;
-    for (float v : ls) acc += v;
-    *loss = acc / (double)n;
+    for (float x : ls) acc += x;
+    *loss = acc / (double)N;
     return SPAC_OK;
 }
 
@@ -368,6 +499,20 @@ static spac_status mlp_infer_dev(Mlp* m, const double* dx, const double* dy, int
     return SPAC_OK;
 }
 
+/* Capture `k` training steps into one executable graph. */
+static spac_status capture_steps(Mlp* m, int k, float lam, int t0, int steps, float lr0, float lr1,
+                                 hipGraphExec_t* exec) {
+    hipGraph_t g = nullptr;
+    MLP_HIP_CHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < k; ++i) enqueue_step(m, lam, true, t0, steps, lr0, lr1);
+    hipError_t e = hipStreamEndCapture(m->stream, &g);
+    if (e != hipSuccess) return set_error(SPAC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_error(SPAC_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    return SPAC_OK;
+}
+
 } // namespace spac
 
 using namespace spac;
@@ -390,8 +535,8 @@ spac_status spac_mlp_create(const spac_mlp_desc* d, spac_mlp** out) {
     MLP_HIP_CHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
     spac_status rc;
     if ((rc = m->params.ensure(kNParams * 4)) || (rc = m->grads.ensure(kNParams * 4)) ||
-        (rc = m->am.ensure(kNParams * 4)) || (rc = m->av.ensure(kNParams * 4))) {
-        delete m;
+        (rc = m->am.ensure(kNParams * 4)) || (rc = m->av.ensure(kNParams * 4)) || (rc = m->ctr.ensure(64))) {
+        spac_mlp_destroy(reinterpret_cast<spac_mlp*>(m));
         return rc;
     }
     /* Glorot-uniform weights, zero biases */
@@ -409,6 +554,7 @@ spac_status spac_mlp_create(const spac_mlp_desc* d, spac_mlp** out) {
     MLP_HIP_CHECK(hipMemcpy(m->params.ptr, p.data(), kNParams * 4, hipMemcpyHostToDevice));
     MLP_HIP_CHECK(hipMemset(m->am.ptr, 0, kNParams * 4));
     MLP_HIP_CHECK(hipMemset(m->av.ptr, 0, kNParams * 4));
+    MLP_HIP_CHECK(hipMemset(m->ctr.ptr, 0, 64));
     *out = reinterpret_cast<spac_mlp*>(m);
     return SPAC_OK;
 }
@@ -416,10 +562,11 @@ spac_status spac_mlp_create(const spac_mlp_desc* d, spac_mlp** out) {
 spac_status spac_mlp_destroy(spac_mlp* hm) {
     Mlp* m = reinterpret_cast<Mlp*>(hm);
     if (!m) return SPAC_OK;
-    (void)hipStreamSynchronize(m->stream);
-    for (DeviceBuffer* b : {&m->params, &m->grads, &m->am, &m->av, &m->wbf, &m->act, &m->pts, &m->tgt, &m->table})
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (DeviceBuffer* b : {&m->params, &m->grads, &m->am, &m->av, &m->wbf, &m->act, &m->pts, &m->tgt, &m->table,
+                            &m->part, &m->ctr})
         b->release();
-    (void)hipStreamDestroy(m->stream);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return SPAC_OK;
 }
@@ -438,6 +585,7 @@ spac_status spac_mlp_set_params(spac_mlp* hm, const float* in) {
     m->adam_t = 0;
     MLP_HIP_CHECK(hipMemset(m->am.ptr, 0, kNParams * 4));
     MLP_HIP_CHECK(hipMemset(m->av.ptr, 0, kNParams * 4));
+    MLP_HIP_CHECK(hipMemset(m->ctr.ptr, 0, 64));
     return SPAC_OK;
 }
 
@@ -447,9 +595,11 @@ spac_status spac_mlp_loss_grad(spac_mlp* hm, const double* x, const double* y, c
     if (!m || !x || !y || !u || n < 1) return set_error(SPAC_E_INVALID_ARG, "bad arguments");
     spac_status rc;
     if ((rc = mlp_prepare_batch(m, x, y, u, n))) return rc;
-    if ((rc = mlp_fwd_bwd(m, n, (float)lam, grad != nullptr))) return rc;
-    if (loss && (rc = mlp_loss_host(m, n, loss))) return rc;
-    if (grad) MLP_HIP_CHECK(hipMemcpy(grad, m->grads.ptr, kNParams * 4, hipMemcpyDeviceToHost));
+    if (grad) enqueue_step(m, (float)lam, false, 0, 1, 0.0f, 0.0f);
+    else enqueue_forward(m, true, (float)lam);
+    MLP_HIP_CHECK(hipGetLastError());
+    if (loss && (rc = mlp_loss_host(m, loss))) return rc;
+    if (grad) MLP_HIP_CHECK(hipMemcpyAsync(grad, m->grads.ptr, kNParams * 4, hipMemcpyDeviceToHost, m->stream));
     MLP_HIP_CHECK(hipStreamSynchronize(m->stream));
     return SPAC_OK;
 }
@@ -457,37 +607,44 @@ spac_status spac_mlp_loss_grad(spac_mlp* hm, const double* x, const double* y, c
 spac_status spac_mlp_train(spac_mlp* hm, const double* x, const double* y, const double* u, int32_t n,
                            const spac_mlp_train_cfg* cfg, spac_mlp_train_report* rep) {
     Mlp* m = reinterpret_cast<Mlp*>(hm);
-    if (!m || !x || !y || !u || n < 1 || !cfg || cfg->steps < 1) return set_error(SPAC_E_INVALID_ARG, "bad arguments");
+    if (!m || !x || !y || !u || n < 1 || !cfg || cfg->steps < 1 || !(cfg->lr > 0.0))
+        return set_error(SPAC_E_INVALID_ARG, "bad arguments (n >= 1, steps >= 1, lr > 0)");
     spac_status rc;
     if ((rc = mlp_prepare_batch(m, x, y, u, n))) return rc;
+    const float lam = (float)cfg->grad_weight, lr0 = (float)cfg->lr;
+    const float lr1 = cfg->lr_final > 0.0 ? (float)cfg->lr_final : 0.0f;
+    const int t0 = m->adam_t;
+    /* graphs of kPerGraph steps, plus one for the remainder */
+    const int kPerGraph = 50;
+    const int full = cfg->steps / kPerGraph, rest = cfg->steps % kPerGraph;
+    hipGraphExec_t gfull = nullptr, grest = nullptr;
+    if (full && (rc = capture_steps(m, kPerGraph, lam, t0, cfg->steps, lr0, lr1, &gfull))) return rc;
+    if (rest && (rc = capture_steps(m, rest, lam, t0, cfg->steps, lr0, lr1, &grest))) {
+        if (gfull) (void)hipGraphExecDestroy(gfull);
+        return rc;
+    }
     hipEvent_t e0, e1;
     MLP_HIP_CHECK(hipEventCreate(&e0));
     MLP_HIP_CHECK(hipEventCreate(&e1));
     MLP_HIP_CHECK(hipEventRecord(e0, m->stream));
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int it = 0; it < cfg->steps; ++it) {
-        float lam = (float)cfg->grad_weight;
-        if ((rc = mlp_fwd_bwd(m, n, lam, true))) return rc;
-        m->adam_t++;
-        float lr = (float)cfg->lr;
-        if (cfg->lr_final > 0.0) /* geometric decay from lr to lr_final */
-            lr = (float)(cfg->lr * std::pow(cfg->lr_final / cfg->lr, (double)it / std::max(1, cfg->steps - 1)));
-        float c1 = 1.0f - std::pow(b1, (float)m->adam_t), c2 = 1.0f - std::pow(b2, (float)m->adam_t);
-        hipLaunchKernelGGL(k_adam, dim3((kNParams + 255) / 256), dim3(256), 0, m->stream, (float*)m->params.ptr,
-                           (const float*)m->grads.ptr, (float*)m->am.ptr, (float*)m->av.ptr, kNParams, lr, b1, b2,
-                           eps, c1, c2);
-    }
+    for (int i = 0; i < full; ++i) MLP_HIP_CHECK(hipGraphLaunch(gfull, m->stream));
+    if (rest) MLP_HIP_CHECK(hipGraphLaunch(grest, m->stream));
     MLP_HIP_CHECK(hipEventRecord(e1, m->stream));
     MLP_HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0.f;
     MLP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (gfull) (void)hipGraphExecDestroy(gfull);
+    if (grest) (void)hipGraphExecDestroy(grest);
+    m->adam_t += cfg->steps;
     if (rep) {
         rep->train_ms = ms;
         rep->steps = cfg->steps;
-        if ((rc = mlp_fwd_bwd(m, n, (float)cfg->grad_weight, false))) return rc;
-        if ((rc = mlp_loss_host(m, n, &rep->final_loss))) return rc;
+        rep->reserved = 0;
+        enqueue_forward(m, true, lam);
+        MLP_HIP_CHECK(hipGetLastError());
+        if ((rc = mlp_loss_host(m, &rep->final_loss))) return rc;
     }
     return SPAC_OK;
 }
@@ -503,40 +660,26 @@ spac_status spac_mlp_infer(spac_mlp* hm, const double* x, const double* y, int32
     float* dout = (float*)(dy + n);
     MLP_HIP_CHECK(hipMemcpyAsync(dx, x, (size_t)n * 8, hipMemcpyHostToDevice, m->stream));
     MLP_HIP_CHECK(hipMemcpyAsync(dy, y, (size_t)n * 8, hipMemcpyHostToDevice, m->stream));
-    if ((rc = mlp_infer_dev(m, dx, dy, n, 0, dout))) { io.release(); return rc; }
+    if ((rc = mlp_infer_dev(m, dx, dy, n, 0, dout))) {
+        io.release();
+        return rc;
+    }
     MLP_HIP_CHECK(hipMemcpyAsync(out4, dout, (size_t)n * 16, hipMemcpyDeviceToHost, m->stream));
     MLP_HIP_CHECK(hipStreamSynchronize(m->stream));
     io.release();
     return SPAC_OK;
 }
 
-/* FP32 reference forward (the training kernels) for checks: out3 = [3][n] */
 spac_status spac_mlp_forward_fp32(spac_mlp* hm, const double* x, const double* y, int32_t n, float* out3) {
     Mlp* m = reinterpret_cast<Mlp*>(hm);
     if (!m || !x || !y || !out3 || n < 1) return set_error(SPAC_E_INVALID_ARG, "bad arguments");
     spac_status rc;
     if ((rc = mlp_prepare_batch(m, x, y, nullptr, n))) return rc;
-    hipStream_t s = m->stream;
-    const size_t N = (size_t)n;
-    float* A = (float*)m->act.ptr;
-    float* z = A;
-    float* h1 = z + 2 * N;
-    float* h2 = h1 + kH * N;
-    float* h3 = h2 + kH * N;
-    float* t3x = h3 + 5 * kH * N; /* not computed here: k_output reads it only for the loss */
-    float* out = h3 + 7 * kH * N + 3 * kH * N;
-    const float* P = (const float*)m->params.ptr;
-    const double* dx = (const double*)m->pts.ptr;
-    dim3 blk(256), g1((n + 255) / 256), gH((n + 255) / 256, kH);
-    hipLaunchKernelGGL(k_input, g1, blk, 0, s, dx, dx + N, n, m->lox, m->loy, m->sx, m->sy, z);
-    hipLaunchKernelGGL(k_dense_tanh, gH, blk, 0, s, P + oW1, P + ob1, z, 2, h1, kH, n);
-    hipLaunchKernelGGL(k_dense_tanh, gH, blk, 0, s, P + oW2, P + ob2, h1, kH, h2, kH, n);
-    hipLaunchKernelGGL(k_dense_tanh, gH, blk, 0, s, P + oW3, P + ob3, h2, kH, h3, kH, n);
-    hipLaunchKernelGGL(k_output, g1, blk, 0, s, P + oW4, P + ob4, h3, t3x, t3x, (const double*)nullptr, n, 0.0f, out,
-                       (float*)nullptr, (float*)nullptr);
+    enqueue_forward(m, false, 0.0f);
     MLP_HIP_CHECK(hipGetLastError());
-    MLP_HIP_CHECK(hipMemcpyAsync(out3, out, N * 3 * 4, hipMemcpyDeviceToHost, s));
-    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    ActView v = act_view((float*)m->act.ptr, (size_t)n);
+    MLP_HIP_CHECK(hipMemcpyAsync(out3, v.out3, (size_t)n * 3 * 4, hipMemcpyDeviceToHost, m->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(m->stream));
     return SPAC_OK;
 }
 

@@ -148,7 +148,7 @@ def test_learned_cv_gain(gpu):
     noise = np.sqrt(np.mean([(s["phiphi"] - s["phi"] ** 2) / s["paths"] for s in st]))
     say(f"training set: 4096 points, MC noise {noise:.2e}, rms vs exact {np.sqrt(np.mean((u_mc - franke_u(xs, ys)) ** 2)):.2e}")
     with MLP(DOM, seed=0) as m:
-        rep = m.train(xs, ys, u_mc, steps=4000, lr=3e-3, lr_final=1e-5, grad_weight=0.1)
+        rep = m.train(xs, ys, u_mc, steps=6000, lr=3e-3, lr_final=1e-5, grad_weight=0.03)
         t_ms = m.build_table(1024)
         say(f"trained: loss {rep['final_loss']:.3e} in {rep['train_ms']:.0f} ms; table {t_ms:.2f} ms")
         rng = np.random.default_rng(5)
