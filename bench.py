@@ -215,6 +215,117 @@ def bench_rows(args):
     return out
 
 
+def _ttr_mesh():
+    """Config-5 mesh: 64x64 circles on the unit square, 25 nodes per circle (~100k nodes);
+    h scaled with the circle radius squared so paths take as many steps as at 4x4 circles."""
+    mesh = Mesh(mesh_desc(n_side=64, nodes_per_circle=25))
+    h = 5e-4 * (mesh.info.radius / (1.2 / (3 * np.sqrt(2.0)))) ** 2
+    return mesh, h
+
+
+def _replicas(mesh, prob, paths, h, seeds, u0=None):
+    """Independent assemblies (one per seed) at `paths` per row, each solved by GPU GMRES.
+
+    Returns per-replica timings and two error measures: the statistical RMS of the solved
+    node values (sample variance across replicas, averaged over nodes) and the RMS of the
+    row residuals G_k u_bar - B_k around the replica-mean solution (the per-row Monte-Carlo
+    noise, independent across rows)."""
+    import scipy.sparse as sp
+
+    out, us, mats = [], [], []
+    n = mesh.info.n_nodes
+    for s in seeds:
+        rows = prob.assemble_rows(mesh, paths, h, seed=s)
+        t = prob.timing()
+        mesh.accumulate(rows)
+        Gi, Gj, Gv, B = mesh.csr()
+        u, info = gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=20000, restart=200, u0=u0, device=True)
+        u0 = u
+        us.append(u)
+        mats.append((sp.csr_matrix((Gv, Gj, Gi), shape=(n, n)), B))
+        out.append(dict(ms=t.total_ms, path_ms=t.path_kernel_ms, steps=int(sum(x["path_steps"] for x in rows["stats"])),
+                        fallbacks=int(sum(x["cv_fallback"] for x in rows["stats"])), gmres_ms=info.seconds * 1e3,
+                        gmres_device_ms=info.device_ms, gmres_it=int(info.iterations), converged=bool(info.converged)))
+    U = np.array(us)
+    ubar = U.mean(0)
+    rms_u = float(np.sqrt(U.var(0, ddof=1).mean()))
+    R = np.array([A @ ubar - B for A, B in mats])
+    rms_row = float(np.sqrt(R.var(0, ddof=1).mean()))
+    return out, rms_u, rms_row, ubar
+
+
+def bench_ttr(target=1e-3, probe_paths=256, train_steps=2000, lr=(1e-2, 1e-4), n_rep=8, seed0=100):
+    """Time to a statistical RMS error `target` of the solved interface values, learned control
+    variate against plain Monte Carlo, both FP32 stepping on this GPU.
+
+    Statistical RMS at a path count: n_rep independent replicas (seeds), sample variance per
+    node, averaged over nodes.  A probe at probe_paths fixes the path count for the target
+    (RMS ~ N^-1/2, +5 %); the method then runs n_rep replicas at that count, which are timed and
+    re-measured.  The learned-CV time includes its training data (plain-MC point solves at 4096
+    points), training and tabulation; GMRES (GPU, FP64) is reported beside the Monte-Carlo time."""
+    from spac_amd.mlp import MLP
+    from spac_amd.problems import franke_u
+
+    mesh, h = _ttr_mesh()
+    dom = (0.0, 0.0, 1.0, 1.0)
+    desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=dom, precision=abi.SPAC_FP32)
+    exact = franke_u(mesh.node_x, mesh.node_y)
+    res = {"target_rms": target, "replicas": n_rep,
+           "mesh": {"nodes": int(mesh.info.n_nodes), "rows": int(mesh.info.n_jobs),
+                    "circles": int(mesh.info.n_interfaces), "h": h, "nnz": int(mesh.info.nnz)}}
+
+    def measure(prob, name, extra):
+        probe = [seed0 + k for k in range(n_rep)]
+        _, rms0, row0, _ = _replicas(mesh, prob, probe_paths, h, probe)
+        n_star = int(np.ceil(1.05 * probe_paths * (rms0 / target) ** 2))
+        log(f"ttr {name}: rms {rms0:.3e} (row {row0:.3e}) at {probe_paths} paths -> {n_star} paths")
+        runs, rms, row, ubar = _replicas(mesh, prob, n_star, h, [seed0 + 1000 + k for k in range(n_rep)])
+        r = {"paths_per_row": n_star, "rms": rms, "row_residual_rms": row, "probe_rms": rms0,
+             "probe_row_residual_rms": row0, "mc_ms": float(np.mean([x["ms"] for x in runs])),
+             "path_kernel_ms": float(np.mean([x["path_ms"] for x in runs])),
+             "gmres_ms": float(np.mean([x["gmres_ms"] for x in runs])),
+             "gmres_iterations": runs[0]["gmres_it"], "path_steps": runs[0]["steps"],
+             "fallback_rows": runs[0]["fallbacks"], "rms_vs_exact": float(np.sqrt(np.mean((ubar - exact) ** 2)))}
+        r.update(extra)
+        log(f"ttr {name}: {r}")
+        return r
+
+    with FKProblem(desc) as prob:
+        prob.set_control_variate(abi.SPAC_CV_NONE)
+        prob.assemble_rows(mesh, 16, h, seed=1)  # warm-up
+        res["plain"] = measure(prob, "plain", {})
+    # learned control variate: training data, training, tabulation
+    g = (np.arange(64) + 0.5) / 64
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    rng = np.random.default_rng(0)
+    xs = np.clip(X.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
+    ys = np.clip(Y.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
+    with FKProblem(problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=dom, precision=abi.SPAC_FP32)) as pp:
+        pp.set_control_variate(abi.SPAC_CV_NONE)
+        st, _ = pp.solve_points(xs, ys, 2000, 5e-4, seed=1)
+        data_ms = pp.timing().total_ms
+    u_mc = np.array([x["phi"] for x in st])
+    with MLP(dom, seed=0) as net, FKProblem(desc) as prob:
+        rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03)
+        table_ms = net.build_table(1024)
+        net.attach(prob, fallback=True)
+        res["learned"] = measure(prob, "learned", {"training_data_ms": data_ms, "train_ms": rep["train_ms"],
+                                                   "train_steps": train_steps, "train_lr": list(lr),
+                                                   "train_loss": rep["final_loss"],
+                                                   "table_ms": table_ms})
+    P, L = res["plain"], res["learned"]
+    setup = L["training_data_ms"] + L["train_ms"] + L["table_ms"]
+    res["time_to_rms_ms"] = {"plain_mc": P["mc_ms"], "learned_mc": L["mc_ms"], "learned_setup": setup,
+                             "plain_with_gmres": P["mc_ms"] + P["gmres_ms"],
+                             "learned_with_setup_and_gmres": setup + L["mc_ms"] + L["gmres_ms"]}
+    res["row_variance_ratio_at_probe"] = (P["probe_row_residual_rms"] / L["probe_row_residual_rms"]) ** 2
+    res["speedup"] = {"mc_only": P["mc_ms"] / L["mc_ms"],
+                      "including_training": P["mc_ms"] / (setup + L["mc_ms"]),
+                      "end_to_end_with_gmres": (P["mc_ms"] + P["gmres_ms"]) / (setup + L["mc_ms"] + L["gmres_ms"])}
+    mesh.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,12 +334,17 @@ def main():
     ap.add_argument("--workload", default="rows", choices=["rows", "points"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-RMS comparison (config 5)")
+    ap.add_argument("--train-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by node batches, "
-              "see the sharding note in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by node batches "
+              "(rows of one circle stay together), one GPU per share", file=sys.stderr)
         sys.exit(2)
     out = bench_rows(args) if args.workload == "rows" else bench_points(args)
+    if not args.no_ttr:
+        log("time to RMS 1e-3 (config 5) ...")
+        out["time_to_rms"] = bench_ttr(train_steps=args.train_steps)
     print(json.dumps(out), flush=True)
 
 

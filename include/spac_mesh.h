@@ -80,12 +80,25 @@ typedef struct {
     double residual;            /* final ||B - G u|| / ||B|| */
     int32_t converged;
     int32_t reserved;
+    double seconds;             /* wall time of the call */
+    double device_ms;           /* GPU solver: kernel time on its stream (HIP events); host solver: 0 */
 } spac_gmres_info;
 
 /* Restarted FP64 GMRES (modified Gram–Schmidt, Givens rotations) with a
  * Jacobi preconditioner on the CSR system; u holds the initial guess on entry. */
 spac_status spac_gmres_solve(int32_t n, const int32_t* G_i, const int32_t* G_j, const double* G_ij, const double* B,
                              double* u, double tol, int32_t max_iter, int32_t restart, spac_gmres_info* info);
+
+/* The same GMRES(m) with right Jacobi preconditioning on the GPU
+ * (SistemaLinealSolverGMRES::Resuelve, Solvers/SistemaLinealSolverGMRES_OMP.hpp:519-577,
+ * without its RAS preconditioner): FP64 CSR SpMV (one wavefront per row),
+ * classical Gram–Schmidt applied twice (CGS2) for the Arnoldi basis, every
+ * reduction in a fixed order; the (m+1) x m Hessenberg least-squares problem
+ * and its Givens rotations on the host.  Host arrays in and out; u holds the
+ * initial guess on entry. */
+spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* G_i, const int32_t* G_j, const double* G_ij,
+                                 const double* B, double* u, double tol, int32_t max_iter, int32_t restart,
+                                 spac_gmres_info* info);
 /* Read a COO file ("nnz n" then "i j v") into CSR (Lee_GB_COO). Call with
  * G_i == NULL to get n and nnz first. */
 spac_status spac_read_coo(const char* path, int32_t* n, int32_t* nnz, int32_t* G_i, int32_t* G_j, double* G_ij);

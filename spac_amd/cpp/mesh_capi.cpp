@@ -2,6 +2,7 @@
  * mesh_capi.cpp — C ABI of include/spac_mesh.h over Malla / Interfaz /
  * SistemaLineal, and the host FP64 GMRES.
  */
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -265,6 +266,7 @@ static double dot(int n, const double* a, const double* b) {
 spac_status spac_gmres_solve(int32_t n, const int32_t* Ip, const int32_t* J, const double* V, const double* B, double* u,
                              double tol, int32_t max_iter, int32_t restart, spac_gmres_info* info) {
     if (n < 1 || !Ip || !J || !V || !B || !u) return set_error(SPAC_E_INVALID_ARG, "bad GMRES arguments");
+    const auto t_start = std::chrono::steady_clock::now();
     if (restart < 1) restart = 50;
     if (max_iter < 1) max_iter = 1000;
     std::vector<double> dinv(n, 1.0);
@@ -342,6 +344,9 @@ spac_status spac_gmres_solve(int32_t n, const int32_t* Ip, const int32_t* J, con
         info->restarts = restarts;
         info->residual = res;
         info->converged = conv ? 1 : 0;
+        info->reserved = 0;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        info->device_ms = 0.0;
     }
     return SPAC_OK;
 }

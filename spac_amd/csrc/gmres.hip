@@ -1,0 +1,334 @@
+/*
+ * gmres.hip — restarted GMRES(m) with right Jacobi preconditioning on the GPU,
+ * for the assembled interface system G u = B (the reference solves it with
+ * SistemaLinealSolverGMRES, Solvers/SistemaLinealSolverGMRES_OMP.hpp:519-577).
+ *
+ * Device work per Arnoldi step: z = D^-1 v_k fused into a CSR SpMV (one
+ * wavefront per row), classical Gram–Schmidt applied twice (CGS2: block dot
+ * products of w against all basis vectors, then one fused update), the norm
+ * and the scaling of the new basis vector.  The host keeps the (m+1) x m
+ * Hessenberg matrix, applies the Givens rotations and tests convergence, as
+ * the host solver does.  Every reduction has a fixed order (per-block tree,
+ * blocks summed in index order), so a solve is bit-reproducible.
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "../../include/spac_mesh.h"
+#include "fk_internal.h"
+
+namespace spac {
+
+#define GM_CHECK(expr)                                                                         \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            st.release();                                                                      \
+            return set_error(SPAC_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                             __FILE__, __LINE__);                                              \
+        }                                                                                      \
+    } while (0)
+
+constexpr int kDotBlock = 256;
+constexpr int kDotPerThread = 8; /* rows per thread in the dot kernels */
+constexpr int kDotChunk = kDotBlock * kDotPerThread;
+
+/* inverse diagonal (1 where the diagonal is missing or zero) */
+__global__ void k_diag_inv(int n, const int32_t* Ip, const int32_t* J, const double* V, double* dinv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 1.0;
+    for (int k = Ip[i]; k < Ip[i + 1]; ++k)
+        if (J[k] == i && V[k] != 0.0) d = 1.0 / V[k];
+    dinv[i] = d;
+}
+
+/* y = A (s .* x) (s = nullptr: y = A x); one wavefront per row, fixed lane order. */
+__global__ void __launch_bounds__(256) k_spmv(int n, const int32_t* __restrict__ Ip, const int32_t* __restrict__ J,
+                                              const double* __restrict__ V, const double* __restrict__ s,
+                                              const double* __restrict__ x, double* __restrict__ y) {
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int k = Ip[row] + lane; k < Ip[row + 1]; k += 64) {
+        const int c = J[k];
+        const double xc = s ? s[c] * x[c] : x[c];
+        acc = fma(V[k], xc, acc);
+    }
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) y[row] = acc;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    return t;
+}
+
+/* part[j][chunk] = sum over the chunk's rows of Vb[j][i] w[i], j = 0..nv-1 (grid: chunks x nv) */
+__global__ void __launch_bounds__(kDotBlock) k_multidot(int n, const double* __restrict__ Vb, size_t ld,
+                                                        const double* __restrict__ w, double* __restrict__ part,
+                                                        int nchunks) {
+    __shared__ double sh[kDotBlock / 64];
+    const int chunk = blockIdx.x, j = blockIdx.y;
+    const double* v = Vb + (size_t)j * ld;
+    double acc = 0.0;
+    const int base = chunk * kDotChunk + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < kDotPerThread; ++r) {
+        const int i = base + r * kDotBlock;
+        if (i < n) acc = fma(v[i], w[i], acc);
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(size_t)j * nchunks + chunk] = t;
+}
+
+/* h[j] (+)= sum over chunks in order of part[j][chunk]; one wavefront per j */
+__global__ void k_finish_dots(int nv, int nchunks, const double* __restrict__ part, double* __restrict__ h, int add) {
+    const int j = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (j >= nv) return;
+    double acc = 0.0;
+    for (int c = lane; c < nchunks; c += 64) acc += part[(size_t)j * nchunks + c];
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) h[j] = add ? h[j] + acc : acc;
+}
+
+/* w[i] -= sum_j h[j] Vb[j][i] (j = 0..nv-1), h staged in LDS */
+__global__ void __launch_bounds__(256) k_update(int n, const double* __restrict__ Vb, size_t ld,
+                                                const double* __restrict__ hdev, int nv, double* __restrict__ w) {
+    __shared__ double hs[1024];
+    for (int j = threadIdx.x; j < nv; j += blockDim.x) hs[j] = hdev[j];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < nv; ++j) s = fma(hs[j], Vb[(size_t)j * ld + i], s);
+    w[i] -= s;
+}
+
+/* out = a * x  (scalar from device memory: out = x / *den when den is given) */
+__global__ void k_scale(int n, const double* __restrict__ x, const double* __restrict__ den, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = x[i] / den[0];
+}
+
+/* r = B - r */
+__global__ void k_residual(int n, const double* __restrict__ B, double* __restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = B[i] - r[i];
+}
+
+/* u[i] += dinv[i] sum_j y[j] Vb[j][i] */
+__global__ void __launch_bounds__(256) k_correct(int n, const double* __restrict__ Vb, size_t ld,
+                                                 const double* __restrict__ y, int k, const double* __restrict__ dinv,
+                                                 double* __restrict__ u) {
+    __shared__ double ys[1024];
+    for (int j = threadIdx.x; j < k; j += blockDim.x) ys[j] = y[j];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s = fma(ys[j], Vb[(size_t)j * ld + i], s);
+    u[i] += dinv[i] * s;
+}
+
+/* sqrt of a dot result, in place (h[0] = ||.||) */
+__global__ void k_sqrt(double* h) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) h[0] = sqrt(h[0]);
+}
+
+struct GmresState {
+    DeviceBuffer Ip, J, V, B, u, dinv, Vb, w, r, part, h, nrm;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    void release() {
+        for (DeviceBuffer* b : {&Ip, &J, &V, &B, &u, &dinv, &Vb, &w, &r, &part, &h, &nrm}) b->release();
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+        e0 = e1 = nullptr;
+        stream = nullptr;
+    }
+};
+
+} // namespace spac
+
+using namespace spac;
+
+extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const int32_t* J, const double* V,
+                                            const double* B, double* u, double tol, int32_t max_iter,
+                                            int32_t restart, spac_gmres_info* info) {
+    if (n < 1 || !Ip || !J || !V || !B || !u) return set_error(SPAC_E_INVALID_ARG, "bad GMRES arguments");
+    if (restart < 1) restart = 50;
+    if (restart > 1000) return set_error(SPAC_E_INVALID_ARG, "restart must be <= 1000");
+    if (max_iter < 1) max_iter = 1000;
+    if (Ip[0] != 0 || Ip[n] < 0) return set_error(SPAC_E_INVALID_ARG, "CSR row pointer must start at 0");
+    for (int32_t i = 0; i < n; ++i)
+        if (Ip[i + 1] < Ip[i]) return set_error(SPAC_E_INVALID_ARG, "CSR row pointer decreases at row %d", i);
+    const int64_t nnz = Ip[n];
+    for (int64_t k = 0; k < nnz; ++k)
+        if (J[k] < 0 || J[k] >= n) return set_error(SPAC_E_INVALID_ARG, "CSR column %d out of range at %lld", J[k], (long long)k);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(SPAC_E_NO_DEVICE, "no HIP device visible");
+    const auto t_start = std::chrono::steady_clock::now();
+    const int m = restart;
+    const size_t N = (size_t)n;
+    const int nchunks = (int)((N + kDotChunk - 1) / kDotChunk);
+    /* host-visible column block: h1[m+1] | h2[m+1] | ||w|| */
+    const size_t hlen = 2 * (size_t)(m + 1) + 1;
+    GmresState st;
+    spac_status rc;
+    if ((rc = st.Ip.ensure((N + 1) * 4)) || (rc = st.J.ensure((size_t)nnz * 4 + 4)) ||
+        (rc = st.V.ensure((size_t)nnz * 8 + 8)) || (rc = st.B.ensure(N * 8)) || (rc = st.u.ensure(N * 8)) ||
+        (rc = st.dinv.ensure(N * 8)) || (rc = st.Vb.ensure((size_t)(m + 1) * N * 8)) || (rc = st.w.ensure(N * 8)) ||
+        (rc = st.r.ensure(N * 8)) || (rc = st.part.ensure((size_t)(m + 1) * nchunks * 8)) ||
+        (rc = st.h.ensure(hlen * 8)) || (rc = st.nrm.ensure(64))) {
+        st.release();
+        return rc;
+    }
+    GM_CHECK(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+    GM_CHECK(hipEventCreate(&st.e0));
+    GM_CHECK(hipEventCreate(&st.e1));
+    hipStream_t s = st.stream;
+    GM_CHECK(hipMemcpyAsync(st.Ip.ptr, Ip, (N + 1) * 4, hipMemcpyHostToDevice, s));
+    GM_CHECK(hipMemcpyAsync(st.J.ptr, J, (size_t)nnz * 4, hipMemcpyHostToDevice, s));
+    GM_CHECK(hipMemcpyAsync(st.V.ptr, V, (size_t)nnz * 8, hipMemcpyHostToDevice, s));
+    GM_CHECK(hipMemcpyAsync(st.B.ptr, B, N * 8, hipMemcpyHostToDevice, s));
+    GM_CHECK(hipMemcpyAsync(st.u.ptr, u, N * 8, hipMemcpyHostToDevice, s));
+    GM_CHECK(hipEventRecord(st.e0, s));
+
+    const int32_t* dIp = (const int32_t*)st.Ip.ptr;
+    const int32_t* dJ = (const int32_t*)st.J.ptr;
+    const double* dV = (const double*)st.V.ptr;
+    const double* dB = (const double*)st.B.ptr;
+    double* du = (double*)st.u.ptr;
+    double* dinv = (double*)st.dinv.ptr;
+    double* Vb = (double*)st.Vb.ptr;
+    double* w = (double*)st.w.ptr;
+    double* r = (double*)st.r.ptr;
+    double* part = (double*)st.part.ptr;
+    double* h1 = (double*)st.h.ptr;
+    double* h2 = h1 + (m + 1);
+    double* hw = h2 + (m + 1);
+    double* dn = (double*)st.nrm.ptr;
+    const dim3 g1((n + 255) / 256), b1(256), gw((n + 3) / 4);
+
+    auto dots = [&](const double* vecs, int nv, const double* x, double* out) {
+        hipLaunchKernelGGL(k_multidot, dim3(nchunks, nv), dim3(kDotBlock), 0, s, n, vecs, N, x, part, nchunks);
+        hipLaunchKernelGGL(k_finish_dots, dim3(nv), dim3(64), 0, s, nv, nchunks, (const double*)part, out, 0);
+    };
+    auto norm = [&](const double* x, double* out) {
+        dots(x, 1, x, out);
+        hipLaunchKernelGGL(k_sqrt, dim3(1), dim3(64), 0, s, out);
+    };
+
+    hipLaunchKernelGGL(k_diag_inv, g1, b1, 0, s, n, dIp, dJ, dV, dinv);
+    norm(dB, dn);
+    double bnorm = 0.0;
+    GM_CHECK(hipMemcpyAsync(&bnorm, dn, 8, hipMemcpyDeviceToHost, s));
+    GM_CHECK(hipStreamSynchronize(s));
+    if (bnorm == 0.0) bnorm = 1.0;
+
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc(hlen), yv(m);
+    int it = 0, restarts = 0;
+    double res = 0.0;
+    bool conv = false;
+    auto true_residual = [&](double* out) -> spac_status {
+        hipLaunchKernelGGL(k_spmv, gw, b1, 0, s, n, dIp, dJ, dV, (const double*)nullptr, (const double*)du, r);
+        hipLaunchKernelGGL(k_residual, g1, b1, 0, s, n, dB, r);
+        norm(r, dn);
+        GM_CHECK(hipMemcpyAsync(out, dn, 8, hipMemcpyDeviceToHost, s));
+        GM_CHECK(hipStreamSynchronize(s));
+        return SPAC_OK;
+    };
+    while (it < max_iter) {
+        double beta = 0.0;
+        if ((rc = true_residual(&beta))) return rc;
+        res = beta / bnorm;
+        if (res < tol) { conv = true; break; }
+        hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)r, (const double*)dn, Vb);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        for (; k < m && it < max_iter; ++k, ++it) {
+            /* w = A D^-1 v_k, then CGS2 against v_0..v_k */
+            hipLaunchKernelGGL(k_spmv, gw, b1, 0, s, n, dIp, dJ, dV, (const double*)dinv,
+                               (const double*)(Vb + (size_t)k * N), w);
+            const int nv = k + 1;
+            dots(Vb, nv, w, h1);
+            hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, nv, w);
+            dots(Vb, nv, w, h2);
+            hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h2, nv, w);
+            norm(w, hw);
+            hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)w, (const double*)hw,
+                               Vb + (size_t)(k + 1) * N);
+            GM_CHECK(hipGetLastError());
+            GM_CHECK(hipMemcpyAsync(hc.data(), h1, hlen * 8, hipMemcpyDeviceToHost, s));
+            GM_CHECK(hipStreamSynchronize(s));
+            for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = hc[j] + hc[(size_t)(m + 1) + j];
+            const double hn = hc[hlen - 1];
+            H[(size_t)(k + 1) * m + k] = hn;
+            for (int j = 0; j < k; ++j) {
+                double a = H[(size_t)j * m + k], b = H[(size_t)(j + 1) * m + k];
+                H[(size_t)j * m + k] = cs[j] * a + sn[j] * b;
+                H[(size_t)(j + 1) * m + k] = -sn[j] * a + cs[j] * b;
+            }
+            double a = H[(size_t)k * m + k], b = H[(size_t)(k + 1) * m + k];
+            double rr = std::hypot(a, b);
+            cs[k] = rr > 0 ? a / rr : 1.0;
+            sn[k] = rr > 0 ? b / rr : 0.0;
+            H[(size_t)k * m + k] = rr;
+            H[(size_t)(k + 1) * m + k] = 0.0;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            res = std::fabs(g[k + 1]) / bnorm;
+            if (res < tol || hn == 0.0) { ++k; ++it; break; }
+        }
+        /* y from the triangular system, u += D^-1 V y */
+        for (int i = k - 1; i >= 0; --i) {
+            double sum = g[i];
+            for (int j = i + 1; j < k; ++j) sum -= H[(size_t)i * m + j] * yv[j];
+            yv[i] = sum / H[(size_t)i * m + i];
+        }
+        if (k > 0) {
+            GM_CHECK(hipMemcpyAsync(h1, yv.data(), (size_t)k * 8, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_correct, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, k,
+                               (const double*)dinv, du);
+        }
+        restarts++;
+        if (res < tol) {
+            double rn = 0.0;
+            if ((rc = true_residual(&rn))) return rc;
+            res = rn / bnorm;
+            conv = res < tol * 10.0;
+            if (res < tol) break;
+        }
+    }
+    GM_CHECK(hipEventRecord(st.e1, s));
+    GM_CHECK(hipMemcpyAsync(u, du, N * 8, hipMemcpyDeviceToHost, s));
+    GM_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    GM_CHECK(hipEventElapsedTime(&ms, st.e0, st.e1));
+    if (info) {
+        info->iterations = it;
+        info->restarts = restarts;
+        info->residual = res;
+        info->converged = conv ? 1 : 0;
+        info->reserved = 0;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        info->device_ms = ms;
+    }
+    st.release();
+    return SPAC_OK;
+}

@@ -45,7 +45,7 @@ constexpr int oW1 = 0, ob1 = oW1 + kH * 2, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, 
 static_assert(kNParams == SPAC_MLP_N_PARAMS, "parameter count");
 
 /* ------------------------------------------------------------ training -- */
-constexpr int kGroup = 16; /* outputs per thread in the dense kernels */
+constexpr int kGroup = 8;  /* outputs per thread in the dense kernels */
 constexpr int kChunk = 32; /* samples per weight-gradient partial sum */
 
 /* z = 2 (x - lo) / (hi - lo) - 1, once per batch */
@@ -204,9 +204,13 @@ __global__ void __launch_bounds__(256) k_wgrad_partial(WgradArgs a) {
     }
 }
 
-/* grad[k] = sum over chunks (in order) of part[chunk][k]; then, in training,
- * one Adam step with a geometric learning-rate decay.  The step counter lives
- * in device memory so a captured graph of steps replays unchanged. */
+/* grad[k] = sum over chunks of part[chunk][k] in a fixed order (8 interleaved
+ * groups of chunks, each summed in index order, then the groups in order);
+ * then, in training, one Adam step with a geometric learning-rate decay.  The
+ * step counter lives in device memory so that a captured graph of steps
+ * replays unchanged: every block reads it, and the last block to finish
+ * advances it. */
+constexpr int kRedParams = 32, kRedGroups = 8;
 struct AdamArgs {
     float* p;
     float* g;
@@ -214,36 +218,54 @@ struct AdamArgs {
     float* v;
     const float* part;
     int chunks;
-    const int* ctr; /* Adam steps taken so far */
-    int t0;         /* counter value at the first step of this schedule */
+    int* ctr;           /* [0]: Adam steps taken so far; [1]: blocks finished in this step */
+    int t0;             /* counter value at the first step of this schedule */
     int steps;
-    float lr0, lr1; /* lr1 <= 0: constant lr0 */
+    float lr0, lr1;     /* lr1 <= 0: constant lr0 */
     int adam;
 };
-__global__ void k_reduce_adam(AdamArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= kNParams) return;
-    float g = 0.0f;
-    for (int c = 0; c < a.chunks; ++c) g += a.part[(size_t)c * kNParams + k];
-    a.g[k] = g;
+__global__ void __launch_bounds__(kRedParams * kRedGroups) k_reduce_adam(AdamArgs a) {
+    __shared__ float sh[kRedGroups][kRedParams];
+    const int pl = threadIdx.x % kRedParams, grp = threadIdx.x / kRedParams;
+    const int k = blockIdx.x * kRedParams + pl;
+    float acc = 0.0f;
+    if (k < kNParams)
+        for (int c = grp; c < a.chunks; c += kRedGroups) acc += a.part[(size_t)c * kNParams + k];
+    sh[grp][pl] = acc;
+    __syncthreads();
+    if (grp == 0 && k < kNParams) {
+        float g = 0.0f;
+        for (int q = 0; q < kRedGroups; ++q) g += sh[q][pl];
+        a.g[k] = g;
+        if (a.adam) {
+            const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+            const int tp = a.ctr[0];
+            const float t = (float)(tp + 1);
+            float lr = a.lr0;
+            if (a.lr1 > 0.0f && a.steps > 1)
+                lr = a.lr0 * powf(a.lr1 / a.lr0, (float)(tp - a.t0) / (float)(a.steps - 1));
+            const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
+            const float mk = b1 * a.m[k] + (1.0f - b1) * g;
+            const float vk = b2 * a.v[k] + (1.0f - b2) * g * g;
+            a.m[k] = mk;
+            a.v[k] = vk;
+            a.p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+        }
+    }
     if (!a.adam) return;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const int tp = a.ctr[0];
-    const float t = (float)(tp + 1);
-    float lr = a.lr0;
-    if (a.lr1 > 0.0f && a.steps > 1)
-        lr = a.lr0 * powf(a.lr1 / a.lr0, (float)(tp - a.t0) / (float)(a.steps - 1));
-    const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
-    const float mk = b1 * a.m[k] + (1.0f - b1) * g;
-    const float vk = b2 * a.v[k] + (1.0f - b2) * g * g;
-    a.m[k] = mk;
-    a.v[k] = vk;
-    a.p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    /* every block has read ctr[0] before it counts itself finished */
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int done = atomicAdd(&a.ctr[1], 1);
+        if (done == (int)gridDim.x - 1) {
+            a.ctr[0] = a.ctr[0] + 1;
+            a.ctr[1] = 0;
+            __threadfence();
+        }
+    }
 }
 
-__global__ void k_tick(int* ctr) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] += 1;
-}
 
 /* ----------------------------------------------------- MFMA inference -- */
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -450,14 +472,14 @@ static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float 
     a.v = (float*)m->av.ptr;
     a.part = (const float*)m->part.ptr;
     a.chunks = n_chunks(n);
-    a.ctr = (const int*)m->ctr.ptr;
+    a.ctr = (int*)m->ctr.ptr;
     a.t0 = t0;
     a.steps = steps;
     a.lr0 = lr0;
     a.lr1 = lr1;
     a.adam = adam ? 1 : 0;
-    hipLaunchKernelGGL(k_reduce_adam, dim3((kNParams + 255) / 256), dim3(256), 0, s, a);
-    if (adam) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, (int*)m->ctr.ptr);
+    hipLaunchKernelGGL(k_reduce_adam, dim3((kNParams + kRedParams - 1) / kRedParams), dim3(kRedParams * kRedGroups),
+                       0, s, a);
 }
 
 static spac_status mlp_loss_host(Mlp* m, double* loss) {

@@ -49,6 +49,8 @@ class GmresInfo(C.Structure):
         ("residual", C.c_double),
         ("converged", C.c_int32),
         ("reserved", C.c_int32),
+        ("seconds", C.c_double),
+        ("device_ms", C.c_double),
     ]
 
 
@@ -71,6 +73,7 @@ def _setup(lib):
     lib.spac_mesh_write_positions.argtypes = [vp, C.c_int32, C.c_char_p]
     lib.spac_gmres_solve.argtypes = [C.c_int32, abi.i32p, abi.i32p, abi.dp, abi.dp, abi.dp, C.c_double, C.c_int32,
                                      C.c_int32, C.POINTER(GmresInfo)]
+    lib.spac_gmres_solve_gpu.argtypes = lib.spac_gmres_solve.argtypes
     lib.spac_read_coo.argtypes = [C.c_char_p, abi.i32p, abi.i32p, abi.i32p, abi.i32p, abi.dp]
     lib._mesh_ready = True
     return lib
@@ -210,12 +213,14 @@ class Mesh:
         return Gi, Gj, Gv, B
 
 
-def gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=2000, restart=100, u0=None):
+def gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=2000, restart=100, u0=None, device=False):
+    """spac_gmres_solve (host) or, with device=True, spac_gmres_solve_gpu."""
     lib = _setup(abi.load())
     n = B.size
     u = np.zeros(n) if u0 is None else np.array(u0, dtype=np.float64)
     info = GmresInfo()
-    abi.check(lib.spac_gmres_solve(n, abi.ptr(np.ascontiguousarray(Gi, np.int32), C.c_int32),
+    fn = lib.spac_gmres_solve_gpu if device else lib.spac_gmres_solve
+    abi.check(fn(n, abi.ptr(np.ascontiguousarray(Gi, np.int32), C.c_int32),
                                    abi.ptr(np.ascontiguousarray(Gj, np.int32), C.c_int32),
                                    abi.ptr(np.ascontiguousarray(Gv, np.float64), C.c_double),
                                    abi.ptr(np.ascontiguousarray(B, np.float64), C.c_double),
