@@ -18,7 +18,7 @@
  *   - exp/log/sin/cos are the project's own FP64 functions (spac_math.h) so
  *     that CPU and GPU bits agree;
  *   - the estimator sums follow one fixed compensated reduction order
- *     (tiles of 64 paths folded pairwise, tiles added in order) instead of
+ *     (64 lane sums over strided paths, folded pairwise) instead of
  *     per-thread partial sums merged under omp critical (:217-228, 276-290);
  *   - xi and xixi are the true moments (the reference's enum alias puts
  *     both into sums[8], Solvers/FeynmanKacSolver.hpp:16-23);
@@ -444,9 +444,9 @@ int oracle_simulate(const spac_problem_desc* desc, const spac_solve_batch* batch
 
 /* ------------------------------------------------------ Reduce_Analytic --
  * Solvers/FeynmanKacSolver.cpp:198-231 (point) and 233-292 (node) with the
- * fixed compensated order: tiles of 64 consecutive paths, each folded
- * pairwise (lane i += lane i+off for off = 32, 16, ..., 1) with TwoSum
- * error terms, tiles accumulated in order.  Update(): :294-334. */
+ * fixed compensated order: 64 lane sums (lane l takes paths l, l+64, ... in
+ * order) with TwoSum error terms, folded pairwise (lane i += lane i+off for
+ * off = 32, 16, ..., 1).  Update(): :294-334. */
 #define NSUMS 10
 enum { S_SCORE, S_SCORE2, S_XI, S_XI2, S_VR, S_VR2, S_XISCORE, S_RNG, S_TAU, S_B };
 
@@ -471,31 +471,26 @@ static void reduce_one(int node_mode, int64_t np, const double* Y, const double*
                        const double* tau, const double* gval, const int32_t* steps, const int8_t* outcome,
                        spac_point_stats* st) {
     if (np == 0) { memset(st, 0, sizeof *st); return; }
-    spac_ksum tot[NSUMS];
-    for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_zero();
-    int64_t ntile = (np + 63) / 64;
+    /* lane l (0..63) sums paths l, l + 64, ... in order, then the lane sums
+     * are folded pairwise: lane i += lane i + off for off = 32, 16, ..., 1 */
+    spac_ksum lane[NSUMS][64];
+    for (int k = 0; k < NSUMS; ++k)
+        for (int l = 0; l < 64; ++l) lane[k][l] = spac_ksum_zero();
     int64_t total_steps = 0, nmax = 0, ndom = 0;
     int has_nan = 0;
-    for (int64_t t = 0; t < ntile; ++t) {
-        spac_ksum lane[NSUMS][64];
-        for (int l = 0; l < 64; ++l) {
-            int64_t p = t * 64 + l;
-            double v[NSUMS];
-            if (p < np) {
-                oracle_path_terms(node_mode, Y[p], Z[p], xi[p], tau[p], gval[p], steps[p], outcome[p], v);
-                total_steps += steps[p];
-                if (outcome[p] == SPAC_OUT_MAX_STEPS) nmax++;
-                if (outcome[p] != SPAC_OUT_SUBDOMAIN) ndom++;
-            } else {
-                for (int k = 0; k < NSUMS; ++k) v[k] = 0.0;
-            }
-            for (int k = 0; k < NSUMS; ++k) { lane[k][l].s = v[k]; lane[k][l].c = 0.0; }
-        }
-        for (int off = 32; off >= 1; off >>= 1)
-            for (int l = 0; l < off; ++l)
-                for (int k = 0; k < NSUMS; ++k) lane[k][l] = spac_ksum_merge(lane[k][l], lane[k][l + off]);
-        for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_merge(tot[k], lane[k][0]);
+    for (int64_t p = 0; p < np; ++p) {
+        double v[NSUMS];
+        oracle_path_terms(node_mode, Y[p], Z[p], xi[p], tau[p], gval[p], steps[p], outcome[p], v);
+        total_steps += steps[p];
+        if (outcome[p] == SPAC_OUT_MAX_STEPS) nmax++;
+        if (outcome[p] != SPAC_OUT_SUBDOMAIN) ndom++;
+        for (int k = 0; k < NSUMS; ++k) lane[k][p % 64] = spac_ksum_add(lane[k][p % 64], v[k]);
     }
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int l = 0; l < off; ++l)
+            for (int k = 0; k < NSUMS; ++k) lane[k][l] = spac_ksum_merge(lane[k][l], lane[k][l + off]);
+    spac_ksum tot[NSUMS];
+    for (int k = 0; k < NSUMS; ++k) tot[k] = lane[k][0];
     double N = (double)np;
     double s[NSUMS];
     for (int k = 0; k < NSUMS; ++k) { s[k] = spac_ksum_value(tot[k]); if (s[k] != s[k]) has_nan = 1; }
@@ -663,12 +658,18 @@ int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_tab
         int64_t np = paths[k];
         int stencil = t->rbf_kind == SPAC_RBF_GAUSS_STENCIL;
         int S = stencil ? (t->stencil_size < M ? t->stencil_size : M) : 0;
-        spac_ksum* acc = (spac_ksum*)malloc(sizeof(spac_ksum) * (size_t)M);
+        /* Column sums in the fixed order of the GPU: paths in chunks of 256,
+         * partial w holds paths 64w..64w+63 of every chunk in path order, and
+         * the four partials are merged in order. */
+        spac_ksum* part = (spac_ksum*)malloc(sizeof(spac_ksum) * 4 * (size_t)M);
+        spac_ksum* acc4[4];
+        for (int w = 0; w < 4; ++w) acc4[w] = part + (size_t)w * M;
         double* dsum = (double*)calloc((size_t)M, sizeof(double));
-        for (int j = 0; j < M; ++j) acc[j] = spac_ksum_zero();
+        for (int j = 0; j < 4 * M; ++j) part[j] = spac_ksum_zero();
         for (int64_t p = 0; p < np; ++p) {
             int64_t g = off[k] + p;
             if (rec.outcome[g] != SPAC_OUT_SUBDOMAIN) continue;
+            spac_ksum* acc = acc4[(p % 256) / 64];
             double theta = 0.0;
             spac_exit_angle(code, rec.x_tau[g] - cx, rec.y_tau[g] - cy, th[0], &theta);
             double Yp = rec.Y[g];
@@ -693,6 +694,9 @@ int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_tab
             }
         }
         double N = (double)np;
+        spac_ksum* acc = acc4[0];
+        for (int j = 0; j < M; ++j)
+            for (int w = 1; w < 4; ++w) acc[j] = spac_ksum_merge(acc[j], acc4[w][j]);
         if (stencil) {
             for (int j = 0; j < M; ++j) Gk[j] = -spac_ksum_value(acc[j]) / N;
         } else if (direct) {
@@ -704,7 +708,7 @@ int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_tab
                 Gk[j] = -w / N;
             }
         }
-        free(acc);
+        free(part);
         free(dsum);
     }
     free(rec.x_tau); free(rec.y_tau); free(rec.Y); free(rec.Z); free(rec.xi); free(rec.tau); free(rec.gval);

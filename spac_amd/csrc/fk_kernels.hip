@@ -12,9 +12,9 @@
  *     shape ran it;
  *   - per-path exit data written to fixed SoA slots (solve offset + path).
  * Reduction kernel (Reduce_Analytic + Update, :198-334): one wave per solve,
- * tiles of 64 paths folded pairwise with TwoSum error terms, tiles added in
- * order — a fixed order, so estimators are bit-stable across launches and
- * equal to the CPU restatement's.
+ * lane l sums paths l, l+64, ... with TwoSum error terms, then the 64 lane
+ * sums are folded pairwise — a fixed order, so estimators are bit-stable
+ * across launches and equal to the CPU restatement's.
  */
 #include <hip/hip_runtime.h>
 
@@ -408,48 +408,40 @@ __global__ void __launch_bounds__(64) fk_reduce_kernel(ReduceArgs a) {
     if (solve >= a.n_solves) return;
     int64_t np = a.paths ? (int64_t)a.paths[solve] : (int64_t)a.paths_uniform;
     int64_t off = a.rec_off ? a.rec_off[solve] : (int64_t)solve * (int64_t)a.paths_uniform;
+    /* lane l sums paths l, l + 64, l + 128, ... in order; the 64 lane sums are
+     * then folded pairwise (lane i += lane i + off, off = 32, 16, ..., 1) */
     spac_ksum tot[NSUMS];
 #pragma unroll
     for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_zero();
     int64_t my_steps = 0, my_max = 0, my_dom = 0;
-    int64_t ntile = (np + 63) / 64;
-    for (int64_t tl = 0; tl < ntile; ++tl) {
-        int64_t p = tl * 64 + lane;
+    for (int64_t p = lane; p < np; p += 64) {
+        const int64_t g = off + p;
+        const double Y = a.Y[g], Z = a.Z[g], xi = a.xi[g], gv = a.gval[g];
+        const int32_t st = a.steps[g];
+        const int8_t oc = a.outcome[g];
+        const double score = Z + Y * gv;
+        const double vr = score + xi;
         double v[NSUMS];
-        if (p < np) {
-            int64_t g = off + p;
-            double Y = a.Y[g], Z = a.Z[g], xi = a.xi[g], gv = a.gval[g];
-            int32_t st = a.steps[g];
-            int8_t oc = a.outcome[g];
-            double score = Z + Y * gv;
-            double vr = score + xi;
-            v[S_SCORE] = score;
-            v[S_SCORE2] = score * score;
-            v[S_XI] = xi;
-            v[S_XI2] = xi * xi;
-            v[S_VR] = vr;
-            v[S_VR2] = vr * vr;
-            v[S_XISCORE] = xi * score;
-            v[S_RNG] = 2.0 * (double)st;
-            v[S_TAU] = a.tau[g];
-            v[S_B] = a.node_mode ? ((oc == SPAC_OUT_SUBDOMAIN) ? (Z + xi) : vr) : 0.0;
-            my_steps += st;
-            my_max += (oc == SPAC_OUT_MAX_STEPS);
-            my_dom += (oc != SPAC_OUT_SUBDOMAIN);
-        } else {
+        v[S_SCORE] = score;
+        v[S_SCORE2] = score * score;
+        v[S_XI] = xi;
+        v[S_XI2] = xi * xi;
+        v[S_VR] = vr;
+        v[S_VR2] = vr * vr;
+        v[S_XISCORE] = xi * score;
+        v[S_RNG] = 2.0 * (double)st;
+        v[S_TAU] = a.tau[g];
+        v[S_B] = a.node_mode ? ((oc == SPAC_OUT_SUBDOMAIN) ? (Z + xi) : vr) : 0.0;
 #pragma unroll
-            for (int k = 0; k < NSUMS; ++k) v[k] = 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < NSUMS; ++k) {
-            spac_ksum acc;
-            acc.s = v[k];
-            acc.c = 0.0;
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) acc = spac_ksum_merge(acc, shfl_down_ksum(acc, o));
-            tot[k] = spac_ksum_merge(tot[k], acc);
-        }
+        for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_add(tot[k], v[k]);
+        my_steps += st;
+        my_max += (oc == SPAC_OUT_MAX_STEPS);
+        my_dom += (oc != SPAC_OUT_SUBDOMAIN);
     }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_merge(tot[k], shfl_down_ksum(tot[k], o));
     /* exact integer totals */
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {

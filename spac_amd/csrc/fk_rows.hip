@@ -14,8 +14,10 @@
  *      S_i = sum_p Y_p phi(theta_p, theta_i), G_j = -sum_i iPsi(i,j) S_i / N;
  *      Gaussian stencils accumulate G_j += Y_p H_j(theta_p) directly for the
  *      stencil_size columns of each path's stencil.  Every column is a
- *      sequential TwoSum-compensated sum over the node's paths in path order,
- *      so the result does not depend on the launch and equals the CPU
+ *      TwoSum-compensated sum in one fixed order: paths go in chunks of 256,
+ *      wave w of the block owns paths 64w..64w+63 of every chunk and sums them
+ *      in path order, and the four wave partials are merged in wave order.
+ *      The result does not depend on the launch and equals the CPU
  *      restatement's.
  */
 #include <hip/hip_runtime.h>
@@ -40,8 +42,10 @@ namespace spac {
     } while (0)
 
 constexpr int kRowsBlock = 256;
+constexpr int kRowsWaves = kRowsBlock / 64;
 constexpr int kMaxStencil = 16;
-constexpr int kMaxColsPerThread = 4; /* interfaces up to 1024 nodes */
+constexpr int kColGroups = 4;             /* columns per lane */
+constexpr int kMaxCols = 64 * kColGroups; /* interfaces up to 256 nodes */
 constexpr int kWStride = kMaxStencil + 1; /* odd stride: conflict-free LDS rows */
 
 struct RowsArgs {
@@ -83,22 +87,24 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
     const double* ipsi = a.ipsi + (a.ipsi_offset[iface] >= 0 ? a.ipsi_offset[iface] : 0);
     double* G = a.G + a.g_off[job];
 
-    __shared__ double s_th[kMaxColsPerThread * kRowsBlock];
+    __shared__ double s_th[kMaxCols];
     __shared__ double s_Y[kRowsBlock];
     __shared__ double s_t[kRowsBlock];           /* exit angle (IMQ) */
     __shared__ __align__(16) int s_s[kRowsBlock]; /* stencil start, -1: not a circle exit */
     __shared__ double s_w[kRowsBlock * kWStride]; /* per-path stencil weights H_{s+r}(theta) */
-    __shared__ double s_S[kMaxColsPerThread * kRowsBlock];
+    __shared__ spac_ksum s_part[(kRowsWaves - 1) * kMaxCols];
+    __shared__ double s_S[kMaxCols];
 
+    const int lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < M; i += kRowsBlock) s_th[i] = th[i];
     __syncthreads();
     if (np == 0) {
         for (int j = tid; j < M; j += kRowsBlock) G[j] = 0.0;
         return;
     }
-    spac_ksum acc[kMaxColsPerThread];
+    spac_ksum acc[kColGroups];
 #pragma unroll
-    for (int q = 0; q < kMaxColsPerThread; ++q) acc[q] = spac_ksum_zero();
+    for (int q = 0; q < kColGroups; ++q) acc[q] = spac_ksum_zero();
     const int64_t off = a.rec_off[job];
     for (int64_t base = 0; base < np; base += kRowsBlock) {
         const int cnt = (np - base < (int64_t)kRowsBlock) ? (int)(np - base) : kRowsBlock;
@@ -128,58 +134,66 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
             s_s[tid] = s;
         }
         __syncthreads();
-        /* Paths in order, 4 at a time, without branches: a path that does not
-         * reach column j adds +0.0.  TwoSum with +0.0 leaves s unchanged and can
-         * only turn a -0.0 error term into +0.0, which never changes s + c, so the
-         * column values equal those of skipping the path (the CPU restatement). */
-        if ((tid & ~63) < M) {
-            for (int p = 0; p < cnt; p += 4) {
-                const int4 s4 = *reinterpret_cast<const int4*>(&s_s[p]);
-                int sv[4] = {s4.x, s4.y, s4.z, s4.w};
-                double yv[4], tv[4];
+        /* Wave w sums paths 64w..64w+63 of the chunk in order, 4 at a time and
+         * without branches: a path that does not reach column j adds +0.0.
+         * TwoSum with +0.0 leaves s unchanged and can only turn a -0.0 error term
+         * into +0.0, which never changes s + c, so the column values equal those
+         * of skipping the path (the CPU restatement). */
+        const int p0 = wave * 64, p1 = min(cnt, p0 + 64);
+        for (int p = p0; p < p1; p += 4) {
+            const int4 s4 = *reinterpret_cast<const int4*>(&s_s[p]);
+            int sv[4] = {s4.x, s4.y, s4.z, s4.w};
+            double yv[4], tv[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    if (p + k >= cnt) sv[k] = -1;
-                    yv[k] = s_Y[p + k];
-                    tv[k] = s_t[p + k];
-                }
+            for (int k = 0; k < 4; ++k) {
+                if (p + k >= p1) sv[k] = -1;
+                yv[k] = s_Y[p + k];
+                tv[k] = s_t[p + k];
+            }
 #pragma unroll
-                for (int q = 0; q < kMaxColsPerThread; ++q) {
-                    const int j = tid + q * kRowsBlock;
-                    if (q * kRowsBlock >= M) break; /* uniform */
-                    if (stencil) {
+            for (int q = 0; q < kColGroups; ++q) {
+                const int j = lane + q * 64;
+                if (q * 64 >= M) break; /* uniform */
+                if (stencil) {
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            int rel = j - sv[k];
-                            if (code == 1 && rel < 0) rel += M;
-                            const bool ok = sv[k] >= 0 && j < M && rel >= 0 && rel < S;
-                            const double w = s_w[(p + k) * kWStride + (ok ? rel : 0)];
-                            acc[q] = spac_ksum_add(acc[q], ok ? yv[k] * w : 0.0);
-                        }
-                    } else if (j < M) {
-#pragma unroll
-                        for (int k = 0; k < 4; ++k)
-                            if (sv[k] >= 0) acc[q] = spac_ksum_add(acc[q], yv[k] * spac_imq(tv[k], s_th[j], c2));
+                    for (int k = 0; k < 4; ++k) {
+                        int rel = j - sv[k];
+                        if (code == 1 && rel < 0) rel += M;
+                        const bool ok = sv[k] >= 0 && j < M && rel >= 0 && rel < S;
+                        const double w = s_w[(p + k) * kWStride + (ok ? rel : 0)];
+                        acc[q] = spac_ksum_add(acc[q], ok ? yv[k] * w : 0.0);
                     }
+                } else if (j < M) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (sv[k] >= 0) acc[q] = spac_ksum_add(acc[q], yv[k] * spac_imq(tv[k], s_th[j], c2));
                 }
             }
         }
         __syncthreads();
     }
-    const double N = (double)np;
-    if (stencil) {
+    /* merge the wave partials in wave order */
+    if (wave > 0) {
 #pragma unroll
-        for (int q = 0; q < kMaxColsPerThread; ++q) {
-            const int j = tid + q * kRowsBlock;
-            if (j < M) G[j] = -spac_ksum_value(acc[q]) / N;
+        for (int q = 0; q < kColGroups; ++q) {
+            const int j = lane + q * 64;
+            if (j < M) s_part[(wave - 1) * kMaxCols + j] = acc[q];
         }
-        return;
     }
+    __syncthreads();
+    const double N = (double)np;
+    if (wave == 0) {
 #pragma unroll
-    for (int q = 0; q < kMaxColsPerThread; ++q) {
-        const int j = tid + q * kRowsBlock;
-        if (j < M) s_S[j] = spac_ksum_value(acc[q]);
+        for (int q = 0; q < kColGroups; ++q) {
+            const int j = lane + q * 64;
+            if (j >= M) continue;
+            spac_ksum t = acc[q];
+            for (int w = 1; w < kRowsWaves; ++w) t = spac_ksum_merge(t, s_part[(w - 1) * kMaxCols + j]);
+            if (stencil) G[j] = -spac_ksum_value(t) / N;
+            else s_S[j] = spac_ksum_value(t);
+        }
     }
+    if (stencil) return; /* uniform */
     __syncthreads();
     for (int j = tid; j < M; j += kRowsBlock) {
         double w = 0.0;
@@ -209,9 +223,8 @@ static spac_status validate(const spac_interface_table* t, const spac_node_batch
         return set_error(SPAC_E_INVALID_ARG, "stencil_size must be in [1, %d] and gauss_eps > 0", kMaxStencil);
     for (int k = 0; k < t->n_interfaces; ++k) {
         int M = t->node_offset[k + 1] - t->node_offset[k];
-        if (M < 1 || M > kMaxColsPerThread * kRowsBlock)
-            return set_error(SPAC_E_INVALID_ARG, "interface %d has %d nodes (1..%d supported)", k, M,
-                             kMaxColsPerThread * kRowsBlock);
+        if (M < 1 || M > kMaxCols)
+            return set_error(SPAC_E_INVALID_ARG, "interface %d has %d nodes (1..%d supported)", k, M, kMaxCols);
         int c = t->code[k];
         bool known = c == 1 || c == -1 || c == -2 || c == -3 || c == -4 || c == -11 || c == -22 || c == -33 || c == -44;
         if (!known) return set_error(SPAC_E_BAD_INTERFACE, "interface %d has unknown code %d", k, c);
