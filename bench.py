@@ -215,6 +215,47 @@ def bench_rows(args):
     return out
 
 
+def bench_batched(steps=2, seed=0, fraction=1.0 / 32):
+    """Uneven batch on one GPU: 16x16 circles (Nt = 110, ~25k nodes), per-row paths log-uniform in
+    [1e3, 1e5] and h log-uniform in [1e-4, 1e-3] (numpy default_rng(0)), FP32 stepping; the launch
+    shape and refill order come from the autotuner (on the batch with paths x fraction), then the
+    assembly runs as one captured graph (plan) per step."""
+    from spac_amd.fk import RowsPlan
+
+    mesh = Mesh(mesh_desc(n_side=16, nodes_per_circle=110))
+    n = mesh.info.n_jobs
+    rng = np.random.default_rng(0)
+    paths = np.round(10 ** rng.uniform(3, 5, n)).astype(np.uint32)
+    h = 10 ** rng.uniform(-4, -3, n)
+    live = mesh.job_frontera == 0
+    desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=(0.0, 0.0, 1.0, 1.0), precision=abi.SPAC_FP32)
+    out = {"workload": "16x16 circles (overlap 1.2, 110 nodes per circle), per-row paths log-uniform [1e3, 1e5], "
+                       "h log-uniform [1e-4, 1e-3], FP32 stepping, FP64 compensated sums",
+           "rows": int(n), "nodes": int(mesh.info.n_nodes), "paths_total": int(paths[live].astype(np.int64).sum())}
+    with FKProblem(desc) as prob:
+        t0 = time.perf_counter()
+        rep, table = prob.autotune_rows(mesh, paths, h, path_fraction=fraction)
+        c = rep.chosen
+        out["autotune"] = {"seconds": time.perf_counter() - t0, "candidates": int(rep.n_candidates),
+                           "chosen": {"block_size": c.block_size, "blocks_per_cu": c.blocks_per_cu,
+                                      "chunk_paths": c.chunk_paths, "item_order": c.item_order},
+                           "sample_fraction": fraction, "default_ms": rep.default_ms, "best_ms": rep.best_ms}
+        log(f"batched: autotune {out['autotune']}")
+        with RowsPlan(prob, mesh, paths, h) as plan:
+            plan.execute(seed=seed)  # warm-up
+            ms, steps_total = [], 0
+            for i in range(steps):
+                rows = plan.execute(seed=seed + 1 + i)
+                ms.append(plan.last_ms)
+                steps_total = int(sum(x["path_steps"] for x in rows["stats"]))
+        out["graph_ms"] = float(np.mean(ms))
+        out["path_steps"] = steps_total
+        out["path_steps_per_s"] = steps_total / (np.mean(ms) * 1e-3)
+    log(f"batched: {out}")
+    mesh.close()
+    return out
+
+
 def _ttr_mesh():
     """Config-5 mesh: 64x64 circles on the unit square, 25 nodes per circle (~100k nodes);
     h scaled with the circle radius squared so paths take as many steps as at 4x4 circles."""
@@ -254,7 +295,7 @@ def _replicas(mesh, prob, paths, h, seeds, u0=None):
     return out, rms_u, rms_row, ubar
 
 
-def bench_ttr(target=1e-3, probe_paths=256, train_steps=2000, lr=(1e-2, 1e-4), n_rep=8, seed0=100):
+def bench_ttr(target=1e-3, probe_paths=256, train_steps=2000, lr=(1e-2, 1e-4), n_rep=8, seed0=100, eager=False):
     """Time to a statistical RMS error `target` of the solved interface values, learned control
     variate against plain Monte Carlo, both FP32 stepping on this GPU.
 
@@ -306,7 +347,7 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=2000, lr=(1e-2, 1e-4), n
         data_ms = pp.timing().total_ms
     u_mc = np.array([x["phi"] for x in st])
     with MLP(dom, seed=0) as net, FKProblem(desc) as prob:
-        rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03)
+        rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager)
         table_ms = net.build_table(1024)
         net.attach(prob, fallback=True)
         res["learned"] = measure(prob, "learned", {"training_data_ms": data_ms, "train_ms": rep["train_ms"],
@@ -331,20 +372,34 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rows", choices=["rows", "points"])
+    ap.add_argument("--workload", default="rows", choices=["rows", "points", "batched", "ttr"],
+                    help="rows (default line) | points | batched or ttr alone (for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-RMS comparison (config 5)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven workload (config 3)")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--ttr-replicas", type=int, default=8)
+    ap.add_argument("--train-eager", action="store_true",
+                    help="launch training steps one by one instead of replaying captured graphs (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by node batches "
               "(rows of one circle stay together), one GPU per share", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        print(json.dumps({"batched": bench_batched(steps=args.steps)}), flush=True)
+        return
+    if args.workload == "ttr":
+        print(json.dumps({"time_to_rms": bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, eager=args.train_eager)}), flush=True)
+        return
     out = bench_rows(args) if args.workload == "rows" else bench_points(args)
+    if not args.no_batched:
+        log("batched uneven workload (config 3) ...")
+        out["batched"] = bench_batched()
     if not args.no_ttr:
         log("time to RMS 1e-3 (config 5) ...")
-        out["time_to_rms"] = bench_ttr(train_steps=args.train_steps)
+        out["time_to_rms"] = bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, eager=args.train_eager)
     print(json.dumps(out), flush=True)
 
 

@@ -228,14 +228,54 @@ typedef struct {
 } spac_timing;
 spac_status spac_fk_last_timing(spac_problem* p, spac_timing* out);
 
-/* Launch shape of the persistent path kernel (0 = library default). */
+/* Work-item order of the persistent kernel's queue. */
+enum {
+    SPAC_ORDER_NODE = 0, /* items in node order */
+    SPAC_ORDER_COST = 1  /* items of the nodes with the smallest h (most steps per path) first */
+};
+
+/* Launch shape and refill schedule of the persistent path kernel (0 = library
+ * default; results never depend on it). */
 typedef struct {
     int32_t block_size;       /* threads per block (multiple of 64) */
     int32_t blocks_per_cu;    /* persistent blocks per compute unit */
     int32_t chunk_paths;      /* paths per work-queue item */
-    int32_t reserved;
+    int32_t item_order;       /* SPAC_ORDER_* */
 } spac_launch_config;
 spac_status spac_fk_set_launch(spac_problem* p, const spac_launch_config* cfg);
+
+/* A row-assembly plan for repeated assemblies of one node batch (config-3
+ * style batches: thousands of nodes with uneven paths and h).  Creation
+ * uploads the batch, the interface table and the work list once into buffers
+ * owned by the plan, and captures the whole device sequence of
+ * spac_fk_assemble_rows (queue reset, persistent path kernel, estimator
+ * reduction, B and G kernels) in one HIP graph; execution writes the seed to
+ * device memory, replays the graph and copies the rows out.  The plan copies
+ * the problem's launch configuration and control variate at creation.
+ * Results equal spac_fk_assemble_rows with the same seed. */
+typedef struct spac_rows_plan spac_rows_plan;
+spac_status spac_fk_plan_create(spac_problem* p, const spac_interface_table* ifaces, const spac_node_batch* nodes,
+                                spac_rows_plan** out);
+/* device_ms (optional): time of the graph on the plan's stream (HIP events). */
+spac_status spac_fk_plan_execute(spac_rows_plan* plan, uint64_t seed, spac_row_block* out, double* device_ms);
+spac_status spac_fk_plan_destroy(spac_rows_plan* plan);
+
+/* Autotuner of the persistent path kernel for a node batch: coordinate search
+ * (two passes) over block_size {64, 128, 256}, blocks_per_cu {1, 2, 4, 8, 16},
+ * chunk_paths {32 .. 1024} and item_order, each candidate timed (best of two
+ * runs, HIP events) on the batch with its paths scaled by path_fraction.  The
+ * fastest configuration is installed on p (spac_fk_set_launch).  tried /
+ * tried_ms (optional, cap entries) receive every measured candidate. */
+typedef struct {
+    int32_t n_candidates;
+    int32_t reserved;
+    spac_launch_config chosen;
+    double best_ms;     /* path kernel on the sample, chosen configuration */
+    double default_ms;  /* path kernel on the sample, configuration at entry */
+} spac_autotune_report;
+spac_status spac_fk_autotune_rows(spac_problem* p, const spac_interface_table* ifaces, const spac_node_batch* nodes,
+                                  double path_fraction, spac_launch_config* tried, double* tried_ms, int32_t cap,
+                                  spac_autotune_report* report);
 
 /* Control variate F in xi += Y F(X).dW (Solvers/FeynmanKacSolver.cpp:13). */
 enum {

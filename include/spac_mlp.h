@@ -38,7 +38,7 @@ typedef struct {
  * with (gx, gy) the forward-mode gradient of o0, held fixed per step. */
 typedef struct {
     int32_t steps;
-    int32_t reserved;
+    int32_t eager;   /* 0: steps replayed from captured HIP graphs; 1: launched one by one (same results) */
     double lr;
     double lr_final;
     double grad_weight;

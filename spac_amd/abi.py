@@ -39,6 +39,8 @@ SPAC_OUT_REFLECTED = 3
 SPAC_OUT_MAX_STEPS = 4
 SPAC_RBF_IMQ_GLOBAL = 0
 SPAC_RBF_GAUSS_STENCIL = 1
+SPAC_ORDER_NODE = 0
+SPAC_ORDER_COST = 1
 SPAC_CV_ANALYTIC = 0
 SPAC_CV_NONE = 1
 SPAC_CV_TABLE = 2
@@ -176,7 +178,17 @@ class LaunchConfig(C.Structure):
         ("block_size", C.c_int32),
         ("blocks_per_cu", C.c_int32),
         ("chunk_paths", C.c_int32),
+        ("item_order", C.c_int32),
+    ]
+
+
+class AutotuneReport(C.Structure):
+    _fields_ = [
+        ("n_candidates", C.c_int32),
         ("reserved", C.c_int32),
+        ("chosen", LaunchConfig),
+        ("best_ms", C.c_double),
+        ("default_ms", C.c_double),
     ]
 
 
@@ -199,7 +211,7 @@ class MlpDesc(C.Structure):
 class MlpTrainCfg(C.Structure):
     _fields_ = [
         ("steps", C.c_int32),
-        ("reserved", C.c_int32),
+        ("eager", C.c_int32),
         ("lr", C.c_double),
         ("lr_final", C.c_double),
         ("grad_weight", C.c_double),
@@ -251,6 +263,13 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.spac_fk_last_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
     lib.spac_fk_set_launch.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
     lib.spac_fk_set_control_variate.argtypes = [C.c_void_p, C.POINTER(CvConfig)]
+    lib.spac_fk_plan_create.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
+                                        C.POINTER(C.c_void_p)]
+    lib.spac_fk_plan_execute.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(RowBlock), C.POINTER(C.c_double)]
+    lib.spac_fk_plan_destroy.argtypes = [C.c_void_p]
+    lib.spac_fk_autotune_rows.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch), C.c_double,
+                                          C.POINTER(LaunchConfig), C.POINTER(C.c_double), C.c_int32,
+                                          C.POINTER(AutotuneReport)]
     fp = C.POINTER(C.c_float)
     lib.spac_mlp_create.argtypes = [C.POINTER(MlpDesc), C.POINTER(C.c_void_p)]
     lib.spac_mlp_destroy.argtypes = [C.c_void_p]

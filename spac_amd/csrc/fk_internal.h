@@ -64,15 +64,22 @@ struct Problem {
     int32_t cv_t = 0;
     double cv_dom[4] = {0.0, 0.0, 1.0, 1.0};
 
+    DeviceBuffer seed_dev; /* seed read by captured kernels (plans) */
+
     spac_status ensure_records(int64_t total_paths);
     spac_status upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths, const double* h,
                               const uint32_t* strm, const double* circle, SolveSet* out);
+    /* Enqueue without synchronising (capturable): queue reset + path kernel,
+     * then (d_stats) the reduction.  seed_dev != nullptr: kernels read the seed
+     * from device memory instead of `seed`. */
+    spac_status enqueue_paths(const SolveSet& ss, uint64_t seed, const uint64_t* seed_dev, int* grid_blocks);
+    spac_status enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode);
     spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode);
     void release_all() {
         DeviceBuffer* all[] = {&rec_xt, &rec_yt, &rec_Y, &rec_Z, &rec_xi, &rec_tau, &rec_gval, &rec_steps,
                                &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
                                &queue, &stats, &ifc, &ifnode, &ifipsi, &ifoff, &rows_iface, &rows_goff,
-                               &rows_G, &rows_part, &rows_col, &cv_table};
+                               &rows_G, &rows_part, &rows_col, &cv_table, &seed_dev};
         for (DeviceBuffer* b : all) b->release();
     }
 };

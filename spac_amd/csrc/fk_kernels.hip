@@ -60,6 +60,7 @@ struct PathArgs {
     double dom[4];
     double gobet;
     uint64_t seed;
+    const uint64_t* seed_dev; /* non-null: the seed is read here (captured graphs) */
     int32_t max_steps;
     int32_t n_items;
     /* solves */
@@ -116,6 +117,7 @@ __device__ __forceinline__ double exit_u(const PathArgs& a, double x, double y) 
 template <class Coef, int SUB, int CV>
 __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t seed = a.seed_dev ? *a.seed_dev : a.seed;
     bool active = false;
     /* lane path state */
     int32_t solve = 0;
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             /* Paso_Absorbente (Solvers/FeynmanKacSolver.cpp:7-18) */
             StepCoef cf = Coef::step(X, Yc);
             double n0, n1;
-            spac_normal_pair(a.seed, stream, path, (uint32_t)nstep, &n0, &n1);
+            spac_normal_pair(seed, stream, path, (uint32_t)nstep, &n0, &n1);
             double dWx = sqrth * n0, dWy = sqrth * n1;
             Z = Z + (cf.f * Yw) * h;
             if (CV == CV_ANALYTIC) {
@@ -249,6 +251,7 @@ template <class Coef32, int SUB, int CV>
 __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
     using Coef = typename Coef32::F64;
     const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t seed = a.seed_dev ? *a.seed_dev : a.seed;
     bool active = false;
     int32_t solve = 0;
     uint32_t path = 0;
@@ -316,7 +319,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
         if (active) {
             StepCoef32 cf = Coef32::step(xf, yf);
             float n0, n1;
-            normal_pair32(a.seed, stream, path, (uint32_t)nstep, &n0, &n1);
+            normal_pair32(seed, stream, path, (uint32_t)nstep, &n0, &n1);
             const float dWx = sqrthf * n0, dWy = sqrthf * n1;
             const float Ywf = (float)Yw;
             Z += (double)((cf.f * Ywf) * hf);
@@ -576,6 +579,8 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
         }
     }
     if (items.size() > 0x7fffffffu) return set_error(SPAC_E_INVALID_ARG, "too many work items");
+    if (launch.item_order == SPAC_ORDER_COST)
+        std::stable_sort(items.begin(), items.end(), [&](const int4& u, const int4& v) { return h[u.x] < h[v.x]; });
     spac_status rc;
     size_t nb = (size_t)n;
     if ((rc = sx.ensure(nb * 8))) return rc;
@@ -631,16 +636,14 @@ static spac_status check_problem(const spac_problem_desc* d) {
     return SPAC_OK;
 }
 
-/* Launch the persistent path kernel and, optionally, the reduction. */
-spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode) {
-    spac_status rc;
-    if ((rc = ensure_records(ss.total_paths))) return rc;
-    if ((rc = queue.ensure(64))) return rc;
+/* Queue reset and the persistent path kernel (no synchronisation). */
+spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint64_t* sdev, int* grid_blocks) {
     PathArgs a;
     memset(&a, 0, sizeof a);
     for (int i = 0; i < 4; ++i) a.dom[i] = desc.domain_params[i];
     a.gobet = desc.gobet_constant;
     a.seed = seed;
+    a.seed_dev = sdev;
     a.max_steps = desc.max_steps > 0 ? desc.max_steps : 0x7fffffff;
     a.x0 = ss.x0; a.y0 = ss.y0;
     a.h = ss.h; a.h_uniform = ss.h_uniform;
@@ -673,35 +676,48 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
         a.cv32.iy = (float)a.cv.iy;
         a.cv32.top = (float)(cv_t - 1);
     }
-
     int sub = (ss.circle || desc.subdomain_kind == SPAC_BOUNDARY_CIRCLE) ? SUB_CIRCLE : SUB_NONE;
     int block = launch.block_size;
     int blocks = num_cus * launch.blocks_per_cu;
-    int64_t waves_needed = (ss.n_items + 0);
-    int64_t max_blocks = std::max<int64_t>(1, (waves_needed + (block / 64) - 1) / (block / 64));
+    int64_t max_blocks = std::max<int64_t>(1, ((int64_t)ss.n_items + (block / 64) - 1) / (block / 64));
     if (blocks > max_blocks) blocks = (int)max_blocks;
-
-    SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
     launch_paths(desc.coef_kind, desc.precision, sub, cv_mode, dim3(blocks), dim3(block), stream, a);
     SPAC_HIP_CHECK(hipGetLastError());
+    if (grid_blocks) *grid_blocks = blocks;
+    return SPAC_OK;
+}
+
+/* Estimator reduction (no synchronisation). */
+spac_status Problem::enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode) {
+    ReduceArgs r;
+    memset(&r, 0, sizeof r);
+    r.n_solves = ss.n;
+    r.node_mode = node_mode;
+    r.paths = ss.paths;
+    r.paths_uniform = ss.paths_uniform;
+    r.rec_off = ss.rec_off;
+    r.Y = (const double*)rec_Y.ptr; r.Z = (const double*)rec_Z.ptr; r.xi = (const double*)rec_xi.ptr;
+    r.tau = (const double*)rec_tau.ptr; r.gval = (const double*)rec_gval.ptr;
+    r.steps = (const int32_t*)rec_steps.ptr; r.outcome = (const int8_t*)rec_outcome.ptr;
+    r.stats = d_stats;
+    r.cv_fallback = (cv_mode == SPAC_CV_TABLE) ? cv_fallback : 0;
+    r.cv_max_ratio = cv_max_ratio;
+    hipLaunchKernelGGL(fk_reduce_kernel, dim3(ss.n), dim3(64), 0, stream, r);
+    SPAC_HIP_CHECK(hipGetLastError());
+    return SPAC_OK;
+}
+
+/* Launch the persistent path kernel and, optionally, the reduction; synchronous, timed. */
+spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode) {
+    spac_status rc;
+    if ((rc = ensure_records(ss.total_paths))) return rc;
+    if ((rc = queue.ensure(64))) return rc;
+    int blocks = 0;
+    SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
+    if ((rc = enqueue_paths(ss, seed, nullptr, &blocks))) return rc;
     SPAC_HIP_CHECK(hipEventRecord(ev[1], stream));
-    if (d_stats) {
-        ReduceArgs r;
-        memset(&r, 0, sizeof r);
-        r.n_solves = ss.n;
-        r.node_mode = node_mode;
-        r.paths = ss.paths;
-        r.paths_uniform = ss.paths_uniform;
-        r.rec_off = ss.rec_off;
-        r.Y = a.Y; r.Z = a.Z; r.xi = a.xi; r.tau = a.tau; r.gval = a.gval;
-        r.steps = a.steps; r.outcome = a.outcome;
-        r.stats = d_stats;
-        r.cv_fallback = (cv_mode == SPAC_CV_TABLE) ? cv_fallback : 0;
-        r.cv_max_ratio = cv_max_ratio;
-        hipLaunchKernelGGL(fk_reduce_kernel, dim3(ss.n), dim3(64), 0, stream, r);
-        SPAC_HIP_CHECK(hipGetLastError());
-    }
+    if (d_stats && (rc = enqueue_reduce(ss, d_stats, node_mode))) return rc;
     SPAC_HIP_CHECK(hipEventRecord(ev[2], stream));
     SPAC_HIP_CHECK(hipStreamSynchronize(stream));
     float ms_path = 0.f, ms_red = 0.f;
@@ -812,6 +828,9 @@ spac_status spac_fk_set_launch(spac_problem* hp, const spac_launch_config* cfg) 
         if (cfg->chunk_paths < 1) return set_error(SPAC_E_INVALID_ARG, "chunk_paths must be >= 1");
         p->launch.chunk_paths = cfg->chunk_paths;
     }
+    if (cfg->item_order != SPAC_ORDER_NODE && cfg->item_order != SPAC_ORDER_COST)
+        return set_error(SPAC_E_INVALID_ARG, "item_order must be SPAC_ORDER_NODE or SPAC_ORDER_COST");
+    p->launch.item_order = cfg->item_order;
     return SPAC_OK;
 }
 

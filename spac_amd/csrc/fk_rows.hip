@@ -94,9 +94,13 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
     __shared__ double s_w[kRowsBlock * kWStride]; /* per-path stencil weights H_{s+r}(theta) */
     __shared__ spac_ksum s_part[(kRowsWaves - 1) * kMaxCols];
     __shared__ double s_S[kMaxCols];
+    __shared__ double s_blk[kMaxStencil * kMaxStencil]; /* full circles: the one stencil inverse */
 
     const int lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < M; i += kRowsBlock) s_th[i] = th[i];
+    const bool circle_blk = stencil && code == 1;
+    if (circle_blk)
+        for (int i = tid; i < S * S; i += kRowsBlock) s_blk[i] = ipsi[i];
     __syncthreads();
     if (np == 0) {
         for (int j = tid; j < M; j += kRowsBlock) G[j] = 0.0;
@@ -121,7 +125,7 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
                     s = (code == 1) ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
                                     : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
                     /* H_{s+r} = sum_i iPsiS(i, r) phi_i, i in order (the per-column formula) */
-                    const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s * S * S);
+                    const double* blk = circle_blk ? s_blk : ipsi + (int64_t)s * S * S;
                     for (int r = 0; r < S; ++r) {
                         double w = 0.0;
                         for (int i = 0; i < S; ++i) w = w + blk[i * S + r] * phi[i];
@@ -245,22 +249,27 @@ static spac_status validate(const spac_interface_table* t, const spac_node_batch
     return SPAC_OK;
 }
 
-} // namespace spac
 
-using namespace spac;
+/* Everything one row assembly needs on the device, uploaded once. */
+struct RowsSetup {
+    SolveSet ss;
+    RowsArgs ra;
+    int n = 0;
+    const int32_t* d_front = nullptr;
+    double* d_B = nullptr;
+    spac_point_stats* d_stats = nullptr;
+    std::vector<int64_t> goff;  /* host copy of G_offset */
+    std::vector<int32_t> gcol;  /* host copy of G_col */
+};
 
-extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interface_table* t,
-                                             const spac_node_batch* nb, uint64_t seed, spac_row_block* out) {
-    if (!hp || !out || !out->B || !out->G || !out->G_col || !out->G_offset)
-        return set_error(SPAC_E_INVALID_ARG, "null argument");
-    spac_status rc = validate(t, nb);
-    if (rc) return rc;
-    Problem* p = reinterpret_cast<Problem*>(hp);
+/* Upload the node batch, interface table and work list into p's buffers and
+ * allocate the records, so that rows_enqueue only launches kernels. */
+static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const spac_node_batch* nb, RowsSetup* rs) {
     const int n = nb->n;
-    /* host-side job tables */
     std::vector<uint32_t> paths(n);
     std::vector<double> circle(3 * (size_t)n);
-    std::vector<int64_t> goff(n + 1, 0);
+    rs->n = n;
+    rs->goff.assign(n + 1, 0);
     for (int k = 0; k < n; ++k) {
         int f = nb->iface[k];
         paths[k] = nb->frontera[k] ? 0u : nb->paths[k];
@@ -268,13 +277,16 @@ extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interf
         circle[3 * k + 1] = t->cx[f];
         circle[3 * k + 2] = t->cy[f];
         int M = t->node_offset[f + 1] - t->node_offset[f];
-        goff[k + 1] = goff[k] + M;
-        for (int j = 0; j < M; ++j) out->G_col[goff[k] + j] = t->node_global[t->node_offset[f] + j];
+        rs->goff[k + 1] = rs->goff[k] + M;
     }
-    for (int k = 0; k <= n; ++k) out->G_offset[k] = goff[k];
-    SolveSet ss;
-    if ((rc = p->upload_solves(n, nb->x, nb->y, paths.data(), nb->h, nb->stream, circle.data(), &ss))) return rc;
-    /* interface table */
+    rs->gcol.resize((size_t)rs->goff[n]);
+    for (int k = 0; k < n; ++k) {
+        int f = nb->iface[k];
+        int M = t->node_offset[f + 1] - t->node_offset[f];
+        for (int j = 0; j < M; ++j) rs->gcol[rs->goff[k] + j] = t->node_global[t->node_offset[f] + j];
+    }
+    spac_status rc;
+    if ((rc = p->upload_solves(n, nb->x, nb->y, paths.data(), nb->h, nb->stream, circle.data(), &rs->ss))) return rc;
     const int K = t->n_interfaces;
     const int nodes = t->node_offset[K];
     int64_t nipsi = 0;
@@ -290,14 +302,16 @@ extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interf
         nipsi = std::max(nipsi, t->ipsi_offset[k] + sz);
     }
     const size_t szK = (size_t)K;
-    if ((rc = p->ifc.ensure(szK * (3 * 8 + 4)))) return rc;           /* cx, cy, c2, code */
+    if ((rc = p->ifc.ensure(szK * (3 * 8 + 4)))) return rc; /* cx, cy, c2, code */
     if ((rc = p->ifnode.ensure((size_t)nodes * 8))) return rc;
     if ((rc = p->ifipsi.ensure((size_t)std::max<int64_t>(nipsi, 1) * 8))) return rc;
     if ((rc = p->rows_iface.ensure((size_t)n * 4 + (size_t)n * 4))) return rc; /* job_iface, frontera */
     if ((rc = p->rows_goff.ensure((size_t)(n + 1) * 8))) return rc;
-    if ((rc = p->rows_G.ensure((size_t)goff[n] * 8))) return rc;
-    if ((rc = p->rows_part.ensure((size_t)n * 8))) return rc;            /* B */
+    if ((rc = p->rows_G.ensure((size_t)std::max<int64_t>(rs->goff[n], 1) * 8))) return rc;
+    if ((rc = p->rows_part.ensure((size_t)n * 8))) return rc; /* B */
     if ((rc = p->stats.ensure((size_t)n * sizeof(spac_point_stats)))) return rc;
+    if ((rc = p->ensure_records(rs->ss.total_paths))) return rc;
+    if ((rc = p->queue.ensure(64))) return rc;
     char* ifc = (char*)p->ifc.ptr;
     double* d_cx = (double*)ifc;
     double* d_cy = d_cx + K;
@@ -320,30 +334,17 @@ extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interf
     if (nipsi > 0) ROWS_HIP_CHECK(hipMemcpyAsync(p->ifipsi.ptr, t->ipsi, (size_t)nipsi * 8, hipMemcpyHostToDevice, s));
     ROWS_HIP_CHECK(hipMemcpyAsync(d_iface, nb->iface, (size_t)n * 4, hipMemcpyHostToDevice, s));
     ROWS_HIP_CHECK(hipMemcpyAsync(d_front, nb->frontera, (size_t)n * 4, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(p->rows_goff.ptr, goff.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(p->rows_goff.ptr, rs->goff.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, s));
     ROWS_HIP_CHECK(hipStreamSynchronize(s));
-    /* 1 + 2: paths and node estimators (node mode) */
-    spac_point_stats* d_stats = (spac_point_stats*)p->stats.ptr;
-    if ((rc = p->run(ss, seed, d_stats, 1))) return rc;
-    /* 3: B, including boundary nodes */
-    double* d_B = (double*)p->rows_part.ptr;
-    int nblk = (n + 255) / 256;
-    if (p->desc.coef_kind == SPAC_COEF_FRANKE_POISSON)
-        hipLaunchKernelGGL(fk_rows_b_kernel<CoefFranke>, dim3(nblk), dim3(256), 0, s, n, d_front, ss.x0, ss.y0, d_stats, d_B);
-    else
-        hipLaunchKernelGGL(fk_rows_b_kernel<CoefVarFranke>, dim3(nblk), dim3(256), 0, s, n, d_front, ss.x0, ss.y0, d_stats,
-                           d_B);
-    ROWS_HIP_CHECK(hipGetLastError());
-    /* 4: G */
-    RowsArgs ra;
+    RowsArgs& ra = rs->ra;
     std::memset(&ra, 0, sizeof ra);
     ra.n_jobs = n;
     ra.rbf_kind = t->rbf_kind;
     ra.stencil_size = t->stencil_size;
     ra.gauss_eps = t->gauss_eps;
     ra.job_iface = d_iface;
-    ra.paths = ss.paths;
-    ra.rec_off = ss.rec_off;
+    ra.paths = rs->ss.paths;
+    ra.rec_off = rs->ss.rec_off;
     ra.g_off = (const int64_t*)p->rows_goff.ptr;
     ra.cx = d_cx; ra.cy = d_cy; ra.code = d_code; ra.c2 = d_c2;
     ra.node_offset = d_noff;
@@ -355,25 +356,266 @@ extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interf
     ra.Y = (const double*)p->rec_Y.ptr;
     ra.outcome = (const int8_t*)p->rec_outcome.ptr;
     ra.G = (double*)p->rows_G.ptr;
-    ROWS_HIP_CHECK(hipEventRecord(p->ev[2], s));
-    hipLaunchKernelGGL(fk_rows_g_kernel, dim3(n), dim3(kRowsBlock), 0, s, ra);
+    rs->d_front = d_front;
+    rs->d_B = (double*)p->rows_part.ptr;
+    rs->d_stats = (spac_point_stats*)p->stats.ptr;
+    return SPAC_OK;
+}
+
+/* B (boundary nodes: g at the node; others: the estimator's B) and G, no synchronisation. */
+static spac_status rows_enqueue_bg(Problem* p, const RowsSetup& rs) {
+    hipStream_t s = p->stream;
+    const int n = rs.n;
+    const int nblk = (n + 255) / 256;
+    if (p->desc.coef_kind == SPAC_COEF_FRANKE_POISSON)
+        hipLaunchKernelGGL(fk_rows_b_kernel<CoefFranke>, dim3(nblk), dim3(256), 0, s, n, rs.d_front, rs.ss.x0, rs.ss.y0,
+                           (const spac_point_stats*)rs.d_stats, rs.d_B);
+    else
+        hipLaunchKernelGGL(fk_rows_b_kernel<CoefVarFranke>, dim3(nblk), dim3(256), 0, s, n, rs.d_front, rs.ss.x0,
+                           rs.ss.y0, (const spac_point_stats*)rs.d_stats, rs.d_B);
     ROWS_HIP_CHECK(hipGetLastError());
-    ROWS_HIP_CHECK(hipEventRecord(p->ev[3], s));
-    ROWS_HIP_CHECK(hipStreamSynchronize(s));
-    float ms_g = 0.f, ms_all = 0.f;
-    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_g, p->ev[2], p->ev[3]));
-    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_all, p->ev[0], p->ev[3]));
-    p->last.reduce_kernel_ms += ms_g;
-    p->last.total_ms = ms_all; /* all device work of the call, first kernel to last */
-    ROWS_HIP_CHECK(hipMemcpy(out->B, d_B, (size_t)n * 8, hipMemcpyDeviceToHost));
-    ROWS_HIP_CHECK(hipMemcpy(out->G, p->rows_G.ptr, (size_t)goff[n] * 8, hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL(fk_rows_g_kernel, dim3(n), dim3(kRowsBlock), 0, s, rs.ra);
+    ROWS_HIP_CHECK(hipGetLastError());
+    return SPAC_OK;
+}
+
+/* Copy the rows out and fold the per-node statuses. */
+static spac_status rows_finish(Problem* p, const RowsSetup& rs, spac_row_block* out) {
+    const int n = rs.n;
+    for (int k = 0; k <= n; ++k) out->G_offset[k] = rs.goff[k];
+    if (!rs.gcol.empty()) std::memcpy(out->G_col, rs.gcol.data(), rs.gcol.size() * 4);
+    ROWS_HIP_CHECK(hipMemcpy(out->B, rs.d_B, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (rs.goff[n] > 0) ROWS_HIP_CHECK(hipMemcpy(out->G, p->rows_G.ptr, (size_t)rs.goff[n] * 8, hipMemcpyDeviceToHost));
     std::vector<spac_point_stats> st(n);
-    ROWS_HIP_CHECK(hipMemcpy(st.data(), d_stats, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
+    ROWS_HIP_CHECK(hipMemcpy(st.data(), rs.d_stats, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
     if (out->stats) std::memcpy(out->stats, st.data(), (size_t)n * sizeof(spac_point_stats));
     spac_status worst = SPAC_OK;
     for (int k = 0; k < n; ++k)
         if (st[k].status < worst) worst = st[k].status;
     if (worst == SPAC_E_NAN) return set_error(SPAC_E_NAN, "NaN reached a row estimator");
     if (worst == SPAC_E_MAX_STEPS) return set_error(SPAC_E_MAX_STEPS, "paths stopped by max_steps");
+    return SPAC_OK;
+}
+
+} // namespace spac
+
+using namespace spac;
+
+extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interface_table* t,
+                                             const spac_node_batch* nb, uint64_t seed, spac_row_block* out) {
+    if (!hp || !out || !out->B || !out->G || !out->G_col || !out->G_offset)
+        return set_error(SPAC_E_INVALID_ARG, "null argument");
+    spac_status rc = validate(t, nb);
+    if (rc) return rc;
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    RowsSetup rs;
+    if ((rc = rows_prepare(p, t, nb, &rs))) return rc;
+    hipStream_t s = p->stream;
+    int blocks = 0;
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[0], s));
+    if ((rc = p->enqueue_paths(rs.ss, seed, nullptr, &blocks))) return rc;
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[1], s));
+    if ((rc = p->enqueue_reduce(rs.ss, rs.d_stats, 1))) return rc;
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[2], s));
+    if ((rc = rows_enqueue_bg(p, rs))) return rc;
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[3], s));
+    ROWS_HIP_CHECK(hipStreamSynchronize(s));
+    float ms_path = 0.f, ms_rest = 0.f, ms_all = 0.f;
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_path, p->ev[0], p->ev[1]));
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_rest, p->ev[1], p->ev[3]));
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_all, p->ev[0], p->ev[3]));
+    p->last.path_kernel_ms = ms_path;
+    p->last.reduce_kernel_ms = ms_rest; /* reduction, B and G kernels */
+    p->last.total_ms = ms_all;          /* all device work of the call, first kernel to last */
+    p->last.path_kernel_launches = 1;
+    p->last.grid_blocks = blocks;
+    p->last.path_steps = 0;
+    return rows_finish(p, rs, out);
+}
+
+/* ------------------------------------------------------------- plans -- */
+namespace spac {
+struct RowsPlan {
+    Problem* p = nullptr; /* private problem: its buffers are what the graph captured */
+    RowsSetup rs;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint64_t seed_host = 0;
+    double last_ms = 0.0;
+    ~RowsPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (p) spac_fk_problem_destroy(reinterpret_cast<spac_problem*>(p));
+    }
+};
+} // namespace spac
+
+extern "C" spac_status spac_fk_plan_create(spac_problem* hp, const spac_interface_table* t, const spac_node_batch* nb,
+                                           spac_rows_plan** out) {
+    if (!hp || !out) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    *out = nullptr;
+    spac_status rc = validate(t, nb);
+    if (rc) return rc;
+    Problem* src = reinterpret_cast<Problem*>(hp);
+    spac_problem* hclone = nullptr;
+    if ((rc = spac_fk_problem_create(&src->desc, &hclone))) return rc;
+    RowsPlan* plan = new RowsPlan();
+    plan->p = reinterpret_cast<Problem*>(hclone);
+    Problem* p = plan->p;
+    p->launch = src->launch;
+    if (src->cv_mode == SPAC_CV_TABLE) {
+        spac_cv_config cfg;
+        std::memset(&cfg, 0, sizeof cfg);
+        cfg.mode = SPAC_CV_TABLE;
+        cfg.fallback = src->cv_fallback;
+        cfg.max_ratio = src->cv_max_ratio;
+        cfg.table_size = src->cv_t;
+        for (int i = 0; i < 4; ++i) cfg.table_domain[i] = src->cv_dom[i];
+        rc = problem_set_cv(p, &cfg, src->cv_table.ptr, true);
+    } else {
+        p->cv_mode = src->cv_mode;
+        p->cv_fallback = src->cv_fallback;
+        p->cv_max_ratio = src->cv_max_ratio;
+    }
+    if (!rc) rc = rows_prepare(p, t, nb, &plan->rs);
+    if (!rc) rc = p->seed_dev.ensure(64);
+    if (rc) {
+        delete plan;
+        return rc;
+    }
+    hipStream_t s = p->stream;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = p->enqueue_paths(plan->rs.ss, 0, (const uint64_t*)p->seed_dev.ptr, nullptr);
+        if (!rc) rc = p->enqueue_reduce(plan->rs.ss, plan->rs.d_stats, 1);
+        if (!rc) rc = rows_enqueue_bg(p, plan->rs);
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (!rc && e2 != hipSuccess) rc = set_error(SPAC_E_HIP, "graph capture failed: %s", hipGetErrorString(e2));
+    } else {
+        rc = set_error(SPAC_E_HIP, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+        e = hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = set_error(SPAC_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (!rc && (hipEventCreate(&plan->e0) != hipSuccess || hipEventCreate(&plan->e1) != hipSuccess))
+        rc = set_error(SPAC_E_HIP, "hipEventCreate failed");
+    if (rc) {
+        delete plan;
+        return rc;
+    }
+    *out = reinterpret_cast<spac_rows_plan*>(plan);
+    return SPAC_OK;
+}
+
+extern "C" spac_status spac_fk_plan_execute(spac_rows_plan* hplan, uint64_t seed, spac_row_block* out,
+                                            double* device_ms) {
+    RowsPlan* plan = reinterpret_cast<RowsPlan*>(hplan);
+    if (!plan || !out || !out->B || !out->G || !out->G_col || !out->G_offset)
+        return set_error(SPAC_E_INVALID_ARG, "null argument");
+    Problem* p = plan->p;
+    hipStream_t s = p->stream;
+    plan->seed_host = seed;
+    ROWS_HIP_CHECK(hipMemcpyAsync(p->seed_dev.ptr, &plan->seed_host, 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipEventRecord(plan->e0, s));
+    ROWS_HIP_CHECK(hipGraphLaunch(plan->exec, s));
+    ROWS_HIP_CHECK(hipEventRecord(plan->e1, s));
+    ROWS_HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms, plan->e0, plan->e1));
+    plan->last_ms = ms;
+    if (device_ms) *device_ms = ms;
+    return rows_finish(p, plan->rs, out);
+}
+
+extern "C" spac_status spac_fk_plan_destroy(spac_rows_plan* hplan) {
+    RowsPlan* plan = reinterpret_cast<RowsPlan*>(hplan);
+    if (!plan) return SPAC_OK;
+    if (plan->p) (void)hipStreamSynchronize(plan->p->stream);
+    delete plan;
+    return SPAC_OK;
+}
+
+/* ---------------------------------------------------------- autotuner -- */
+extern "C" spac_status spac_fk_autotune_rows(spac_problem* hp, const spac_interface_table* t,
+                                             const spac_node_batch* nb, double path_fraction,
+                                             spac_launch_config* tried, double* tried_ms, int32_t cap,
+                                             spac_autotune_report* rep) {
+    if (!hp || !rep) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (!(path_fraction > 0.0) || path_fraction > 1.0)
+        return set_error(SPAC_E_INVALID_ARG, "path_fraction must be in (0, 1]");
+    spac_status rc = validate(t, nb);
+    if (rc) return rc;
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    /* sample: every node, paths scaled by path_fraction (at least one) */
+    const int n = nb->n;
+    std::vector<uint32_t> paths(n);
+    std::vector<double> circle(3 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        int f = nb->iface[k];
+        paths[k] = nb->frontera[k] ? 0u
+                                   : (uint32_t)std::max<double>(1.0, std::floor((double)nb->paths[k] * path_fraction));
+        circle[3 * k] = t->radius[f];
+        circle[3 * k + 1] = t->cx[f];
+        circle[3 * k + 2] = t->cy[f];
+    }
+    const spac_launch_config start = p->launch;
+    int count = 0;
+    auto measure = [&](const spac_launch_config& c, double* ms) -> spac_status {
+        p->launch = c;
+        SolveSet ss;
+        spac_status r = p->upload_solves(n, nb->x, nb->y, paths.data(), nb->h, nb->stream, circle.data(), &ss);
+        if (r) return r;
+        double best = 1e300;
+        for (int rep_i = 0; rep_i < 2; ++rep_i) {
+            if ((r = p->run(ss, 0x5eedULL + (uint64_t)rep_i, nullptr, 1))) return r;
+            best = std::min(best, p->last.path_kernel_ms);
+        }
+        if (tried && tried_ms && count < cap) {
+            tried[count] = c;
+            tried_ms[count] = best;
+        }
+        ++count;
+        *ms = best;
+        return SPAC_OK;
+    };
+    spac_launch_config best = start;
+    double best_ms = 0.0;
+    if ((rc = measure(best, &best_ms))) { p->launch = start; return rc; } /* warm-up */
+    count = 0;
+    if ((rc = measure(best, &best_ms))) { p->launch = start; return rc; }
+    const double default_ms = best_ms;
+    const int blocks[] = {64, 128, 256};
+    const int bpc[] = {1, 2, 4, 8, 16};
+    const int chunks[] = {32, 64, 128, 256, 512, 1024};
+    const int orders[] = {SPAC_ORDER_NODE, SPAC_ORDER_COST};
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int field = 0; field < 4; ++field) {
+            const int* vals = field == 0 ? blocks : field == 1 ? bpc : field == 2 ? chunks : orders;
+            const int nv = field == 0 ? 3 : field == 1 ? 5 : field == 2 ? 6 : 2;
+            for (int i = 0; i < nv; ++i) {
+                spac_launch_config c = best;
+                int* slot = field == 0 ? &c.block_size : field == 1 ? &c.blocks_per_cu
+                                                       : field == 2 ? &c.chunk_paths : &c.item_order;
+                if (*slot == vals[i]) continue;
+                *slot = vals[i];
+                double ms = 0.0;
+                if ((rc = measure(c, &ms))) { p->launch = start; return rc; }
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+        }
+    }
+    p->launch = best;
+    rep->n_candidates = count;
+    rep->reserved = 0;
+    rep->chosen = best;
+    rep->best_ms = best_ms;
+    rep->default_ms = default_ms;
     return SPAC_OK;
 }

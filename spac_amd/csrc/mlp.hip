@@ -640,17 +640,24 @@ spac_status spac_mlp_train(spac_mlp* hm, const double* x, const double* y, const
     const int kPerGraph = 50;
     const int full = cfg->steps / kPerGraph, rest = cfg->steps % kPerGraph;
     hipGraphExec_t gfull = nullptr, grest = nullptr;
-    if (full && (rc = capture_steps(m, kPerGraph, lam, t0, cfg->steps, lr0, lr1, &gfull))) return rc;
-    if (rest && (rc = capture_steps(m, rest, lam, t0, cfg->steps, lr0, lr1, &grest))) {
-        if (gfull) (void)hipGraphExecDestroy(gfull);
-        return rc;
+    if (!cfg->eager) {
+        if (full && (rc = capture_steps(m, kPerGraph, lam, t0, cfg->steps, lr0, lr1, &gfull))) return rc;
+        if (rest && (rc = capture_steps(m, rest, lam, t0, cfg->steps, lr0, lr1, &grest))) {
+            if (gfull) (void)hipGraphExecDestroy(gfull);
+            return rc;
+        }
     }
     hipEvent_t e0, e1;
     MLP_HIP_CHECK(hipEventCreate(&e0));
     MLP_HIP_CHECK(hipEventCreate(&e1));
     MLP_HIP_CHECK(hipEventRecord(e0, m->stream));
-    for (int i = 0; i < full; ++i) MLP_HIP_CHECK(hipGraphLaunch(gfull, m->stream));
-    if (rest) MLP_HIP_CHECK(hipGraphLaunch(grest, m->stream));
+    if (cfg->eager) {
+        for (int i = 0; i < cfg->steps; ++i) enqueue_step(m, lam, true, t0, cfg->steps, lr0, lr1);
+        MLP_HIP_CHECK(hipGetLastError());
+    } else {
+        for (int i = 0; i < full; ++i) MLP_HIP_CHECK(hipGraphLaunch(gfull, m->stream));
+        if (rest) MLP_HIP_CHECK(hipGraphLaunch(grest, m->stream));
+    }
     MLP_HIP_CHECK(hipEventRecord(e1, m->stream));
     MLP_HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0.f;

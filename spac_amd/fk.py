@@ -110,8 +110,8 @@ class FKProblem:
         except Exception:
             pass
 
-    def set_launch(self, block_size=0, blocks_per_cu=0, chunk_paths=0):
-        cfg = abi.LaunchConfig(block_size, blocks_per_cu, chunk_paths, 0)
+    def set_launch(self, block_size=0, blocks_per_cu=0, chunk_paths=0, item_order=abi.SPAC_ORDER_NODE):
+        cfg = abi.LaunchConfig(block_size, blocks_per_cu, chunk_paths, item_order)
         abi.check(self.lib.spac_fk_set_launch(self.h, C.byref(cfg)), self.lib)
 
     def set_control_variate(self, mode, table=None, domain=None, fallback=False, max_ratio=1.0):
@@ -167,3 +167,61 @@ class FKProblem:
         rows["stats"] = stats_to_dicts(rows["stats"])
         rows["rc"] = rc
         return rows
+
+    def autotune_rows(self, mesh, paths, h, path_fraction=0.125, jobs=None):
+        """spac_fk_autotune_rows: installs the fastest launch configuration; returns the report and
+        the measured candidates [(block_size, blocks_per_cu, chunk_paths, item_order, ms), ...]."""
+        nb, keep = mesh.node_batch(paths, h, jobs)
+        cap = 64
+        tried = (abi.LaunchConfig * cap)()
+        ms = np.zeros(cap)
+        rep = abi.AutotuneReport()
+        abi.check(self.lib.spac_fk_autotune_rows(self.h, C.byref(mesh.table), C.byref(nb), path_fraction, tried,
+                                                 abi.ptr(ms, C.c_double), cap, C.byref(rep)), self.lib)
+        n = min(rep.n_candidates, cap)
+        table = [(t.block_size, t.blocks_per_cu, t.chunk_paths, t.item_order, float(ms[i]))
+                 for i, t in enumerate(tried[:n])]
+        return rep, table
+
+
+class RowsPlan:
+    """spac_rows_plan: one node batch uploaded once, its assembly captured in a HIP graph."""
+
+    def __init__(self, problem: FKProblem, mesh, paths, h, jobs=None):
+        self.lib = problem.lib
+        self.mesh = mesh
+        self.jobs = jobs
+        nb, self._keep = mesh.node_batch(paths, h, jobs)
+        hp = C.c_void_p()
+        abi.check(self.lib.spac_fk_plan_create(problem.h, C.byref(mesh.table), C.byref(nb), C.byref(hp)), self.lib)
+        self.h = hp
+        self.last_ms = None
+
+    def execute(self, seed=0, allow_status=False):
+        rows = self.mesh.alloc_rows(self.jobs)
+        rb = self.mesh.row_block(rows)
+        ms = C.c_double()
+        rc = self.lib.spac_fk_plan_execute(self.h, seed, C.byref(rb), C.byref(ms))
+        if not allow_status:
+            abi.check(rc, self.lib)
+        self.last_ms = ms.value
+        rows["stats"] = stats_to_dicts(rows["stats"])
+        rows["rc"] = rc
+        return rows
+
+    def close(self):
+        if self.h:
+            self.lib.spac_fk_plan_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

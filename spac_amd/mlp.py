@@ -66,9 +66,9 @@ class MLP:
                                               grad_weight, abi.ptr(g, C.c_float), C.byref(loss)), self.lib)
         return loss.value, g
 
-    def train(self, x, y, u, steps, lr=3e-3, lr_final=1e-4, grad_weight=0.1):
+    def train(self, x, y, u, steps, lr=3e-3, lr_final=1e-4, grad_weight=0.1, eager=False):
         x, y, u = _d(x), _d(y), _d(u)
-        cfg = abi.MlpTrainCfg(steps, 0, lr, lr_final, grad_weight)
+        cfg = abi.MlpTrainCfg(steps, 1 if eager else 0, lr, lr_final, grad_weight)
         rep = abi.MlpTrainReport()
         abi.check(self.lib.spac_mlp_train(self.h, abi.ptr(x, dp), abi.ptr(y, dp), abi.ptr(u, dp), x.size,
                                           C.byref(cfg), C.byref(rep)), self.lib)

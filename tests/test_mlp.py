@@ -168,3 +168,17 @@ def test_training_reduces_loss(gpu):
         xv, yv, _ = _samples(2000, 14)
         pred = m.infer(xv, yv)[:, 0]
         assert np.sqrt(np.mean((pred - franke_u(xv, yv)) ** 2)) < 0.03
+
+
+@pytest.mark.gpu
+def test_eager_training_equals_graph_training(gpu):
+    from spac_amd.mlp import MLP
+
+    x, y, _ = _samples(1000, 15)
+    u = franke_u(x, y)
+    out = []
+    for eager in (False, True):
+        with MLP(DOM, seed=3) as m:
+            m.train(x, y, u, steps=120, lr=3e-3, lr_final=1e-4, grad_weight=0.03, eager=eager)
+            out.append(m.params)
+    assert np.array_equal(out[0], out[1])
