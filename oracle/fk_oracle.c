@@ -658,18 +658,20 @@ int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_tab
         int64_t np = paths[k];
         int stencil = t->rbf_kind == SPAC_RBF_GAUSS_STENCIL;
         int S = stencil ? (t->stencil_size < M ? t->stencil_size : M) : 0;
-        /* Column sums in the fixed order of the GPU: paths in chunks of 256,
-         * partial w holds paths 64w..64w+63 of every chunk in path order, and
-         * the four partials are merged in order. */
-        spac_ksum* part = (spac_ksum*)malloc(sizeof(spac_ksum) * 4 * (size_t)M);
-        spac_ksum* acc4[4];
-        for (int w = 0; w < 4; ++w) acc4[w] = part + (size_t)w * M;
+        /* Column sums in the fixed order of the GPU, paths in chunks of 256:
+         * IMQ — partial w holds paths 64w..64w+63 of every chunk, four partials;
+         * stencils — partial 4 (p % 256 / 64) + p % 4, sixteen partials; each
+         * partial in path order, the partials merged in index order. */
+        const int nparts = stencil ? 16 : 4;
+        spac_ksum* part = (spac_ksum*)malloc(sizeof(spac_ksum) * 16 * (size_t)M);
+        spac_ksum* acc4[16];
+        for (int w = 0; w < nparts; ++w) acc4[w] = part + (size_t)w * M;
         double* dsum = (double*)calloc((size_t)M, sizeof(double));
-        for (int j = 0; j < 4 * M; ++j) part[j] = spac_ksum_zero();
+        for (int j = 0; j < 16 * M; ++j) part[j] = spac_ksum_zero();
         for (int64_t p = 0; p < np; ++p) {
             int64_t g = off[k] + p;
             if (rec.outcome[g] != SPAC_OUT_SUBDOMAIN) continue;
-            spac_ksum* acc = acc4[(p % 256) / 64];
+            spac_ksum* acc = stencil ? acc4[4 * ((p % 256) / 64) + p % 4] : acc4[(p % 256) / 64];
             double theta = 0.0;
             spac_exit_angle(code, rec.x_tau[g] - cx, rec.y_tau[g] - cy, th[0], &theta);
             double Yp = rec.Y[g];
@@ -678,12 +680,11 @@ int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_tab
                 int s0 = (code == 1) ? spac_circle_stencil(M, S, t->gauss_eps, theta, phi)
                                      : spac_arc_stencil(th, M, S, t->gauss_eps, theta, phi);
                 const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s0 * S * S);
-                for (int j = 0; j < M; ++j) {
-                    int rel = j - s0;
-                    if (code == 1 && rel < 0) rel += M;
-                    if (rel < 0 || rel >= S) continue;
+                for (int r = 0; r < S; ++r) {
                     double w = 0.0;
-                    for (int i = 0; i < S; ++i) w = w + blk[i * S + rel] * phi[i];
+                    for (int i = 0; i < S; ++i) w = SPAC_FMA(blk[i * S + r], phi[i], w);
+                    int j = s0 + r;
+                    if (j >= M) j -= M;
                     acc[j] = spac_ksum_add(acc[j], Yp * w);
                 }
             } else if (direct) {
@@ -696,7 +697,7 @@ int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_tab
         double N = (double)np;
         spac_ksum* acc = acc4[0];
         for (int j = 0; j < M; ++j)
-            for (int w = 1; w < 4; ++w) acc[j] = spac_ksum_merge(acc[j], acc4[w][j]);
+            for (int w = 1; w < nparts; ++w) acc[j] = spac_ksum_merge(acc[j], acc4[w][j]);
         if (stencil) {
             for (int j = 0; j < M; ++j) Gk[j] = -spac_ksum_value(acc[j]) / N;
         } else if (direct) {
@@ -739,7 +740,7 @@ int oracle_row_weights(const spac_interface_table* t, int f, double theta, doubl
         const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s0 * S * S);
         for (int r = 0; r < S; ++r) {
             double w = 0.0;
-            for (int i = 0; i < S; ++i) w = w + blk[i * S + r] * phi[i];
+            for (int i = 0; i < S; ++i) w = SPAC_FMA(blk[i * S + r], phi[i], w);
             out[(s0 + r) % M] = w;
         }
         return SPAC_OK;

@@ -9,16 +9,15 @@
  *   2. the estimator reduction gives B = mean of (Z + xi + Y g) over domain
  *      exits and (Z + xi) over circle exits, and the node statistics;
  *   3. boundary nodes (frontera) get B = g(x) with no paths (:346-347);
- *   4. the G kernel (one block per node) accumulates the row weights over the
+ *   4. the G kernels (one block per node) accumulate the row weights over the
  *      circle-exit paths.  Global IMQ interpolation uses the moment form
  *      S_i = sum_p Y_p phi(theta_p, theta_i), G_j = -sum_i iPsi(i,j) S_i / N;
  *      Gaussian stencils accumulate G_j += Y_p H_j(theta_p) directly for the
  *      stencil_size columns of each path's stencil.  Every column is a
- *      TwoSum-compensated sum in one fixed order: paths go in chunks of 256,
- *      wave w of the block owns paths 64w..64w+63 of every chunk and sums them
- *      in path order, and the four wave partials are merged in wave order.
- *      The result does not depend on the launch and equals the CPU
- *      restatement's.
+ *      TwoSum-compensated sum in one fixed order (paths in chunks of 256;
+ *      IMQ: wave w owns paths 64w..64w+63 of every chunk, four wave partials;
+ *      stencils: sixteen partials by p / 64 and p % 4), so the result does not
+ *      depend on the launch and equals the CPU restatement's.
  */
 #include <hip/hip_runtime.h>
 
@@ -46,7 +45,6 @@ constexpr int kRowsWaves = kRowsBlock / 64;
 constexpr int kMaxStencil = 16;
 constexpr int kColGroups = 4;             /* columns per lane */
 constexpr int kMaxCols = 64 * kColGroups; /* interfaces up to 256 nodes */
-constexpr int kWStride = kMaxStencil + 1; /* odd stride: conflict-free LDS rows */
 
 struct RowsArgs {
     int32_t n_jobs;
@@ -72,7 +70,11 @@ struct RowsArgs {
     double* G;
 };
 
-__global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
+/* Global IMQ rows (reference mode, arcs only): moment form S_i = sum_p Y_p phi(theta_p,
+ * theta_i) with lanes = columns; wave w of the block sums paths 64w..64w+63 of every 256-path
+ * chunk in path order, the four wave partials merge in wave order, then G_j = -(sum_i
+ * iPsi(i, j) S_i) / N. */
+__global__ void __launch_bounds__(kRowsBlock) fk_rows_g_imq_kernel(RowsArgs a) {
     const int job = blockIdx.x;
     const int tid = threadIdx.x;
     const int64_t np = a.paths[job];
@@ -82,25 +84,18 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
     const double* th = a.theta + nb;
     const int code = a.code[iface];
     const double cx = a.cx[iface], cy = a.cy[iface], c2 = a.c2[iface];
-    const bool stencil = a.rbf_kind == SPAC_RBF_GAUSS_STENCIL;
-    const int S = stencil ? min(a.stencil_size, M) : 0;
     const double* ipsi = a.ipsi + (a.ipsi_offset[iface] >= 0 ? a.ipsi_offset[iface] : 0);
     double* G = a.G + a.g_off[job];
 
     __shared__ double s_th[kMaxCols];
     __shared__ double s_Y[kRowsBlock];
-    __shared__ double s_t[kRowsBlock];           /* exit angle (IMQ) */
-    __shared__ __align__(16) int s_s[kRowsBlock]; /* stencil start, -1: not a circle exit */
-    __shared__ double s_w[kRowsBlock * kWStride]; /* per-path stencil weights H_{s+r}(theta) */
+    __shared__ double s_t[kRowsBlock];
+    __shared__ __align__(16) int s_s[kRowsBlock]; /* 0: circle exit, -1: not */
     __shared__ spac_ksum s_part[(kRowsWaves - 1) * kMaxCols];
     __shared__ double s_S[kMaxCols];
-    __shared__ double s_blk[kMaxStencil * kMaxStencil]; /* full circles: the one stencil inverse */
 
     const int lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < M; i += kRowsBlock) s_th[i] = th[i];
-    const bool circle_blk = stencil && code == 1;
-    if (circle_blk)
-        for (int i = tid; i < S * S; i += kRowsBlock) s_blk[i] = ipsi[i];
     __syncthreads();
     if (np == 0) {
         for (int j = tid; j < M; j += kRowsBlock) G[j] = 0.0;
@@ -120,29 +115,11 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
                 spac_exit_angle(code, a.xt[g] - cx, a.yt[g] - cy, s_th[0], &theta);
                 s_Y[tid] = a.Y[g];
                 s_t[tid] = theta;
-                if (stencil) {
-                    double phi[kMaxStencil];
-                    s = (code == 1) ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
-                                    : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
-                    /* H_{s+r} = sum_i iPsiS(i, r) phi_i, i in order (the per-column formula) */
-                    const double* blk = circle_blk ? s_blk : ipsi + (int64_t)s * S * S;
-                    for (int r = 0; r < S; ++r) {
-                        double w = 0.0;
-                        for (int i = 0; i < S; ++i) w = w + blk[i * S + r] * phi[i];
-                        s_w[tid * kWStride + r] = w;
-                    }
-                } else {
-                    s = 0;
-                }
+                s = 0;
             }
             s_s[tid] = s;
         }
         __syncthreads();
-        /* Wave w sums paths 64w..64w+63 of the chunk in order, 4 at a time and
-         * without branches: a path that does not reach column j adds +0.0.
-         * TwoSum with +0.0 leaves s unchanged and can only turn a -0.0 error term
-         * into +0.0, which never changes s + c, so the column values equal those
-         * of skipping the path (the CPU restatement). */
         const int p0 = wave * 64, p1 = min(cnt, p0 + 64);
         for (int p = p0; p < p1; p += 4) {
             const int4 s4 = *reinterpret_cast<const int4*>(&s_s[p]);
@@ -158,16 +135,7 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
             for (int q = 0; q < kColGroups; ++q) {
                 const int j = lane + q * 64;
                 if (q * 64 >= M) break; /* uniform */
-                if (stencil) {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        int rel = j - sv[k];
-                        if (code == 1 && rel < 0) rel += M;
-                        const bool ok = sv[k] >= 0 && j < M && rel >= 0 && rel < S;
-                        const double w = s_w[(p + k) * kWStride + (ok ? rel : 0)];
-                        acc[q] = spac_ksum_add(acc[q], ok ? yv[k] * w : 0.0);
-                    }
-                } else if (j < M) {
+                if (j < M) {
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
                         if (sv[k] >= 0) acc[q] = spac_ksum_add(acc[q], yv[k] * spac_imq(tv[k], s_th[j], c2));
@@ -176,7 +144,6 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
         }
         __syncthreads();
     }
-    /* merge the wave partials in wave order */
     if (wave > 0) {
 #pragma unroll
         for (int q = 0; q < kColGroups; ++q) {
@@ -193,16 +160,106 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_kernel(RowsArgs a) {
             if (j >= M) continue;
             spac_ksum t = acc[q];
             for (int w = 1; w < kRowsWaves; ++w) t = spac_ksum_merge(t, s_part[(w - 1) * kMaxCols + j]);
-            if (stencil) G[j] = -spac_ksum_value(t) / N;
-            else s_S[j] = spac_ksum_value(t);
+            s_S[j] = spac_ksum_value(t);
         }
     }
-    if (stencil) return; /* uniform */
     __syncthreads();
     for (int j = tid; j < M; j += kRowsBlock) {
         double w = 0.0;
         for (int i = 0; i < M; ++i) w = w + ipsi[(int64_t)i * M + j] * s_S[i];
         G[j] = -w / N;
+    }
+}
+
+/* Gaussian stencil rows: per 256-path chunk every thread computes its path's exit angle,
+ * stencil start s and the S values Y H_{s+r}(theta) (H_{s+r} = sum_i iPsiS(i, r) phi_i with
+ * exact FMAs, i in order) into LDS.  Then each quarter-wave takes one path at a time — lane r
+ * adds value r into column s + r (mod M) of its own copy of the row — so a path costs a quarter
+ * of a wave instead of every column lane.  Copy (w, q) holds the paths p of every chunk with
+ * p / 64 = w and p % 4 = q, in path order; the 16 copies are merged in (w, q) order.
+ * Dynamic LDS: kRowsBlock * wstride doubles of values, then 16 * M TwoSum accumulators. */
+constexpr int kCopies = kRowsWaves * 4;
+__global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs a, int wstride) {
+    extern __shared__ double s_dyn[];
+    const int job = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int64_t np = a.paths[job];
+    const int iface = a.job_iface[job];
+    const int nb = a.node_offset[iface];
+    const int M = a.node_offset[iface + 1] - nb;
+    const double* th = a.theta + nb;
+    const int code = a.code[iface];
+    const double cx = a.cx[iface], cy = a.cy[iface];
+    const int S = min(a.stencil_size, M);
+    const double* ipsi = a.ipsi + (a.ipsi_offset[iface] >= 0 ? a.ipsi_offset[iface] : 0);
+    double* G = a.G + a.g_off[job];
+
+    double* s_v = s_dyn;
+    spac_ksum* s_acc = reinterpret_cast<spac_ksum*>(s_dyn + (size_t)kRowsBlock * wstride);
+    __shared__ double s_th[kMaxCols];
+    __shared__ int s_s[kRowsBlock]; /* stencil start, -1: not a circle exit */
+    __shared__ double s_blk[kMaxStencil * kMaxStencil];
+
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < M; i += kRowsBlock) s_th[i] = th[i];
+    const bool circle = code == 1;
+    if (circle)
+        for (int i = tid; i < S * S; i += kRowsBlock) s_blk[i] = ipsi[i];
+    for (int i = tid; i < kCopies * M; i += kRowsBlock) s_acc[i] = spac_ksum_zero();
+    __syncthreads();
+    if (np == 0) {
+        for (int j = tid; j < M; j += kRowsBlock) G[j] = 0.0;
+        return;
+    }
+    const int quarter = lane >> 4, r = lane & 15;
+    spac_ksum* my = s_acc + (size_t)(wave * 4 + quarter) * M;
+    const int64_t off = a.rec_off[job];
+    for (int64_t base = 0; base < np; base += kRowsBlock) {
+        const int cnt = (np - base < (int64_t)kRowsBlock) ? (int)(np - base) : kRowsBlock;
+        if (tid < cnt) {
+            const int64_t g = off + base + tid;
+            int s = -1;
+            if (a.outcome[g] == SPAC_OUT_SUBDOMAIN) {
+                double theta = 0.0;
+                spac_exit_angle(code, a.xt[g] - cx, a.yt[g] - cy, s_th[0], &theta);
+                const double Yp = a.Y[g];
+                double phi[kMaxStencil];
+                s = circle ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
+                           : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
+                if (circle) { /* the one stencil inverse, broadcast from LDS */
+                    for (int rr = 0; rr < S; ++rr) {
+                        double w = 0.0;
+                        for (int i = 0; i < S; ++i) w = __builtin_fma(s_blk[i * S + rr], phi[i], w);
+                        s_v[tid * wstride + rr] = Yp * w;
+                    }
+                } else {
+                    const double* blk = ipsi + (int64_t)s * S * S;
+                    for (int rr = 0; rr < S; ++rr) {
+                        double w = 0.0;
+                        for (int i = 0; i < S; ++i) w = __builtin_fma(blk[i * S + rr], phi[i], w);
+                        s_v[tid * wstride + rr] = Yp * w;
+                    }
+                }
+            }
+            s_s[tid] = s;
+        }
+        __syncthreads();
+        const int p1 = min(cnt, wave * 64 + 64);
+        for (int p = wave * 64 + quarter; p < p1; p += 4) {
+            const int s = s_s[p];
+            if (s >= 0 && r < S) {
+                int c = s + r;
+                if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
+                my[c] = spac_ksum_add(my[c], s_v[p * wstride + r]);
+            }
+        }
+        __syncthreads();
+    }
+    const double N = (double)np;
+    for (int c = tid; c < M; c += kRowsBlock) {
+        spac_ksum t = s_acc[c];
+        for (int k = 1; k < kCopies; ++k) t = spac_ksum_merge(t, s_acc[(size_t)k * M + c]);
+        G[c] = -spac_ksum_value(t) / N;
     }
 }
 
@@ -255,12 +312,18 @@ struct RowsSetup {
     SolveSet ss;
     RowsArgs ra;
     int n = 0;
+    int mmax = 1;     /* largest interface of the batch (dynamic LDS of the stencil kernel) */
+    int wstride = 1;  /* odd row stride of the per-path values */
     const int32_t* d_front = nullptr;
     double* d_B = nullptr;
     spac_point_stats* d_stats = nullptr;
     std::vector<int64_t> goff;  /* host copy of G_offset */
     std::vector<int32_t> gcol;  /* host copy of G_col */
 };
+
+static size_t stencil_lds_bytes(const RowsSetup& rs) {
+    return (size_t)kRowsBlock * rs.wstride * 8 + (size_t)kCopies * rs.mmax * sizeof(spac_ksum);
+}
 
 /* Upload the node batch, interface table and work list into p's buffers and
  * allocate the records, so that rows_enqueue only launches kernels. */
@@ -278,6 +341,20 @@ static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const
         circle[3 * k + 2] = t->cy[f];
         int M = t->node_offset[f + 1] - t->node_offset[f];
         rs->goff[k + 1] = rs->goff[k] + M;
+    }
+    rs->mmax = 1;
+    for (int k = 0; k < n; ++k) {
+        int f = nb->iface[k];
+        rs->mmax = std::max(rs->mmax, t->node_offset[f + 1] - t->node_offset[f]);
+    }
+    rs->wstride = std::min(t->stencil_size, rs->mmax) | 1;
+    if (t->rbf_kind == SPAC_RBF_GAUSS_STENCIL) {
+        int lds_max = 0;
+        ROWS_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
+        const size_t need = stencil_lds_bytes(*rs) + 6 * 1024; /* + the kernel's static LDS */
+        if (need > (size_t)lds_max)
+            return set_error(SPAC_E_INVALID_ARG, "interfaces of %d nodes need %zu B of LDS per block (device: %d)",
+                             rs->mmax, need, lds_max);
     }
     rs->gcol.resize((size_t)rs->goff[n]);
     for (int k = 0; k < n; ++k) {
@@ -374,7 +451,12 @@ static spac_status rows_enqueue_bg(Problem* p, const RowsSetup& rs) {
         hipLaunchKernelGGL(fk_rows_b_kernel<CoefVarFranke>, dim3(nblk), dim3(256), 0, s, n, rs.d_front, rs.ss.x0,
                            rs.ss.y0, (const spac_point_stats*)rs.d_stats, rs.d_B);
     ROWS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(fk_rows_g_kernel, dim3(n), dim3(kRowsBlock), 0, s, rs.ra);
+    if (rs.ra.rbf_kind == SPAC_RBF_GAUSS_STENCIL) {
+        const size_t lds = stencil_lds_bytes(rs);
+        hipLaunchKernelGGL(fk_rows_g_stencil_kernel, dim3(n), dim3(kRowsBlock), lds, s, rs.ra, rs.wstride);
+    } else {
+        hipLaunchKernelGGL(fk_rows_g_imq_kernel, dim3(n), dim3(kRowsBlock), 0, s, rs.ra);
+    }
     ROWS_HIP_CHECK(hipGetLastError());
     return SPAC_OK;
 }

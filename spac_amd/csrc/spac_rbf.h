@@ -92,15 +92,32 @@ SPAC_HD int spac_arc_stencil(const double* th, int M, int S, double eps, double 
 
 /* Full-circle stencil (M equally spaced nodes, relative angle r in [0, 2pi)):
  * returns the index of the first stencil node (mod M). */
+/* phi_i = exp(-(a - i b)^2), i = 0..S-1, for equally spaced nodes: four exps
+ * and the recurrence phi_{i+1} = phi_i (E F_i) with E = exp(2 a b) and
+ * F_i = exp(-(2 i + 1) b^2) = F_{i-1} exp(-2 b^2).  Each product adds one
+ * rounding, so phi_i carries at most ~2i ulp more than a direct exp. */
+SPAC_HD void spac_gauss_row(double a, double b, int S, double* phi) {
+    const double b2 = b * b;
+    double p = spac_exp(-(a * a));
+    const double E = spac_exp(2.0 * (a * b));
+    double F = spac_exp(-b2);
+    const double G = spac_exp(-2.0 * b2);
+    phi[0] = p;
+    for (int i = 1; i < S; ++i) {
+        p = p * (E * F);
+        phi[i] = p;
+        F = F * G;
+    }
+}
+
 SPAC_HD int spac_circle_stencil(int M, int S, double eps, double r, double* phi) {
     double delta = SPAC_TWO_PI / (double)M;
     int m = S / 2;
     int k = (int)SPAC_RINT(r / delta);
     double L = (m > 0) ? (double)m * delta : 1.0;
-    for (int i = 0; i < S; ++i) {
-        double d = r - (double)(k - m + i) * delta;
-        phi[i] = spac_gauss_phi(d / L, eps);
-    }
+    /* d_i / L = (r - (k - m) delta) / L - i delta / L */
+    double d0 = r - (double)(k - m) * delta;
+    spac_gauss_row(eps * (d0 / L), eps * (delta / L), S, phi);
     int s = (k - m) % M;
     if (s < 0) s += M;
     return s;
