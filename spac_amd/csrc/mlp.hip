@@ -45,8 +45,10 @@ constexpr int oW1 = 0, ob1 = oW1 + kH * 2, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, 
 static_assert(kNParams == SPAC_MLP_N_PARAMS, "parameter count");
 
 /* ------------------------------------------------------------ training -- */
-constexpr int kGroup = 8;  /* outputs per thread in the dense kernels */
-constexpr int kChunk = 32; /* samples per weight-gradient partial sum */
+constexpr int kTile = 16;        /* samples per block */
+constexpr int kChunk = kTile;    /* samples per weight-gradient partial */
+constexpr int kPad = kTile + 1;  /* padded LDS row */
+constexpr int kOG = 256 / kTile; /* groups of 4 features per block (16 x 4 = 64) */
 
 /* z = 2 (x - lo) / (hi - lo) - 1, once per batch */
 __global__ void k_input(const double* x, const double* y, int n, float lox, float loy, float sx, float sy,
@@ -57,149 +59,428 @@ __global__ void k_input(const double* x, const double* y, int n, float lox, floa
     z[n + s] = 2.0f * ((float)y[s] - loy) * sy - 1.0f;
 }
 
-/* One hidden layer with both forward-mode tangents (d/dx, d/dy):
+struct FwdArgs {
+    const float* W;  /* [64][IN] */
+    const float* b;
+    const float* hin; /* [IN][n] */
+    const float* txin;
+    const float* tyin;
+    float* hout;      /* [64][n] */
+    float* txout;
+    float* tyout;
+    int n;
+    float scx, scy;   /* layer 1: d z / d(x, y) */
+    /* output heads (fused into the last hidden layer) */
+    const float* W4;
+    const float* b4;
+    const double* u;  /* targets, or nullptr: no loss */
+    float lam;
+    float* out3;
+    float* dO;
+    float* loss;
+};
+
+/* One hidden layer with both forward-mode tangents for kTile samples:
  *   h = tanh(W hin + b), t = (1 - h^2) (W tin)   (layer 1: tin = 2 e_d / span_d).
- * Thread = one sample x kGroup outputs; activations are [feature][sample]. */
-template <int IN>
-__global__ void __launch_bounds__(256) k_layer_fwd(const float* __restrict__ W, const float* __restrict__ b,
-                                                   const float* __restrict__ hin, const float* __restrict__ txin,
-                                                   const float* __restrict__ tyin, float* __restrict__ hout,
-                                                   float* __restrict__ txout, float* __restrict__ tyout, int n,
-                                                   float scx, float scy) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    const int o0 = blockIdx.y * kGroup;
-    if (s >= n) return;
-    float a[kGroup], ax[kGroup], ay[kGroup];
+ * Inputs and W^T are staged in LDS; thread t computes features 4 (t / kTile) .. +3 of sample
+ * t % kTile.  HEADS: the block also evaluates the output layer (u, gx, gy heads; gx, gy =
+ * forward-mode d u / d(x, y)), the residuals and dL/do, summing the 16 feature-group
+ * partials in order; the last layer's tangents are then never written. */
+template <int IN, bool HEADS>
+__global__ void __launch_bounds__(256) k_fwd_tile(FwdArgs a) {
+    __shared__ __align__(16) float Lw[IN * 64]; /* W^T: [i][o] */
+    __shared__ float Lh[IN * kTile], Ltx[IN * kTile], Lty[IN * kTile];
+    __shared__ float Lred[kOG][5][kTile];
+    const int tid = threadIdx.x, c = tid % kTile, og = tid / kTile;
+    const int s0 = blockIdx.x * kTile, s = s0 + c, n = a.n;
+    for (int k = tid; k < 64 * IN; k += 256) Lw[(k % IN) * 64 + k / IN] = a.W[k];
+    for (int k = tid; k < IN * kTile; k += 256) {
+        const int i = k / kTile, cc = k % kTile, gs = s0 + cc;
+        const bool in = gs < n;
+        Lh[k] = in ? a.hin[(size_t)i * n + gs] : 0.0f;
+        if (IN != 2) {
+            Ltx[k] = in ? a.txin[(size_t)i * n + gs] : 0.0f;
+            Lty[k] = in ? a.tyin[(size_t)i * n + gs] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int f0 = 4 * og;
+    float acc[4], ax[4], ay[4];
 #pragma unroll
-    for (int o = 0; o < kGroup; ++o) a[o] = ax[o] = ay[o] = 0.0f;
+    for (int o = 0; o < 4; ++o) acc[o] = ax[o] = ay[o] = 0.0f;
+#pragma unroll 4
     for (int i = 0; i < IN; ++i) {
-        const float hv = hin[(size_t)i * n + s];
+        const float hv = Lh[i * kTile + c];
         float tx = 0.0f, ty = 0.0f;
         if (IN != 2) {
-            tx = txin[(size_t)i * n + s];
-            ty = tyin[(size_t)i * n + s];
+            tx = Ltx[i * kTile + c];
+            ty = Lty[i * kTile + c];
         }
+        const float4 w4 = *reinterpret_cast<const float4*>(Lw + i * 64 + f0);
+        const float w[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-        for (int o = 0; o < kGroup; ++o) {
-            const float w = W[(o0 + o) * IN + i];
-            a[o] = fmaf(w, hv, a[o]);
+        for (int o = 0; o < 4; ++o) {
+            acc[o] = fmaf(w[o], hv, acc[o]);
             if (IN != 2) {
-                ax[o] = fmaf(w, tx, ax[o]);
-                ay[o] = fmaf(w, ty, ay[o]);
+                ax[o] = fmaf(w[o], tx, ax[o]);
+                ay[o] = fmaf(w[o], ty, ay[o]);
             }
         }
     }
+    float po[3] = {0.0f, 0.0f, 0.0f}, pgx = 0.0f, pgy = 0.0f;
 #pragma unroll
-    for (int o = 0; o < kGroup; ++o) {
-        const float h = tanhf(a[o] + b[o0 + o]);
+    for (int o = 0; o < 4; ++o) {
+        const int f = f0 + o;
+        const float h = tanhf(acc[o] + a.b[f]);
         const float d = 1.0f - h * h;
         float tx, ty;
         if (IN == 2) {
-            tx = d * (W[(o0 + o) * 2] * scx);
-            ty = d * (W[(o0 + o) * 2 + 1] * scy);
+            tx = d * (Lw[f] * a.scx);
+            ty = d * (Lw[64 + f] * a.scy);
         } else {
             tx = d * ax[o];
             ty = d * ay[o];
         }
-        const size_t k = (size_t)(o0 + o) * n + s;
-        hout[k] = h;
-        txout[k] = tx;
-        tyout[k] = ty;
-    }
-}
-
-/* outputs, forward-mode gradient of o0, residuals and dL/do (thread = sample) */
-__global__ void k_output(const float* W4, const float* b4, const float* h3, const float* t3x, const float* t3y,
-                         const double* u, int n, float lam, float* out3, float* dO, float* loss_s) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    float o[3], gx = 0.0f, gy = 0.0f;
-    for (int c = 0; c < 3; ++c) o[c] = 0.0f;
-    for (int i = 0; i < kH; ++i) {
-        float hv = h3[(size_t)i * n + s];
-        for (int c = 0; c < 3; ++c) o[c] = fmaf(W4[c * kH + i], hv, o[c]);
-        if (u) {
-            gx = fmaf(W4[i], t3x[(size_t)i * n + s], gx);
-            gy = fmaf(W4[i], t3y[(size_t)i * n + s], gy);
+        if (s < n) {
+            const size_t k = (size_t)f * n + s;
+            a.hout[k] = h;
+            if (!HEADS) {
+                a.txout[k] = tx;
+                a.tyout[k] = ty;
+            }
+        }
+        if (HEADS) {
+            for (int q = 0; q < 3; ++q) po[q] = fmaf(a.W4[q * 64 + f], h, po[q]);
+            pgx = fmaf(a.W4[f], tx, pgx);
+            pgy = fmaf(a.W4[f], ty, pgy);
         }
     }
-    for (int c = 0; c < 3; ++c) o[c] += b4[c];
-    for (int c = 0; c < 3; ++c) out3[(size_t)c * n + s] = o[c];
-    if (!u) return;
-    float r0 = o[0] - (float)u[s], r1 = o[1] - gx, r2 = o[2] - gy;
-    float inv = 1.0f / (float)n;
-    dO[s] = 2.0f * r0 * inv;
-    dO[(size_t)n + s] = 2.0f * lam * r1 * inv;
-    dO[(size_t)2 * n + s] = 2.0f * lam * r2 * inv;
-    loss_s[s] = r0 * r0 + lam * (r1 * r1 + r2 * r2);
+    if (!HEADS) return;
+    Lred[og][0][c] = po[0];
+    Lred[og][1][c] = po[1];
+    Lred[og][2][c] = po[2];
+    Lred[og][3][c] = pgx;
+    Lred[og][4][c] = pgy;
+    __syncthreads();
+    if (og != 0 || s >= n) return;
+    float v[5];
+    for (int k = 0; k < 5; ++k) {
+        float t = 0.0f;
+        for (int g = 0; g < kOG; ++g) t += Lred[g][k][c];
+        v[k] = t;
+    }
+    float o3[3];
+    for (int q = 0; q < 3; ++q) {
+        o3[q] = v[q] + a.b4[q];
+        a.out3[(size_t)q * n + s] = o3[q];
+    }
+    if (!a.u) return;
+    const float r0 = o3[0] - (float)a.u[s], r1 = o3[1] - v[3], r2 = o3[2] - v[4];
+    const float inv = 1.0f / (float)n;
+    a.dO[s] = 2.0f * r0 * inv;
+    a.dO[(size_t)n + s] = 2.0f * a.lam * r1 * inv;
+    a.dO[(size_t)2 * n + s] = 2.0f * a.lam * r2 * inv;
+    a.loss[s] = r0 * r0 + a.lam * (r1 * r1 + r2 * r2);
 }
 
-/* dout[i][s] = (1 - h[i][s]^2) sum_o W[o][i] dn[o][s]; thread = sample x kGroup inputs */
-template <int OUT>
-__global__ void __launch_bounds__(256) k_backprop(const float* __restrict__ W, const float* __restrict__ dn,
-                                                  const float* __restrict__ h, float* __restrict__ dout, int n) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i0 = blockIdx.y * kGroup;
-    if (s >= n) return;
-    float a[kGroup];
-#pragma unroll
-    for (int i = 0; i < kGroup; ++i) a[i] = 0.0f;
-    for (int o = 0; o < OUT; ++o) {
-        const float dv = dn[(size_t)o * n + s];
-#pragma unroll
-        for (int i = 0; i < kGroup; ++i) a[i] = fmaf(W[o * kH + i0 + i], dv, a[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < kGroup; ++i) {
-        const float hv = h[(size_t)(i0 + i) * n + s];
-        dout[(size_t)(i0 + i) * n + s] = (1.0f - hv * hv) * a[i];
-    }
-}
-
-/* Weight and bias gradients of all four layers, split over chunks of kChunk
- * samples: part[chunk][param] = sum over the chunk's samples (sequential), both
- * operands staged in LDS.  grid = (chunks, 4 layers). */
-struct WgradArgs {
-    const float* d[4];   /* layer deltas [OUT][n] */
-    const float* x[4];   /* layer inputs [IN][n] */
-    float* part;         /* [chunks][kNParams] */
+struct BpArgs {
+    const float* W;     /* this layer's weights [OUT][IN] */
+    const float* dn;    /* this layer's output deltas [OUT][n] */
+    const float* hprev; /* this layer's input activations [IN][n] (tanh outputs; z for layer 1) */
+    float* dout;        /* previous layer's deltas [IN][n], or nullptr */
+    float* part;        /* [chunk][kNParams] */
+    int offW, offb;
     int n;
 };
-__global__ void __launch_bounds__(256) k_wgrad_partial(WgradArgs a) {
-    constexpr int kPad = kChunk + 1;
-    __shared__ float dL[kH * kPad];
-    __shared__ float xL[kH * kPad];
-    const int L = blockIdx.y, chunk = blockIdx.x, tid = threadIdx.x;
-    const int OUT = (L == 3) ? 3 : kH;
-    const int IN = (L == 0) ? 2 : kH;
-    const int offW = (L == 0) ? oW1 : (L == 1) ? oW2 : (L == 2) ? oW3 : oW4;
-    const int offb = (L == 0) ? ob1 : (L == 1) ? ob2 : (L == 2) ? ob3 : ob4;
-    const float* d = a.d[L];
-    const float* x = a.x[L];
-    const int s0 = chunk * kChunk;
-    for (int k = tid; k < OUT * kChunk; k += 256) {
-        int o = k / kChunk, c = k % kChunk, s = s0 + c;
-        dL[o * kPad + c] = (s < a.n) ? d[(size_t)o * a.n + s] : 0.0f;
+
+/* Backward step of one layer for kTile samples: the previous layer's deltas
+ *   dout[i][s] = (1 - hprev[i][s]^2) sum_o W[o][i] dn[o][s]
+ * (thread t: inputs 4 (t / kTile) .. +3 of sample t % kTile) and this layer's weight / bias
+ * gradient partials over the block's samples, part[block][offW + o IN + i] =
+ * sum_s dn[o][s] hprev[i][s] (samples in order), part[block][offb + o] = sum_s dn[o][s]. */
+template <int OUT, int IN>
+__global__ void __launch_bounds__(256) k_bp_tile(BpArgs a) {
+    __shared__ float Ld[OUT * kPad];
+    __shared__ float Lx[IN * kPad];
+    __shared__ __align__(16) float Lw[(IN == 64) ? OUT * 64 : 4];
+    const int tid = threadIdx.x;
+    const int s0 = blockIdx.x * kTile, n = a.n;
+    for (int k = tid; k < OUT * kTile; k += 256) {
+        const int o = k / kTile, cc = k % kTile, gs = s0 + cc;
+        Ld[o * kPad + cc] = gs < n ? a.dn[(size_t)o * n + gs] : 0.0f;
     }
-    for (int k = tid; k < IN * kChunk; k += 256) {
-        int i = k / kChunk, c = k % kChunk, s = s0 + c;
-        xL[i * kPad + c] = (s < a.n) ? x[(size_t)i * a.n + s] : 0.0f;
+    for (int k = tid; k < IN * kTile; k += 256) {
+        const int i = k / kTile, cc = k % kTile, gs = s0 + cc;
+        Lx[i * kPad + cc] = gs < n ? a.hprev[(size_t)i * n + gs] : 0.0f;
+    }
+    if (IN == 64 && a.dout)
+        for (int k = tid; k < OUT * 64; k += 256) Lw[k] = a.W[k];
+    __syncthreads();
+    if (IN == 64 && a.dout) {
+        const int c = tid % kTile, i0 = 4 * (tid / kTile), s = s0 + c;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int o = 0; o < OUT; ++o) {
+            const float dv = Ld[o * kPad + c];
+            const float4 w4 = *reinterpret_cast<const float4*>(Lw + o * 64 + i0);
+            acc[0] = fmaf(w4.x, dv, acc[0]);
+            acc[1] = fmaf(w4.y, dv, acc[1]);
+            acc[2] = fmaf(w4.z, dv, acc[2]);
+            acc[3] = fmaf(w4.w, dv, acc[3]);
+        }
+        if (s < n) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float hv = Lx[(i0 + q) * kPad + c];
+                a.dout[(size_t)(i0 + q) * n + s] = (1.0f - hv * hv) * acc[q];
+            }
+        }
+    }
+    float* out = a.part + (size_t)blockIdx.x * kNParams;
+    if (OUT == 64 && IN == 64) {
+        /* 4 x 4 register tiles: thread t owns o = 4 (t / 16) .. +3, i = 4 (t % 16) .. +3 */
+        const int o0 = 4 * (tid >> 4), i0 = 4 * (tid & 15);
+        float g[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[p][q] = 0.0f;
+#pragma unroll 4
+        for (int cc = 0; cc < kTile; ++cc) {
+            float dv[4], xv[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) dv[p] = Ld[(o0 + p) * kPad + cc];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xv[q] = Lx[(i0 + q) * kPad + cc];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) g[p][q] = fmaf(dv[p], xv[q], g[p][q]);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) out[a.offW + (o0 + p) * IN + i0 + q] = g[p][q];
+    } else {
+        for (int k = tid; k < OUT * IN; k += 256) {
+            const int o = k / IN, i = k % IN;
+            float acc = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) acc = fmaf(Ld[o * kPad + cc], Lx[i * kPad + cc], acc);
+            out[a.offW + k] = acc;
+        }
+    }
+    for (int o = tid; o < OUT; o += 256) {
+        float acc = 0.0f;
+        for (int cc = 0; cc < kTile; ++cc) acc += Ld[o * kPad + cc];
+        out[a.offb + o] = acc;
+    }
+}
+
+/* One whole training step for kTile samples in one block, all activations in LDS: the
+ * forward pass with both tangents, the heads and dL/do, the backward pass and the weight
+ * and bias gradient partials of the block's samples (part[block][...], samples in order).
+ * Thread t works on sample t % kTile and features / inputs 4 (t / kTile) .. +3; the only
+ * cross-sample sums are the gradient partials, reduced afterwards by k_reduce_adam. */
+struct StepArgs {
+    const float* P;   /* parameters */
+    const float* z;   /* [2][n] scaled inputs */
+    const double* u;  /* targets */
+    float lam;
+    float scx, scy;
+    float* part;      /* [chunk][kNParams] */
+    float* loss;      /* [n] per-sample loss (reports) */
+    int n;
+};
+__global__ void __launch_bounds__(256) k_train_step_tile(StepArgs a) {
+    __shared__ __align__(16) float Lw[64 * 64];
+    __shared__ float Lz[2 * kPad];
+    __shared__ float Lh[3][64 * kPad];
+    __shared__ float Lt[2][2][64 * kPad]; /* [buffer][direction] tangents */
+    __shared__ float Ld[2][64 * kPad];    /* deltas, ping-pong */
+    __shared__ float LdO[3 * kPad];
+    __shared__ float Lred[kOG][5][kTile];
+    const int tid = threadIdx.x, c = tid % kTile, og = tid / kTile, f0 = 4 * og;
+    const int s0 = blockIdx.x * kTile, s = s0 + c, n = a.n;
+    const float* P = a.P;
+    for (int k = tid; k < 2 * kTile; k += 256) {
+        const int i = k / kTile, cc = k % kTile, gs = s0 + cc;
+        Lz[i * kPad + cc] = gs < n ? a.z[(size_t)i * n + gs] : 0.0f;
     }
     __syncthreads();
-    float* out = a.part + (size_t)chunk * kNParams;
-    for (int k = tid; k < OUT * (IN + 1); k += 256) {
-        const int o = k / (IN + 1), i = k % (IN + 1);
-        const float* dr = dL + o * kPad;
-        float acc = 0.0f;
-        if (i < IN) {
-            const float* xr = xL + i * kPad;
-#pragma unroll 8
-            for (int c = 0; c < kChunk; ++c) acc = fmaf(dr[c], xr[c], acc);
-            out[offW + o * IN + i] = acc;
-        } else {
-#pragma unroll 8
-            for (int c = 0; c < kChunk; ++c) acc += dr[c];
-            out[offb + o] = acc;
+    /* layer 1 */
+    {
+        float t0[4], t1[4];
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int f = f0 + o;
+            const float w0 = P[oW1 + 2 * f], w1 = P[oW1 + 2 * f + 1];
+            const float h = tanhf(fmaf(w1, Lz[kPad + c], w0 * Lz[c]) + P[ob1 + f]);
+            const float d = 1.0f - h * h;
+            Lh[0][f * kPad + c] = h;
+            t0[o] = d * (w0 * a.scx);
+            t1[o] = d * (w1 * a.scy);
+        }
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            Lt[0][0][(f0 + o) * kPad + c] = t0[o];
+            Lt[0][1][(f0 + o) * kPad + c] = t1[o];
+        }
+    }
+    /* layers 2 and 3: W^T in LDS, inputs from buffer L-1 */
+    float pg[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int L = 1; L <= 2; ++L) {
+        const float* W = P + (L == 1 ? oW2 : oW3);
+        const float* b = P + (L == 1 ? ob2 : ob3);
+        __syncthreads();
+        for (int k = tid; k < 64 * 64; k += 256) Lw[(k & 63) * 64 + (k >> 6)] = W[k];
+        __syncthreads();
+        const float* hin = Lh[L - 1];
+        const float* txin = Lt[(L - 1) & 1][0];
+        const float* tyin = Lt[(L - 1) & 1][1];
+        float acc[4], ax[4], ay[4];
+#pragma unroll
+        for (int o = 0; o < 4; ++o) acc[o] = ax[o] = ay[o] = 0.0f;
+#pragma unroll 4
+        for (int i = 0; i < 64; ++i) {
+            const float hv = hin[i * kPad + c], tx = txin[i * kPad + c], ty = tyin[i * kPad + c];
+            const float4 w4 = *reinterpret_cast<const float4*>(Lw + i * 64 + f0);
+            const float w[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                acc[o] = fmaf(w[o], hv, acc[o]);
+                ax[o] = fmaf(w[o], tx, ax[o]);
+                ay[o] = fmaf(w[o], ty, ay[o]);
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int f = f0 + o;
+            const float h = tanhf(acc[o] + b[f]);
+            const float d = 1.0f - h * h;
+            Lh[L][f * kPad + c] = h;
+            if (L == 1) {
+                Lt[1][0][f * kPad + c] = d * ax[o];
+                Lt[1][1][f * kPad + c] = d * ay[o];
+            } else {
+                for (int q = 0; q < 3; ++q) pg[q] = fmaf(P[oW4 + q * 64 + f], h, pg[q]);
+                pg[3] = fmaf(P[oW4 + f], d * ax[o], pg[3]);
+                pg[4] = fmaf(P[oW4 + f], d * ay[o], pg[4]);
+            }
+        }
+    }
+    for (int k = 0; k < 5; ++k) Lred[og][k][c] = pg[k];
+    __syncthreads();
+    /* heads, residuals, dL/do */
+    if (og == 0) {
+        float v[5];
+        for (int k = 0; k < 5; ++k) {
+            float t = 0.0f;
+            for (int g = 0; g < kOG; ++g) t += Lred[g][k][c];
+            v[k] = t;
+        }
+        float dO0 = 0.0f, dO1 = 0.0f, dO2 = 0.0f;
+        if (s < n) {
+            const float o0 = v[0] + P[ob4], o1 = v[1] + P[ob4 + 1], o2 = v[2] + P[ob4 + 2];
+            const float r0 = o0 - (float)a.u[s], r1 = o1 - v[3], r2 = o2 - v[4];
+            const float inv = 1.0f / (float)n;
+            dO0 = 2.0f * r0 * inv;
+            dO1 = 2.0f * a.lam * r1 * inv;
+            dO2 = 2.0f * a.lam * r2 * inv;
+            a.loss[s] = r0 * r0 + a.lam * (r1 * r1 + r2 * r2);
+        }
+        LdO[c] = dO0;
+        LdO[kPad + c] = dO1;
+        LdO[2 * kPad + c] = dO2;
+    }
+    __syncthreads();
+    float* out = a.part + (size_t)blockIdx.x * kNParams;
+    /* d3 = (1 - h3^2) W4^T dO; gradient of W4, b4 */
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = f0 + q;
+            float t = 0.0f;
+            for (int o = 0; o < 3; ++o) t = fmaf(P[oW4 + o * 64 + i], LdO[o * kPad + c], t);
+            const float hv = Lh[2][i * kPad + c];
+            Ld[0][i * kPad + c] = (1.0f - hv * hv) * t;
+        }
+        for (int k = tid; k < 3 * 64; k += 256) {
+            const int o = k / 64, i = k % 64;
+            float g = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) g = fmaf(LdO[o * kPad + cc], Lh[2][i * kPad + cc], g);
+            out[oW4 + k] = g;
+        }
+        if (tid < 3) {
+            float g = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) g += LdO[tid * kPad + cc];
+            out[ob4 + tid] = g;
+        }
+    }
+    /* layers 3 and 2: previous deltas and this layer's weight gradient (4 x 4 tiles) */
+    for (int L = 2; L >= 1; --L) {
+        const float* W = P + (L == 2 ? oW3 : oW2);
+        const int offW = (L == 2 ? oW3 : oW2), offb = (L == 2 ? ob3 : ob2);
+        const float* dn = Ld[L & 1 ? 1 : 0];   /* L = 2: Ld[0], L = 1: Ld[1] */
+        float* dprev = Ld[L & 1 ? 0 : 1];
+        const float* xin = Lh[L - 1];
+        __syncthreads();
+        for (int k = tid; k < 64 * 64; k += 256) Lw[k] = W[k];
+        __syncthreads();
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int o = 0; o < 64; ++o) {
+            const float dv = dn[o * kPad + c];
+            const float4 w4 = *reinterpret_cast<const float4*>(Lw + o * 64 + f0);
+            acc[0] = fmaf(w4.x, dv, acc[0]);
+            acc[1] = fmaf(w4.y, dv, acc[1]);
+            acc[2] = fmaf(w4.z, dv, acc[2]);
+            acc[3] = fmaf(w4.w, dv, acc[3]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float hv = xin[(f0 + q) * kPad + c];
+            dprev[(f0 + q) * kPad + c] = (1.0f - hv * hv) * acc[q];
+        }
+        const int o0 = 4 * (tid >> 4), i0 = 4 * (tid & 15);
+        float g[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[p][q] = 0.0f;
+#pragma unroll 4
+        for (int cc = 0; cc < kTile; ++cc) {
+            float dv[4], xv[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) dv[p] = dn[(o0 + p) * kPad + cc];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xv[q] = xin[(i0 + q) * kPad + cc];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) g[p][q] = fmaf(dv[p], xv[q], g[p][q]);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) out[offW + (o0 + p) * 64 + i0 + q] = g[p][q];
+        if (tid < 64) {
+            float gb = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) gb += dn[tid * kPad + cc];
+            out[offb + tid] = gb;
+        }
+    }
+    __syncthreads();
+    /* layer 1: gradient of W1, b1 from d1 (in Ld[0]) and z */
+    {
+        const float* d1 = Ld[0];
+        if (tid < 128) {
+            const int o = tid >> 1, i = tid & 1;
+            float g = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) g = fmaf(d1[o * kPad + cc], Lz[i * kPad + cc], g);
+            out[oW1 + tid] = g;
+        } else if (tid < 192) {
+            const int o = tid - 128;
+            float g = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) g += d1[o * kPad + cc];
+            out[ob1 + o] = g;
         }
     }
 }
@@ -433,38 +714,41 @@ static void enqueue_forward(Mlp* m, bool with_targets, float lam) {
     hipStream_t s = m->stream;
     ActView v = act_view((float*)m->act.ptr, N);
     const float* P = (const float*)m->params.ptr;
-    dim3 blk(256), gH((n + 255) / 256, kH / kGroup), g1((n + 255) / 256);
-    hipLaunchKernelGGL(k_layer_fwd<2>, gH, blk, 0, s, P + oW1, P + ob1, v.z, (const float*)nullptr,
-                       (const float*)nullptr, v.h[0], v.tx[0], v.ty[0], n, 2.0f * m->sx, 2.0f * m->sy);
-    hipLaunchKernelGGL(k_layer_fwd<kH>, gH, blk, 0, s, P + oW2, P + ob2, v.h[0], v.tx[0], v.ty[0], v.h[1], v.tx[1],
-                       v.ty[1], n, 0.0f, 0.0f);
-    hipLaunchKernelGGL(k_layer_fwd<kH>, gH, blk, 0, s, P + oW3, P + ob3, v.h[1], v.tx[1], v.ty[1], v.h[2], v.tx[2],
-                       v.ty[2], n, 0.0f, 0.0f);
-    hipLaunchKernelGGL(k_output, g1, blk, 0, s, P + oW4, P + ob4, v.h[2], v.tx[2], v.ty[2],
-                       with_targets ? (const double*)m->tgt.ptr : (const double*)nullptr, n, lam, v.out3, v.dO,
-                       v.loss);
+    const dim3 blk(256), grid(n_chunks(n));
+    FwdArgs f;
+    std::memset(&f, 0, sizeof f);
+    f.n = n;
+    f.W = P + oW1; f.b = P + ob1; f.hin = v.z; f.hout = v.h[0]; f.txout = v.tx[0]; f.tyout = v.ty[0];
+    f.scx = 2.0f * m->sx; f.scy = 2.0f * m->sy;
+    hipLaunchKernelGGL((k_fwd_tile<2, false>), grid, blk, 0, s, f);
+    f.W = P + oW2; f.b = P + ob2; f.hin = v.h[0]; f.txin = v.tx[0]; f.tyin = v.ty[0];
+    f.hout = v.h[1]; f.txout = v.tx[1]; f.tyout = v.ty[1];
+    hipLaunchKernelGGL((k_fwd_tile<64, false>), grid, blk, 0, s, f);
+    f.W = P + oW3; f.b = P + ob3; f.hin = v.h[1]; f.txin = v.tx[1]; f.tyin = v.ty[1];
+    f.hout = v.h[2]; f.txout = nullptr; f.tyout = nullptr;
+    f.W4 = P + oW4; f.b4 = P + ob4; f.u = with_targets ? (const double*)m->tgt.ptr : nullptr; f.lam = lam;
+    f.out3 = v.out3; f.dO = v.dO; f.loss = v.loss;
+    hipLaunchKernelGGL((k_fwd_tile<64, true>), grid, blk, 0, s, f);
 }
 
-/* One full step: forward, backward, gradient reduction and (adam) the update. */
+/* One full step: forward, backward with per-chunk gradient partials, the fixed-order
+ * reduction and (adam) the update. */
 static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float lr0, float lr1) {
     const int n = m->n;
     const size_t N = (size_t)n;
     hipStream_t s = m->stream;
     ActView v = act_view((float*)m->act.ptr, N);
-    const float* P = (const float*)m->params.ptr;
-    enqueue_forward(m, true, lam);
-    dim3 blk(256), gH((n + 255) / 256, kH / kGroup);
-    hipLaunchKernelGGL(k_backprop<3>, gH, blk, 0, s, P + oW4, v.dO, v.h[2], v.d3, n);
-    hipLaunchKernelGGL(k_backprop<kH>, gH, blk, 0, s, P + oW3, v.d3, v.h[1], v.d2, n);
-    hipLaunchKernelGGL(k_backprop<kH>, gH, blk, 0, s, P + oW2, v.d2, v.h[0], v.d1, n);
-    WgradArgs w;
-    w.d[0] = v.d1; w.x[0] = v.z;
-    w.d[1] = v.d2; w.x[1] = v.h[0];
-    w.d[2] = v.d3; w.x[2] = v.h[1];
-    w.d[3] = v.dO; w.x[3] = v.h[2];
-    w.part = (float*)m->part.ptr;
-    w.n = n;
-    hipLaunchKernelGGL(k_wgrad_partial, dim3(n_chunks(n), 4), dim3(256), 0, s, w);
+    StepArgs st;
+    st.P = (const float*)m->params.ptr;
+    st.z = v.z;
+    st.u = (const double*)m->tgt.ptr;
+    st.lam = lam;
+    st.scx = 2.0f * m->sx;
+    st.scy = 2.0f * m->sy;
+    st.part = (float*)m->part.ptr;
+    st.loss = v.loss;
+    st.n = n;
+    hipLaunchKernelGGL(k_train_step_tile, dim3(n_chunks(n)), dim3(256), 0, s, st);
     AdamArgs a;
     a.p = (float*)m->params.ptr;
     a.g = (float*)m->grads.ptr;
