@@ -76,6 +76,7 @@ struct CoefFranke {
     static constexpr bool kConstSigma = true;
     static constexpr double kSigma = 1.41421356237;
     __device__ __forceinline__ static double sigma(double, double) { return kSigma; }
+    __device__ __forceinline__ static StepCoef step_s(double x, double y, double) { return step(x, y); }
     __device__ __forceinline__ static StepCoef step(double x, double y) {
         FrankeTerms t = franke_terms(x, y);
         StepCoef c;
@@ -100,10 +101,11 @@ struct CoefVarFranke {
     __device__ __forceinline__ static double sigma(double x, double y) {
         return 1.0 + 0.5 * spac_sinpi(x) * spac_cospi(y);
     }
-    __device__ __forceinline__ static StepCoef step(double x, double y) {
+    __device__ __forceinline__ static StepCoef step(double x, double y) { return step_s(x, y, sigma(x, y)); }
+    /* s = sigma(x, y), already known to the caller */
+    __device__ __forceinline__ static StepCoef step_s(double x, double y, double s) {
         FrankeTerms t = franke_terms(x, y);
         StepCoef c;
-        double s = sigma(x, y);
         c.s = s;
         c.bx = 0.3 * y;
         c.by = -0.3 * x;
@@ -232,6 +234,7 @@ struct CoefFranke32 {
     static constexpr bool kHasKilling = false;
     using F64 = CoefFranke;
     __device__ __forceinline__ static float sigma(float, float) { return 1.41421356237f; }
+    __device__ __forceinline__ static StepCoef32 step_s(float x, float y, float) { return step(x, y); }
     __device__ __forceinline__ static StepCoef32 step(float x, float y) {
         FrankeTerms32 t = franke_terms32(x, y);
         StepCoef32 c;
@@ -249,10 +252,10 @@ struct CoefVarFranke32 {
     static constexpr bool kHasKilling = true;
     using F64 = CoefVarFranke;
     __device__ __forceinline__ static float sigma(float x, float y) { return 1.0f + 0.5f * fsinpi(x) * fcospi(y); }
-    __device__ __forceinline__ static StepCoef32 step(float x, float y) {
+    __device__ __forceinline__ static StepCoef32 step(float x, float y) { return step_s(x, y, sigma(x, y)); }
+    __device__ __forceinline__ static StepCoef32 step_s(float x, float y, float s) {
         FrankeTerms32 t = franke_terms32(x, y);
         StepCoef32 c;
-        float s = sigma(x, y);
         c.s = s;
         c.bx = 0.3f * y;
         c.by = -0.3f * x;

@@ -61,6 +61,8 @@ struct PathArgs {
     double gobet;
     uint64_t seed;
     const uint64_t* seed_dev; /* non-null: the seed is read here (captured graphs) */
+    int32_t sigma_fresh;      /* 1: re-evaluate sigma(X) each step (a stop rule that can keep a path
+                                 alive outside the boundary, gobet_constant >= 0) */
     int32_t max_steps;
     int32_t n_items;
     /* solves */
@@ -125,6 +127,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     int32_t nstep = 0;
     double X = 0.0, Yc = 0.0, Yw = 0.0, Z = 0.0, xi = 0.0, t = 0.0;
     double h = 0.0, sqrth = 0.0, R = 0.0, cx = 0.0, cy = 0.0;
+    double sig = 0.0; /* sigma at the current X */
     uint32_t stream = 0;
     int64_t rec = 0;
     /* wave-uniform pool */
@@ -171,6 +174,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                 }
                 X = a.x0[solve];
                 Yc = a.y0[solve];
+                sig = Coef::sigma(X, Yc);
                 Yw = 1.0; Z = 0.0; xi = 0.0; t = 0.0;
                 nstep = 0;
                 active = true;
@@ -182,7 +186,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
         if (__ballot(active) == 0ull) break;
         if (active) {
             /* Paso_Absorbente (Solvers/FeynmanKacSolver.cpp:7-18) */
-            StepCoef cf = Coef::step(X, Yc);
+            StepCoef cf = Coef::step_s(X, Yc, sig);
             double n0, n1;
             spac_normal_pair(seed, stream, path, (uint32_t)nstep, &n0, &n1);
             double dWx = sqrth * n0, dWy = sqrth * n1;
@@ -221,7 +225,11 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                     hit.d = -__builtin_inf(); hit.ex = X; hit.ey = Yc; hit.nx = 0.0; hit.ny = 0.0;
                 }
             }
-            double s = (hit.d < 0.0) ? Coef::sigma(X, Yc) : Coef::sigma(hit.ex, hit.ey);
+            /* sigma at X inside the boundary, at the projection outside (one evaluation) */
+            const double px = (hit.d < 0.0) ? X : hit.ex, py = (hit.d < 0.0) ? Yc : hit.ey;
+            double s = Coef::sigma(px, py);
+            /* a path that goes on is inside (d < gobet |N sigma| sqrt h <= 0): s = sigma(X) */
+            sig = a.sigma_fresh ? Coef::sigma(X, Yc) : s;
             double r0 = hit.nx * s, r1 = hit.ny * s;
             double nrm = __builtin_sqrt(r0 * r0 + r1 * r1);
             bool inside = hit.d < a.gobet * nrm * sqrth;
@@ -257,6 +265,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
     uint32_t path = 0;
     int32_t nstep = 0;
     float xf = 0.f, yf = 0.f, hf = 0.f, sqrthf = 0.f, Rf = 0.f, cxf = 0.f, cyf = 0.f;
+    float sigf = 0.f; /* sigma at the current X */
     double Yw = 0.0, Z = 0.0, xi = 0.0, h = 0.0, R = 0.0, cx = 0.0, cy = 0.0;
     uint32_t stream = 0;
     int64_t rec = 0;
@@ -307,6 +316,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
                 }
                 xf = (float)a.x0[solve];
                 yf = (float)a.y0[solve];
+                sigf = Coef32::sigma(xf, yf);
                 Yw = 1.0; Z = 0.0; xi = 0.0;
                 nstep = 0;
                 active = true;
@@ -317,7 +327,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
         }
         if (__ballot(active) == 0ull) break;
         if (active) {
-            StepCoef32 cf = Coef32::step(xf, yf);
+            StepCoef32 cf = Coef32::step_s(xf, yf, sigf);
             float n0, n1;
             normal_pair32(seed, stream, path, (uint32_t)nstep, &n0, &n1);
             const float dWx = sqrthf * n0, dWy = sqrthf * n1;
@@ -351,7 +361,9 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
                 use_dom = d >= ds;
                 if (!use_dom) { d = ds; nx = dx * ir; ny = dy * ir; ex = nx * Rf + cxf; ey = ny * Rf + cyf; }
             }
-            float sg = (d < 0.0f) ? Coef32::sigma(xf, yf) : Coef32::sigma(ex, ey);
+            const float pxf = (d < 0.0f) ? xf : ex, pyf = (d < 0.0f) ? yf : ey;
+            float sg = Coef32::sigma(pxf, pyf);
+            sigf = a.sigma_fresh ? Coef32::sigma(xf, yf) : sg;
             float r0 = nx * sg, r1 = ny * sg;
             bool inside = d < gobetf * __builtin_sqrtf(r0 * r0 + r1 * r1) * sqrthf;
             int8_t out = inside ? (int8_t)SPAC_OUT_INSIDE : (use_dom ? (int8_t)SPAC_OUT_DOMAIN : (int8_t)SPAC_OUT_SUBDOMAIN);
@@ -642,6 +654,7 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
     memset(&a, 0, sizeof a);
     for (int i = 0; i < 4; ++i) a.dom[i] = desc.domain_params[i];
     a.gobet = desc.gobet_constant;
+    a.sigma_fresh = desc.gobet_constant >= 0.0 ? 1 : 0;
     a.seed = seed;
     a.seed_dev = sdev;
     a.max_steps = desc.max_steps > 0 ? desc.max_steps : 0x7fffffff;
