@@ -51,7 +51,43 @@ int FeynmanKacSolver::ensure_problem(const BVP& P, bool with_subdomain) {
     int rc = spac_fk_problem_create(&d, &prob_);
     if (rc) return rc;
     desc_ = d;
-    return SPAC_OK;
+    return apply_cv();
+}
+
+int FeynmanKacSolver::apply_cv() {
+    if (!prob_) return SPAC_OK; /* applied when the problem is created */
+    spac_cv_config c = cv_;
+    c.table = cv_table_.empty() ? nullptr : cv_table_.data();
+    return spac_fk_set_control_variate(prob_, &c);
+}
+
+int FeynmanKacSolver::Variable_Control_Plugin() {
+    std::memset(&cv_, 0, sizeof cv_);
+    cv_.mode = SPAC_CV_ANALYTIC;
+    cv_table_.clear();
+    return apply_cv();
+}
+
+int FeynmanKacSolver::Sin_Variable_Control() {
+    std::memset(&cv_, 0, sizeof cv_);
+    cv_.mode = SPAC_CV_NONE;
+    cv_table_.clear();
+    return apply_cv();
+}
+
+int FeynmanKacSolver::Variable_Control(const Interpolador_Tabla& u_hat, bool fallback, double max_ratio) {
+    if (u_hat.tabla().empty()) return set_error(SPAC_E_INVALID_ARG, "Interpolador_Tabla has no valid table");
+    std::memset(&cv_, 0, sizeof cv_);
+    cv_.mode = SPAC_CV_TABLE;
+    cv_.fallback = fallback ? 1 : 0;
+    cv_.max_ratio = max_ratio;
+    cv_.table_size = u_hat.valores.T;
+    cv_.table_domain[0] = u_hat.valores.xmin;
+    cv_.table_domain[1] = u_hat.valores.ymin;
+    cv_.table_domain[2] = u_hat.valores.xmax;
+    cv_.table_domain[3] = u_hat.valores.ymax;
+    cv_table_ = u_hat.tabla();
+    return apply_cv();
 }
 
 void FeynmanKacSolver::copy_stats(const spac_point_stats& s) {

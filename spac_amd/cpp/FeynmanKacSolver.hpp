@@ -25,6 +25,7 @@
 #include "../../include/spac_fk.h"
 #include "BVP.hpp"
 #include "Interfaz.hpp"
+#include "Interpolador.hpp"
 #include "Nudo.hpp"
 
 namespace spac {
@@ -56,6 +57,14 @@ public:
      * nodes) and nudo.B */
     int Solve_OMP_Analytic(Nudo& nudo, unsigned int numero_trayectorias, double discretizacion_temporal, double rho,
                            const BVP& Problema, const Interfaz& interfaz);
+    /* Control variate F of xi += Y F(X).dW (Solvers/FeynmanKacSolver.cpp:13):
+     * the BVP's F plugin (the reference's behaviour, default), none (plain Monte
+     * Carlo), or F = -sigma grad u_hat from a tabulated u_hat read on the device;
+     * fallback drops xi per node where it adds variance. */
+    int Variable_Control_Plugin();
+    int Sin_Variable_Control();
+    int Variable_Control(const Interpolador_Tabla& u_hat, bool fallback = true, double max_ratio = 1.0);
+
     /* Batched form: every (node, interface) pair in one GPU launch. */
     int Solve_Nodes(std::vector<Nudo*>& nudos, const std::vector<const Interfaz*>& interfaces,
                     unsigned int numero_trayectorias, double discretizacion_temporal, const BVP& Problema);
@@ -65,6 +74,9 @@ private:
     spac_problem_desc desc_{};
     uint64_t seed_ = 0;
     uint32_t calls_ = 0;
+    spac_cv_config cv_{};           /* applied to every problem this solver creates */
+    std::vector<float> cv_table_;
+    int apply_cv();
     int ensure_problem(const BVP& P, bool with_subdomain);
     void copy_stats(const spac_point_stats& s);
 };

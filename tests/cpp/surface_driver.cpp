@@ -6,6 +6,8 @@
  *   surface_driver gpu   FK rows of every perimeter node through
  *                        FeynmanKacSolver::Solve_OMP_Analytic(Nudo&, ...),
  *                        accumulated with SistemaLineal::Actualiza_GB
+ *   surface_driver cv    one point solve with the plugin control variate, none,
+ *                        and a tabulated u_hat (TablaDeValores + Interpolador_Tabla)
  * Prints one JSON object on stdout.
  */
 #include <cmath>
@@ -19,11 +21,29 @@
 #include "BVPs.hpp"
 #include "Dominios.hpp"
 #include "FeynmanKacSolver.hpp"
+#include "Interpolador.hpp"
 #include "Malla.hpp"
 
 using namespace spac;
 
 static double unregistered_f(Vec2 x) { return x(0); }
+
+/* Franke u and its gradient on a T x T grid over [x0, x1]^2 */
+static TablaDeValores franke_table(int T, double x0, double x1, bool with_derivatives) {
+    TablaDeValores tab;
+    std::vector<double> z((size_t)T * T), dx, dy;
+    if (with_derivatives) { dx.resize(z.size()); dy.resize(z.size()); }
+    tab.Inicializa_TablaDeValores(x0, x0, x1, x1, T, {});
+    for (int j = 0; j < T; ++j)
+        for (int i = 0; i < T; ++i) {
+            Vec2 X(tab.x(i), tab.y(j));
+            const size_t k = (size_t)j * T + i;
+            z[k] = Franke_u(X);
+            if (with_derivatives) { dx[k] = Franke_dudx(X); dy[k] = Franke_dudy(X); }
+        }
+    tab.Inicializa_TablaDeValores(x0, x0, x1, x1, T, z, dx, dy);
+    return tab;
+}
 
 int main(int argc, char** argv) {
     std::string mode = argc > 1 ? argv[1] : "cpu";
@@ -66,11 +86,48 @@ int main(int argc, char** argv) {
         int urc = it.Update_G(1.0, X, G);
         double sumG = 0.0;
         for (auto& kv : G) sumG += kv.second;
+        /* TablaDeValores + Interpolador_Tabla on the host */
+        Interpolador_Tabla itab, itfd;
+        itab.Inicializa(franke_table(513, 0.0, 1.25, true));
+        itfd.Inicializa(franke_table(513, 0.0, 1.25, false));
+        double eu = 0.0, eg = 0.0, egfd = 0.0, gmax = 0.0;
+        for (int k = 0; k < 400; ++k) {
+            Vec2 X(0.0125 + 1.225 * std::fmod(0.618034 * k, 1.0), 0.0125 + 1.225 * std::fmod(0.414214 * k, 1.0));
+            eu = std::fmax(eu, std::fabs(itab.Interpola(X) - Franke_u(X)));
+            eg = std::fmax(eg, std::fmax(std::fabs(itab.Deriva_x(X) - Franke_dudx(X)), std::fabs(itab.Deriva_y(X) - Franke_dudy(X))));
+            egfd = std::fmax(egfd, std::fmax(std::fabs(itfd.Deriva_x(X) - Franke_dudx(X)),
+                                             std::fabs(itfd.Deriva_y(X) - Franke_dudy(X))));
+            gmax = std::fmax(gmax, std::fmax(std::fabs(Franke_dudx(X)), std::fabs(Franke_dudy(X))));
+        }
+        Interpolador base;
+        const bool base_nan = std::isnan(base.Interpola(Vec2(0.1, 0.1))) && std::isnan(base.Deriva_x(Vec2(0.1, 0.1)));
         std::printf("{\"mesh_rc\": %d, \"nodes\": %d, \"nnz\": %d, \"coef_rc\": %d, \"coef_kind\": %d, "
                     "\"dom_rc\": %d, \"dom_kind\": %d, \"sub_rc\": %d, \"sub_kind\": %d, \"bad_rc\": %d, "
-                    "\"update_g_rc\": %d, \"update_g_sum\": %.17g}\n",
+                    "\"update_g_rc\": %d, \"update_g_sum\": %.17g, \"interp_err_u\": %.3e, \"interp_err_grad\": %.3e, "
+                    "\"interp_err_grad_fd\": %.3e, \"grad_max\": %.3e, \"base_nan\": %d}\n",
                     mrc, mesh.numero_total_nudos, mesh.sistema.nnz, rc_coef, kind, rc_dom, dk, rc_sub, sk, rc_bad, urc,
-                    sumG);
+                    sumG, eu, eg, egfd, gmax, base_nan ? 1 : 0);
+        return 0;
+    }
+
+    if (mode == "cv") {
+        BVP pb = bvp;
+        pb.frontera_subdominio = Frontera();
+        Interpolador_Tabla it;
+        it.Inicializa(franke_table(513, 0.0, 1.25, true));
+        FeynmanKacSolver s_plugin, s_none, s_table;
+        int r1 = s_plugin.Solve_OMP_Analytic(Vec2(0.6, 0.7), 4000, 1e-3, 0.1, pb);
+        int r2 = s_none.Sin_Variable_Control();
+        if (!r2) r2 = s_none.Solve_OMP_Analytic(Vec2(0.6, 0.7), 4000, 1e-3, 0.1, pb);
+        int r3 = s_table.Variable_Control(it, false);
+        if (!r3) r3 = s_table.Solve_OMP_Analytic(Vec2(0.6, 0.7), 4000, 1e-3, 0.1, pb);
+        FeynmanKacSolver* S[3] = {&s_plugin, &s_none, &s_table};
+        std::printf("{\"rc\": [%d, %d, %d], \"u\": %.17g, \"modes\": [", r1, r2, r3, Franke_u(Vec2(0.6, 0.7)));
+        for (int k = 0; k < 3; ++k)
+            std::printf("%s{\"phi\": %.17g, \"phiphi\": %.17g, \"phi_VR\": %.17g, \"phiphi_VR\": %.17g, "
+                        "\"xixi\": %.17g}", k ? ", " : "", S[k]->phi, S[k]->phiphi, S[k]->phi_VR, S[k]->phiphi_VR,
+                        S[k]->xixi);
+        std::printf("]}\n");
         return 0;
     }
 

@@ -35,6 +35,12 @@ def test_surface_cpu():
     m = Mesh(mesh_desc(domain=(0.0, 0.0, 1.25, 1.25), n_side=4, nodes_per_circle=44, overlap=1.0, theta0=0.1428,
                        rbf_kind=abi.SPAC_RBF_IMQ_GLOBAL, fk_all_circles=False))
     assert r["nnz"] == m.info.nnz
+    # TablaDeValores + Interpolador_Tabla (513^2 grid of Franke u on [0, 1.25]^2): bilinear
+    # errors of order h^2 |D2 u| / 8 for u and h^2 |D3 u| for the gradient; the base class is NaN
+    assert r["interp_err_u"] < 2e-4
+    assert r["interp_err_grad"] < 3e-3 * r["grad_max"]
+    assert r["interp_err_grad_fd"] < 5e-3 * r["grad_max"]
+    assert r["base_nan"] == 1
 
 
 @pytest.mark.gpu
@@ -66,3 +72,23 @@ def test_surface_gpu_rows_match_restatement(gpu):
     so = o.reduce(b, o.simulate(problem_desc(domain=(0.0, 0.0, 1.25, 1.25)), b, seed=0))[0]
     assert r["point_rc"] == 0 and r["point_phi"] == so["phi"] and r["point_phi_VR"] == so["phi_VR"]
     assert r["point_steps"] == so["path_steps"]
+
+
+@pytest.mark.gpu
+def test_surface_gpu_tabulated_control_variate(gpu):
+    """FeynmanKacSolver with the plugin control variate, none, and a tabulated u_hat
+    (Interpolador_Tabla of the exact Franke u): the same paths, so phi is identical; the
+    table reproduces the plugin's variance reduction."""
+    print("surface driver cv", flush=True)
+    r = run_driver("cv")
+    assert r["rc"] == [0, 0, 0]
+    plugin, none, table = r["modes"]
+    assert plugin["phi"] == none["phi"] == table["phi"]
+    var = plugin["phiphi"] - plugin["phi"] ** 2
+    var_plugin = plugin["phiphi_VR"] - plugin["phi_VR"] ** 2
+    var_table = table["phiphi_VR"] - table["phi_VR"] ** 2
+    assert none["xixi"] == 0.0 and none["phi_VR"] == none["phi"]
+    assert var_plugin < 0.25 * var
+    assert abs(var_table / var_plugin - 1.0) < 0.05
+    assert abs(table["phi_VR"] - plugin["phi_VR"]) < 1e-3
+    assert abs(table["phi_VR"] - r["u"]) < 4 * np.sqrt(var_table / 4000) + 5e-3
