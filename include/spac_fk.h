@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SPAC_ABI_VERSION 2
+#define SPAC_ABI_VERSION 3
 
 typedef int32_t spac_status;
 enum {
@@ -51,8 +51,11 @@ enum {
  * F, mu, sigma) registered by the C++ plugin surface. */
 enum {
     SPAC_COEF_FRANKE_POISSON = 1, /* BVPs/Franke_Poisson.hpp: sigma = 1.41421356237 I, b = c = 0 */
-    SPAC_COEF_VARCOEF_FRANKE = 2  /* sigma = (1 + 0.5 sin(pi x) cos(pi y)) I, b = 0.3 (y, -x),
+    SPAC_COEF_VARCOEF_FRANKE = 2, /* sigma = (1 + 0.5 sin(pi x) cos(pi y)) I, b = 0.3 (y, -x),
                                      c = -1 - x^2, u = Franke, f = -(L u) */
+    SPAC_COEF_FRANKE_ROBIN = 3    /* Franke Poisson plus the Robin data of reflecting sides:
+                                     varphi = -0.5, psi(X, N) = grad u(X).N + 0.5 u(X), so that
+                                     du/dN - varphi u = psi holds for the Franke u */
 };
 
 /* Boundary distance plugins (Dominios/Rectangulo.hpp:7-229, Dominios/Circulo.hpp:7-22). */
@@ -60,6 +63,16 @@ enum {
     SPAC_BOUNDARY_NONE = 0,      /* no subdomain: distance -inf, never the nearest */
     SPAC_BOUNDARY_RECTANGLE = 1, /* params: xmin, ymin, xmax, ymax */
     SPAC_BOUNDARY_CIRCLE = 2     /* params: R, cx, cy */
+};
+
+/* Reflecting sides of the rectangle domain (spac_problem_desc.reflecting), the
+ * device form of Frontera::Absorbente returning false on a side
+ * (BVP/Frontera.hpp:23,34).  A corner reflects when both of its sides do. */
+enum {
+    SPAC_REFLECT_XMIN = 1,
+    SPAC_REFLECT_YMIN = 2,
+    SPAC_REFLECT_XMAX = 4,
+    SPAC_REFLECT_YMAX = 8
 };
 
 enum { SPAC_FP64 = 0, SPAC_FP32 = 1 };
@@ -88,7 +101,11 @@ typedef struct {
     double subdomain_params[4];
     double gobet_constant;      /* -0.5826 in the reference (Solvers/FeynmanKacSolver.cpp:268,283) */
     int32_t max_steps;          /* cap on steps per path; the reference has none */
-    int32_t reserved;
+    int32_t reflecting;         /* SPAC_REFLECT_* mask; 0 (every side absorbing) is the reference's
+                                   shipped configuration.  Nonzero needs SPAC_FP64. */
+    double reflect_rho;         /* rho of Solve_OMP_Analytic (Solvers/FeynmanKacSolver.hpp:95-98):
+                                   paths closer than rho to a reflecting side take the
+                                   Lepingle step (Solvers/FeynmanKacSolver.cpp:19-67) */
 } spac_problem_desc;
 
 typedef struct spac_problem spac_problem;

@@ -63,6 +63,8 @@ typedef struct {
     pfvector b, gradiente_u, F, mu;
     pfmatriz sigma;
     frontera_t frontera_dominio, frontera_subdominio;
+    int reflecting;  /* SPAC_REFLECT_* sides of the rectangle domain */
+    double rho;      /* Lepingle band */
 } bvp_t;
 
 /* ------------------------------------------------------------ defaults --
@@ -124,6 +126,16 @@ static vec2 franke_F(vec2 X) {
     vec2 r = {-(s.a00 * g.x + s.a10 * g.y), -(s.a01 * g.x + s.a11 * g.y)};
     return r;
 }
+
+/* Robin data for reflecting sides with the Franke solution: varphi = -0.5 and
+ * psi(X, N) = grad u(X).N + 0.5 u(X), so that du/dN - varphi u = psi holds on
+ * every side (the reflected Feynman–Kac representation needs exactly this
+ * boundary relation).  Pinned by: tests/test_reflect.py (the estimator of
+ * reflected paths converges to the Franke u). */
+static double robin_psi(vec2 X, vec2 N) {
+    return franke_dudx(X) * N.x + franke_dudy(X) * N.y + 0.5 * franke_u(X);
+}
+static double robin_varphi(vec2 X, vec2 N) { (void)X; (void)N; return -0.5; }
 
 /* ----------------------------------------- variable-coefficient Franke --
  * Operator L = 1/2 tr(sigma sigma^T D2) + b.grad + c with sigma = s(x) I,
@@ -248,25 +260,51 @@ static int bvp_init(bvp_t* bvp, const spac_problem_desc* d) {
         bvp->b = vc_b; bvp->gradiente_u = franke_gradu; bvp->F = vc_F;
         bvp->sigma = vc_sigma;
         break;
+    case SPAC_COEF_FRANKE_ROBIN:
+        bvp->u = franke_u; bvp->g = franke_u; bvp->c = franke_c; bvp->f = franke_f;
+        bvp->b = franke_b; bvp->gradiente_u = franke_gradu; bvp->F = franke_F;
+        bvp->sigma = franke_sigma;
+        bvp->psi = robin_psi; bvp->varphi = robin_varphi;
+        break;
     default:
         return SPAC_E_UNREGISTERED_PLUGIN;
     }
+    if (d->reflecting & ~15) return SPAC_E_INVALID_ARG;
+    if (d->reflecting && d->domain_kind != SPAC_BOUNDARY_RECTANGLE) return SPAC_E_INVALID_ARG;
+    bvp->reflecting = d->reflecting;
+    bvp->rho = d->reflect_rho;
     frontera_init(&bvp->frontera_dominio, d->domain_kind, d->domain_params);
     frontera_init(&bvp->frontera_subdominio, d->subdomain_kind, d->subdomain_params);
     (void)escalar_defecto;
     return SPAC_OK;
 }
 
+/* Frontera::Evalua_Absorbente (BVP/Frontera.hpp:23,34) for the rectangle with a
+ * reflecting-side mask: the side, or the corner, is read off the outward normal
+ * the distance plugin returns; a corner reflects only when both sides do. */
+static int side_reflects(int mask, vec2 N) {
+    int bits = 0;
+    if (N.x < 0.0) bits |= SPAC_REFLECT_XMIN;
+    if (N.x > 0.0) bits |= SPAC_REFLECT_XMAX;
+    if (N.y < 0.0) bits |= SPAC_REFLECT_YMIN;
+    if (N.y > 0.0) bits |= SPAC_REFLECT_YMAX;
+    return bits != 0 && (bits & mask) == bits;
+}
+
 /* ----------------------------------------------------------- Dentro --
- * Solvers/FeynmanKacSolver.cpp:68-106 (absorbing boundaries: Stopping
- * returns true for every shipped frontier, Dominios/Rectangulo.hpp:230-232). */
+ * Solvers/FeynmanKacSolver.cpp:68-106.  Absorbing frontiers stop a path at the
+ * Gobet-shifted band; a reflecting domain side (:88-90) keeps it, and a point
+ * beyond it is reported as reflejada for the caller to project (:93-96).
+ * *refl says whether the next step is the reflecting one (the reference's
+ * ccabsorbentes == false). */
 static int dentro(const bvp_t* bvp, vec2 X, double sqrth, double gobet, vec2* Npro, vec2* N,
-                  double* distancia) {
+                  double* distancia, int* refl) {
     vec2 Npro_aux, N_aux;
     double distancia_aux = bvp->frontera_dominio.distancia(bvp->frontera_dominio.params, X, &Npro_aux, &N_aux);
     double d = bvp->frontera_subdominio.distancia(bvp->frontera_subdominio.params, X, Npro, N);
     mat2 s;
     int status;
+    *refl = 0;
     if (distancia_aux >= d) {
         d = distancia_aux;
         *Npro = Npro_aux;
@@ -275,7 +313,12 @@ static int dentro(const bvp_t* bvp, vec2 X, double sqrth, double gobet, vec2* Np
         double r0 = N->x * s.a00 + N->y * s.a10;
         double r1 = N->x * s.a01 + N->y * s.a11;
         double nrm = sqrt(r0 * r0 + r1 * r1);
-        status = (d < gobet * nrm * sqrth) ? SPAC_OUT_INSIDE : SPAC_OUT_DOMAIN;
+        if (bvp->reflecting && side_reflects(bvp->reflecting, *N)) {
+            *refl = 1;
+            status = (d <= 0.0) ? SPAC_OUT_INSIDE : SPAC_OUT_REFLECTED;
+        } else {
+            status = (d < gobet * nrm * sqrth) ? SPAC_OUT_INSIDE : SPAC_OUT_DOMAIN;
+        }
     } else {
         s = (d < 0.0) ? bvp->sigma(X) : bvp->sigma(*Npro);
         double r0 = N->x * s.a00 + N->y * s.a10;
@@ -318,6 +361,23 @@ int oracle_set_cv(int mode, int fallback, double max_ratio, const float* table, 
     return SPAC_OK;
 }
 
+/* xi += Y F(X).dW for the selected control variate */
+static double cv_increment(const bvp_t* bvp, vec2 X, mat2 sig, double Y, vec2 dW) {
+    if (g_cv.mode == SPAC_CV_ANALYTIC) {
+        vec2 F = bvp->F(X);
+        return Y * (F.x * dW.x + F.y * dW.y);
+    }
+    if (g_cv.mode == SPAC_CV_TABLE) {
+        int32_t k00;
+        double ax, ay;
+        spac_cv_cell(&g_cv.tab, X.x, X.y, &k00, &ax, &ay);
+        double Fx = -(sig.a00 * spac_cv_eval(&g_cv.tab, k00, ax, ay, 1));
+        double Fy = -(sig.a00 * spac_cv_eval(&g_cv.tab, k00, ax, ay, 2));
+        return Y * (Fx * dW.x + Fy * dW.y);
+    }
+    return 0.0;
+}
+
 /* --------------------------------------------------- Paso_Absorbente --
  * Solvers/FeynmanKacSolver.cpp:7-18, update order kept: Z, xi, Y, X, t.
  * ji_t is 0 for absorbing boundaries (:101-103). */
@@ -332,17 +392,7 @@ static void paso_absorbente(const bvp_t* bvp, path_state* st, double h, double s
     vec2 dW = {sqrth * n0, sqrth * n1};
     double Y = st->Y;
     st->Z = st->Z + (bvp->f(st->X) * Y * h + bvp->psi(st->X, st->N) * Y * ji_t);
-    if (g_cv.mode == SPAC_CV_ANALYTIC) {
-        vec2 F = bvp->F(st->X);
-        st->xi = st->xi + Y * (F.x * dW.x + F.y * dW.y);
-    } else if (g_cv.mode == SPAC_CV_TABLE) {
-        int32_t k00;
-        double ax, ay;
-        spac_cv_cell(&g_cv.tab, st->X.x, st->X.y, &k00, &ax, &ay);
-        double Fx = -(sig.a00 * spac_cv_eval(&g_cv.tab, k00, ax, ay, 1));
-        double Fy = -(sig.a00 * spac_cv_eval(&g_cv.tab, k00, ax, ay, 2));
-        st->xi = st->xi + Y * (Fx * dW.x + Fy * dW.y);
-    }
+    if (g_cv.mode != SPAC_CV_NONE) st->xi = st->xi + cv_increment(bvp, st->X, sig, Y, dW);
     vec2 mu = bvp->mu(st->X);
     st->Y = Y + (bvp->c(st->X) * Y * h + Y * (mu.x * dW.x + mu.y * dW.y) +
                  bvp->varphi(st->X, st->N) * Y * ji_t);
@@ -357,6 +407,73 @@ static void paso_absorbente(const bvp_t* bvp, path_state* st, double h, double s
     st->t = st->t + h;
 }
 
+/* -------------------------------------------------- Paso_Reflejante --
+ * Solvers/FeynmanKacSolver.cpp:19-67, restated as the Lepingle scheme it
+ * implements (D. Lepingle, Euler scheme for reflected stochastic differential
+ * equations, Math. Comput. Simulation 38 (1995) 119-126).  With N the outward
+ * normal of the nearest reflecting side and d <= 0 the signed distance of X:
+ *   X' = X + (b - sigma mu) h + sigma dW;
+ *   if d > -rho: the largest outward excursion of the step's Brownian bridge,
+ *     nu = (uc + sqrt(|N^T sigma|^2 omega + uc^2)) / 2, uc = N.(X' - X),
+ *     omega = 2 h E with E unit exponential (the reference's comment at :30:
+ *     rate 1/(2h)), gives the local-time increment chi = max(0, nu + d) and
+ *     X' -= chi N;
+ *   a point still beyond a reflecting side is projected onto it and the
+ *   overshoot is added to chi (:40-45, :51-56);
+ *   Z += f Y h + psi(W, N) Y chi, xi += Y F.dW, Y += c Y h + Y mu.dW +
+ *   varphi(W, N) Y chi, t += h (update order of :58-66), with the Robin
+ *   data psi, varphi read at W = X - d N, the foot of X on the side: they are
+ *   boundary data, and the reference's psi(X, N) at the interior point adds a
+ *   bias of order rho.
+ * The reference's defects are not reproduced: its rejection loop reads an
+ * uninitialised point (:24,38), its d_k is the distance of an earlier point
+ * (clamped to 0 when negative, :35; set from a stale value at :182), its
+ * exponential has mean 1 instead of 2h, and it steps the clock by -h (:65).
+ * chi enters with the overshoot of a reflejada Dentro, which the reference
+ * drops.  Pinned by: tests/test_reflect.py (pure projection at rho = 0 against
+ * a hand-computed walk; reflected estimator converges to the Franke u under
+ * Robin data; unit-exponential moments of the draws). */
+static void paso_reflejante(const bvp_t* bvp, path_state* st, double h, double sqrth, double d, double chi,
+                            uint64_t seed, uint32_t stream, uint32_t path, uint32_t step) {
+    mat2 sig = bvp->sigma(st->X);
+    double n0, n1;
+    spac_normal_pair(seed, stream, path, step, &n0, &n1);
+    vec2 dW = {sqrth * n0, sqrth * n1};
+    vec2 b = bvp->b(st->X);
+    vec2 mu = bvp->mu(st->X);
+    double smx = sig.a00 * mu.x + sig.a01 * mu.y;
+    double smy = sig.a10 * mu.x + sig.a11 * mu.y;
+    double sdx = sig.a00 * dW.x + sig.a01 * dW.y;
+    double sdy = sig.a10 * dW.x + sig.a11 * dW.y;
+    double drx = b.x - smx, dry = b.y - smy;
+    vec2 Xp = {st->X.x + (drx * h + sdx), st->X.y + (dry * h + sdy)};
+    if (d > -bvp->rho) {
+        double omega = (2.0 * h) * spac_exponential(seed, stream, path, step);
+        double r0 = st->N.x * sig.a00 + st->N.y * sig.a10;
+        double r1 = st->N.x * sig.a01 + st->N.y * sig.a11;
+        double uc = (r0 * dW.x + r1 * dW.y) + (st->N.x * drx + st->N.y * dry) * h;
+        double nu = 0.5 * (uc + sqrt((r0 * r0 + r1 * r1) * omega + uc * uc));
+        double v = nu + d;
+        double chil = (v > 0.0) ? v : 0.0;
+        Xp.x = Xp.x - chil * st->N.x;
+        Xp.y = Xp.y - chil * st->N.y;
+        chi = chi + chil;
+    }
+    vec2 Pp, Np;
+    double dp = bvp->frontera_dominio.distancia(bvp->frontera_dominio.params, Xp, &Pp, &Np);
+    if (dp > 0.0 && side_reflects(bvp->reflecting, Np)) {
+        Xp = Pp;
+        chi = chi + dp;
+    }
+    double Y = st->Y;
+    vec2 W = {st->X.x - d * st->N.x, st->X.y - d * st->N.y}; /* foot of X on the side */
+    st->Z = st->Z + (bvp->f(st->X) * Y * h + bvp->psi(W, st->N) * Y * chi);
+    if (g_cv.mode != SPAC_CV_NONE) st->xi = st->xi + cv_increment(bvp, st->X, sig, Y, dW);
+    st->Y = Y + (bvp->c(st->X) * Y * h + Y * (mu.x * dW.x + mu.y * dW.y) + bvp->varphi(W, st->N) * Y * chi);
+    st->X = Xp;
+    st->t = st->t + h;
+}
+
 /* ----------------------------------------------------- Simulacion_OMP --
  * Solvers/FeynmanKacSolver.cpp:137-197: initial Dentro, then do { step;
  * Dentro } while not stopped — at least one step even when X0 starts in the
@@ -368,16 +485,24 @@ static void simulate_one(const bvp_t* bvp, vec2 X0, double h, double sqrth, doub
     path_state st;
     st.X = X0; st.Y = 1.0; st.Z = 0.0; st.xi = 0.0; st.t = 0.0;
     vec2 Npro, N;
-    double dist;
-    int estado = dentro(bvp, st.X, sqrth, gobet, &Npro, &N, &dist);
+    double dist, chi = 0.0;
+    int refl;
+    int estado = dentro(bvp, st.X, sqrth, gobet, &Npro, &N, &dist, &refl);
+    if (estado == SPAC_OUT_REFLECTED) { st.X = Npro; chi = dist; dist = 0.0; } /* :93-96 */
     st.N = N;
     int steps = 0;
     do {
-        paso_absorbente(bvp, &st, h, sqrth, seed, stream, path, (uint32_t)steps);
+        if (refl) paso_reflejante(bvp, &st, h, sqrth, dist, chi, seed, stream, path, (uint32_t)steps);
+        else paso_absorbente(bvp, &st, h, sqrth, seed, stream, path, (uint32_t)steps);
+        chi = 0.0;
         steps++;
-        estado = dentro(bvp, st.X, sqrth, gobet, &Npro, &N, &dist);
+        estado = dentro(bvp, st.X, sqrth, gobet, &Npro, &N, &dist, &refl);
+        if (estado == SPAC_OUT_REFLECTED) { st.X = Npro; chi = dist; dist = 0.0; }
         st.N = N;
-        if (estado == SPAC_OUT_INSIDE && steps >= max_steps) { estado = SPAC_OUT_MAX_STEPS; break; }
+        if (estado != SPAC_OUT_DOMAIN && estado != SPAC_OUT_SUBDOMAIN && steps >= max_steps) {
+            estado = SPAC_OUT_MAX_STEPS;
+            break;
+        }
     } while (estado != SPAC_OUT_DOMAIN && estado != SPAC_OUT_SUBDOMAIN);
     if (xt) *xt = Npro.x;
     if (yt) *yt = Npro.y;
@@ -558,8 +683,13 @@ void oracle_normals(uint64_t seed, uint32_t stream, uint32_t path, int32_t nstep
     for (int32_t s = 0; s < nsteps; ++s) spac_normal_pair(seed, stream, path, (uint32_t)s, &out[2 * s], &out[2 * s + 1]);
 }
 
+void oracle_exponentials(uint64_t seed, uint32_t stream, uint32_t path, int32_t nsteps, double* out) {
+    for (int32_t s = 0; s < nsteps; ++s) out[s] = spac_exponential(seed, stream, path, (uint32_t)s);
+}
+
 /* Coefficient probes for the plugin pins: which = 0 u, 1 f, 2 F.x, 3 F.y,
- * 4 sigma00, 5 b.x, 6 b.y, 7 c, 8 dudx, 9 dudy, 10 d2udx2 + d2udy2 */
+ * 4 sigma00, 5 b.x, 6 b.y, 7 c, 8 dudx, 9 dudy, 10 d2udx2 + d2udy2,
+ * 11 psi(X, (-1, 0)), 12 psi(X, (0, 1)), 13 varphi */
 int oracle_coef_eval(int coef_kind, int which, const double* x, const double* y, double* out, int64_t n) {
     spac_problem_desc d;
     memset(&d, 0, sizeof d);
@@ -576,6 +706,9 @@ int oracle_coef_eval(int coef_kind, int which, const double* x, const double* y,
         case 4: out[i] = bvp.sigma(X).a00; break;
         case 5: out[i] = bvp.b(X).x; break;
         case 6: out[i] = bvp.b(X).y; break;
+        case 11: { vec2 N = {-1.0, 0.0}; out[i] = bvp.psi(X, N); break; }
+        case 12: { vec2 N = {0.0, 1.0}; out[i] = bvp.psi(X, N); break; }
+        case 13: { vec2 N = {1.0, 0.0}; out[i] = bvp.varphi(X, N); break; }
         case 7: out[i] = bvp.c(X); break;
         case 8: out[i] = franke_dudx(X); break;
         case 9: out[i] = franke_dudy(X); break;

@@ -27,6 +27,11 @@ SPAC_E_OUT_OF_MEMORY = -9
 
 SPAC_COEF_FRANKE_POISSON = 1
 SPAC_COEF_VARCOEF_FRANKE = 2
+SPAC_COEF_FRANKE_ROBIN = 3
+SPAC_REFLECT_XMIN = 1
+SPAC_REFLECT_YMIN = 2
+SPAC_REFLECT_XMAX = 4
+SPAC_REFLECT_YMAX = 8
 SPAC_BOUNDARY_NONE = 0
 SPAC_BOUNDARY_RECTANGLE = 1
 SPAC_BOUNDARY_CIRCLE = 2
@@ -65,7 +70,8 @@ class ProblemDesc(C.Structure):
         ("subdomain_params", C.c_double * 4),
         ("gobet_constant", C.c_double),
         ("max_steps", C.c_int32),
-        ("reserved", C.c_int32),
+        ("reflecting", C.c_int32),
+        ("reflect_rho", C.c_double),
     ]
 
 

@@ -131,6 +131,11 @@ int resolve_coefficients(const BVP& bvp, int* kind);
 /* SPAC_OK and the device boundary kind, or SPAC_E_UNREGISTERED_PLUGIN.  A
  * frontier with no distance function (the default) resolves to NONE. */
 int resolve_boundary(const Frontera& fr, int* kind);
+/* The SPAC_REFLECT_* mask of a rectangle frontier: a side reflects where
+ * Evalua_Absorbente is false.  The predicate is sampled at three interior
+ * points of each side; SPAC_E_UNREGISTERED_PLUGIN if it changes along a side
+ * (the device has no mixed sides). */
+int resolve_reflecting(const Frontera& fr, int* mask);
 
 } // namespace spac
 

@@ -59,4 +59,7 @@ Vec2 VarCoef_F(Vec2 X) {
     return -(VarCoef_sigma(X).transpose() * g);
 }
 
+double Robin_psi(Vec2 X, Vec2 N) { return Franke_dudx(X) * N(0) + Franke_dudy(X) * N(1) + 0.5 * Franke_u(X); }
+double Robin_varphi(Vec2, Vec2) { return -0.5; }
+
 } // namespace spac

@@ -38,6 +38,10 @@ Vec2 VarCoef_b(Vec2 X);
 double VarCoef_c(Vec2 X);
 double VarCoef_f(Vec2 X);
 Vec2 VarCoef_F(Vec2 X);
+/* Robin data of reflecting sides for the Franke u: du/dN - varphi u = psi with
+ * varphi = -0.5 (the psi/varphi slots of BVP/BVP.hpp:17-28) */
+double Robin_psi(Vec2 X, Vec2 N);
+double Robin_varphi(Vec2 X, Vec2 N);
 
 } // namespace spac
 

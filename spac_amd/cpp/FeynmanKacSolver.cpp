@@ -16,7 +16,7 @@ FeynmanKacSolver::~FeynmanKacSolver() {
     if (prob_) spac_fk_problem_destroy(prob_);
 }
 
-int FeynmanKacSolver::ensure_problem(const BVP& P, bool with_subdomain) {
+int FeynmanKacSolver::ensure_problem(const BVP& P, bool with_subdomain, double rho) {
     spac_problem_desc d;
     std::memset(&d, 0, sizeof d);
     int kind = 0;
@@ -29,6 +29,11 @@ int FeynmanKacSolver::ensure_problem(const BVP& P, bool with_subdomain) {
         return set_error(SPAC_E_UNREGISTERED_PLUGIN, "domain frontier must be the registered Rectangulo");
     d.domain_kind = dk;
     for (int i = 0; i < 4; ++i) d.domain_params[i] = P.frontera_dominio.parametros[i];
+    int mask = 0;
+    if (resolve_reflecting(P.frontera_dominio, &mask) != SPAC_OK)
+        return set_error(SPAC_E_UNREGISTERED_PLUGIN, "Absorbente must be constant along each side of the rectangle");
+    d.reflecting = mask;
+    d.reflect_rho = mask ? rho : 0.0; /* rho only matters with reflecting sides */
     d.subdomain_kind = SPAC_BOUNDARY_NONE;
     if (with_subdomain) {
         int sk = 0;
@@ -103,11 +108,11 @@ void FeynmanKacSolver::copy_stats(const spac_point_stats& s) {
     path_steps = s.path_steps;
 }
 
-int FeynmanKacSolver::Solve_OMP_Analytic(Vec2 X0, unsigned int n, double dt, double, const BVP& P) {
+int FeynmanKacSolver::Solve_OMP_Analytic(Vec2 X0, unsigned int n, double dt, double rho, const BVP& P) {
     N = n;
     h = dt;
     sqrth = std::sqrt(dt);
-    int rc = ensure_problem(P, true);
+    int rc = ensure_problem(P, true, rho);
     if (rc) return rc;
     double x = X0(0), y = X0(1);
     uint32_t stream = calls_++;
@@ -173,13 +178,13 @@ static int build_table(const std::vector<const Interfaz*>& ifs, OneInterface& o)
 }
 
 int FeynmanKacSolver::Solve_Nodes(std::vector<Nudo*>& nudos, const std::vector<const Interfaz*>& ifs, unsigned int n,
-                                  double dt, const BVP& P) {
+                                  double dt, const BVP& P, double rho) {
     if (nudos.size() != ifs.size() || nudos.empty())
         return set_error(SPAC_E_INVALID_ARG, "one interface per node is needed");
     N = n;
     h = dt;
     sqrth = std::sqrt(dt);
-    int rc = ensure_problem(P, false);
+    int rc = ensure_problem(P, false, rho);
     if (rc) return rc;
     /* distinct interfaces -> table rows */
     std::vector<const Interfaz*> uniq;
@@ -228,7 +233,7 @@ int FeynmanKacSolver::Solve_Nodes(std::vector<Nudo*>& nudos, const std::vector<c
     return rc;
 }
 
-int FeynmanKacSolver::Solve_OMP_Analytic(Nudo& nudo, unsigned int n, double dt, double, const BVP& P,
+int FeynmanKacSolver::Solve_OMP_Analytic(Nudo& nudo, unsigned int n, double dt, double rho, const BVP& P,
                                          const Interfaz& interfaz) {
     if (nudo.frontera) { /* Solvers/FeynmanKacSolver.cpp:346-347 */
         nudo.B = P.g.Evalua(nudo.posicion_cartesiana);
@@ -236,7 +241,7 @@ int FeynmanKacSolver::Solve_OMP_Analytic(Nudo& nudo, unsigned int n, double dt, 
     }
     std::vector<Nudo*> ns{&nudo};
     std::vector<const Interfaz*> is{&interfaz};
-    return Solve_Nodes(ns, is, n, dt, P);
+    return Solve_Nodes(ns, is, n, dt, P, rho);
 }
 
 } // namespace spac

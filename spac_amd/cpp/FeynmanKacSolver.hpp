@@ -13,8 +13,10 @@
  *   - xi/xixi are the true moments and RNGC counts normals drawn;
  *   - a default (unset) subdomain frontier means "no subdomain", not a
  *     distance-0 boundary that stops every path after one step;
- *   - rho is accepted and ignored: the reflecting step is never reached with
- *     absorbing (Stopping) frontiers, the only kind the shipped drivers use.
+ *   - a domain Rectangulo whose Absorbente predicate is false on a side makes
+ *     that side reflecting (the Lepingle step, Solvers/FeynmanKacSolver.cpp:19-67,
+ *     restated without its defects); rho is the band of that step and is
+ *     ignored when every side absorbs, the setting of the shipped drivers.
  */
 #ifndef SPAC_FEYNMANKACSOLVER_HPP
 #define SPAC_FEYNMANKACSOLVER_HPP
@@ -67,7 +69,8 @@ public:
 
     /* Batched form: every (node, interface) pair in one GPU launch. */
     int Solve_Nodes(std::vector<Nudo*>& nudos, const std::vector<const Interfaz*>& interfaces,
-                    unsigned int numero_trayectorias, double discretizacion_temporal, const BVP& Problema);
+                    unsigned int numero_trayectorias, double discretizacion_temporal, const BVP& Problema,
+                    double rho = 0.0);
 
 private:
     spac_problem* prob_ = nullptr;
@@ -77,7 +80,7 @@ private:
     spac_cv_config cv_{};           /* applied to every problem this solver creates */
     std::vector<float> cv_table_;
     int apply_cv();
-    int ensure_problem(const BVP& P, bool with_subdomain);
+    int ensure_problem(const BVP& P, bool with_subdomain, double rho);
     void copy_stats(const spac_point_stats& s);
 };
 

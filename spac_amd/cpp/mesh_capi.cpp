@@ -226,7 +226,8 @@ static double franke_xy(double x, double y) { return Franke_u(Vec2(x, y)); }
 
 spac_status spac_mesh_write_positions(const spac_mesh* hm, int32_t coef_kind, const char* path) {
     if (!hm || !path) return set_error(SPAC_E_INVALID_ARG, "null argument");
-    if (coef_kind != SPAC_COEF_FRANKE_POISSON && coef_kind != SPAC_COEF_VARCOEF_FRANKE)
+    if (coef_kind != SPAC_COEF_FRANKE_POISSON && coef_kind != SPAC_COEF_VARCOEF_FRANKE &&
+        coef_kind != SPAC_COEF_FRANKE_ROBIN)
         return set_error(SPAC_E_UNREGISTERED_PLUGIN, "coefficient set %d unknown", coef_kind);
     if (!reinterpret_cast<const MeshHandle*>(hm)->malla.Escribe_Posiciones(path, franke_xy))
         return set_error(SPAC_E_INVALID_ARG, "cannot write %s", path);

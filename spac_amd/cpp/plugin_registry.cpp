@@ -63,11 +63,32 @@ int resolve_coefficients(const BVP& bvp, int* kind) {
 
 int resolve_boundary(const Frontera& fr, int* kind) {
     if (fr.Distancia == Distancia_Defecto) { *kind = SPAC_BOUNDARY_NONE; return SPAC_OK; }
-    if (fr.Absorbente != Tipo_Defecto && fr.Absorbente != Stopping) return SPAC_E_UNREGISTERED_PLUGIN;
     std::lock_guard<std::mutex> lk(reg_mutex());
     for (const BoundaryEntry& e : boundaries())
         if (fr.Distancia == e.distancia) { *kind = e.kind; return SPAC_OK; }
     return SPAC_E_UNREGISTERED_PLUGIN;
+}
+
+int resolve_reflecting(const Frontera& fr, int* mask) {
+    *mask = 0;
+    if (fr.Absorbente == Tipo_Defecto || fr.Absorbente == Stopping) return SPAC_OK;
+    if (fr.Distancia != Rectangulo || fr.parametros.size() < 4) return SPAC_E_UNREGISTERED_PLUGIN;
+    const double x0 = fr.parametros[0], y0 = fr.parametros[1], x1 = fr.parametros[2], y1 = fr.parametros[3];
+    const int bits[4] = {SPAC_REFLECT_XMIN, SPAC_REFLECT_YMIN, SPAC_REFLECT_XMAX, SPAC_REFLECT_YMAX};
+    for (int side = 0; side < 4; ++side) {
+        int absorbing = 0;
+        for (int k = 1; k <= 3; ++k) {
+            const double t = 0.25 * k;
+            Vec2 P = side == 0 ? Vec2(x0, y0 + t * (y1 - y0))
+                   : side == 1 ? Vec2(x0 + t * (x1 - x0), y0)
+                   : side == 2 ? Vec2(x1, y0 + t * (y1 - y0))
+                               : Vec2(x0 + t * (x1 - x0), y1);
+            absorbing += fr.Evalua_Absorbente(P) ? 1 : 0;
+        }
+        if (absorbing != 0 && absorbing != 3) return SPAC_E_UNREGISTERED_PLUGIN;
+        if (absorbing == 0) *mask |= bits[side];
+    }
+    return SPAC_OK;
 }
 
 double Inverse_Multiquadric_RBF(double theta, double theta_j, double c2) {
@@ -130,6 +151,8 @@ SPAC_REGISTER_COEFFICIENT_SET(SPAC_COEF_FRANKE_POISSON, Franke_u, Franke_g, Fran
                               Vector_Defecto, Franke_sigma, EscalarN_Defecto, EscalarN_Defecto);
 SPAC_REGISTER_COEFFICIENT_SET(SPAC_COEF_VARCOEF_FRANKE, Franke_u, Franke_g, VarCoef_c, VarCoef_f, VarCoef_b,
                               VarCoef_F, Vector_Defecto, VarCoef_sigma, EscalarN_Defecto, EscalarN_Defecto);
+SPAC_REGISTER_COEFFICIENT_SET(SPAC_COEF_FRANKE_ROBIN, Franke_u, Franke_g, Franke_c, Franke_f, Franke_b, Franke_F,
+                              Vector_Defecto, Franke_sigma, Robin_psi, Robin_varphi);
 SPAC_REGISTER_BOUNDARY(SPAC_BOUNDARY_RECTANGLE, Rectangulo);
 SPAC_REGISTER_BOUNDARY(SPAC_BOUNDARY_CIRCLE, Circulo);
 

@@ -73,6 +73,7 @@ __device__ __forceinline__ double franke_uyy(const FrankeTerms& t) {
 struct CoefFranke {
     static constexpr bool kHasDrift = false;
     static constexpr bool kHasKilling = false;
+    static constexpr bool kHasRobin = false; /* psi = varphi = 0 */
     static constexpr bool kConstSigma = true;
     static constexpr double kSigma = 1.41421356237;
     __device__ __forceinline__ static double sigma(double, double) { return kSigma; }
@@ -89,6 +90,22 @@ struct CoefFranke {
     }
     __device__ __forceinline__ static double g(double x, double y) { return franke_u(franke_terms(x, y)); }
     __device__ __forceinline__ static double u(double x, double y) { return franke_u(franke_terms(x, y)); }
+    __device__ __forceinline__ static void robin(double, double, double, double, double* psi, double* phi) {
+        *psi = 0.0;
+        *phi = 0.0;
+    }
+};
+
+/* Franke Poisson with Robin data on reflecting sides: varphi = -0.5,
+ * psi(W, N) = grad u(W).N + 0.5 u(W), so du/dN - varphi u = psi for the Franke u. */
+struct CoefFrankeRobin : CoefFranke {
+    static constexpr bool kHasRobin = true;
+    __device__ __forceinline__ static void robin(double x, double y, double nx, double ny, double* psi,
+                                                 double* phi) {
+        FrankeTerms t = franke_terms(x, y);
+        *psi = franke_ux(t) * nx + franke_uy(t) * ny + 0.5 * franke_u(t);
+        *phi = -0.5;
+    }
 };
 
 /* Variable-coefficient operator with the Franke manufactured solution:
@@ -97,6 +114,7 @@ struct CoefFranke {
 struct CoefVarFranke {
     static constexpr bool kHasDrift = true;
     static constexpr bool kHasKilling = true;
+    static constexpr bool kHasRobin = false;
     static constexpr bool kConstSigma = false;
     __device__ __forceinline__ static double sigma(double x, double y) {
         return 1.0 + 0.5 * spac_sinpi(x) * spac_cospi(y);
@@ -122,6 +140,10 @@ struct CoefVarFranke {
     }
     __device__ __forceinline__ static double g(double x, double y) { return franke_u(franke_terms(x, y)); }
     __device__ __forceinline__ static double u(double x, double y) { return franke_u(franke_terms(x, y)); }
+    __device__ __forceinline__ static void robin(double, double, double, double, double* psi, double* phi) {
+        *psi = 0.0;
+        *phi = 0.0;
+    }
 };
 
 /* --------------------------------------------------------- boundaries -- */
@@ -164,6 +186,14 @@ __device__ __forceinline__ BoundaryHit rectangle_distance(const double* p, doubl
     }
     r.ey = sy * h1 + c1; r.nx = 0.0; r.ny = sy; r.d = d1 - h1;
     return r;
+}
+
+/* Frontera::Evalua_Absorbente == false (BVP/Frontera.hpp:23,34) on a rectangle
+ * with the SPAC_REFLECT_* mask: the side or corner is read off the outward
+ * normal; a corner reflects only when both of its sides do. */
+__device__ __forceinline__ bool side_reflects(int32_t mask, double nx, double ny) {
+    int32_t bits = (nx < 0.0 ? 1 : 0) | (ny < 0.0 ? 2 : 0) | (nx > 0.0 ? 4 : 0) | (ny > 0.0 ? 8 : 0);
+    return bits != 0 && (bits & mask) == bits;
 }
 
 /* Circulo (Dominios/Circulo.hpp:7-22): params R, cx, cy; radius clamp 5e-4. */

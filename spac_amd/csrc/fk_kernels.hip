@@ -65,6 +65,8 @@ struct PathArgs {
                                  alive outside the boundary, gobet_constant >= 0) */
     int32_t max_steps;
     int32_t n_items;
+    int32_t reflecting;     /* SPAC_REFLECT_* sides (REFL kernels) */
+    double rho;             /* Lepingle band */
     /* solves */
     const double* x0;
     const double* y0;
@@ -116,7 +118,32 @@ __device__ __forceinline__ double exit_u(const PathArgs& a, double x, double y) 
     return Coef::u(x, y);
 }
 
-template <class Coef, int SUB, int CV>
+/* The frontier Dentro tests (Solvers/FeynmanKacSolver.cpp:72-80): the domain
+ * rectangle or, when nearer, the solve's circle; *dom says which. */
+template <int SUB>
+__device__ __forceinline__ BoundaryHit nearest_boundary(const PathArgs& a, double R, double cx, double cy, double X,
+                                                        double Yc, bool* dom) {
+    BoundaryHit hit = rectangle_distance(a.dom, X, Yc);
+    bool use_dom = true;
+    if (SUB == SUB_CIRCLE) {
+        BoundaryHit sub = circle_distance(R, cx, cy, X, Yc);
+        use_dom = hit.d >= sub.d;
+        if (!use_dom) hit = sub;
+    } else {
+        if (!(hit.d >= -__builtin_inf())) {
+            use_dom = false;
+            hit.d = -__builtin_inf(); hit.ex = X; hit.ey = Yc; hit.nx = 0.0; hit.ny = 0.0;
+        }
+    }
+    *dom = use_dom;
+    return hit;
+}
+
+/* REFL = true adds reflecting rectangle sides: the Lepingle step
+ * (Paso_Reflejante, Solvers/FeynmanKacSolver.cpp:19-67, restated without its
+ * defects) for paths whose nearest frontier is a reflecting side, and the
+ * projection of points found beyond one (reflejada, :88-96). */
+template <class Coef, int SUB, int CV, bool REFL>
 __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t seed = a.seed_dev ? *a.seed_dev : a.seed;
@@ -128,6 +155,10 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     double X = 0.0, Yc = 0.0, Yw = 0.0, Z = 0.0, xi = 0.0, t = 0.0;
     double h = 0.0, sqrth = 0.0, R = 0.0, cx = 0.0, cy = 0.0;
     double sig = 0.0; /* sigma at the current X */
+    /* reflecting sides: the last Dentro's distance and normal, whether the next
+     * step reflects, and local time carried from a projection */
+    double rd = 0.0, rnx = 0.0, rny = 0.0, chi_p = 0.0;
+    bool rrefl = false;
     uint32_t stream = 0;
     int64_t rec = 0;
     /* wave-uniform pool */
@@ -174,6 +205,14 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                 }
                 X = a.x0[solve];
                 Yc = a.y0[solve];
+                if constexpr (REFL) {
+                    /* initial Dentro (:166): a start beyond a reflecting side is projected */
+                    bool dom0;
+                    BoundaryHit h0 = nearest_boundary<SUB>(a, R, cx, cy, X, Yc, &dom0);
+                    rrefl = dom0 && side_reflects(a.reflecting, h0.nx, h0.ny);
+                    rnx = h0.nx; rny = h0.ny; rd = h0.d; chi_p = 0.0;
+                    if (rrefl && !(h0.d <= 0.0)) { X = h0.ex; Yc = h0.ey; chi_p = h0.d; rd = 0.0; }
+                }
                 sig = Coef::sigma(X, Yc);
                 Yw = 1.0; Z = 0.0; xi = 0.0; t = 0.0;
                 nstep = 0;
@@ -184,7 +223,66 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             idle = __ballot(!active);
         }
         if (__ballot(active) == 0ull) break;
-        if (active) {
+        bool reflect_step = false;
+        if constexpr (REFL) reflect_step = rrefl;
+        if (active && reflect_step) {
+            /* Paso_Reflejante: X' = X + b h + sigma dW; within rho of the side the
+             * step's largest outward bridge excursion nu (exponential draw, mean
+             * 2h) gives the local time chi = max(0, nu + d) and X' -= chi N; a point
+             * still beyond a reflecting side is projected (chi += overshoot); Robin
+             * data at the foot W = X - d N of X on the side. */
+            StepCoef cf = Coef::step_s(X, Yc, sig);
+            double n0, n1;
+            spac_normal_pair(seed, stream, path, (uint32_t)nstep, &n0, &n1);
+            double dWx = sqrth * n0, dWy = sqrth * n1;
+            double xp, yp;
+            if (Coef::kHasDrift) {
+                xp = X + (cf.bx * h + cf.s * dWx);
+                yp = Yc + (cf.by * h + cf.s * dWy);
+            } else {
+                xp = X + cf.s * dWx;
+                yp = Yc + cf.s * dWy;
+            }
+            double chi = chi_p;
+            if (rd > -a.rho) {
+                double omega = (2.0 * h) * spac_exponential(seed, stream, path, (uint32_t)nstep);
+                double r0 = rnx * cf.s, r1 = rny * cf.s;
+                double uc = (r0 * dWx + r1 * dWy) + (rnx * cf.bx + rny * cf.by) * h;
+                double nu = 0.5 * (uc + __builtin_sqrt((r0 * r0 + r1 * r1) * omega + uc * uc));
+                double v = nu + rd;
+                double chil = (v > 0.0) ? v : 0.0;
+                xp = xp - chil * rnx;
+                yp = yp - chil * rny;
+                chi = chi + chil;
+            }
+            BoundaryHit ph = rectangle_distance(a.dom, xp, yp);
+            if (ph.d > 0.0 && side_reflects(a.reflecting, ph.nx, ph.ny)) {
+                xp = ph.ex;
+                yp = ph.ey;
+                chi = chi + ph.d;
+            }
+            double psi = 0.0, phi = 0.0;
+            if (Coef::kHasRobin) Coef::robin(X - rd * rnx, Yc - rd * rny, rnx, rny, &psi, &phi);
+            if (Coef::kHasRobin) Z = Z + ((cf.f * Yw) * h + (psi * Yw) * chi);
+            else Z = Z + (cf.f * Yw) * h;
+            if (CV == CV_ANALYTIC) {
+                xi = xi + Yw * (cf.Fx * dWx + cf.Fy * dWy);
+            } else if (CV == CV_TABLE) {
+                int32_t k00;
+                double ax, ay;
+                spac_cv_cell(&a.cv, X, Yc, &k00, &ax, &ay);
+                double Fx = -(cf.s * spac_cv_eval(&a.cv, k00, ax, ay, 1));
+                double Fy = -(cf.s * spac_cv_eval(&a.cv, k00, ax, ay, 2));
+                xi = xi + Yw * (Fx * dWx + Fy * dWy);
+            }
+            if (Coef::kHasKilling && Coef::kHasRobin) Yw = Yw + ((cf.c * Yw) * h + (phi * Yw) * chi);
+            else if (Coef::kHasKilling) Yw = Yw + (cf.c * Yw) * h;
+            else if (Coef::kHasRobin) Yw = Yw + (phi * Yw) * chi;
+            X = xp;
+            Yc = yp;
+            t = t + h;
+            nstep++;
+        } else if (active) {
             /* Paso_Absorbente (Solvers/FeynmanKacSolver.cpp:7-18) */
             StepCoef cf = Coef::step_s(X, Yc, sig);
             double n0, n1;
@@ -212,19 +310,11 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             }
             t = t + h;
             nstep++;
+        }
+        if (active) {
             /* Dentro (Solvers/FeynmanKacSolver.cpp:68-106) */
-            BoundaryHit hit = rectangle_distance(a.dom, X, Yc);
-            bool use_dom = true;
-            if (SUB == SUB_CIRCLE) {
-                BoundaryHit sub = circle_distance(R, cx, cy, X, Yc);
-                use_dom = hit.d >= sub.d;
-                if (!use_dom) hit = sub;
-            } else {
-                if (!(hit.d >= -__builtin_inf())) {
-                    use_dom = false;
-                    hit.d = -__builtin_inf(); hit.ex = X; hit.ey = Yc; hit.nx = 0.0; hit.ny = 0.0;
-                }
-            }
+            bool use_dom;
+            BoundaryHit hit = nearest_boundary<SUB>(a, R, cx, cy, X, Yc, &use_dom);
             /* sigma at X inside the boundary, at the projection outside (one evaluation) */
             const double px = (hit.d < 0.0) ? X : hit.ex, py = (hit.d < 0.0) ? Yc : hit.ey;
             double s = Coef::sigma(px, py);
@@ -234,7 +324,23 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             double nrm = __builtin_sqrt(r0 * r0 + r1 * r1);
             bool inside = hit.d < a.gobet * nrm * sqrth;
             int8_t out = inside ? (int8_t)SPAC_OUT_INSIDE : (use_dom ? (int8_t)SPAC_OUT_DOMAIN : (int8_t)SPAC_OUT_SUBDOMAIN);
-            if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
+            if constexpr (REFL) {
+                /* a reflecting side keeps the path; beyond it, reflejada: project */
+                bool rside = use_dom && side_reflects(a.reflecting, hit.nx, hit.ny);
+                if (rside) out = (hit.d <= 0.0) ? (int8_t)SPAC_OUT_INSIDE : (int8_t)SPAC_OUT_REFLECTED;
+                if ((out == SPAC_OUT_INSIDE || out == SPAC_OUT_REFLECTED) && nstep >= a.max_steps)
+                    out = SPAC_OUT_MAX_STEPS;
+                rnx = hit.nx; rny = hit.ny; rrefl = rside;
+                rd = hit.d; chi_p = 0.0;
+                if (out == SPAC_OUT_REFLECTED) {
+                    X = hit.ex; Yc = hit.ey; chi_p = hit.d; rd = 0.0;
+                    out = SPAC_OUT_INSIDE;
+                }
+                /* the next step's sigma at the current (possibly projected) X */
+                sig = (px == X && py == Yc) ? s : Coef::sigma(X, Yc);
+            } else {
+                if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
+            }
             if (out != SPAC_OUT_INSIDE) {
                 int64_t g = rec + (int64_t)path;
                 a.xt[g] = hit.ex;
@@ -511,11 +617,18 @@ __global__ void __launch_bounds__(64) fk_reduce_kernel(ReduceArgs a) {
 }
 
 /* ------------------------------------------------------ host side -- */
+template <class Coef, int CV, bool REFL>
+static void launch_fp64(int sub, dim3 grid, dim3 block, hipStream_t s, const PathArgs& a) {
+    if (sub == SUB_CIRCLE) hipLaunchKernelGGL((fk_path_kernel_fp64<Coef, SUB_CIRCLE, CV, REFL>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((fk_path_kernel_fp64<Coef, SUB_NONE, CV, REFL>), grid, block, 0, s, a);
+}
+
 template <int CV>
-static void launch_paths_cv(int coef, int precision, int sub, dim3 grid, dim3 block, hipStream_t s,
+static void launch_paths_cv(int coef, int precision, int sub, bool refl, dim3 grid, dim3 block, hipStream_t s,
                             const PathArgs& a) {
     if (precision == SPAC_FP32) {
-        if (coef == SPAC_COEF_FRANKE_POISSON) {
+        /* FRANKE_ROBIN differs from FRANKE_POISSON only on reflecting sides (FP64 only) */
+        if (coef != SPAC_COEF_VARCOEF_FRANKE) {
             if (sub == SUB_CIRCLE) hipLaunchKernelGGL((fk_path_kernel_fp32<CoefFranke32, SUB_CIRCLE, CV>), grid, block, 0, s, a);
             else hipLaunchKernelGGL((fk_path_kernel_fp32<CoefFranke32, SUB_NONE, CV>), grid, block, 0, s, a);
         } else {
@@ -524,20 +637,21 @@ static void launch_paths_cv(int coef, int precision, int sub, dim3 grid, dim3 bl
         }
         return;
     }
-    if (coef == SPAC_COEF_FRANKE_POISSON) {
-        if (sub == SUB_CIRCLE) hipLaunchKernelGGL((fk_path_kernel_fp64<CoefFranke, SUB_CIRCLE, CV>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((fk_path_kernel_fp64<CoefFranke, SUB_NONE, CV>), grid, block, 0, s, a);
-    } else {
-        if (sub == SUB_CIRCLE) hipLaunchKernelGGL((fk_path_kernel_fp64<CoefVarFranke, SUB_CIRCLE, CV>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((fk_path_kernel_fp64<CoefVarFranke, SUB_NONE, CV>), grid, block, 0, s, a);
+    if (refl) {
+        if (coef == SPAC_COEF_FRANKE_POISSON) launch_fp64<CoefFranke, CV, true>(sub, grid, block, s, a);
+        else if (coef == SPAC_COEF_FRANKE_ROBIN) launch_fp64<CoefFrankeRobin, CV, true>(sub, grid, block, s, a);
+        else launch_fp64<CoefVarFranke, CV, true>(sub, grid, block, s, a);
+        return;
     }
+    if (coef != SPAC_COEF_VARCOEF_FRANKE) launch_fp64<CoefFranke, CV, false>(sub, grid, block, s, a);
+    else launch_fp64<CoefVarFranke, CV, false>(sub, grid, block, s, a);
 }
 
-static void launch_paths(int coef, int precision, int sub, int cv, dim3 grid, dim3 block, hipStream_t s,
+static void launch_paths(int coef, int precision, int sub, int cv, bool refl, dim3 grid, dim3 block, hipStream_t s,
                          const PathArgs& a) {
-    if (cv == CV_NONE) launch_paths_cv<CV_NONE>(coef, precision, sub, grid, block, s, a);
-    else if (cv == CV_TABLE) launch_paths_cv<CV_TABLE>(coef, precision, sub, grid, block, s, a);
-    else launch_paths_cv<CV_ANALYTIC>(coef, precision, sub, grid, block, s, a);
+    if (cv == CV_NONE) launch_paths_cv<CV_NONE>(coef, precision, sub, refl, grid, block, s, a);
+    else if (cv == CV_TABLE) launch_paths_cv<CV_TABLE>(coef, precision, sub, refl, grid, block, s, a);
+    else launch_paths_cv<CV_ANALYTIC>(coef, precision, sub, refl, grid, block, s, a);
 }
 
 spac_status DeviceBuffer::ensure(size_t bytes) {
@@ -632,7 +746,8 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
 }
 
 static spac_status check_problem(const spac_problem_desc* d) {
-    if (d->coef_kind != SPAC_COEF_FRANKE_POISSON && d->coef_kind != SPAC_COEF_VARCOEF_FRANKE)
+    if (d->coef_kind != SPAC_COEF_FRANKE_POISSON && d->coef_kind != SPAC_COEF_VARCOEF_FRANKE &&
+        d->coef_kind != SPAC_COEF_FRANKE_ROBIN)
         return set_error(SPAC_E_UNREGISTERED_PLUGIN, "coefficient set %d has no device implementation", d->coef_kind);
     if (d->domain_kind != SPAC_BOUNDARY_RECTANGLE)
         return set_error(SPAC_E_UNREGISTERED_PLUGIN, "domain boundary kind %d has no device implementation", d->domain_kind);
@@ -645,6 +760,12 @@ static spac_status check_problem(const spac_problem_desc* d) {
         return set_error(SPAC_E_INVALID_ARG, "rectangle domain must have xmax > xmin and ymax > ymin");
     if (d->subdomain_kind == SPAC_BOUNDARY_CIRCLE && !(d->subdomain_params[0] > 0.0))
         return set_error(SPAC_E_INVALID_ARG, "circle subdomain radius must be > 0");
+    if (d->reflecting & ~(SPAC_REFLECT_XMIN | SPAC_REFLECT_YMIN | SPAC_REFLECT_XMAX | SPAC_REFLECT_YMAX))
+        return set_error(SPAC_E_INVALID_ARG, "reflecting mask 0x%x has bits outside SPAC_REFLECT_*", d->reflecting);
+    if (d->reflecting && d->precision != SPAC_FP64)
+        return set_error(SPAC_E_INVALID_ARG, "reflecting sides need SPAC_FP64 stepping");
+    if (d->reflecting && !(d->reflect_rho >= 0.0 && d->reflect_rho < 1e300))
+        return set_error(SPAC_E_INVALID_ARG, "reflect_rho must be finite and >= 0");
     return SPAC_OK;
 }
 
@@ -658,6 +779,8 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
     a.seed = seed;
     a.seed_dev = sdev;
     a.max_steps = desc.max_steps > 0 ? desc.max_steps : 0x7fffffff;
+    a.reflecting = desc.reflecting;
+    a.rho = desc.reflect_rho;
     a.x0 = ss.x0; a.y0 = ss.y0;
     a.h = ss.h; a.h_uniform = ss.h_uniform;
     a.stream = ss.stream;
@@ -695,7 +818,8 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
     int64_t max_blocks = std::max<int64_t>(1, ((int64_t)ss.n_items + (block / 64) - 1) / (block / 64));
     if (blocks > max_blocks) blocks = (int)max_blocks;
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
-    launch_paths(desc.coef_kind, desc.precision, sub, cv_mode, dim3(blocks), dim3(block), stream, a);
+    launch_paths(desc.coef_kind, desc.precision, sub, cv_mode, desc.reflecting != 0, dim3(blocks), dim3(block), stream,
+                 a);
     SPAC_HIP_CHECK(hipGetLastError());
     if (grid_blocks) *grid_blocks = blocks;
     return SPAC_OK;
@@ -757,7 +881,8 @@ spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* sr
         const double* d = cfg->table_domain;
         if (!(d[2] > d[0]) || !(d[3] > d[1]))
             return set_error(SPAC_E_INVALID_ARG, "table domain must have xmax > xmin and ymax > ymin");
-        if (p->desc.coef_kind != SPAC_COEF_FRANKE_POISSON && p->desc.coef_kind != SPAC_COEF_VARCOEF_FRANKE)
+        if (p->desc.coef_kind != SPAC_COEF_FRANKE_POISSON && p->desc.coef_kind != SPAC_COEF_VARCOEF_FRANKE &&
+            p->desc.coef_kind != SPAC_COEF_FRANKE_ROBIN)
             return set_error(SPAC_E_INVALID_ARG, "table mode needs a scalar-diffusion coefficient set");
         size_t bytes = (size_t)cfg->table_size * (size_t)cfg->table_size * 16;
         spac_status rc;

@@ -444,7 +444,7 @@ static spac_status rows_enqueue_bg(Problem* p, const RowsSetup& rs) {
     hipStream_t s = p->stream;
     const int n = rs.n;
     const int nblk = (n + 255) / 256;
-    if (p->desc.coef_kind == SPAC_COEF_FRANKE_POISSON)
+    if (p->desc.coef_kind != SPAC_COEF_VARCOEF_FRANKE) /* Franke g for both Franke sets */
         hipLaunchKernelGGL(fk_rows_b_kernel<CoefFranke>, dim3(nblk), dim3(256), 0, s, n, rs.d_front, rs.ss.x0, rs.ss.y0,
                            (const spac_point_stats*)rs.d_stats, rs.d_B);
     else

@@ -275,6 +275,18 @@ SPAC_HD void spac_normal_pair(uint64_t seed, uint32_t stream, uint32_t path, uin
     *n1 = rad * s;
 }
 
+/* Unit exponential E = -log U, U in (0, 1], for the reflecting step's bridge
+ * maximum: its own Philox block per path-step (tag SPAC_TAG_REFLECT), so a path
+ * that never reflects draws the same normals as under absorbing sides. */
+#define SPAC_TAG_REFLECT 1u
+
+SPAC_HD double spac_exponential(uint64_t seed, uint32_t stream, uint32_t path, uint32_t step) {
+    spac_u32x4 r = spac_philox4x32_10(step, path, (uint32_t)(seed >> 32), SPAC_TAG_REFLECT,
+                                      (uint32_t)seed, stream);
+    double u = 1.0 - spac_u01_53(r.v[0], r.v[1]); /* (0, 1] */
+    return -spac_log(u);
+}
+
 /* -------------------------------------------- compensated accumulation -- */
 /* Knuth TwoSum: s + e == a + b exactly. */
 typedef struct { double s, c; } spac_ksum;

@@ -15,7 +15,10 @@ from . import abi
 
 def problem_desc(coef=abi.SPAC_COEF_FRANKE_POISSON, domain=(-1.0, -1.0, 1.0, 1.0),
                  subdomain_kind=abi.SPAC_BOUNDARY_NONE, subdomain=(0.0, 0.0, 0.0),
-                 gobet=abi.GOBET_CONSTANT, max_steps=0, precision=abi.SPAC_FP64) -> abi.ProblemDesc:
+                 gobet=abi.GOBET_CONSTANT, max_steps=0, precision=abi.SPAC_FP64, reflecting=0,
+                 rho=0.0) -> abi.ProblemDesc:
+    """reflecting: SPAC_REFLECT_* mask of rectangle sides that reflect (0: all
+    absorbing, the reference's shipped setting); rho: the Lepingle band width."""
     d = abi.ProblemDesc()
     d.coef_kind = coef
     d.domain_kind = abi.SPAC_BOUNDARY_RECTANGLE
@@ -27,6 +30,8 @@ def problem_desc(coef=abi.SPAC_COEF_FRANKE_POISSON, domain=(-1.0, -1.0, 1.0, 1.0
     d.gobet_constant = gobet
     d.max_steps = max_steps
     d.precision = precision
+    d.reflecting = reflecting
+    d.reflect_rho = rho
     return d
 
 

@@ -8,6 +8,8 @@
  *                        accumulated with SistemaLineal::Actualiza_GB
  *   surface_driver cv    one point solve with the plugin control variate, none,
  *                        and a tabulated u_hat (TablaDeValores + Interpolador_Tabla)
+ *   surface_driver reflect  one point solve of the Robin problem with the left
+ *                        side of the unit square reflecting (Absorbente false there)
  * Prints one JSON object on stdout.
  */
 #include <cmath>
@@ -27,6 +29,9 @@
 using namespace spac;
 
 static double unregistered_f(Vec2 x) { return x(0); }
+/* Absorbente predicates: the left side (x = 0) reflects; a side that changes type halfway */
+static bool left_reflects(Vec2 x) { return !(x(0) <= 0.0); }
+static bool mixed_left(Vec2 x) { return !(x(0) <= 0.0 && x(1) < 0.5); }
 
 /* Franke u and its gradient on a T x T grid over [x0, x1]^2 */
 static TablaDeValores franke_table(int T, double x0, double x1, bool with_derivatives) {
@@ -99,14 +104,39 @@ int main(int argc, char** argv) {
                                              std::fabs(itfd.Deriva_y(X) - Franke_dudy(X))));
             gmax = std::fmax(gmax, std::fmax(std::fabs(Franke_dudx(X)), std::fabs(Franke_dudy(X))));
         }
+        /* reflecting sides from the Absorbente predicate */
+        Frontera fl, fm;
+        fl.Inicializa({0.0, 0.0, 1.0, 1.0}, Rectangulo, left_reflects, Stopping);
+        fm.Inicializa({0.0, 0.0, 1.0, 1.0}, Rectangulo, mixed_left, Stopping);
+        int refl_mask = -1, mixed_mask = -1, stop_mask = -1;
+        int refl_rc = resolve_reflecting(fl, &refl_mask);
+        int mixed_rc = resolve_reflecting(fm, &mixed_mask);
+        int stop_rc = resolve_reflecting(bvp.frontera_dominio, &stop_mask);
         Interpolador base;
         const bool base_nan = std::isnan(base.Interpola(Vec2(0.1, 0.1))) && std::isnan(base.Deriva_x(Vec2(0.1, 0.1)));
         std::printf("{\"mesh_rc\": %d, \"nodes\": %d, \"nnz\": %d, \"coef_rc\": %d, \"coef_kind\": %d, "
                     "\"dom_rc\": %d, \"dom_kind\": %d, \"sub_rc\": %d, \"sub_kind\": %d, \"bad_rc\": %d, "
                     "\"update_g_rc\": %d, \"update_g_sum\": %.17g, \"interp_err_u\": %.3e, \"interp_err_grad\": %.3e, "
-                    "\"interp_err_grad_fd\": %.3e, \"grad_max\": %.3e, \"base_nan\": %d}\n",
+                    "\"interp_err_grad_fd\": %.3e, \"grad_max\": %.3e, \"base_nan\": %d, "
+                    "\"refl\": [%d, %d], \"refl_mixed_rc\": %d, \"refl_stopping\": [%d, %d]}\n",
                     mrc, mesh.numero_total_nudos, mesh.sistema.nnz, rc_coef, kind, rc_dom, dk, rc_sub, sk, rc_bad, urc,
-                    sumG, eu, eg, egfd, gmax, base_nan ? 1 : 0);
+                    sumG, eu, eg, egfd, gmax, base_nan ? 1 : 0, refl_rc, refl_mask, mixed_rc, stop_rc, stop_mask);
+        return 0;
+    }
+
+    if (mode == "reflect") {
+        BVP pb = bvp;
+        pb.psi.Inicializa(Robin_psi);
+        pb.varphi.Inicializa(Robin_varphi);
+        pb.frontera_dominio.Inicializa({0.0, 0.0, 1.0, 1.0}, Rectangulo, left_reflects, Stopping);
+        pb.frontera_subdominio = Frontera();
+        int kr = -1;
+        int krc = resolve_coefficients(pb, &kr);
+        FeynmanKacSolver s;
+        int rc = s.Solve_OMP_Analytic(Vec2(0.05, 0.5), 4000, 1e-3, 0.08, pb);
+        std::printf("{\"coef_rc\": %d, \"coef_kind\": %d, \"rc\": %d, \"phi\": %.17g, \"phi_VR\": %.17g, "
+                    "\"steps\": %lld, \"u\": %.17g}\n",
+                    krc, kr, rc, s.phi, s.phi_VR, (long long)s.path_steps, Franke_u(Vec2(0.05, 0.5)));
         return 0;
     }
 

@@ -39,6 +39,8 @@ def load() -> C.CDLL:
     lib.oracle_philox.restype = None
     lib.oracle_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, dp]
     lib.oracle_normals.restype = None
+    lib.oracle_exponentials.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, dp]
+    lib.oracle_exponentials.restype = None
     lib.oracle_coef_eval.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int64]
     lib.oracle_coef_eval.restype = C.c_int
     lib.oracle_boundary_eval.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int64]
@@ -73,6 +75,12 @@ def philox(rounds: int, ctr, key) -> list[int]:
 def normals(seed: int, stream: int, path: int, nsteps: int) -> np.ndarray:
     out = np.empty(2 * nsteps, dtype=np.float64)
     load().oracle_normals(seed, stream, path, nsteps, _p(out))
+    return out
+
+
+def exponentials(seed: int, stream: int, path: int, nsteps: int) -> np.ndarray:
+    out = np.empty(nsteps, dtype=np.float64)
+    load().oracle_exponentials(seed, stream, path, nsteps, _p(out))
     return out
 
 

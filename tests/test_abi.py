@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(header):
 
 def test_abi_version_and_error_channel():
     lib = abi.load()
-    assert lib.spac_abi_version() == 2
+    assert lib.spac_abi_version() == 3
     assert lib.spac_fk_problem_create(None, None) == abi.SPAC_E_INVALID_ARG
     assert "null" in abi.last_error(lib)
 

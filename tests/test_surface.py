@@ -41,6 +41,11 @@ def test_surface_cpu():
     assert r["interp_err_grad"] < 3e-3 * r["grad_max"]
     assert r["interp_err_grad_fd"] < 5e-3 * r["grad_max"]
     assert r["base_nan"] == 1
+    # Frontera::Absorbente false on the left side -> SPAC_REFLECT_XMIN; a side whose type
+    # changes along it has no device form; Stopping everywhere -> no reflecting side
+    assert r["refl"] == [0, abi.SPAC_REFLECT_XMIN]
+    assert r["refl_mixed_rc"] == abi.SPAC_E_UNREGISTERED_PLUGIN
+    assert r["refl_stopping"] == [0, 0]
 
 
 @pytest.mark.gpu
@@ -92,3 +97,18 @@ def test_surface_gpu_tabulated_control_variate(gpu):
     assert abs(var_table / var_plugin - 1.0) < 0.05
     assert abs(table["phi_VR"] - plugin["phi_VR"]) < 1e-3
     assert abs(table["phi_VR"] - r["u"]) < 4 * np.sqrt(var_table / 4000) + 5e-3
+
+
+@pytest.mark.gpu
+def test_surface_gpu_reflecting_side(gpu):
+    """Solve_OMP_Analytic with the Robin plugins and Absorbente false on the left
+    side: the surface resolves FRANKE_ROBIN with SPAC_REFLECT_XMIN and rho, and
+    equals the restatement bit for bit."""
+    print("surface driver reflect", flush=True)
+    r = run_driver("reflect")
+    assert r["coef_rc"] == 0 and r["coef_kind"] == abi.SPAC_COEF_FRANKE_ROBIN and r["rc"] == 0
+    desc = problem_desc(coef=abi.SPAC_COEF_FRANKE_ROBIN, domain=(0.0, 0.0, 1.0, 1.0),
+                        reflecting=abi.SPAC_REFLECT_XMIN, rho=0.08)
+    b = SolveBatchArrays([0.05], [0.5], 4000, 1e-3, [0])
+    so = o.reduce(b, o.simulate(desc, b, seed=0))[0]
+    assert r["phi"] == so["phi"] and r["phi_VR"] == so["phi_VR"] and r["steps"] == so["path_steps"]
