@@ -367,6 +367,22 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=2000, lr=(1e-2, 1e-4), n
     return res
 
 
+def copy_calibration(mib: int, reps: int = 3) -> None:
+    """Plain device copies of known size (torch's copy kernel, `mib` MiB read and
+    written per launch): the reference dispatches that scale rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} copies of {mib} MiB")
+    del a, b
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,6 +396,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven workload (config 3)")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--ttr-replicas", type=int, default=8)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run plain device copies of this many MiB (counter calibration for profiles)")
     ap.add_argument("--train-eager", action="store_true",
                     help="launch training steps one by one instead of replaying captured graphs (for kernel profiles)")
     args = ap.parse_args()
@@ -387,6 +405,8 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by node batches "
               "(rows of one circle stay together), one GPU per share", file=sys.stderr)
         sys.exit(2)
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib)
     if args.workload == "batched":
         print(json.dumps({"batched": bench_batched(steps=args.steps)}), flush=True)
         return

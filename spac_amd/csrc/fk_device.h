@@ -262,6 +262,7 @@ __device__ __forceinline__ float franke_u32(const FrankeTerms32& t) {
 struct CoefFranke32 {
     static constexpr bool kHasDrift = false;
     static constexpr bool kHasKilling = false;
+    static constexpr bool kConstSigma = true;
     using F64 = CoefFranke;
     __device__ __forceinline__ static float sigma(float, float) { return 1.41421356237f; }
     __device__ __forceinline__ static StepCoef32 step_s(float x, float y, float) { return step(x, y); }
@@ -280,6 +281,7 @@ struct CoefFranke32 {
 struct CoefVarFranke32 {
     static constexpr bool kHasDrift = true;
     static constexpr bool kHasKilling = true;
+    static constexpr bool kConstSigma = false;
     using F64 = CoefVarFranke;
     __device__ __forceinline__ static float sigma(float x, float y) { return 1.0f + 0.5f * fsinpi(x) * fcospi(y); }
     __device__ __forceinline__ static StepCoef32 step(float x, float y) { return step_s(x, y, sigma(x, y)); }

@@ -31,6 +31,10 @@ struct SolveSet {
     const double* y0 = nullptr;
     const double* h = nullptr;
     double h_uniform = 0.0;
+    const double* sqrth = nullptr; /* per solve sqrt(h), or nullptr: sqrth_uniform */
+    double sqrth_uniform = 0.0;
+    double* sig0 = nullptr;        /* per solve sigma(X0), FP64 and FP32 (filled on the device) */
+    float* sig0f = nullptr;
     const uint32_t* stream = nullptr;
     const uint32_t* paths = nullptr;
     uint32_t paths_uniform = 0;
@@ -54,6 +58,7 @@ struct Problem {
     DeviceBuffer rec_xt, rec_yt, rec_Y, rec_Z, rec_xi, rec_tau, rec_gval, rec_steps, rec_outcome;
     /* solve table, work queue, outputs */
     DeviceBuffer sx, sy, sh, sstream, spaths, soff, scircle, sitems, queue, stats;
+    DeviceBuffer ssq, ssig, ssigf; /* per solve sqrt(h) and sigma(X0) */
     /* interface tables and row outputs */
     DeviceBuffer ifc, ifnode, ifipsi, ifoff, rows_iface, rows_goff, rows_G, rows_part, rows_col;
     /* control variate (SPAC_CV_*) and its table (T x T x float4) */
@@ -67,17 +72,22 @@ struct Problem {
     DeviceBuffer seed_dev; /* seed read by captured kernels (plans) */
 
     spac_status ensure_records(int64_t total_paths);
+    /* per-solve sigma(X0) slots for n solves (allocated here, never during capture) */
+    spac_status ensure_solve_scratch(int32_t n, SolveSet* ss);
     spac_status upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths, const double* h,
                               const uint32_t* strm, const double* circle, SolveSet* out);
-    /* Enqueue without synchronising (capturable): queue reset + path kernel,
-     * then (d_stats) the reduction.  seed_dev != nullptr: kernels read the seed
-     * from device memory instead of `seed`. */
-    spac_status enqueue_paths(const SolveSet& ss, uint64_t seed, const uint64_t* seed_dev, int* grid_blocks);
+    /* Enqueue without synchronising (capturable): per-solve sigma(X0), queue
+     * reset, path kernel (after_path recorded behind it when given), then the
+     * exit values g(X_tau) / u(X_tau) of every record.  seed_dev != nullptr:
+     * kernels read the seed from device memory instead of `seed`. */
+    spac_status enqueue_paths(const SolveSet& ss, uint64_t seed, const uint64_t* seed_dev, int* grid_blocks,
+                              hipEvent_t after_path = nullptr);
     spac_status enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode);
     spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode);
     void release_all() {
         DeviceBuffer* all[] = {&rec_xt, &rec_yt, &rec_Y, &rec_Z, &rec_xi, &rec_tau, &rec_gval, &rec_steps,
                                &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
+                               &ssq, &ssig, &ssigf,
                                &queue, &stats, &ifc, &ifnode, &ifipsi, &ifoff, &rows_iface, &rows_goff,
                                &rows_G, &rows_part, &rows_col, &cv_table, &seed_dev};
         for (DeviceBuffer* b : all) b->release();

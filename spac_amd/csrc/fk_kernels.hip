@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -72,6 +73,10 @@ struct PathArgs {
     const double* y0;
     const double* h;        /* per solve, or nullptr: h_uniform */
     double h_uniform;
+    const double* sqrth;    /* per solve sqrt(h), or nullptr: sqrth_uniform */
+    double sqrth_uniform;
+    const double* sig0;     /* per solve sigma(X0) (coefficient sets with a variable sigma) */
+    const float* sig0f;
     const uint32_t* stream; /* per solve, or nullptr: solve index */
     const double* circle;   /* 3 per solve (R, cx, cy), or nullptr: circle_uniform */
     double circle_uniform[3];
@@ -193,7 +198,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                 solve = pool_solve;
                 path = pool_next + rank;
                 h = a.h ? a.h[solve] : a.h_uniform;
-                sqrth = __builtin_sqrt(h);
+                sqrth = a.sqrth ? a.sqrth[solve] : a.sqrth_uniform;
                 stream = a.stream ? a.stream[solve] : (uint32_t)solve;
                 rec = a.rec_off ? a.rec_off[solve] : (int64_t)solve * (int64_t)a.paths_uniform;
                 if (SUB == SUB_CIRCLE) {
@@ -205,15 +210,19 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                 }
                 X = a.x0[solve];
                 Yc = a.y0[solve];
+                /* sigma(X0), evaluated once per solve before the launch */
+                sig = Coef::kConstSigma ? Coef::sigma(X, Yc) : a.sig0[solve];
                 if constexpr (REFL) {
                     /* initial Dentro (:166): a start beyond a reflecting side is projected */
                     bool dom0;
                     BoundaryHit h0 = nearest_boundary<SUB>(a, R, cx, cy, X, Yc, &dom0);
                     rrefl = dom0 && side_reflects(a.reflecting, h0.nx, h0.ny);
                     rnx = h0.nx; rny = h0.ny; rd = h0.d; chi_p = 0.0;
-                    if (rrefl && !(h0.d <= 0.0)) { X = h0.ex; Yc = h0.ey; chi_p = h0.d; rd = 0.0; }
+                    if (rrefl && !(h0.d <= 0.0)) {
+                        X = h0.ex; Yc = h0.ey; chi_p = h0.d; rd = 0.0;
+                        sig = Coef::sigma(X, Yc);
+                    }
                 }
-                sig = Coef::sigma(X, Yc);
                 Yw = 1.0; Z = 0.0; xi = 0.0; t = 0.0;
                 nstep = 0;
                 active = true;
@@ -349,8 +358,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                 a.Z[g] = Z;
                 a.xi[g] = xi;
                 a.tau[g] = t;
-                a.gval[g] = (out == SPAC_OUT_SUBDOMAIN) ? exit_u<Coef, CV>(a, hit.ex, hit.ey) : Coef::g(hit.ex, hit.ey);
-                a.steps[g] = nstep;
+                a.steps[g] = nstep; /* gval: fk_exit_values_kernel, lane-dense after the launch */
                 a.outcome[g] = out;
                 active = false;
             }
@@ -422,7 +430,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
                 }
                 xf = (float)a.x0[solve];
                 yf = (float)a.y0[solve];
-                sigf = Coef32::sigma(xf, yf);
+                sigf = Coef32::kConstSigma ? Coef32::sigma(xf, yf) : a.sig0f[solve];
                 Yw = 1.0; Z = 0.0; xi = 0.0;
                 nstep = 0;
                 active = true;
@@ -485,12 +493,36 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
                 a.Z[g] = Z;
                 a.xi[g] = xi;
                 a.tau[g] = (double)nstep * h;
-                a.gval[g] = (out == SPAC_OUT_SUBDOMAIN) ? exit_u<Coef, CV>(a, hit.ex, hit.ey) : Coef::g(hit.ex, hit.ey);
                 a.steps[g] = nstep;
                 a.outcome[g] = out;
                 active = false;
             }
         }
+    }
+}
+
+/* ------------------------------------------- per-solve set-up, exits -- */
+/* sigma(X0) of every solve in FP64 and in FP32 (the FP32 kernel's rounding of X0). */
+template <class Coef, class Coef32>
+__global__ void __launch_bounds__(256) fk_solve_sigma_kernel(int32_t n, const double* x0, const double* y0, double* sig0,
+                                                             float* sig0f) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double x = x0[k], y = y0[k];
+    sig0[k] = Coef::sigma(x, y);
+    sig0f[k] = Coef32::sigma((float)x, (float)y);
+}
+
+/* g(X_tau) for domain exits and capped paths, u(X_tau) (or u_hat) for subdomain
+ * exits (Solvers/FeynmanKacSolver.cpp:249-254): one lane per record, so the four
+ * Franke exponentials of an exit no longer run under a divergent branch in the
+ * path kernel. */
+template <class Coef, int CV>
+__global__ void __launch_bounds__(256) fk_exit_values_kernel(PathArgs a, int64_t total) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        const double x = a.xt[g], y = a.yt[g];
+        a.gval[g] = (a.outcome[g] == SPAC_OUT_SUBDOMAIN) ? exit_u<Coef, CV>(a, x, y) : Coef::g(x, y);
     }
 }
 
@@ -727,6 +759,10 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
     if (circle) SPAC_HIP_CHECK(hipMemcpyAsync(scircle.ptr, circle, nb * 24, hipMemcpyHostToDevice, s));
     if (!items.empty())
         SPAC_HIP_CHECK(hipMemcpyAsync(sitems.ptr, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    std::vector<double> sq(nb);
+    for (size_t k = 0; k < nb; ++k) sq[k] = std::sqrt(h[k]); /* correctly rounded, as the device's */
+    if ((rc = ssq.ensure(nb * 8))) return rc;
+    SPAC_HIP_CHECK(hipMemcpyAsync(ssq.ptr, sq.data(), nb * 8, hipMemcpyHostToDevice, s));
     /* the host vectors die at return: wait for the copies */
     SPAC_HIP_CHECK(hipStreamSynchronize(s));
     SolveSet ss;
@@ -741,6 +777,8 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
     ss.items = (const int4*)sitems.ptr;
     ss.n_items = (int32_t)items.size();
     ss.total_paths = off[n];
+    ss.sqrth = (const double*)ssq.ptr;
+    if ((rc = ensure_solve_scratch(n, &ss))) return rc;
     *out = ss;
     return SPAC_OK;
 }
@@ -770,7 +808,25 @@ static spac_status check_problem(const spac_problem_desc* d) {
 }
 
 /* Queue reset and the persistent path kernel (no synchronisation). */
-spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint64_t* sdev, int* grid_blocks) {
+template <int CV>
+static void launch_exit_values(int coef, dim3 grid, hipStream_t s, const PathArgs& a, int64_t total) {
+    if (coef == SPAC_COEF_VARCOEF_FRANKE)
+        hipLaunchKernelGGL((fk_exit_values_kernel<CoefVarFranke, CV>), grid, dim3(256), 0, s, a, total);
+    else
+        hipLaunchKernelGGL((fk_exit_values_kernel<CoefFranke, CV>), grid, dim3(256), 0, s, a, total);
+}
+
+spac_status Problem::ensure_solve_scratch(int32_t n, SolveSet* ss) {
+    spac_status rc;
+    if ((rc = ssig.ensure((size_t)std::max(n, 1) * 8))) return rc;
+    if ((rc = ssigf.ensure((size_t)std::max(n, 1) * 4))) return rc;
+    ss->sig0 = (double*)ssig.ptr;
+    ss->sig0f = (float*)ssigf.ptr;
+    return SPAC_OK;
+}
+
+spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint64_t* sdev, int* grid_blocks,
+                                   hipEvent_t after_path) {
     PathArgs a;
     memset(&a, 0, sizeof a);
     for (int i = 0; i < 4; ++i) a.dom[i] = desc.domain_params[i];
@@ -783,6 +839,8 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
     a.rho = desc.reflect_rho;
     a.x0 = ss.x0; a.y0 = ss.y0;
     a.h = ss.h; a.h_uniform = ss.h_uniform;
+    a.sqrth = ss.sqrth; a.sqrth_uniform = ss.sqrth_uniform;
+    a.sig0 = ss.sig0; a.sig0f = ss.sig0f;
     a.stream = ss.stream;
     a.circle = ss.circle;
     for (int i = 0; i < 3; ++i) a.circle_uniform[i] = desc.subdomain_params[i];
@@ -817,10 +875,24 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
     int blocks = num_cus * launch.blocks_per_cu;
     int64_t max_blocks = std::max<int64_t>(1, ((int64_t)ss.n_items + (block / 64) - 1) / (block / 64));
     if (blocks > max_blocks) blocks = (int)max_blocks;
+    if (!ss.sqrth && !(ss.sqrth_uniform > 0.0)) return set_error(SPAC_E_INVALID_ARG, "solve set without sqrt(h)");
+    if (desc.coef_kind == SPAC_COEF_VARCOEF_FRANKE) {
+        if (!ss.sig0 || !ss.sig0f) return set_error(SPAC_E_INVALID_ARG, "solve set without sigma(X0) slots");
+        hipLaunchKernelGGL((fk_solve_sigma_kernel<CoefVarFranke, CoefVarFranke32>), dim3((ss.n + 255) / 256), dim3(256),
+                           0, stream, ss.n, ss.x0, ss.y0, ss.sig0, ss.sig0f);
+    }
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
     launch_paths(desc.coef_kind, desc.precision, sub, cv_mode, desc.reflecting != 0, dim3(blocks), dim3(block), stream,
                  a);
     SPAC_HIP_CHECK(hipGetLastError());
+    if (after_path) SPAC_HIP_CHECK(hipEventRecord(after_path, stream));
+    if (ss.total_paths > 0) {
+        const int64_t want = (ss.total_paths + 255) / 256;
+        const int gblocks = (int)std::min<int64_t>(want, (int64_t)num_cus * 16);
+        if (cv_mode == SPAC_CV_TABLE) launch_exit_values<CV_TABLE>(desc.coef_kind, dim3(gblocks), stream, a, ss.total_paths);
+        else launch_exit_values<CV_ANALYTIC>(desc.coef_kind, dim3(gblocks), stream, a, ss.total_paths);
+        SPAC_HIP_CHECK(hipGetLastError());
+    }
     if (grid_blocks) *grid_blocks = blocks;
     return SPAC_OK;
 }
@@ -852,8 +924,7 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
     if ((rc = queue.ensure(64))) return rc;
     int blocks = 0;
     SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
-    if ((rc = enqueue_paths(ss, seed, nullptr, &blocks))) return rc;
-    SPAC_HIP_CHECK(hipEventRecord(ev[1], stream));
+    if ((rc = enqueue_paths(ss, seed, nullptr, &blocks, ev[1]))) return rc;
     if (d_stats && (rc = enqueue_reduce(ss, d_stats, node_mode))) return rc;
     SPAC_HIP_CHECK(hipEventRecord(ev[2], stream));
     SPAC_HIP_CHECK(hipStreamSynchronize(stream));
@@ -990,13 +1061,16 @@ static spac_status solve_points_impl(Problem* p, const double* d_x0, const doubl
     ss.x0 = d_x0;
     ss.y0 = d_y0;
     ss.h_uniform = h;
+    ss.sqrth_uniform = std::sqrt(h);
     ss.paths_uniform = paths;
     ss.total_paths = (int64_t)n * (int64_t)paths;
     ss.chunks_per_solve = (int32_t)((paths + (uint32_t)p->launch.chunk_paths - 1) / (uint32_t)p->launch.chunk_paths);
     int64_t items = (int64_t)ss.chunks_per_solve * n;
     if (items > 0x7fffffff) return set_error(SPAC_E_INVALID_ARG, "too many work items (%lld)", (long long)items);
     ss.n_items = (int32_t)items;
-    spac_status rc = p->run(ss, seed, d_stats, 0);
+    spac_status rc = p->ensure_solve_scratch(n, &ss);
+    if (rc) return rc;
+    rc = p->run(ss, seed, d_stats, 0);
     if (rc) return rc;
     p->last.path_steps = -1;
     return SPAC_OK;

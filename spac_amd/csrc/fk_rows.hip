@@ -495,8 +495,7 @@ extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interf
     hipStream_t s = p->stream;
     int blocks = 0;
     ROWS_HIP_CHECK(hipEventRecord(p->ev[0], s));
-    if ((rc = p->enqueue_paths(rs.ss, seed, nullptr, &blocks))) return rc;
-    ROWS_HIP_CHECK(hipEventRecord(p->ev[1], s));
+    if ((rc = p->enqueue_paths(rs.ss, seed, nullptr, &blocks, p->ev[1]))) return rc;
     if ((rc = p->enqueue_reduce(rs.ss, rs.d_stats, 1))) return rc;
     ROWS_HIP_CHECK(hipEventRecord(p->ev[2], s));
     if ((rc = rows_enqueue_bg(p, rs))) return rc;
