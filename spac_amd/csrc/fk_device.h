@@ -162,29 +162,22 @@ __device__ __forceinline__ BoundaryHit rectangle_distance(const double* p, doubl
     double d0 = __builtin_fabs(x - c0), d1 = __builtin_fabs(y - c1);
     double sx = (x > c0) ? 1.0 : -1.0;
     double sy = (y > c1) ? 1.0 : -1.0;
+    /* the quadrant tree as selects (lanes of a wave take different sides): the
+     * nearest side inside, the side crossed outside; corners below */
+    const bool inside = (d0 <= h0) && (d1 <= h1);
+    const bool xside = inside ? (d0 - h0 > d1 - h1) : (d0 > h0);
     BoundaryHit r;
-    r.ex = x; r.ey = y;
-    bool inside = (d0 <= h0) && (d1 <= h1);
-    if (inside) {
-        if (d0 - h0 > d1 - h1) {
-            r.ex = sx * h0 + c0; r.nx = sx; r.ny = 0.0; r.d = d0 - h0;
-        } else {
-            r.ey = sy * h1 + c1; r.nx = 0.0; r.ny = sy; r.d = d1 - h1;
-        }
-        return r;
+    r.ex = xside ? sx * h0 + c0 : x;
+    r.ey = xside ? y : sy * h1 + c1;
+    r.nx = xside ? sx : 0.0;
+    r.ny = xside ? 0.0 : sy;
+    r.d = xside ? d0 - h0 : d1 - h1;
+    if (!inside && d0 > h0 && d1 > h1) { /* outside a corner (rare) */
+        double e0 = d0 - h0, e1 = d1 - h1;
+        r.d = __builtin_sqrt(e0 * e0 + e1 * e1);
+        r.ex = sx * h0 + c0; r.ey = sy * h1 + c1;
+        r.nx = sx * K; r.ny = sy * K;
     }
-    if (d0 > h0) {
-        if (d1 > h1) {
-            double e0 = d0 - h0, e1 = d1 - h1;
-            r.d = __builtin_sqrt(e0 * e0 + e1 * e1);
-            r.ex = sx * h0 + c0; r.ey = sy * h1 + c1;
-            r.nx = sx * K; r.ny = sy * K;
-            return r;
-        }
-        r.ex = sx * h0 + c0; r.nx = sx; r.ny = 0.0; r.d = d0 - h0;
-        return r;
-    }
-    r.ey = sy * h1 + c1; r.nx = 0.0; r.ny = sy; r.d = d1 - h1;
     return r;
 }
 

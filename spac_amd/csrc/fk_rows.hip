@@ -172,10 +172,11 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_imq_kernel(RowsArgs a) {
 }
 
 /* Gaussian stencil rows: per 256-path chunk every thread computes its path's exit angle,
- * stencil start s and the S values Y H_{s+r}(theta) (H_{s+r} = sum_i iPsiS(i, r) phi_i with
- * exact FMAs, i in order) into LDS.  Then each quarter-wave takes one path at a time — lane r
- * adds value r into column s + r (mod M) of its own copy of the row — so a path costs a quarter
- * of a wave instead of every column lane.  Copy (w, q) holds the paths p of every chunk with
+ * stencil start s and the S Gaussian values phi_i into LDS.  Then each quarter-wave takes one
+ * path at a time: lane r forms Y H_{s+r}(theta), H_{s+r} = sum_i iPsiS(i, r) phi_i with exact
+ * FMAs, i in order (the stencil inverse is read a row of 16 contiguous values at a time, from
+ * LDS for full circles and L1/L2 for arcs), and adds it into column s + r (mod M) of the
+ * quarter-wave's own copy of the row.  Copy (w, q) holds the paths p of every chunk with
  * p / 64 = w and p % 4 = q, in path order; the 16 copies are merged in (w, q) order.
  * Dynamic LDS: kRowsBlock * wstride doubles of values, then 16 * M TwoSum accumulators. */
 constexpr int kCopies = kRowsWaves * 4;
@@ -194,10 +195,11 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
     const double* ipsi = a.ipsi + (a.ipsi_offset[iface] >= 0 ? a.ipsi_offset[iface] : 0);
     double* G = a.G + a.g_off[job];
 
-    double* s_v = s_dyn;
+    double* s_v = s_dyn; /* per path of the chunk: its S Gaussian values phi */
     spac_ksum* s_acc = reinterpret_cast<spac_ksum*>(s_dyn + (size_t)kRowsBlock * wstride);
     __shared__ double s_th[kMaxCols];
     __shared__ int s_s[kRowsBlock]; /* stencil start, -1: not a circle exit */
+    __shared__ double s_y[kRowsBlock];
     __shared__ double s_blk[kMaxStencil * kMaxStencil];
 
     const int lane = tid & 63, wave = tid >> 6;
@@ -206,6 +208,7 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
     if (circle)
         for (int i = tid; i < S * S; i += kRowsBlock) s_blk[i] = ipsi[i];
     for (int i = tid; i < kCopies * M; i += kRowsBlock) s_acc[i] = spac_ksum_zero();
+    s_s[tid] = -1; /* phase 2 may read entries of paths beyond the chunk: -1 or a valid start */
     __syncthreads();
     if (np == 0) {
         for (int j = tid; j < M; j += kRowsBlock) G[j] = 0.0;
@@ -216,41 +219,54 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
     const int64_t off = a.rec_off[job];
     for (int64_t base = 0; base < np; base += kRowsBlock) {
         const int cnt = (np - base < (int64_t)kRowsBlock) ? (int)(np - base) : kRowsBlock;
+        /* phase 1, one thread per path: exit angle, stencil start, Gaussians */
         if (tid < cnt) {
             const int64_t g = off + base + tid;
             int s = -1;
             if (a.outcome[g] == SPAC_OUT_SUBDOMAIN) {
                 double theta = 0.0;
                 spac_exit_angle(code, a.xt[g] - cx, a.yt[g] - cy, s_th[0], &theta);
-                const double Yp = a.Y[g];
                 double phi[kMaxStencil];
                 s = circle ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
                            : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
-                if (circle) { /* the one stencil inverse, broadcast from LDS */
-                    for (int rr = 0; rr < S; ++rr) {
-                        double w = 0.0;
-                        for (int i = 0; i < S; ++i) w = __builtin_fma(s_blk[i * S + rr], phi[i], w);
-                        s_v[tid * wstride + rr] = Yp * w;
-                    }
-                } else {
-                    const double* blk = ipsi + (int64_t)s * S * S;
-                    for (int rr = 0; rr < S; ++rr) {
-                        double w = 0.0;
-                        for (int i = 0; i < S; ++i) w = __builtin_fma(blk[i * S + rr], phi[i], w);
-                        s_v[tid * wstride + rr] = Yp * w;
-                    }
-                }
+                for (int i = 0; i < S; ++i) s_v[tid * wstride + i] = phi[i];
+                s_y[tid] = a.Y[g];
             }
             s_s[tid] = s;
         }
         __syncthreads();
+        /* phase 2, one quarter-wave per path, lane r = stencil column r: the weight
+         * w_r = sum_i iPsi_s[i][r] phi_i (the stencil's inverse read row by row, 16
+         * contiguous values per quarter-wave).  The quarter-wave's 16 paths of the chunk
+         * are independent here, so their loads overlap: indices are clamped to valid
+         * entries and unused results dropped instead of branching. */
+        const int p0 = wave * 64 + quarter;
         const int p1 = min(cnt, wave * 64 + 64);
-        for (int p = wave * 64 + quarter; p < p1; p += 4) {
-            const int s = s_s[p];
-            if (s >= 0 && r < S) {
-                int c = s + r;
-                if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
-                my[c] = spac_ksum_add(my[c], s_v[p * wstride + r]);
+        const int rr = (r < S) ? r : 0;
+        double yw[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int p = p0 + 4 * k;
+            const int pc = (p < p1) ? p : p0;
+            const int s = s_s[pc];
+            const int sc = (s >= 0) ? s : 0;
+            const double* blk = circle ? s_blk : ipsi + (int64_t)sc * S * S;
+            const double* v = s_v + pc * wstride;
+            double w = 0.0;
+            for (int i = 0; i < S; ++i) w = __builtin_fma(blk[i * S + rr], v[i], w);
+            yw[k] = s_y[pc] * w;
+        }
+        /* then Y w_r into column s + r (mod M) of the quarter-wave's own copy, paths in order */
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int p = p0 + 4 * k;
+            if (p < p1) {
+                const int s = s_s[p];
+                if (s >= 0 && r < S) {
+                    int c = s + r;
+                    if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
+                    my[c] = spac_ksum_add(my[c], yw[k]);
+                }
             }
         }
         __syncthreads();

@@ -101,9 +101,11 @@ SPAC_HD double spac_exp(double x) {
     int k = (int)kd;
     int k1 = k >> 1;
     double res = (p * spac_pow2i(k1)) * spac_pow2i(k - k1);
-    if (!(x < 709.782712893384)) res = (x != x) ? x : spac_u64_as_f64(0x7ff0000000000000ULL);
-    if (x < -745.1332191019412) res = 0.0;
-    return res;
+    /* special cases as selects, not branches, so that a caller's independent
+     * exponentials can be interleaved by the GPU scheduler */
+    double big = (x != x) ? x : spac_u64_as_f64(0x7ff0000000000000ULL);
+    res = (x < 709.782712893384) ? res : big;
+    return (x < -745.1332191019412) ? 0.0 : res;
 }
 
 /* ----------------------------------------------------------------- log -- */
@@ -111,15 +113,15 @@ SPAC_HD double spac_exp(double x) {
 SPAC_HD double spac_log(double x) {
     const double LN2_HI = 0.6931471803691238;
     const double LN2_LO = 1.9082149292705877e-10;
-    if (!(x > 0.0)) return (x == 0.0) ? -spac_u64_as_f64(0x7ff0000000000000ULL)
-                                      : spac_u64_as_f64(0x7ff8000000000000ULL);
-    int scale = 0;
-    if (x < 2.2250738585072014e-308) { x *= 18014398509481984.0; scale = -54; } /* 2^54 */
-    uint64_t bits = spac_f64_as_u64(x);
+    /* straight-line code (selects, no branches); x <= 0 and NaN are fixed up at the end */
+    const int sub = x < 2.2250738585072014e-308;
+    const double xs = sub ? x * 18014398509481984.0 : x; /* 2^54 */
+    uint64_t bits = spac_f64_as_u64(xs);
     int e = (int)((bits >> 52) & 0x7ff) - 1023;
     double m = spac_u64_as_f64((bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL); /* [1,2) */
-    if (m > 1.4142135623730951) { m *= 0.5; e += 1; }
-    e += scale;
+    const int hi = m > 1.4142135623730951;
+    m = hi ? m * 0.5 : m;
+    e += hi + (sub ? -54 : 0);
     double f = (m - 1.0) / (m + 1.0);
     double s = f * f;
     /* log m = 2f (1 + s/3 + s^2/5 + ... + s^11/23) */
@@ -136,7 +138,9 @@ SPAC_HD double spac_log(double x) {
     p = SPAC_FMA(p, s, 0.6666666666666666);
     double logm = SPAC_FMA(f * s, p, 2.0 * f);
     double ed = (double)e;
-    return SPAC_FMA(ed, LN2_HI, SPAC_FMA(ed, LN2_LO, logm));
+    double res = SPAC_FMA(ed, LN2_HI, SPAC_FMA(ed, LN2_LO, logm));
+    double bad = (x == 0.0) ? -spac_u64_as_f64(0x7ff0000000000000ULL) : spac_u64_as_f64(0x7ff8000000000000ULL);
+    return (x > 0.0) ? res : bad;
 }
 
 /* ------------------------------------------------------------ sincospi -- */
