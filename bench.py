@@ -29,6 +29,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# HBM bytes per launch of the path kernel, from two rocprofv3 --pmc passes of this
+# script (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by what the counters
+# read for plain device copies of 512 MiB in the same runs (--copy-mib 512):
+# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.  Commands and figures:
+# profiles/fk_path_kernel_fp64_franke_points_256x10k.md and
+# profiles/pddsparse_rows_fp64_4x4_1952rows_20k.md.
+PATH_KERNEL_TRAFFIC_BYTES = {
+    "points": (85.5 * 2.0 + 507894.5) * 1024,
+    "rows": (7177.5 * 2.0 + 7598712.3) * 1024,
+}
+RECORD_BYTES_PER_PATH = 6 * 8 + 4 + 1  # x_tau, y_tau, Y, Z, xi, tau; steps; outcome
+
+
 def config1_points():
     t = -1.0 + 2.0 * (np.arange(16) + 1) / 17
     X, Y = np.meshgrid(t, t, indexing="ij")
@@ -115,7 +128,8 @@ def bench_points(args):
                                "10000 paths/point, h=1e-3, sigma=sqrt(2) I, Philox seed 0",
                    "points": n, "paths_per_point": paths, "h": h, "path_steps_per_step": path_steps},
         "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": PATH_KERNEL_TRAFFIC_BYTES["points"],
+                     "traffic_unit": "B per launch", "algorithmic_bytes": RECORD_BYTES_PER_PATH * n * paths,
                      "kernel": "fk_path_kernel_fp64<CoefFranke, SUB_NONE>", "kernel_avg_ms": avg_ms,
                      "flops_per_path_step": fps},
     }
@@ -169,6 +183,7 @@ def bench_rows(args):
         path_ms.append(t.path_kernel_ms)
     log(f"timed {args.steps} steps")
     path_steps = int(sum(x["path_steps"] for x in rows["stats"]))
+    n_paths = int(sum(x["paths"] for x in rows["stats"]))
     mesh.accumulate(rows)
     Gi, Gj, Gv, B = mesh.csr()
     t0 = time.perf_counter()
@@ -200,7 +215,8 @@ def bench_rows(args):
                    "rows": int(mesh.info.n_jobs), "nodes": int(mesh.info.n_nodes), "paths_per_row": paths, "h": h,
                    "path_steps_per_step": path_steps},
         "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": PATH_KERNEL_TRAFFIC_BYTES["rows"],
+                     "traffic_unit": "B per launch", "algorithmic_bytes": RECORD_BYTES_PER_PATH * n_paths,
                      "kernel": "fk_path_kernel_fp64<CoefVarFranke, SUB_CIRCLE>", "kernel_avg_ms": avg_path_ms,
                      "flops_per_path_step": fps},
         "pcie_inclusive_path_steps_per_s": path_steps / float(np.mean(wall)),
