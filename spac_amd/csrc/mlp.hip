@@ -46,6 +46,9 @@ static_assert(kNParams == SPAC_MLP_N_PARAMS, "parameter count");
 
 /* ------------------------------------------------------------ training -- */
 constexpr int kTile = 16;        /* samples per block */
+constexpr int kWs = 68;          /* LDS row stride of a transposed 64-wide weight matrix: float4
+                                    aligned, and rows fall on different banks (stride 64 puts a
+                                    wave's transposing writes on one bank) */
 constexpr int kChunk = kTile;    /* samples per weight-gradient partial */
 constexpr int kPad = kTile + 1;  /* padded LDS row */
 constexpr int kOG = 256 / kTile; /* groups of 4 features per block (16 x 4 = 64) */
@@ -88,12 +91,12 @@ struct FwdArgs {
  * partials in order; the last layer's tangents are then never written. */
 template <int IN, bool HEADS>
 __global__ void __launch_bounds__(256) k_fwd_tile(FwdArgs a) {
-    __shared__ __align__(16) float Lw[IN * 64]; /* W^T: [i][o] */
+    __shared__ __align__(16) float Lw[IN * kWs]; /* W^T: [i][o] */
     __shared__ float Lh[IN * kTile], Ltx[IN * kTile], Lty[IN * kTile];
     __shared__ float Lred[kOG][5][kTile];
     const int tid = threadIdx.x, c = tid % kTile, og = tid / kTile;
     const int s0 = blockIdx.x * kTile, s = s0 + c, n = a.n;
-    for (int k = tid; k < 64 * IN; k += 256) Lw[(k % IN) * 64 + k / IN] = a.W[k];
+    for (int k = tid; k < 64 * IN; k += 256) Lw[(k % IN) * kWs + k / IN] = a.W[k];
     for (int k = tid; k < IN * kTile; k += 256) {
         const int i = k / kTile, cc = k % kTile, gs = s0 + cc;
         const bool in = gs < n;
@@ -116,7 +119,7 @@ __global__ void __launch_bounds__(256) k_fwd_tile(FwdArgs a) {
             tx = Ltx[i * kTile + c];
             ty = Lty[i * kTile + c];
         }
-        const float4 w4 = *reinterpret_cast<const float4*>(Lw + i * 64 + f0);
+        const float4 w4 = *reinterpret_cast<const float4*>(Lw + i * kWs + f0);
         const float w[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
         for (int o = 0; o < 4; ++o) {
@@ -136,7 +139,7 @@ __global__ void __launch_bounds__(256) k_fwd_tile(FwdArgs a) {
         float tx, ty;
         if (IN == 2) {
             tx = d * (Lw[f] * a.scx);
-            ty = d * (Lw[64 + f] * a.scy);
+            ty = d * (Lw[kWs + f] * a.scy);
         } else {
             tx = d * ax[o];
             ty = d * ay[o];
@@ -291,7 +294,7 @@ struct StepArgs {
     int n;
 };
 __global__ void __launch_bounds__(256) k_train_step_tile(StepArgs a) {
-    __shared__ __align__(16) float Lw[64 * 64];
+    __shared__ __align__(16) float Lw[64 * kWs];
     __shared__ float Lz[2 * kPad];
     __shared__ float Lh[3][64 * kPad];
     __shared__ float Lt[2][2][64 * kPad]; /* [buffer][direction] tangents */
@@ -331,7 +334,7 @@ __global__ void __launch_bounds__(256) k_train_step_tile(StepArgs a) {
         const float* W = P + (L == 1 ? oW2 : oW3);
         const float* b = P + (L == 1 ? ob2 : ob3);
         __syncthreads();
-        for (int k = tid; k < 64 * 64; k += 256) Lw[(k & 63) * 64 + (k >> 6)] = W[k];
+        for (int k = tid; k < 64 * 64; k += 256) Lw[(k & 63) * kWs + (k >> 6)] = W[k];
         __syncthreads();
         const float* hin = Lh[L - 1];
         const float* txin = Lt[(L - 1) & 1][0];
@@ -342,7 +345,7 @@ __global__ void __launch_bounds__(256) k_train_step_tile(StepArgs a) {
 #pragma unroll 4
         for (int i = 0; i < 64; ++i) {
             const float hv = hin[i * kPad + c], tx = txin[i * kPad + c], ty = tyin[i * kPad + c];
-            const float4 w4 = *reinterpret_cast<const float4*>(Lw + i * 64 + f0);
+            const float4 w4 = *reinterpret_cast<const float4*>(Lw + i * kWs + f0);
             const float w[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
             for (int o = 0; o < 4; ++o) {
@@ -485,13 +488,13 @@ __global__ void __launch_bounds__(256) k_train_step_tile(StepArgs a) {
     }
 }
 
-/* grad[k] = sum over chunks of part[chunk][k] in a fixed order (8 interleaved
+/* grad[k] = sum over chunks of part[chunk][k] in a fixed order (32 interleaved
  * groups of chunks, each summed in index order, then the groups in order);
  * then, in training, one Adam step with a geometric learning-rate decay.  The
  * step counter lives in device memory so that a captured graph of steps
  * replays unchanged: every block reads it, and the last block to finish
  * advances it. */
-constexpr int kRedParams = 32, kRedGroups = 8;
+constexpr int kRedParams = 32, kRedGroups = 32;
 struct AdamArgs {
     float* p;
     float* g;
@@ -510,8 +513,11 @@ __global__ void __launch_bounds__(kRedParams * kRedGroups) k_reduce_adam(AdamArg
     const int pl = threadIdx.x % kRedParams, grp = threadIdx.x / kRedParams;
     const int k = blockIdx.x * kRedParams + pl;
     float acc = 0.0f;
-    if (k < kNParams)
+    if (k < kNParams) {
+        /* sequential sum in chunk order; unrolled so that the loads overlap */
+#pragma unroll 8
         for (int c = grp; c < a.chunks; c += kRedGroups) acc += a.part[(size_t)c * kNParams + k];
+    }
     sh[grp][pl] = acc;
     __syncthreads();
     if (grp == 0 && k < kNParams) {
