@@ -311,20 +311,20 @@ __device__ __forceinline__ float rect_distance32(const float* p, float x, float 
     float h0 = p[2] - c0, h1 = p[3] - c1;
     float d0 = __builtin_fabsf(x - c0), d1 = __builtin_fabsf(y - c1);
     float sx = (x > c0) ? 1.0f : -1.0f, sy = (y > c1) ? 1.0f : -1.0f;
-    *ex = x; *ey = y;
-    if (d0 <= h0 && d1 <= h1) {
-        if (d0 - h0 > d1 - h1) { *ex = sx * h0 + c0; *nx = sx; *ny = 0.0f; return d0 - h0; }
-        *ey = sy * h1 + c1; *nx = 0.0f; *ny = sy; return d1 - h1;
+    /* the quadrant tree as selects, corners in a (rare) branch */
+    const bool inside = d0 <= h0 && d1 <= h1;
+    const bool xside = inside ? (d0 - h0 > d1 - h1) : (d0 > h0);
+    *ex = xside ? sx * h0 + c0 : x;
+    *ey = xside ? y : sy * h1 + c1;
+    *nx = xside ? sx : 0.0f;
+    *ny = xside ? 0.0f : sy;
+    float d = xside ? d0 - h0 : d1 - h1;
+    if (!inside && d0 > h0 && d1 > h1) {
+        float e0 = d0 - h0, e1 = d1 - h1;
+        *ex = sx * h0 + c0; *ey = sy * h1 + c1; *nx = sx * K; *ny = sy * K;
+        d = __builtin_sqrtf(e0 * e0 + e1 * e1);
     }
-    if (d0 > h0) {
-        if (d1 > h1) {
-            float e0 = d0 - h0, e1 = d1 - h1;
-            *ex = sx * h0 + c0; *ey = sy * h1 + c1; *nx = sx * K; *ny = sy * K;
-            return __builtin_sqrtf(e0 * e0 + e1 * e1);
-        }
-        *ex = sx * h0 + c0; *nx = sx; *ny = 0.0f; return d0 - h0;
-    }
-    *ey = sy * h1 + c1; *nx = 0.0f; *ny = sy; return d1 - h1;
+    return d;
 }
 
 /* Tabulated control variate in FP32: bilinear d u_hat/dx, d u_hat/dy at (x, y),
