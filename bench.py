@@ -311,7 +311,8 @@ def _replicas(mesh, prob, paths, h, seeds, u0=None):
     return out, rms_u, rms_row, ubar
 
 
-def bench_ttr(target=1e-3, probe_paths=256, train_steps=2000, lr=(1e-2, 1e-4), n_rep=8, seed0=100, eager=False):
+def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n_rep=8, seed0=100, eager=False,
+              beta2=0.95):
     """Time to a statistical RMS error `target` of the solved interface values, learned control
     variate against plain Monte Carlo, both FP32 stepping on this GPU.
 
@@ -363,11 +364,12 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=2000, lr=(1e-2, 1e-4), n
         data_ms = pp.timing().total_ms
     u_mc = np.array([x["phi"] for x in st])
     with MLP(dom, seed=0) as net, FKProblem(desc) as prob:
-        rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager)
+        rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager,
+                        beta2=beta2)
         table_ms = net.build_table(1024)
         net.attach(prob, fallback=True)
         res["learned"] = measure(prob, "learned", {"training_data_ms": data_ms, "train_ms": rep["train_ms"],
-                                                   "train_steps": train_steps, "train_lr": list(lr),
+                                                   "train_steps": train_steps, "train_lr": list(lr), "train_beta2": beta2 or 0.999,
                                                    "train_loss": rep["final_loss"],
                                                    "table_ms": table_ms})
     P, L = res["plain"], res["learned"]
@@ -410,7 +412,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-RMS comparison (config 5)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven workload (config 3)")
-    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--train-steps", type=int, default=1000)
+    ap.add_argument("--train-lr", type=float, nargs=2, default=(2e-2, 1e-4), metavar=("LR0", "LR1"))
+    ap.add_argument("--train-beta2", type=float, default=0.95, help="Adam beta2 (0: 0.999)")
     ap.add_argument("--ttr-replicas", type=int, default=8)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run plain device copies of this many MiB (counter calibration for profiles)")
@@ -427,7 +431,8 @@ def main():
         print(json.dumps({"batched": bench_batched(steps=args.steps)}), flush=True)
         return
     if args.workload == "ttr":
-        print(json.dumps({"time_to_rms": bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, eager=args.train_eager)}), flush=True)
+        print(json.dumps({"time_to_rms": bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, eager=args.train_eager,
+                                     lr=tuple(args.train_lr), beta2=args.train_beta2)}), flush=True)
         return
     out = bench_rows(args) if args.workload == "rows" else bench_points(args)
     if not args.no_batched:
@@ -435,7 +440,8 @@ def main():
         out["batched"] = bench_batched()
     if not args.no_ttr:
         log("time to RMS 1e-3 (config 5) ...")
-        out["time_to_rms"] = bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, eager=args.train_eager)
+        out["time_to_rms"] = bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, eager=args.train_eager,
+                                     lr=tuple(args.train_lr), beta2=args.train_beta2)
     print(json.dumps(out), flush=True)
 
 

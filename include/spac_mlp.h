@@ -32,8 +32,8 @@ typedef struct {
     uint64_t seed;    /* Glorot-uniform initialisation */
 } spac_mlp_desc;
 
-/* Adam on the full batch; learning rate decays geometrically from lr to
- * lr_final (lr_final <= 0: constant).  Loss
+/* Adam on the full batch (beta1 0.9, beta2 as given, 0: 0.999); learning rate
+ * decays geometrically from lr to lr_final (lr_final <= 0: constant).  Loss
  *   L = mean_k (o0_k - u_k)^2 + grad_weight ((o1_k - gx_k)^2 + (o2_k - gy_k)^2)
  * with (gx, gy) the forward-mode gradient of o0, held fixed per step. */
 typedef struct {
@@ -42,6 +42,7 @@ typedef struct {
     double lr;
     double lr_final;
     double grad_weight;
+    double beta2;    /* Adam second-moment decay in (0, 1); 0 selects 0.999 */
 } spac_mlp_train_cfg;
 
 typedef struct {

@@ -221,6 +221,7 @@ class MlpTrainCfg(C.Structure):
         ("lr", C.c_double),
         ("lr_final", C.c_double),
         ("grad_weight", C.c_double),
+        ("beta2", C.c_double),
     ]
 
 

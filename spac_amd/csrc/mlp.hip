@@ -506,6 +506,7 @@ struct AdamArgs {
     int t0;             /* counter value at the first step of this schedule */
     int steps;
     float lr0, lr1;     /* lr1 <= 0: constant lr0 */
+    float beta2;
     int adam;
 };
 __global__ void __launch_bounds__(kRedParams * kRedGroups) k_reduce_adam(AdamArgs a) {
@@ -525,7 +526,7 @@ __global__ void __launch_bounds__(kRedParams * kRedGroups) k_reduce_adam(AdamArg
         for (int q = 0; q < kRedGroups; ++q) g += sh[q][pl];
         a.g[k] = g;
         if (a.adam) {
-            const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+            const float b1 = 0.9f, b2 = a.beta2, eps = 1e-8f;
             const int tp = a.ctr[0];
             const float t = (float)(tp + 1);
             float lr = a.lr0;
@@ -668,6 +669,7 @@ struct Mlp {
     int n = 0; /* prepared batch size */
     int adam_t = 0;
     int table_t = 0;
+    float beta2 = 0.999f; /* of the training call being enqueued */
     float lox = 0, loy = 0, sx = 1, sy = 1;
 };
 
@@ -767,6 +769,7 @@ static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float 
     a.steps = steps;
     a.lr0 = lr0;
     a.lr1 = lr1;
+    a.beta2 = m->beta2;
     a.adam = adam ? 1 : 0;
     hipLaunchKernelGGL(k_reduce_adam, dim3((kNParams + kRedParams - 1) / kRedParams), dim3(kRedParams * kRedGroups),
                        0, s, a);
@@ -921,8 +924,11 @@ spac_status spac_mlp_train(spac_mlp* hm, const double* x, const double* y, const
     Mlp* m = reinterpret_cast<Mlp*>(hm);
     if (!m || !x || !y || !u || n < 1 || !cfg || cfg->steps < 1 || !(cfg->lr > 0.0))
         return set_error(SPAC_E_INVALID_ARG, "bad arguments (n >= 1, steps >= 1, lr > 0)");
+    if (cfg->beta2 != 0.0 && !(cfg->beta2 > 0.0 && cfg->beta2 < 1.0))
+        return set_error(SPAC_E_INVALID_ARG, "beta2 must be in (0, 1), or 0 for 0.999");
     spac_status rc;
     if ((rc = mlp_prepare_batch(m, x, y, u, n))) return rc;
+    m->beta2 = cfg->beta2 > 0.0 ? (float)cfg->beta2 : 0.999f;
     const float lam = (float)cfg->grad_weight, lr0 = (float)cfg->lr;
     const float lr1 = cfg->lr_final > 0.0 ? (float)cfg->lr_final : 0.0f;
     const int t0 = m->adam_t;

@@ -66,9 +66,10 @@ class MLP:
                                               grad_weight, abi.ptr(g, C.c_float), C.byref(loss)), self.lib)
         return loss.value, g
 
-    def train(self, x, y, u, steps, lr=3e-3, lr_final=1e-4, grad_weight=0.1, eager=False):
+    def train(self, x, y, u, steps, lr=3e-3, lr_final=1e-4, grad_weight=0.1, eager=False, beta2=0.0):
+        """beta2: Adam second-moment decay (0: 0.999)."""
         x, y, u = _d(x), _d(y), _d(u)
-        cfg = abi.MlpTrainCfg(steps, 1 if eager else 0, lr, lr_final, grad_weight)
+        cfg = abi.MlpTrainCfg(steps, 1 if eager else 0, lr, lr_final, grad_weight, beta2)
         rep = abi.MlpTrainReport()
         abi.check(self.lib.spac_mlp_train(self.h, abi.ptr(x, dp), abi.ptr(y, dp), abi.ptr(u, dp), x.size,
                                           C.byref(cfg), C.byref(rep)), self.lib)

@@ -182,3 +182,19 @@ def test_eager_training_equals_graph_training(gpu):
             m.train(x, y, u, steps=120, lr=3e-3, lr_final=1e-4, grad_weight=0.03, eager=eager)
             out.append(m.params)
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.gpu
+def test_training_beta2_option(gpu):
+    """beta2 of Adam is selectable (0 = 0.999); out-of-range values are rejected."""
+    from spac_amd import abi
+    from spac_amd.mlp import MLP
+
+    x, y, _ = _samples(1024, 16)
+    u = franke_u(x, y)
+    with MLP(DOM, seed=2) as m:
+        l0, _ = m.loss_grad(x, y, u, 0.03)
+        rep = m.train(x, y, u, steps=400, lr=2e-2, lr_final=1e-4, grad_weight=0.03, beta2=0.95)
+        assert rep["final_loss"] < 0.05 * l0
+        with pytest.raises(abi.SpacError):
+            m.train(x, y, u, steps=10, lr=1e-3, lr_final=1e-4, grad_weight=0.03, beta2=1.5)
