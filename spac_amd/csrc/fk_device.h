@@ -213,6 +213,11 @@ struct StepCoef32 {
     float s, bx, by, c, f, Fx, Fy;
 };
 
+/* The FP32 path is checked against the FP64 run within stated tolerances, not bit
+ * for bit, so its arithmetic may contract a * b + c into FMAs (the build disables
+ * contraction elsewhere, for the bit-reproducible FP64 path). */
+#pragma clang fp contract(fast)
+
 __device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 /* sin(pi x), cos(pi x) */
 __device__ __forceinline__ float fsinpi(float x) { return __builtin_amdgcn_sinf(0.5f * x); }
@@ -345,6 +350,8 @@ __device__ __forceinline__ void cv_grad32(const CvTable32& c, float x, float y, 
     *gx = by * (bx * v00.y + ax * v10.y) + ay * (bx * v01.y + ax * v11.y);
     *gy = by * (bx * v00.z + ax * v10.z) + ay * (bx * v01.z + ax * v11.z);
 }
+
+#pragma clang fp contract(off)
 
 } // namespace spac
 

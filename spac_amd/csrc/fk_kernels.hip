@@ -371,6 +371,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
  * work queue, lane refill and record layout as the FP64 kernel. */
 template <class Coef32, int SUB, int CV>
 __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
+#pragma clang fp contract(fast) /* FP32 path: tolerance-checked, not bit-pinned */
     using Coef = typename Coef32::F64;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t seed = a.seed_dev ? *a.seed_dev : a.seed;
@@ -1004,7 +1005,7 @@ spac_status spac_fk_problem_create(const spac_problem_desc* desc, spac_problem**
     for (int i = 0; i < 4; ++i) SPAC_HIP_CHECK(hipEventCreate(&p->ev[i]));
     p->launch.block_size = 256;
     p->launch.blocks_per_cu = 8;
-    p->launch.chunk_paths = 256;
+    p->launch.chunk_paths = 64; /* paths per work item: smaller items balance the tail of heavy nodes */
     *out = reinterpret_cast<spac_problem*>(p);
     return SPAC_OK;
 }
