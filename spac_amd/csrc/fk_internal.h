@@ -43,6 +43,7 @@ struct SolveSet {
     const int4* items = nullptr;
     int32_t n_items = 0;
     int32_t chunks_per_solve = 0;
+    uint32_t chunk = 0;            /* paths per work item (see work_item_paths) */
     int64_t total_paths = 0;
 };
 

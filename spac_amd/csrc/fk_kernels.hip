@@ -720,13 +720,24 @@ spac_status Problem::ensure_records(int64_t total_paths) {
     return SPAC_OK;
 }
 
+/* Paths per work item: the launch's chunk size, enlarged for very large batches so
+ * that the queue holds at most ~2^20 items (every item costs one atomic on a single
+ * counter; 1.6e7 of them cost ~120 ms).  Results do not depend on it. */
+static uint32_t work_item_paths(uint32_t chunk, int64_t total_paths) {
+    const int64_t kMaxItems = (int64_t)1 << 20;
+    const int64_t c = (total_paths + kMaxItems - 1) / kMaxItems;
+    return (uint32_t)std::max<int64_t>((int64_t)chunk, c);
+}
+
 /* Upload a host batch of solves into the handle's solve table and build the
  * work queue (solves with zero paths get no work items). */
 spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths,
                                    const double* h, const uint32_t* strm, const double* circle, SolveSet* out) {
     std::vector<int64_t> off(n + 1, 0);
     std::vector<int4> items;
-    const uint32_t chunk = (uint32_t)launch.chunk_paths;
+    int64_t total = 0;
+    for (int k = 0; k < n; ++k) total += paths[k];
+    const uint32_t chunk = work_item_paths((uint32_t)launch.chunk_paths, total);
     for (int k = 0; k < n; ++k) {
         if (!(h[k] > 0.0) || !(h[k] < 1e30)) return set_error(SPAC_E_INVALID_ARG, "solve %d has bad h", k);
         if (circle && !(circle[3 * k] > 0.0))
@@ -778,6 +789,7 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
     ss.items = (const int4*)sitems.ptr;
     ss.n_items = (int32_t)items.size();
     ss.total_paths = off[n];
+    ss.chunk = chunk;
     ss.sqrth = (const double*)ssq.ptr;
     if ((rc = ensure_solve_scratch(n, &ss))) return rc;
     *out = ss;
@@ -847,7 +859,7 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
     for (int i = 0; i < 3; ++i) a.circle_uniform[i] = desc.subdomain_params[i];
     a.rec_off = ss.rec_off;
     a.paths_uniform = ss.paths_uniform;
-    a.chunk = (uint32_t)launch.chunk_paths;
+    a.chunk = ss.chunk ? ss.chunk : (uint32_t)launch.chunk_paths;
     a.items = ss.items;
     a.n_items = ss.n_items;
     a.chunks_per_solve = ss.chunks_per_solve;
@@ -1065,7 +1077,8 @@ static spac_status solve_points_impl(Problem* p, const double* d_x0, const doubl
     ss.sqrth_uniform = std::sqrt(h);
     ss.paths_uniform = paths;
     ss.total_paths = (int64_t)n * (int64_t)paths;
-    ss.chunks_per_solve = (int32_t)((paths + (uint32_t)p->launch.chunk_paths - 1) / (uint32_t)p->launch.chunk_paths);
+    ss.chunk = work_item_paths((uint32_t)p->launch.chunk_paths, ss.total_paths);
+    ss.chunks_per_solve = (int32_t)((paths + ss.chunk - 1) / ss.chunk);
     int64_t items = (int64_t)ss.chunks_per_solve * n;
     if (items > 0x7fffffff) return set_error(SPAC_E_INVALID_ARG, "too many work items (%lld)", (long long)items);
     ss.n_items = (int32_t)items;

@@ -217,20 +217,42 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
     const int quarter = lane >> 4, r = lane & 15;
     spac_ksum* my = s_acc + (size_t)(wave * 4 + quarter) * M;
     const int64_t off = a.rec_off[job];
+    /* the records of the next chunk are loaded while this one is processed (one
+     * round trip per chunk instead of two dependent ones on the critical path) */
+    int8_t n_out = 0;
+    double n_x = 0.0, n_y = 0.0, n_Y = 0.0;
+    if (tid < np) {
+        const int64_t g = off + tid;
+        n_out = a.outcome[g]; n_x = a.xt[g]; n_y = a.yt[g]; n_Y = a.Y[g];
+    }
     for (int64_t base = 0; base < np; base += kRowsBlock) {
         const int cnt = (np - base < (int64_t)kRowsBlock) ? (int)(np - base) : kRowsBlock;
+        const int8_t c_out = n_out;
+        const double c_x = n_x, c_y = n_y, c_Y = n_Y;
+        if (base + kRowsBlock + tid < np) {
+            const int64_t g = off + base + kRowsBlock + tid;
+            n_out = a.outcome[g]; n_x = a.xt[g]; n_y = a.yt[g]; n_Y = a.Y[g];
+        }
         /* phase 1, one thread per path: exit angle, stencil start, Gaussians */
         if (tid < cnt) {
-            const int64_t g = off + base + tid;
             int s = -1;
-            if (a.outcome[g] == SPAC_OUT_SUBDOMAIN) {
+            if (c_out == SPAC_OUT_SUBDOMAIN) {
                 double theta = 0.0;
-                spac_exit_angle(code, a.xt[g] - cx, a.yt[g] - cy, s_th[0], &theta);
+                spac_exit_angle(code, c_x - cx, c_y - cy, s_th[0], &theta);
                 double phi[kMaxStencil];
                 s = circle ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
                            : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
-                for (int i = 0; i < S; ++i) s_v[tid * wstride + i] = phi[i];
-                s_y[tid] = a.Y[g];
+                const double Yp = c_Y;
+                if (circle) { /* the one stencil inverse, broadcast from LDS: Y w_r straight away */
+                    for (int rr = 0; rr < S; ++rr) {
+                        double w = 0.0;
+                        for (int i = 0; i < S; ++i) w = __builtin_fma(s_blk[i * S + rr], phi[i], w);
+                        s_v[tid * wstride + rr] = Yp * w;
+                    }
+                } else {
+                    for (int i = 0; i < S; ++i) s_v[tid * wstride + i] = phi[i];
+                }
+                s_y[tid] = Yp;
             }
             s_s[tid] = s;
         }
@@ -244,6 +266,14 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
         const int p1 = min(cnt, wave * 64 + 64);
         const int rr = (r < S) ? r : 0;
         double yw[16];
+        if (circle) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int p = p0 + 4 * k;
+                const int pc = (p < p1) ? p : p0;
+                yw[k] = s_v[pc * wstride + rr];
+            }
+        } else
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int p = p0 + 4 * k;
