@@ -280,6 +280,11 @@ def _ttr_mesh():
     return mesh, h
 
 
+# GMRES relative-residual tolerance of the time-to-RMS solves: the reference's default
+# toleranciaGMRES (main_test_mpi.cpp:32).
+GMRES_TOL = 1e-8
+
+
 def _replicas(mesh, prob, paths, h, seeds, u0=None):
     """Independent assemblies (one per seed) at `paths` per row, each solved by GPU GMRES.
 
@@ -296,7 +301,7 @@ def _replicas(mesh, prob, paths, h, seeds, u0=None):
         t = prob.timing()
         mesh.accumulate(rows)
         Gi, Gj, Gv, B = mesh.csr()
-        u, info = gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=20000, restart=200, u0=u0, device=True)
+        u, info = gmres(Gi, Gj, Gv, B, tol=GMRES_TOL, max_iter=20000, restart=200, u0=u0, device=True)
         u0 = u
         us.append(u)
         mats.append((sp.csr_matrix((Gv, Gj, Gi), shape=(n, n)), B))
@@ -312,15 +317,21 @@ def _replicas(mesh, prob, paths, h, seeds, u0=None):
 
 
 def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n_rep=8, seed0=100, eager=False,
-              beta2=0.95):
+              beta2=0.95, data_paths=500, data_h=2e-3, max_refine=3, refine_rep=16):
     """Time to a statistical RMS error `target` of the solved interface values, learned control
     variate against plain Monte Carlo, both FP32 stepping on this GPU.
 
-    Statistical RMS at a path count: n_rep independent replicas (seeds), sample variance per
-    node, averaged over nodes.  A probe at probe_paths fixes the path count for the target
-    (RMS ~ N^-1/2, +5 %); the method then runs n_rep replicas at that count, which are timed and
-    re-measured.  The learned-CV time includes its training data (plain-MC point solves at 4096
-    points), training and tabulation; GMRES (GPU, FP64) is reported beside the Monte-Carlo time."""
+    Statistical RMS at a path count: independent replicas (seeds), sample variance per
+    node, averaged over nodes.  Path count: a probe at probe_paths (n_rep replicas), then up
+    to max_refine re-estimates N <- 1.05 N (RMS / target)^2, each from refine_rep fresh replicas
+    at the previous estimate, until the RMS lies in [0.9, 1] x target (at small N the noise of
+    the assembled matrix adds a higher-order term, so a single probe overestimates N).  The
+    node errors are dominated by a few smooth modes of the solve, so an RMS from R replicas
+    scatters by roughly +-sqrt(1 / R); the row-residual RMS (independent rows) does not.  The
+    same rule serves both methods; the replicas of the last count are the timed ones.  The learned-CV time
+    includes its training data (plain-MC point solves at 4096 points, data_paths paths each at
+    step data_h: the control variate needs grad u, not an unbiased u), training and tabulation;
+    GMRES (GPU, FP64, relative residual GMRES_TOL) is reported beside the Monte-Carlo time."""
     from spac_amd.mlp import MLP
     from spac_amd.problems import franke_u
 
@@ -328,18 +339,28 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
     dom = (0.0, 0.0, 1.0, 1.0)
     desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=dom, precision=abi.SPAC_FP32)
     exact = franke_u(mesh.node_x, mesh.node_y)
-    res = {"target_rms": target, "replicas": n_rep,
+    res = {"target_rms": target, "replicas": {"probe": n_rep, "refinement": refine_rep},
            "mesh": {"nodes": int(mesh.info.n_nodes), "rows": int(mesh.info.n_jobs),
                     "circles": int(mesh.info.n_interfaces), "h": h, "nnz": int(mesh.info.nnz)}}
 
     def measure(prob, name, extra):
-        probe = [seed0 + k for k in range(n_rep)]
-        _, rms0, row0, _ = _replicas(mesh, prob, probe_paths, h, probe)
-        n_star = int(np.ceil(1.05 * probe_paths * (rms0 / target) ** 2))
-        log(f"ttr {name}: rms {rms0:.3e} (row {row0:.3e}) at {probe_paths} paths -> {n_star} paths")
-        runs, rms, row, ubar = _replicas(mesh, prob, n_star, h, [seed0 + 1000 + k for k in range(n_rep)])
-        r = {"paths_per_row": n_star, "rms": rms, "row_residual_rms": row, "probe_rms": rms0,
-             "probe_row_residual_rms": row0, "mc_ms": float(np.mean([x["ms"] for x in runs])),
+        paths, hist = probe_paths, []
+        for it in range(max_refine + 1):
+            reps = n_rep if it == 0 else refine_rep
+            runs, rms, row, ubar = _replicas(mesh, prob, paths, h, [seed0 + 1000 * it + k for k in range(reps)])
+            hist.append({"paths_per_row": paths, "rms": rms, "row_residual_rms": row})
+            if it == 0:
+                rms0, row0 = rms, row
+            elif 0.9 * target <= rms <= target:
+                break
+            if it == max_refine:
+                break
+            nxt = int(np.ceil(1.05 * paths * (rms / target) ** 2))
+            log(f"ttr {name}: rms {rms:.3e} (row {row:.3e}) at {paths} paths -> {nxt} paths")
+            paths = nxt
+        r = {"paths_per_row": paths, "rms": rms, "reached_target": bool(rms <= target), "row_residual_rms": row,
+             "probe_rms": rms0, "probe_row_residual_rms": row0, "path_count_history": hist,
+             "mc_ms": float(np.mean([x["ms"] for x in runs])),
              "path_kernel_ms": float(np.mean([x["path_ms"] for x in runs])),
              "gmres_ms": float(np.mean([x["gmres_ms"] for x in runs])),
              "gmres_iterations": runs[0]["gmres_it"], "path_steps": runs[0]["steps"],
@@ -360,7 +381,7 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
     ys = np.clip(Y.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
     with FKProblem(problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=dom, precision=abi.SPAC_FP32)) as pp:
         pp.set_control_variate(abi.SPAC_CV_NONE)
-        st, _ = pp.solve_points(xs, ys, 2000, 5e-4, seed=1)
+        st, _ = pp.solve_points(xs, ys, data_paths, data_h, seed=1)
         data_ms = pp.timing().total_ms
     u_mc = np.array([x["phi"] for x in st])
     with MLP(dom, seed=0) as net, FKProblem(desc) as prob:
@@ -368,7 +389,9 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
                         beta2=beta2)
         table_ms = net.build_table(1024)
         net.attach(prob, fallback=True)
-        res["learned"] = measure(prob, "learned", {"training_data_ms": data_ms, "train_ms": rep["train_ms"],
+        res["learned"] = measure(prob, "learned", {"training_data_ms": data_ms, "training_data": {
+                                                       "points": int(xs.size), "paths": data_paths, "h": data_h},
+                                                   "train_ms": rep["train_ms"],
                                                    "train_steps": train_steps, "train_lr": list(lr), "train_beta2": beta2 or 0.999,
                                                    "train_loss": rep["final_loss"],
                                                    "table_ms": table_ms})
@@ -415,7 +438,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--train-lr", type=float, nargs=2, default=(2e-2, 1e-4), metavar=("LR0", "LR1"))
     ap.add_argument("--train-beta2", type=float, default=0.95, help="Adam beta2 (0: 0.999)")
-    ap.add_argument("--ttr-replicas", type=int, default=8)
+    ap.add_argument("--ttr-replicas", type=int, default=8, help="replicas of the probe")
+    ap.add_argument("--ttr-refine-replicas", type=int, default=16, help="replicas of each path-count refinement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run plain device copies of this many MiB (counter calibration for profiles)")
     ap.add_argument("--train-eager", action="store_true",
@@ -431,7 +455,8 @@ def main():
         print(json.dumps({"batched": bench_batched(steps=args.steps)}), flush=True)
         return
     if args.workload == "ttr":
-        print(json.dumps({"time_to_rms": bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, eager=args.train_eager,
+        print(json.dumps({"time_to_rms": bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, refine_rep=args.ttr_refine_replicas,
+                                     eager=args.train_eager,
                                      lr=tuple(args.train_lr), beta2=args.train_beta2)}), flush=True)
         return
     out = bench_rows(args) if args.workload == "rows" else bench_points(args)
@@ -440,7 +465,8 @@ def main():
         out["batched"] = bench_batched()
     if not args.no_ttr:
         log("time to RMS 1e-3 (config 5) ...")
-        out["time_to_rms"] = bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, eager=args.train_eager,
+        out["time_to_rms"] = bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, refine_rep=args.ttr_refine_replicas,
+                                     eager=args.train_eager,
                                      lr=tuple(args.train_lr), beta2=args.train_beta2)
     print(json.dumps(out), flush=True)
 

@@ -9,9 +9,11 @@
  * estimates and its two gradient heads learn d u_hat / d(x, y) (forward-mode
  * targets, held fixed per step), so one forward pass gives the field F.
  *
- * Training (FP32, deterministic: every sum is sequential in a fixed order):
- *   input scaling -> dense+tanh x3 with forward-mode tangents -> output + loss
- *   gradient -> back-propagation -> weight gradients -> Adam.
+ * Training (FP32, deterministic: every sum has a fixed order): one kernel per step
+ *   runs, for 16 samples per block, dense+tanh x3 with forward-mode tangents, the
+ *   heads and the loss gradient, back-propagation and the weight gradients, the
+ *   64 x 64 products on v_mfma_f32_16x16x4f32; a second kernel sums the per-block
+ *   gradients in order and takes the Adam step.
  * Inference (BF16 operands, FP32 accumulate): each wave evaluates 32 points;
  * hidden layers are Act^T = W H^T on v_mfma_f32_32x32x8bf16_1k with the points
  * on the N axis, so a layer's accumulator layout is the next layer's B operand
@@ -50,7 +52,6 @@ constexpr int kWs = 68;          /* LDS row stride of a transposed 64-wide weigh
                                     aligned, and rows fall on different banks (stride 64 puts a
                                     wave's transposing writes on one bank) */
 constexpr int kChunk = kTile;    /* samples per weight-gradient partial */
-constexpr int kPad = kTile + 1;  /* padded LDS row */
 constexpr int kOG = 256 / kTile; /* groups of 4 features per block (16 x 4 = 64) */
 
 /* z = 2 (x - lo) / (hi - lo) - 1, once per batch */
@@ -186,103 +187,27 @@ __global__ void __launch_bounds__(256) k_fwd_tile(FwdArgs a) {
     a.loss[s] = r0 * r0 + a.lam * (r1 * r1 + r2 * r2);
 }
 
-struct BpArgs {
-    const float* W;     /* this layer's weights [OUT][IN] */
-    const float* dn;    /* this layer's output deltas [OUT][n] */
-    const float* hprev; /* this layer's input activations [IN][n] (tanh outputs; z for layer 1) */
-    float* dout;        /* previous layer's deltas [IN][n], or nullptr */
-    float* part;        /* [chunk][kNParams] */
-    int offW, offb;
-    int n;
-};
+/* One whole training step for kTile samples in one block of 4 waves, on FP32 MFMA
+ * (v_mfma_f32_16x16x4f32: D[i][j] += sum_k A[i][k] B[k][j], 16 x 16 x 4; lane l supplies
+ * A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16], r = 0..3).
+ * Lane l of wave w works on sample c = l % 16 of the tile and lane group g = l / 16.
+ *
+ * Forward: every lane evaluates layer 1 (and its tangents) for features 16 g .. 16 g + 15 of
+ * its sample, which is the B operand of layer 2 with the K order k = 16 g + j at step j.
+ * Layers 2 and 3 are W [h | tx | ty] (M = 64, N = 3 x 16 samples, K = 64): wave w computes
+ * output features 16 w .. 16 w + 15, so its A operand W[16 w + c][16 g + j] is four float4
+ * loads per lane; the outputs go through LDS ([sample][feature]) to the next layer.
+ * Heads and dL/do: per-lane partial sums over the lane's four features, then summed over
+ * (wave, lane group) in order.  Backward: delta_{L-1} = (1 - h^2) W_L^T delta_L on the same
+ * tiling; weight gradients dW_L = delta_L h_{L-1}^T (M = 64, N = 64, K = 16 samples) with wave
+ * w on rows 16 w .. 16 w + 15.  Per-block gradient partials go to part[block][...] for the
+ * fixed-order reduction in k_reduce_adam.  Samples past n have z = 0 and dL/do = 0, so they
+ * add nothing. */
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kXs = 68; /* LDS row stride of a [sample][feature] tile: b128 accesses of the 16
+                           samples fall on distinct banks, and rows 4 samples apart start 16
+                           banks apart */
 
-/* Backward step of one layer for kTile samples: the previous layer's deltas
- *   dout[i][s] = (1 - hprev[i][s]^2) sum_o W[o][i] dn[o][s]
- * (thread t: inputs 4 (t / kTile) .. +3 of sample t % kTile) and this layer's weight / bias
- * gradient partials over the block's samples, part[block][offW + o IN + i] =
- * sum_s dn[o][s] hprev[i][s] (samples in order), part[block][offb + o] = sum_s dn[o][s]. */
-template <int OUT, int IN>
-__global__ void __launch_bounds__(256) k_bp_tile(BpArgs a) {
-    __shared__ float Ld[OUT * kPad];
-    __shared__ float Lx[IN * kPad];
-    __shared__ __align__(16) float Lw[(IN == 64) ? OUT * 64 : 4];
-    const int tid = threadIdx.x;
-    const int s0 = blockIdx.x * kTile, n = a.n;
-    for (int k = tid; k < OUT * kTile; k += 256) {
-        const int o = k / kTile, cc = k % kTile, gs = s0 + cc;
-        Ld[o * kPad + cc] = gs < n ? a.dn[(size_t)o * n + gs] : 0.0f;
-    }
-    for (int k = tid; k < IN * kTile; k += 256) {
-        const int i = k / kTile, cc = k % kTile, gs = s0 + cc;
-        Lx[i * kPad + cc] = gs < n ? a.hprev[(size_t)i * n + gs] : 0.0f;
-    }
-    if (IN == 64 && a.dout)
-        for (int k = tid; k < OUT * 64; k += 256) Lw[k] = a.W[k];
-    __syncthreads();
-    if (IN == 64 && a.dout) {
-        const int c = tid % kTile, i0 = 4 * (tid / kTile), s = s0 + c;
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int o = 0; o < OUT; ++o) {
-            const float dv = Ld[o * kPad + c];
-            const float4 w4 = *reinterpret_cast<const float4*>(Lw + o * 64 + i0);
-            acc[0] = fmaf(w4.x, dv, acc[0]);
-            acc[1] = fmaf(w4.y, dv, acc[1]);
-            acc[2] = fmaf(w4.z, dv, acc[2]);
-            acc[3] = fmaf(w4.w, dv, acc[3]);
-        }
-        if (s < n) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float hv = Lx[(i0 + q) * kPad + c];
-                a.dout[(size_t)(i0 + q) * n + s] = (1.0f - hv * hv) * acc[q];
-            }
-        }
-    }
-    float* out = a.part + (size_t)blockIdx.x * kNParams;
-    if (OUT == 64 && IN == 64) {
-        /* 4 x 4 register tiles: thread t owns o = 4 (t / 16) .. +3, i = 4 (t % 16) .. +3 */
-        const int o0 = 4 * (tid >> 4), i0 = 4 * (tid & 15);
-        float g[4][4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) g[p][q] = 0.0f;
-#pragma unroll 4
-        for (int cc = 0; cc < kTile; ++cc) {
-            float dv[4], xv[4];
-#pragma unroll
-            for (int p = 0; p < 4; ++p) dv[p] = Ld[(o0 + p) * kPad + cc];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) xv[q] = Lx[(i0 + q) * kPad + cc];
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) g[p][q] = fmaf(dv[p], xv[q], g[p][q]);
-        }
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) out[a.offW + (o0 + p) * IN + i0 + q] = g[p][q];
-    } else {
-        for (int k = tid; k < OUT * IN; k += 256) {
-            const int o = k / IN, i = k % IN;
-            float acc = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) acc = fmaf(Ld[o * kPad + cc], Lx[i * kPad + cc], acc);
-            out[a.offW + k] = acc;
-        }
-    }
-    for (int o = tid; o < OUT; o += 256) {
-        float acc = 0.0f;
-        for (int cc = 0; cc < kTile; ++cc) acc += Ld[o * kPad + cc];
-        out[a.offb + o] = acc;
-    }
-}
-
-/* One whole training step for kTile samples in one block, all activations in LDS: the
- * forward pass with both tangents, the heads and dL/do, the backward pass and the weight
- * and bias gradient partials of the block's samples (part[block][...], samples in order).
- * Thread t works on sample t % kTile and features / inputs 4 (t / kTile) .. +3; the only
- * cross-sample sums are the gradient partials, reduced afterwards by k_reduce_adam. */
 struct StepArgs {
     const float* P;   /* parameters */
     const float* z;   /* [2][n] scaled inputs */
@@ -293,91 +218,163 @@ struct StepArgs {
     float* loss;      /* [n] per-sample loss (reports) */
     int n;
 };
-__global__ void __launch_bounds__(256) k_train_step_tile(StepArgs a) {
-    __shared__ __align__(16) float Lw[64 * kWs];
-    __shared__ float Lz[2 * kPad];
-    __shared__ float Lh[3][64 * kPad];
-    __shared__ float Lt[2][2][64 * kPad]; /* [buffer][direction] tangents */
-    __shared__ float Ld[2][64 * kPad];    /* deltas, ping-pong */
-    __shared__ float LdO[3 * kPad];
-    __shared__ float Lred[kOG][5][kTile];
-    const int tid = threadIdx.x, c = tid % kTile, og = tid / kTile, f0 = 4 * og;
+
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ void load16(const float* p, float (&v)[16]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 t = *reinterpret_cast<const float4*>(p + 4 * q);
+        v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
+    }
+}
+__device__ __forceinline__ void store4(float* p, const float (&v)[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+/* tanh for training: 1 - 2 / (e^{2x} + 1) with the hardware 2^x and reciprocal (a few ulp),
+ * and for |x| < 1/4 the odd Taylor polynomial to x^9 (truncation below 2.2e-9), so that small
+ * arguments keep their relative precision. */
+__device__ __forceinline__ float train_tanh(float x) {
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);
+    const float big = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+    const float x2 = x * x;
+    const float p =
+        x * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 62.0f / 2835.0f, -17.0f / 315.0f), 2.0f / 15.0f), -1.0f / 3.0f), 1.0f);
+    return fabsf(x) < 0.25f ? p : big;
+}
+
+/* Weight gradient rows 16 w .. 16 w + 15 of a 64 x 64 layer over the tile's 16 samples:
+ * dW[o][i] = sum_s D[s][o] X[s][i] (D, X in [sample][feature] LDS tiles; K order s = 4 g + j). */
+__device__ __forceinline__ void wgrad64(const float* D, const float* X, float* out, int w, int g, int c) {
+    float av[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) av[j] = D[(4 * g + j) * kXs + 16 * w + c];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc = mfma16(av[j], X[(4 * g + j) * kXs + 16 * nt + c], acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[(16 * w + 4 * g + r) * 64 + 16 * nt + c] = acc[r];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
+    __shared__ __align__(16) float X2[3][kTile * kXs]; /* h2, t2x, t2y */
+    __shared__ __align__(16) float H1[kTile * kXs];
+    __shared__ __align__(16) float H3[kTile * kXs];
+    __shared__ __align__(16) float D3[kTile * kXs];
+    __shared__ __align__(16) float D2[kTile * kXs];
+    __shared__ __align__(16) float D1[kTile * kXs];
+    __shared__ float Lz[2][kTile];
+    __shared__ float Lred[4][4][5][kTile]; /* [wave][lane group][head term][sample] */
+    __shared__ float LdO[3][kTile];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, c = lane & 15;
     const int s0 = blockIdx.x * kTile, s = s0 + c, n = a.n;
     const float* P = a.P;
-    for (int k = tid; k < 2 * kTile; k += 256) {
-        const int i = k / kTile, cc = k % kTile, gs = s0 + cc;
-        Lz[i * kPad + cc] = gs < n ? a.z[(size_t)i * n + gs] : 0.0f;
+    const int fo = 16 * w + 4 * g; /* first of the lane's four output features */
+
+    /* weight operands: forward A = W[16 w + c][16 g + j], backward A = W[16 g + j][16 w + c] */
+    float aW2[16], aW3[16], aW2t[16], aW3t[16];
+    load16(P + oW2 + (16 * w + c) * 64 + 16 * g, aW2);
+    load16(P + oW3 + (16 * w + c) * 64 + 16 * g, aW3);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        aW2t[j] = P[oW2 + (16 * g + j) * 64 + 16 * w + c];
+        aW3t[j] = P[oW3 + (16 * g + j) * 64 + 16 * w + c];
     }
-    __syncthreads();
-    /* layer 1 */
+    const float z0 = s < n ? a.z[s] : 0.0f, z1 = s < n ? a.z[(size_t)n + s] : 0.0f;
+    if (tid < kTile) {
+        Lz[0][c] = z0;
+        Lz[1][c] = z1;
+    }
+
+    /* layer 1 for features 16 g + j of sample c */
+    float xh[16], xtx[16], xty[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int f = 16 * g + j;
+        const float w0 = P[oW1 + 2 * f], w1 = P[oW1 + 2 * f + 1];
+        const float h = train_tanh(fmaf(w1, z1, w0 * z0) + P[ob1 + f]);
+        const float d = 1.0f - h * h;
+        xh[j] = h;
+        xtx[j] = d * (w0 * a.scx);
+        xty[j] = d * (w1 * a.scy);
+    }
+    if (w == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float v[4] = {xh[4 * q], xh[4 * q + 1], xh[4 * q + 2], xh[4 * q + 3]};
+            store4(H1 + c * kXs + 16 * g + 4 * q, v);
+        }
+    }
+
+    /* layer 2 */
+    float h2r[4];
     {
-        float t0[4], t1[4];
+        f32x4 ah = {0.0f, 0.0f, 0.0f, 0.0f}, ax = ah, ay = ah;
 #pragma unroll
-        for (int o = 0; o < 4; ++o) {
-            const int f = f0 + o;
-            const float w0 = P[oW1 + 2 * f], w1 = P[oW1 + 2 * f + 1];
-            const float h = tanhf(fmaf(w1, Lz[kPad + c], w0 * Lz[c]) + P[ob1 + f]);
+        for (int j = 0; j < 16; ++j) {
+            ah = mfma16(aW2[j], xh[j], ah);
+            ax = mfma16(aW2[j], xtx[j], ax);
+            ay = mfma16(aW2[j], xty[j], ay);
+        }
+        float tx[4], ty[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float h = train_tanh(ah[r] + P[ob2 + fo + r]);
             const float d = 1.0f - h * h;
-            Lh[0][f * kPad + c] = h;
-            t0[o] = d * (w0 * a.scx);
-            t1[o] = d * (w1 * a.scy);
+            h2r[r] = h;
+            tx[r] = d * ax[r];
+            ty[r] = d * ay[r];
         }
-#pragma unroll
-        for (int o = 0; o < 4; ++o) {
-            Lt[0][0][(f0 + o) * kPad + c] = t0[o];
-            Lt[0][1][(f0 + o) * kPad + c] = t1[o];
-        }
+        store4(X2[0] + c * kXs + fo, h2r);
+        store4(X2[1] + c * kXs + fo, tx);
+        store4(X2[2] + c * kXs + fo, ty);
     }
-    /* layers 2 and 3: W^T in LDS, inputs from buffer L-1 */
-    float pg[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    for (int L = 1; L <= 2; ++L) {
-        const float* W = P + (L == 1 ? oW2 : oW3);
-        const float* b = P + (L == 1 ? ob2 : ob3);
-        __syncthreads();
-        for (int k = tid; k < 64 * 64; k += 256) Lw[(k & 63) * kWs + (k >> 6)] = W[k];
-        __syncthreads();
-        const float* hin = Lh[L - 1];
-        const float* txin = Lt[(L - 1) & 1][0];
-        const float* tyin = Lt[(L - 1) & 1][1];
-        float acc[4], ax[4], ay[4];
-#pragma unroll
-        for (int o = 0; o < 4; ++o) acc[o] = ax[o] = ay[o] = 0.0f;
-#pragma unroll 4
-        for (int i = 0; i < 64; ++i) {
-            const float hv = hin[i * kPad + c], tx = txin[i * kPad + c], ty = tyin[i * kPad + c];
-            const float4 w4 = *reinterpret_cast<const float4*>(Lw + i * kWs + f0);
-            const float w[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-            for (int o = 0; o < 4; ++o) {
-                acc[o] = fmaf(w[o], hv, acc[o]);
-                ax[o] = fmaf(w[o], tx, ax[o]);
-                ay[o] = fmaf(w[o], ty, ay[o]);
-            }
-        }
-#pragma unroll
-        for (int o = 0; o < 4; ++o) {
-            const int f = f0 + o;
-            const float h = tanhf(acc[o] + b[f]);
-            const float d = 1.0f - h * h;
-            Lh[L][f * kPad + c] = h;
-            if (L == 1) {
-                Lt[1][0][f * kPad + c] = d * ax[o];
-                Lt[1][1][f * kPad + c] = d * ay[o];
-            } else {
-                for (int q = 0; q < 3; ++q) pg[q] = fmaf(P[oW4 + q * 64 + f], h, pg[q]);
-                pg[3] = fmaf(P[oW4 + f], d * ax[o], pg[3]);
-                pg[4] = fmaf(P[oW4 + f], d * ay[o], pg[4]);
-            }
-        }
-    }
-    for (int k = 0; k < 5; ++k) Lred[og][k][c] = pg[k];
     __syncthreads();
+
+    /* layer 3 and the heads' partial sums over the lane's features */
+    float h3r[4];
+    {
+        load16(X2[0] + c * kXs + 16 * g, xh);
+        load16(X2[1] + c * kXs + 16 * g, xtx);
+        load16(X2[2] + c * kXs + 16 * g, xty);
+        f32x4 ah = {0.0f, 0.0f, 0.0f, 0.0f}, ax = ah, ay = ah;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            ah = mfma16(aW3[j], xh[j], ah);
+            ax = mfma16(aW3[j], xtx[j], ax);
+            ay = mfma16(aW3[j], xty[j], ay);
+        }
+        float pg[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int f = fo + r;
+            const float h = train_tanh(ah[r] + P[ob3 + f]);
+            const float d = 1.0f - h * h;
+            h3r[r] = h;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) pg[q] = fmaf(P[oW4 + q * 64 + f], h, pg[q]);
+            pg[3] = fmaf(P[oW4 + f], d * ax[r], pg[3]);
+            pg[4] = fmaf(P[oW4 + f], d * ay[r], pg[4]);
+        }
+        store4(H3 + c * kXs + fo, h3r);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) Lred[w][g][k][c] = pg[k];
+    }
+    __syncthreads();
+
     /* heads, residuals, dL/do */
-    if (og == 0) {
+    if (tid < kTile) {
         float v[5];
+#pragma unroll
         for (int k = 0; k < 5; ++k) {
             float t = 0.0f;
-            for (int g = 0; g < kOG; ++g) t += Lred[g][k][c];
+            for (int ww = 0; ww < 4; ++ww)
+                for (int gg = 0; gg < 4; ++gg) t += Lred[ww][gg][k][c];
             v[k] = t;
         }
         float dO0 = 0.0f, dO1 = 0.0f, dO2 = 0.0f;
@@ -390,101 +387,88 @@ __global__ void __launch_bounds__(256) k_train_step_tile(StepArgs a) {
             dO2 = 2.0f * a.lam * r2 * inv;
             a.loss[s] = r0 * r0 + a.lam * (r1 * r1 + r2 * r2);
         }
-        LdO[c] = dO0;
-        LdO[kPad + c] = dO1;
-        LdO[2 * kPad + c] = dO2;
+        LdO[0][c] = dO0;
+        LdO[1][c] = dO1;
+        LdO[2][c] = dO2;
     }
     __syncthreads();
+
     float* out = a.part + (size_t)blockIdx.x * kNParams;
-    /* d3 = (1 - h3^2) W4^T dO; gradient of W4, b4 */
+    /* delta3 = (1 - h3^2) W4^T dO for the lane's features; dW4, db4 */
     {
+        float d3[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = f0 + q;
+        for (int r = 0; r < 4; ++r) {
             float t = 0.0f;
-            for (int o = 0; o < 3; ++o) t = fmaf(P[oW4 + o * 64 + i], LdO[o * kPad + c], t);
-            const float hv = Lh[2][i * kPad + c];
-            Ld[0][i * kPad + c] = (1.0f - hv * hv) * t;
+            for (int q = 0; q < 3; ++q) t = fmaf(P[oW4 + q * 64 + fo + r], LdO[q][c], t);
+            d3[r] = (1.0f - h3r[r] * h3r[r]) * t;
         }
-        for (int k = tid; k < 3 * 64; k += 256) {
-            const int o = k / 64, i = k % 64;
-            float g = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) g = fmaf(LdO[o * kPad + cc], Lh[2][i * kPad + cc], g);
-            out[oW4 + k] = g;
-        }
-        if (tid < 3) {
-            float g = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) g += LdO[tid * kPad + cc];
-            out[ob4 + tid] = g;
-        }
-    }
-    /* layers 3 and 2: previous deltas and this layer's weight gradient (4 x 4 tiles) */
-    for (int L = 2; L >= 1; --L) {
-        const float* W = P + (L == 2 ? oW3 : oW2);
-        const int offW = (L == 2 ? oW3 : oW2), offb = (L == 2 ? ob3 : ob2);
-        const float* dn = Ld[L & 1 ? 1 : 0];   /* L = 2: Ld[0], L = 1: Ld[1] */
-        float* dprev = Ld[L & 1 ? 0 : 1];
-        const float* xin = Lh[L - 1];
-        __syncthreads();
-        for (int k = tid; k < 64 * 64; k += 256) Lw[k] = W[k];
-        __syncthreads();
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int o = 0; o < 64; ++o) {
-            const float dv = dn[o * kPad + c];
-            const float4 w4 = *reinterpret_cast<const float4*>(Lw + o * 64 + f0);
-            acc[0] = fmaf(w4.x, dv, acc[0]);
-            acc[1] = fmaf(w4.y, dv, acc[1]);
-            acc[2] = fmaf(w4.z, dv, acc[2]);
-            acc[3] = fmaf(w4.w, dv, acc[3]);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float hv = xin[(f0 + q) * kPad + c];
-            dprev[(f0 + q) * kPad + c] = (1.0f - hv * hv) * acc[q];
-        }
-        const int o0 = 4 * (tid >> 4), i0 = 4 * (tid & 15);
-        float g[4][4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) g[p][q] = 0.0f;
-#pragma unroll 4
-        for (int cc = 0; cc < kTile; ++cc) {
-            float dv[4], xv[4];
-#pragma unroll
-            for (int p = 0; p < 4; ++p) dv[p] = dn[(o0 + p) * kPad + cc];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) xv[q] = xin[(i0 + q) * kPad + cc];
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) g[p][q] = fmaf(dv[p], xv[q], g[p][q]);
-        }
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) out[offW + (o0 + p) * 64 + i0 + q] = g[p][q];
-        if (tid < 64) {
-            float gb = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) gb += dn[tid * kPad + cc];
-            out[offb + tid] = gb;
+        store4(D3 + c * kXs + fo, d3);
+        if (tid < 3 * 64) {
+            const int q = tid >> 6, i = tid & 63;
+            float gs = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) gs = fmaf(LdO[q][cc], H3[cc * kXs + i], gs);
+            out[oW4 + tid] = gs;
+        } else if (tid < 3 * 64 + 3) {
+            const int q = tid - 3 * 64;
+            float gs = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) gs += LdO[q][cc];
+            out[ob4 + q] = gs;
         }
     }
     __syncthreads();
-    /* layer 1: gradient of W1, b1 from d1 (in Ld[0]) and z */
+
+    /* delta2 = (1 - h2^2) W3^T delta3; dW3 = delta3 h2^T, db3 */
     {
-        const float* d1 = Ld[0];
-        if (tid < 128) {
-            const int o = tid >> 1, i = tid & 1;
-            float g = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) g = fmaf(d1[o * kPad + cc], Lz[i * kPad + cc], g);
-            out[oW1 + tid] = g;
-        } else if (tid < 192) {
-            const int o = tid - 128;
-            float g = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) g += d1[o * kPad + cc];
-            out[ob1 + o] = g;
+        load16(D3 + c * kXs + 16 * g, xh);
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc = mfma16(aW3t[j], xh[j], acc);
+        float d2[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) d2[r] = (1.0f - h2r[r] * h2r[r]) * acc[r];
+        store4(D2 + c * kXs + fo, d2);
+        wgrad64(D3, X2[0], out + oW3, w, g, c);
+        if (tid < 64) {
+            float gs = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) gs += D3[cc * kXs + tid];
+            out[ob3 + tid] = gs;
         }
+    }
+    __syncthreads();
+
+    /* delta1 = (1 - h1^2) W2^T delta2; dW2 = delta2 h1^T, db2 */
+    {
+        load16(D2 + c * kXs + 16 * g, xh);
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc = mfma16(aW2t[j], xh[j], acc);
+        const float4 h1 = *reinterpret_cast<const float4*>(H1 + c * kXs + fo);
+        const float hv[4] = {h1.x, h1.y, h1.z, h1.w};
+        float d1[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) d1[r] = (1.0f - hv[r] * hv[r]) * acc[r];
+        store4(D1 + c * kXs + fo, d1);
+        wgrad64(D2, H1, out + oW2, w, g, c);
+        if (tid < 64) {
+            float gs = 0.0f;
+            for (int cc = 0; cc < kTile; ++cc) gs += D2[cc * kXs + tid];
+            out[ob2 + tid] = gs;
+        }
+    }
+    __syncthreads();
+
+    /* dW1 = delta1 z^T, db1 */
+    if (tid < 128) {
+        const int o = tid >> 1, i = tid & 1;
+        float gs = 0.0f;
+        for (int cc = 0; cc < kTile; ++cc) gs = fmaf(D1[cc * kXs + o], Lz[i][cc], gs);
+        out[oW1 + tid] = gs;
+    } else if (tid < 192) {
+        const int o = tid - 128;
+        float gs = 0.0f;
+        for (int cc = 0; cc < kTile; ++cc) gs += D1[cc * kXs + o];
+        out[ob1 + o] = gs;
     }
 }
 
@@ -492,8 +476,11 @@ __global__ void __launch_bounds__(256) k_train_step_tile(StepArgs a) {
  * groups of chunks, each summed in index order, then the groups in order);
  * then, in training, one Adam step with a geometric learning-rate decay.  The
  * step counter lives in device memory so that a captured graph of steps
- * replays unchanged: every block reads it, and the last block to finish
- * advances it. */
+ * replays unchanged.  It is double-buffered by step parity: a step of parity p
+ * reads ctr[p] (the steps taken so far) in every block, and block 0 writes
+ * ctr[1 - p] = ctr[p] + 1 for the next step, which nothing reads during this
+ * launch.  Captured graphs hold an even number of steps, so a step's parity is
+ * fixed at capture. */
 constexpr int kRedParams = 32, kRedGroups = 32;
 struct AdamArgs {
     float* p;
@@ -502,7 +489,8 @@ struct AdamArgs {
     float* v;
     const float* part;
     int chunks;
-    int* ctr;           /* [0]: Adam steps taken so far; [1]: blocks finished in this step */
+    int* ctr;           /* [2]: Adam steps taken so far, by step parity */
+    int par;            /* this step's parity */
     int t0;             /* counter value at the first step of this schedule */
     int steps;
     float lr0, lr1;     /* lr1 <= 0: constant lr0 */
@@ -527,7 +515,7 @@ __global__ void __launch_bounds__(kRedParams * kRedGroups) k_reduce_adam(AdamArg
         a.g[k] = g;
         if (a.adam) {
             const float b1 = 0.9f, b2 = a.beta2, eps = 1e-8f;
-            const int tp = a.ctr[0];
+            const int tp = a.ctr[a.par];
             const float t = (float)(tp + 1);
             float lr = a.lr0;
             if (a.lr1 > 0.0f && a.steps > 1)
@@ -540,18 +528,7 @@ __global__ void __launch_bounds__(kRedParams * kRedGroups) k_reduce_adam(AdamArg
             a.p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
         }
     }
-    if (!a.adam) return;
-    /* every block has read ctr[0] before it counts itself finished */
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const int done = atomicAdd(&a.ctr[1], 1);
-        if (done == (int)gridDim.x - 1) {
-            a.ctr[0] = a.ctr[0] + 1;
-            a.ctr[1] = 0;
-            __threadfence();
-        }
-    }
+    if (a.adam && blockIdx.x == 0 && threadIdx.x == 0) a.ctr[1 - a.par] = a.ctr[a.par] + 1;
 }
 
 
@@ -740,8 +717,8 @@ static void enqueue_forward(Mlp* m, bool with_targets, float lam) {
 }
 
 /* One full step: forward, backward with per-chunk gradient partials, the fixed-order
- * reduction and (adam) the update. */
-static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float lr0, float lr1) {
+ * reduction and (adam) the update; `par` is the parity of the step's index. */
+static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float lr0, float lr1, int par) {
     const int n = m->n;
     const size_t N = (size_t)n;
     hipStream_t s = m->stream;
@@ -756,7 +733,7 @@ static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float 
     st.part = (float*)m->part.ptr;
     st.loss = v.loss;
     st.n = n;
-    hipLaunchKernelGGL(k_train_step_tile, dim3(n_chunks(n)), dim3(256), 0, s, st);
+    hipLaunchKernelGGL(k_train_step_mfma, dim3(n_chunks(n)), dim3(256), 0, s, st);
     AdamArgs a;
     a.p = (float*)m->params.ptr;
     a.g = (float*)m->grads.ptr;
@@ -765,6 +742,7 @@ static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float 
     a.part = (const float*)m->part.ptr;
     a.chunks = n_chunks(n);
     a.ctr = (int*)m->ctr.ptr;
+    a.par = par & 1;
     a.t0 = t0;
     a.steps = steps;
     a.lr0 = lr0;
@@ -819,7 +797,7 @@ static spac_status capture_steps(Mlp* m, int k, float lam, int t0, int steps, fl
                                  hipGraphExec_t* exec) {
     hipGraph_t g = nullptr;
     MLP_HIP_CHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < k; ++i) enqueue_step(m, lam, true, t0, steps, lr0, lr1);
+    for (int i = 0; i < k; ++i) enqueue_step(m, lam, true, t0, steps, lr0, lr1, t0 + i);
     hipError_t e = hipStreamEndCapture(m->stream, &g);
     if (e != hipSuccess) return set_error(SPAC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
@@ -910,7 +888,7 @@ spac_status spac_mlp_loss_grad(spac_mlp* hm, const double* x, const double* y, c
     if (!m || !x || !y || !u || n < 1) return set_error(SPAC_E_INVALID_ARG, "bad arguments");
     spac_status rc;
     if ((rc = mlp_prepare_batch(m, x, y, u, n))) return rc;
-    if (grad) enqueue_step(m, (float)lam, false, 0, 1, 0.0f, 0.0f);
+    if (grad) enqueue_step(m, (float)lam, false, 0, 1, 0.0f, 0.0f, 0);
     else enqueue_forward(m, true, (float)lam);
     MLP_HIP_CHECK(hipGetLastError());
     if (loss && (rc = mlp_loss_host(m, loss))) return rc;
@@ -932,8 +910,10 @@ spac_status spac_mlp_train(spac_mlp* hm, const double* x, const double* y, const
     const float lam = (float)cfg->grad_weight, lr0 = (float)cfg->lr;
     const float lr1 = cfg->lr_final > 0.0 ? (float)cfg->lr_final : 0.0f;
     const int t0 = m->adam_t;
-    /* graphs of kPerGraph steps, plus one for the remainder */
-    const int kPerGraph = 50;
+    /* graphs of kPerGraph steps, plus one for the remainder; kPerGraph is even, so every
+       replay of a graph starts at the parity it was captured with */
+    constexpr int kPerGraph = 50;
+    static_assert(kPerGraph % 2 == 0, "graph length must be even (step-counter parity)");
     const int full = cfg->steps / kPerGraph, rest = cfg->steps % kPerGraph;
     hipGraphExec_t gfull = nullptr, grest = nullptr;
     if (!cfg->eager) {
@@ -948,7 +928,7 @@ spac_status spac_mlp_train(spac_mlp* hm, const double* x, const double* y, const
     MLP_HIP_CHECK(hipEventCreate(&e1));
     MLP_HIP_CHECK(hipEventRecord(e0, m->stream));
     if (cfg->eager) {
-        for (int i = 0; i < cfg->steps; ++i) enqueue_step(m, lam, true, t0, cfg->steps, lr0, lr1);
+        for (int i = 0; i < cfg->steps; ++i) enqueue_step(m, lam, true, t0, cfg->steps, lr0, lr1, t0 + i);
         MLP_HIP_CHECK(hipGetLastError());
     } else {
         for (int i = 0; i < full; ++i) MLP_HIP_CHECK(hipGraphLaunch(gfull, m->stream));
