@@ -51,7 +51,7 @@ constexpr int kTile = 16;        /* samples per block */
 constexpr int kWs = 68;          /* LDS row stride of a transposed 64-wide weight matrix: float4
                                     aligned, and rows fall on different banks (stride 64 puts a
                                     wave's transposing writes on one bank) */
-constexpr int kChunk = kTile;    /* samples per weight-gradient partial */
+constexpr int kChunk = kTile;    /* samples per block of the forward kernels */
 constexpr int kOG = 256 / kTile; /* groups of 4 features per block (16 x 4 = 64) */
 
 /* z = 2 (x - lo) / (hi - lo) - 1, once per batch */
@@ -187,10 +187,11 @@ __global__ void __launch_bounds__(256) k_fwd_tile(FwdArgs a) {
     a.loss[s] = r0 * r0 + a.lam * (r1 * r1 + r2 * r2);
 }
 
-/* One whole training step for kTile samples in one block of 4 waves, on FP32 MFMA
+/* One whole training step for kStepTiles tiles of kTile samples in one block, on FP32 MFMA
  * (v_mfma_f32_16x16x4f32: D[i][j] += sum_k A[i][k] B[k][j], 16 x 16 x 4; lane l supplies
  * A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16], r = 0..3).
- * Lane l of wave w works on sample c = l % 16 of the tile and lane group g = l / 16.
+ * Four waves per tile: lane l of wave w of a tile works on sample c = l % 16 of the tile and
+ * lane group g = l / 16.
  *
  * Forward: every lane evaluates layer 1 (and its tangents) for features 16 g .. 16 g + 15 of
  * its sample, which is the B operand of layer 2 with the K order k = 16 g + j at step j.
@@ -199,14 +200,19 @@ __global__ void __launch_bounds__(256) k_fwd_tile(FwdArgs a) {
  * loads per lane; the outputs go through LDS ([sample][feature]) to the next layer.
  * Heads and dL/do: per-lane partial sums over the lane's four features, then summed over
  * (wave, lane group) in order.  Backward: delta_{L-1} = (1 - h^2) W_L^T delta_L on the same
- * tiling; weight gradients dW_L = delta_L h_{L-1}^T (M = 64, N = 64, K = 16 samples) with wave
- * w on rows 16 w .. 16 w + 15.  Per-block gradient partials go to part[block][...] for the
- * fixed-order reduction in k_reduce_adam.  Samples past n have z = 0 and dL/do = 0, so they
- * add nothing. */
+ * tiling.  Weight gradients dW_L = delta_L h_{L-1}^T over all the block's samples (M = 64,
+ * N = 64, K = 16 kStepTiles): wave v of the block takes rows 16 (v % 4) .. +15 and
+ * 4 / kStepTiles column tiles.  The block's gradient sums go to part[block][...] for the
+ * fixed-order reduction in k_reduce_adam, so a step writes and reads n / (16 kStepTiles)
+ * gradient rows.  Samples past n have z = 0 and dL/do = 0, so they add nothing. */
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int kXs = 68; /* LDS row stride of a [sample][feature] tile: b128 accesses of the 16
-                           samples fall on distinct banks, and rows 4 samples apart start 16
-                           banks apart */
+constexpr int kStepTiles = 1;
+constexpr int kStepSamples = kTile * kStepTiles;
+constexpr int kStepThreads = 256 * kStepTiles;
+static_assert(4 % kStepTiles == 0, "column tiles per wave");
+constexpr int kXs = 68; /* LDS row stride of a [sample][feature] tile: b128 accesses of 16
+                           consecutive samples fall on distinct banks, and rows 4 samples apart
+                           start 16 banks apart */
 
 struct StepArgs {
     const float* P;   /* parameters */
@@ -214,7 +220,7 @@ struct StepArgs {
     const double* u;  /* targets */
     float lam;
     float scx, scy;
-    float* part;      /* [chunk][kNParams] */
+    float* part;      /* [block][kNParams] */
     float* loss;      /* [n] per-sample loss (reports) */
     int n;
 };
@@ -222,7 +228,7 @@ struct StepArgs {
 __device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ void load16(const float* p, float (&v)[16]) {
+__device__ __forceinline__ void load16(const float* p, float* v) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const float4 t = *reinterpret_cast<const float4*>(p + 4 * q);
@@ -245,59 +251,82 @@ __device__ __forceinline__ float train_tanh(float x) {
     return fabsf(x) < 0.25f ? p : big;
 }
 
-/* Weight gradient rows 16 w .. 16 w + 15 of a 64 x 64 layer over the tile's 16 samples:
- * dW[o][i] = sum_s D[s][o] X[s][i] (D, X in [sample][feature] LDS tiles; K order s = 4 g + j). */
-__device__ __forceinline__ void wgrad64(const float* D, const float* X, float* out, int w, int g, int c) {
-    float av[4];
+/* Weight gradient of a 64 x 64 layer over the block's samples, rows 16 (v % 4) .. +15 and
+ * column tiles (4 / kStepTiles) (v / 4) .. : dW[o][i] = sum_s D[s][o] X[s][i] (D, X are
+ * [sample][feature] LDS tiles; K order: step j, lane group g -> sample 16 (j / 4) + 4 g + j % 4). */
+__device__ __forceinline__ void wgrad64(const float* D, const float* X, float* out, int v, int g, int c) {
+    constexpr int kSteps = 4 * kStepTiles, kCols = 4 / kStepTiles;
+    const int o0 = 16 * (v & 3);
+    float av[kSteps];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) av[j] = D[(4 * g + j) * kXs + 16 * w + c];
+    for (int j = 0; j < kSteps; ++j) av[j] = D[(16 * (j >> 2) + 4 * g + (j & 3)) * kXs + o0 + c];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
+    for (int q = 0; q < kCols; ++q) {
+        const int nt = kCols * (v >> 2) + q;
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc = mfma16(av[j], X[(4 * g + j) * kXs + 16 * nt + c], acc);
+        for (int j = 0; j < kSteps; ++j)
+            acc = mfma16(av[j], X[(16 * (j >> 2) + 4 * g + (j & 3)) * kXs + 16 * nt + c], acc);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) out[(16 * w + 4 * g + r) * 64 + 16 * nt + c] = acc[r];
+        for (int r = 0; r < 4; ++r) out[(o0 + 4 * g + r) * 64 + 16 * nt + c] = acc[r];
     }
 }
 
-__global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
-    __shared__ __align__(16) float X2[3][kTile * kXs]; /* h2, t2x, t2y */
-    __shared__ __align__(16) float H1[kTile * kXs];
-    __shared__ __align__(16) float H3[kTile * kXs];
-    __shared__ __align__(16) float D3[kTile * kXs];
-    __shared__ __align__(16) float D2[kTile * kXs];
-    __shared__ __align__(16) float D1[kTile * kXs];
-    __shared__ float Lz[2][kTile];
-    __shared__ float Lred[4][4][5][kTile]; /* [wave][lane group][head term][sample] */
-    __shared__ float LdO[3][kTile];
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, c = lane & 15;
-    const int s0 = blockIdx.x * kTile, s = s0 + c, n = a.n;
+__global__ void __launch_bounds__(kStepThreads) k_train_step_mfma(StepArgs a) {
+    /* [sample][feature] tiles of all the block's samples.  X2 holds h2, t2x, t2y; after layer 3
+     * has read them, X2[1] takes delta3 and X2[2] delta2, and H3 takes delta1 once dW4 is done. */
+    __shared__ __align__(16) float X2[3][kStepSamples * kXs];
+    __shared__ __align__(16) float H1[kStepSamples * kXs];
+    __shared__ __align__(16) float H3[kStepSamples * kXs];
+    __shared__ float Lz[2][kStepSamples];
+    __shared__ float Lred[kStepTiles][4][4][5][kTile]; /* [tile][wave][lane group][head term][sample] */
+    __shared__ float LdO[3][kStepSamples];
+    float* const D3 = X2[1];
+    float* const D2 = X2[2];
+    float* const D1 = H3;
+    const int tid = threadIdx.x, v = tid >> 6, tt = v >> 2, w = v & 3, lane = tid & 63, g = lane >> 4,
+              c = lane & 15;
+    const int sc = kTile * tt + c; /* the lane's sample in the block */
+    const int s = blockIdx.x * kStepSamples + sc, n = a.n;
     const float* P = a.P;
     const int fo = 16 * w + 4 * g; /* first of the lane's four output features */
 
-    /* weight operands: forward A = W[16 w + c][16 g + j], backward A = W[16 g + j][16 w + c] */
+    /* Every global operand of the step is loaded here, in the order of first use, so that
+     * no later phase waits on memory: the lane's inputs, layer 1's weights and biases for
+     * features 16 g .. 16 g + 15, the MFMA A operands (forward W[16 w + c][16 g + j],
+     * backward W[16 g + j][16 w + c]), the biases and head weights of the lane's four output
+     * features, and the target of the sample whose residual this thread forms. */
+    const float z0 = s < n ? a.z[s] : 0.0f, z1 = s < n ? a.z[(size_t)n + s] : 0.0f;
+    float w1v[32], b1v[16];
+    load16(P + oW1 + 32 * g, w1v);
+    load16(P + oW1 + 32 * g + 16, w1v + 16);
+    load16(P + ob1 + 16 * g, b1v);
     float aW2[16], aW3[16], aW2t[16], aW3t[16];
     load16(P + oW2 + (16 * w + c) * 64 + 16 * g, aW2);
     load16(P + oW3 + (16 * w + c) * 64 + 16 * g, aW3);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        aW2t[j] = P[oW2 + (16 * g + j) * 64 + 16 * w + c];
-        aW3t[j] = P[oW3 + (16 * g + j) * 64 + 16 * w + c];
-    }
-    const float z0 = s < n ? a.z[s] : 0.0f, z1 = s < n ? a.z[(size_t)n + s] : 0.0f;
-    if (tid < kTile) {
-        Lz[0][c] = z0;
-        Lz[1][c] = z1;
-    }
+    for (int j = 0; j < 16; ++j) aW3t[j] = P[oW3 + (16 * g + j) * 64 + 16 * w + c];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) aW2t[j] = P[oW2 + (16 * g + j) * 64 + 16 * w + c];
+    const float4 b2v = *reinterpret_cast<const float4*>(P + ob2 + fo);
+    const float4 b3v = *reinterpret_cast<const float4*>(P + ob3 + fo);
+    float4 w4v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) w4v[q] = *reinterpret_cast<const float4*>(P + oW4 + q * 64 + fo);
+    const int sh = blockIdx.x * kStepSamples + tid; /* sample of the residual (tid < kStepSamples) */
+    const float b40 = P[ob4], b41 = P[ob4 + 1], b42 = P[ob4 + 2];
+    const float ut = (tid < kStepSamples && sh < n) ? (float)a.u[sh] : 0.0f;
+    const float b2a[4] = {b2v.x, b2v.y, b2v.z, b2v.w}, b3a[4] = {b3v.x, b3v.y, b3v.z, b3v.w};
+    const float w4a[3][4] = {{w4v[0].x, w4v[0].y, w4v[0].z, w4v[0].w},
+                             {w4v[1].x, w4v[1].y, w4v[1].z, w4v[1].w},
+                             {w4v[2].x, w4v[2].y, w4v[2].z, w4v[2].w}};
 
-    /* layer 1 for features 16 g + j of sample c */
+    /* layer 1 for features 16 g + j of the lane's sample */
     float xh[16], xtx[16], xty[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const int f = 16 * g + j;
-        const float w0 = P[oW1 + 2 * f], w1 = P[oW1 + 2 * f + 1];
-        const float h = train_tanh(fmaf(w1, z1, w0 * z0) + P[ob1 + f]);
+        const float w0 = w1v[2 * j], w1 = w1v[2 * j + 1];
+        const float h = train_tanh(fmaf(w1, z1, w0 * z0) + b1v[j]);
         const float d = 1.0f - h * h;
         xh[j] = h;
         xtx[j] = d * (w0 * a.scx);
@@ -306,8 +335,12 @@ __global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
     if (w == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float v[4] = {xh[4 * q], xh[4 * q + 1], xh[4 * q + 2], xh[4 * q + 3]};
-            store4(H1 + c * kXs + 16 * g + 4 * q, v);
+            const float hv[4] = {xh[4 * q], xh[4 * q + 1], xh[4 * q + 2], xh[4 * q + 3]};
+            store4(H1 + sc * kXs + 16 * g + 4 * q, hv);
+        }
+        if (g == 0) {
+            Lz[0][sc] = z0;
+            Lz[1][sc] = z1;
         }
     }
 
@@ -324,24 +357,24 @@ __global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
         float tx[4], ty[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float h = train_tanh(ah[r] + P[ob2 + fo + r]);
+            const float h = train_tanh(ah[r] + b2a[r]);
             const float d = 1.0f - h * h;
             h2r[r] = h;
             tx[r] = d * ax[r];
             ty[r] = d * ay[r];
         }
-        store4(X2[0] + c * kXs + fo, h2r);
-        store4(X2[1] + c * kXs + fo, tx);
-        store4(X2[2] + c * kXs + fo, ty);
+        store4(X2[0] + sc * kXs + fo, h2r);
+        store4(X2[1] + sc * kXs + fo, tx);
+        store4(X2[2] + sc * kXs + fo, ty);
     }
     __syncthreads();
 
     /* layer 3 and the heads' partial sums over the lane's features */
     float h3r[4];
     {
-        load16(X2[0] + c * kXs + 16 * g, xh);
-        load16(X2[1] + c * kXs + 16 * g, xtx);
-        load16(X2[2] + c * kXs + 16 * g, xty);
+        load16(X2[0] + sc * kXs + 16 * g, xh);
+        load16(X2[1] + sc * kXs + 16 * g, xtx);
+        load16(X2[2] + sc * kXs + 16 * g, xty);
         f32x4 ah = {0.0f, 0.0f, 0.0f, 0.0f}, ax = ah, ay = ah;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
@@ -353,43 +386,44 @@ __global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int f = fo + r;
-            const float h = train_tanh(ah[r] + P[ob3 + f]);
+            const float h = train_tanh(ah[r] + b3a[r]);
             const float d = 1.0f - h * h;
             h3r[r] = h;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) pg[q] = fmaf(P[oW4 + q * 64 + f], h, pg[q]);
-            pg[3] = fmaf(P[oW4 + f], d * ax[r], pg[3]);
-            pg[4] = fmaf(P[oW4 + f], d * ay[r], pg[4]);
+            for (int q = 0; q < 3; ++q) pg[q] = fmaf(w4a[q][r], h, pg[q]);
+            pg[3] = fmaf(w4a[0][r], d * ax[r], pg[3]);
+            pg[4] = fmaf(w4a[0][r], d * ay[r], pg[4]);
         }
-        store4(H3 + c * kXs + fo, h3r);
+        store4(H3 + sc * kXs + fo, h3r);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) Lred[w][g][k][c] = pg[k];
+        for (int k = 0; k < 5; ++k) Lred[tt][w][g][k][c] = pg[k];
     }
     __syncthreads();
 
     /* heads, residuals, dL/do */
-    if (tid < kTile) {
-        float v[5];
+    if (tid < kStepSamples) {
+        const int t2 = tid / kTile, c2 = tid % kTile;
+        float hv[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             float t = 0.0f;
             for (int ww = 0; ww < 4; ++ww)
-                for (int gg = 0; gg < 4; ++gg) t += Lred[ww][gg][k][c];
-            v[k] = t;
+                for (int gg = 0; gg < 4; ++gg) t += Lred[t2][ww][gg][k][c2];
+            hv[k] = t;
         }
         float dO0 = 0.0f, dO1 = 0.0f, dO2 = 0.0f;
-        if (s < n) {
-            const float o0 = v[0] + P[ob4], o1 = v[1] + P[ob4 + 1], o2 = v[2] + P[ob4 + 2];
-            const float r0 = o0 - (float)a.u[s], r1 = o1 - v[3], r2 = o2 - v[4];
+        if (sh < n) {
+            const float o0 = hv[0] + b40, o1 = hv[1] + b41, o2 = hv[2] + b42;
+            const float r0 = o0 - ut, r1 = o1 - hv[3], r2 = o2 - hv[4];
             const float inv = 1.0f / (float)n;
             dO0 = 2.0f * r0 * inv;
             dO1 = 2.0f * a.lam * r1 * inv;
             dO2 = 2.0f * a.lam * r2 * inv;
-            a.loss[s] = r0 * r0 + a.lam * (r1 * r1 + r2 * r2);
+            a.loss[sh] = r0 * r0 + a.lam * (r1 * r1 + r2 * r2);
         }
-        LdO[0][c] = dO0;
-        LdO[1][c] = dO1;
-        LdO[2][c] = dO2;
+        LdO[0][tid] = dO0;
+        LdO[1][tid] = dO1;
+        LdO[2][tid] = dO2;
     }
     __syncthreads();
 
@@ -400,19 +434,19 @@ __global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             float t = 0.0f;
-            for (int q = 0; q < 3; ++q) t = fmaf(P[oW4 + q * 64 + fo + r], LdO[q][c], t);
+            for (int q = 0; q < 3; ++q) t = fmaf(w4a[q][r], LdO[q][sc], t);
             d3[r] = (1.0f - h3r[r] * h3r[r]) * t;
         }
-        store4(D3 + c * kXs + fo, d3);
+        store4(D3 + sc * kXs + fo, d3);
         if (tid < 3 * 64) {
             const int q = tid >> 6, i = tid & 63;
             float gs = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) gs = fmaf(LdO[q][cc], H3[cc * kXs + i], gs);
+            for (int cc = 0; cc < kStepSamples; ++cc) gs = fmaf(LdO[q][cc], H3[cc * kXs + i], gs);
             out[oW4 + tid] = gs;
         } else if (tid < 3 * 64 + 3) {
             const int q = tid - 3 * 64;
             float gs = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) gs += LdO[q][cc];
+            for (int cc = 0; cc < kStepSamples; ++cc) gs += LdO[q][cc];
             out[ob4 + q] = gs;
         }
     }
@@ -420,18 +454,18 @@ __global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
 
     /* delta2 = (1 - h2^2) W3^T delta3; dW3 = delta3 h2^T, db3 */
     {
-        load16(D3 + c * kXs + 16 * g, xh);
+        load16(D3 + sc * kXs + 16 * g, xh);
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int j = 0; j < 16; ++j) acc = mfma16(aW3t[j], xh[j], acc);
         float d2[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) d2[r] = (1.0f - h2r[r] * h2r[r]) * acc[r];
-        store4(D2 + c * kXs + fo, d2);
-        wgrad64(D3, X2[0], out + oW3, w, g, c);
+        store4(D2 + sc * kXs + fo, d2);
+        wgrad64(D3, X2[0], out + oW3, v, g, c);
         if (tid < 64) {
             float gs = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) gs += D3[cc * kXs + tid];
+            for (int cc = 0; cc < kStepSamples; ++cc) gs += D3[cc * kXs + tid];
             out[ob3 + tid] = gs;
         }
     }
@@ -439,20 +473,20 @@ __global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
 
     /* delta1 = (1 - h1^2) W2^T delta2; dW2 = delta2 h1^T, db2 */
     {
-        load16(D2 + c * kXs + 16 * g, xh);
+        load16(D2 + sc * kXs + 16 * g, xh);
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int j = 0; j < 16; ++j) acc = mfma16(aW2t[j], xh[j], acc);
-        const float4 h1 = *reinterpret_cast<const float4*>(H1 + c * kXs + fo);
+        const float4 h1 = *reinterpret_cast<const float4*>(H1 + sc * kXs + fo);
         const float hv[4] = {h1.x, h1.y, h1.z, h1.w};
         float d1[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) d1[r] = (1.0f - hv[r] * hv[r]) * acc[r];
-        store4(D1 + c * kXs + fo, d1);
-        wgrad64(D2, H1, out + oW2, w, g, c);
+        store4(D1 + sc * kXs + fo, d1);
+        wgrad64(D2, H1, out + oW2, v, g, c);
         if (tid < 64) {
             float gs = 0.0f;
-            for (int cc = 0; cc < kTile; ++cc) gs += D2[cc * kXs + tid];
+            for (int cc = 0; cc < kStepSamples; ++cc) gs += D2[cc * kXs + tid];
             out[ob2 + tid] = gs;
         }
     }
@@ -462,12 +496,12 @@ __global__ void __launch_bounds__(256) k_train_step_mfma(StepArgs a) {
     if (tid < 128) {
         const int o = tid >> 1, i = tid & 1;
         float gs = 0.0f;
-        for (int cc = 0; cc < kTile; ++cc) gs = fmaf(D1[cc * kXs + o], Lz[i][cc], gs);
+        for (int cc = 0; cc < kStepSamples; ++cc) gs = fmaf(D1[cc * kXs + o], Lz[i][cc], gs);
         out[oW1 + tid] = gs;
     } else if (tid < 192) {
         const int o = tid - 128;
         float gs = 0.0f;
-        for (int cc = 0; cc < kTile; ++cc) gs += D1[cc * kXs + o];
+        for (int cc = 0; cc < kStepSamples; ++cc) gs += D1[cc * kXs + o];
         out[ob1 + o] = gs;
     }
 }
@@ -671,6 +705,7 @@ static ActView act_view(float* A, size_t N) {
     return v;
 }
 static int n_chunks(int n) { return (n + kChunk - 1) / kChunk; }
+static int n_step_blocks(int n) { return (n + kStepSamples - 1) / kStepSamples; }
 
 static spac_status mlp_prepare_batch(Mlp* m, const double* x, const double* y, const double* u, int n) {
     spac_status rc;
@@ -678,7 +713,7 @@ static spac_status mlp_prepare_batch(Mlp* m, const double* x, const double* y, c
     if ((rc = m->act.ensure(act_floats(N) * 4))) return rc;
     if ((rc = m->pts.ensure(N * 16))) return rc;
     if ((rc = m->tgt.ensure(N * 8))) return rc;
-    if ((rc = m->part.ensure((size_t)n_chunks(n) * kNParams * 4))) return rc;
+    if ((rc = m->part.ensure((size_t)n_step_blocks(n) * kNParams * 4))) return rc;
     MLP_HIP_CHECK(hipMemcpyAsync(m->pts.ptr, x, N * 8, hipMemcpyHostToDevice, m->stream));
     MLP_HIP_CHECK(hipMemcpyAsync((char*)m->pts.ptr + N * 8, y, N * 8, hipMemcpyHostToDevice, m->stream));
     if (u) MLP_HIP_CHECK(hipMemcpyAsync(m->tgt.ptr, u, N * 8, hipMemcpyHostToDevice, m->stream));
@@ -733,14 +768,14 @@ static void enqueue_step(Mlp* m, float lam, bool adam, int t0, int steps, float 
     st.part = (float*)m->part.ptr;
     st.loss = v.loss;
     st.n = n;
-    hipLaunchKernelGGL(k_train_step_mfma, dim3(n_chunks(n)), dim3(256), 0, s, st);
+    hipLaunchKernelGGL(k_train_step_mfma, dim3(n_step_blocks(n)), dim3(kStepThreads), 0, s, st);
     AdamArgs a;
     a.p = (float*)m->params.ptr;
     a.g = (float*)m->grads.ptr;
     a.m = (float*)m->am.ptr;
     a.v = (float*)m->av.ptr;
     a.part = (const float*)m->part.ptr;
-    a.chunks = n_chunks(n);
+    a.chunks = n_step_blocks(n);
     a.ctr = (int*)m->ctr.ptr;
     a.par = par & 1;
     a.t0 = t0;
