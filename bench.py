@@ -387,7 +387,7 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
     with MLP(dom, seed=0) as net, FKProblem(desc) as prob:
         rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager,
                         beta2=beta2)
-        table_ms = net.build_table(1024)
+        table_ms = net.build_table(256)
         net.attach(prob, fallback=True)
         res["learned"] = measure(prob, "learned", {"training_data_ms": data_ms, "training_data": {
                                                        "points": int(xs.size), "paths": data_paths, "h": data_h},

@@ -92,15 +92,22 @@ __global__ void __launch_bounds__(kDotBlock) k_multidot(int n, const double* __r
     if (threadIdx.x == 0) part[(size_t)j * nchunks + chunk] = t;
 }
 
-/* h[j] (+)= sum over chunks in order of part[j][chunk]; one wavefront per j */
-__global__ void k_finish_dots(int nv, int nchunks, const double* __restrict__ part, double* __restrict__ h, int add) {
+/* h[j] = sum over chunks in order of part[j][chunk] (root: its square root, a norm);
+ * one wavefront per j.  mirror (host-mapped pinned memory, or nullptr) receives a copy
+ * for the host's Hessenberg column. */
+__global__ void k_finish_dots(int nv, int nchunks, const double* __restrict__ part, double* __restrict__ h,
+                              double* __restrict__ mirror, int root) {
     const int j = blockIdx.x;
     const int lane = threadIdx.x;
     if (j >= nv) return;
     double acc = 0.0;
     for (int c = lane; c < nchunks; c += 64) acc += part[(size_t)j * nchunks + c];
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) h[j] = add ? h[j] + acc : acc;
+    if (lane == 0) {
+        const double v = root ? sqrt(acc) : acc;
+        h[j] = v;
+        if (mirror) mirror[j] = v;
+    }
 }
 
 /* w[i] -= sum_j h[j] Vb[j][i] (j = 0..nv-1), h staged in LDS */
@@ -143,17 +150,28 @@ __global__ void __launch_bounds__(256) k_correct(int n, const double* __restrict
     u[i] += dinv[i] * s;
 }
 
-/* sqrt of a dot result, in place (h[0] = ||.||) */
-__global__ void k_sqrt(double* h) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) h[0] = sqrt(h[0]);
-}
+/* Arnoldi steps queued ahead of the host's Givens processing: the host waits for step k's
+ * Hessenberg column while steps k + 1 .. k + kAhead run, so the GPU never idles on a host
+ * round trip.  Steps queued past convergence only add basis vectors that are not used. */
+constexpr int kAhead = 3;
+constexpr int kEvents = kAhead + 2;
 
 struct GmresState {
     DeviceBuffer Ip, J, V, B, u, dinv, Vb, w, r, part, h, nrm;
+    double* col = nullptr;     /* pinned, host-mapped: one Hessenberg column block per Arnoldi step */
+    double* col_dev = nullptr; /* its device address */
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t step_done[kEvents] = {};
     void release() {
         for (DeviceBuffer* b : {&Ip, &J, &V, &B, &u, &dinv, &Vb, &w, &r, &part, &h, &nrm}) b->release();
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (col) (void)hipHostFree(col);
+        col = col_dev = nullptr;
+        for (hipEvent_t& e : step_done) {
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
+        }
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -200,6 +218,9 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     GM_CHECK(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
     GM_CHECK(hipEventCreate(&st.e0));
     GM_CHECK(hipEventCreate(&st.e1));
+    for (hipEvent_t& e : st.step_done) GM_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    GM_CHECK(hipHostMalloc((void**)&st.col, (size_t)m * hlen * 8, hipHostMallocMapped));
+    GM_CHECK(hipHostGetDevicePointer((void**)&st.col_dev, st.col, 0));
     hipStream_t s = st.stream;
     GM_CHECK(hipMemcpyAsync(st.Ip.ptr, Ip, (N + 1) * 4, hipMemcpyHostToDevice, s));
     GM_CHECK(hipMemcpyAsync(st.J.ptr, J, (size_t)nnz * 4, hipMemcpyHostToDevice, s));
@@ -224,14 +245,12 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     double* dn = (double*)st.nrm.ptr;
     const dim3 g1((n + 255) / 256), b1(256), gw((n + 3) / 4);
 
-    auto dots = [&](const double* vecs, int nv, const double* x, double* out) {
+    auto dots = [&](const double* vecs, int nv, const double* x, double* out, double* mirror, int root) {
         hipLaunchKernelGGL(k_multidot, dim3(nchunks, nv), dim3(kDotBlock), 0, s, n, vecs, N, x, part, nchunks);
-        hipLaunchKernelGGL(k_finish_dots, dim3(nv), dim3(64), 0, s, nv, nchunks, (const double*)part, out, 0);
+        hipLaunchKernelGGL(k_finish_dots, dim3(nv), dim3(64), 0, s, nv, nchunks, (const double*)part, out, mirror,
+                           root);
     };
-    auto norm = [&](const double* x, double* out) {
-        dots(x, 1, x, out);
-        hipLaunchKernelGGL(k_sqrt, dim3(1), dim3(64), 0, s, out);
-    };
+    auto norm = [&](const double* x, double* out) { dots(x, 1, x, out, (double*)nullptr, 1); };
 
     hipLaunchKernelGGL(k_diag_inv, g1, b1, 0, s, n, dIp, dJ, dV, dinv);
     norm(dB, dn);
@@ -240,7 +259,7 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     GM_CHECK(hipStreamSynchronize(s));
     if (bnorm == 0.0) bnorm = 1.0;
 
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc(hlen), yv(m);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), yv(m);
     int it = 0, restarts = 0;
     double res = 0.0;
     bool conv = false;
@@ -252,6 +271,23 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
         GM_CHECK(hipStreamSynchronize(s));
         return SPAC_OK;
     };
+    /* Arnoldi step k: w = A D^-1 v_k, CGS2 against v_0..v_k, v_{k+1} = w / ||w||; the two
+       coefficient vectors and the norm also go to the step's pinned column block. */
+    auto enqueue_step = [&](int k) -> spac_status {
+        double* colk = st.col_dev + (size_t)k * hlen;
+        hipLaunchKernelGGL(k_spmv, gw, b1, 0, s, n, dIp, dJ, dV, (const double*)dinv,
+                           (const double*)(Vb + (size_t)k * N), w);
+        const int nv = k + 1;
+        dots(Vb, nv, w, h1, colk, 0);
+        hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, nv, w);
+        dots(Vb, nv, w, h2, colk + (m + 1), 0);
+        hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h2, nv, w);
+        dots(w, 1, w, hw, colk + 2 * (m + 1), 1);
+        hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)w, (const double*)hw, Vb + (size_t)(k + 1) * N);
+        GM_CHECK(hipGetLastError());
+        GM_CHECK(hipEventRecord(st.step_done[k % kEvents], s));
+        return SPAC_OK;
+    };
     while (it < max_iter) {
         double beta = 0.0;
         if ((rc = true_residual(&beta))) return rc;
@@ -260,24 +296,17 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
         hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)r, (const double*)dn, Vb);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
-        int k = 0;
-        for (; k < m && it < max_iter; ++k, ++it) {
-            /* w = A D^-1 v_k, then CGS2 against v_0..v_k */
-            hipLaunchKernelGGL(k_spmv, gw, b1, 0, s, n, dIp, dJ, dV, (const double*)dinv,
-                               (const double*)(Vb + (size_t)k * N), w);
-            const int nv = k + 1;
-            dots(Vb, nv, w, h1);
-            hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, nv, w);
-            dots(Vb, nv, w, h2);
-            hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h2, nv, w);
-            norm(w, hw);
-            hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)w, (const double*)hw,
-                               Vb + (size_t)(k + 1) * N);
-            GM_CHECK(hipGetLastError());
-            GM_CHECK(hipMemcpyAsync(hc.data(), h1, hlen * 8, hipMemcpyDeviceToHost, s));
-            GM_CHECK(hipStreamSynchronize(s));
+        const int steps = std::min(m, max_iter - it); /* Arnoldi steps this cycle may take */
+        int k = 0, queued = 0;
+        while (k < steps) {
+            while (queued < steps && queued - k <= kAhead) {
+                if ((rc = enqueue_step(queued))) return rc;
+                ++queued;
+            }
+            GM_CHECK(hipEventSynchronize(st.step_done[k % kEvents]));
+            const double* hc = st.col + (size_t)k * hlen;
             for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = hc[j] + hc[(size_t)(m + 1) + j];
-            const double hn = hc[hlen - 1];
+            const double hn = hc[2 * (m + 1)];
             H[(size_t)(k + 1) * m + k] = hn;
             for (int j = 0; j < k; ++j) {
                 double a = H[(size_t)j * m + k], b = H[(size_t)(j + 1) * m + k];
@@ -293,7 +322,9 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
             g[k + 1] = -sn[k] * g[k];
             g[k] = cs[k] * g[k];
             res = std::fabs(g[k + 1]) / bnorm;
-            if (res < tol || hn == 0.0) { ++k; ++it; break; }
+            ++k;
+            ++it;
+            if (res < tol || hn == 0.0) break;
         }
         /* y from the triangular system, u += D^-1 V y */
         for (int i = k - 1; i >= 0; --i) {
