@@ -363,6 +363,7 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
              "mc_ms": float(np.mean([x["ms"] for x in runs])),
              "path_kernel_ms": float(np.mean([x["path_ms"] for x in runs])),
              "gmres_ms": float(np.mean([x["gmres_ms"] for x in runs])),
+             "gmres_device_ms": float(np.mean([x["gmres_device_ms"] for x in runs])),
              "gmres_iterations": runs[0]["gmres_it"], "path_steps": runs[0]["steps"],
              "fallback_rows": runs[0]["fallbacks"], "rms_vs_exact": float(np.sqrt(np.mean((ubar - exact) ** 2)))}
         r.update(extra)

@@ -3,13 +3,15 @@
  * for the assembled interface system G u = B (the reference solves it with
  * SistemaLinealSolverGMRES, Solvers/SistemaLinealSolverGMRES_OMP.hpp:519-577).
  *
- * Device work per Arnoldi step: z = D^-1 v_k fused into a CSR SpMV (one
- * wavefront per row), classical Gram–Schmidt applied twice (CGS2: block dot
- * products of w against all basis vectors, then one fused update), the norm
- * and the scaling of the new basis vector.  The host keeps the (m+1) x m
- * Hessenberg matrix, applies the Givens rotations and tests convergence, as
- * the host solver does.  Every reduction has a fixed order (per-block tree,
- * blocks summed in index order), so a solve is bit-reproducible.
+ * Device work per Arnoldi step: the CSR SpMV w = A z_k of z_k = D^-1 v_k (written
+ * together with v_k; a few lanes per row), classical Gram–Schmidt applied twice
+ * (CGS2: block dot products of w against all basis vectors, then one fused
+ * update), the norm and the scaling of the new basis vector.  The host keeps
+ * the (m+1) x m Hessenberg matrix, applies the Givens rotations and tests
+ * convergence, as the host solver does, a few steps behind the device: the
+ * columns reach it through mapped pinned memory.  Every reduction has a fixed
+ * order (per-block tree, blocks summed in index order), so a solve is
+ * bit-reproducible.
  */
 #include <hip/hip_runtime.h>
 
@@ -46,21 +48,19 @@ __global__ void k_diag_inv(int n, const int32_t* Ip, const int32_t* J, const dou
     dinv[i] = d;
 }
 
-/* y = A (s .* x) (s = nullptr: y = A x); one wavefront per row, fixed lane order. */
+/* y = A x with L lanes per row (64 / L rows per wavefront), each row summed in a fixed lane
+ * order.  L is chosen from the mean row length so that a row takes about three passes. */
+template <int L>
 __global__ void __launch_bounds__(256) k_spmv(int n, const int32_t* __restrict__ Ip, const int32_t* __restrict__ J,
-                                              const double* __restrict__ V, const double* __restrict__ s,
-                                              const double* __restrict__ x, double* __restrict__ y) {
-    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
+                                              const double* __restrict__ V, const double* __restrict__ x,
+                                              double* __restrict__ y) {
+    const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) / L);
+    const int sub = (int)(threadIdx.x % L);
+    if (row >= n) return; /* whole L-lane groups leave together */
     double acc = 0.0;
-    for (int k = Ip[row] + lane; k < Ip[row + 1]; k += 64) {
-        const int c = J[k];
-        const double xc = s ? s[c] * x[c] : x[c];
-        acc = fma(V[k], xc, acc);
-    }
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) y[row] = acc;
+    for (int k = Ip[row] + sub; k < Ip[row + 1]; k += L) acc = fma(V[k], x[J[k]], acc);
+    for (int off = L / 2; off >= 1; off >>= 1) acc += __shfl_down(acc, off, L);
+    if (sub == 0) y[row] = acc;
 }
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -123,11 +123,15 @@ __global__ void __launch_bounds__(256) k_update(int n, const double* __restrict_
     w[i] -= s;
 }
 
-/* out = a * x  (scalar from device memory: out = x / *den when den is given) */
-__global__ void k_scale(int n, const double* __restrict__ x, const double* __restrict__ den, double* __restrict__ out) {
+/* out = x / *den (the next Krylov vector) and z = dinv .* out (the SpMV input of the next
+ * Arnoldi step, right preconditioning) */
+__global__ void k_scale(int n, const double* __restrict__ x, const double* __restrict__ den,
+                        const double* __restrict__ dinv, double* __restrict__ out, double* __restrict__ z) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = x[i] / den[0];
+    const double v = x[i] / den[0];
+    out[i] = v;
+    z[i] = dinv[i] * v;
 }
 
 /* r = B - r */
@@ -157,15 +161,15 @@ constexpr int kAhead = 3;
 constexpr int kEvents = kAhead + 2;
 
 struct GmresState {
-    DeviceBuffer Ip, J, V, B, u, dinv, Vb, w, r, part, h, nrm;
+    DeviceBuffer Ip, J, V, B, u, dinv, Vb, w, r, part, h, nrm, z;
     double* col = nullptr;     /* pinned, host-mapped: one Hessenberg column block per Arnoldi step */
     double* col_dev = nullptr; /* its device address */
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t step_done[kEvents] = {};
     void release() {
-        for (DeviceBuffer* b : {&Ip, &J, &V, &B, &u, &dinv, &Vb, &w, &r, &part, &h, &nrm}) b->release();
         if (stream) (void)hipStreamSynchronize(stream);
+        for (DeviceBuffer* b : {&Ip, &J, &V, &B, &u, &dinv, &Vb, &w, &r, &part, &h, &nrm, &z}) b->release();
         if (col) (void)hipHostFree(col);
         col = col_dev = nullptr;
         for (hipEvent_t& e : step_done) {
@@ -211,7 +215,7 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
         (rc = st.V.ensure((size_t)nnz * 8 + 8)) || (rc = st.B.ensure(N * 8)) || (rc = st.u.ensure(N * 8)) ||
         (rc = st.dinv.ensure(N * 8)) || (rc = st.Vb.ensure((size_t)(m + 1) * N * 8)) || (rc = st.w.ensure(N * 8)) ||
         (rc = st.r.ensure(N * 8)) || (rc = st.part.ensure((size_t)(m + 1) * nchunks * 8)) ||
-        (rc = st.h.ensure(hlen * 8)) || (rc = st.nrm.ensure(64))) {
+        (rc = st.h.ensure(hlen * 8)) || (rc = st.nrm.ensure(64)) || (rc = st.z.ensure(N * 8))) {
         st.release();
         return rc;
     }
@@ -243,7 +247,18 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     double* h2 = h1 + (m + 1);
     double* hw = h2 + (m + 1);
     double* dn = (double*)st.nrm.ptr;
-    const dim3 g1((n + 255) / 256), b1(256), gw((n + 3) / 4);
+    double* dz = (double*)st.z.ptr;
+    /* lanes per SpMV row: about three passes over the mean row */
+    const double mean_row = (double)nnz / (double)n;
+    const int row_lanes = mean_row > 96.0 ? 64 : mean_row > 48.0 ? 32 : mean_row > 24.0 ? 16 : 8;
+    auto spmv = [&](const double* x, double* y) {
+        const dim3 grid((unsigned)(((size_t)n * row_lanes + 255) / 256));
+        if (row_lanes == 64) hipLaunchKernelGGL(k_spmv<64>, grid, dim3(256), 0, s, n, dIp, dJ, dV, x, y);
+        else if (row_lanes == 32) hipLaunchKernelGGL(k_spmv<32>, grid, dim3(256), 0, s, n, dIp, dJ, dV, x, y);
+        else if (row_lanes == 16) hipLaunchKernelGGL(k_spmv<16>, grid, dim3(256), 0, s, n, dIp, dJ, dV, x, y);
+        else hipLaunchKernelGGL(k_spmv<8>, grid, dim3(256), 0, s, n, dIp, dJ, dV, x, y);
+    };
+    const dim3 g1((n + 255) / 256), b1(256);
 
     auto dots = [&](const double* vecs, int nv, const double* x, double* out, double* mirror, int root) {
         hipLaunchKernelGGL(k_multidot, dim3(nchunks, nv), dim3(kDotBlock), 0, s, n, vecs, N, x, part, nchunks);
@@ -264,7 +279,7 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     double res = 0.0;
     bool conv = false;
     auto true_residual = [&](double* out) -> spac_status {
-        hipLaunchKernelGGL(k_spmv, gw, b1, 0, s, n, dIp, dJ, dV, (const double*)nullptr, (const double*)du, r);
+        spmv((const double*)du, r);
         hipLaunchKernelGGL(k_residual, g1, b1, 0, s, n, dB, r);
         norm(r, dn);
         GM_CHECK(hipMemcpyAsync(out, dn, 8, hipMemcpyDeviceToHost, s));
@@ -275,15 +290,15 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
        coefficient vectors and the norm also go to the step's pinned column block. */
     auto enqueue_step = [&](int k) -> spac_status {
         double* colk = st.col_dev + (size_t)k * hlen;
-        hipLaunchKernelGGL(k_spmv, gw, b1, 0, s, n, dIp, dJ, dV, (const double*)dinv,
-                           (const double*)(Vb + (size_t)k * N), w);
+        spmv((const double*)dz, w);
         const int nv = k + 1;
         dots(Vb, nv, w, h1, colk, 0);
         hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, nv, w);
         dots(Vb, nv, w, h2, colk + (m + 1), 0);
         hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h2, nv, w);
         dots(w, 1, w, hw, colk + 2 * (m + 1), 1);
-        hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)w, (const double*)hw, Vb + (size_t)(k + 1) * N);
+        hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)w, (const double*)hw, (const double*)dinv,
+                           Vb + (size_t)(k + 1) * N, dz);
         GM_CHECK(hipGetLastError());
         GM_CHECK(hipEventRecord(st.step_done[k % kEvents], s));
         return SPAC_OK;
@@ -293,7 +308,7 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
         if ((rc = true_residual(&beta))) return rc;
         res = beta / bnorm;
         if (res < tol) { conv = true; break; }
-        hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)r, (const double*)dn, Vb);
+        hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)r, (const double*)dn, (const double*)dinv, Vb, dz);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         const int steps = std::min(m, max_iter - it); /* Arnoldi steps this cycle may take */
