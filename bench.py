@@ -409,6 +409,27 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
     return res
 
 
+def bench_train(steps=1000, lr=(2e-2, 1e-4), beta2=0.95, eager=False):
+    """The learned control variate's training alone (for kernel profiles): 4096 jittered grid
+    points on the unit square, Franke values plus N(0, 0.01^2) noise as targets, `steps`
+    full-batch Adam steps after 20 warm-up steps."""
+    from spac_amd.mlp import MLP
+    from spac_amd.problems import franke_u
+
+    g = (np.arange(64) + 0.5) / 64
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    rng = np.random.default_rng(0)
+    xs = np.clip(X.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
+    ys = np.clip(Y.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
+    u = franke_u(xs, ys) + 0.01 * rng.standard_normal(xs.size)
+    with MLP((0.0, 0.0, 1.0, 1.0), seed=0) as net:
+        net.train(xs, ys, u, steps=20, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager, beta2=beta2)
+        rep = net.train(xs, ys, u, steps=steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager,
+                        beta2=beta2)
+    return {"samples": int(xs.size), "steps": steps, "eager": bool(eager), "train_ms": rep["train_ms"],
+            "us_per_step": rep["train_ms"] * 1e3 / steps, "final_loss": rep["final_loss"]}
+
+
 def copy_calibration(mib: int, reps: int = 3) -> None:
     """Plain device copies of known size (torch's copy kernel, `mib` MiB read and
     written per launch): the reference dispatches that scale rocprofv3's
@@ -430,8 +451,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rows", choices=["rows", "points", "batched", "ttr"],
-                    help="rows (default line) | points | batched or ttr alone (for profiling)")
+    ap.add_argument("--workload", default="rows", choices=["rows", "points", "batched", "ttr", "train"],
+                    help="rows (default line) | points | batched, ttr or train alone (for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-RMS comparison (config 5)")
@@ -454,6 +475,10 @@ def main():
         copy_calibration(args.copy_mib)
     if args.workload == "batched":
         print(json.dumps({"batched": bench_batched(steps=args.steps)}), flush=True)
+        return
+    if args.workload == "train":
+        print(json.dumps({"train": bench_train(steps=args.train_steps, lr=tuple(args.train_lr), beta2=args.train_beta2,
+                                               eager=args.train_eager)}), flush=True)
         return
     if args.workload == "ttr":
         print(json.dumps({"time_to_rms": bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, refine_rep=args.ttr_refine_replicas,
