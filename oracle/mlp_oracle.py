@@ -13,7 +13,9 @@ the derivative of the u head, so one forward pass gives the control-variate fiel
 
 Only tests/ import this module, as the checker of the HIP training kernels.
 Pinned by tests/test_mlp.py: the analytic backward equals central finite
-differences of the loss.
+differences of the loss; adam_run's first step moves every parameter with a
+nonzero gradient by lr0 against the gradient's sign (the bias-corrected first
+step), and a constant gradient moves it by lr at every step.
 """
 import numpy as np
 
@@ -97,3 +99,26 @@ def loss_only(theta, x, y, u, lam, domain, g_fixed):
     f = forward(theta, x, y, domain)
     o = f["o"]
     return np.mean((o[0] - u) ** 2 + lam * ((o[1] - g_fixed[0]) ** 2 + (o[2] - g_fixed[1]) ** 2))
+
+
+def adam(theta, grad, steps, lr0, lr1, beta2, beta1=0.9, eps=1e-8):
+    """`steps` Adam steps in FP64 on the gradient function grad(theta) (Kingma & Ba, "Adam: a
+    method for stochastic optimization", ICLR 2015, Algorithm 1: bias-corrected first and second
+    moments from zero), with the learning rate decayed geometrically from lr0 at the first step
+    to lr1 at the last (lr1 <= 0: constant lr0).  Returns the parameters after the last step."""
+    th = np.array(theta, dtype=np.float64)
+    m = np.zeros_like(th)
+    v = np.zeros_like(th)
+    for k in range(steps):
+        g = grad(th)
+        t = k + 1
+        lr = lr0 * (lr1 / lr0) ** (k / (steps - 1)) if (lr1 > 0 and steps > 1) else lr0
+        m = beta1 * m + (1 - beta1) * g
+        v = beta2 * v + (1 - beta2) * g * g
+        th = th - lr * (m / (1 - beta1 ** t)) / (np.sqrt(v / (1 - beta2 ** t)) + eps)
+    return th
+
+
+def adam_run(theta, x, y, u, lam, domain, steps, lr0, lr1, beta2):
+    """adam() on the full-batch training loss of the network."""
+    return adam(theta, lambda th: loss_grad(th, x, y, u, lam, domain)[1], steps, lr0, lr1, beta2)

@@ -87,3 +87,37 @@ def test_gpu_gmres_matches_host_on_interface_system(gpu):
     assert np.abs(ud - uh).max() < 1e-8 * np.abs(uh).max()
     A = sp.csr_matrix((Gv, Gj, Gi), shape=(B.size, B.size))
     assert np.linalg.norm(A @ ud - B) < 1.5e-10 * np.linalg.norm(B)
+
+
+@pytest.mark.gpu
+def test_gpu_gmres_breakdown_with_steps_queued_ahead(gpu):
+    """A = I: the first Arnoldi step breaks down (||w|| = 0) while the next steps are already
+    queued; the solve ends after one iteration with u = B and nothing of the unused steps."""
+    say("gpu gmres: identity (Arnoldi breakdown at step 1)")
+    n = 1000
+    A = sp.eye(n, format="csr")
+    b = np.random.default_rng(3).normal(size=n)
+    u, info = gmres(A.indptr, A.indices, A.data, b, tol=1e-12, max_iter=100, restart=50, device=True)
+    assert info.converged and info.iterations == 1
+    assert np.all(np.isfinite(u)) and np.abs(u - b).max() < 1e-13 * np.abs(b).max()
+
+
+@pytest.mark.gpu
+def test_gpu_gmres_iteration_cap_and_short_restarts(gpu):
+    """max_iter ending inside a restart cycle stops at exactly max_iter iterations with the
+    cycle's partial correction applied; restart 2 (shorter than the queue-ahead depth) still
+    converges to the direct solution."""
+    say("gpu gmres: iteration cap mid-cycle, restart 2")
+    rng = np.random.default_rng(5)
+    n = 2000
+    A = (sp.random(n, n, density=0.003, random_state=7, format="csr") + sp.eye(n) * 2.5).tocsr()
+    A.sort_indices()
+    b = rng.normal(size=n)
+    u, info = gmres(A.indptr, A.indices, A.data, b, tol=1e-14, max_iter=7, restart=50, device=True)
+    assert info.iterations == 7 and not info.converged
+    r = np.linalg.norm(b - A @ u) / np.linalg.norm(b)
+    assert np.all(np.isfinite(u)) and r < 0.5 and abs(r - info.residual) < 1e-6
+    u2, info2 = gmres(A.indptr, A.indices, A.data, b, tol=1e-11, max_iter=5000, restart=2, device=True)
+    assert info2.converged and info2.restarts > 3
+    assert np.allclose(u2, spl.spsolve(A.tocsc(), b), rtol=1e-8, atol=1e-9)
+
