@@ -317,21 +317,23 @@ def _replicas(mesh, prob, paths, h, seeds, u0=None):
 
 
 def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n_rep=8, seed0=100, eager=False,
-              beta2=0.95, data_paths=500, data_h=2e-3, max_refine=3, refine_rep=16):
+              beta2=0.95, data_paths=500, data_h=2e-3, max_refine=3, refine_rep=24, grad_weight=0.003):
     """Time to a statistical RMS error `target` of the solved interface values, learned control
     variate against plain Monte Carlo, both FP32 stepping on this GPU.
 
-    Statistical RMS at a path count: independent replicas (seeds), sample variance per
-    node, averaged over nodes.  Path count: a probe at probe_paths (n_rep replicas), then up
-    to max_refine re-estimates N <- 1.05 N (RMS / target)^2, each from refine_rep fresh replicas
-    at the previous estimate, until the RMS lies in [0.9, 1] x target (at small N the noise of
-    the assembled matrix adds a higher-order term, so a single probe overestimates N).  The
-    node errors are dominated by a few smooth modes of the solve, so an RMS from R replicas
-    scatters by roughly +-sqrt(1 / R); the row-residual RMS (independent rows) does not.  The
-    same rule serves both methods; the replicas of the last count are the timed ones.  The learned-CV time
+    Statistical RMS at a path count: independent replicas (seeds), sample variance per node,
+    averaged over nodes.  The node errors are dominated by a few smooth modes of the solve, so
+    an RMS from R replicas scatters by roughly +-(1 / R)^(1/2); no single measurement decides
+    the path count.  Instead: a probe at probe_paths (n_rep replicas), then max_refine sets of
+    refine_rep replicas, each at the path count where RMS^2 N = a + b / N, fitted by weighted
+    least squares over every set so far, gives RMS = 0.9 x target (b: the higher-order term the
+    noise of the assembled matrix adds at small N).  The last set is the timed one.  `rms` is the
+    fit over every set (the last included) at the timed count; the last set's own measurement is
+    beside it.  The same rule serves both methods.  The learned-CV time
     includes its training data (plain-MC point solves at 4096 points, data_paths paths each at
-    step data_h: the control variate needs grad u, not an unbiased u), training and tabulation;
-    GMRES (GPU, FP64, relative residual GMRES_TOL) is reported beside the Monte-Carlo time."""
+    step data_h: the control variate needs grad u, not an unbiased u), training and
+    tabulation; GMRES (GPU, FP64, relative residual GMRES_TOL) is reported beside the
+    Monte-Carlo time."""
     from spac_amd.mlp import MLP
     from spac_amd.problems import franke_u
 
@@ -343,24 +345,47 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
            "mesh": {"nodes": int(mesh.info.n_nodes), "rows": int(mesh.info.n_jobs),
                     "circles": int(mesh.info.n_interfaces), "h": h, "nnz": int(mesh.info.nnz)}}
 
+    aim = 0.9 * target
+
+    def fit_paths(sets):
+        """Path count at which RMS reaches `aim` under RMS^2 N = a + b / N, fitted by least
+        squares over every replica set so far (weights = replicas); b >= 0."""
+        N = np.array([x[0] for x in sets], float)
+        y = np.array([x[1] ** 2 * x[0] for x in sets])
+        w = np.array([x[2] for x in sets], float)
+        A = np.stack([np.ones_like(N), 1.0 / N], 1) * np.sqrt(w)[:, None]
+        a, b = np.linalg.lstsq(A, y * np.sqrt(w), rcond=None)[0] if len(sets) > 1 else (y[0], 0.0)
+        if b < 0 or a <= 0:
+            a, b = float(np.sum(w * y) / np.sum(w)), 0.0
+        T2 = aim ** 2
+        return int(np.ceil((a + np.sqrt(a * a + 4.0 * b * T2)) / (2.0 * T2))), float(a), float(b)
+
     def measure(prob, name, extra):
-        paths, hist = probe_paths, []
+        paths, hist, sets = probe_paths, [], []
         for it in range(max_refine + 1):
             reps = n_rep if it == 0 else refine_rep
             runs, rms, row, ubar = _replicas(mesh, prob, paths, h, [seed0 + 1000 * it + k for k in range(reps)])
-            hist.append({"paths_per_row": paths, "rms": rms, "row_residual_rms": row})
+            hist.append({"paths_per_row": paths, "rms": rms, "row_residual_rms": row, "replicas": reps})
+            sets.append((paths, rms, reps))
             if it == 0:
                 rms0, row0 = rms, row
-            elif 0.9 * target <= rms <= target:
-                break
             if it == max_refine:
                 break
-            nxt = int(np.ceil(1.05 * paths * (rms / target) ** 2))
-            log(f"ttr {name}: rms {rms:.3e} (row {row:.3e}) at {paths} paths -> {nxt} paths")
-            paths = nxt
-        r = {"paths_per_row": paths, "rms": rms, "reached_target": bool(rms <= target), "row_residual_rms": row,
+            paths, fa, fb = fit_paths(sets)
+            log(f"ttr {name}: rms {rms:.3e} (row {row:.3e}) at {hist[-1]['paths_per_row']} paths; "
+                f"fit a {fa:.3e} b {fb:.3e} -> {paths} paths")
+        # the RMS at the timed count from the fit over every set (the last included): a pooled
+        # estimate, steadier than the last set's own measurement
+        _, fa, fb = fit_paths(sets)
+        rms_fit = float(np.sqrt(fa / paths + fb / paths ** 2))
+        # the count where the fit reaches the target itself (both methods compared at equal RMS)
+        at_target = float((fa + np.sqrt(fa * fa + 4.0 * fb * target ** 2)) / (2.0 * target ** 2))
+        r = {"paths_per_row": paths, "rms": rms_fit, "rms_last_set": rms, "reached_target": bool(rms_fit <= target),
+             "row_residual_rms": row, "fit": {"a": fa, "b": fb, "aim_rms": aim,
+                                             "replicas": int(sum(x[2] for x in sets))},
              "probe_rms": rms0, "probe_row_residual_rms": row0, "path_count_history": hist,
              "mc_ms": float(np.mean([x["ms"] for x in runs])),
+             "paths_at_target": at_target,
              "path_kernel_ms": float(np.mean([x["path_ms"] for x in runs])),
              "gmres_ms": float(np.mean([x["gmres_ms"] for x in runs])),
              "gmres_device_ms": float(np.mean([x["gmres_device_ms"] for x in runs])),
@@ -386,17 +411,21 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
         data_ms = pp.timing().total_ms
     u_mc = np.array([x["phi"] for x in st])
     with MLP(dom, seed=0) as net, FKProblem(desc) as prob:
-        rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager,
-                        beta2=beta2)
+        rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=grad_weight,
+                        eager=eager, beta2=beta2)
         table_ms = net.build_table(256)
         net.attach(prob, fallback=True)
         res["learned"] = measure(prob, "learned", {"training_data_ms": data_ms, "training_data": {
                                                        "points": int(xs.size), "paths": data_paths, "h": data_h},
                                                    "train_ms": rep["train_ms"],
                                                    "train_steps": train_steps, "train_lr": list(lr), "train_beta2": beta2 or 0.999,
+                                                   "train_grad_weight": grad_weight,
                                                    "train_loss": rep["final_loss"],
                                                    "table_ms": table_ms})
     P, L = res["plain"], res["learned"]
+    for M in (P, L):
+        # Monte-Carlo time projected linearly in N to the count where the fitted RMS is the target
+        M["mc_ms_at_target"] = M["mc_ms"] * M["paths_at_target"] / M["paths_per_row"]
     setup = L["training_data_ms"] + L["train_ms"] + L["table_ms"]
     res["time_to_rms_ms"] = {"plain_mc": P["mc_ms"], "learned_mc": L["mc_ms"], "learned_setup": setup,
                              "plain_with_gmres": P["mc_ms"] + P["gmres_ms"],
@@ -405,6 +434,9 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
     res["speedup"] = {"mc_only": P["mc_ms"] / L["mc_ms"],
                       "including_training": P["mc_ms"] / (setup + L["mc_ms"]),
                       "end_to_end_with_gmres": (P["mc_ms"] + P["gmres_ms"]) / (setup + L["mc_ms"] + L["gmres_ms"])}
+    Pt, Lt = P["mc_ms_at_target"], L["mc_ms_at_target"]
+    res["speedup_at_target"] = {"mc_only": Pt / Lt, "including_training": Pt / (setup + Lt),
+                                "end_to_end_with_gmres": (Pt + P["gmres_ms"]) / (setup + Lt + L["gmres_ms"])}
     mesh.close()
     return res
 
@@ -423,8 +455,8 @@ def bench_train(steps=1000, lr=(2e-2, 1e-4), beta2=0.95, eager=False):
     ys = np.clip(Y.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
     u = franke_u(xs, ys) + 0.01 * rng.standard_normal(xs.size)
     with MLP((0.0, 0.0, 1.0, 1.0), seed=0) as net:
-        net.train(xs, ys, u, steps=20, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager, beta2=beta2)
-        rep = net.train(xs, ys, u, steps=steps, lr=lr[0], lr_final=lr[1], grad_weight=0.03, eager=eager,
+        net.train(xs, ys, u, steps=20, lr=lr[0], lr_final=lr[1], grad_weight=0.003, eager=eager, beta2=beta2)
+        rep = net.train(xs, ys, u, steps=steps, lr=lr[0], lr_final=lr[1], grad_weight=0.003, eager=eager,
                         beta2=beta2)
     return {"samples": int(xs.size), "steps": steps, "eager": bool(eager), "train_ms": rep["train_ms"],
             "us_per_step": rep["train_ms"] * 1e3 / steps, "final_loss": rep["final_loss"]}
@@ -460,8 +492,10 @@ def main():
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--train-lr", type=float, nargs=2, default=(2e-2, 1e-4), metavar=("LR0", "LR1"))
     ap.add_argument("--train-beta2", type=float, default=0.95, help="Adam beta2 (0: 0.999)")
+    ap.add_argument("--train-grad-weight", type=float, default=0.003,
+                    help="loss weight of the gradient heads (lambda)")
     ap.add_argument("--ttr-replicas", type=int, default=8, help="replicas of the probe")
-    ap.add_argument("--ttr-refine-replicas", type=int, default=16, help="replicas of each path-count refinement")
+    ap.add_argument("--ttr-refine-replicas", type=int, default=24, help="replicas of each path-count refinement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run plain device copies of this many MiB (counter calibration for profiles)")
     ap.add_argument("--train-eager", action="store_true",
@@ -483,7 +517,8 @@ def main():
     if args.workload == "ttr":
         print(json.dumps({"time_to_rms": bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, refine_rep=args.ttr_refine_replicas,
                                      eager=args.train_eager,
-                                     lr=tuple(args.train_lr), beta2=args.train_beta2)}), flush=True)
+                                     lr=tuple(args.train_lr), beta2=args.train_beta2,
+                                     grad_weight=args.train_grad_weight)}), flush=True)
         return
     out = bench_rows(args) if args.workload == "rows" else bench_points(args)
     if not args.no_batched:
@@ -493,7 +528,8 @@ def main():
         log("time to RMS 1e-3 (config 5) ...")
         out["time_to_rms"] = bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, refine_rep=args.ttr_refine_replicas,
                                      eager=args.train_eager,
-                                     lr=tuple(args.train_lr), beta2=args.train_beta2)
+                                     lr=tuple(args.train_lr), beta2=args.train_beta2,
+                                     grad_weight=args.train_grad_weight)
     print(json.dumps(out), flush=True)
 
 

@@ -66,8 +66,11 @@ class MLP:
                                               grad_weight, abi.ptr(g, C.c_float), C.byref(loss)), self.lib)
         return loss.value, g
 
-    def train(self, x, y, u, steps, lr=3e-3, lr_final=1e-4, grad_weight=0.1, eager=False, beta2=0.0):
-        """beta2: Adam second-moment decay (0: 0.999)."""
+    def train(self, x, y, u, steps, lr=2e-2, lr_final=1e-4, grad_weight=0.003, eager=False, beta2=0.95):
+        """Full-batch Adam with a geometric learning-rate decay lr -> lr_final; grad_weight is the
+        loss weight of the gradient heads, beta2 Adam's second-moment decay (0: 0.999).  The
+        defaults are the settings that gave the highest row-variance ratio of the learned control
+        variate at 64x64 circles in 1000 steps (48.9 against 16.8 at grad_weight 0.03)."""
         x, y, u = _d(x), _d(y), _d(u)
         cfg = abi.MlpTrainCfg(steps, 1 if eager else 0, lr, lr_final, grad_weight, beta2)
         rep = abi.MlpTrainReport()
