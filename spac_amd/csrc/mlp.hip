@@ -45,6 +45,9 @@ constexpr int kH = 64;
 constexpr int oW1 = 0, ob1 = oW1 + kH * 2, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, oW3 = ob2 + kH,
               ob3 = oW3 + kH * kH, oW4 = ob3 + kH, ob4 = oW4 + 3 * kH, kNParams = ob4 + 3;
 static_assert(kNParams == SPAC_MLP_N_PARAMS, "parameter count");
+/* row stride of the per-block gradient rows, padded to a multiple of 4 floats (16-byte rows;
+ * the pad float is never written nor read) */
+constexpr int kPartStride = (kNParams + 3) & ~3;
 
 /* ------------------------------------------------------------ training -- */
 constexpr int kTile = 16;        /* samples per block */
@@ -202,8 +205,8 @@ __global__ void __launch_bounds__(256) k_fwd_tile(FwdArgs a) {
  * (wave, lane group) in order.  Backward: delta_{L-1} = (1 - h^2) W_L^T delta_L on the same
  * tiling.  Weight gradients dW_L = delta_L h_{L-1}^T over all the block's samples (M = 64,
  * N = 64, K = 16 kStepTiles): wave v of the block takes rows 16 (v % 4) .. +15 and
- * 4 / kStepTiles column tiles.  The block's gradient sums go to part[block][...] for the
- * fixed-order reduction in k_reduce_adam, so a step writes and reads n / (16 kStepTiles)
+ * 4 / kStepTiles column tiles.  The block's gradient sums go to part[block][...] (non-temporal
+ * stores: read once, by the reduction) for the fixed-order reduction in k_reduce_adam, so a step writes and reads n / (16 kStepTiles)
  * gradient rows.  Samples past n have z = 0 and dL/do = 0, so they add nothing. */
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kStepTiles = 1;
@@ -268,7 +271,7 @@ __device__ __forceinline__ void wgrad64(const float* D, const float* X, float* o
         for (int j = 0; j < kSteps; ++j)
             acc = mfma16(av[j], X[(16 * (j >> 2) + 4 * g + (j & 3)) * kXs + 16 * nt + c], acc);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) out[(o0 + 4 * g + r) * 64 + 16 * nt + c] = acc[r];
+        for (int r = 0; r < 4; ++r) __builtin_nontemporal_store(acc[r], &out[(o0 + 4 * g + r) * 64 + 16 * nt + c]);
     }
 }
 
@@ -385,7 +388,6 @@ __global__ void __launch_bounds__(kStepThreads) k_train_step_mfma(StepArgs a) {
         float pg[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int f = fo + r;
             const float h = train_tanh(ah[r] + b3a[r]);
             const float d = 1.0f - h * h;
             h3r[r] = h;
@@ -427,7 +429,7 @@ __global__ void __launch_bounds__(kStepThreads) k_train_step_mfma(StepArgs a) {
     }
     __syncthreads();
 
-    float* out = a.part + (size_t)blockIdx.x * kNParams;
+    float* out = a.part + (size_t)blockIdx.x * kPartStride;
     /* delta3 = (1 - h3^2) W4^T dO for the lane's features; dW4, db4 */
     {
         float d3[4];
@@ -442,12 +444,12 @@ __global__ void __launch_bounds__(kStepThreads) k_train_step_mfma(StepArgs a) {
             const int q = tid >> 6, i = tid & 63;
             float gs = 0.0f;
             for (int cc = 0; cc < kStepSamples; ++cc) gs = fmaf(LdO[q][cc], H3[cc * kXs + i], gs);
-            out[oW4 + tid] = gs;
+            __builtin_nontemporal_store(gs, &out[oW4 + tid]);
         } else if (tid < 3 * 64 + 3) {
             const int q = tid - 3 * 64;
             float gs = 0.0f;
             for (int cc = 0; cc < kStepSamples; ++cc) gs += LdO[q][cc];
-            out[ob4 + q] = gs;
+            __builtin_nontemporal_store(gs, &out[ob4 + q]);
         }
     }
     __syncthreads();
@@ -466,7 +468,7 @@ __global__ void __launch_bounds__(kStepThreads) k_train_step_mfma(StepArgs a) {
         if (tid < 64) {
             float gs = 0.0f;
             for (int cc = 0; cc < kStepSamples; ++cc) gs += D3[cc * kXs + tid];
-            out[ob3 + tid] = gs;
+            __builtin_nontemporal_store(gs, &out[ob3 + tid]);
         }
     }
     __syncthreads();
@@ -487,7 +489,7 @@ __global__ void __launch_bounds__(kStepThreads) k_train_step_mfma(StepArgs a) {
         if (tid < 64) {
             float gs = 0.0f;
             for (int cc = 0; cc < kStepSamples; ++cc) gs += D2[cc * kXs + tid];
-            out[ob2 + tid] = gs;
+            __builtin_nontemporal_store(gs, &out[ob2 + tid]);
         }
     }
     __syncthreads();
@@ -497,31 +499,31 @@ __global__ void __launch_bounds__(kStepThreads) k_train_step_mfma(StepArgs a) {
         const int o = tid >> 1, i = tid & 1;
         float gs = 0.0f;
         for (int cc = 0; cc < kStepSamples; ++cc) gs = fmaf(D1[cc * kXs + o], Lz[i][cc], gs);
-        out[oW1 + tid] = gs;
+        __builtin_nontemporal_store(gs, &out[oW1 + tid]);
     } else if (tid < 192) {
         const int o = tid - 128;
         float gs = 0.0f;
         for (int cc = 0; cc < kStepSamples; ++cc) gs += D1[cc * kXs + o];
-        out[ob1 + o] = gs;
+        __builtin_nontemporal_store(gs, &out[ob1 + o]);
     }
 }
 
 /* grad[k] = sum over chunks of part[chunk][k] in a fixed order (32 interleaved
- * groups of chunks, each summed in index order, then the groups in order);
- * then, in training, one Adam step with a geometric learning-rate decay.  The
- * step counter lives in device memory so that a captured graph of steps
- * replays unchanged.  It is double-buffered by step parity: a step of parity p
- * reads ctr[p] (the steps taken so far) in every block, and block 0 writes
- * ctr[1 - p] = ctr[p] + 1 for the next step, which nothing reads during this
- * launch.  Captured graphs hold an even number of steps, so a step's parity is
- * fixed at capture. */
+ * groups of chunks, each summed in index order, then the groups in order;
+ * non-temporal loads: every row is read once); then, in training, one Adam
+ * step with a geometric learning-rate decay.  The step counter lives in device
+ * memory so that a captured graph of steps replays unchanged.  It is
+ * double-buffered by step parity: a step of parity p reads ctr[p] (the steps
+ * taken so far) in every block, and block 0 writes ctr[1 - p] = ctr[p] + 1 for
+ * the next step, which nothing reads during this launch.  Captured graphs hold
+ * an even number of steps, so a step's parity is fixed at capture. */
 constexpr int kRedParams = 32, kRedGroups = 32;
 struct AdamArgs {
     float* p;
     float* g;
     float* m;
     float* v;
-    const float* part;
+    const float* part;  /* [chunk][kPartStride] */
     int chunks;
     int* ctr;           /* [2]: Adam steps taken so far, by step parity */
     int par;            /* this step's parity */
@@ -539,7 +541,8 @@ __global__ void __launch_bounds__(kRedParams * kRedGroups) k_reduce_adam(AdamArg
     if (k < kNParams) {
         /* sequential sum in chunk order; unrolled so that the loads overlap */
 #pragma unroll 8
-        for (int c = grp; c < a.chunks; c += kRedGroups) acc += a.part[(size_t)c * kNParams + k];
+        for (int c = grp; c < a.chunks; c += kRedGroups)
+            acc += __builtin_nontemporal_load(a.part + (size_t)c * kPartStride + k);
     }
     sh[grp][pl] = acc;
     __syncthreads();
@@ -713,7 +716,7 @@ static spac_status mlp_prepare_batch(Mlp* m, const double* x, const double* y, c
     if ((rc = m->act.ensure(act_floats(N) * 4))) return rc;
     if ((rc = m->pts.ensure(N * 16))) return rc;
     if ((rc = m->tgt.ensure(N * 8))) return rc;
-    if ((rc = m->part.ensure((size_t)n_step_blocks(n) * kNParams * 4))) return rc;
+    if ((rc = m->part.ensure((size_t)n_step_blocks(n) * kPartStride * 4))) return rc;
     MLP_HIP_CHECK(hipMemcpyAsync(m->pts.ptr, x, N * 8, hipMemcpyHostToDevice, m->stream));
     MLP_HIP_CHECK(hipMemcpyAsync((char*)m->pts.ptr + N * 8, y, N * 8, hipMemcpyHostToDevice, m->stream));
     if (u) MLP_HIP_CHECK(hipMemcpyAsync(m->tgt.ptr, u, N * 8, hipMemcpyHostToDevice, m->stream));
