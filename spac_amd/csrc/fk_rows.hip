@@ -178,14 +178,27 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_imq_kernel(RowsArgs a) {
  * LDS for full circles and L1/L2 for arcs), and adds it into column s + r (mod M) of the
  * quarter-wave's own copy of the row.  Copy (w, q) holds the paths p of every chunk with
  * p / 64 = w and p % 4 = q, in path order; the 16 copies are merged in (w, q) order.
- * Dynamic LDS: kRowsBlock * wstride doubles of values, then 16 * M TwoSum accumulators. */
+ * PAIR: rows of at most 128 paths, two per block (threads 128 h .. 128 h + 127 take row
+ * 2 blockIdx + h), one chunk each.  Such a row only fills copies w = 0, 1, so its 8 copies
+ * merged in order give the unpaired sum (the other 8 are zero), and half the block no longer
+ * idles.  Every barrier is reached by the whole block: one chunk per half, no early return.
+ * Dynamic LDS: kRowsBlock * wstride doubles of values, then 16 * mmax TwoSum accumulators. */
 constexpr int kCopies = kRowsWaves * 4;
-__global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs a, int wstride) {
+constexpr int kPairMaxPaths = kRowsBlock / 2;
+template <bool PAIR>
+__global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs a, int wstride, int mmax) {
     extern __shared__ double s_dyn[];
-    const int job = blockIdx.x;
+    constexpr int kRowThreads = PAIR ? kRowsBlock / 2 : kRowsBlock;
+    constexpr int kRowCopies = PAIR ? kCopies / 2 : kCopies;
     const int tid = threadIdx.x;
-    const int64_t np = a.paths[job];
-    const int iface = a.job_iface[job];
+    const int half = PAIR ? (tid >> 7) : 0;
+    const int ltid = PAIR ? (tid & (kRowThreads - 1)) : tid; /* the thread's index within its row */
+    const int sbase = half * kRowThreads;                     /* the row's slots in s_v, s_s, s_y */
+    const int job = PAIR ? 2 * (int)blockIdx.x + half : (int)blockIdx.x;
+    const bool has_job = job < a.n_jobs; /* PAIR: the last block's second half may be empty */
+    const int jl = has_job ? job : a.n_jobs - 1;
+    const int64_t np = has_job ? a.paths[jl] : 0;
+    const int iface = a.job_iface[jl];
     const int nb = a.node_offset[iface];
     const int M = a.node_offset[iface + 1] - nb;
     const double* th = a.theta + nb;
@@ -193,48 +206,53 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
     const double cx = a.cx[iface], cy = a.cy[iface];
     const int S = min(a.stencil_size, M);
     const double* ipsi = a.ipsi + (a.ipsi_offset[iface] >= 0 ? a.ipsi_offset[iface] : 0);
-    double* G = a.G + a.g_off[job];
+    double* G = a.G + a.g_off[jl];
 
     double* s_v = s_dyn; /* per path of the chunk: its S Gaussian values phi */
-    spac_ksum* s_acc = reinterpret_cast<spac_ksum*>(s_dyn + (size_t)kRowsBlock * wstride);
-    __shared__ double s_th[kMaxCols];
+    spac_ksum* s_acc =
+        reinterpret_cast<spac_ksum*>(s_dyn + (size_t)kRowsBlock * wstride) + (size_t)half * kRowCopies * mmax;
+    __shared__ double s_th_all[PAIR ? 2 : 1][kMaxCols];
     __shared__ int s_s[kRowsBlock]; /* stencil start, -1: not a circle exit */
     __shared__ double s_y[kRowsBlock];
-    __shared__ double s_blk[kMaxStencil * kMaxStencil];
+    __shared__ double s_blk_all[PAIR ? 2 : 1][kMaxStencil * kMaxStencil];
+    double* const s_th = s_th_all[half];
+    double* const s_blk = s_blk_all[half];
 
-    const int lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < M; i += kRowsBlock) s_th[i] = th[i];
+    const int lane = tid & 63, wave = (tid >> 6) & (kRowThreads / 64 - 1);
+    for (int i = ltid; i < M; i += kRowThreads) s_th[i] = th[i];
     const bool circle = code == 1;
     if (circle)
-        for (int i = tid; i < S * S; i += kRowsBlock) s_blk[i] = ipsi[i];
-    for (int i = tid; i < kCopies * M; i += kRowsBlock) s_acc[i] = spac_ksum_zero();
+        for (int i = ltid; i < S * S; i += kRowThreads) s_blk[i] = ipsi[i];
+    for (int i = ltid; i < kRowCopies * M; i += kRowThreads) s_acc[i] = spac_ksum_zero();
     s_s[tid] = -1; /* phase 2 may read entries of paths beyond the chunk: -1 or a valid start */
     __syncthreads();
-    if (np == 0) {
+    if (!PAIR && np == 0) {
         for (int j = tid; j < M; j += kRowsBlock) G[j] = 0.0;
         return;
     }
     const int quarter = lane >> 4, r = lane & 15;
     spac_ksum* my = s_acc + (size_t)(wave * 4 + quarter) * M;
-    const int64_t off = a.rec_off[job];
+    const int64_t off = a.rec_off[jl];
     /* the records of the next chunk are loaded while this one is processed (one
      * round trip per chunk instead of two dependent ones on the critical path) */
     int8_t n_out = 0;
     double n_x = 0.0, n_y = 0.0, n_Y = 0.0;
-    if (tid < np) {
-        const int64_t g = off + tid;
+    if (ltid < np) {
+        const int64_t g = off + ltid;
         n_out = a.outcome[g]; n_x = a.xt[g]; n_y = a.yt[g]; n_Y = a.Y[g];
     }
-    for (int64_t base = 0; base < np; base += kRowsBlock) {
-        const int cnt = (np - base < (int64_t)kRowsBlock) ? (int)(np - base) : kRowsBlock;
+    /* PAIR: exactly one chunk for both halves (np <= 128), so the barriers stay uniform */
+    const int64_t nloop = PAIR ? 1 : np;
+    for (int64_t base = 0; base < nloop; base += kRowThreads) {
+        const int cnt = (int)((np - base < (int64_t)kRowThreads) ? (np - base) : kRowThreads);
         const int8_t c_out = n_out;
         const double c_x = n_x, c_y = n_y, c_Y = n_Y;
-        if (base + kRowsBlock + tid < np) {
-            const int64_t g = off + base + kRowsBlock + tid;
+        if (base + kRowThreads + ltid < np) {
+            const int64_t g = off + base + kRowThreads + ltid;
             n_out = a.outcome[g]; n_x = a.xt[g]; n_y = a.yt[g]; n_Y = a.Y[g];
         }
         /* phase 1, one thread per path: exit angle, stencil start, Gaussians */
-        if (tid < cnt) {
+        if (ltid < cnt) {
             int s = -1;
             if (c_out == SPAC_OUT_SUBDOMAIN) {
                 double theta = 0.0;
@@ -261,7 +279,8 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
          * w_r = sum_i iPsi_s[i][r] phi_i (the stencil's inverse read row by row, 16
          * contiguous values per quarter-wave).  The quarter-wave's 16 paths of the chunk
          * are independent here, so their loads overlap: indices are clamped to valid
-         * entries and unused results dropped instead of branching. */
+         * entries and unused results dropped instead of branching.  Path p of the row
+         * sits in slot sbase + p. */
         const int p0 = wave * 64 + quarter;
         const int p1 = min(cnt, wave * 64 + 64);
         const int rr = (r < S) ? r : 0;
@@ -270,14 +289,14 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int p = p0 + 4 * k;
-                const int pc = (p < p1) ? p : p0;
+                const int pc = sbase + ((p < p1) ? p : p0);
                 yw[k] = s_v[pc * wstride + rr];
             }
         } else
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int p = p0 + 4 * k;
-            const int pc = (p < p1) ? p : p0;
+            const int pc = sbase + ((p < p1) ? p : p0);
             const int s = s_s[pc];
             const int sc = (s >= 0) ? s : 0;
             const double* blk = circle ? s_blk : ipsi + (int64_t)sc * S * S;
@@ -291,7 +310,7 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
         for (int k = 0; k < 16; ++k) {
             const int p = p0 + 4 * k;
             if (p < p1) {
-                const int s = s_s[p];
+                const int s = s_s[sbase + p];
                 if (s >= 0 && r < S) {
                     int c = s + r;
                     if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
@@ -301,10 +320,15 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
         }
         __syncthreads();
     }
+    if (!has_job) return;
+    if (PAIR && np == 0) {
+        for (int j = ltid; j < M; j += kRowThreads) G[j] = 0.0;
+        return;
+    }
     const double N = (double)np;
-    for (int c = tid; c < M; c += kRowsBlock) {
+    for (int c = ltid; c < M; c += kRowThreads) {
         spac_ksum t = s_acc[c];
-        for (int k = 1; k < kCopies; ++k) t = spac_ksum_merge(t, s_acc[(size_t)k * M + c]);
+        for (int k = 1; k < kRowCopies; ++k) t = spac_ksum_merge(t, s_acc[(size_t)k * M + c]);
         G[c] = -spac_ksum_value(t) / N;
     }
 }
@@ -360,6 +384,7 @@ struct RowsSetup {
     int n = 0;
     int mmax = 1;     /* largest interface of the batch (dynamic LDS of the stencil kernel) */
     int wstride = 1;  /* odd row stride of the per-path values */
+    bool pair_rows = false; /* every row has at most kPairMaxPaths paths: two rows per G block */
     const int32_t* d_front = nullptr;
     double* d_B = nullptr;
     spac_point_stats* d_stats = nullptr;
@@ -401,6 +426,10 @@ static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const
         if (need > (size_t)lds_max)
             return set_error(SPAC_E_INVALID_ARG, "interfaces of %d nodes need %zu B of LDS per block (device: %d)",
                              rs->mmax, need, lds_max);
+        uint32_t pmax = 0;
+        for (int k = 0; k < n; ++k) pmax = std::max(pmax, paths[k]);
+        /* the paired kernel's static LDS holds two rows' angles and stencil inverses */
+        rs->pair_rows = pmax <= (uint32_t)kPairMaxPaths && stencil_lds_bytes(*rs) + 12 * 1024 <= (size_t)lds_max;
     }
     rs->gcol.resize((size_t)rs->goff[n]);
     for (int k = 0; k < n; ++k) {
@@ -499,7 +528,12 @@ static spac_status rows_enqueue_bg(Problem* p, const RowsSetup& rs) {
     ROWS_HIP_CHECK(hipGetLastError());
     if (rs.ra.rbf_kind == SPAC_RBF_GAUSS_STENCIL) {
         const size_t lds = stencil_lds_bytes(rs);
-        hipLaunchKernelGGL(fk_rows_g_stencil_kernel, dim3(n), dim3(kRowsBlock), lds, s, rs.ra, rs.wstride);
+        if (rs.pair_rows)
+            hipLaunchKernelGGL(fk_rows_g_stencil_kernel<true>, dim3((n + 1) / 2), dim3(kRowsBlock), lds, s, rs.ra,
+                               rs.wstride, rs.mmax);
+        else
+            hipLaunchKernelGGL(fk_rows_g_stencil_kernel<false>, dim3(n), dim3(kRowsBlock), lds, s, rs.ra, rs.wstride,
+                               rs.mmax);
     } else {
         hipLaunchKernelGGL(fk_rows_g_imq_kernel, dim3(n), dim3(kRowsBlock), 0, s, rs.ra);
     }

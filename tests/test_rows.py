@@ -99,6 +99,25 @@ def test_gpu_rows_bit_identical_stencil(gpu):
 
 
 @pytest.mark.gpu
+def test_gpu_rows_bit_identical_stencil_short_rows(gpu):
+    """Rows of at most 128 paths take the paired G kernel (two rows per block); 129 paths the
+    one-row kernel.  Both equal the oracle bit for bit, with boundary rows (no paths) among the
+    jobs and an odd job count (the last block's second half has no row)."""
+    say("rows parity: Gaussian stencils, rows of 100 / 128 / 129 paths")
+    desc = problem_desc(coef=VC, domain=(0.0, 0.0, 1.0, 1.0))
+    for m, h, step in ((Mesh(mesh_desc()), 5e-4, 31), (Mesh(mesh_desc(n_side=16, nodes_per_circle=40)), 2e-4, 97)):
+        jobs = np.concatenate([np.arange(0, m.info.n_jobs, step), np.nonzero(m.job_frontera)[0][:2]])
+        if jobs.size % 2 == 0:
+            jobs = jobs[:-1]
+        for paths in (100, 128, 129):
+            B, G, st = o.assemble_rows(desc, m, paths, h, seed=13, jobs=jobs)
+            with FKProblem(desc) as p:
+                rows = p.assemble_rows(m, paths, h, seed=13, jobs=jobs)
+            _rows_equal(rows, B, G, st, jobs, m)
+        m.close()
+
+
+@pytest.mark.gpu
 def test_gpu_rows_bit_identical_imq(gpu):
     say("rows parity: global IMQ (reference mode), Franke")
     m = serial_mesh()
