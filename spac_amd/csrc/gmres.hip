@@ -4,13 +4,22 @@
  * SistemaLinealSolverGMRES, Solvers/SistemaLinealSolverGMRES_OMP.hpp:519-577).
  *
  * Device work per Arnoldi step: the CSR SpMV w = A z_k of z_k = D^-1 v_k (written
- * together with v_k; a few lanes per row), classical Gram–Schmidt applied twice
- * (CGS2: block dot products of w against all basis vectors, then one fused
- * update), the norm and the scaling of the new basis vector.  The host keeps
- * the (m+1) x m Hessenberg matrix, applies the Givens rotations and tests
- * convergence, as the host solver does, a few steps behind the device: the
- * columns reach it through mapped pinned memory.  Every reduction has a fixed
- * order (per-block tree, blocks summed in index order), so a solve is
+ * together with v_k; a few lanes per row) into the slot of v_{k+1}, classical
+ * Gram–Schmidt (block dot products of w against all basis vectors, which also
+ * give ||w||^2, then one fused update), the norm and the scaling of the new
+ * basis vector.  Steps start with one Gram–Schmidt pass; the host watches a
+ * DGKS-type test of every step (Daniel, Gragg, Kaufman, Stewart, Math. Comp. 30
+ * (1976) 772-795: how much of ||w|| the pass keeps) and, from the first step
+ * that keeps less than kKeepMin of it, queues every later step with the second
+ * pass (CGS2).  The classical bound 1/sqrt(2) asks for the second pass on most
+ * steps of the interface systems, where one pass already gives the two-pass
+ * iteration counts and residuals; kKeepMin = 0.01 switches only on heavy
+ * cancellation.  Convergence is only declared on the true residual, so a step
+ * with a less orthogonal basis can cost iterations but not correctness.  The host
+ * keeps the (m+1) x m Hessenberg matrix, applies the Givens rotations and
+ * tests convergence, as the host solver does, a few steps behind the device:
+ * the columns reach it through mapped pinned memory.  Every reduction has a
+ * fixed order (per-block tree, blocks summed in index order), so a solve is
  * bit-reproducible.
  */
 #include <hip/hip_runtime.h>
@@ -158,10 +167,11 @@ __global__ void __launch_bounds__(256) k_correct(int n, const double* __restrict
  * Hessenberg column while steps k + 1 .. k + kAhead run, so the GPU never idles on a host
  * round trip.  Steps queued past convergence only add basis vectors that are not used. */
 constexpr int kAhead = 3;
+constexpr double kKeepMin = 0.01; /* one-pass steps must keep this share of ||w|| */
 constexpr int kEvents = kAhead + 2;
 
 struct GmresState {
-    DeviceBuffer Ip, J, V, B, u, dinv, Vb, w, r, part, h, nrm, z;
+    DeviceBuffer Ip, J, V, B, u, dinv, Vb, r, part, h, nrm, z;
     double* col = nullptr;     /* pinned, host-mapped: one Hessenberg column block per Arnoldi step */
     double* col_dev = nullptr; /* its device address */
     hipStream_t stream = nullptr;
@@ -169,7 +179,7 @@ struct GmresState {
     hipEvent_t step_done[kEvents] = {};
     void release() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (DeviceBuffer* b : {&Ip, &J, &V, &B, &u, &dinv, &Vb, &w, &r, &part, &h, &nrm, &z}) b->release();
+        for (DeviceBuffer* b : {&Ip, &J, &V, &B, &u, &dinv, &Vb, &r, &part, &h, &nrm, &z}) b->release();
         if (col) (void)hipHostFree(col);
         col = col_dev = nullptr;
         for (hipEvent_t& e : step_done) {
@@ -213,7 +223,7 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     spac_status rc;
     if ((rc = st.Ip.ensure((N + 1) * 4)) || (rc = st.J.ensure((size_t)nnz * 4 + 4)) ||
         (rc = st.V.ensure((size_t)nnz * 8 + 8)) || (rc = st.B.ensure(N * 8)) || (rc = st.u.ensure(N * 8)) ||
-        (rc = st.dinv.ensure(N * 8)) || (rc = st.Vb.ensure((size_t)(m + 1) * N * 8)) || (rc = st.w.ensure(N * 8)) ||
+        (rc = st.dinv.ensure(N * 8)) || (rc = st.Vb.ensure((size_t)(m + 1) * N * 8)) ||
         (rc = st.r.ensure(N * 8)) || (rc = st.part.ensure((size_t)(m + 1) * nchunks * 8)) ||
         (rc = st.h.ensure(hlen * 8)) || (rc = st.nrm.ensure(64)) || (rc = st.z.ensure(N * 8))) {
         st.release();
@@ -240,7 +250,6 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     double* du = (double*)st.u.ptr;
     double* dinv = (double*)st.dinv.ptr;
     double* Vb = (double*)st.Vb.ptr;
-    double* w = (double*)st.w.ptr;
     double* r = (double*)st.r.ptr;
     double* part = (double*)st.part.ptr;
     double* h1 = (double*)st.h.ptr;
@@ -275,6 +284,8 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     if (bnorm == 0.0) bnorm = 1.0;
 
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), yv(m);
+    std::vector<char> pass2(m, 0); /* whether step k of the cycle took the second Gram–Schmidt pass */
+    bool two_pass = false;
     int it = 0, restarts = 0;
     double res = 0.0;
     bool conv = false;
@@ -286,19 +297,23 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
         GM_CHECK(hipStreamSynchronize(s));
         return SPAC_OK;
     };
-    /* Arnoldi step k: w = A D^-1 v_k, CGS2 against v_0..v_k, v_{k+1} = w / ||w||; the two
-       coefficient vectors and the norm also go to the step's pinned column block. */
-    auto enqueue_step = [&](int k) -> spac_status {
+    /* Arnoldi step k: w = A D^-1 v_k in the slot of v_{k+1}, Gram–Schmidt against v_0..v_k
+       (once, or twice when two_pass), v_{k+1} = w / ||w||.  The coefficient vectors, ||w||^2
+       before the first pass (h1[k + 1]) and the final norm also go to the step's pinned
+       column block. */
+    auto enqueue_step = [&](int k, bool two_pass) -> spac_status {
         double* colk = st.col_dev + (size_t)k * hlen;
+        double* w = Vb + (size_t)(k + 1) * N;
         spmv((const double*)dz, w);
         const int nv = k + 1;
-        dots(Vb, nv, w, h1, colk, 0);
+        dots(Vb, nv + 1, w, h1, colk, 0);
         hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, nv, w);
-        dots(Vb, nv, w, h2, colk + (m + 1), 0);
-        hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h2, nv, w);
+        if (two_pass) {
+            dots(Vb, nv, w, h2, colk + (m + 1), 0);
+            hipLaunchKernelGGL(k_update, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h2, nv, w);
+        }
         dots(w, 1, w, hw, colk + 2 * (m + 1), 1);
-        hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)w, (const double*)hw, (const double*)dinv,
-                           Vb + (size_t)(k + 1) * N, dz);
+        hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)w, (const double*)hw, (const double*)dinv, w, dz);
         GM_CHECK(hipGetLastError());
         GM_CHECK(hipEventRecord(st.step_done[k % kEvents], s));
         return SPAC_OK;
@@ -315,13 +330,16 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
         int k = 0, queued = 0;
         while (k < steps) {
             while (queued < steps && queued - k <= kAhead) {
-                if ((rc = enqueue_step(queued))) return rc;
+                pass2[queued] = two_pass ? 1 : 0;
+                if ((rc = enqueue_step(queued, two_pass))) return rc;
                 ++queued;
             }
             GM_CHECK(hipEventSynchronize(st.step_done[k % kEvents]));
             const double* hc = st.col + (size_t)k * hlen;
-            for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = hc[j] + hc[(size_t)(m + 1) + j];
+            for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = hc[j] + (pass2[k] ? hc[(size_t)(m + 1) + j] : 0.0);
             const double hn = hc[2 * (m + 1)];
+            /* a one-pass step that kept too little of ||w||: from now on, steps take two passes */
+            if (!pass2[k] && hn * hn < kKeepMin * kKeepMin * hc[k + 1]) two_pass = true;
             H[(size_t)(k + 1) * m + k] = hn;
             for (int j = 0; j < k; ++j) {
                 double a = H[(size_t)j * m + k], b = H[(size_t)(j + 1) * m + k];

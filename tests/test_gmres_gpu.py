@@ -121,3 +121,24 @@ def test_gpu_gmres_iteration_cap_and_short_restarts(gpu):
     assert info2.converged and info2.restarts > 3
     assert np.allclose(u2, spl.spsolve(A.tocsc(), b), rtol=1e-8, atol=1e-9)
 
+
+@pytest.mark.gpu
+def test_gpu_gmres_switches_to_two_passes_on_cancellation(gpu):
+    """Eigenvalues in five tight clusters: after about five steps the Krylov space nearly holds
+    A v_k, a one-pass step keeps far less than 1 % of ||w|| and the solve continues with two
+    Gram-Schmidt passes; it still converges to the direct solution."""
+    say("gpu gmres: clustered spectrum (one-pass -> two-pass switch)")
+    rng = np.random.default_rng(9)
+    n = 400
+    centres = np.array([1.0, 2.0, 3.5, 5.0, 8.0])
+    lam = centres[rng.integers(0, 5, n)] * (1.0 + 1e-7 * rng.standard_normal(n))
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = sp.csr_matrix((Q * lam) @ Q.T)
+    A.sort_indices()
+    b = rng.normal(size=n)
+    u, info = gmres(A.indptr, A.indices, A.data, b, tol=1e-10, max_iter=2000, restart=60, device=True)
+    assert info.converged
+    x = np.linalg.solve(A.toarray(), b)
+    assert np.abs(u - x).max() < 1e-7 * np.abs(x).max()
+    assert np.linalg.norm(b - A @ u) < 2e-10 * np.linalg.norm(b)
+
