@@ -22,7 +22,7 @@ sys.path.insert(0, str(ROOT))
 from spac_amd import abi  # noqa: E402
 from spac_amd.fk import FKProblem, SolveBatchArrays, problem_desc  # noqa: E402
 from spac_amd.flops import FLOPS_PER_STEP, PEAK_FP64_TFLOPS  # noqa: E402
-from spac_amd.mesh import Mesh, gmres, mesh_desc  # noqa: E402
+from spac_amd.mesh import DeviceSystem, Mesh, gmres, mesh_desc  # noqa: E402
 
 
 def log(msg):
@@ -285,29 +285,34 @@ def _ttr_mesh():
 GMRES_TOL = 1e-8
 
 
-def _replicas(mesh, prob, paths, h, seeds, u0=None):
-    """Independent assemblies (one per seed) at `paths` per row, each solved by GPU GMRES.
+def _replicas(mesh, system, prob, paths, h, seeds, u0=None):
+    """Independent assemblies (one per seed) at `paths` per row into the device-resident system,
+    each solved there by GPU GMRES.
 
     Returns per-replica timings and two error measures: the statistical RMS of the solved
     node values (sample variance across replicas, averaged over nodes) and the RMS of the
     row residuals G_k u_bar - B_k around the replica-mean solution (the per-row Monte-Carlo
-    noise, independent across rows)."""
+    noise, independent across rows).  The system is copied out for these statistics outside
+    the timed calls."""
     import scipy.sparse as sp
 
     out, us, mats = [], [], []
     n = mesh.info.n_nodes
     for s in seeds:
-        rows = prob.assemble_rows(mesh, paths, h, seed=s)
+        t0 = time.perf_counter()
+        st = prob.assemble_system(mesh, system, paths, h, seed=s)
+        t1 = time.perf_counter()
         t = prob.timing()
-        mesh.accumulate(rows)
-        Gi, Gj, Gv, B = mesh.csr()
-        u, info = gmres(Gi, Gj, Gv, B, tol=GMRES_TOL, max_iter=20000, restart=200, u0=u0, device=True)
+        u, info = system.solve(tol=GMRES_TOL, max_iter=20000, restart=200, u0=u0)
+        t2 = time.perf_counter()
+        Gi, Gj, Gv, B = system.read()
         u0 = u
         us.append(u)
         mats.append((sp.csr_matrix((Gv, Gj, Gi), shape=(n, n)), B))
-        out.append(dict(ms=t.total_ms, path_ms=t.path_kernel_ms, steps=int(sum(x["path_steps"] for x in rows["stats"])),
-                        fallbacks=int(sum(x["cv_fallback"] for x in rows["stats"])), gmres_ms=info.seconds * 1e3,
-                        gmres_device_ms=info.device_ms, gmres_it=int(info.iterations), converged=bool(info.converged)))
+        out.append(dict(ms=t.total_ms, path_ms=t.path_kernel_ms, steps=int(st["path_steps"].sum()),
+                        fallbacks=int(st["cv_fallback"].sum()), gmres_ms=info.seconds * 1e3,
+                        gmres_device_ms=info.device_ms, gmres_it=int(info.iterations), converged=bool(info.converged),
+                        assemble_wall_ms=(t1 - t0) * 1e3, wall_ms=(t2 - t0) * 1e3))
     U = np.array(us)
     ubar = U.mean(0)
     rms_u = float(np.sqrt(U.var(0, ddof=1).mean()))
@@ -364,7 +369,8 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
         paths, hist, sets = probe_paths, [], []
         for it in range(max_refine + 1):
             reps = n_rep if it == 0 else refine_rep
-            runs, rms, row, ubar = _replicas(mesh, prob, paths, h, [seed0 + 1000 * it + k for k in range(reps)])
+            runs, rms, row, ubar = _replicas(mesh, system, prob, paths, h,
+                                             [seed0 + 1000 * it + k for k in range(reps)])
             hist.append({"paths_per_row": paths, "rms": rms, "row_residual_rms": row, "replicas": reps})
             sets.append((paths, rms, reps))
             if it == 0:
@@ -389,15 +395,18 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
              "path_kernel_ms": float(np.mean([x["path_ms"] for x in runs])),
              "gmres_ms": float(np.mean([x["gmres_ms"] for x in runs])),
              "gmres_device_ms": float(np.mean([x["gmres_device_ms"] for x in runs])),
+             "assemble_wall_ms": float(np.mean([x["assemble_wall_ms"] for x in runs])),
+             "wall_ms": float(np.mean([x["wall_ms"] for x in runs])),
              "gmres_iterations": runs[0]["gmres_it"], "path_steps": runs[0]["steps"],
              "fallback_rows": runs[0]["fallbacks"], "rms_vs_exact": float(np.sqrt(np.mean((ubar - exact) ** 2)))}
         r.update(extra)
         log(f"ttr {name}: {r}")
         return r
 
+    system = DeviceSystem(mesh)
     with FKProblem(desc) as prob:
         prob.set_control_variate(abi.SPAC_CV_NONE)
-        prob.assemble_rows(mesh, 16, h, seed=1)  # warm-up
+        prob.assemble_system(mesh, system, 16, h, seed=1)  # warm-up
         res["plain"] = measure(prob, "plain", {})
     # learned control variate: training data, training, tabulation
     g = (np.arange(64) + 0.5) / 64
@@ -429,14 +438,17 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
     setup = L["training_data_ms"] + L["train_ms"] + L["table_ms"]
     res["time_to_rms_ms"] = {"plain_mc": P["mc_ms"], "learned_mc": L["mc_ms"], "learned_setup": setup,
                              "plain_with_gmres": P["mc_ms"] + P["gmres_ms"],
-                             "learned_with_setup_and_gmres": setup + L["mc_ms"] + L["gmres_ms"]}
+                             "learned_with_setup_and_gmres": setup + L["mc_ms"] + L["gmres_ms"],
+                             "plain_wall": P["wall_ms"], "learned_wall_with_setup": setup + L["wall_ms"]}
     res["row_variance_ratio_at_probe"] = (P["probe_row_residual_rms"] / L["probe_row_residual_rms"]) ** 2
     res["speedup"] = {"mc_only": P["mc_ms"] / L["mc_ms"],
                       "including_training": P["mc_ms"] / (setup + L["mc_ms"]),
-                      "end_to_end_with_gmres": (P["mc_ms"] + P["gmres_ms"]) / (setup + L["mc_ms"] + L["gmres_ms"])}
+                      "end_to_end_with_gmres": (P["mc_ms"] + P["gmres_ms"]) / (setup + L["mc_ms"] + L["gmres_ms"]),
+                      "end_to_end_wall": P["wall_ms"] / (setup + L["wall_ms"])}
     Pt, Lt = P["mc_ms_at_target"], L["mc_ms_at_target"]
     res["speedup_at_target"] = {"mc_only": Pt / Lt, "including_training": Pt / (setup + Lt),
                                 "end_to_end_with_gmres": (Pt + P["gmres_ms"]) / (setup + Lt + L["gmres_ms"])}
+    system.close()
     mesh.close()
     return res
 

@@ -233,6 +233,19 @@ spac_status spac_fk_simulate(spac_problem* p, const spac_solve_batch* batch, uin
 spac_status spac_fk_assemble_rows(spac_problem* p, const spac_interface_table* ifaces,
                                   const spac_node_batch* nodes, uint64_t seed, spac_row_block* out);
 
+/* A mesh's interface system resident on the device (created by spac_mesh_system_create,
+ * spac_mesh.h). */
+typedef struct spac_system spac_system;
+
+/* spac_fk_assemble_rows for every job of the system's mesh (nodes: the mesh's jobs in job
+ * order), accumulated into the device-resident system instead of copied out: the rows path of
+ * Reduce_Analytic + SistemaLineal::Actualiza_GB (Solvers/FeynmanKacSolver.cpp:233-292,
+ * Mesh/SistemaLineal.cpp:68-88) without the host round trip.  stats (optional): the nodes->n
+ * estimator records.  The timing of the call (spac_fk_timing) includes the accumulation. */
+spac_status spac_fk_assemble_system(spac_problem* p, const spac_interface_table* ifaces,
+                                    const spac_node_batch* nodes, uint64_t seed, spac_system* sys,
+                                    spac_point_stats* stats);
+
 /* Timing of the last call's kernels, measured with HIP events on the handle's
  * stream: path kernel, reduction kernels (ms), and launches counted. */
 typedef struct {

@@ -69,6 +69,19 @@ spac_status spac_mesh_reset_system(spac_mesh* m);
 spac_status spac_mesh_accumulate(spac_mesh* m, const spac_row_block* rows);
 /* CSR export: G_i[n+1], G_j[nnz], G_ij[nnz], B[n] (any pointer may be NULL). */
 spac_status spac_mesh_csr(const spac_mesh* m, int32_t* G_i, int32_t* G_j, double* G_ij, double* B);
+/* Device-resident interface system: the mesh's CSR pattern, its initial values (as after
+ * spac_mesh_reset_system) and each job's first slot in its row, on the device.  No counterpart
+ * in the reference, whose rows travel through host memory into SistemaLineal::Actualiza_GB
+ * (Mesh/SistemaLineal.cpp:68-88) before GMRES reads the host CSR.  spac_fk_assemble_system
+ * accumulates rows into it with Actualiza_GB's arithmetic (each G slot: its initial value plus
+ * the one job contribution it receives; B: the row's jobs summed in job order), and
+ * spac_gmres_solve_system solves it in place.  Creation fails with SPAC_E_PATTERN for a mesh
+ * whose jobs would add twice into one CSR slot (interface node sets are disjoint, so a square
+ * mesh never does). */
+spac_status spac_mesh_system_create(const spac_mesh* m, spac_system** out);
+spac_status spac_mesh_system_destroy(spac_system* s);
+/* Copy the system out: G_i[n+1], G_j[nnz], G_ij[nnz], B[n] (any pointer may be NULL). */
+spac_status spac_mesh_system_read(const spac_system* s, int32_t* G_i, int32_t* G_j, double* G_ij, double* B);
 spac_status spac_mesh_write_coo(const spac_mesh* m, const char* G_path, const char* B_path);
 spac_status spac_mesh_write_csr(const spac_mesh* m, const char* dir);
 /* Escribe_Posiciones with u = the registered coefficient set's exact solution. */
@@ -91,14 +104,19 @@ spac_status spac_gmres_solve(int32_t n, const int32_t* G_i, const int32_t* G_j, 
 
 /* The same GMRES(m) with right Jacobi preconditioning on the GPU
  * (SistemaLinealSolverGMRES::Resuelve, Solvers/SistemaLinealSolverGMRES_OMP.hpp:519-577,
- * without its RAS preconditioner): FP64 CSR SpMV (one wavefront per row),
- * classical Gram–Schmidt applied twice (CGS2) for the Arnoldi basis, every
- * reduction in a fixed order; the (m+1) x m Hessenberg least-squares problem
- * and its Givens rotations on the host.  Host arrays in and out; u holds the
+ * without its RAS preconditioner): FP64 CSR SpMV (a few lanes per row),
+ * classical Gram–Schmidt for the Arnoldi basis (one pass, two from the first
+ * step with heavy cancellation), every reduction in a fixed order; the
+ * (m+1) x m Hessenberg least-squares problem and its Givens rotations on the
+ * host, a few steps behind the device.  Host arrays in and out; u holds the
  * initial guess on entry. */
 spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* G_i, const int32_t* G_j, const double* G_ij,
                                  const double* B, double* u, double tol, int32_t max_iter, int32_t restart,
                                  spac_gmres_info* info);
+/* spac_gmres_solve_gpu on a device-resident system (no upload); u (host, n_rows) holds the
+ * initial guess on entry and the solution on return. */
+spac_status spac_gmres_solve_system(spac_system* s, double* u, double tol, int32_t max_iter, int32_t restart,
+                                    spac_gmres_info* info);
 /* Read a COO file ("nnz n" then "i j v") into CSR (Lee_GB_COO). Call with
  * G_i == NULL to get n and nnz first. */
 spac_status spac_read_coo(const char* path, int32_t* n, int32_t* nnz, int32_t* G_i, int32_t* G_j, double* G_ij);

@@ -267,6 +267,8 @@ def load(path: Path | None = None) -> C.CDLL:
                                      C.POINTER(PointStats)]
     lib.spac_fk_assemble_rows.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch), C.c_uint64,
                                           C.POINTER(RowBlock)]
+    lib.spac_fk_assemble_system.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
+                                            C.c_uint64, C.c_void_p, C.POINTER(PointStats)]
     lib.spac_fk_last_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
     lib.spac_fk_set_launch.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
     lib.spac_fk_set_control_variate.argtypes = [C.c_void_p, C.POINTER(CvConfig)]

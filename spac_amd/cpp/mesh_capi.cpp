@@ -11,6 +11,7 @@
 #include "../../include/spac_mesh.h"
 #include "BVPs.hpp"
 #include "Malla.hpp"
+#include "fk_internal.h"
 #include "spac_error.h"
 
 namespace spac {
@@ -205,6 +206,121 @@ spac_status spac_mesh_csr(const spac_mesh* hm, int32_t* G_i, int32_t* G_j, doubl
     if (G_j) std::memcpy(G_j, s.G_j.data(), sizeof(int32_t) * s.G_j.size());
     if (G_ij) std::memcpy(G_ij, s.G_ij.data(), sizeof(double) * s.G_ij.size());
     if (B) std::memcpy(B, s.B.data(), sizeof(double) * s.B.size());
+    return SPAC_OK;
+}
+
+#define SYS_HIP_CHECK(expr)                                                                     \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess) {                                                                 \
+            d->release();                                                                       \
+            delete d;                                                                           \
+            return set_error(SPAC_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e));        \
+        }                                                                                       \
+    } while (0)
+
+spac_status spac_mesh_system_create(const spac_mesh* hm, spac_system** out) {
+    if (!hm || !out) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(SPAC_E_NO_DEVICE, "no HIP device visible");
+    const MeshHandle* h = reinterpret_cast<const MeshHandle*>(hm);
+    const SistemaLineal& s = h->malla.sistema;
+    const int n = s.n_filas;
+    const int64_t nnz = s.G_i[n];
+    /* the values right after Inicializa_CSR (SistemaLineal.cpp: diagonal = containing circles) */
+    std::vector<double> init((size_t)nnz);
+    for (int r = 0; r < n; ++r)
+        for (int j = s.G_i[r]; j < s.G_i[r + 1]; ++j) init[j] = (s.G_j[j] == r) ? s.diagonal[r] : 1.0E-23;
+    /* each job's first slot, with Actualiza_GB's checks, and at most one job per slot */
+    const int nj = (int)h->jobs.size();
+    std::vector<int64_t> slot(nj, -1);
+    std::vector<int32_t> job_row(nj), job_iface(nj), job_m(nj), job_front(nj);
+    std::vector<uint8_t> hit((size_t)nnz, 0);
+    for (int k = 0; k < nj; ++k) {
+        const TrabajoFila& jb = h->jobs[k];
+        const int r = jb.nudo_global;
+        const int o = h->node_offset[jb.interfaz], m = h->node_offset[jb.interfaz + 1] - o;
+        job_row[k] = r;
+        job_iface[k] = jb.interfaz;
+        job_m[k] = m;
+        job_front[k] = jb.frontera ? 1 : 0;
+        if (jb.frontera) continue;
+        if (m <= 0) continue;
+        const int32_t* cols = h->node_global.data() + o;
+        int j = s.G_i[r];
+        for (; j < s.G_i[r + 1]; ++j)
+            if (s.G_j[j] == cols[0]) break;
+        if (j == s.G_i[r + 1] || j + m > s.G_i[r + 1])
+            return set_error(SPAC_E_PATTERN, "row %d: G columns outside the CSR pattern", r);
+        for (int q = 0; q < m; ++q) {
+            if (s.G_j[j + q] != cols[q]) return set_error(SPAC_E_PATTERN, "row %d: G columns not contiguous", r);
+            if (hit[j + q]++) return set_error(SPAC_E_PATTERN, "row %d: two jobs add into CSR slot %d", r, j + q);
+        }
+        slot[k] = j;
+    }
+    /* the jobs of every row, in job order (B sums them in that order) */
+    std::vector<int32_t> rptr(n + 1, 0), rjobs(nj), cur;
+    for (int k = 0; k < nj; ++k) rptr[job_row[k] + 1]++;
+    for (int r = 0; r < n; ++r) rptr[r + 1] += rptr[r];
+    cur.assign(rptr.begin(), rptr.end() - 1);
+    for (int k = 0; k < nj; ++k) rjobs[cur[job_row[k]]++] = k;
+
+    DeviceSystem* d = new DeviceSystem();
+    d->n_rows = n;
+    d->n_jobs = nj;
+    d->nnz = nnz;
+    SYS_HIP_CHECK(hipGetDevice(&d->device));
+    spac_status rc;
+    if ((rc = d->Ip.ensure((size_t)(n + 1) * 4)) || (rc = d->J.ensure((size_t)nnz * 4 + 4)) ||
+        (rc = d->init.ensure((size_t)nnz * 8 + 8)) || (rc = d->vals.ensure((size_t)nnz * 8 + 8)) ||
+        (rc = d->B.ensure((size_t)n * 8)) || (rc = d->job_slot.ensure((size_t)nj * 8 + 8)) ||
+        (rc = d->row_job_ptr.ensure((size_t)(n + 1) * 4)) || (rc = d->row_jobs.ensure((size_t)nj * 4 + 4))) {
+        d->release();
+        delete d;
+        return rc;
+    }
+    SYS_HIP_CHECK(hipMemcpy(d->Ip.ptr, s.G_i.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        SYS_HIP_CHECK(hipMemcpy(d->J.ptr, s.G_j.data(), (size_t)nnz * 4, hipMemcpyHostToDevice));
+        SYS_HIP_CHECK(hipMemcpy(d->init.ptr, init.data(), (size_t)nnz * 8, hipMemcpyHostToDevice));
+        SYS_HIP_CHECK(hipMemcpy(d->vals.ptr, init.data(), (size_t)nnz * 8, hipMemcpyHostToDevice));
+    }
+    SYS_HIP_CHECK(hipMemset(d->B.ptr, 0, (size_t)n * 8));
+    if (nj > 0) {
+        SYS_HIP_CHECK(hipMemcpy(d->job_slot.ptr, slot.data(), (size_t)nj * 8, hipMemcpyHostToDevice));
+        SYS_HIP_CHECK(hipMemcpy(d->row_jobs.ptr, rjobs.data(), (size_t)nj * 4, hipMemcpyHostToDevice));
+    }
+    SYS_HIP_CHECK(hipMemcpy(d->row_job_ptr.ptr, rptr.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    d->h_job_row = std::move(job_row);
+    d->h_job_iface = std::move(job_iface);
+    d->h_job_m = std::move(job_m);
+    d->h_job_frontera = std::move(job_front);
+    d->h_Ip.assign(s.G_i.begin(), s.G_i.end());
+    d->h_J.assign(s.G_j.begin(), s.G_j.end());
+    d->h_node_offset.assign(h->node_offset.begin(), h->node_offset.end());
+    d->h_node_global.assign(h->node_global.begin(), h->node_global.end());
+    *out = reinterpret_cast<spac_system*>(d);
+    return SPAC_OK;
+}
+
+spac_status spac_mesh_system_destroy(spac_system* hs) {
+    DeviceSystem* d = reinterpret_cast<DeviceSystem*>(hs);
+    if (!d) return SPAC_OK;
+    d->release();
+    delete d;
+    return SPAC_OK;
+}
+
+spac_status spac_mesh_system_read(const spac_system* hs, int32_t* G_i, int32_t* G_j, double* G_ij, double* B) {
+    const DeviceSystem* d = reinterpret_cast<const DeviceSystem*>(hs);
+    if (!d) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (G_i) std::memcpy(G_i, d->h_Ip.data(), d->h_Ip.size() * 4);
+    if (G_j) std::memcpy(G_j, d->h_J.data(), d->h_J.size() * 4);
+    if (G_ij && d->nnz > 0 && hipMemcpy(G_ij, d->vals.ptr, (size_t)d->nnz * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(SPAC_E_HIP, "reading the system values failed");
+    if (B && hipMemcpy(B, d->B.ptr, (size_t)d->n_rows * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(SPAC_E_HIP, "reading the system B failed");
     return SPAC_OK;
 }
 

@@ -8,6 +8,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/spac_fk.h"
 
@@ -98,6 +99,26 @@ struct Problem {
 /* Install a control variate; the table (cfg->table ignored) is copied from
  * src, a host pointer or (src_on_device) a device pointer. */
 spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* src, bool src_on_device);
+
+/* A mesh's interface system held on the device: the CSR pattern, the values right after
+ * SistemaLineal::Inicializa_CSR (diagonal = containing circles, 1e-23 elsewhere), each job's
+ * first slot in its row, and its rows' jobs in job order.  Rows assembled on the device are
+ * accumulated here and solved here (spac_fk_assemble_system, spac_gmres_solve_system). */
+struct DeviceSystem {
+    int32_t n_rows = 0, n_jobs = 0;
+    int64_t nnz = 0;
+    int device = 0;
+    DeviceBuffer Ip, J, init, vals, B; /* int32 [n_rows + 1], int32 [nnz], f64 [nnz] x 2, f64 [n_rows] */
+    DeviceBuffer job_slot;             /* int64 [n_jobs]: first CSR slot of the job's G, -1: none */
+    DeviceBuffer row_job_ptr, row_jobs; /* int32 [n_rows + 1], int32 [n_jobs]: jobs of a row, in order */
+    std::vector<int32_t> h_job_row;    /* host copy: row of every job */
+    std::vector<int32_t> h_job_iface, h_job_m, h_job_frontera; /* interface, G length, boundary flag */
+    std::vector<int32_t> h_Ip, h_J;    /* host copy of the pattern (reads) */
+    std::vector<int32_t> h_node_offset, h_node_global; /* the mesh's interface nodes (batch check) */
+    void release() {
+        for (DeviceBuffer* b : {&Ip, &J, &init, &vals, &B, &job_slot, &row_job_ptr, &row_jobs}) b->release();
+    }
+};
 
 } // namespace spac
 

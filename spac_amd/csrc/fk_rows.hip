@@ -594,6 +594,99 @@ extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interf
     return rows_finish(p, rs, out);
 }
 
+/* ------------------------------------------------ device-resident system -- */
+namespace spac {
+/* G of every job into its CSR slots: the slot's initial value plus the job's value (each slot
+ * receives one job, checked at system creation), Actualiza_GB's single += per slot. */
+__global__ void __launch_bounds__(64) k_system_g(int32_t n_jobs, const int64_t* __restrict__ g_off,
+                                                 const int64_t* __restrict__ slot, const double* __restrict__ G,
+                                                 const double* __restrict__ init, double* __restrict__ vals) {
+    const int k = blockIdx.x;
+    if (k >= n_jobs) return;
+    const int64_t s = slot[k];
+    if (s < 0) return;
+    const int64_t o = g_off[k];
+    const int m = (int)(g_off[k + 1] - o);
+    for (int q = threadIdx.x; q < m; q += blockDim.x) vals[s + q] = init[s + q] + G[o + q];
+}
+/* B of every row: 0 + the B of its jobs, in job order (Actualiza_GB's B[fila] += b) */
+__global__ void k_system_b(int32_t n_rows, const int32_t* __restrict__ row_job_ptr, const int32_t* __restrict__ row_jobs,
+                           const double* __restrict__ Bj, double* __restrict__ B) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    double b = 0.0;
+    for (int i = row_job_ptr[r]; i < row_job_ptr[r + 1]; ++i) b += Bj[row_jobs[i]];
+    B[r] = b;
+}
+} // namespace spac
+
+extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_interface_table* t,
+                                               const spac_node_batch* nb, uint64_t seed, spac_system* hsys,
+                                               spac_point_stats* stats) {
+    if (!hp || !hsys) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    spac_status rc = validate(t, nb);
+    if (rc) return rc;
+    DeviceSystem* sys = reinterpret_cast<DeviceSystem*>(hsys);
+    if (nb->n != sys->n_jobs)
+        return set_error(SPAC_E_INVALID_ARG, "the node batch has %d jobs, the system's mesh %d", nb->n, sys->n_jobs);
+    /* the batch must be the mesh's jobs: each job's G goes into the slots validated for it */
+    for (int k = 0; k < nb->n; ++k) {
+        const int f = nb->iface[k];
+        if (f != sys->h_job_iface[k] || (nb->frontera[k] ? 1 : 0) != sys->h_job_frontera[k] ||
+            t->node_offset[f + 1] - t->node_offset[f] != sys->h_job_m[k])
+            return set_error(SPAC_E_INVALID_ARG, "job %d of the batch is not the system mesh's job %d", k, k);
+    }
+    /* and the table the mesh's interfaces: G columns come from it */
+    const size_t K = sys->h_node_offset.size() - 1;
+    if ((size_t)t->n_interfaces != K || std::memcmp(t->node_offset, sys->h_node_offset.data(), (K + 1) * 4) != 0 ||
+        std::memcmp(t->node_global, sys->h_node_global.data(), sys->h_node_global.size() * 4) != 0)
+        return set_error(SPAC_E_INVALID_ARG, "the interface table is not the system mesh's");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    if (p->device != sys->device) return set_error(SPAC_E_INVALID_ARG, "problem and system are on different devices");
+    RowsSetup rs;
+    if ((rc = rows_prepare(p, t, nb, &rs))) return rc;
+    hipStream_t s = p->stream;
+    int blocks = 0;
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[0], s));
+    if ((rc = p->enqueue_paths(rs.ss, seed, nullptr, &blocks, p->ev[1]))) return rc;
+    if ((rc = p->enqueue_reduce(rs.ss, rs.d_stats, 1))) return rc;
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[2], s));
+    if ((rc = rows_enqueue_bg(p, rs))) return rc;
+    const int n = rs.n;
+    if (sys->nnz > 0)
+        ROWS_HIP_CHECK(hipMemcpyAsync(sys->vals.ptr, sys->init.ptr, (size_t)sys->nnz * 8, hipMemcpyDeviceToDevice, s));
+    if (n > 0)
+        hipLaunchKernelGGL(k_system_g, dim3(n), dim3(64), 0, s, n, rs.ra.g_off, (const int64_t*)sys->job_slot.ptr,
+                           (const double*)rs.ra.G, (const double*)sys->init.ptr, (double*)sys->vals.ptr);
+    if (sys->n_rows > 0)
+        hipLaunchKernelGGL(k_system_b, dim3((sys->n_rows + 255) / 256), dim3(256), 0, s, sys->n_rows,
+                           (const int32_t*)sys->row_job_ptr.ptr, (const int32_t*)sys->row_jobs.ptr,
+                           (const double*)rs.d_B, (double*)sys->B.ptr);
+    ROWS_HIP_CHECK(hipGetLastError());
+    ROWS_HIP_CHECK(hipEventRecord(p->ev[3], s));
+    ROWS_HIP_CHECK(hipStreamSynchronize(s));
+    float ms_path = 0.f, ms_rest = 0.f, ms_all = 0.f;
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_path, p->ev[0], p->ev[1]));
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_rest, p->ev[1], p->ev[3]));
+    ROWS_HIP_CHECK(hipEventElapsedTime(&ms_all, p->ev[0], p->ev[3]));
+    p->last.path_kernel_ms = ms_path;
+    p->last.reduce_kernel_ms = ms_rest; /* reduction, B and G kernels, accumulation */
+    p->last.total_ms = ms_all;
+    p->last.path_kernel_launches = 1;
+    p->last.grid_blocks = blocks;
+    p->last.path_steps = 0;
+    std::vector<spac_point_stats> st(n);
+    if (n > 0)
+        ROWS_HIP_CHECK(hipMemcpy(st.data(), rs.d_stats, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
+    if (stats && n > 0) std::memcpy(stats, st.data(), (size_t)n * sizeof(spac_point_stats));
+    spac_status worst = SPAC_OK;
+    for (int k = 0; k < n; ++k)
+        if (st[k].status < worst) worst = st[k].status;
+    if (worst == SPAC_E_NAN) return set_error(SPAC_E_NAN, "NaN reached a row estimator");
+    if (worst == SPAC_E_MAX_STEPS) return set_error(SPAC_E_MAX_STEPS, "paths stopped by max_steps");
+    return SPAC_OK;
+}
+
 /* ------------------------------------------------------------- plans -- */
 namespace spac {
 struct RowsPlan {

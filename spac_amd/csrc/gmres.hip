@@ -24,6 +24,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -198,19 +199,39 @@ struct GmresState {
 
 using namespace spac;
 
-extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const int32_t* J, const double* V,
-                                            const double* B, double* u, double tol, int32_t max_iter,
-                                            int32_t restart, spac_gmres_info* info) {
-    if (n < 1 || !Ip || !J || !V || !B || !u) return set_error(SPAC_E_INVALID_ARG, "bad GMRES arguments");
+/* The solve on either a host CSR (validated, uploaded into the state's buffers) or a
+ * device-resident system (sys: its arrays are used in place; its pattern came from a mesh). */
+static spac_status gmres_impl(int32_t n, const int32_t* Ip, const int32_t* J, const double* V, const double* B,
+                              const DeviceSystem* sys, double* u, double tol, int32_t max_iter, int32_t restart,
+                              spac_gmres_info* info) {
+    if (sys) n = sys->n_rows;
+    if (n < 1 || !u || (!sys && (!Ip || !J || !V || !B))) return set_error(SPAC_E_INVALID_ARG, "bad GMRES arguments");
     if (restart < 1) restart = 50;
     if (restart > 1000) return set_error(SPAC_E_INVALID_ARG, "restart must be <= 1000");
     if (max_iter < 1) max_iter = 1000;
-    if (Ip[0] != 0 || Ip[n] < 0) return set_error(SPAC_E_INVALID_ARG, "CSR row pointer must start at 0");
-    for (int32_t i = 0; i < n; ++i)
-        if (Ip[i + 1] < Ip[i]) return set_error(SPAC_E_INVALID_ARG, "CSR row pointer decreases at row %d", i);
-    const int64_t nnz = Ip[n];
-    for (int64_t k = 0; k < nnz; ++k)
-        if (J[k] < 0 || J[k] >= n) return set_error(SPAC_E_INVALID_ARG, "CSR column %d out of range at %lld", J[k], (long long)k);
+    int64_t nnz = 0;
+    if (sys) {
+        nnz = sys->nnz;
+    } else {
+        if (Ip[0] != 0 || Ip[n] < 0) return set_error(SPAC_E_INVALID_ARG, "CSR row pointer must start at 0");
+        for (int32_t i = 0; i < n; ++i)
+            if (Ip[i + 1] < Ip[i]) return set_error(SPAC_E_INVALID_ARG, "CSR row pointer decreases at row %d", i);
+        nnz = Ip[n];
+        /* column range: a branch-free min / max pass (vectorised), the offending index only on error */
+        int32_t jmin = 0, jmax = 0;
+        if (nnz > 0) {
+            jmin = J[0];
+            jmax = J[0];
+            for (int64_t k = 1; k < nnz; ++k) {
+                jmin = std::min(jmin, J[k]);
+                jmax = std::max(jmax, J[k]);
+            }
+        }
+        if (jmin < 0 || jmax >= n)
+            for (int64_t k = 0; k < nnz; ++k)
+                if (J[k] < 0 || J[k] >= n)
+                    return set_error(SPAC_E_INVALID_ARG, "CSR column %d out of range at %lld", J[k], (long long)k);
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(SPAC_E_NO_DEVICE, "no HIP device visible");
     const auto t_start = std::chrono::steady_clock::now();
@@ -221,9 +242,9 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     const size_t hlen = 2 * (size_t)(m + 1) + 1;
     GmresState st;
     spac_status rc;
-    if ((rc = st.Ip.ensure((N + 1) * 4)) || (rc = st.J.ensure((size_t)nnz * 4 + 4)) ||
-        (rc = st.V.ensure((size_t)nnz * 8 + 8)) || (rc = st.B.ensure(N * 8)) || (rc = st.u.ensure(N * 8)) ||
-        (rc = st.dinv.ensure(N * 8)) || (rc = st.Vb.ensure((size_t)(m + 1) * N * 8)) ||
+    if ((!sys && ((rc = st.Ip.ensure((N + 1) * 4)) || (rc = st.J.ensure((size_t)nnz * 4 + 4)) ||
+                  (rc = st.V.ensure((size_t)nnz * 8 + 8)) || (rc = st.B.ensure(N * 8)))) ||
+        (rc = st.u.ensure(N * 8)) || (rc = st.dinv.ensure(N * 8)) || (rc = st.Vb.ensure((size_t)(m + 1) * N * 8)) ||
         (rc = st.r.ensure(N * 8)) || (rc = st.part.ensure((size_t)(m + 1) * nchunks * 8)) ||
         (rc = st.h.ensure(hlen * 8)) || (rc = st.nrm.ensure(64)) || (rc = st.z.ensure(N * 8))) {
         st.release();
@@ -236,17 +257,19 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     GM_CHECK(hipHostMalloc((void**)&st.col, (size_t)m * hlen * 8, hipHostMallocMapped));
     GM_CHECK(hipHostGetDevicePointer((void**)&st.col_dev, st.col, 0));
     hipStream_t s = st.stream;
-    GM_CHECK(hipMemcpyAsync(st.Ip.ptr, Ip, (N + 1) * 4, hipMemcpyHostToDevice, s));
-    GM_CHECK(hipMemcpyAsync(st.J.ptr, J, (size_t)nnz * 4, hipMemcpyHostToDevice, s));
-    GM_CHECK(hipMemcpyAsync(st.V.ptr, V, (size_t)nnz * 8, hipMemcpyHostToDevice, s));
-    GM_CHECK(hipMemcpyAsync(st.B.ptr, B, N * 8, hipMemcpyHostToDevice, s));
+    if (!sys) {
+        GM_CHECK(hipMemcpyAsync(st.Ip.ptr, Ip, (N + 1) * 4, hipMemcpyHostToDevice, s));
+        GM_CHECK(hipMemcpyAsync(st.J.ptr, J, (size_t)nnz * 4, hipMemcpyHostToDevice, s));
+        GM_CHECK(hipMemcpyAsync(st.V.ptr, V, (size_t)nnz * 8, hipMemcpyHostToDevice, s));
+        GM_CHECK(hipMemcpyAsync(st.B.ptr, B, N * 8, hipMemcpyHostToDevice, s));
+    }
     GM_CHECK(hipMemcpyAsync(st.u.ptr, u, N * 8, hipMemcpyHostToDevice, s));
     GM_CHECK(hipEventRecord(st.e0, s));
 
-    const int32_t* dIp = (const int32_t*)st.Ip.ptr;
-    const int32_t* dJ = (const int32_t*)st.J.ptr;
-    const double* dV = (const double*)st.V.ptr;
-    const double* dB = (const double*)st.B.ptr;
+    const int32_t* dIp = (const int32_t*)(sys ? sys->Ip.ptr : st.Ip.ptr);
+    const int32_t* dJ = (const int32_t*)(sys ? sys->J.ptr : st.J.ptr);
+    const double* dV = (const double*)(sys ? sys->vals.ptr : st.V.ptr);
+    const double* dB = (const double*)(sys ? sys->B.ptr : st.B.ptr);
     double* du = (double*)st.u.ptr;
     double* dinv = (double*)st.dinv.ptr;
     double* Vb = (double*)st.Vb.ptr;
@@ -395,4 +418,17 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
     }
     st.release();
     return SPAC_OK;
+}
+
+extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const int32_t* J, const double* V,
+                                            const double* B, double* u, double tol, int32_t max_iter,
+                                            int32_t restart, spac_gmres_info* info) {
+    return gmres_impl(n, Ip, J, V, B, nullptr, u, tol, max_iter, restart, info);
+}
+
+extern "C" spac_status spac_gmres_solve_system(spac_system* hs, double* u, double tol, int32_t max_iter,
+                                               int32_t restart, spac_gmres_info* info) {
+    if (!hs) return set_error(SPAC_E_INVALID_ARG, "null system");
+    return gmres_impl(0, nullptr, nullptr, nullptr, nullptr, reinterpret_cast<const DeviceSystem*>(hs), u, tol,
+                      max_iter, restart, info);
 }

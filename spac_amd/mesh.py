@@ -74,6 +74,10 @@ def _setup(lib):
     lib.spac_gmres_solve.argtypes = [C.c_int32, abi.i32p, abi.i32p, abi.dp, abi.dp, abi.dp, C.c_double, C.c_int32,
                                      C.c_int32, C.POINTER(GmresInfo)]
     lib.spac_gmres_solve_gpu.argtypes = lib.spac_gmres_solve.argtypes
+    lib.spac_mesh_system_create.argtypes = [vp, C.POINTER(vp)]
+    lib.spac_mesh_system_destroy.argtypes = [vp]
+    lib.spac_mesh_system_read.argtypes = [vp, abi.i32p, abi.i32p, abi.dp, abi.dp]
+    lib.spac_gmres_solve_system.argtypes = [vp, abi.dp, C.c_double, C.c_int32, C.c_int32, C.POINTER(GmresInfo)]
     lib.spac_read_coo.argtypes = [C.c_char_p, abi.i32p, abi.i32p, abi.i32p, abi.i32p, abi.dp]
     lib._mesh_ready = True
     return lib
@@ -211,6 +215,55 @@ class Mesh:
         abi.check(self.lib.spac_mesh_csr(self.h, abi.ptr(Gi, C.c_int32), abi.ptr(Gj, C.c_int32),
                                          abi.ptr(Gv, C.c_double), abi.ptr(B, C.c_double)), self.lib)
         return Gi, Gj, Gv, B
+
+
+class DeviceSystem:
+    """A mesh's interface system resident on the device (spac_mesh_system_create): rows assembled
+    by FKProblem.assemble_system land in it and solve() runs GPU GMRES on it in place."""
+
+    def __init__(self, mesh: Mesh):
+        self.lib = mesh.lib
+        self.mesh = mesh
+        h = C.c_void_p()
+        abi.check(self.lib.spac_mesh_system_create(mesh.h, C.byref(h)), self.lib)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.spac_mesh_system_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def read(self):
+        """(G_i, G_j, G_ij, B) copied out of the device."""
+        n, nnz = self.mesh.info.n_nodes, self.mesh.info.nnz
+        Gi = np.zeros(n + 1, np.int32)
+        Gj = np.zeros(nnz, np.int32)
+        Gv = np.zeros(nnz)
+        B = np.zeros(n)
+        abi.check(self.lib.spac_mesh_system_read(self.h, abi.ptr(Gi, C.c_int32), abi.ptr(Gj, C.c_int32),
+                                                 abi.ptr(Gv, C.c_double), abi.ptr(B, C.c_double)), self.lib)
+        return Gi, Gj, Gv, B
+
+    def solve(self, tol=1e-10, max_iter=2000, restart=100, u0=None):
+        """spac_gmres_solve_system; u0: initial guess (zeros by default)."""
+        n = self.mesh.info.n_nodes
+        u = np.zeros(n) if u0 is None else np.array(u0, dtype=np.float64)
+        info = GmresInfo()
+        abi.check(self.lib.spac_gmres_solve_system(self.h, abi.ptr(u, C.c_double), tol, max_iter, restart,
+                                                   C.byref(info)), self.lib)
+        return u, info
 
 
 def gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=2000, restart=100, u0=None, device=False):

@@ -1,0 +1,101 @@
+"""Device-resident interface system: rows accumulated and solved without a host round trip.
+
+The device system must hold exactly what the host path builds from the same rows:
+spac_fk_assemble_rows -> spac_mesh_accumulate (SistemaLineal::Actualiza_GB,
+Mesh/SistemaLineal.cpp:68-88) -> spac_mesh_csr.  G_ij, B and the estimator records are compared
+bit for bit (one add per G slot, B summed in job order on both sides), and GMRES on the
+resident system must return the same iterate as spac_gmres_solve_gpu on the copied-out CSR.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from spac_amd import abi
+from spac_amd.fk import FKProblem, problem_desc
+from spac_amd.mesh import DeviceSystem, Mesh, gmres, mesh_desc
+
+VC = abi.SPAC_COEF_VARCOEF_FRANKE
+DOM = (0.0, 0.0, 1.0, 1.0)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_system_calls_without_handle_fail_cleanly():
+    lib = abi.load()
+    from spac_amd import mesh as meshmod
+    meshmod._setup(lib)
+    out = C.c_void_p()
+    assert lib.spac_mesh_system_create(None, C.byref(out)) == abi.SPAC_E_INVALID_ARG
+    assert lib.spac_mesh_system_destroy(None) == abi.SPAC_OK
+    assert lib.spac_mesh_system_read(None, None, None, None, None) == abi.SPAC_E_INVALID_ARG
+    assert lib.spac_gmres_solve_system(None, None, 1e-8, 10, 10, None) == abi.SPAC_E_INVALID_ARG
+    assert lib.spac_fk_assemble_system(None, None, None, 0, None, None) == abi.SPAC_E_INVALID_ARG
+
+
+def test_system_create_without_device_is_a_status():
+    lib = abi.load()
+    if lib.spac_device_count() > 0:
+        pytest.skip("a device is visible here")
+    m = Mesh(mesh_desc())
+    with pytest.raises(abi.SpacError) as e:
+        DeviceSystem(m)
+    assert e.value.code == abi.SPAC_E_NO_DEVICE
+
+
+def _host_system(prob, mesh, paths, h, seed):
+    rows = prob.assemble_rows(mesh, paths, h, seed=seed)
+    mesh.accumulate(rows)
+    return mesh.csr(), rows
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [abi.SPAC_FP64, abi.SPAC_FP32])
+def test_gpu_system_equals_host_accumulation(gpu, precision):
+    for desc, paths, h in ((mesh_desc(), 300, 5e-4), (mesh_desc(n_side=16, nodes_per_circle=40), 100, 2e-4)):
+        m = Mesh(desc)
+        say(f"device system: precision {precision}, {m.info.n_jobs} jobs, {m.info.nnz} nnz")
+        with FKProblem(problem_desc(coef=VC, domain=DOM, precision=precision)) as p, DeviceSystem(m) as s:
+            for seed in (3, 4):  # the second call must start again from the initial values
+                (Gi, Gj, Gv, B), rows = _host_system(p, m, paths, h, seed)
+                st = p.assemble_system(m, s, paths, h, seed=seed)
+                Gi2, Gj2, Gv2, B2 = s.read()
+                assert np.array_equal(Gi, Gi2) and np.array_equal(Gj, Gj2)
+                assert np.array_equal(Gv.view(np.int64), Gv2.view(np.int64))
+                assert np.array_equal(B.view(np.int64), B2.view(np.int64))
+                for f in ("phi", "phiphi", "B", "paths", "path_steps", "n_domain_exits", "status"):
+                    ref = np.array([x[f] for x in rows["stats"]])
+                    assert np.array_equal(ref, st[f]), f
+                t = p.timing()
+                assert t.total_ms > 0 and t.path_kernel_ms > 0
+
+
+@pytest.mark.gpu
+def test_gpu_system_solve_equals_gmres_on_copied_csr(gpu):
+    m = Mesh(mesh_desc())
+    say(f"device system solve: {m.info.n_jobs} jobs")
+    with FKProblem(problem_desc(coef=VC, domain=DOM, precision=abi.SPAC_FP32)) as p, DeviceSystem(m) as s:
+        p.assemble_system(m, s, 2000, 5e-4, seed=0)
+        Gi, Gj, Gv, B = s.read()
+        u_ref, info_ref = gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=3000, restart=60, device=True)
+        u, info = s.solve(tol=1e-10, max_iter=3000, restart=60)
+        assert info.converged and info.iterations == info_ref.iterations
+        assert np.array_equal(u.view(np.int64), u_ref.view(np.int64))
+        # warm start from the solution: nothing left to do
+        u2, info2 = s.solve(tol=1e-10, restart=60, u0=u)
+        assert info2.converged and info2.iterations == 0
+
+
+@pytest.mark.gpu
+def test_gpu_system_rejects_another_mesh_batch(gpu):
+    m4 = Mesh(mesh_desc())
+    m8 = Mesh(mesh_desc(n_side=8, nodes_per_circle=40))
+    say("device system: batch of another mesh")
+    with FKProblem(problem_desc(coef=VC, domain=DOM)) as p, DeviceSystem(m4) as s:
+        with pytest.raises(abi.SpacError) as e:
+            p.assemble_system(m8, s, 10, 1e-3, seed=0)
+        assert e.value.code == abi.SPAC_E_INVALID_ARG
