@@ -298,9 +298,10 @@ def _replicas(mesh, system, prob, paths, h, seeds, u0=None):
 
     out, us, mats = [], [], []
     n = mesh.info.n_nodes
-    for s in seeds:
+    for k, s in enumerate(seeds):
         t0 = time.perf_counter()
-        st = prob.assemble_system(mesh, system, paths, h, seed=s)
+        # the estimator records (path steps, fallbacks) of the set's first replica only
+        st = prob.assemble_system(mesh, system, paths, h, seed=s, stats=k == 0)
         t1 = time.perf_counter()
         t = prob.timing()
         u, info = system.solve(tol=GMRES_TOL, max_iter=20000, restart=200, u0=u0)
@@ -309,8 +310,10 @@ def _replicas(mesh, system, prob, paths, h, seeds, u0=None):
         u0 = u
         us.append(u)
         mats.append((sp.csr_matrix((Gv, Gj, Gi), shape=(n, n)), B))
-        out.append(dict(ms=t.total_ms, path_ms=t.path_kernel_ms, steps=int(st["path_steps"].sum()),
-                        fallbacks=int(st["cv_fallback"].sum()), gmres_ms=info.seconds * 1e3,
+        out.append(dict(ms=t.total_ms, path_ms=t.path_kernel_ms,
+                        steps=int(st["path_steps"].sum()) if st is not None else None,
+                        fallbacks=int(st["cv_fallback"].sum()) if st is not None else None,
+                        gmres_ms=info.seconds * 1e3,
                         gmres_device_ms=info.device_ms, gmres_it=int(info.iterations), converged=bool(info.converged),
                         assemble_wall_ms=(t1 - t0) * 1e3, wall_ms=(t2 - t0) * 1e3))
     U = np.array(us)

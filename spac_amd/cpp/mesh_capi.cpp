@@ -292,14 +292,32 @@ spac_status spac_mesh_system_create(const spac_mesh* hm, spac_system** out) {
         SYS_HIP_CHECK(hipMemcpy(d->row_jobs.ptr, rjobs.data(), (size_t)nj * 4, hipMemcpyHostToDevice));
     }
     SYS_HIP_CHECK(hipMemcpy(d->row_job_ptr.ptr, rptr.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    /* the interface table and the job arrays the row kernels read */
+    spac_interface_table tab;
+    spac_mesh_interfaces(hm, &tab);
+    std::vector<int32_t> jarr(2 * (size_t)nj);
+    std::vector<int64_t> goff(nj + 1, 0);
+    for (int k = 0; k < nj; ++k) {
+        jarr[k] = job_iface[k];
+        jarr[nj + k] = job_front[k];
+        goff[k + 1] = goff[k] + job_m[k];
+    }
+    if ((rc = iface_table_upload(&tab, nullptr, d->t_ifc, d->t_ifoff, d->t_ifnode, d->t_ifipsi)) ||
+        (rc = d->t_jobs.ensure(jarr.size() * 4 + 8)) || (rc = d->t_goff.ensure(goff.size() * 8)) ||
+        (rc = d->flag.ensure(64))) {
+        d->release();
+        delete d;
+        return rc;
+    }
+    if (nj > 0) SYS_HIP_CHECK(hipMemcpy(d->t_jobs.ptr, jarr.data(), jarr.size() * 4, hipMemcpyHostToDevice));
+    SYS_HIP_CHECK(hipMemcpy(d->t_goff.ptr, goff.data(), goff.size() * 8, hipMemcpyHostToDevice));
     d->h_job_row = std::move(job_row);
     d->h_job_iface = std::move(job_iface);
     d->h_job_m = std::move(job_m);
     d->h_job_frontera = std::move(job_front);
     d->h_Ip.assign(s.G_i.begin(), s.G_i.end());
     d->h_J.assign(s.G_j.begin(), s.G_j.end());
-    d->h_node_offset.assign(h->node_offset.begin(), h->node_offset.end());
-    d->h_node_global.assign(h->node_global.begin(), h->node_global.end());
+    d->h_table_sig = table_signature(&tab);
     *out = reinterpret_cast<spac_system*>(d);
     return SPAC_OK;
 }

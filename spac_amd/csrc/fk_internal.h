@@ -114,11 +114,25 @@ struct DeviceSystem {
     std::vector<int32_t> h_job_row;    /* host copy: row of every job */
     std::vector<int32_t> h_job_iface, h_job_m, h_job_frontera; /* interface, G length, boundary flag */
     std::vector<int32_t> h_Ip, h_J;    /* host copy of the pattern (reads) */
-    std::vector<int32_t> h_node_offset, h_node_global; /* the mesh's interface nodes (batch check) */
+    std::vector<char> h_table_sig; /* table_signature of the mesh's interface table (batch check) */
+    /* the mesh's interface table (iface_table_upload layout), its jobs' interface and boundary
+     * flag (int32 [n_jobs] each, one buffer) and G offsets (int64 [n_jobs + 1]) */
+    DeviceBuffer t_ifc, t_ifoff, t_ifnode, t_ifipsi, t_jobs, t_goff;
+    DeviceBuffer flag; /* int32: worst estimator status of the last assembly */
     void release() {
-        for (DeviceBuffer* b : {&Ip, &J, &init, &vals, &B, &job_slot, &row_job_ptr, &row_jobs}) b->release();
+        for (DeviceBuffer* b : {&Ip, &J, &init, &vals, &B, &job_slot, &row_job_ptr, &row_jobs, &t_ifc, &t_ifoff,
+                                &t_ifnode, &t_ifipsi, &t_jobs, &t_goff, &flag})
+            b->release();
     }
 };
+
+/* The parts of an interface table a row assembly reads on the host, plus its node lists and
+ * kinds, as bytes: two tables with equal signatures give the same rows (fk_rows.hip). */
+std::vector<char> table_signature(const spac_interface_table* t);
+
+/* Copy an interface table to the device (fk_rows.hip); returns once the copies are done. */
+spac_status iface_table_upload(const spac_interface_table* t, hipStream_t s, DeviceBuffer& ifc, DeviceBuffer& ifoff,
+                               DeviceBuffer& ifnode, DeviceBuffer& ifipsi);
 
 } // namespace spac
 

@@ -396,9 +396,77 @@ static size_t stencil_lds_bytes(const RowsSetup& rs) {
     return (size_t)kRowsBlock * rs.wstride * 8 + (size_t)kCopies * rs.mmax * sizeof(spac_ksum);
 }
 
+} // namespace spac
+
+/* The interface table on the device: cx, cy, c2, code in `ifc`; node_offset, then 8-aligned
+ * ipsi_offset in `ifoff`; node angles in `ifnode`; stencil / global inverses in `ifipsi`.
+ * Returns after the copies have finished (the host arrays may change afterwards). */
+spac_status spac::iface_table_upload(const spac_interface_table* t, hipStream_t s, DeviceBuffer& ifc,
+                                     DeviceBuffer& ifoff, DeviceBuffer& ifnode, DeviceBuffer& ifipsi) {
+    const int K = t->n_interfaces;
+    const int nodes = t->node_offset[K];
+    int64_t nipsi = 0;
+    for (int k = 0; k < K; ++k) {
+        if (t->ipsi_offset[k] < 0) continue;
+        int M = t->node_offset[k + 1] - t->node_offset[k];
+        int64_t sz;
+        if (t->rbf_kind == SPAC_RBF_IMQ_GLOBAL) sz = (int64_t)M * M;
+        else {
+            int S = std::min(t->stencil_size, M);
+            sz = (t->code[k] == 1 ? 1 : (M - S + 1)) * (int64_t)S * S;
+        }
+        nipsi = std::max(nipsi, t->ipsi_offset[k] + sz);
+    }
+    const size_t szK = (size_t)K;
+    const size_t noff_bytes = (((szK + 1) * 4 + 7) / 8) * 8;
+    spac_status rc;
+    if ((rc = ifc.ensure(szK * (3 * 8 + 4))) || (rc = ifnode.ensure((size_t)nodes * 8 + 8)) ||
+        (rc = ifipsi.ensure((size_t)std::max<int64_t>(nipsi, 1) * 8)) || (rc = ifoff.ensure(noff_bytes + szK * 8)))
+        return rc;
+    double* d_cx = (double*)ifc.ptr;
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_cx, t->cx, szK * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_cx + K, t->cy, szK * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_cx + 2 * K, t->c2, szK * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(d_cx + 3 * K, t->code, szK * 4, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(ifoff.ptr, t->node_offset, (szK + 1) * 4, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync((char*)ifoff.ptr + noff_bytes, t->ipsi_offset, szK * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(ifnode.ptr, t->node_theta, (size_t)nodes * 8, hipMemcpyHostToDevice, s));
+    if (nipsi > 0) ROWS_HIP_CHECK(hipMemcpyAsync(ifipsi.ptr, t->ipsi, (size_t)nipsi * 8, hipMemcpyHostToDevice, s));
+    ROWS_HIP_CHECK(hipStreamSynchronize(s));
+    return SPAC_OK;
+}
+
+std::vector<char> spac::table_signature(const spac_interface_table* t) {
+    const size_t K = (size_t)t->n_interfaces;
+    const size_t nodes = (size_t)t->node_offset[K];
+    std::vector<char> sig;
+    auto put = [&sig](const void* src, size_t bytes) {
+        const char* c = (const char*)src;
+        sig.insert(sig.end(), c, c + bytes);
+    };
+    put(&t->n_interfaces, 4);
+    put(&t->rbf_kind, 4);
+    put(&t->stencil_size, 4);
+    put(&t->gauss_eps, 8);
+    put(t->cx, K * 8);
+    put(t->cy, K * 8);
+    put(t->radius, K * 8);
+    put(t->c2, K * 8);
+    put(t->code, K * 4);
+    put(t->ipsi_offset, K * 8);
+    put(t->node_offset, (K + 1) * 4);
+    put(t->node_global, nodes * 4);
+    return sig;
+}
+
+namespace spac {
+
 /* Upload the node batch, interface table and work list into p's buffers and
- * allocate the records, so that rows_enqueue only launches kernels. */
-static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const spac_node_batch* nb, RowsSetup* rs) {
+ * allocate the records, so that rows_enqueue only launches kernels.  With sys (a device
+ * system of the batch's mesh), the interface table and the job arrays are the copies the
+ * system holds, and the host column list (gcol) is not built. */
+static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const spac_node_batch* nb, RowsSetup* rs,
+                                const DeviceSystem* sys = nullptr) {
     const int n = nb->n;
     std::vector<uint32_t> paths(n);
     std::vector<double> circle(3 * (size_t)n);
@@ -431,84 +499,74 @@ static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const
         /* the paired kernel's static LDS holds two rows' angles and stencil inverses */
         rs->pair_rows = pmax <= (uint32_t)kPairMaxPaths && stencil_lds_bytes(*rs) + 12 * 1024 <= (size_t)lds_max;
     }
-    rs->gcol.resize((size_t)rs->goff[n]);
-    for (int k = 0; k < n; ++k) {
-        int f = nb->iface[k];
-        int M = t->node_offset[f + 1] - t->node_offset[f];
-        for (int j = 0; j < M; ++j) rs->gcol[rs->goff[k] + j] = t->node_global[t->node_offset[f] + j];
+    if (!sys) {
+        rs->gcol.resize((size_t)rs->goff[n]);
+        for (int k = 0; k < n; ++k) {
+            int f = nb->iface[k];
+            int M = t->node_offset[f + 1] - t->node_offset[f];
+            for (int j = 0; j < M; ++j) rs->gcol[rs->goff[k] + j] = t->node_global[t->node_offset[f] + j];
+        }
     }
     spac_status rc;
     if ((rc = p->upload_solves(n, nb->x, nb->y, paths.data(), nb->h, nb->stream, circle.data(), &rs->ss))) return rc;
     const int K = t->n_interfaces;
-    const int nodes = t->node_offset[K];
-    int64_t nipsi = 0;
-    for (int k = 0; k < K; ++k) {
-        if (t->ipsi_offset[k] < 0) continue;
-        int M = t->node_offset[k + 1] - t->node_offset[k];
-        int64_t sz;
-        if (t->rbf_kind == SPAC_RBF_IMQ_GLOBAL) sz = (int64_t)M * M;
-        else {
-            int S = std::min(t->stencil_size, M);
-            sz = (t->code[k] == 1 ? 1 : (M - S + 1)) * (int64_t)S * S;
-        }
-        nipsi = std::max(nipsi, t->ipsi_offset[k] + sz);
-    }
     const size_t szK = (size_t)K;
-    if ((rc = p->ifc.ensure(szK * (3 * 8 + 4)))) return rc; /* cx, cy, c2, code */
-    if ((rc = p->ifnode.ensure((size_t)nodes * 8))) return rc;
-    if ((rc = p->ifipsi.ensure((size_t)std::max<int64_t>(nipsi, 1) * 8))) return rc;
-    if ((rc = p->rows_iface.ensure((size_t)n * 4 + (size_t)n * 4))) return rc; /* job_iface, frontera */
-    if ((rc = p->rows_goff.ensure((size_t)(n + 1) * 8))) return rc;
     if ((rc = p->rows_G.ensure((size_t)std::max<int64_t>(rs->goff[n], 1) * 8))) return rc;
     if ((rc = p->rows_part.ensure((size_t)n * 8))) return rc; /* B */
     if ((rc = p->stats.ensure((size_t)n * sizeof(spac_point_stats)))) return rc;
     if ((rc = p->ensure_records(rs->ss.total_paths))) return rc;
     if ((rc = p->queue.ensure(64))) return rc;
-    char* ifc = (char*)p->ifc.ptr;
-    double* d_cx = (double*)ifc;
-    double* d_cy = d_cx + K;
-    double* d_c2 = d_cy + K;
-    int32_t* d_code = (int32_t*)(d_c2 + K);
-    const size_t noff_bytes = (((szK + 1) * 4 + 7) / 8) * 8; /* node_offset, then 8-aligned ipsi_offset */
-    if ((rc = p->ifoff.ensure(noff_bytes + szK * 8))) return rc;
-    int32_t* d_noff = (int32_t*)p->ifoff.ptr;
-    int64_t* d_ioff = (int64_t*)((char*)p->ifoff.ptr + noff_bytes);
-    int32_t* d_iface = (int32_t*)p->rows_iface.ptr;
-    int32_t* d_front = d_iface + n;
     hipStream_t s = p->stream;
-    ROWS_HIP_CHECK(hipMemcpyAsync(d_cx, t->cx, szK * 8, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(d_cy, t->cy, szK * 8, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(d_c2, t->c2, szK * 8, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(d_code, t->code, szK * 4, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(d_noff, t->node_offset, (szK + 1) * 4, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(d_ioff, t->ipsi_offset, szK * 8, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(p->ifnode.ptr, t->node_theta, (size_t)nodes * 8, hipMemcpyHostToDevice, s));
-    if (nipsi > 0) ROWS_HIP_CHECK(hipMemcpyAsync(p->ifipsi.ptr, t->ipsi, (size_t)nipsi * 8, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(d_iface, nb->iface, (size_t)n * 4, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(d_front, nb->frontera, (size_t)n * 4, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipMemcpyAsync(p->rows_goff.ptr, rs->goff.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, s));
-    ROWS_HIP_CHECK(hipStreamSynchronize(s));
+    const DeviceBuffer *ifc, *ifoff, *ifnode, *ifipsi, *jobs, *goff;
+    if (sys) {
+        ifc = &sys->t_ifc;
+        ifoff = &sys->t_ifoff;
+        ifnode = &sys->t_ifnode;
+        ifipsi = &sys->t_ifipsi;
+        jobs = &sys->t_jobs;
+        goff = &sys->t_goff;
+    } else {
+        if ((rc = iface_table_upload(t, s, p->ifc, p->ifoff, p->ifnode, p->ifipsi))) return rc;
+        if ((rc = p->rows_iface.ensure((size_t)n * 4 + (size_t)n * 4))) return rc; /* job_iface, frontera */
+        if ((rc = p->rows_goff.ensure((size_t)(n + 1) * 8))) return rc;
+        int32_t* d_iface = (int32_t*)p->rows_iface.ptr;
+        ROWS_HIP_CHECK(hipMemcpyAsync(d_iface, nb->iface, (size_t)n * 4, hipMemcpyHostToDevice, s));
+        ROWS_HIP_CHECK(hipMemcpyAsync(d_iface + n, nb->frontera, (size_t)n * 4, hipMemcpyHostToDevice, s));
+        ROWS_HIP_CHECK(hipMemcpyAsync(p->rows_goff.ptr, rs->goff.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, s));
+        ROWS_HIP_CHECK(hipStreamSynchronize(s));
+        ifc = &p->ifc;
+        ifoff = &p->ifoff;
+        ifnode = &p->ifnode;
+        ifipsi = &p->ifipsi;
+        jobs = &p->rows_iface;
+        goff = &p->rows_goff;
+    }
+    const size_t noff_bytes = (((szK + 1) * 4 + 7) / 8) * 8;
+    const double* d_cx = (const double*)ifc->ptr;
     RowsArgs& ra = rs->ra;
     std::memset(&ra, 0, sizeof ra);
     ra.n_jobs = n;
     ra.rbf_kind = t->rbf_kind;
     ra.stencil_size = t->stencil_size;
     ra.gauss_eps = t->gauss_eps;
-    ra.job_iface = d_iface;
+    ra.job_iface = (const int32_t*)jobs->ptr;
     ra.paths = rs->ss.paths;
     ra.rec_off = rs->ss.rec_off;
-    ra.g_off = (const int64_t*)p->rows_goff.ptr;
-    ra.cx = d_cx; ra.cy = d_cy; ra.code = d_code; ra.c2 = d_c2;
-    ra.node_offset = d_noff;
-    ra.theta = (const double*)p->ifnode.ptr;
-    ra.ipsi_offset = d_ioff;
-    ra.ipsi = (const double*)p->ifipsi.ptr;
+    ra.g_off = (const int64_t*)goff->ptr;
+    ra.cx = d_cx;
+    ra.cy = d_cx + K;
+    ra.c2 = d_cx + 2 * K;
+    ra.code = (const int32_t*)(d_cx + 3 * K);
+    ra.node_offset = (const int32_t*)ifoff->ptr;
+    ra.theta = (const double*)ifnode->ptr;
+    ra.ipsi_offset = (const int64_t*)((const char*)ifoff->ptr + noff_bytes);
+    ra.ipsi = (const double*)ifipsi->ptr;
     ra.xt = (const double*)p->rec_xt.ptr;
     ra.yt = (const double*)p->rec_yt.ptr;
     ra.Y = (const double*)p->rec_Y.ptr;
     ra.outcome = (const int8_t*)p->rec_outcome.ptr;
     ra.G = (double*)p->rows_G.ptr;
-    rs->d_front = d_front;
+    rs->d_front = (const int32_t*)jobs->ptr + n;
     rs->d_B = (double*)p->rows_part.ptr;
     rs->d_stats = (spac_point_stats*)p->stats.ptr;
     return SPAC_OK;
@@ -597,21 +655,32 @@ extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interf
 /* ------------------------------------------------ device-resident system -- */
 namespace spac {
 /* G of every job into its CSR slots: the slot's initial value plus the job's value (each slot
- * receives one job, checked at system creation), Actualiza_GB's single += per slot. */
+ * receives one job, checked at system creation), Actualiza_GB's single += per slot.  Lane 0 of
+ * each job also folds the job's estimator status into *flag (the smallest, i.e. worst, code). */
 __global__ void __launch_bounds__(64) k_system_g(int32_t n_jobs, const int64_t* __restrict__ g_off,
                                                  const int64_t* __restrict__ slot, const double* __restrict__ G,
-                                                 const double* __restrict__ init, double* __restrict__ vals) {
+                                                 const double* __restrict__ init, double* __restrict__ vals,
+                                                 const spac_point_stats* __restrict__ st, int32_t* flag) {
     const int k = blockIdx.x;
     if (k >= n_jobs) return;
+    if (threadIdx.x == 0) {
+        const int32_t c = st[k].status;
+        if (c < 0) atomicMin(flag, c);
+    }
     const int64_t s = slot[k];
     if (s < 0) return;
     const int64_t o = g_off[k];
     const int m = (int)(g_off[k + 1] - o);
     for (int q = threadIdx.x; q < m; q += blockDim.x) vals[s + q] = init[s + q] + G[o + q];
 }
-/* B of every row: 0 + the B of its jobs, in job order (Actualiza_GB's B[fila] += b) */
+/* B of every row: 0 + the B of its jobs, in job order (Actualiza_GB's B[fila] += b).  The grid's
+ * last block clears *flag (launched before k_system_g, on the same stream). */
 __global__ void k_system_b(int32_t n_rows, const int32_t* __restrict__ row_job_ptr, const int32_t* __restrict__ row_jobs,
-                           const double* __restrict__ Bj, double* __restrict__ B) {
+                           const double* __restrict__ Bj, double* __restrict__ B, int32_t* flag) {
+    if (blockIdx.x == gridDim.x - 1) {
+        if (threadIdx.x == 0) *flag = SPAC_OK;
+        return;
+    }
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
     double b = 0.0;
@@ -636,15 +705,13 @@ extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_inte
             t->node_offset[f + 1] - t->node_offset[f] != sys->h_job_m[k])
             return set_error(SPAC_E_INVALID_ARG, "job %d of the batch is not the system mesh's job %d", k, k);
     }
-    /* and the table the mesh's interfaces: G columns come from it */
-    const size_t K = sys->h_node_offset.size() - 1;
-    if ((size_t)t->n_interfaces != K || std::memcmp(t->node_offset, sys->h_node_offset.data(), (K + 1) * 4) != 0 ||
-        std::memcmp(t->node_global, sys->h_node_global.data(), sys->h_node_global.size() * 4) != 0)
+    /* and the table the mesh's: the system holds that table's device copy */
+    if (table_signature(t) != sys->h_table_sig)
         return set_error(SPAC_E_INVALID_ARG, "the interface table is not the system mesh's");
     Problem* p = reinterpret_cast<Problem*>(hp);
     if (p->device != sys->device) return set_error(SPAC_E_INVALID_ARG, "problem and system are on different devices");
     RowsSetup rs;
-    if ((rc = rows_prepare(p, t, nb, &rs))) return rc;
+    if ((rc = rows_prepare(p, t, nb, &rs, sys))) return rc;
     hipStream_t s = p->stream;
     int blocks = 0;
     ROWS_HIP_CHECK(hipEventRecord(p->ev[0], s));
@@ -653,17 +720,22 @@ extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_inte
     ROWS_HIP_CHECK(hipEventRecord(p->ev[2], s));
     if ((rc = rows_enqueue_bg(p, rs))) return rc;
     const int n = rs.n;
+    int32_t* d_flag = (int32_t*)sys->flag.ptr;
     if (sys->nnz > 0)
         ROWS_HIP_CHECK(hipMemcpyAsync(sys->vals.ptr, sys->init.ptr, (size_t)sys->nnz * 8, hipMemcpyDeviceToDevice, s));
-    if (n > 0)
-        hipLaunchKernelGGL(k_system_g, dim3(n), dim3(64), 0, s, n, rs.ra.g_off, (const int64_t*)sys->job_slot.ptr,
-                           (const double*)rs.ra.G, (const double*)sys->init.ptr, (double*)sys->vals.ptr);
-    if (sys->n_rows > 0)
-        hipLaunchKernelGGL(k_system_b, dim3((sys->n_rows + 255) / 256), dim3(256), 0, s, sys->n_rows,
-                           (const int32_t*)sys->row_job_ptr.ptr, (const int32_t*)sys->row_jobs.ptr,
-                           (const double*)rs.d_B, (double*)sys->B.ptr);
+    /* k_system_b clears the status word, k_system_g folds the job statuses into it */
+    hipLaunchKernelGGL(k_system_b, dim3((sys->n_rows + 255) / 256 + 1), dim3(256), 0, s, sys->n_rows,
+                       (const int32_t*)sys->row_job_ptr.ptr, (const int32_t*)sys->row_jobs.ptr, (const double*)rs.d_B,
+                       (double*)sys->B.ptr, d_flag);
+    hipLaunchKernelGGL(k_system_g, dim3(n), dim3(64), 0, s, n, rs.ra.g_off, (const int64_t*)sys->job_slot.ptr,
+                       (const double*)rs.ra.G, (const double*)sys->init.ptr, (double*)sys->vals.ptr,
+                       (const spac_point_stats*)rs.d_stats, d_flag);
     ROWS_HIP_CHECK(hipGetLastError());
     ROWS_HIP_CHECK(hipEventRecord(p->ev[3], s));
+    int32_t worst = SPAC_OK;
+    ROWS_HIP_CHECK(hipMemcpyAsync(&worst, d_flag, 4, hipMemcpyDeviceToHost, s));
+    if (stats)
+        ROWS_HIP_CHECK(hipMemcpyAsync(stats, rs.d_stats, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost, s));
     ROWS_HIP_CHECK(hipStreamSynchronize(s));
     float ms_path = 0.f, ms_rest = 0.f, ms_all = 0.f;
     ROWS_HIP_CHECK(hipEventElapsedTime(&ms_path, p->ev[0], p->ev[1]));
@@ -675,13 +747,6 @@ extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_inte
     p->last.path_kernel_launches = 1;
     p->last.grid_blocks = blocks;
     p->last.path_steps = 0;
-    std::vector<spac_point_stats> st(n);
-    if (n > 0)
-        ROWS_HIP_CHECK(hipMemcpy(st.data(), rs.d_stats, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
-    if (stats && n > 0) std::memcpy(stats, st.data(), (size_t)n * sizeof(spac_point_stats));
-    spac_status worst = SPAC_OK;
-    for (int k = 0; k < n; ++k)
-        if (st[k].status < worst) worst = st[k].status;
     if (worst == SPAC_E_NAN) return set_error(SPAC_E_NAN, "NaN reached a row estimator");
     if (worst == SPAC_E_MAX_STEPS) return set_error(SPAC_E_MAX_STEPS, "paths stopped by max_steps");
     return SPAC_OK;

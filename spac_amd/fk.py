@@ -173,15 +173,15 @@ class FKProblem:
         rows["rc"] = rc
         return rows
 
-    def assemble_system(self, mesh, system, paths, h, seed=0):
+    def assemble_system(self, mesh, system, paths, h, seed=0, stats=True):
         """spac_fk_assemble_system: every job of the mesh, accumulated into the device-resident
-        system (a mesh.DeviceSystem of the same mesh); returns the per-job estimator records as a
-        numpy structured array (fields of spac_point_stats)."""
+        system (a mesh.DeviceSystem of the same mesh).  Returns the per-job estimator records as a
+        numpy structured array (fields of spac_point_stats), or None with stats=False (no copy)."""
         nb, keep = mesh.node_batch(paths, h)
-        st = (abi.PointStats * mesh.info.n_jobs)()
+        st = (abi.PointStats * mesh.info.n_jobs)() if stats else None
         abi.check(self.lib.spac_fk_assemble_system(self.h, C.byref(mesh.table), C.byref(nb), seed, system.h, st),
                   self.lib)
-        return np.ctypeslib.as_array(st).copy()
+        return np.ctypeslib.as_array(st) if stats else None
 
     def autotune_rows(self, mesh, paths, h, path_fraction=0.125, jobs=None):
         """spac_fk_autotune_rows: installs the fastest launch configuration; returns the report and

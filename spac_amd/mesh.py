@@ -160,7 +160,18 @@ class Mesh:
         return c.value
 
     def node_batch(self, paths, h, jobs=None) -> tuple[abi.NodeBatch, dict]:
-        """spac_node_batch over all jobs or a subset; RNG stream id = job index."""
+        """spac_node_batch over all jobs or a subset; RNG stream id = job index.  The batch over
+        all jobs at scalar (paths, h) is kept and handed out again for the same pair."""
+        key = (int(paths), float(h)) if jobs is None and np.ndim(paths) == 0 and np.ndim(h) == 0 else None
+        cached = getattr(self, "_batch_cache", None)
+        if key is not None and cached is not None and cached[0] == key:
+            return cached[1], cached[2]
+        b, keep = self._node_batch(paths, h, jobs)
+        if key is not None:
+            self._batch_cache = (key, b, keep)
+        return b, keep
+
+    def _node_batch(self, paths, h, jobs):
         idx = np.arange(self.info.n_jobs) if jobs is None else np.asarray(jobs)
         n = idx.size
         keep = dict(x=np.ascontiguousarray(self.job_x[idx]), y=np.ascontiguousarray(self.job_y[idx]),

@@ -99,3 +99,26 @@ def test_gpu_system_rejects_another_mesh_batch(gpu):
         with pytest.raises(abi.SpacError) as e:
             p.assemble_system(m8, s, 10, 1e-3, seed=0)
         assert e.value.code == abi.SPAC_E_INVALID_ARG
+
+
+@pytest.mark.gpu
+def test_gpu_system_status_fold_and_stats_free_call(gpu):
+    """The worst estimator status is folded on the device: a step cap reaches the caller as
+    SPAC_E_MAX_STEPS, as from spac_fk_assemble_rows; without the records copy the system holds
+    the same values."""
+    m = Mesh(mesh_desc())
+    say("device system: status fold, no stats copy")
+    with FKProblem(problem_desc(coef=VC, domain=DOM, max_steps=5)) as p, DeviceSystem(m) as s:
+        rows = p.assemble_rows(m, 50, 5e-4, seed=1, allow_status=True)
+        assert rows["rc"] == abi.SPAC_E_MAX_STEPS
+        with pytest.raises(abi.SpacError) as e:
+            p.assemble_system(m, s, 50, 5e-4, seed=1)
+        assert e.value.code == abi.SPAC_E_MAX_STEPS
+    with FKProblem(problem_desc(coef=VC, domain=DOM)) as p, DeviceSystem(m) as s:
+        st = p.assemble_system(m, s, 200, 5e-4, seed=7)
+        assert (st["status"] == 0).all()
+        a = s.read()
+        assert p.assemble_system(m, s, 200, 5e-4, seed=7, stats=False) is None
+        b = s.read()
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
