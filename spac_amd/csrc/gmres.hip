@@ -59,16 +59,35 @@ __global__ void k_diag_inv(int n, const int32_t* Ip, const int32_t* J, const dou
 }
 
 /* y = A x with L lanes per row (64 / L rows per wavefront), each row summed in a fixed lane
- * order.  L is chosen from the mean row length so that a row takes about three passes. */
+ * order: lane `sub` adds entries sub, sub + L, sub + 2L, ... in that order, then the lanes are
+ * combined by a fixed shuffle tree.  L is chosen from the mean row length so that a row takes
+ * about three passes.  Four passes are issued together (column indices and values first, then
+ * the gathers of x), so a row costs one dependent load round trip instead of one per pass. */
 template <int L>
 __global__ void __launch_bounds__(256) k_spmv(int n, const int32_t* __restrict__ Ip, const int32_t* __restrict__ J,
                                               const double* __restrict__ V, const double* __restrict__ x,
                                               double* __restrict__ y) {
+    constexpr int U = 4;
     const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) / L);
     const int sub = (int)(threadIdx.x % L);
     if (row >= n) return; /* whole L-lane groups leave together */
+    const int e = Ip[row + 1];
     double acc = 0.0;
-    for (int k = Ip[row] + sub; k < Ip[row + 1]; k += L) acc = fma(V[k], x[J[k]], acc);
+    for (int k0 = Ip[row] + sub; k0 < e; k0 += U * L) {
+        int jj[U];
+        double vv[U], xx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * L;
+            jj[u] = (k < e) ? J[k] : -1;
+            vv[u] = (k < e) ? V[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xx[u] = (jj[u] >= 0) ? x[jj[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u * L < e) acc = fma(vv[u], xx[u], acc);
+    }
     for (int off = L / 2; off >= 1; off >>= 1) acc += __shfl_down(acc, off, L);
     if (sub == 0) y[row] = acc;
 }
