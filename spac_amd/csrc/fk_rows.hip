@@ -721,8 +721,8 @@ extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_inte
     if ((rc = rows_enqueue_bg(p, rs))) return rc;
     const int n = rs.n;
     int32_t* d_flag = (int32_t*)sys->flag.ptr;
-    if (sys->nnz > 0)
-        ROWS_HIP_CHECK(hipMemcpyAsync(sys->vals.ptr, sys->init.ptr, (size_t)sys->nnz * 8, hipMemcpyDeviceToDevice, s));
+    /* slots no job adds into keep their initial values from system creation; every job slot is
+       rewritten below as initial value + G, so no reset of the values is needed */
     /* k_system_b clears the status word, k_system_g folds the job statuses into it */
     hipLaunchKernelGGL(k_system_b, dim3((sys->n_rows + 255) / 256 + 1), dim3(256), 0, s, sys->n_rows,
                        (const int32_t*)sys->row_job_ptr.ptr, (const int32_t*)sys->row_jobs.ptr, (const double*)rs.d_B,
