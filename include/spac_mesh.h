@@ -114,7 +114,8 @@ spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* G_i, const int32_t* G
                                  const double* B, double* u, double tol, int32_t max_iter, int32_t restart,
                                  spac_gmres_info* info);
 /* spac_gmres_solve_gpu on a device-resident system (no upload); u (host, n_rows) holds the
- * initial guess on entry and the solution on return. */
+ * initial guess on entry and the solution on return.  The system's device must be current.
+ * Calls that use one system (assembly, solve, read) must not overlap. */
 spac_status spac_gmres_solve_system(spac_system* s, double* u, double tol, int32_t max_iter, int32_t restart,
                                     spac_gmres_info* info);
 /* Read a COO file ("nnz n" then "i j v") into CSR (Lee_GB_COO). Call with

@@ -448,6 +448,10 @@ extern "C" spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* Ip, const 
 extern "C" spac_status spac_gmres_solve_system(spac_system* hs, double* u, double tol, int32_t max_iter,
                                                int32_t restart, spac_gmres_info* info) {
     if (!hs) return set_error(SPAC_E_INVALID_ARG, "null system");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != reinterpret_cast<const DeviceSystem*>(hs)->device)
+        return set_error(SPAC_E_INVALID_ARG, "the system lives on device %d; make it current first",
+                         reinterpret_cast<const DeviceSystem*>(hs)->device);
     return gmres_impl(0, nullptr, nullptr, nullptr, nullptr, reinterpret_cast<const DeviceSystem*>(hs), u, tol,
                       max_iter, restart, info);
 }
