@@ -185,6 +185,8 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_imq_kernel(RowsArgs a) {
  * Dynamic LDS: kRowsBlock * wstride doubles of values, then 16 * mmax TwoSum accumulators. */
 constexpr int kCopies = kRowsWaves * 4;
 constexpr int kPairMaxPaths = kRowsBlock / 2;
+constexpr int kPhase2Group = 4; /* paired rows: paths of a quarter-wave whose weights are formed together */
+
 template <bool PAIR>
 __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs a, int wstride, int mmax) {
     extern __shared__ double s_dyn[];
@@ -261,7 +263,7 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
                 s = circle ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
                            : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
                 const double Yp = c_Y;
-                if (circle) { /* the one stencil inverse, broadcast from LDS: Y w_r straight away */
+                if (!PAIR && circle) { /* the one stencil inverse, broadcast from LDS: Y w_r straight away */
                     for (int rr = 0; rr < S; ++rr) {
                         double w = 0.0;
                         for (int i = 0; i < S; ++i) w = __builtin_fma(s_blk[i * S + rr], phi[i], w);
@@ -277,44 +279,80 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs 
         __syncthreads();
         /* phase 2, one quarter-wave per path, lane r = stencil column r: the weight
          * w_r = sum_i iPsi_s[i][r] phi_i (the stencil's inverse read row by row, 16
-         * contiguous values per quarter-wave).  The quarter-wave's 16 paths of the chunk
+         * contiguous values per quarter-wave; for full circles the unpaired kernel formed Y w_r
+         * in phase 1 already, the paired one forms it here from the inverse in LDS — the same
+         * FMA chain either way).  The quarter-wave's 16 paths of the chunk
          * are independent here, so their loads overlap: indices are clamped to valid
          * entries and unused results dropped instead of branching.  Path p of the row
          * sits in slot sbase + p. */
         const int p0 = wave * 64 + quarter;
         const int p1 = min(cnt, wave * 64 + 64);
         const int rr = (r < S) ? r : 0;
-        double yw[16];
-        if (circle) {
+        if constexpr (PAIR) {
+            /* paired rows (short rows, many of them): full circles too form w_r here, from the
+               one inverse in LDS, which keeps phase 1 short.  Groups of kPhase2Group paths form
+               their weights together, then add them in path order. */
+#pragma unroll
+            for (int k0 = 0; k0 < 16; k0 += kPhase2Group) {
+                double yw[kPhase2Group];
+#pragma unroll
+                for (int u = 0; u < kPhase2Group; ++u) {
+                    const int p = p0 + 4 * (k0 + u);
+                    const int pc = sbase + ((p < p1) ? p : p0);
+                    const int s = s_s[pc];
+                    const int sc = (s >= 0) ? s : 0;
+                    const double* blk = circle ? s_blk : ipsi + (int64_t)sc * S * S;
+                    const double* v = s_v + pc * wstride;
+                    double w = 0.0;
+                    for (int i = 0; i < S; ++i) w = __builtin_fma(blk[i * S + rr], v[i], w);
+                    yw[u] = s_y[pc] * w;
+                }
+#pragma unroll
+                for (int u = 0; u < kPhase2Group; ++u) {
+                    const int p = p0 + 4 * (k0 + u);
+                    if (p < p1) {
+                        const int s = s_s[sbase + p];
+                        if (s >= 0 && r < S) {
+                            int c = s + r;
+                            if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
+                            my[c] = spac_ksum_add(my[c], yw[u]);
+                        }
+                    }
+                }
+            }
+        } else {
+            double yw[16];
+            if (circle) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int p = p0 + 4 * k;
+                    const int pc = sbase + ((p < p1) ? p : p0);
+                    yw[k] = s_v[pc * wstride + rr];
+                }
+            } else
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int p = p0 + 4 * k;
                 const int pc = sbase + ((p < p1) ? p : p0);
-                yw[k] = s_v[pc * wstride + rr];
+                const int s = s_s[pc];
+                const int sc = (s >= 0) ? s : 0;
+                const double* blk = circle ? s_blk : ipsi + (int64_t)sc * S * S;
+                const double* v = s_v + pc * wstride;
+                double w = 0.0;
+                for (int i = 0; i < S; ++i) w = __builtin_fma(blk[i * S + rr], v[i], w);
+                yw[k] = s_y[pc] * w;
             }
-        } else
+            /* then Y w_r into column s + r (mod M) of the quarter-wave's own copy, paths in order */
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int p = p0 + 4 * k;
-            const int pc = sbase + ((p < p1) ? p : p0);
-            const int s = s_s[pc];
-            const int sc = (s >= 0) ? s : 0;
-            const double* blk = circle ? s_blk : ipsi + (int64_t)sc * S * S;
-            const double* v = s_v + pc * wstride;
-            double w = 0.0;
-            for (int i = 0; i < S; ++i) w = __builtin_fma(blk[i * S + rr], v[i], w);
-            yw[k] = s_y[pc] * w;
-        }
-        /* then Y w_r into column s + r (mod M) of the quarter-wave's own copy, paths in order */
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int p = p0 + 4 * k;
-            if (p < p1) {
-                const int s = s_s[sbase + p];
-                if (s >= 0 && r < S) {
-                    int c = s + r;
-                    if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
-                    my[c] = spac_ksum_add(my[c], yw[k]);
+            for (int k = 0; k < 16; ++k) {
+                const int p = p0 + 4 * k;
+                if (p < p1) {
+                    const int s = s_s[sbase + p];
+                    if (s >= 0 && r < S) {
+                        int c = s + r;
+                        if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
+                        my[c] = spac_ksum_add(my[c], yw[k]);
+                    }
                 }
             }
         }
