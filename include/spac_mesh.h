@@ -80,6 +80,15 @@ spac_status spac_mesh_csr(const spac_mesh* m, int32_t* G_i, int32_t* G_j, double
  * mesh never does). */
 spac_status spac_mesh_system_create(const spac_mesh* m, spac_system** out);
 spac_status spac_mesh_system_destroy(spac_system* s);
+/* Two-level preconditioner for spac_gmres_solve_system (no counterpart in the reference, whose
+ * GMRES uses RAS through MUMPS, Solvers/SistemaLinealSolverGMRES_OMP.hpp:519-577): the rows are
+ * grouped by their node's cell on an nx x ny grid over the nodes' bounding box (empty cells
+ * dropped, at most 576 aggregates), and right preconditioning uses
+ * M^-1 = D^-1 + Z (Z^T A Z)^-1 Z^T with Z the aggregates' indicator vectors (additive coarse
+ * correction).  Z^T A Z is formed on the device per solve (fixed-point sums, so results do not
+ * depend on scheduling) and inverted on the host.  nx = ny = 0 switches it off.  The converged
+ * solution is that of the same system to the same tolerance; only the iterations differ. */
+spac_status spac_mesh_system_set_coarse(const spac_mesh* m, spac_system* s, int32_t nx, int32_t ny);
 /* Copy the system out: G_i[n+1], G_j[nnz], G_ij[nnz], B[n] (any pointer may be NULL). */
 spac_status spac_mesh_system_read(const spac_system* s, int32_t* G_i, int32_t* G_j, double* G_ij, double* B);
 spac_status spac_mesh_write_coo(const spac_mesh* m, const char* G_path, const char* B_path);

@@ -322,6 +322,58 @@ spac_status spac_mesh_system_create(const spac_mesh* hm, spac_system** out) {
     return SPAC_OK;
 }
 
+spac_status spac_mesh_system_set_coarse(const spac_mesh* hm, spac_system* hs, int32_t nx, int32_t ny) {
+    DeviceSystem* d = reinterpret_cast<DeviceSystem*>(hs);
+    if (!hm || !d) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (nx == 0 && ny == 0) {
+        d->coarse_k = 0;
+        return SPAC_OK;
+    }
+    if (nx < 1 || ny < 1 || nx * ny > kMaxCoarse)
+        return set_error(SPAC_E_INVALID_ARG, "coarse grid %d x %d: 1..%d aggregates", nx, ny, kMaxCoarse);
+    spac_mesh_info info;
+    spac_status rc;
+    if ((rc = spac_mesh_get_info(hm, &info))) return rc;
+    const int n = d->n_rows;
+    if (info.n_nodes != n) return set_error(SPAC_E_INVALID_ARG, "the mesh has %d nodes, the system %d rows", info.n_nodes, n);
+    std::vector<double> x(n), y(n);
+    std::vector<int32_t> fr(n);
+    if ((rc = spac_mesh_nodes(hm, x.data(), y.data(), fr.data()))) return rc;
+    double x0 = x[0], x1 = x[0], y0 = y[0], y1 = y[0];
+    for (int i = 0; i < n; ++i) {
+        x0 = std::min(x0, x[i]); x1 = std::max(x1, x[i]);
+        y0 = std::min(y0, y[i]); y1 = std::max(y1, y[i]);
+    }
+    /* cell of each node on the nx x ny grid over the nodes' bounding box; empty cells dropped */
+    std::vector<int32_t> cell(n), id(nx * ny, -1);
+    for (int i = 0; i < n; ++i) {
+        int cx = x1 > x0 ? (int)((x[i] - x0) / (x1 - x0) * nx) : 0;
+        int cy = y1 > y0 ? (int)((y[i] - y0) / (y1 - y0) * ny) : 0;
+        cx = std::min(std::max(cx, 0), nx - 1);
+        cy = std::min(std::max(cy, 0), ny - 1);
+        cell[i] = cx + nx * cy;
+    }
+    int k = 0;
+    for (int i = 0; i < n; ++i)
+        if (id[cell[i]] < 0) id[cell[i]] = 1;
+    for (int c = 0; c < nx * ny; ++c) id[c] = id[c] > 0 ? k++ : -1;
+    std::vector<int32_t> agg(n), ptr(k + 1, 0), rows(n);
+    for (int i = 0; i < n; ++i) ptr[(agg[i] = id[cell[i]]) + 1]++;
+    for (int a = 0; a < k; ++a) ptr[a + 1] += ptr[a];
+    std::vector<int32_t> cur(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; ++i) rows[cur[agg[i]]++] = i;
+    if ((rc = d->c_agg.ensure((size_t)n * 4)) || (rc = d->c_ptr.ensure((size_t)(k + 1) * 4)) ||
+        (rc = d->c_rows.ensure((size_t)n * 4)) || (rc = d->c_efix.ensure((size_t)k * k * 8)) ||
+        (rc = d->c_einv.ensure((size_t)k * k * 8)) || (rc = d->c_vec.ensure((size_t)2 * k * 8)))
+        return rc;
+    if (hipMemcpy(d->c_agg.ptr, agg.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d->c_ptr.ptr, ptr.data(), (size_t)(k + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d->c_rows.ptr, rows.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(SPAC_E_HIP, "uploading the coarse space failed");
+    d->coarse_k = k;
+    return SPAC_OK;
+}
+
 spac_status spac_mesh_system_destroy(spac_system* hs) {
     DeviceSystem* d = reinterpret_cast<DeviceSystem*>(hs);
     if (!d) return SPAC_OK;

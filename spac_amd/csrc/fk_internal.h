@@ -104,6 +104,9 @@ spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* sr
  * SistemaLineal::Inicializa_CSR (diagonal = containing circles, 1e-23 elsewhere), each job's
  * first slot in its row, and its rows' jobs in job order.  Rows assembled on the device are
  * accumulated here and solved here (spac_fk_assemble_system, spac_gmres_solve_system). */
+/* largest coarse space of the system GMRES: its Galerkin matrix is inverted on the host per solve */
+constexpr int kMaxCoarse = 576;
+
 struct DeviceSystem {
     int32_t n_rows = 0, n_jobs = 0;
     int64_t nnz = 0;
@@ -119,9 +122,18 @@ struct DeviceSystem {
      * flag (int32 [n_jobs] each, one buffer) and G offsets (int64 [n_jobs + 1]) */
     DeviceBuffer t_ifc, t_ifoff, t_ifnode, t_ifipsi, t_jobs, t_goff;
     DeviceBuffer flag; /* int32: worst estimator status of the last assembly */
+    /* optional coarse space of the GMRES preconditioner (spac_mesh_system_set_coarse):
+     * coarse_k aggregates of rows, piecewise constant */
+    int32_t coarse_k = 0;
+    DeviceBuffer c_agg;          /* int32 [n_rows]: aggregate of each row */
+    DeviceBuffer c_ptr, c_rows;  /* int32 [k + 1], [n_rows]: rows of each aggregate, ascending */
+    DeviceBuffer c_efix;         /* int64 [k * k]: Z^T A Z in 2^-40 fixed point */
+    DeviceBuffer c_einv;         /* f64 [k * k]: its inverse */
+    DeviceBuffer c_vec;          /* f64 [2 k]: aggregate sums, coarse correction */
     void release() {
         for (DeviceBuffer* b : {&Ip, &J, &init, &vals, &B, &job_slot, &row_job_ptr, &row_jobs, &t_ifc, &t_ifoff,
-                                &t_ifnode, &t_ifipsi, &t_jobs, &t_goff, &flag})
+                                &t_ifnode, &t_ifipsi, &t_jobs, &t_goff, &flag, &c_agg, &c_ptr, &c_rows, &c_efix,
+                                &c_einv, &c_vec})
             b->release();
     }
 };

@@ -163,6 +163,128 @@ __global__ void k_scale(int n, const double* __restrict__ x, const double* __res
     z[i] = dinv[i] * v;
 }
 
+/* ------------------------------------------------ coarse space (aggregates) -- */
+constexpr double kFix = 1099511627776.0; /* 2^40: fixed-point scale of the Galerkin sums */
+
+__global__ void k_fill_u64(size_t n, unsigned long long* x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0ull;
+}
+
+/* efix[agg(i)][agg(j)] += A_ij in fixed point.  Integer sums are exact, so the result does not
+ * depend on the order of the atomics; a row's consecutive columns of one aggregate are summed
+ * in a register first. */
+__global__ void k_coarse_galerkin(int n, const int32_t* __restrict__ Ip, const int32_t* __restrict__ J,
+                                  const double* __restrict__ V, const int32_t* __restrict__ agg, int k,
+                                  unsigned long long* efix) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long* row = efix + (size_t)agg[i] * k;
+    int cur = -1;
+    long long acc = 0;
+    for (int j = Ip[i]; j < Ip[i + 1]; ++j) {
+        const int b = agg[J[j]];
+        if (b != cur) {
+            if (cur >= 0) atomicAdd(row + cur, (unsigned long long)acc);
+            cur = b;
+            acc = 0;
+        }
+        acc += __double2ll_rn(V[j] * kFix);
+    }
+    if (cur >= 0) atomicAdd(row + cur, (unsigned long long)acc);
+}
+
+/* cs[a] = sum of x over the rows of aggregate a (rows ascending, lane-strided, fixed tree) */
+__global__ void __launch_bounds__(256) k_coarse_restrict(const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ rows,
+                                                         const double* __restrict__ x, double* __restrict__ cs) {
+    __shared__ double sh[4];
+    const int a = blockIdx.x;
+    double acc = 0.0;
+    for (int q = ptr[a] + (int)threadIdx.x; q < ptr[a + 1]; q += 256) acc += x[rows[q]];
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) cs[a] = t;
+}
+
+/* cc = E^-1 cs (einv_t: the inverse transposed, so that consecutive threads read consecutive
+ * entries) */
+__global__ void k_coarse_solve(int k, const double* __restrict__ einv_t, const double* __restrict__ cs,
+                               double* __restrict__ cc) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= k) return;
+    double t = 0.0;
+    for (int b = 0; b < k; ++b) t = fma(einv_t[(size_t)b * k + a], cs[b], t);
+    cc[a] = t;
+}
+
+/* z[i] += cc[agg(i)] */
+__global__ void k_coarse_prolong(int n, const int32_t* __restrict__ agg, const double* __restrict__ cc,
+                                 double* __restrict__ z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] += cc[agg[i]];
+}
+
+/* t[i] = sum_j y[j] Vb[j][i] (the Krylov combination, before preconditioning) */
+__global__ void __launch_bounds__(256) k_combine(int n, const double* __restrict__ Vb, size_t ld,
+                                                 const double* __restrict__ y, int k, double* __restrict__ t) {
+    __shared__ double ys[1024];
+    for (int j = threadIdx.x; j < k; j += blockDim.x) ys[j] = y[j];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s = fma(ys[j], Vb[(size_t)j * ld + i], s);
+    t[i] = s;
+}
+
+/* u[i] += dinv[i] t[i] + cc[agg(i)]: u += M^-1 t with the two-level M^-1 */
+__global__ void k_correct_coarse(int n, const double* __restrict__ t, const double* __restrict__ dinv,
+                                 const int32_t* __restrict__ agg, const double* __restrict__ cc,
+                                 double* __restrict__ u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] += dinv[i] * t[i] + cc[agg[i]];
+}
+
+/* In-place inverse of the k x k row-major matrix a by Gauss-Jordan elimination with partial
+ * pivoting; the inverse is written transposed into out.  False if a pivot vanishes. */
+static bool invert_transposed(int k, std::vector<double>& a, std::vector<double>& out) {
+    std::vector<double> inv((size_t)k * k, 0.0);
+    for (int i = 0; i < k; ++i) inv[(size_t)i * k + i] = 1.0;
+    for (int c = 0; c < k; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < k; ++r)
+            if (std::fabs(a[(size_t)r * k + c]) > std::fabs(a[(size_t)piv * k + c])) piv = r;
+        if (!(std::fabs(a[(size_t)piv * k + c]) > 0.0)) return false;
+        if (piv != c)
+            for (int j = 0; j < k; ++j) {
+                std::swap(a[(size_t)c * k + j], a[(size_t)piv * k + j]);
+                std::swap(inv[(size_t)c * k + j], inv[(size_t)piv * k + j]);
+            }
+        const double d = 1.0 / a[(size_t)c * k + c];
+        for (int j = 0; j < k; ++j) {
+            a[(size_t)c * k + j] *= d;
+            inv[(size_t)c * k + j] *= d;
+        }
+        for (int r = 0; r < k; ++r) {
+            if (r == c) continue;
+            const double f = a[(size_t)r * k + c];
+            if (f == 0.0) continue;
+            double* ar = &a[(size_t)r * k];
+            double* ir = &inv[(size_t)r * k];
+            const double* ac = &a[(size_t)c * k];
+            const double* ic = &inv[(size_t)c * k];
+            for (int j = 0; j < k; ++j) {
+                ar[j] -= f * ac[j];
+                ir[j] -= f * ic[j];
+            }
+        }
+    }
+    out.resize((size_t)k * k);
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) out[(size_t)j * k + i] = inv[(size_t)i * k + j];
+    return true;
+}
+
 /* r = B - r */
 __global__ void k_residual(int n, const double* __restrict__ B, double* __restrict__ r) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -319,6 +441,36 @@ static spac_status gmres_impl(int32_t n, const int32_t* Ip, const int32_t* J, co
     auto norm = [&](const double* x, double* out) { dots(x, 1, x, out, (double*)nullptr, 1); };
 
     hipLaunchKernelGGL(k_diag_inv, g1, b1, 0, s, n, dIp, dJ, dV, dinv);
+    /* two-level preconditioner of a device system: Galerkin matrix on the device, inverse on the host */
+    const int ck = sys ? sys->coarse_k : 0;
+    const int32_t* cagg = ck ? (const int32_t*)sys->c_agg.ptr : nullptr;
+    double* ccs = ck ? (double*)sys->c_vec.ptr : nullptr;
+    double* ccc = ccs + ck;
+    if (ck > 0) {
+        const size_t kk = (size_t)ck * ck;
+        unsigned long long* efix = (unsigned long long*)sys->c_efix.ptr;
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((kk + 255) / 256)), b1, 0, s, kk, efix);
+        hipLaunchKernelGGL(k_coarse_galerkin, g1, b1, 0, s, n, dIp, dJ, dV, cagg, ck, efix);
+        std::vector<long long> ef(kk);
+        GM_CHECK(hipMemcpyAsync(ef.data(), efix, kk * 8, hipMemcpyDeviceToHost, s));
+        GM_CHECK(hipStreamSynchronize(s));
+        std::vector<double> e(kk), einv_t;
+        for (size_t q = 0; q < kk; ++q) e[q] = (double)ef[q] / kFix;
+        if (!invert_transposed(ck, e, einv_t)) {
+            st.release();
+            return set_error(SPAC_E_INVALID_ARG, "the coarse Galerkin matrix is singular");
+        }
+        GM_CHECK(hipMemcpyAsync(sys->c_einv.ptr, einv_t.data(), kk * 8, hipMemcpyHostToDevice, s));
+        GM_CHECK(hipStreamSynchronize(s));
+    }
+    /* z += Z E^-1 Z^T x (after z = D^-1 x) */
+    auto coarse = [&](const double* x, double* z) {
+        hipLaunchKernelGGL(k_coarse_restrict, dim3(ck), b1, 0, s, (const int32_t*)sys->c_ptr.ptr,
+                           (const int32_t*)sys->c_rows.ptr, x, ccs);
+        hipLaunchKernelGGL(k_coarse_solve, dim3((ck + 255) / 256), b1, 0, s, ck, (const double*)sys->c_einv.ptr,
+                           (const double*)ccs, ccc);
+        hipLaunchKernelGGL(k_coarse_prolong, g1, b1, 0, s, n, cagg, (const double*)ccc, z);
+    };
     norm(dB, dn);
     double bnorm = 0.0;
     GM_CHECK(hipMemcpyAsync(&bnorm, dn, 8, hipMemcpyDeviceToHost, s));
@@ -356,6 +508,7 @@ static spac_status gmres_impl(int32_t n, const int32_t* Ip, const int32_t* J, co
         }
         dots(w, 1, w, hw, colk + 2 * (m + 1), 1);
         hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)w, (const double*)hw, (const double*)dinv, w, dz);
+        if (ck > 0) coarse((const double*)w, dz);
         GM_CHECK(hipGetLastError());
         GM_CHECK(hipEventRecord(st.step_done[k % kEvents], s));
         return SPAC_OK;
@@ -366,6 +519,7 @@ static spac_status gmres_impl(int32_t n, const int32_t* Ip, const int32_t* J, co
         res = beta / bnorm;
         if (res < tol) { conv = true; break; }
         hipLaunchKernelGGL(k_scale, g1, b1, 0, s, n, (const double*)r, (const double*)dn, (const double*)dinv, Vb, dz);
+        if (ck > 0) coarse((const double*)Vb, dz);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         const int steps = std::min(m, max_iter - it); /* Arnoldi steps this cycle may take */
@@ -409,8 +563,18 @@ static spac_status gmres_impl(int32_t n, const int32_t* Ip, const int32_t* J, co
         }
         if (k > 0) {
             GM_CHECK(hipMemcpyAsync(h1, yv.data(), (size_t)k * 8, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_correct, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, k,
-                               (const double*)dinv, du);
+            if (ck > 0) {
+                hipLaunchKernelGGL(k_combine, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, k, r);
+                hipLaunchKernelGGL(k_coarse_restrict, dim3(ck), b1, 0, s, (const int32_t*)sys->c_ptr.ptr,
+                                   (const int32_t*)sys->c_rows.ptr, (const double*)r, ccs);
+                hipLaunchKernelGGL(k_coarse_solve, dim3((ck + 255) / 256), b1, 0, s, ck,
+                                   (const double*)sys->c_einv.ptr, (const double*)ccs, ccc);
+                hipLaunchKernelGGL(k_correct_coarse, g1, b1, 0, s, n, (const double*)r, (const double*)dinv, cagg,
+                                   (const double*)ccc, du);
+            } else {
+                hipLaunchKernelGGL(k_correct, g1, b1, 0, s, n, (const double*)Vb, N, (const double*)h1, k,
+                                   (const double*)dinv, du);
+            }
         }
         restarts++;
         if (res < tol) {

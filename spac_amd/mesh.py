@@ -77,6 +77,7 @@ def _setup(lib):
     lib.spac_mesh_system_create.argtypes = [vp, C.POINTER(vp)]
     lib.spac_mesh_system_destroy.argtypes = [vp]
     lib.spac_mesh_system_read.argtypes = [vp, abi.i32p, abi.i32p, abi.dp, abi.dp]
+    lib.spac_mesh_system_set_coarse.argtypes = [vp, vp, C.c_int32, C.c_int32]
     lib.spac_gmres_solve_system.argtypes = [vp, abi.dp, C.c_double, C.c_int32, C.c_int32, C.POINTER(GmresInfo)]
     lib.spac_read_coo.argtypes = [C.c_char_p, abi.i32p, abi.i32p, abi.i32p, abi.i32p, abi.dp]
     lib._mesh_ready = True
@@ -255,6 +256,12 @@ class DeviceSystem:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_coarse(self, nx, ny=None):
+        """spac_mesh_system_set_coarse: two-level GMRES preconditioner over an nx x ny grid of
+        row aggregates (0: off)."""
+        ny = nx if ny is None else ny
+        abi.check(self.lib.spac_mesh_system_set_coarse(self.mesh.h, self.h, nx, ny), self.lib)
 
     def read(self):
         """(G_i, G_j, G_ij, B) copied out of the device."""

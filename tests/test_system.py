@@ -122,3 +122,40 @@ def test_gpu_system_status_fold_and_stats_free_call(gpu):
         b = s.read()
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+
+
+def test_system_coarse_without_handle_fails_cleanly():
+    lib = abi.load()
+    from spac_amd import mesh as meshmod
+    meshmod._setup(lib)
+    assert lib.spac_mesh_system_set_coarse(None, None, 4, 4) == abi.SPAC_E_INVALID_ARG
+
+
+@pytest.mark.gpu
+def test_gpu_system_two_level_gmres(gpu):
+    """The coarse correction changes the iterations, not the answer: the solution agrees with
+    the Jacobi-only solve to the solver tolerance, a repeated solve is bit-identical (the
+    Galerkin sums are fixed point), and switching it off restores the Jacobi iterate."""
+    m = Mesh(mesh_desc(n_side=16, nodes_per_circle=40))
+    say(f"device system two-level GMRES: {m.info.n_nodes} rows")
+    with FKProblem(problem_desc(coef=VC, domain=DOM, precision=abi.SPAC_FP32)) as p, DeviceSystem(m) as s:
+        p.assemble_system(m, s, 200, 2e-4, seed=5, stats=False)
+        u_j, info_j = s.solve(tol=1e-10, max_iter=5000, restart=100)
+        with pytest.raises(abi.SpacError):
+            s.set_coarse(30, 30)  # 900 aggregates: above the limit
+        s.set_coarse(4, 4)
+        u_c, info_c = s.solve(tol=1e-10, max_iter=5000, restart=100)
+        u_c2, _ = s.solve(tol=1e-10, max_iter=5000, restart=100)
+        say(f"iterations: Jacobi {info_j.iterations}, two-level {info_c.iterations}")
+        assert info_j.converged and info_c.converged
+        assert info_c.iterations < info_j.iterations
+        assert np.abs(u_c - u_j).max() < 1e-7 * np.abs(u_j).max()
+        assert np.array_equal(u_c.view(np.int64), u_c2.view(np.int64))
+        Gi, Gj, Gv, B = s.read()
+        import scipy.sparse as sp
+        A = sp.csr_matrix((Gv, Gj, Gi), shape=(B.size, B.size))
+        assert np.linalg.norm(A @ u_c - B) <= 1e-9 * np.linalg.norm(B)
+        s.set_coarse(0, 0)
+        u_o, info_o = s.solve(tol=1e-10, max_iter=5000, restart=100)
+        assert info_o.iterations == info_j.iterations
+        assert np.array_equal(u_o.view(np.int64), u_j.view(np.int64))
