@@ -86,7 +86,13 @@ struct Problem {
                               hipEvent_t after_path = nullptr);
     spac_status enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode);
     spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode);
+    /* pinned host staging of upload_solves (DMA from pinned memory instead of pageable copies) */
+    void* stage = nullptr;
+    size_t stage_cap = 0;
     void release_all() {
+        if (stage) (void)hipHostFree(stage);
+        stage = nullptr;
+        stage_cap = 0;
         DeviceBuffer* all[] = {&rec_xt, &rec_yt, &rec_Y, &rec_Z, &rec_xi, &rec_tau, &rec_gval, &rec_steps,
                                &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
                                &ssq, &ssig, &ssigf,

@@ -733,26 +733,52 @@ static uint32_t work_item_paths(uint32_t chunk, int64_t total_paths) {
  * work queue (solves with zero paths get no work items). */
 spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths,
                                    const double* h, const uint32_t* strm, const double* circle, SolveSet* out) {
-    std::vector<int64_t> off(n + 1, 0);
-    std::vector<int4> items;
     int64_t total = 0;
+    size_t n_items = 0;
     for (int k = 0; k < n; ++k) total += paths[k];
     const uint32_t chunk = work_item_paths((uint32_t)launch.chunk_paths, total);
     for (int k = 0; k < n; ++k) {
         if (!(h[k] > 0.0) || !(h[k] < 1e30)) return set_error(SPAC_E_INVALID_ARG, "solve %d has bad h", k);
         if (circle && !(circle[3 * k] > 0.0))
             return set_error(SPAC_E_INVALID_ARG, "solve %d has a circle of radius <= 0", k);
-        off[k + 1] = off[k] + paths[k];
-        for (uint32_t p0 = 0; p0 < paths[k]; p0 += chunk) {
-            uint32_t p1 = std::min(p0 + chunk, paths[k]);
-            items.push_back(make_int4(k, (int)p0, (int)p1, 0));
-        }
+        n_items += (paths[k] + chunk - 1) / chunk;
     }
-    if (items.size() > 0x7fffffffu) return set_error(SPAC_E_INVALID_ARG, "too many work items");
+    if (n_items > 0x7fffffffu) return set_error(SPAC_E_INVALID_ARG, "too many work items");
+    /* every array is built in (or copied to) one pinned staging block and goes up by DMA */
+    const size_t nb = (size_t)n;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_x = 0, o_y = o_x + al(nb * 8), o_h = o_y + al(nb * 8), o_st = o_h + al(nb * 8),
+                 o_p = o_st + al(nb * 4), o_off = o_p + al(nb * 4), o_c = o_off + al((nb + 1) * 8),
+                 o_it = o_c + al(nb * 24), o_sq = o_it + al(std::max<size_t>(n_items, 1) * sizeof(int4)),
+                 o_end = o_sq + al(nb * 8);
+    if (o_end > stage_cap) {
+        if (stage) (void)hipHostFree(stage);
+        stage = nullptr;
+        stage_cap = 0;
+        const size_t cap = o_end + o_end / 4;
+        SPAC_HIP_CHECK(hipHostMalloc(&stage, cap, hipHostMallocDefault));
+        stage_cap = cap;
+    }
+    char* sb = (char*)stage;
+    std::memcpy(sb + o_x, x0, nb * 8);
+    std::memcpy(sb + o_y, y0, nb * 8);
+    std::memcpy(sb + o_h, h, nb * 8);
+    std::memcpy(sb + o_st, strm, nb * 4);
+    std::memcpy(sb + o_p, paths, nb * 4);
+    if (circle) std::memcpy(sb + o_c, circle, nb * 24);
+    int64_t* off = (int64_t*)(sb + o_off);
+    int4* items = (int4*)(sb + o_it);
+    double* sq = (double*)(sb + o_sq);
+    off[0] = 0;
+    size_t q = 0;
+    for (int k = 0; k < n; ++k) {
+        off[k + 1] = off[k] + paths[k];
+        for (uint32_t p0 = 0; p0 < paths[k]; p0 += chunk) items[q++] = make_int4(k, (int)p0, (int)std::min(p0 + chunk, paths[k]), 0);
+        sq[k] = std::sqrt(h[k]); /* correctly rounded, as the device's */
+    }
     if (launch.item_order == SPAC_ORDER_COST)
-        std::stable_sort(items.begin(), items.end(), [&](const int4& u, const int4& v) { return h[u.x] < h[v.x]; });
+        std::stable_sort(items, items + n_items, [&](const int4& u, const int4& v) { return h[u.x] < h[v.x]; });
     spac_status rc;
-    size_t nb = (size_t)n;
     if ((rc = sx.ensure(nb * 8))) return rc;
     if ((rc = sy.ensure(nb * 8))) return rc;
     if ((rc = sh.ensure(nb * 8))) return rc;
@@ -760,23 +786,21 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
     if ((rc = spaths.ensure(nb * 4))) return rc;
     if ((rc = soff.ensure((nb + 1) * 8))) return rc;
     if ((rc = scircle.ensure(nb * 24))) return rc;
-    if ((rc = sitems.ensure(std::max<size_t>(items.size(), 1) * sizeof(int4)))) return rc;
-    hipStream_t s = stream;
-    SPAC_HIP_CHECK(hipMemcpyAsync(sx.ptr, x0, nb * 8, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(sy.ptr, y0, nb * 8, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(sh.ptr, h, nb * 8, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(sstream.ptr, strm, nb * 4, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(spaths.ptr, paths, nb * 4, hipMemcpyHostToDevice, s));
-    SPAC_HIP_CHECK(hipMemcpyAsync(soff.ptr, off.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
-    if (circle) SPAC_HIP_CHECK(hipMemcpyAsync(scircle.ptr, circle, nb * 24, hipMemcpyHostToDevice, s));
-    if (!items.empty())
-        SPAC_HIP_CHECK(hipMemcpyAsync(sitems.ptr, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s));
-    std::vector<double> sq(nb);
-    for (size_t k = 0; k < nb; ++k) sq[k] = std::sqrt(h[k]); /* correctly rounded, as the device's */
+    if ((rc = sitems.ensure(std::max<size_t>(n_items, 1) * sizeof(int4)))) return rc;
     if ((rc = ssq.ensure(nb * 8))) return rc;
-    SPAC_HIP_CHECK(hipMemcpyAsync(ssq.ptr, sq.data(), nb * 8, hipMemcpyHostToDevice, s));
-    /* the host vectors die at return: wait for the copies */
+    hipStream_t s = stream;
+    SPAC_HIP_CHECK(hipMemcpyAsync(sx.ptr, sb + o_x, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(sy.ptr, sb + o_y, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(sh.ptr, sb + o_h, nb * 8, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(sstream.ptr, sb + o_st, nb * 4, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(spaths.ptr, sb + o_p, nb * 4, hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(soff.ptr, off, (nb + 1) * 8, hipMemcpyHostToDevice, s));
+    if (circle) SPAC_HIP_CHECK(hipMemcpyAsync(scircle.ptr, sb + o_c, nb * 24, hipMemcpyHostToDevice, s));
+    if (n_items > 0) SPAC_HIP_CHECK(hipMemcpyAsync(sitems.ptr, items, n_items * sizeof(int4), hipMemcpyHostToDevice, s));
+    SPAC_HIP_CHECK(hipMemcpyAsync(ssq.ptr, sq, nb * 8, hipMemcpyHostToDevice, s));
+    /* the staging block is rewritten by the next call: wait for the copies */
     SPAC_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t total_paths = off[n];
     SolveSet ss;
     ss.n = n;
     ss.x0 = (const double*)sx.ptr;
@@ -787,8 +811,8 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
     ss.rec_off = (const int64_t*)soff.ptr;
     ss.circle = circle ? (const double*)scircle.ptr : nullptr;
     ss.items = (const int4*)sitems.ptr;
-    ss.n_items = (int32_t)items.size();
-    ss.total_paths = off[n];
+    ss.n_items = (int32_t)n_items;
+    ss.total_paths = total_paths;
     ss.chunk = chunk;
     ss.sqrth = (const double*)ssq.ptr;
     if ((rc = ensure_solve_scratch(n, &ss))) return rc;
