@@ -86,9 +86,14 @@ struct Problem {
                               hipEvent_t after_path = nullptr);
     spac_status enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode);
     spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode);
-    /* pinned host staging of upload_solves (DMA from pinned memory instead of pageable copies) */
+    /* pinned host staging of upload_solves and copy_out (DMA through pinned memory instead of
+     * pageable copies) */
     void* stage = nullptr;
     size_t stage_cap = 0;
+    spac_status ensure_stage(size_t bytes);
+    /* device -> host copy of `bytes` on the handle's stream, synchronous; large copies go
+     * through the pinned staging block */
+    spac_status copy_out(void* dst, const void* src, size_t bytes);
     void release_all() {
         if (stage) (void)hipHostFree(stage);
         stage = nullptr;

@@ -729,6 +729,32 @@ static uint32_t work_item_paths(uint32_t chunk, int64_t total_paths) {
     return (uint32_t)std::max<int64_t>((int64_t)chunk, c);
 }
 
+spac_status Problem::ensure_stage(size_t bytes) {
+    if (bytes <= stage_cap) return SPAC_OK;
+    if (stage) (void)hipHostFree(stage);
+    stage = nullptr;
+    stage_cap = 0;
+    const size_t cap = bytes + bytes / 4;
+    SPAC_HIP_CHECK(hipHostMalloc(&stage, cap, hipHostMallocDefault));
+    stage_cap = cap;
+    return SPAC_OK;
+}
+
+spac_status Problem::copy_out(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return SPAC_OK;
+    if (bytes < ((size_t)1 << 20)) {
+        SPAC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+        SPAC_HIP_CHECK(hipStreamSynchronize(stream));
+        return SPAC_OK;
+    }
+    spac_status rc;
+    if ((rc = ensure_stage(bytes))) return rc;
+    SPAC_HIP_CHECK(hipMemcpyAsync(stage, src, bytes, hipMemcpyDeviceToHost, stream));
+    SPAC_HIP_CHECK(hipStreamSynchronize(stream));
+    std::memcpy(dst, stage, bytes);
+    return SPAC_OK;
+}
+
 /* Upload a host batch of solves into the handle's solve table and build the
  * work queue (solves with zero paths get no work items). */
 spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths,
@@ -751,14 +777,8 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
                  o_p = o_st + al(nb * 4), o_off = o_p + al(nb * 4), o_c = o_off + al((nb + 1) * 8),
                  o_it = o_c + al(nb * 24), o_sq = o_it + al(std::max<size_t>(n_items, 1) * sizeof(int4)),
                  o_end = o_sq + al(nb * 8);
-    if (o_end > stage_cap) {
-        if (stage) (void)hipHostFree(stage);
-        stage = nullptr;
-        stage_cap = 0;
-        const size_t cap = o_end + o_end / 4;
-        SPAC_HIP_CHECK(hipHostMalloc(&stage, cap, hipHostMallocDefault));
-        stage_cap = cap;
-    }
+    spac_status rc;
+    if ((rc = ensure_stage(o_end))) return rc;
     char* sb = (char*)stage;
     std::memcpy(sb + o_x, x0, nb * 8);
     std::memcpy(sb + o_y, y0, nb * 8);
@@ -778,7 +798,6 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
     }
     if (launch.item_order == SPAC_ORDER_COST)
         std::stable_sort(items, items + n_items, [&](const int4& u, const int4& v) { return h[u.x] < h[v.x]; });
-    spac_status rc;
     if ((rc = sx.ensure(nb * 8))) return rc;
     if ((rc = sy.ensure(nb * 8))) return rc;
     if ((rc = sh.ensure(nb * 8))) return rc;
@@ -1147,7 +1166,7 @@ spac_status spac_fk_solve_points(spac_problem* hp, const double* x0, const doubl
     rc = solve_points_impl(p, (const double*)p->sx.ptr, (const double*)p->sy.ptr, n, paths, h, seed,
                            (spac_point_stats*)p->stats.ptr);
     if (rc) return rc;
-    SPAC_HIP_CHECK(hipMemcpy(stats, p->stats.ptr, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
+    if ((rc = p->copy_out(stats, p->stats.ptr, (size_t)n * sizeof(spac_point_stats)))) return rc;
     spac_status worst = SPAC_OK;
     for (int i = 0; i < n; ++i)
         if (stats[i].status < worst) worst = stats[i].status;
@@ -1174,17 +1193,17 @@ spac_status spac_fk_simulate(spac_problem* hp, const spac_solve_batch* b, uint64
     if (rc) return rc;
     size_t tp = (size_t)ss.total_paths;
     if (rec) {
-        if (rec->x_tau) SPAC_HIP_CHECK(hipMemcpy(rec->x_tau, p->rec_xt.ptr, tp * 8, hipMemcpyDeviceToHost));
-        if (rec->y_tau) SPAC_HIP_CHECK(hipMemcpy(rec->y_tau, p->rec_yt.ptr, tp * 8, hipMemcpyDeviceToHost));
-        if (rec->Y) SPAC_HIP_CHECK(hipMemcpy(rec->Y, p->rec_Y.ptr, tp * 8, hipMemcpyDeviceToHost));
-        if (rec->Z) SPAC_HIP_CHECK(hipMemcpy(rec->Z, p->rec_Z.ptr, tp * 8, hipMemcpyDeviceToHost));
-        if (rec->xi) SPAC_HIP_CHECK(hipMemcpy(rec->xi, p->rec_xi.ptr, tp * 8, hipMemcpyDeviceToHost));
-        if (rec->tau) SPAC_HIP_CHECK(hipMemcpy(rec->tau, p->rec_tau.ptr, tp * 8, hipMemcpyDeviceToHost));
-        if (rec->gval) SPAC_HIP_CHECK(hipMemcpy(rec->gval, p->rec_gval.ptr, tp * 8, hipMemcpyDeviceToHost));
-        if (rec->steps) SPAC_HIP_CHECK(hipMemcpy(rec->steps, p->rec_steps.ptr, tp * 4, hipMemcpyDeviceToHost));
-        if (rec->outcome) SPAC_HIP_CHECK(hipMemcpy(rec->outcome, p->rec_outcome.ptr, tp, hipMemcpyDeviceToHost));
+        if (rec->x_tau && (rc = p->copy_out(rec->x_tau, p->rec_xt.ptr, tp * 8))) return rc;
+        if (rec->y_tau && (rc = p->copy_out(rec->y_tau, p->rec_yt.ptr, tp * 8))) return rc;
+        if (rec->Y && (rc = p->copy_out(rec->Y, p->rec_Y.ptr, tp * 8))) return rc;
+        if (rec->Z && (rc = p->copy_out(rec->Z, p->rec_Z.ptr, tp * 8))) return rc;
+        if (rec->xi && (rc = p->copy_out(rec->xi, p->rec_xi.ptr, tp * 8))) return rc;
+        if (rec->tau && (rc = p->copy_out(rec->tau, p->rec_tau.ptr, tp * 8))) return rc;
+        if (rec->gval && (rc = p->copy_out(rec->gval, p->rec_gval.ptr, tp * 8))) return rc;
+        if (rec->steps && (rc = p->copy_out(rec->steps, p->rec_steps.ptr, tp * 4))) return rc;
+        if (rec->outcome && (rc = p->copy_out(rec->outcome, p->rec_outcome.ptr, tp))) return rc;
     }
-    if (stats) SPAC_HIP_CHECK(hipMemcpy(stats, p->stats.ptr, nb * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
+    if (stats && (rc = p->copy_out(stats, p->stats.ptr, nb * sizeof(spac_point_stats)))) return rc;
     return SPAC_OK;
 }
 

@@ -642,10 +642,11 @@ static spac_status rows_finish(Problem* p, const RowsSetup& rs, spac_row_block* 
     const int n = rs.n;
     for (int k = 0; k <= n; ++k) out->G_offset[k] = rs.goff[k];
     if (!rs.gcol.empty()) std::memcpy(out->G_col, rs.gcol.data(), rs.gcol.size() * 4);
-    ROWS_HIP_CHECK(hipMemcpy(out->B, rs.d_B, (size_t)n * 8, hipMemcpyDeviceToHost));
-    if (rs.goff[n] > 0) ROWS_HIP_CHECK(hipMemcpy(out->G, p->rows_G.ptr, (size_t)rs.goff[n] * 8, hipMemcpyDeviceToHost));
+    spac_status rc;
+    if ((rc = p->copy_out(out->B, rs.d_B, (size_t)n * 8))) return rc;
+    if ((rc = p->copy_out(out->G, p->rows_G.ptr, (size_t)rs.goff[n] * 8))) return rc;
     std::vector<spac_point_stats> st(n);
-    ROWS_HIP_CHECK(hipMemcpy(st.data(), rs.d_stats, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost));
+    if ((rc = p->copy_out(st.data(), rs.d_stats, (size_t)n * sizeof(spac_point_stats)))) return rc;
     if (out->stats) std::memcpy(out->stats, st.data(), (size_t)n * sizeof(spac_point_stats));
     spac_status worst = SPAC_OK;
     for (int k = 0; k < n; ++k)
@@ -772,9 +773,8 @@ extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_inte
     ROWS_HIP_CHECK(hipEventRecord(p->ev[3], s));
     int32_t worst = SPAC_OK;
     ROWS_HIP_CHECK(hipMemcpyAsync(&worst, d_flag, 4, hipMemcpyDeviceToHost, s));
-    if (stats)
-        ROWS_HIP_CHECK(hipMemcpyAsync(stats, rs.d_stats, (size_t)n * sizeof(spac_point_stats), hipMemcpyDeviceToHost, s));
     ROWS_HIP_CHECK(hipStreamSynchronize(s));
+    if (stats && (rc = p->copy_out(stats, rs.d_stats, (size_t)n * sizeof(spac_point_stats)))) return rc;
     float ms_path = 0.f, ms_rest = 0.f, ms_all = 0.f;
     ROWS_HIP_CHECK(hipEventElapsedTime(&ms_path, p->ev[0], p->ev[1]));
     ROWS_HIP_CHECK(hipEventElapsedTime(&ms_rest, p->ev[1], p->ev[3]));
