@@ -217,11 +217,16 @@ __global__ void k_coarse_solve(int k, const double* __restrict__ einv_t, const d
     cc[a] = t;
 }
 
-/* z[i] += cc[agg(i)] */
-__global__ void k_coarse_prolong(int n, const int32_t* __restrict__ agg, const double* __restrict__ cc,
-                                 double* __restrict__ z) {
+/* z[i] += (E^-1 cs)[agg(i)], the coarse value recomputed by every row of the aggregate with
+ * k_coarse_solve's FMA order (rows of a wave share aggregates, so the loads are broadcasts) */
+__global__ void k_coarse_prolong(int n, const int32_t* __restrict__ agg, int k, const double* __restrict__ einv_t,
+                                 const double* __restrict__ cs, double* __restrict__ z) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) z[i] += cc[agg[i]];
+    if (i >= n) return;
+    const int a = agg[i];
+    double t = 0.0;
+    for (int b = 0; b < k; ++b) t = fma(einv_t[(size_t)b * k + a], cs[b], t);
+    z[i] += t;
 }
 
 /* t[i] = sum_j y[j] Vb[j][i] (the Krylov combination, before preconditioning) */
@@ -467,9 +472,8 @@ static spac_status gmres_impl(int32_t n, const int32_t* Ip, const int32_t* J, co
     auto coarse = [&](const double* x, double* z) {
         hipLaunchKernelGGL(k_coarse_restrict, dim3(ck), b1, 0, s, (const int32_t*)sys->c_ptr.ptr,
                            (const int32_t*)sys->c_rows.ptr, x, ccs);
-        hipLaunchKernelGGL(k_coarse_solve, dim3((ck + 255) / 256), b1, 0, s, ck, (const double*)sys->c_einv.ptr,
-                           (const double*)ccs, ccc);
-        hipLaunchKernelGGL(k_coarse_prolong, g1, b1, 0, s, n, cagg, (const double*)ccc, z);
+        hipLaunchKernelGGL(k_coarse_prolong, g1, b1, 0, s, n, cagg, ck, (const double*)sys->c_einv.ptr,
+                           (const double*)ccs, z);
     };
     norm(dB, dn);
     double bnorm = 0.0;
