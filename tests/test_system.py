@@ -143,6 +143,10 @@ def test_gpu_system_two_level_gmres(gpu):
         u_j, info_j = s.solve(tol=1e-10, max_iter=5000, restart=100)
         with pytest.raises(abi.SpacError):
             s.set_coarse(30, 30)  # 900 aggregates: above the limit
+        other = Mesh(mesh_desc())  # node positions of another mesh
+        with pytest.raises(abi.SpacError) as e:
+            abi.check(s.lib.spac_mesh_system_set_coarse(other.h, s.h, 4, 4), s.lib)
+        assert e.value.code == abi.SPAC_E_INVALID_ARG
         s.set_coarse(4, 4)
         u_c, info_c = s.solve(tol=1e-10, max_iter=5000, restart=100)
         u_c2, _ = s.solve(tol=1e-10, max_iter=5000, restart=100)
