@@ -335,7 +335,7 @@ __device__ __forceinline__ float rect_distance32(const float* p, float x, float 
 /* Tabulated control variate in FP32: bilinear d u_hat/dx, d u_hat/dy at (x, y),
  * the same cell rule as spac_cv_cell (clamped to the grid, NaN to the origin). */
 struct CvTable32 {
-    const float4* table;
+    const float2* grad; /* (du/dx, du/dy) of the T x T table */
     int32_t t;
     float lox, loy, ix, iy, top;
 };
@@ -345,10 +345,10 @@ __device__ __forceinline__ void cv_grad32(const CvTable32& c, float x, float y, 
     fy = __builtin_fminf(__builtin_fmaxf(fy, 0.0f), c.top);
     int32_t i = min((int32_t)fx, c.t - 2), j = min((int32_t)fy, c.t - 2);
     float ax = fx - (float)i, ay = fy - (float)j, bx = 1.0f - ax, by = 1.0f - ay;
-    const float4* p = c.table + (j * c.t + i);
-    float4 v00 = p[0], v10 = p[1], v01 = p[c.t], v11 = p[c.t + 1];
-    *gx = by * (bx * v00.y + ax * v10.y) + ay * (bx * v01.y + ax * v11.y);
-    *gy = by * (bx * v00.z + ax * v10.z) + ay * (bx * v01.z + ax * v11.z);
+    const float2* p = c.grad + (j * c.t + i);
+    const float2 v00 = p[0], v10 = p[1], v01 = p[c.t], v11 = p[c.t + 1];
+    *gx = by * (bx * v00.x + ax * v10.x) + ay * (bx * v01.x + ax * v11.x);
+    *gy = by * (bx * v00.y + ax * v10.y) + ay * (bx * v01.y + ax * v11.y);
 }
 
 #pragma clang fp contract(off)

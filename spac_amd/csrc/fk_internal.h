@@ -65,6 +65,7 @@ struct Problem {
     DeviceBuffer ifc, ifnode, ifipsi, ifoff, rows_iface, rows_goff, rows_G, rows_part, rows_col;
     /* control variate (SPAC_CV_*) and its table (T x T x float4) */
     DeviceBuffer cv_table;
+    DeviceBuffer cv_grad; /* float2 [T * T]: the table's (du/dx, du/dy), read by the FP32 path kernel */
     int32_t cv_mode = SPAC_CV_ANALYTIC;
     int32_t cv_fallback = 0;
     double cv_max_ratio = 1.0;
@@ -102,7 +103,7 @@ struct Problem {
                                &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
                                &ssq, &ssig, &ssigf,
                                &queue, &stats, &ifc, &ifnode, &ifipsi, &ifoff, &rows_iface, &rows_goff,
-                               &rows_G, &rows_part, &rows_col, &cv_table, &seed_dev};
+                               &rows_G, &rows_part, &rows_col, &cv_table, &cv_grad, &seed_dev};
         for (DeviceBuffer* b : all) b->release();
     }
 };

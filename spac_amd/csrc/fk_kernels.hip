@@ -918,7 +918,7 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
         a.cv.loy = cv_dom[1];
         a.cv.ix = (double)(cv_t - 1) / (cv_dom[2] - cv_dom[0]);
         a.cv.iy = (double)(cv_t - 1) / (cv_dom[3] - cv_dom[1]);
-        a.cv32.table = (const float4*)cv_table.ptr;
+        a.cv32.grad = (const float2*)cv_grad.ptr;
         a.cv32.t = cv_t;
         a.cv32.lox = (float)a.cv.lox;
         a.cv32.loy = (float)a.cv.loy;
@@ -996,6 +996,12 @@ spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_
     return SPAC_OK;
 }
 
+/* (du/dx, du/dy) channels of a T x T float4 control-variate table */
+__global__ void k_cv_gradients(int n, const float4* __restrict__ tab, float2* __restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) grad[i] = make_float2(tab[i].y, tab[i].z);
+}
+
 spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* src, bool src_on_device) {
     if (cfg->mode != SPAC_CV_ANALYTIC && cfg->mode != SPAC_CV_NONE && cfg->mode != SPAC_CV_TABLE)
         return set_error(SPAC_E_INVALID_ARG, "unknown control-variate mode %d", cfg->mode);
@@ -1016,6 +1022,12 @@ spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* sr
         if ((rc = p->cv_table.ensure(bytes))) return rc;
         SPAC_HIP_CHECK(hipMemcpyAsync(p->cv_table.ptr, src, bytes,
                                       src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
+        /* the FP32 kernel reads only the two gradient channels: half the bytes per lookup */
+        const int tt = cfg->table_size * cfg->table_size;
+        if ((rc = p->cv_grad.ensure((size_t)tt * 8))) return rc;
+        hipLaunchKernelGGL(k_cv_gradients, dim3((tt + 255) / 256), dim3(256), 0, p->stream, tt,
+                           (const float4*)p->cv_table.ptr, (float2*)p->cv_grad.ptr);
+        SPAC_HIP_CHECK(hipGetLastError());
         SPAC_HIP_CHECK(hipStreamSynchronize(p->stream));
         p->cv_t = cfg->table_size;
         for (int i = 0; i < 4; ++i) p->cv_dom[i] = d[i];
