@@ -289,8 +289,8 @@ def _ttr_mesh():
 # toleranciaGMRES (main_test_mpi.cpp:32).
 GMRES_TOL = 1e-8
 # Coarse grid of the two-level GMRES preconditioner in the time-to-RMS solves (aggregates per
-# side); 0 gives the Jacobi-only solver.  At 64x64 circles: 309 -> 94 iterations from a cold start.
-COARSE_GRID = 8
+# side); 0 gives the Jacobi-only solver.  At 64x64 circles: 309 -> 67 iterations from a cold start.
+COARSE_GRID = 12
 
 
 def _replicas(mesh, system, prob, paths, h, seeds, u0=None):
@@ -415,7 +415,7 @@ def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n
         return r
 
     system = DeviceSystem(mesh)
-    # two-level GMRES: 8 x 8 aggregates of rows (8 x 8 circles each) as coarse space, both methods
+    # two-level GMRES: 12 x 12 aggregates of rows (~5 x 5 circles each) as coarse space, both methods
     system.set_coarse(COARSE_GRID, COARSE_GRID)
     with FKProblem(desc) as prob:
         prob.set_control_variate(abi.SPAC_CV_NONE)

@@ -206,26 +206,26 @@ __global__ void __launch_bounds__(256) k_coarse_restrict(const int32_t* __restri
     if (threadIdx.x == 0) cs[a] = t;
 }
 
-/* cc = E^-1 cs (einv_t: the inverse transposed, so that consecutive threads read consecutive
- * entries) */
-__global__ void k_coarse_solve(int k, const double* __restrict__ einv_t, const double* __restrict__ cs,
+/* cc = E^-1 cs (einv row-major) */
+__global__ void k_coarse_solve(int k, const double* __restrict__ einv, const double* __restrict__ cs,
                                double* __restrict__ cc) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= k) return;
     double t = 0.0;
-    for (int b = 0; b < k; ++b) t = fma(einv_t[(size_t)b * k + a], cs[b], t);
+    for (int b = 0; b < k; ++b) t = fma(einv[(size_t)a * k + b], cs[b], t);
     cc[a] = t;
 }
 
 /* z[i] += (E^-1 cs)[agg(i)], the coarse value recomputed by every row of the aggregate with
- * k_coarse_solve's FMA order (rows of a wave share aggregates, so the loads are broadcasts) */
-__global__ void k_coarse_prolong(int n, const int32_t* __restrict__ agg, int k, const double* __restrict__ einv_t,
+ * k_coarse_solve's FMA order (rows of a wave share aggregates, so the loads of a row of E^-1
+ * are contiguous broadcasts) */
+__global__ void k_coarse_prolong(int n, const int32_t* __restrict__ agg, int k, const double* __restrict__ einv,
                                  const double* __restrict__ cs, double* __restrict__ z) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int a = agg[i];
+    const double* e = einv + (size_t)agg[i] * k;
     double t = 0.0;
-    for (int b = 0; b < k; ++b) t = fma(einv_t[(size_t)b * k + a], cs[b], t);
+    for (int b = 0; b < k; ++b) t = fma(e[b], cs[b], t);
     z[i] += t;
 }
 
@@ -251,8 +251,8 @@ __global__ void k_correct_coarse(int n, const double* __restrict__ t, const doub
 }
 
 /* In-place inverse of the k x k row-major matrix a by Gauss-Jordan elimination with partial
- * pivoting; the inverse is written transposed into out.  False if a pivot vanishes. */
-static bool invert_transposed(int k, std::vector<double>& a, std::vector<double>& out) {
+ * pivoting, written row-major into out.  False if a pivot vanishes. */
+static bool invert_matrix(int k, std::vector<double>& a, std::vector<double>& out) {
     std::vector<double> inv((size_t)k * k, 0.0);
     for (int i = 0; i < k; ++i) inv[(size_t)i * k + i] = 1.0;
     for (int c = 0; c < k; ++c) {
@@ -284,9 +284,7 @@ static bool invert_transposed(int k, std::vector<double>& a, std::vector<double>
             }
         }
     }
-    out.resize((size_t)k * k);
-    for (int i = 0; i < k; ++i)
-        for (int j = 0; j < k; ++j) out[(size_t)j * k + i] = inv[(size_t)i * k + j];
+    out.swap(inv);
     return true;
 }
 
@@ -459,13 +457,13 @@ static spac_status gmres_impl(int32_t n, const int32_t* Ip, const int32_t* J, co
         std::vector<long long> ef(kk);
         GM_CHECK(hipMemcpyAsync(ef.data(), efix, kk * 8, hipMemcpyDeviceToHost, s));
         GM_CHECK(hipStreamSynchronize(s));
-        std::vector<double> e(kk), einv_t;
+        std::vector<double> e(kk), einv;
         for (size_t q = 0; q < kk; ++q) e[q] = (double)ef[q] / kFix;
-        if (!invert_transposed(ck, e, einv_t)) {
+        if (!invert_matrix(ck, e, einv)) {
             st.release();
             return set_error(SPAC_E_INVALID_ARG, "the coarse Galerkin matrix is singular");
         }
-        GM_CHECK(hipMemcpyAsync(sys->c_einv.ptr, einv_t.data(), kk * 8, hipMemcpyHostToDevice, s));
+        GM_CHECK(hipMemcpyAsync(sys->c_einv.ptr, einv.data(), kk * 8, hipMemcpyHostToDevice, s));
         GM_CHECK(hipStreamSynchronize(s));
     }
     /* z += Z E^-1 Z^T x (after z = D^-1 x) */
