@@ -159,6 +159,10 @@ def test_gpu_system_two_level_gmres(gpu):
         import scipy.sparse as sp
         A = sp.csr_matrix((Gv, Gj, Gi), shape=(B.size, B.size))
         assert np.linalg.norm(A @ u_c - B) <= 1e-9 * np.linalg.norm(B)
+        for nx, ny in ((1, 1), (3, 7)):  # one aggregate; an uneven grid
+            s.set_coarse(nx, ny)
+            u_g, info_g = s.solve(tol=1e-10, max_iter=5000, restart=100)
+            assert info_g.converged and np.abs(u_g - u_j).max() < 1e-7 * np.abs(u_j).max(), (nx, ny)
         s.set_coarse(0, 0)
         u_o, info_o = s.solve(tol=1e-10, max_iter=5000, restart=100)
         assert info_o.iterations == info_j.iterations
