@@ -222,6 +222,24 @@ spac_status spac_fk_solve_points_device(spac_problem* p, const double* d_x0, con
                                         int32_t n, uint32_t paths, double h, uint64_t seed,
                                         spac_point_stats* d_stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): spac_fk_solve_points_device on the same inputs and outputs
+ * with the same results, but every buffer beyond them (per-path records,
+ * per-point sigma(X0), the work-queue counter) lives in one static device block
+ * of SPAC_NOALLOC_SCRATCH_BYTES, which each GPU the library runs on holds until
+ * the process ends.  Nothing is allocated or freed by the call.  At most
+ * SPAC_NOALLOC_MAX_SOLVES points and n * paths <= SPAC_NOALLOC_MAX_PATHS, else
+ * SPAC_E_INVALID_ARG.  The problem's device must be current.  Every call on one
+ * GPU shares that GPU's block (it keeps the last call's records): calls that
+ * overlap on one GPU, from any handle or thread, race.  Synchronous. */
+#define SPAC_NOALLOC_MAX_PATHS (1 << 19)
+#define SPAC_NOALLOC_MAX_SOLVES (1 << 15)
+#define SPAC_NOALLOC_SCRATCH_BYTES (61 * SPAC_NOALLOC_MAX_PATHS + 12 * SPAC_NOALLOC_MAX_SOLVES + 256)
+spac_status spac_fk_solve_points_device_noalloc(spac_problem* p, const double* d_x0, const double* d_y0,
+                                                int32_t n, uint32_t paths, double h, uint64_t seed,
+                                                spac_point_stats* d_stats);
+
 /* General batch of solves with per-path exit data (Simulacion_OMP,
  * Solvers/FeynmanKacSolver.cpp:137-197) and, optionally, per-solve estimators. */
 spac_status spac_fk_simulate(spac_problem* p, const spac_solve_batch* batch, uint64_t seed,

@@ -9,6 +9,7 @@
 #ifndef SPAC_FK_TESTING_H
 #define SPAC_FK_TESTING_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "spac_fk.h"
@@ -21,6 +22,14 @@ extern "C" {
  * 6 x / y, 7 Franke u(x, y), 8 first Brownian normal of (seed 0, stream 0,
  * path x, step y).  Host arrays of n values; runs one device kernel. */
 spac_status spac_debug_math(int32_t fn, const double* x, const double* y, double* out, int64_t n);
+
+/* Device memory for tests of the device-resident entry points (a process that
+ * loads this library's HIP runtime needs no second runtime for it): allocate
+ * `bytes` on the current device, free, and synchronous copies to and from it. */
+spac_status spac_debug_device_alloc(size_t bytes, void** out);
+spac_status spac_debug_device_free(void* ptr);
+spac_status spac_debug_copy_to_device(void* dst, const void* src, size_t bytes);
+spac_status spac_debug_copy_to_host(void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,8 @@ SPAC_E_PATTERN = -6
 SPAC_E_BAD_INTERFACE = -7
 SPAC_E_NO_DEVICE = -8
 SPAC_E_OUT_OF_MEMORY = -9
+SPAC_NOALLOC_MAX_PATHS = 1 << 19
+SPAC_NOALLOC_MAX_SOLVES = 1 << 15
 
 SPAC_COEF_FRANKE_POISSON = 1
 SPAC_COEF_VARCOEF_FRANKE = 2
@@ -263,6 +265,12 @@ def load(path: Path | None = None) -> C.CDLL:
                                          C.POINTER(PointStats)]
     lib.spac_fk_solve_points_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32,
                                                 C.c_double, C.c_uint64, C.c_void_p]
+    lib.spac_fk_solve_points_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32,
+                                                        C.c_double, C.c_uint64, C.c_void_p]
+    lib.spac_debug_device_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+    lib.spac_debug_device_free.argtypes = [C.c_void_p]
+    lib.spac_debug_copy_to_device.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.spac_debug_copy_to_host.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     lib.spac_fk_simulate.argtypes = [C.c_void_p, C.POINTER(SolveBatch), C.c_uint64, C.POINTER(PathRecords),
                                      C.POINTER(PointStats)]
     lib.spac_fk_assemble_rows.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch), C.c_uint64,
@@ -325,6 +333,47 @@ def ptr(arr, ctype):
     if arr is None:
         return C.cast(None, C.POINTER(ctype))
     return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+class DeviceArray:
+    """A device allocation made through the library's test hooks (include/spac_fk_testing.h),
+    so that a test needs no second HIP runtime in the process."""
+
+    def __init__(self, nbytes: int, lib: C.CDLL | None = None):
+        self.lib = lib or load()
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(self.lib.spac_debug_device_alloc(self.nbytes, C.byref(p)), self.lib)
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr, lib: C.CDLL | None = None):
+        import numpy as np
+
+        a = np.ascontiguousarray(arr)
+        d = cls(max(a.nbytes, 1), lib)
+        check(d.lib.spac_debug_copy_to_device(d.ptr, a.ctypes.data, a.nbytes), d.lib)
+        return d
+
+    def to_bytes(self) -> bytes:
+        buf = (C.c_char * self.nbytes)()
+        check(self.lib.spac_debug_copy_to_host(C.addressof(buf), self.ptr, self.nbytes), self.lib)
+        return bytes(buf)
+
+    def zero(self):
+        z = bytes(self.nbytes)
+        check(self.lib.spac_debug_copy_to_device(self.ptr, z, self.nbytes), self.lib)
+
+    def free(self):
+        if self.ptr:
+            self.lib.spac_debug_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def env_flag(name: str) -> bool:

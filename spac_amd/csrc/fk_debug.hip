@@ -65,3 +65,38 @@ extern "C" spac_status spac_debug_math(int32_t fn, const double* x, const double
     (void)hipFree(dx); (void)hipFree(dy); (void)hipFree(dout);
     return rc;
 }
+
+extern "C" spac_status spac_debug_device_alloc(size_t bytes, void** out) {
+    using namespace spac;
+    if (!out || bytes == 0) return set_error(SPAC_E_INVALID_ARG, "bad arguments");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(SPAC_E_NO_DEVICE, "no HIP device visible");
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) return set_error(SPAC_E_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return SPAC_OK;
+}
+
+extern "C" spac_status spac_debug_device_free(void* ptr) {
+    using namespace spac;
+    if (!ptr) return SPAC_OK;
+    hipError_t e = hipFree(ptr);
+    if (e != hipSuccess) return set_error(SPAC_E_HIP, "hipFree failed: %s", hipGetErrorString(e));
+    return SPAC_OK;
+}
+
+extern "C" spac_status spac_debug_copy_to_device(void* dst, const void* src, size_t bytes) {
+    using namespace spac;
+    if (!dst || !src) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_error(SPAC_E_HIP, "copy to device failed: %s", hipGetErrorString(e));
+    return SPAC_OK;
+}
+
+extern "C" spac_status spac_debug_copy_to_host(void* dst, const void* src, size_t bytes) {
+    using namespace spac;
+    if (!dst || !src) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_error(SPAC_E_HIP, "copy to host failed: %s", hipGetErrorString(e));
+    return SPAC_OK;
+}

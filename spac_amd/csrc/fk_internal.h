@@ -48,6 +48,16 @@ struct SolveSet {
     int64_t total_paths = 0;
 };
 
+/* Device arrays one path launch writes: the per-path records (SoA over the
+ * batch's paths) and the work-queue counter.  A Problem's own buffers, or the
+ * library's static scratch block (spac_fk_solve_points_device_noalloc). */
+struct RecordSet {
+    double *xt = nullptr, *yt = nullptr, *Y = nullptr, *Z = nullptr, *xi = nullptr, *tau = nullptr, *gval = nullptr;
+    int32_t* steps = nullptr;
+    int8_t* outcome = nullptr;
+    int32_t* queue = nullptr;
+};
+
 struct Problem {
     spac_problem_desc desc{};
     int device = 0;
@@ -73,6 +83,13 @@ struct Problem {
     double cv_dom[4] = {0.0, 0.0, 1.0, 1.0};
 
     DeviceBuffer seed_dev; /* seed read by captured kernels (plans) */
+    /* this device's address of the static scratch block (fk_kernels.hip), looked up when the
+     * handle is created; nullptr if the lookup failed (scratch_status holds its status) */
+    unsigned char* scratch = nullptr;
+    spac_status scratch_status = SPAC_OK;
+
+    /* the handle's own record buffers (valid after ensure_records) */
+    RecordSet own_records() const;
 
     spac_status ensure_records(int64_t total_paths);
     /* per-solve sigma(X0) slots for n solves (allocated here, never during capture) */
@@ -84,9 +101,13 @@ struct Problem {
      * exit values g(X_tau) / u(X_tau) of every record.  seed_dev != nullptr:
      * kernels read the seed from device memory instead of `seed`. */
     spac_status enqueue_paths(const SolveSet& ss, uint64_t seed, const uint64_t* seed_dev, int* grid_blocks,
-                              hipEvent_t after_path = nullptr);
-    spac_status enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode);
-    spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode);
+                              hipEvent_t after_path = nullptr, const RecordSet* recs = nullptr);
+    spac_status enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode,
+                               const RecordSet* recs = nullptr);
+    /* recs == nullptr: the handle's own records (grown here); otherwise the given arrays,
+     * with nothing allocated */
+    spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode,
+                    const RecordSet* recs = nullptr);
     /* pinned host staging of upload_solves and copy_out (DMA through pinned memory instead of
      * pageable copies) */
     void* stage = nullptr;

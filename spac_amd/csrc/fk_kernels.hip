@@ -881,8 +881,19 @@ spac_status Problem::ensure_solve_scratch(int32_t n, SolveSet* ss) {
     return SPAC_OK;
 }
 
+RecordSet Problem::own_records() const {
+    RecordSet r;
+    r.xt = (double*)rec_xt.ptr; r.yt = (double*)rec_yt.ptr;
+    r.Y = (double*)rec_Y.ptr; r.Z = (double*)rec_Z.ptr; r.xi = (double*)rec_xi.ptr;
+    r.tau = (double*)rec_tau.ptr; r.gval = (double*)rec_gval.ptr;
+    r.steps = (int32_t*)rec_steps.ptr; r.outcome = (int8_t*)rec_outcome.ptr;
+    r.queue = (int32_t*)queue.ptr;
+    return r;
+}
+
 spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint64_t* sdev, int* grid_blocks,
-                                   hipEvent_t after_path) {
+                                   hipEvent_t after_path, const RecordSet* recs) {
+    const RecordSet rs = recs ? *recs : own_records();
     PathArgs a;
     memset(&a, 0, sizeof a);
     for (int i = 0; i < 4; ++i) a.dom[i] = desc.domain_params[i];
@@ -906,11 +917,11 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
     a.items = ss.items;
     a.n_items = ss.n_items;
     a.chunks_per_solve = ss.chunks_per_solve;
-    a.queue = (int32_t*)queue.ptr;
-    a.xt = (double*)rec_xt.ptr; a.yt = (double*)rec_yt.ptr;
-    a.Y = (double*)rec_Y.ptr; a.Z = (double*)rec_Z.ptr; a.xi = (double*)rec_xi.ptr;
-    a.tau = (double*)rec_tau.ptr; a.gval = (double*)rec_gval.ptr;
-    a.steps = (int32_t*)rec_steps.ptr; a.outcome = (int8_t*)rec_outcome.ptr;
+    a.queue = rs.queue;
+    a.xt = rs.xt; a.yt = rs.yt;
+    a.Y = rs.Y; a.Z = rs.Z; a.xi = rs.xi;
+    a.tau = rs.tau; a.gval = rs.gval;
+    a.steps = rs.steps; a.outcome = rs.outcome;
     if (cv_mode == SPAC_CV_TABLE) {
         a.cv.table = (const float*)cv_table.ptr;
         a.cv.t = cv_t;
@@ -954,7 +965,9 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
 }
 
 /* Estimator reduction (no synchronisation). */
-spac_status Problem::enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode) {
+spac_status Problem::enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stats, int node_mode,
+                                    const RecordSet* recs) {
+    const RecordSet rs = recs ? *recs : own_records();
     ReduceArgs r;
     memset(&r, 0, sizeof r);
     r.n_solves = ss.n;
@@ -962,9 +975,9 @@ spac_status Problem::enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stat
     r.paths = ss.paths;
     r.paths_uniform = ss.paths_uniform;
     r.rec_off = ss.rec_off;
-    r.Y = (const double*)rec_Y.ptr; r.Z = (const double*)rec_Z.ptr; r.xi = (const double*)rec_xi.ptr;
-    r.tau = (const double*)rec_tau.ptr; r.gval = (const double*)rec_gval.ptr;
-    r.steps = (const int32_t*)rec_steps.ptr; r.outcome = (const int8_t*)rec_outcome.ptr;
+    r.Y = rs.Y; r.Z = rs.Z; r.xi = rs.xi;
+    r.tau = rs.tau; r.gval = rs.gval;
+    r.steps = rs.steps; r.outcome = rs.outcome;
     r.stats = d_stats;
     r.cv_fallback = (cv_mode == SPAC_CV_TABLE) ? cv_fallback : 0;
     r.cv_max_ratio = cv_max_ratio;
@@ -974,14 +987,17 @@ spac_status Problem::enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stat
 }
 
 /* Launch the persistent path kernel and, optionally, the reduction; synchronous, timed. */
-spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode) {
+spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode,
+                         const RecordSet* recs) {
     spac_status rc;
-    if ((rc = ensure_records(ss.total_paths))) return rc;
-    if ((rc = queue.ensure(64))) return rc;
+    if (!recs) {
+        if ((rc = ensure_records(ss.total_paths))) return rc;
+        if ((rc = queue.ensure(64))) return rc;
+    }
     int blocks = 0;
     SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
-    if ((rc = enqueue_paths(ss, seed, nullptr, &blocks, ev[1]))) return rc;
-    if (d_stats && (rc = enqueue_reduce(ss, d_stats, node_mode))) return rc;
+    if ((rc = enqueue_paths(ss, seed, nullptr, &blocks, ev[1], recs))) return rc;
+    if (d_stats && (rc = enqueue_reduce(ss, d_stats, node_mode, recs))) return rc;
     SPAC_HIP_CHECK(hipEventRecord(ev[2], stream));
     SPAC_HIP_CHECK(hipStreamSynchronize(stream));
     float ms_path = 0.f, ms_red = 0.f;
@@ -1038,6 +1054,50 @@ spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* sr
     return SPAC_OK;
 }
 
+/* ------------------------------------------------ static scratch block --
+ * Device memory of spac_fk_solve_points_device_noalloc: the per-path records,
+ * the per-solve sigma(X0) slots and the work-queue counter of one point solve
+ * of at most SPAC_NOALLOC_MAX_PATHS paths over SPAC_NOALLOC_MAX_SOLVES points.
+ * Every GPU the library runs on holds its own copy (at its own address) from
+ * the code object's load until the process ends.  Host code never passes the
+ * array's name to a kernel: it passes the address hipGetSymbolAddress returns
+ * for the current device (noalloc_scratch_address). */
+constexpr size_t kNaPaths = (size_t)SPAC_NOALLOC_MAX_PATHS, kNaSolves = (size_t)SPAC_NOALLOC_MAX_SOLVES;
+constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kNaXt = 0, kNaYt = kNaXt + 8 * kNaPaths, kNaY = kNaYt + 8 * kNaPaths, kNaZ = kNaY + 8 * kNaPaths,
+                 kNaXi = kNaZ + 8 * kNaPaths, kNaTau = kNaXi + 8 * kNaPaths, kNaGval = kNaTau + 8 * kNaPaths,
+                 kNaSteps = kNaGval + 8 * kNaPaths, kNaOutcome = kNaSteps + 4 * kNaPaths,
+                 kNaSig = na_align(kNaOutcome + kNaPaths), kNaSigf = kNaSig + 8 * kNaSolves,
+                 kNaQueue = na_align(kNaSigf + 4 * kNaSolves), kNaBytes = kNaQueue + 256;
+static_assert(kNaBytes == (size_t)SPAC_NOALLOC_SCRATCH_BYTES, "scratch layout and the header's size disagree");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+/* This device's address of `scratch`, looked up once per device (the device must be
+ * current); the status of the lookup is kept with it. */
+static spac_status noalloc_scratch_address(int device, unsigned char** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static unsigned char* addr[kMaxDevices] = {};
+    static hipError_t err[kMaxDevices] = {};
+    static bool done[kMaxDevices] = {};
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(SPAC_E_INVALID_ARG, "device %d outside the scratch table (%d devices)", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!done[device]) {
+        void* ptr = nullptr;
+        err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        addr[device] = err[device] == hipSuccess ? (unsigned char*)ptr : nullptr;
+        done[device] = true;
+    }
+    if (err[device] != hipSuccess || !addr[device])
+        return set_error(SPAC_E_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                         hipGetErrorString(err[device]));
+    *out = addr[device];
+    return SPAC_OK;
+}
+
 } // namespace spac
 
 using namespace spac;
@@ -1073,6 +1133,8 @@ spac_status spac_fk_problem_create(const spac_problem_desc* desc, spac_problem**
     p->launch.block_size = 256;
     p->launch.blocks_per_cu = 8;
     p->launch.chunk_paths = 64; /* paths per work item: smaller items balance the tail of heavy nodes */
+    /* set-up of the no-allocation entry point on this (current) device */
+    p->scratch_status = noalloc_scratch_address(p->device, &p->scratch);
     *out = reinterpret_cast<spac_problem*>(p);
     return SPAC_OK;
 }
@@ -1162,6 +1224,52 @@ spac_status spac_fk_solve_points_device(spac_problem* hp, const double* d_x0, co
     if (p->desc.precision != SPAC_FP64 && p->desc.precision != SPAC_FP32)
         return set_error(SPAC_E_INVALID_ARG, "bad precision");
     return solve_points_impl(p, d_x0, d_y0, n, paths, h, seed, d_stats);
+}
+
+spac_status spac_fk_solve_points_device_noalloc(spac_problem* hp, const double* d_x0, const double* d_y0, int32_t n,
+                                                uint32_t paths, double h, uint64_t seed, spac_point_stats* d_stats) {
+    if (!hp) return set_error(SPAC_E_INVALID_ARG, "null problem");
+    spac_status rc = check_points_args(d_x0, d_y0, n, paths, h, d_stats);
+    if (rc) return rc;
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    if (p->desc.precision != SPAC_FP64 && p->desc.precision != SPAC_FP32)
+        return set_error(SPAC_E_INVALID_ARG, "bad precision");
+    if ((size_t)n > kNaSolves)
+        return set_error(SPAC_E_INVALID_ARG, "no-allocation solve: %d points exceed the scratch maximum of %d", n,
+                         (int)SPAC_NOALLOC_MAX_SOLVES);
+    const uint64_t total = (uint64_t)n * (uint64_t)paths;
+    if (total > kNaPaths)
+        return set_error(SPAC_E_INVALID_ARG, "no-allocation solve: %llu paths exceed the scratch maximum of %d",
+                         (unsigned long long)total, (int)SPAC_NOALLOC_MAX_PATHS);
+    if (!p->scratch) return set_error(p->scratch_status ? p->scratch_status : SPAC_E_HIP,
+                                      "no scratch address for device %d", p->device);
+    int cur = -1;
+    SPAC_HIP_CHECK(hipGetDevice(&cur));
+    if (cur != p->device)
+        return set_error(SPAC_E_INVALID_ARG, "device %d is current, the problem lives on device %d", cur, p->device);
+    unsigned char* b = p->scratch;
+    RecordSet rs;
+    rs.xt = (double*)(b + kNaXt); rs.yt = (double*)(b + kNaYt);
+    rs.Y = (double*)(b + kNaY); rs.Z = (double*)(b + kNaZ); rs.xi = (double*)(b + kNaXi);
+    rs.tau = (double*)(b + kNaTau); rs.gval = (double*)(b + kNaGval);
+    rs.steps = (int32_t*)(b + kNaSteps); rs.outcome = (int8_t*)(b + kNaOutcome);
+    rs.queue = (int32_t*)(b + kNaQueue); /* zeroed by a kernel before the path kernel */
+    SolveSet ss;
+    ss.n = n;
+    ss.x0 = d_x0;
+    ss.y0 = d_y0;
+    ss.h_uniform = h;
+    ss.sqrth_uniform = std::sqrt(h);
+    ss.paths_uniform = paths;
+    ss.total_paths = (int64_t)total;
+    ss.chunk = work_item_paths((uint32_t)p->launch.chunk_paths, ss.total_paths);
+    ss.chunks_per_solve = (int32_t)((paths + ss.chunk - 1) / ss.chunk);
+    ss.n_items = ss.chunks_per_solve * n; /* <= total <= 2^19 */
+    ss.sig0 = (double*)(b + kNaSig);
+    ss.sig0f = (float*)(b + kNaSigf);
+    if ((rc = p->run(ss, seed, d_stats, 0, &rs))) return rc;
+    p->last.path_steps = -1;
+    return SPAC_OK;
 }
 
 spac_status spac_fk_solve_points(spac_problem* hp, const double* x0, const double* y0, int32_t n, uint32_t paths,
