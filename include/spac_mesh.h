@@ -99,8 +99,8 @@ spac_status spac_mesh_write_positions(const spac_mesh* m, int32_t coef_kind, con
 typedef struct {
     int32_t iterations;
     int32_t restarts;
-    double residual;            /* final ||B - G u|| / ||B|| */
-    int32_t converged;
+    double residual;            /* ||B - G u|| / ||B|| of the returned u, always a true residual */
+    int32_t converged;          /* 1 only when that true residual is below tol */
     int32_t reserved;
     double seconds;             /* wall time of the call */
     double device_ms;           /* GPU solver: kernel time on its stream (HIP events); host solver: 0 */

@@ -336,6 +336,13 @@ spac_status spac_mesh_system_set_coarse(const spac_mesh* hm, spac_system* hs, in
     if ((rc = spac_mesh_get_info(hm, &info))) return rc;
     const int n = d->n_rows;
     if (info.n_nodes != n) return set_error(SPAC_E_INVALID_ARG, "the mesh has %d nodes, the system %d rows", info.n_nodes, n);
+    spac_interface_table tab;
+    if ((rc = spac_mesh_interfaces(hm, &tab))) return rc;
+    if (table_signature(&tab) != d->h_table_sig)
+        return set_error(SPAC_E_INVALID_ARG, "the mesh is not the one the system was created from");
+    /* no coarse space until the new one is fully in place: a failure below leaves Jacobi GMRES,
+       never kernels over half-replaced aggregate arrays */
+    d->coarse_k = 0;
     std::vector<double> x(n), y(n);
     std::vector<int32_t> fr(n);
     if ((rc = spac_mesh_nodes(hm, x.data(), y.data(), fr.data()))) return rc;
@@ -522,9 +529,16 @@ spac_status spac_gmres_solve(int32_t n, const int32_t* Ip, const int32_t* J, con
             spmv(n, Ip, J, V, u, r.data());
             for (int i = 0; i < n; ++i) r[i] = B[i] - r[i];
             res = std::sqrt(dot(n, r.data(), r.data())) / bnorm;
-            conv = res < tol * 10.0;
-            if (res < tol) break;
+            conv = res < tol;
+            if (conv) break;
         }
+    }
+    /* max_iter ran out: report the true residual of the last iterate, not the Givens estimate */
+    if (!conv) {
+        spmv(n, Ip, J, V, u, r.data());
+        for (int i = 0; i < n; ++i) r[i] = B[i] - r[i];
+        res = std::sqrt(dot(n, r.data(), r.data())) / bnorm;
+        conv = res < tol;
     }
     if (info) {
         info->iterations = it;

@@ -337,6 +337,7 @@ struct GmresState {
         e0 = e1 = nullptr;
         stream = nullptr;
     }
+    ~GmresState() { release(); } /* error returns leave nothing behind */
 };
 
 } // namespace spac
@@ -583,9 +584,16 @@ static spac_status gmres_impl(int32_t n, const int32_t* Ip, const int32_t* J, co
             double rn = 0.0;
             if ((rc = true_residual(&rn))) return rc;
             res = rn / bnorm;
-            conv = res < tol * 10.0;
-            if (res < tol) break;
+            conv = res < tol;
+            if (conv) break;
         }
+    }
+    /* max_iter ran out: report the true residual of the last iterate, not the Givens estimate */
+    if (!conv) {
+        double rn = 0.0;
+        if ((rc = true_residual(&rn))) return rc;
+        res = rn / bnorm;
+        conv = res < tol;
     }
     GM_CHECK(hipEventRecord(st.e1, s));
     GM_CHECK(hipMemcpyAsync(u, du, N * 8, hipMemcpyDeviceToHost, s));

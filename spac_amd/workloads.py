@@ -1,0 +1,79 @@
+"""The synthetic workloads of the benchmark and of the full-size parity tests, defined once.
+
+- batched uneven rows: 16 x 16 circles on the unit square (overlap 1.2, 110 nodes per circle,
+  ~25k nodes), per-row paths log-uniform in [1e3, 1e5] and h log-uniform in [1e-4, 1e-3],
+  drawn from numpy's default_rng(0) in job order;
+- full scale: 64 x 64 circles (25 nodes per circle, 99,604 nodes, 186,920 rows), h scaled with
+  the circle radius squared so that paths take as many steps as at 4 x 4 circles and h = 5e-4;
+- the learned control variate's recipe: plain-MC point estimates at 4096 jittered grid points
+  as training data, full-batch Adam on the MLP, a T x T table from BF16 MFMA inference.
+
+Both meshes use the variable-coefficient operator with the manufactured Franke u
+(SPAC_COEF_VARCOEF_FRANKE) and the reference's mesh construction (Mesh/Malla.cpp:12-98).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import abi
+from .fk import FKProblem, problem_desc
+from .mesh import Mesh, mesh_desc
+
+UNIT_SQUARE = (0.0, 0.0, 1.0, 1.0)
+
+
+def batched_uneven():
+    """(mesh, paths[n_jobs] uint32, h[n_jobs]) of the batched uneven workload."""
+    mesh = Mesh(mesh_desc(n_side=16, nodes_per_circle=110))
+    n = mesh.info.n_jobs
+    rng = np.random.default_rng(0)
+    paths = np.round(10 ** rng.uniform(3, 5, n)).astype(np.uint32)
+    h = 10 ** rng.uniform(-4, -3, n)
+    return mesh, paths, h
+
+
+def full_scale():
+    """(mesh, h) of the 64 x 64-circle interface system."""
+    mesh = Mesh(mesh_desc(n_side=64, nodes_per_circle=25))
+    h = 5e-4 * (mesh.info.radius / (1.2 / (3 * np.sqrt(2.0)))) ** 2
+    return mesh, h
+
+
+def training_points(n_side=64, seed=0):
+    """4096 jittered points of a 64 x 64 grid on the unit square (the training inputs)."""
+    g = (np.arange(n_side) + 0.5) / n_side
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    rng = np.random.default_rng(seed)
+    xs = np.clip(X.ravel() + (rng.random(X.size) - 0.5) / n_side, 1e-3, 1 - 1e-3)
+    ys = np.clip(Y.ravel() + (rng.random(X.size) - 0.5) / n_side, 1e-3, 1 - 1e-3)
+    return xs, ys
+
+
+# settings of the learned control variate (the best row-variance ratio at 64 x 64 circles in
+# 1000 Adam steps; the sweep behind them is in the design notes)
+LEARNED_CV = dict(data_paths=500, data_h=2e-3, steps=1000, lr=(2e-2, 1e-4), beta2=0.95, grad_weight=0.003,
+                  table=256, seed=0)
+
+
+def train_learned_cv(net, data_paths=500, data_h=2e-3, steps=1000, lr=(2e-2, 1e-4), beta2=0.95,
+                     grad_weight=0.003, table=256, seed=0, eager=False):
+    """Train `net` (an mlp.MLP on the unit square) on plain-MC point estimates of the
+    variable-coefficient operator (FP32 stepping) and tabulate it.  Returns the device times of
+    the three parts (ms) and the training report."""
+    xs, ys = training_points()
+    with FKProblem(problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=UNIT_SQUARE,
+                                precision=abi.SPAC_FP32)) as pp:
+        pp.set_control_variate(abi.SPAC_CV_NONE)
+        st, _ = pp.solve_points(xs, ys, data_paths, data_h, seed=1)
+        data_ms = pp.timing().total_ms
+    u_mc = np.array([x["phi"] for x in st])
+    t0 = time.perf_counter()
+    rep = net.train(xs, ys, u_mc, steps=steps, lr=lr[0], lr_final=lr[1], grad_weight=grad_weight, eager=eager,
+                    beta2=beta2)
+    train_wall = (time.perf_counter() - t0) * 1e3
+    table_ms = net.build_table(table)
+    return {"training_data_ms": data_ms, "train_ms": rep["train_ms"], "train_wall_ms": train_wall,
+            "table_ms": table_ms, "train_loss": rep["final_loss"],
+            "training_data": {"points": int(xs.size), "paths": data_paths, "h": data_h}}

@@ -152,6 +152,24 @@ def test_gmres_against_spsolve_fixture():
     assert np.linalg.norm(A @ u - B) < 1e-11 * np.linalg.norm(B)
 
 
+@pytest.mark.parametrize("max_iter,restart", [(7, 50), (13, 5), (18, 18)])
+def test_gmres_iteration_cap_reports_true_residual(max_iter, restart):
+    """A solve stopped by max_iter reports converged = 0 and the true relative residual of the
+    u it returns (not the last Givens estimate), also when the cap falls inside a cycle."""
+    fx = json.loads((GOLDEN / "gmres_system.json").read_text())
+    Gi, Gj, Gv, B = (np.array(fx[k], dtype=t) for k, t in (("G_i", np.int32), ("G_j", np.int32), ("G_ij", float),
+                                                          ("B", float)))
+    u, info = gmres(Gi, Gj, Gv, B, tol=1e-15, max_iter=max_iter, restart=restart)
+    A = sp.csr_matrix((Gv, Gj, Gi), shape=(B.size, B.size))
+    true = np.linalg.norm(B - A @ u) / np.linalg.norm(B)
+    assert info.iterations == max_iter and not info.converged
+    assert info.residual == pytest.approx(true, rel=1e-9)
+    # a tolerance just above what the cap reaches is met and reported as met
+    u2, info2 = gmres(Gi, Gj, Gv, B, tol=2.0 * true, max_iter=max_iter, restart=restart)
+    assert info2.converged and info2.residual < 2.0 * true
+    assert info2.residual == pytest.approx(np.linalg.norm(B - A @ u2) / np.linalg.norm(B), rel=1e-9)
+
+
 def test_gmres_random_nonsymmetric():
     rng = np.random.default_rng(0)
     n = 300

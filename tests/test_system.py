@@ -147,6 +147,15 @@ def test_gpu_system_two_level_gmres(gpu):
         with pytest.raises(abi.SpacError) as e:
             abi.check(s.lib.spac_mesh_system_set_coarse(other.h, s.h, 4, 4), s.lib)
         assert e.value.code == abi.SPAC_E_INVALID_ARG
+        # same node count, other geometry: refused by the interface-table signature, and the
+        # refusal leaves the system on Jacobi GMRES
+        same_n = Mesh(mesh_desc(n_side=16, nodes_per_circle=40, overlap=1.3))
+        assert same_n.info.n_nodes == m.info.n_nodes
+        with pytest.raises(abi.SpacError) as e:
+            abi.check(s.lib.spac_mesh_system_set_coarse(same_n.h, s.h, 4, 4), s.lib)
+        assert e.value.code == abi.SPAC_E_INVALID_ARG
+        u_r, info_r = s.solve(tol=1e-10, max_iter=5000, restart=100)
+        assert np.array_equal(u_r.view(np.int64), u_j.view(np.int64))
         s.set_coarse(4, 4)
         u_c, info_c = s.solve(tol=1e-10, max_iter=5000, restart=100)
         u_c2, _ = s.solve(tol=1e-10, max_iter=5000, restart=100)
