@@ -1,15 +1,23 @@
 """Benchmark of the Feynman–Kac hot path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload point|rows]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ttr|rows|points|batched|train]
 
-One step = one pass of the hot path over the whole workload with inputs resident
-in HBM.  Prints one JSON line (metric path-steps/s) with a roofline object for
-the path kernel (HIP events on the kernel's stream, measured inside the library)
-and the CPU restatement timed on the host cores over a bounded sample.
+Default (--workload ttr): the headline line, time to a statistical RMS error of 1e-3 of the
+solved interface values on the full-scale system (64 x 64 circles, 99,604 nodes, 186,920
+rows), the learned control variate end to end against plain Monte Carlo on the same GPU.
+One step = one learned-CV solve from scratch: training data, training, table, assembly of
+every row at the calibrated path count into the device-resident system, and GMRES from a
+zero start.  Inputs (mesh, interface tables, job arrays) are resident on the device before
+the timed region.  Beside it: plain MC at its own calibrated count, bootstrap confidence
+intervals of the speed-ups, path-steps/s, the FP32-against-FP64 difference, the roofline of
+the dominant kernel (HIP events on its stream, measured inside the library), the PDDSparse
+rows and batched workloads, and the CPU restatement timed on the host cores over a bounded
+sample.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -21,30 +29,64 @@ sys.path.insert(0, str(ROOT))
 
 from spac_amd import abi  # noqa: E402
 from spac_amd.fk import FKProblem, SolveBatchArrays, problem_desc  # noqa: E402
-from spac_amd.flops import FLOPS_PER_STEP, PEAK_FP64_TFLOPS  # noqa: E402
+from spac_amd.flops import FLOPS_PER_STEP, PEAK_FP32_TFLOPS, PEAK_FP64_TFLOPS  # noqa: E402
 from spac_amd.mesh import DeviceSystem, Mesh, gmres, mesh_desc  # noqa: E402
+from spac_amd.workloads import LEARNED_CV, UNIT_SQUARE, batched_uneven, full_scale, train_learned_cv  # noqa: E402
+
+VC = abi.SPAC_COEF_VARCOEF_FRANKE
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# HBM bytes per launch of the path kernel, from two rocprofv3 --pmc passes of this
-# script (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by what the counters
-# read for plain device copies of 512 MiB in the same runs (--copy-mib 512):
-# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.  Commands and figures:
-# profiles/fk_path_kernel_fp64_franke_points_256x10k.md and
-# profiles/pddsparse_rows_fp64_4x4_1952rows_20k.md.
+# HBM bytes per launch of the path kernels, from two rocprofv3 --pmc passes of this script
+# (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by what the counters read for plain
+# device copies of 512 MiB in the same runs (--copy-mib 512): FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.
+# Commands and figures: profiles/fk_path_kernel_fp64_franke_points_256x10k.md,
+# profiles/pddsparse_rows_fp64_4x4_1952rows_20k.md, profiles/learned_cv_time_to_rms_64x64_fp32.md.
 PATH_KERNEL_TRAFFIC_BYTES = {
     "points": (85.5 * 2.0 + 507894.5) * 1024,
     "rows": (7177.5 * 2.0 + 7598712.3) * 1024,
 }
+# The learned-CV path kernel of the full-scale system, per path of the launch: measured at 72
+# paths per row (13,458,240 paths; FETCH_SIZE 95,095 KiB x 2.0, WRITE_SIZE 3,404,908 KiB x 1.0,
+# mean of the last four dispatches); its launches in the bench run at the calibrated count,
+# so the bytes per launch are this figure times the launch's paths.
+LEARNED_PATH_KERNEL_BYTES_PER_PATH = (95094.4 * 2.0 + 3404908.5) * 1024 / (186920 * 72)
 RECORD_BYTES_PER_PATH = 6 * 8 + 4 + 1  # x_tau, y_tau, Y, Z, xi, tau; steps; outcome
-# The path kernel is compute-bound on FP64 vector FMAs, not on matrix cores: "mfma" names the
-# compute roof, whose FP64 figure (78.6 TFLOP/s) is the same for the vector units and dense MFMA.
-COMPUTE_UNIT = "VALU FP64 (no matrix-core work on this path; FP64 vector peak = dense FP64 MFMA peak = 78.6 TFLOP/s)"
+COMPUTE_UNIT_64 = "VALU FP64 (no matrix-core work on this path; FP64 vector peak = dense FP64 MFMA peak = 78.6 TFLOP/s)"
+COMPUTE_UNIT_32 = "VALU FP32 (no matrix-core work in the path kernel; MI355X FP32 vector peak 157.3 TFLOP/s)"
 
 
+# ---------------------------------------------------------------- host --
+def cpu_threads():
+    """Threads for the CPU restatement: every CPU this process may run on (nproc), capped by
+    OMP_NUM_THREADS where the environment sets it (the GPU box grants each GPU a CPU share and
+    sets it)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def host_info():
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"cpu_model": model, "nproc": nproc, "os_cpu_count": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+# ------------------------------------------------------ config 1 points --
 def config1_points():
     t = -1.0 + 2.0 * (np.arange(16) + 1) / 17
     X, Y = np.meshgrid(t, t, indexing="ij")
@@ -56,9 +98,8 @@ def cpu_baseline_points(x, y, paths, h, seed, budget_s=20.0):
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as o
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     desc = problem_desc()
-    # calibrate on 2 points x 500 paths, then size the sample to ~budget_s
     idx = np.arange(0, x.size, max(1, x.size // 2))[:2]
     b = SolveBatchArrays(x[idx], y[idx], 500, h, idx)
     t0 = time.perf_counter()
@@ -73,70 +114,54 @@ def cpu_baseline_points(x, y, paths, h, seed, budget_s=20.0):
     rec = o.simulate(desc, b, seed=seed, nthreads=threads)
     dt = time.perf_counter() - t0
     steps = int(rec["steps"].astype(np.int64).sum())
-    return {"value": steps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
+    return {"value": steps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port", **host_info(),
             "sample": f"{npts} of the 256 grid points x {paths} paths, h={h} ({steps} path-steps, {dt:.1f} s)"}
 
 
 def bench_points(args):
-    import torch
-
+    """Config 1 on device-resident inputs (spac_fk_solve_points_device)."""
     x, y = config1_points()
     n, paths, h, seed = x.size, 10000, 1e-3, 0
-    dev = torch.device("cuda:0")
-    dx = torch.from_numpy(x).to(dev)
-    dy = torch.from_numpy(y).to(dev)
-    dstats = torch.zeros(n * abi.C.sizeof(abi.PointStats), dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
     prob = FKProblem(problem_desc())
     lib = prob.lib
+    dx, dy = abi.DeviceArray.from_numpy(x, lib), abi.DeviceArray.from_numpy(y, lib)
+    dstats = abi.DeviceArray(n * abi.C.sizeof(abi.PointStats), lib)
 
     def step():
-        abi.check(lib.spac_fk_solve_points_device(prob.h, dx.data_ptr(), dy.data_ptr(), n, paths, h, seed,
-                                                  dstats.data_ptr()), lib)
+        abi.check(lib.spac_fk_solve_points_device(prob.h, dx.ptr, dy.ptr, n, paths, h, seed, dstats.ptr), lib)
 
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
-    torch.cuda.synchronize()
     kern_ms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step()
+        step()  # synchronous
         kern_ms.append(prob.timing().path_kernel_ms)
-    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"timed {args.steps} steps in {elapsed:.3f} s")
-    host = dstats.cpu().numpy().view(np.uint8)
-    stats = (abi.PointStats * n).from_buffer_copy(host.tobytes())
+    stats = (abi.PointStats * n).from_buffer_copy(dstats.to_bytes())
     path_steps = int(sum(s.path_steps for s in stats))
     assert all(s.status == 0 for s in stats)
-    value = path_steps * args.steps / elapsed
     avg_ms = float(np.mean(kern_ms))
     fps = FLOPS_PER_STEP["franke_point_fp64"]
     achieved = path_steps * fps / (avg_ms * 1e-3) / 1e12
     out = {
-        "metric": "path-steps/s",
-        "value": value,
-        "unit": "path-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
+        "metric": "path-steps/s", "value": path_steps * args.steps / elapsed, "unit": "path-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "plain Feynman-Kac FP64, 2D Poisson with Franke data on [-1,1]^2, 16x16 interior grid, "
                                "10000 paths/point, h=1e-3, sigma=sqrt(2) I, Philox seed 0",
                    "points": n, "paths_per_point": paths, "h": h, "path_steps_per_step": path_steps},
-        "roofline": {"bound": "mfma", "compute_unit": COMPUTE_UNIT, "achieved": achieved,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": PATH_KERNEL_TRAFFIC_BYTES["points"],
-                     "traffic_unit": "B per launch", "algorithmic_bytes": RECORD_BYTES_PER_PATH * n * paths,
+        "roofline": {"bound": "mfma", "compute_unit": COMPUTE_UNIT_64, "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": PATH_KERNEL_TRAFFIC_BYTES["points"], "traffic_unit": "B per launch",
+                     "algorithmic_bytes": RECORD_BYTES_PER_PATH * n * paths,
                      "kernel": "fk_path_kernel_fp64<CoefFranke, SUB_NONE>", "kernel_avg_ms": avg_ms,
                      "flops_per_path_step": fps},
     }
+    for d in (dx, dy, dstats):
+        d.free()
     prob.close()
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -144,48 +169,57 @@ def bench_points(args):
     return out
 
 
-def cpu_baseline_rows(mesh, desc, paths, h, seed, budget_s=20.0):
-    """The CPU restatement's row assembly (OpenMP over paths) on a bounded sample of the rows."""
+# -------------------------------------------------------- config 2 rows --
+def cpu_baseline_rows(mesh, desc, paths, h, seed, budget_s=20.0, cv=None):
+    """The CPU restatement's row assembly (OpenMP over paths) on a bounded sample of the rows;
+    cv: oracle control-variate mode (None: the plugin F, the reference's behaviour)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as o
 
-    threads = min(16, os.cpu_count() or 1)
-    live = np.nonzero(mesh.job_frontera == 0)[0]
-    probe = live[:: max(1, live.size // 4)][:4]
-    t0 = time.perf_counter()
-    _, _, st = o.assemble_rows(desc, mesh, 1000, h, seed=seed, jobs=probe, nthreads=threads)
-    dt = time.perf_counter() - t0
-    per_job = dt / probe.size * (paths / 1000.0)
-    njobs = int(max(1, min(live.size, budget_s / max(per_job, 1e-9))))
-    jobs = live[np.linspace(0, live.size - 1, njobs).astype(np.int64)]
-    t0 = time.perf_counter()
-    _, _, st = o.assemble_rows(desc, mesh, paths, h, seed=seed, jobs=jobs, nthreads=threads)
-    dt = time.perf_counter() - t0
+    threads = cpu_threads()
+    if cv is not None:
+        o.set_cv(cv)
+    try:
+        live = np.nonzero(mesh.job_frontera == 0)[0]
+        probe = live[:: max(1, live.size // 4)][:4]
+        probe_paths = min(1000, paths)
+        t0 = time.perf_counter()
+        o.assemble_rows(desc, mesh, probe_paths, h, seed=seed, jobs=probe, nthreads=threads)
+        dt = time.perf_counter() - t0
+        per_job = dt / probe.size * (paths / probe_paths)
+        njobs = int(max(1, min(live.size, budget_s / max(per_job, 1e-9))))
+        jobs = live[np.linspace(0, live.size - 1, njobs).astype(np.int64)]
+        t0 = time.perf_counter()
+        _, _, st = o.assemble_rows(desc, mesh, paths, h, seed=seed, jobs=jobs, nthreads=threads)
+        dt = time.perf_counter() - t0
+    finally:
+        if cv is not None:
+            o.set_cv()
     steps = int(sum(x["path_steps"] for x in st))
-    return {"value": steps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{njobs} of the {mesh.info.n_jobs} rows x {paths} paths, h={h}, G and B included "
+    return {"value": steps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port", **host_info(),
+            "sample": f"{njobs} of the {mesh.info.n_jobs} rows x {paths} paths, h={h:.4g}, G and B included "
                       f"({steps} path-steps, {dt:.1f} s)"}
 
 
-def bench_rows(args):
-    """PDDSparse interface assembly: 4x4 circles on the unit square, 1016 nodes, 1952 rows."""
+def bench_rows(args, steps=None, warmup=None, with_cpu=True):
+    """PDDSparse interface assembly FP64: 4x4 circles on the unit square, 1016 nodes, 1952 rows."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     paths, h, seed = 20000, 5e-4, 0
     mesh = Mesh(mesh_desc())
-    desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=(0.0, 0.0, 1.0, 1.0))
+    desc = problem_desc(coef=VC, domain=UNIT_SQUARE)
     prob = FKProblem(desc)
-    for i in range(args.warmup):
+    for i in range(warmup):
         prob.assemble_rows(mesh, paths, h, seed=seed)
-        log(f"warmup {i + 1}/{args.warmup} done")
     dev_ms, path_ms, wall = [], [], []
     rows = None
-    for i in range(args.steps):
+    for i in range(steps):
         t0 = time.perf_counter()
         rows = prob.assemble_rows(mesh, paths, h, seed=seed)
         wall.append(time.perf_counter() - t0)
         t = prob.timing()
         dev_ms.append(t.total_ms)
         path_ms.append(t.path_kernel_ms)
-    log(f"timed {args.steps} steps")
     path_steps = int(sum(x["path_steps"] for x in rows["stats"]))
     n_paths = int(sum(x["paths"] for x in rows["stats"]))
     mesh.accumulate(rows)
@@ -201,55 +235,44 @@ def bench_rows(args):
     fps = FLOPS_PER_STEP["varcoef_circle_fp64"]
     achieved = path_steps * fps / (avg_path_ms * 1e-3) / 1e12
     out = {
-        "metric": "path-steps/s",
-        "value": path_steps / dev_s,
-        "unit": "path-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dev_s * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
+        "metric": "path-steps/s", "value": path_steps / dev_s, "unit": "path-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": dev_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "PDDSparse interface assembly FP64: unit square, 4x4 overlapping circles (overlap 1.2), "
                                "1016 nodes / 1952 rows, 20000 paths/row, h=5e-4, sigma=(1+0.5 sin(pi x) cos(pi y)) I, "
                                "b=0.3(y,-x), c=-1-x^2, manufactured Franke u, Gaussian RBF eps=3 with 15-node stencils",
                    "rows": int(mesh.info.n_jobs), "nodes": int(mesh.info.n_nodes), "paths_per_row": paths, "h": h,
                    "path_steps_per_step": path_steps},
-        "roofline": {"bound": "mfma", "compute_unit": COMPUTE_UNIT, "achieved": achieved,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": PATH_KERNEL_TRAFFIC_BYTES["rows"],
-                     "traffic_unit": "B per launch", "algorithmic_bytes": RECORD_BYTES_PER_PATH * n_paths,
+        "roofline": {"bound": "mfma", "compute_unit": COMPUTE_UNIT_64, "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": PATH_KERNEL_TRAFFIC_BYTES["rows"], "traffic_unit": "B per launch",
+                     "algorithmic_bytes": RECORD_BYTES_PER_PATH * n_paths,
                      "kernel": "fk_path_kernel_fp64<CoefVarFranke, SUB_CIRCLE>", "kernel_avg_ms": avg_path_ms,
                      "flops_per_path_step": fps},
         "pcie_inclusive_path_steps_per_s": path_steps / float(np.mean(wall)),
         "gmres": {"tol": 1e-10, "iterations": int(info.iterations), "residual": float(info.residual),
-                  "seconds_host": t_gmres, "rms_err_vs_exact": float(np.sqrt(np.mean(err ** 2))),
-                  "max_err_vs_exact": float(np.abs(err).max())},
+                  "converged": bool(info.converged), "seconds_host": t_gmres,
+                  "rms_err_vs_exact": float(np.sqrt(np.mean(err ** 2))), "max_err_vs_exact": float(np.abs(err).max())},
     }
     prob.close()
-    if not args.no_cpu:
-        log("cpu baseline ...")
+    if with_cpu and not args.no_cpu:
+        log("cpu baseline (rows) ...")
         out["cpu_baseline"] = cpu_baseline_rows(mesh, desc, paths, h, seed, budget_s=args.cpu_seconds)
+    mesh.close()
     return out
 
 
+# ------------------------------------------------ config 3 batched rows --
 def bench_batched(steps=2, seed=0, fraction=1.0 / 32):
-    """Uneven batch on one GPU: 16x16 circles (Nt = 110, ~25k nodes), per-row paths log-uniform in
-    [1e3, 1e5] and h log-uniform in [1e-4, 1e-3] (numpy default_rng(0)), FP32 stepping; the launch
+    """Uneven batch on one GPU (spac_amd.workloads.batched_uneven), FP32 stepping; the launch
     shape and refill order come from the autotuner (on the batch with paths x fraction), then the
     assembly runs as one captured graph (plan) per step."""
     from spac_amd.fk import RowsPlan
 
-    mesh = Mesh(mesh_desc(n_side=16, nodes_per_circle=110))
+    mesh, paths, h = batched_uneven()
     n = mesh.info.n_jobs
-    rng = np.random.default_rng(0)
-    paths = np.round(10 ** rng.uniform(3, 5, n)).astype(np.uint32)
-    h = 10 ** rng.uniform(-4, -3, n)
     live = mesh.job_frontera == 0
-    desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=(0.0, 0.0, 1.0, 1.0), precision=abi.SPAC_FP32)
+    desc = problem_desc(coef=VC, domain=UNIT_SQUARE, precision=abi.SPAC_FP32)
     out = {"workload": "16x16 circles (overlap 1.2, 110 nodes per circle), per-row paths log-uniform [1e3, 1e5], "
                        "h log-uniform [1e-4, 1e-3], FP32 stepping, FP64 compensated sums",
            "rows": int(n), "nodes": int(mesh.info.n_nodes), "paths_total": int(paths[live].astype(np.int64).sum())}
@@ -261,7 +284,6 @@ def bench_batched(steps=2, seed=0, fraction=1.0 / 32):
                            "chosen": {"block_size": c.block_size, "blocks_per_cu": c.blocks_per_cu,
                                       "chunk_paths": c.chunk_paths, "item_order": c.item_order},
                            "sample_fraction": fraction, "default_ms": rep.default_ms, "best_ms": rep.best_ms}
-        log(f"batched: autotune {out['autotune']}")
         with RowsPlan(prob, mesh, paths, h) as plan:
             plan.execute(seed=seed)  # warm-up
             ms, steps_total = [], 0
@@ -277,193 +299,288 @@ def bench_batched(steps=2, seed=0, fraction=1.0 / 32):
     return out
 
 
-def _ttr_mesh():
-    """Config-5 mesh: 64x64 circles on the unit square, 25 nodes per circle (~100k nodes);
-    h scaled with the circle radius squared so paths take as many steps as at 4x4 circles."""
-    mesh = Mesh(mesh_desc(n_side=64, nodes_per_circle=25))
-    h = 5e-4 * (mesh.info.radius / (1.2 / (3 * np.sqrt(2.0)))) ** 2
-    return mesh, h
-
-
+# ------------------------------------------- config 5 time to RMS 1e-3 --
 # GMRES relative-residual tolerance of the time-to-RMS solves: the reference's default
 # toleranciaGMRES (main_test_mpi.cpp:32).
 GMRES_TOL = 1e-8
-# Coarse grid of the two-level GMRES preconditioner in the time-to-RMS solves (aggregates per
-# side); 0 gives the Jacobi-only solver.  At 64x64 circles: 309 -> 67 iterations from a cold start.
+# Coarse grid of the two-level GMRES preconditioner (aggregates per side); 0: Jacobi only.
 COARSE_GRID = 12
 
 
-def _replicas(mesh, system, prob, paths, h, seeds, u0=None):
-    """Independent assemblies (one per seed) at `paths` per row into the device-resident system,
-    each solved there by GPU GMRES.
-
-    Returns per-replica timings and two error measures: the statistical RMS of the solved
-    node values (sample variance across replicas, averaged over nodes) and the RMS of the
-    row residuals G_k u_bar - B_k around the replica-mean solution (the per-row Monte-Carlo
-    noise, independent across rows).  The system is copied out for these statistics outside
-    the timed calls."""
-    import scipy.sparse as sp
-
-    out, us, mats = [], [], []
-    n = mesh.info.n_nodes
-    for k, s in enumerate(seeds):
-        t0 = time.perf_counter()
-        # the estimator records (path steps, fallbacks) of the set's first replica only
-        st = prob.assemble_system(mesh, system, paths, h, seed=s, stats=k == 0)
-        t1 = time.perf_counter()
-        t = prob.timing()
-        u, info = system.solve(tol=GMRES_TOL, max_iter=20000, restart=200, u0=u0)
-        t2 = time.perf_counter()
-        Gi, Gj, Gv, B = system.read()
-        u0 = u
-        us.append(u)
-        mats.append((sp.csr_matrix((Gv, Gj, Gi), shape=(n, n)), B))
-        out.append(dict(ms=t.total_ms, path_ms=t.path_kernel_ms,
-                        steps=int(st["path_steps"].sum()) if st is not None else None,
-                        fallbacks=int(st["cv_fallback"].sum()) if st is not None else None,
-                        gmres_ms=info.seconds * 1e3,
-                        gmres_device_ms=info.device_ms, gmres_it=int(info.iterations), converged=bool(info.converged),
-                        assemble_wall_ms=(t1 - t0) * 1e3, wall_ms=(t2 - t0) * 1e3))
-    U = np.array(us)
-    ubar = U.mean(0)
-    rms_u = float(np.sqrt(U.var(0, ddof=1).mean()))
-    R = np.array([A @ ubar - B for A, B in mats])
-    rms_row = float(np.sqrt(R.var(0, ddof=1).mean()))
-    return out, rms_u, rms_row, ubar
+def _solve_replica(prob, mesh, system, paths, h, seed, stats=False):
+    """One assembly of every row into the device-resident system and its GPU GMRES solve from a
+    zero start; returns (u, record).  All calls are synchronous, so the host clock brackets the
+    device work."""
+    t0 = time.perf_counter()
+    st = prob.assemble_system(mesh, system, paths, h, seed=seed, stats=stats)
+    t1 = time.perf_counter()
+    t = prob.timing()
+    u, info = system.solve(tol=GMRES_TOL, max_iter=20000, restart=200)
+    t2 = time.perf_counter()
+    if not info.converged:
+        raise RuntimeError(f"GMRES did not converge (seed {seed}): residual {info.residual:.3e}")
+    rec = dict(mc_ms=t.total_ms, path_ms=t.path_kernel_ms, assemble_wall_ms=(t1 - t0) * 1e3,
+               gmres_ms=(t2 - t1) * 1e3, gmres_device_ms=info.device_ms, gmres_it=int(info.iterations),
+               wall_ms=(t2 - t0) * 1e3)
+    if st is not None:
+        if not (st["status"] == 0).all():
+            raise RuntimeError(f"assembly status {int(st['status'].min())} (seed {seed})")
+        rec["path_steps"] = int(st["path_steps"].astype(np.int64).sum())
+        rec["fallback_rows"] = int(st["cv_fallback"].sum())
+    return u, rec
 
 
-def bench_ttr(target=1e-3, probe_paths=256, train_steps=1000, lr=(2e-2, 1e-4), n_rep=8, seed0=100, eager=False,
-              beta2=0.95, data_paths=500, data_h=2e-3, max_refine=3, refine_rep=24, grad_weight=0.003):
-    """Time to a statistical RMS error `target` of the solved interface values, learned control
-    variate against plain Monte Carlo, both FP32 stepping on this GPU.
+def _rms(U):
+    """Statistical RMS of the solved node values: sample variance across replicas, averaged over
+    nodes."""
+    return float(np.sqrt(np.asarray(U).var(0, ddof=1).mean()))
 
-    Statistical RMS at a path count: independent replicas (seeds), sample variance per node,
-    averaged over nodes.  The node errors are dominated by a few smooth modes of the solve, so
-    an RMS from R replicas scatters by roughly +-(1 / R)^(1/2); no single measurement decides
-    the path count.  Instead: a probe at probe_paths (n_rep replicas), then max_refine sets of
-    refine_rep replicas, each at the path count where RMS^2 N = a + b / N, fitted by weighted
-    least squares over every set so far, gives RMS = 0.9 x target (b: the higher-order term the
-    noise of the assembled matrix adds at small N).  The last set is the timed one.  `rms` is the
-    fit over every set (the last included) at the timed count; the last set's own measurement is
-    beside it.  The same rule serves both methods.  The learned-CV time
-    includes its training data (plain-MC point solves at 4096 points, data_paths paths each at
-    step data_h: the control variate needs grad u, not an unbiased u), training and
-    tabulation; GMRES (GPU, FP64, relative residual GMRES_TOL) is reported beside the
-    Monte-Carlo time."""
+
+def _boot_rms(U, n_boot, rng):
+    """Bootstrap distribution of _rms over replicas (resampled with replacement), from the
+    R x R Gram matrix of the centred replicas: with counts c, sum_i c_i |d_i - m|^2 =
+    sum_i c_i K_ii - c^T K c / R."""
+    D = np.asarray(U) - np.asarray(U).mean(0)
+    R, n = D.shape
+    K = D @ D.T
+    c = rng.multinomial(R, np.full(R, 1.0 / R), size=n_boot).astype(np.float64)
+    ss = c @ np.diag(K) - np.einsum("bi,ij,bj->b", c, K, c) / R
+    return np.sqrt(np.maximum(ss, 0.0) / ((R - 1) * n))
+
+
+def _fit_paths(sets, aim):
+    """Path count where RMS reaches `aim` under RMS^2 N = a + b / N, fitted by weighted least
+    squares over every replica set so far (weights = replicas); b >= 0."""
+    N = np.array([x[0] for x in sets], float)
+    y = np.array([x[1] ** 2 * x[0] for x in sets])
+    w = np.array([x[2] for x in sets], float)
+    A = np.stack([np.ones_like(N), 1.0 / N], 1) * np.sqrt(w)[:, None]
+    a, b = np.linalg.lstsq(A, y * np.sqrt(w), rcond=None)[0] if len(sets) > 1 else (y[0], 0.0)
+    if b < 0 or a <= 0:
+        a, b = float(np.sum(w * y) / np.sum(w)), 0.0
+    T2 = aim ** 2
+    return int(np.ceil((a + np.sqrt(a * a + 4.0 * b * T2)) / (2.0 * T2))), float(a), float(b)
+
+
+def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep, seed0, max_extra=2):
+    """Path count per row at which this method's own replica set measures RMS <= target.
+
+    A probe set at probe_paths, then two sets, each at the count where the fit over every set
+    so far gives 0.9 x target, then further sets (at most max_extra) while the last set's own
+    RMS is above the target, each at the count the last set's RMS scales to (RMS ~ N^-1/2) with
+    5 % margin.  The last set (its replicas and timings) is what the bootstrap and the
+    projection use."""
+    sets, hist = [], []
+    paths, reps, it = probe_paths, probe_rep, 0
+    while True:
+        us, recs = [], []
+        for k in range(reps):
+            u, r = _solve_replica(prob, mesh, system, paths, h, seed0 + 1000 * it + k, stats=(k == 0))
+            us.append(u)
+            recs.append(r)
+        U = np.array(us)
+        rms = _rms(U)
+        sets.append((paths, rms, reps))
+        hist.append({"paths_per_row": int(paths), "rms": rms, "replicas": reps})
+        log(f"ttr {name}: set {it}: rms {rms:.3e} at {paths} paths per row ({reps} replicas)")
+        if it >= 2 and rms <= target:
+            break
+        if it >= 2 + max_extra:
+            break
+        if it <= 1:
+            paths = _fit_paths(sets, 0.9 * target)[0]
+        else:
+            paths = int(np.ceil(paths * 1.05 * (rms / target) ** 2))
+        reps, it = rep, it + 1
+    return {"paths": int(paths), "U": U, "recs": recs, "rms": rms, "history": hist,
+            "reached_target": bool(rms <= target)}
+
+
+def _ci(x):
+    q = np.percentile(x, [2.5, 50.0, 97.5])
+    return {"lo": float(q[0]), "median": float(q[1]), "hi": float(q[2])}
+
+
+def bench_ttr(args):
+    """The headline: time to RMS 1e-3 on the full-scale system, learned CV against plain MC."""
     from spac_amd.mlp import MLP
     from spac_amd.problems import franke_u
 
-    mesh, h = _ttr_mesh()
-    dom = (0.0, 0.0, 1.0, 1.0)
-    desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=dom, precision=abi.SPAC_FP32)
+    target = 1e-3
+    mesh, h = full_scale()
     exact = franke_u(mesh.node_x, mesh.node_y)
-    res = {"target_rms": target, "replicas": {"probe": n_rep, "refinement": refine_rep},
-           "mesh": {"nodes": int(mesh.info.n_nodes), "rows": int(mesh.info.n_jobs),
+    system = DeviceSystem(mesh)
+    system.set_coarse(COARSE_GRID, COARSE_GRID)
+    desc32 = problem_desc(coef=VC, domain=UNIT_SQUARE, precision=abi.SPAC_FP32)
+    res = {"mesh": {"nodes": int(mesh.info.n_nodes), "rows": int(mesh.info.n_jobs),
                     "circles": int(mesh.info.n_interfaces), "h": h, "nnz": int(mesh.info.nnz)}}
 
-    aim = 0.9 * target
+    # ---- plain Monte Carlo: calibrate, then time
+    plain_prob = FKProblem(desc32)
+    plain_prob.set_control_variate(abi.SPAC_CV_NONE)
+    _solve_replica(plain_prob, mesh, system, 16, h, 1)  # warm-up (buffers, code objects)
+    cal_p = _calibrate("plain", plain_prob, mesh, system, h, target, args.ttr_probe_paths, args.ttr_replicas,
+                       args.ttr_refine_replicas, 100)
+    Np = cal_p["paths"]
+    _, first = _solve_replica(plain_prob, mesh, system, Np, h, 49999, stats=True)  # untimed: path-steps
+    plain_runs = []
+    for k in range(args.plain_steps):
+        _, r = _solve_replica(plain_prob, mesh, system, Np, h, 50000 + k)
+        r["path_steps"], r["fallback_rows"] = first["path_steps"], first["fallback_rows"]
+        plain_runs.append(r)
+    log(f"ttr plain: {Np} paths per row, {np.mean([r['wall_ms'] for r in plain_runs]):.1f} ms end to end")
 
-    def fit_paths(sets):
-        """Path count at which RMS reaches `aim` under RMS^2 N = a + b / N, fitted by least
-        squares over every replica set so far (weights = replicas); b >= 0."""
-        N = np.array([x[0] for x in sets], float)
-        y = np.array([x[1] ** 2 * x[0] for x in sets])
-        w = np.array([x[2] for x in sets], float)
-        A = np.stack([np.ones_like(N), 1.0 / N], 1) * np.sqrt(w)[:, None]
-        a, b = np.linalg.lstsq(A, y * np.sqrt(w), rcond=None)[0] if len(sets) > 1 else (y[0], 0.0)
-        if b < 0 or a <= 0:
-            a, b = float(np.sum(w * y) / np.sum(w)), 0.0
-        T2 = aim ** 2
-        return int(np.ceil((a + np.sqrt(a * a + 4.0 * b * T2)) / (2.0 * T2))), float(a), float(b)
+    # ---- learned control variate: calibrate with the trained network (training is deterministic,
+    # so every timed step retrains exactly this network), then time the whole method per step
+    cfg = {k: v for k, v in LEARNED_CV.items() if k != "seed"}
+    learned_prob = FKProblem(desc32)
+    data_prob = FKProblem(desc32)  # the training data's point solves
+    net = MLP(UNIT_SQUARE, seed=LEARNED_CV["seed"])
+    theta0 = net.params
+    train_learned_cv(net, **cfg, eager=args.train_eager, data_problem=data_prob)
+    net.attach(learned_prob, fallback=True)
+    _solve_replica(learned_prob, mesh, system, 16, h, 1)  # warm-up
+    cal_l = _calibrate("learned", learned_prob, mesh, system, h, target, args.ttr_probe_paths, args.ttr_replicas,
+                       args.ttr_refine_replicas, 200)
+    Nl = cal_l["paths"]
 
-    def measure(prob, name, extra):
-        paths, hist, sets = probe_paths, [], []
-        for it in range(max_refine + 1):
-            reps = n_rep if it == 0 else refine_rep
-            runs, rms, row, ubar = _replicas(mesh, system, prob, paths, h,
-                                             [seed0 + 1000 * it + k for k in range(reps)])
-            hist.append({"paths_per_row": paths, "rms": rms, "row_residual_rms": row, "replicas": reps})
-            sets.append((paths, rms, reps))
-            if it == 0:
-                rms0, row0 = rms, row
-            if it == max_refine:
-                break
-            paths, fa, fb = fit_paths(sets)
-            log(f"ttr {name}: rms {rms:.3e} (row {row:.3e}) at {hist[-1]['paths_per_row']} paths; "
-                f"fit a {fa:.3e} b {fb:.3e} -> {paths} paths")
-        # the RMS at the timed count from the fit over every set (the last included): a pooled
-        # estimate, steadier than the last set's own measurement
-        _, fa, fb = fit_paths(sets)
-        rms_fit = float(np.sqrt(fa / paths + fb / paths ** 2))
-        # the count where the fit reaches the target itself (both methods compared at equal RMS)
-        at_target = float((fa + np.sqrt(fa * fa + 4.0 * fb * target ** 2)) / (2.0 * target ** 2))
-        r = {"paths_per_row": paths, "rms": rms_fit, "rms_last_set": rms, "reached_target": bool(rms_fit <= target),
-             "row_residual_rms": row, "fit": {"a": fa, "b": fb, "aim_rms": aim,
-                                             "replicas": int(sum(x[2] for x in sets))},
-             "probe_rms": rms0, "probe_row_residual_rms": row0, "path_count_history": hist,
-             "mc_ms": float(np.mean([x["ms"] for x in runs])),
-             "paths_at_target": at_target,
-             "path_kernel_ms": float(np.mean([x["path_ms"] for x in runs])),
-             "gmres_ms": float(np.mean([x["gmres_ms"] for x in runs])),
-             "gmres_device_ms": float(np.mean([x["gmres_device_ms"] for x in runs])),
-             "assemble_wall_ms": float(np.mean([x["assemble_wall_ms"] for x in runs])),
-             "wall_ms": float(np.mean([x["wall_ms"] for x in runs])),
-             "gmres_iterations": runs[0]["gmres_it"], "path_steps": runs[0]["steps"],
-             "fallback_rows": runs[0]["fallbacks"], "rms_vs_exact": float(np.sqrt(np.mean((ubar - exact) ** 2)))}
-        r.update(extra)
-        log(f"ttr {name}: {r}")
-        return r
+    def learned_step(seed, stats=False):
+        t0 = time.perf_counter()
+        net.params = theta0  # fresh network: initial weights, Adam state and step counter reset
+        setup = train_learned_cv(net, **cfg, eager=args.train_eager, data_problem=data_prob)
+        net.attach(learned_prob, fallback=True)
+        t1 = time.perf_counter()
+        u, r = _solve_replica(learned_prob, mesh, system, Nl, h, seed, stats=stats)
+        t2 = time.perf_counter()
+        r.update(setup)
+        r["setup_wall_ms"] = (t1 - t0) * 1e3
+        r["step_wall_ms"] = (t2 - t0) * 1e3
+        return u, r
 
-    system = DeviceSystem(mesh)
-    # two-level GMRES: 12 x 12 aggregates of rows (~5 x 5 circles each) as coarse space, both methods
-    system.set_coarse(COARSE_GRID, COARSE_GRID)
-    with FKProblem(desc) as prob:
-        prob.set_control_variate(abi.SPAC_CV_NONE)
-        prob.assemble_system(mesh, system, 16, h, seed=1)  # warm-up
-        res["plain"] = measure(prob, "plain", {})
-    # learned control variate: training data, training, tabulation
-    g = (np.arange(64) + 0.5) / 64
-    X, Y = np.meshgrid(g, g, indexing="xy")
-    rng = np.random.default_rng(0)
-    xs = np.clip(X.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
-    ys = np.clip(Y.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
-    with FKProblem(problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=dom, precision=abi.SPAC_FP32)) as pp:
-        pp.set_control_variate(abi.SPAC_CV_NONE)
-        st, _ = pp.solve_points(xs, ys, data_paths, data_h, seed=1)
-        data_ms = pp.timing().total_ms
-    u_mc = np.array([x["phi"] for x in st])
-    with MLP(dom, seed=0) as net, FKProblem(desc) as prob:
-        rep = net.train(xs, ys, u_mc, steps=train_steps, lr=lr[0], lr_final=lr[1], grad_weight=grad_weight,
-                        eager=eager, beta2=beta2)
-        table_ms = net.build_table(256)
-        net.attach(prob, fallback=True)
-        res["learned"] = measure(prob, "learned", {"training_data_ms": data_ms, "training_data": {
-                                                       "points": int(xs.size), "paths": data_paths, "h": data_h},
-                                                   "train_ms": rep["train_ms"],
-                                                   "train_steps": train_steps, "train_lr": list(lr), "train_beta2": beta2 or 0.999,
-                                                   "train_grad_weight": grad_weight,
-                                                   "train_loss": rep["final_loss"],
-                                                   "table_ms": table_ms})
-    P, L = res["plain"], res["learned"]
-    for M in (P, L):
-        # Monte-Carlo time projected linearly in N to the count where the fitted RMS is the target
-        M["mc_ms_at_target"] = M["mc_ms"] * M["paths_at_target"] / M["paths_per_row"]
-    setup = L["training_data_ms"] + L["train_ms"] + L["table_ms"]
-    res["time_to_rms_ms"] = {"plain_mc": P["mc_ms"], "learned_mc": L["mc_ms"], "learned_setup": setup,
-                             "plain_with_gmres": P["mc_ms"] + P["gmres_ms"],
-                             "learned_with_setup_and_gmres": setup + L["mc_ms"] + L["gmres_ms"],
-                             "plain_wall": P["wall_ms"], "learned_wall_with_setup": setup + L["wall_ms"]}
-    res["row_variance_ratio_at_probe"] = (P["probe_row_residual_rms"] / L["probe_row_residual_rms"]) ** 2
-    res["speedup"] = {"mc_only": P["mc_ms"] / L["mc_ms"],
-                      "including_training": P["mc_ms"] / (setup + L["mc_ms"]),
-                      "end_to_end_with_gmres": (P["mc_ms"] + P["gmres_ms"]) / (setup + L["mc_ms"] + L["gmres_ms"]),
-                      "end_to_end_wall": P["wall_ms"] / (setup + L["wall_ms"])}
-    Pt, Lt = P["mc_ms_at_target"], L["mc_ms_at_target"]
-    res["speedup_at_target"] = {"mc_only": Pt / Lt, "including_training": Pt / (setup + Lt),
-                                "end_to_end_with_gmres": (Pt + P["gmres_ms"]) / (setup + Lt + L["gmres_ms"])}
+    # warm-up steps (the first also reads the estimator records: path-steps, fallback rows)
+    _, first = learned_step(60000, stats=True)
+    for i in range(1, args.warmup):
+        learned_step(60000 + i)
+    learned_runs, u_learned = [], None
+    for k in range(args.steps):
+        u_learned, r = learned_step(70000 + k)
+        learned_runs.append(r)
+    for r in learned_runs:
+        r["path_steps"], r["fallback_rows"] = first["path_steps"], first["fallback_rows"]
+    log(f"ttr learned: {Nl} paths per row, {np.mean([r['step_wall_ms'] for r in learned_runs]):.1f} ms end to end")
+
+    # ---- FP64 plain-MC reference run at the plain count (same seed as an FP32 run)
+    with FKProblem(problem_desc(coef=VC, domain=UNIT_SQUARE)) as p64:
+        p64.set_control_variate(abi.SPAC_CV_NONE)
+        u64, r64 = _solve_replica(p64, mesh, system, Np, h, 90000, stats=True)
+    u32, r32 = _solve_replica(plain_prob, mesh, system, Np, h, 90000, stats=True)
+    fp = {"paths_per_row": Np, "seed": 90000,
+          "plain_fp32_minus_fp64_rms": float(np.sqrt(np.mean((u32 - u64) ** 2))),
+          "plain_fp32_minus_fp64_max": float(np.abs(u32 - u64).max()),
+          "learned_minus_fp64_plain_rms": float(np.sqrt(np.mean((u_learned - u64) ** 2))),
+          "expected_if_independent": float(np.hypot(cal_p["rms"], cal_l["rms"])),
+          "fp64_path_steps": r64["path_steps"], "fp32_path_steps": r32["path_steps"],
+          "fp64_mc_ms": r64["mc_ms"]}
+    log(f"ttr fp32 vs fp64: {fp}")
+
+    # ---- summaries, projections at RMS = target and bootstrap intervals
+    def summary(cal, runs, learned):
+        R = cal["recs"]
+        mc = float(np.mean([r["mc_ms"] for r in runs]))
+        gm = float(np.mean([r["gmres_ms"] for r in runs]))
+        wall = float(np.mean([r["step_wall_ms" if learned else "wall_ms"] for r in runs]))
+        setup = float(np.mean([r["setup_wall_ms"] for r in runs])) if learned else 0.0
+        ubar = cal["U"].mean(0)
+        s = {"paths_per_row": cal["paths"], "rms_last_set": cal["rms"], "reached_target": cal["reached_target"],
+             "replicas_last_set": int(cal["U"].shape[0]), "path_count_history": cal["history"],
+             "end_to_end_wall_ms": wall, "mc_device_ms": mc,
+             "path_kernel_ms": float(np.mean([r["path_ms"] for r in runs])),
+             "assemble_wall_ms": float(np.mean([r["assemble_wall_ms"] for r in runs])),
+             "gmres_wall_ms": gm, "gmres_device_ms": float(np.mean([r["gmres_device_ms"] for r in runs])),
+             "gmres_iterations": runs[0]["gmres_it"], "path_steps": runs[0]["path_steps"],
+             "path_steps_per_s_path_kernel": runs[0]["path_steps"] / (np.mean([r["path_ms"] for r in runs]) * 1e-3),
+             "fallback_rows": runs[0]["fallback_rows"],
+             "rms_vs_exact": float(np.sqrt(np.mean((ubar - exact) ** 2))),
+             "timed_runs": len(runs)}
+        if learned:
+            s["setup_wall_ms"] = setup
+            for k in ("training_data_ms", "train_ms", "train_wall_ms", "table_ms", "train_loss"):
+                s[k] = float(np.mean([r[k] for r in runs]))
+            s["training_data"] = runs[0]["training_data"]
+            s["train"] = {"steps": cfg["steps"], "lr": list(cfg["lr"]), "beta2": cfg["beta2"],
+                          "grad_weight": cfg["grad_weight"], "table": cfg["table"]}
+        # the part of the wall time that scales with paths per row (assembly), and the rest
+        s["_scaling_ms"] = s["assemble_wall_ms"]
+        s["_fixed_ms"] = wall - s["assemble_wall_ms"]
+        return s
+
+    P = summary(cal_p, plain_runs, False)
+    L = summary(cal_l, learned_runs, True)
+    rng = np.random.default_rng(12345)
+    nb = args.bootstrap
+    bp = _boot_rms(cal_p["U"], nb, rng)
+    bl = _boot_rms(cal_l["U"], nb, rng)
+
+    def at_target(S, rms):
+        """wall time projected to RMS = target: the assembly scales with the path count."""
+        return S["_fixed_ms"] + S["_scaling_ms"] * (rms / target) ** 2
+
+    def mc_at_target(S, rms):
+        return S["mc_device_ms"] * (rms / target) ** 2
+
+    Tp, Tl = at_target(P, P["rms_last_set"]), at_target(L, L["rms_last_set"])
+    setup = L["setup_wall_ms"]
+    speed = {"end_to_end_wall_at_timed_counts": P["end_to_end_wall_ms"] / L["end_to_end_wall_ms"],
+             "mc_only_at_timed_counts": P["mc_device_ms"] / L["mc_device_ms"],
+             "end_to_end_wall_at_equal_rms": Tp / Tl,
+             "mc_only_at_equal_rms": mc_at_target(P, P["rms_last_set"]) / mc_at_target(L, L["rms_last_set"]),
+             "including_training_no_gmres_at_equal_rms":
+                 mc_at_target(P, P["rms_last_set"]) / (setup + mc_at_target(L, L["rms_last_set"]))}
+    ci = {"end_to_end_wall_at_equal_rms": _ci(at_target(P, bp) / at_target(L, bl)),
+          "mc_only_at_equal_rms": _ci(mc_at_target(P, bp) / mc_at_target(L, bl)),
+          "including_training_no_gmres_at_equal_rms": _ci(mc_at_target(P, bp) / (setup + mc_at_target(L, bl))),
+          "learned_end_to_end_ms_at_target": _ci(at_target(L, bl)),
+          "plain_end_to_end_ms_at_target": _ci(at_target(P, bp)),
+          "rms_plain_last_set": _ci(bp), "rms_learned_last_set": _ci(bl),
+          "bootstrap": f"{nb} resamples of each method's last replica set (replicas drawn with replacement); "
+                       "time at RMS = 1e-3 = fixed part + assembly x (RMS / 1e-3)^2"}
+    for S in (P, L):
+        S["end_to_end_ms_at_target"] = at_target(S, S["rms_last_set"])
+        del S["_scaling_ms"], S["_fixed_ms"]
+    res.update({"plain": P, "learned": L, "speedup": speed, "speedup_ci95": ci, "fp32_vs_fp64": fp,
+                "row_variance_ratio_note": "plain/learned variance at equal paths = "
+                                           f"{(P['rms_last_set'] ** 2 * P['paths_per_row']) / (L['rms_last_set'] ** 2 * L['paths_per_row']):.1f}"})
+
+    # roofline of the dominant kernel of the learned step: its path kernel (FP32, table CV)
+    fps = FLOPS_PER_STEP["varcoef_circle_fp32_table"]
+    kms = L["path_kernel_ms"]
+    achieved = L["path_steps"] * fps / (kms * 1e-3) / 1e12
+    n_paths = mesh.info.n_jobs * Nl
+    roof = {"bound": "mfma", "compute_unit": COMPUTE_UNIT_32, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": LEARNED_PATH_KERNEL_BYTES_PER_PATH * n_paths,
+            "traffic_unit": "B per launch", "algorithmic_bytes": RECORD_BYTES_PER_PATH * n_paths,
+            "kernel": "fk_path_kernel_fp32<CoefVarFranke32, SUB_CIRCLE, CV_TABLE>", "kernel_avg_ms": kms,
+            "flops_per_path_step": fps, "path_steps_per_launch": L["path_steps"]}
+    plain_prob.close()
+    learned_prob.close()
+    data_prob.close()
+    net.close()
     system.close()
-    mesh.close()
-    return res
+    return res, roof, mesh
+
+
+def cpu_baseline_ttr(mesh, h, plain, budget_s):
+    """The CPU restatement (FP64, plain Monte Carlo, OpenMP over paths) on a sample of the
+    full-scale rows at the plain GPU count; its time to RMS 1e-3 is projected from the measured
+    path-steps/s and the path-steps plain MC needs there (same estimator, same variance)."""
+    desc = problem_desc(coef=VC, domain=UNIT_SQUARE)
+    c = cpu_baseline_rows(mesh, desc, plain["paths_per_row"], h, 0, budget_s=budget_s, cv=abi.SPAC_CV_NONE)
+    steps_needed = plain["path_steps"] * (plain["rms_last_set"] / 1e-3) ** 2
+    rate = c["value"]
+    c.update({"value": steps_needed / rate * 1e3, "unit": "ms (projected time to RMS 1e-3, plain MC)",
+              "path_steps_per_s": rate, "path_steps_to_target": steps_needed})
+    return c
 
 
 def bench_train(steps=1000, lr=(2e-2, 1e-4), beta2=0.95, eager=False):
@@ -472,14 +589,14 @@ def bench_train(steps=1000, lr=(2e-2, 1e-4), beta2=0.95, eager=False):
     full-batch Adam steps after 20 warm-up steps."""
     from spac_amd.mlp import MLP
     from spac_amd.problems import franke_u
+    from spac_amd.workloads import training_points
 
-    g = (np.arange(64) + 0.5) / 64
-    X, Y = np.meshgrid(g, g, indexing="xy")
+    xs, ys = training_points()
     rng = np.random.default_rng(0)
-    xs = np.clip(X.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
-    ys = np.clip(Y.ravel() + (rng.random(4096) - 0.5) / 64, 1e-3, 1 - 1e-3)
+    rng.random(xs.size)
+    rng.random(xs.size)
     u = franke_u(xs, ys) + 0.01 * rng.standard_normal(xs.size)
-    with MLP((0.0, 0.0, 1.0, 1.0), seed=0) as net:
+    with MLP(UNIT_SQUARE, seed=0) as net:
         net.train(xs, ys, u, steps=20, lr=lr[0], lr_final=lr[1], grad_weight=0.003, eager=eager, beta2=beta2)
         rep = net.train(xs, ys, u, steps=steps, lr=lr[0], lr_final=lr[1], grad_weight=0.003, eager=eager,
                         beta2=beta2)
@@ -488,73 +605,105 @@ def bench_train(steps=1000, lr=(2e-2, 1e-4), beta2=0.95, eager=False):
 
 
 def copy_calibration(mib: int, reps: int = 3) -> None:
-    """Plain device copies of known size (torch's copy kernel, `mib` MiB read and
-    written per launch): the reference dispatches that scale rocprofv3's
-    FETCH_SIZE / WRITE_SIZE counters to bytes."""
-    import torch
-
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
+    """Plain device-to-device copies of known size (`mib` MiB read and written per dispatch):
+    the reference dispatches that scale rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    lib = abi.load()
+    a = abi.DeviceArray(mib << 20, lib)
+    b = abi.DeviceArray(mib << 20, lib)
+    a.zero()
     for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
+        abi.check(lib.spac_debug_copy_device(b.ptr, a.ptr, mib << 20), lib)
     log(f"copy calibration: {reps} copies of {mib} MiB")
-    del a, b
+    a.free()
+    b.free()
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=10, help="timed steps (ttr: learned-CV solves end to end)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rows", choices=["rows", "points", "batched", "ttr", "train"],
-                    help="rows (default line) | points | batched, ttr or train alone (for profiling)")
+    ap.add_argument("--workload", default="ttr", choices=["ttr", "rows", "points", "batched", "train"],
+                    help="ttr (default: the headline line) | rows | points | batched | train")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-RMS comparison (config 5)")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven workload (config 3)")
-    ap.add_argument("--train-steps", type=int, default=1000)
-    ap.add_argument("--train-lr", type=float, nargs=2, default=(2e-2, 1e-4), metavar=("LR0", "LR1"))
-    ap.add_argument("--train-beta2", type=float, default=0.95, help="Adam beta2 (0: 0.999)")
-    ap.add_argument("--train-grad-weight", type=float, default=0.003,
-                    help="loss weight of the gradient heads (lambda)")
-    ap.add_argument("--ttr-replicas", type=int, default=8, help="replicas of the probe")
-    ap.add_argument("--ttr-refine-replicas", type=int, default=24, help="replicas of each path-count refinement")
+    ap.add_argument("--no-extras", action="store_true", help="ttr: skip the rows and batched workloads")
+    ap.add_argument("--plain-steps", type=int, default=2, help="ttr: timed plain-MC solves at its count")
+    ap.add_argument("--ttr-probe-paths", type=int, default=256)
+    ap.add_argument("--ttr-replicas", type=int, default=8, help="replicas of the probe set")
+    ap.add_argument("--ttr-refine-replicas", type=int, default=32, help="replicas of each later set")
+    ap.add_argument("--bootstrap", type=int, default=4000)
+    ap.add_argument("--train-steps", type=int, default=LEARNED_CV["steps"])
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run plain device copies of this many MiB (counter calibration for profiles)")
     ap.add_argument("--train-eager", action="store_true",
-                    help="launch training steps one by one instead of replaying captured graphs (for kernel profiles)")
+                    help="launch training steps one by one instead of replaying captured graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by node batches "
               "(rows of one circle stay together), one GPU per share", file=sys.stderr)
         sys.exit(2)
+    LEARNED_CV["steps"] = args.train_steps
     if args.copy_mib > 0:
         copy_calibration(args.copy_mib)
     if args.workload == "batched":
         print(json.dumps({"batched": bench_batched(steps=args.steps)}), flush=True)
         return
     if args.workload == "train":
-        print(json.dumps({"train": bench_train(steps=args.train_steps, lr=tuple(args.train_lr), beta2=args.train_beta2,
-                                               eager=args.train_eager)}), flush=True)
+        print(json.dumps({"train": bench_train(steps=args.train_steps, eager=args.train_eager)}), flush=True)
         return
-    if args.workload == "ttr":
-        print(json.dumps({"time_to_rms": bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, refine_rep=args.ttr_refine_replicas,
-                                     eager=args.train_eager,
-                                     lr=tuple(args.train_lr), beta2=args.train_beta2,
-                                     grad_weight=args.train_grad_weight)}), flush=True)
+    if args.workload in ("rows", "points"):
+        out = bench_rows(args) if args.workload == "rows" else bench_points(args)
+        print(json.dumps(out), flush=True)
         return
-    out = bench_rows(args) if args.workload == "rows" else bench_points(args)
-    if not args.no_batched:
-        log("batched uneven workload (config 3) ...")
+    t_start = time.perf_counter()
+    res, roof, mesh = bench_ttr(args)
+    P, L = res["plain"], res["learned"]
+    out = {
+        "metric": "time to RMS err 1e-3 (learned-CV vs plain MC, same GPU)",
+        "value": L["end_to_end_wall_ms"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": L["end_to_end_wall_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "full-scale PDDSparse interface system: unit square, 64x64 circles (overlap 1.2, "
+                               "25 nodes per circle), 99,604 nodes / 186,920 rows, sigma=(1+0.5 sin(pi x) cos(pi y)) I, "
+                               "b=0.3(y,-x), c=-1-x^2, manufactured Franke u, Gaussian RBF eps=3 on 15-node stencils; "
+                               "learned control variate (MLP 2-64-64-64-3 trained on the GPU) + FP32 stepping with "
+                               "FP64 sums + two-level FP64 GMRES to 1e-8; one step = one solve end to end",
+                   "target_rms": 1e-3, "rms": "statistical RMS of the solved node values over independent replicas",
+                   "h": res["mesh"]["h"], "nodes": res["mesh"]["nodes"], "rows": res["mesh"]["rows"],
+                   "paths_per_row_learned": L["paths_per_row"], "paths_per_row_plain": P["paths_per_row"],
+                   "precision": "FP32 stepping; FP64 exit projection, compensated sums, rows and GMRES"},
+        "plain_mc_ms": P["end_to_end_wall_ms"],
+        "speedup": res["speedup"],
+        "speedup_ci95": res["speedup_ci95"],
+        "path_steps_per_s": {"learned_path_kernel": L["path_steps_per_s_path_kernel"],
+                             "plain_path_kernel": P["path_steps_per_s_path_kernel"]},
+        "rms_vs_exact": {"learned": L["rms_vs_exact"], "plain": P["rms_vs_exact"],
+                         "note": "replica-mean solution against the exact Franke u: the bias of the time step and "
+                                 "the RBF interpolation, shared by both methods and outside the statistical target"},
+        "fp32_vs_fp64": res["fp32_vs_fp64"],
+        "roofline": roof,
+        "time_to_rms": res,
+    }
+    if not args.no_cpu:
+        log("cpu baseline (full-scale rows, plain MC) ...")
+        out["cpu_baseline"] = cpu_baseline_ttr(mesh, res["mesh"]["h"], P, args.cpu_seconds)
+        out["cpu_baseline"]["gpu_learned_speedup"] = out["cpu_baseline"]["value"] / L["end_to_end_wall_ms"]
+    mesh.close()
+    if not args.no_extras:
+        log("PDDSparse rows FP64 (4x4 circles) ...")
+        out["rows_fp64"] = bench_rows(args, steps=5, warmup=1, with_cpu=False)
+        log("batched uneven workload (16x16 circles) ...")
         out["batched"] = bench_batched()
-    if not args.no_ttr:
-        log("time to RMS 1e-3 (config 5) ...")
-        out["time_to_rms"] = bench_ttr(train_steps=args.train_steps, n_rep=args.ttr_replicas, refine_rep=args.ttr_refine_replicas,
-                                     eager=args.train_eager,
-                                     lr=tuple(args.train_lr), beta2=args.train_beta2,
-                                     grad_weight=args.train_grad_weight)
+    out["bench_seconds"] = time.perf_counter() - t_start
     print(json.dumps(out), flush=True)
 
 

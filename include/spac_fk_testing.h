@@ -30,6 +30,9 @@ spac_status spac_debug_device_alloc(size_t bytes, void** out);
 spac_status spac_debug_device_free(void* ptr);
 spac_status spac_debug_copy_to_device(void* dst, const void* src, size_t bytes);
 spac_status spac_debug_copy_to_host(void* dst, const void* src, size_t bytes);
+/* device-to-device copy of `bytes` (a plain copy of known size: the calibration dispatch of
+ * the HBM traffic counters) */
+spac_status spac_debug_copy_device(void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus
 }

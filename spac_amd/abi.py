@@ -271,6 +271,7 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.spac_debug_device_free.argtypes = [C.c_void_p]
     lib.spac_debug_copy_to_device.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     lib.spac_debug_copy_to_host.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.spac_debug_copy_device.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     lib.spac_fk_simulate.argtypes = [C.c_void_p, C.POINTER(SolveBatch), C.c_uint64, C.POINTER(PathRecords),
                                      C.POINTER(PointStats)]
     lib.spac_fk_assemble_rows.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch), C.c_uint64,

@@ -100,3 +100,11 @@ extern "C" spac_status spac_debug_copy_to_host(void* dst, const void* src, size_
     if (e != hipSuccess) return set_error(SPAC_E_HIP, "copy to host failed: %s", hipGetErrorString(e));
     return SPAC_OK;
 }
+
+extern "C" spac_status spac_debug_copy_device(void* dst, const void* src, size_t bytes) {
+    using namespace spac;
+    if (!dst || !src) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) return set_error(SPAC_E_HIP, "device copy failed: %s", hipGetErrorString(e));
+    return SPAC_OK;
+}

@@ -141,7 +141,9 @@ class FKProblem:
         abi.check(self.lib.spac_fk_last_timing(self.h, C.byref(t)), self.lib)
         return t
 
-    def solve_points(self, x0, y0, paths, h, seed=0, allow_status=False):
+    def solve_points(self, x0, y0, paths, h, seed=0, allow_status=False, as_array=False):
+        """spac_fk_solve_points; the estimators as a list of dicts, or (as_array) as a numpy
+        structured array with the fields of spac_point_stats."""
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         y0 = np.ascontiguousarray(y0, dtype=np.float64)
         n = x0.size
@@ -150,7 +152,7 @@ class FKProblem:
                                            seed, stats)
         if not allow_status:
             abi.check(rc, self.lib)
-        return stats_to_dicts(stats), rc
+        return (np.ctypeslib.as_array(stats) if as_array else stats_to_dicts(stats)), rc
 
     def simulate(self, batch: SolveBatchArrays, seed=0, want_records=True, want_stats=True):
         rec = alloc_records(batch.total_paths) if want_records else None

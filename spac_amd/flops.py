@@ -27,11 +27,24 @@ Variable-coefficient operator, circle subdomain (the PDDSparse row workload):
   circle distance: 2 sub, 2 mul + add + sqrt, max, 2 div, 2 (mul, add),
     sub = 14; nearest-boundary compare 1                                          15
                                                                         total  205
+FP32 stepping (fk_path_kernel_fp32), same operator and circle subdomain:
+  the variable-coefficient total above, less t += h per step (tau = steps x h
+  at the exit)                                                                    -1
+  plain Monte Carlo (no control variate): no F = -sigma grad u (25), no xi
+  update (5)                                                                     -30
+                                                                 total (plain)   174
+  learned control variate (a tabulated grad u_hat, bilinear): no F plugin (25);
+    cell (x - lo) * inv for x and y 4, clamps 4, fractions and complements 4,
+    two channels x (4 mul + 2 add + 2 mul + add) 18 = 30; xi update
+    -s Y (gx dWx + gy dWy) 6 instead of 5                                         +6
+                                                                 total (table)   210
 """
 
 FLOPS_PER_STEP = {
     "franke_point_fp64": 153,
     "varcoef_circle_fp64": 205,
+    "varcoef_circle_fp32_plain": 174,
+    "varcoef_circle_fp32_table": 210,
 }
 
 # MI355X published peaks (AMD): FP64 vector 78.6 TFLOP/s, FP32 vector 157.3 TFLOP/s,

@@ -58,17 +58,23 @@ LEARNED_CV = dict(data_paths=500, data_h=2e-3, steps=1000, lr=(2e-2, 1e-4), beta
 
 
 def train_learned_cv(net, data_paths=500, data_h=2e-3, steps=1000, lr=(2e-2, 1e-4), beta2=0.95,
-                     grad_weight=0.003, table=256, seed=0, eager=False):
+                     grad_weight=0.003, table=256, eager=False, data_problem=None):
     """Train `net` (an mlp.MLP on the unit square) on plain-MC point estimates of the
-    variable-coefficient operator (FP32 stepping) and tabulate it.  Returns the device times of
-    the three parts (ms) and the training report."""
+    variable-coefficient operator (FP32 stepping) and tabulate it.  data_problem: an FP32
+    FKProblem of that operator to run the point solves on (one is made when None).  Returns the
+    device times of the three parts (ms) and the training report."""
     xs, ys = training_points()
-    with FKProblem(problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=UNIT_SQUARE,
-                                precision=abi.SPAC_FP32)) as pp:
+    own = data_problem is None
+    pp = FKProblem(problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=UNIT_SQUARE,
+                                precision=abi.SPAC_FP32)) if own else data_problem
+    try:
         pp.set_control_variate(abi.SPAC_CV_NONE)
-        st, _ = pp.solve_points(xs, ys, data_paths, data_h, seed=1)
+        st, _ = pp.solve_points(xs, ys, data_paths, data_h, seed=1, as_array=True)
         data_ms = pp.timing().total_ms
-    u_mc = np.array([x["phi"] for x in st])
+    finally:
+        if own:
+            pp.close()
+    u_mc = np.array(st["phi"])
     t0 = time.perf_counter()
     rep = net.train(xs, ys, u_mc, steps=steps, lr=lr[0], lr_final=lr[1], grad_weight=grad_weight, eager=eager,
                     beta2=beta2)
