@@ -264,6 +264,37 @@ spac_status spac_fk_assemble_system(spac_problem* p, const spac_interface_table*
                                     const spac_node_batch* nodes, uint64_t seed, spac_system* sys,
                                     spac_point_stats* stats);
 
+/* Persistence and resume of row assemblies.  The reference persists only the finished
+ * system and its solution (Escribe_GB_COO / Escribe_GB_CSR, Mesh/SistemaLineal.cpp:96-138;
+ * solution_GMRES.txt, Solvers/SistemaLinealSolverGMRES_OMP.hpp:606); these keep the rows of
+ * every node batch, so that a batched run restarts at the batch it stopped in. */
+typedef struct {
+    int32_t batches;   /* node batches in the call */
+    int32_t computed;  /* assembled on the device (and written) */
+    int32_t loaded;    /* read from their files */
+    int32_t rejected;  /* files present but for other inputs or damaged: recomputed */
+} spac_resume_report;
+
+/* spac_fk_assemble_rows over `nodes` in batches of batch_nodes nodes.  Batch b's rows are
+ * read from dir/rows_<b>.bin when that file holds the rows of exactly these inputs (the
+ * interface table, the batch's nodes, seed; checked by hash) and is whole (payload hash);
+ * otherwise the batch is assembled and its file written (to a .tmp name, then renamed).
+ * Results equal spac_fk_assemble_rows on the whole batch bit for bit (rows depend only on
+ * seed, node stream id and path id).  out->stats, when given, receives every node's
+ * estimators. */
+spac_status spac_fk_assemble_rows_resumable(spac_problem* p, const spac_interface_table* ifaces,
+                                            const spac_node_batch* nodes, uint64_t seed, int32_t batch_nodes,
+                                            const char* dir, spac_row_block* out, spac_resume_report* report);
+/* One batch's row file (host only): rows->G_offset[0..n] index rows->G / rows->G_col, and
+ * rows->stats (required) holds the n estimators; batch, seed and input_hash identify it. */
+spac_status spac_rows_file_write(const char* path, int64_t batch, uint64_t seed, uint64_t input_hash, int32_t n,
+                                 const spac_row_block* rows);
+/* Read it back into rows (G from rows->G_offset[0] on, stats optional) when the file exists
+ * and matches (batch, seed, input_hash, n, entries) and its payload hash: *found = 1; a
+ * missing file gives 0, a mismatched or damaged one -1 (rows untouched in both cases). */
+spac_status spac_rows_file_read(const char* path, int64_t batch, uint64_t seed, uint64_t input_hash, int32_t n,
+                                int64_t entries, spac_row_block* rows, int32_t* found);
+
 /* Timing of the last call's kernels, measured with HIP events on the handle's
  * stream: path kernel, reduction kernels (ms), and launches counted. */
 typedef struct {

@@ -127,6 +127,13 @@ spac_status spac_gmres_solve_gpu(int32_t n, const int32_t* G_i, const int32_t* G
  * Calls that use one system (assembly, solve, read) must not overlap. */
 spac_status spac_gmres_solve_system(spac_system* s, double* u, double tol, int32_t max_iter, int32_t restart,
                                     spac_gmres_info* info);
+/* The solution file of the reference's GMRES driver: one value per line, fixed notation with
+ * 12 decimals (Output/solution_GMRES.txt, Solvers/SistemaLinealSolverGMRES_OMP.hpp:606-611). */
+spac_status spac_write_solution(const char* path, int32_t n, const double* u);
+/* Per-phase wall times as "phase,rank,seconds" lines, the format of the reference's
+ * Output/times.csv (main_test_mpi.cpp:62,73,114-187); append != 0 adds to an existing file. */
+spac_status spac_write_times_csv(const char* path, int32_t n, const char* const* phases, const int32_t* ranks,
+                                 const double* seconds, int32_t append);
 /* Read a COO file ("nnz n" then "i j v") into CSR (Lee_GB_COO). Call with
  * G_i == NULL to get n and nnz first. */
 spac_status spac_read_coo(const char* path, int32_t* n, int32_t* nnz, int32_t* G_i, int32_t* G_j, double* G_ij);

@@ -170,6 +170,10 @@ class RowBlock(C.Structure):
     ]
 
 
+class ResumeReport(C.Structure):
+    _fields_ = [("batches", C.c_int32), ("computed", C.c_int32), ("loaded", C.c_int32), ("rejected", C.c_int32)]
+
+
 class Timing(C.Structure):
     _fields_ = [
         ("path_kernel_ms", C.c_double),
@@ -279,6 +283,13 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.spac_fk_assemble_system.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
                                             C.c_uint64, C.c_void_p, C.POINTER(PointStats)]
     lib.spac_fk_last_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
+    lib.spac_fk_assemble_rows_resumable.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
+                                                    C.c_uint64, C.c_int32, C.c_char_p, C.POINTER(RowBlock),
+                                                    C.POINTER(ResumeReport)]
+    lib.spac_rows_file_write.argtypes = [C.c_char_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_int32,
+                                         C.POINTER(RowBlock)]
+    lib.spac_rows_file_read.argtypes = [C.c_char_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_int32, C.c_int64,
+                                        C.POINTER(RowBlock), C.POINTER(C.c_int32)]
     lib.spac_fk_set_launch.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
     lib.spac_fk_set_control_variate.argtypes = [C.c_void_p, C.POINTER(CvConfig)]
     lib.spac_fk_plan_create.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),

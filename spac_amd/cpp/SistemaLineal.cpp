@@ -125,6 +125,14 @@ bool SistemaLineal::Escribe_GB_CSR(const std::string& dir) const {
     return ok;
 }
 
+bool SistemaLineal::Escribe_Solucion(const std::string& path) const {
+    FILE* fp = std::fopen(path.c_str(), "w");
+    if (!fp) return false;
+    bool ok = true;
+    for (size_t i = 0; i < u.size() && ok; ++i) ok = std::fprintf(fp, "%.12f\n", u[i]) > 0;
+    return (std::fclose(fp) == 0) && ok;
+}
+
 /* Header "nnz n" then "i j v" lines sorted by row (Mesh/SistemaLineal.cpp:139-164).
  * Rows without entries get an empty range (the reference leaves them unset). */
 bool SistemaLineal::Lee_GB_COO(const std::string& path) {

@@ -35,6 +35,9 @@ public:
     /* Text formats of Mesh/SistemaLineal.cpp:96-164 */
     bool Escribe_GB_COO(const std::string& G_path = "Output/G.txt", const std::string& B_path = "Output/B.txt") const;
     bool Escribe_GB_CSR(const std::string& dir = "Output") const;
+    /* u, one value per line with 12 decimals (Output/solution_GMRES.txt of the GMRES driver,
+     * Solvers/SistemaLinealSolverGMRES_OMP.hpp:606-611) */
+    bool Escribe_Solucion(const std::string& path = "Output/solution_GMRES.txt") const;
     bool Lee_GB_COO(const std::string& path = "Output/G_in.txt");
 };
 

@@ -32,18 +32,6 @@
 
 namespace spac {
 
-/* ------------------------------------------------------------- errors -- */
-static thread_local std::string g_last_error;
-
-spac_status set_error(spac_status code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    return code;
-}
 
 #define SPAC_HIP_CHECK(expr)                                                                   \
     do {                                                                                       \
@@ -1105,8 +1093,6 @@ using namespace spac;
 /* ================================================================ C ABI == */
 extern "C" {
 
-const char* spac_last_error(void) { return g_last_error.c_str(); }
-int32_t spac_abi_version(void) { return SPAC_ABI_VERSION; }
 
 int32_t spac_device_count(void) {
     int n = 0;

@@ -175,6 +175,20 @@ class FKProblem:
         rows["rc"] = rc
         return rows
 
+    def assemble_rows_resumable(self, mesh, paths, h, directory, batch_nodes, seed=0, jobs=None):
+        """spac_fk_assemble_rows_resumable: rows in node batches of batch_nodes, each read from
+        directory/rows_<b>.bin when it holds these inputs' rows, else assembled and written.
+        Returns (rows, report)."""
+        nb, keep = mesh.node_batch(paths, h, jobs)
+        rows = mesh.alloc_rows(jobs)
+        rb = mesh.row_block(rows)
+        rep = abi.ResumeReport()
+        abi.check(self.lib.spac_fk_assemble_rows_resumable(self.h, C.byref(mesh.table), C.byref(nb), seed, batch_nodes,
+                                                           str(directory).encode(), C.byref(rb), C.byref(rep)),
+                  self.lib)
+        rows["stats"] = stats_to_dicts(rows["stats"])
+        return rows, rep
+
     def assemble_system(self, mesh, system, paths, h, seed=0, stats=True):
         """spac_fk_assemble_system: every job of the mesh, accumulated into the device-resident
         system (a mesh.DeviceSystem of the same mesh).  Returns the per-job estimator records as a

@@ -80,6 +80,8 @@ def _setup(lib):
     lib.spac_mesh_system_set_coarse.argtypes = [vp, vp, C.c_int32, C.c_int32]
     lib.spac_gmres_solve_system.argtypes = [vp, abi.dp, C.c_double, C.c_int32, C.c_int32, C.POINTER(GmresInfo)]
     lib.spac_read_coo.argtypes = [C.c_char_p, abi.i32p, abi.i32p, abi.i32p, abi.i32p, abi.dp]
+    lib.spac_write_solution.argtypes = [C.c_char_p, C.c_int32, abi.dp]
+    lib.spac_write_times_csv.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_char_p), abi.i32p, abi.dp, C.c_int32]
     lib._mesh_ready = True
     return lib
 
@@ -297,3 +299,21 @@ def gmres(Gi, Gj, Gv, B, tol=1e-10, max_iter=2000, restart=100, u0=None, device=
                                    abi.ptr(np.ascontiguousarray(B, np.float64), C.c_double),
                                    abi.ptr(u, C.c_double), tol, max_iter, restart, C.byref(info)), lib)
     return u, info
+
+
+def write_solution(path, u):
+    """spac_write_solution: the reference's solution_GMRES.txt format."""
+    lib = _setup(abi.load())
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    abi.check(lib.spac_write_solution(str(path).encode(), u.size, abi.ptr(u, C.c_double)), lib)
+
+
+def write_times_csv(path, rows, append=False):
+    """spac_write_times_csv: rows of (phase, rank, seconds), the reference's times.csv format."""
+    lib = _setup(abi.load())
+    n = len(rows)
+    names = (C.c_char_p * max(n, 1))(*[r[0].encode() for r in rows])
+    ranks = np.array([r[1] for r in rows], dtype=np.int32)
+    secs = np.array([r[2] for r in rows], dtype=np.float64)
+    abi.check(lib.spac_write_times_csv(str(path).encode(), n, names, abi.ptr(ranks, C.c_int32),
+                                       abi.ptr(secs, C.c_double), 1 if append else 0), lib)

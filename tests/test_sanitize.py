@@ -1,0 +1,47 @@
+"""The host code and the CPU restatement under AddressSanitizer + UndefinedBehaviorSanitizer.
+
+tests/cpp/sanitize_driver.cpp runs the mesh and interface-table builder (Malla, Interfaz,
+Calcula_Psi), the CSR pattern and Actualiza_GB, the host GMRES, the text formats and the row
+files, with every row assembled by the restatement (oracle/fk_oracle.c, OpenMP).  The
+sources are compiled with g++/gcc (no HIP code is involved) and any sanitizer report ends the
+run with a failure.
+"""
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+CPP = ROOT / "spac_amd" / "cpp"
+HOST_SOURCES = ["spac_error.cpp", "Malla.cpp", "Interfaz.cpp", "SistemaLineal.cpp", "BVPs.cpp", "mesh_capi.cpp",
+                "rows_io.cpp"]
+SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-g", "-O1"]
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or shutil.which("gcc") is None, reason="needs g++ and gcc")
+def test_host_code_and_restatement_clean_under_asan_ubsan(tmp_path):
+    inc = [f"-I{ROOT / 'include'}", f"-I{ROOT / 'spac_amd' / 'csrc'}", f"-I{CPP}"]
+    objs = []
+    for src in HOST_SOURCES:
+        o = tmp_path / (src + ".o")
+        subprocess.run(["g++", "-std=c++17", "-ffp-contract=off", *SAN, *inc, "-c", str(CPP / src), "-o", str(o)],
+                       check=True)
+        objs.append(str(o))
+    o = tmp_path / "fk_oracle.o"
+    subprocess.run(["gcc", "-std=c11", "-fopenmp", "-ffp-contract=off", *SAN, *inc, "-c",
+                    str(ROOT / "oracle" / "fk_oracle.c"), "-o", str(o)], check=True)
+    objs.append(str(o))
+    exe = tmp_path / "sanitize_driver"
+    subprocess.run(["g++", "-std=c++17", *SAN, *inc, str(ROOT / "tests" / "cpp" / "sanitize_driver.cpp"), *objs,
+                    "-fopenmp", "-lm", "-o", str(exe)], check=True)
+    out_dir = tmp_path / "Output"
+    out_dir.mkdir()
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([str(exe), str(out_dir), "2"], capture_output=True, text=True, env=env, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "sanitized run ok" in r.stdout
+    assert (out_dir / "solution_GMRES.txt").exists() and (out_dir / "times.csv").exists()
