@@ -295,6 +295,25 @@ spac_status spac_rows_file_write(const char* path, int64_t batch, uint64_t seed,
 spac_status spac_rows_file_read(const char* path, int64_t batch, uint64_t seed, uint64_t input_hash, int32_t n,
                                 int64_t entries, spac_row_block* rows, int32_t* found);
 
+/* Asynchronous forms (the reference's calls block inside their OpenMP regions,
+ * Solvers/FeynmanKacSolver.cpp:137-197): the same work as spac_fk_solve_points_device
+ * and spac_fk_assemble_system, enqueued on the handle's stream; the call returns once the
+ * inputs are checked and uploaded and the kernels are launched.  spac_fk_wait blocks until the
+ * call in flight is done and returns its status (the estimator records of
+ * spac_fk_assemble_system_async reach `stats` then; spac_fk_last_timing is valid after it);
+ * spac_fk_query tells whether it is done without blocking.  One call may be in flight per
+ * handle: every other call on the handle returns SPAC_E_INVALID_ARG until spac_fk_wait (or
+ * spac_fk_problem_destroy, which waits).  Device inputs, outputs and the system must stay
+ * valid and untouched until then. */
+spac_status spac_fk_solve_points_device_async(spac_problem* p, const double* d_x0, const double* d_y0, int32_t n,
+                                              uint32_t paths, double h, uint64_t seed, spac_point_stats* d_stats);
+spac_status spac_fk_assemble_system_async(spac_problem* p, const spac_interface_table* ifaces,
+                                          const spac_node_batch* nodes, uint64_t seed, spac_system* sys,
+                                          spac_point_stats* stats);
+spac_status spac_fk_wait(spac_problem* p);
+/* *done = 1 when nothing is in flight on the handle or the call in flight has finished */
+spac_status spac_fk_query(spac_problem* p, int32_t* done);
+
 /* Timing of the last call's kernels, measured with HIP events on the handle's
  * stream: path kernel, reduction kernels (ms), and launches counted. */
 typedef struct {

@@ -10,7 +10,7 @@
  * per use, exactly as Paso_Absorbente and Dentro call them
  * (Solvers/FeynmanKacSolver.cpp:7-18, 68-106), so that it doubles as the
  * "reference OpenMP CPU path" timed beside the GPU.  Differences from the
- * reference, all deliberate and listed in DESIGN.md:
+ * reference, all deliberate:
  *   - Brownian increments come from Philox4x32-10 + Box–Muller keyed by
  *     (seed, stream) and counted by (step, path) instead of one boost::mt19937
  *     per OpenMP thread (Solvers/FeynmanKacSolver.cpp:2-6, 107-136), so a path

@@ -282,6 +282,11 @@ def load(path: Path | None = None) -> C.CDLL:
                                           C.POINTER(RowBlock)]
     lib.spac_fk_assemble_system.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
                                             C.c_uint64, C.c_void_p, C.POINTER(PointStats)]
+    lib.spac_fk_assemble_system_async.argtypes = lib.spac_fk_assemble_system.argtypes
+    lib.spac_fk_solve_points_device_async.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32,
+                                                      C.c_double, C.c_uint64, C.c_void_p]
+    lib.spac_fk_wait.argtypes = [C.c_void_p]
+    lib.spac_fk_query.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     lib.spac_fk_last_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
     lib.spac_fk_assemble_rows_resumable.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
                                                     C.c_uint64, C.c_int32, C.c_char_p, C.POINTER(RowBlock),

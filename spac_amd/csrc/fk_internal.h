@@ -58,6 +58,8 @@ struct RecordSet {
     int32_t* queue = nullptr;
 };
 
+struct DeviceSystem;
+
 struct Problem {
     spac_problem_desc desc{};
     int device = 0;
@@ -91,6 +93,19 @@ struct Problem {
     /* the handle's own record buffers (valid after ensure_records) */
     RecordSet own_records() const;
 
+    /* The asynchronous call in flight on the handle's stream (spac_fk_*_async), which
+     * spac_fk_wait completes; kind 0: none.  Every other call on the handle refuses to run
+     * while one is pending. */
+    struct Pending {
+        int32_t kind = 0; /* PENDING_* */
+        int32_t n = 0;    /* solves or jobs */
+        int32_t blocks = 0;
+        const DeviceSystem* sys = nullptr;
+        const spac_point_stats* d_stats = nullptr;
+        spac_point_stats* host_stats = nullptr;
+    } pending;
+    enum { PENDING_NONE = 0, PENDING_POINTS = 1, PENDING_SYSTEM = 2 };
+
     spac_status ensure_records(int64_t total_paths);
     /* per-solve sigma(X0) slots for n solves (allocated here, never during capture) */
     spac_status ensure_solve_scratch(int32_t n, SolveSet* ss);
@@ -108,6 +123,11 @@ struct Problem {
      * with nothing allocated */
     spac_status run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode,
                     const RecordSet* recs = nullptr);
+    /* run() in two halves: launch_run enqueues (events ev[0..2] around the path kernel and the
+     * reduction) and returns; finish_run waits for the stream and fills `last` */
+    spac_status launch_run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode,
+                       const RecordSet* recs, int* blocks);
+    spac_status finish_run(int blocks);
     /* pinned host staging of upload_solves and copy_out (DMA through pinned memory instead of
      * pageable copies) */
     void* stage = nullptr;
@@ -128,6 +148,9 @@ struct Problem {
         for (DeviceBuffer* b : all) b->release();
     }
 };
+
+/* SPAC_OK when no asynchronous call is pending on p, else an error naming spac_fk_wait. */
+spac_status no_pending(const Problem* p);
 
 /* Install a control variate; the table (cfg->table ignored) is copied from
  * src, a host pointer or (src_on_device) a device pointer. */

@@ -869,6 +869,12 @@ spac_status Problem::ensure_solve_scratch(int32_t n, SolveSet* ss) {
     return SPAC_OK;
 }
 
+spac_status no_pending(const Problem* p) {
+    if (p->pending.kind != Problem::PENDING_NONE)
+        return set_error(SPAC_E_INVALID_ARG, "an asynchronous call is pending on this problem: call spac_fk_wait first");
+    return SPAC_OK;
+}
+
 RecordSet Problem::own_records() const {
     RecordSet r;
     r.xt = (double*)rec_xt.ptr; r.yt = (double*)rec_yt.ptr;
@@ -977,16 +983,27 @@ spac_status Problem::enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stat
 /* Launch the persistent path kernel and, optionally, the reduction; synchronous, timed. */
 spac_status Problem::run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode,
                          const RecordSet* recs) {
+    int blocks = 0;
+    spac_status rc = launch_run(ss, seed, d_stats, node_mode, recs, &blocks);
+    if (rc) return rc;
+    return finish_run(blocks);
+}
+
+spac_status Problem::launch_run(const SolveSet& ss, uint64_t seed, spac_point_stats* d_stats, int node_mode,
+                            const RecordSet* recs, int* blocks) {
     spac_status rc;
     if (!recs) {
         if ((rc = ensure_records(ss.total_paths))) return rc;
         if ((rc = queue.ensure(64))) return rc;
     }
-    int blocks = 0;
     SPAC_HIP_CHECK(hipEventRecord(ev[0], stream));
-    if ((rc = enqueue_paths(ss, seed, nullptr, &blocks, ev[1], recs))) return rc;
+    if ((rc = enqueue_paths(ss, seed, nullptr, blocks, ev[1], recs))) return rc;
     if (d_stats && (rc = enqueue_reduce(ss, d_stats, node_mode, recs))) return rc;
     SPAC_HIP_CHECK(hipEventRecord(ev[2], stream));
+    return SPAC_OK;
+}
+
+spac_status Problem::finish_run(int blocks) {
     SPAC_HIP_CHECK(hipStreamSynchronize(stream));
     float ms_path = 0.f, ms_red = 0.f;
     SPAC_HIP_CHECK(hipEventElapsedTime(&ms_path, ev[0], ev[1]));
@@ -1007,6 +1024,7 @@ __global__ void k_cv_gradients(int n, const float4* __restrict__ tab, float2* __
 }
 
 spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* src, bool src_on_device) {
+    if (spac_status rc = no_pending(p)) return rc;
     if (cfg->mode != SPAC_CV_ANALYTIC && cfg->mode != SPAC_CV_NONE && cfg->mode != SPAC_CV_TABLE)
         return set_error(SPAC_E_INVALID_ARG, "unknown control-variate mode %d", cfg->mode);
     if (cfg->fallback && !(cfg->max_ratio > 0.0))
@@ -1086,6 +1104,9 @@ static spac_status noalloc_scratch_address(int device, unsigned char** out) {
     return SPAC_OK;
 }
 
+/* completes a pending spac_fk_assemble_system_async (fk_rows.hip) */
+spac_status finish_assemble_system(Problem* p, const Problem::Pending& pd);
+
 } // namespace spac
 
 using namespace spac;
@@ -1139,6 +1160,7 @@ spac_status spac_fk_problem_destroy(spac_problem* hp) {
 spac_status spac_fk_set_launch(spac_problem* hp, const spac_launch_config* cfg) {
     if (!hp || !cfg) return set_error(SPAC_E_INVALID_ARG, "null argument");
     Problem* p = reinterpret_cast<Problem*>(hp);
+    if (spac_status pr = no_pending(p)) return pr;
     if (cfg->block_size) {
         if (cfg->block_size % 64 || cfg->block_size > 256 || cfg->block_size < 64)
             return set_error(SPAC_E_INVALID_ARG, "block_size must be 64, 128, 192 or 256");
@@ -1171,7 +1193,7 @@ spac_status spac_fk_last_timing(spac_problem* hp, spac_timing* out) {
 }
 
 static spac_status solve_points_impl(Problem* p, const double* d_x0, const double* d_y0, int32_t n, uint32_t paths,
-                                     double h, uint64_t seed, spac_point_stats* d_stats) {
+                                     double h, uint64_t seed, spac_point_stats* d_stats, bool async = false) {
     SolveSet ss;
     ss.n = n;
     ss.x0 = d_x0;
@@ -1187,6 +1209,14 @@ static spac_status solve_points_impl(Problem* p, const double* d_x0, const doubl
     ss.n_items = (int32_t)items;
     spac_status rc = p->ensure_solve_scratch(n, &ss);
     if (rc) return rc;
+    if (async) {
+        int blocks = 0;
+        if ((rc = p->launch_run(ss, seed, d_stats, 0, nullptr, &blocks))) return rc;
+        p->pending.kind = Problem::PENDING_POINTS;
+        p->pending.n = n;
+        p->pending.blocks = blocks;
+        return SPAC_OK;
+    }
     rc = p->run(ss, seed, d_stats, 0);
     if (rc) return rc;
     p->last.path_steps = -1;
@@ -1207,9 +1237,49 @@ spac_status spac_fk_solve_points_device(spac_problem* hp, const double* d_x0, co
     spac_status rc = check_points_args(d_x0, d_y0, n, paths, h, d_stats);
     if (rc) return rc;
     Problem* p = reinterpret_cast<Problem*>(hp);
+    if ((rc = no_pending(p))) return rc;
     if (p->desc.precision != SPAC_FP64 && p->desc.precision != SPAC_FP32)
         return set_error(SPAC_E_INVALID_ARG, "bad precision");
     return solve_points_impl(p, d_x0, d_y0, n, paths, h, seed, d_stats);
+}
+
+spac_status spac_fk_solve_points_device_async(spac_problem* hp, const double* d_x0, const double* d_y0, int32_t n,
+                                              uint32_t paths, double h, uint64_t seed, spac_point_stats* d_stats) {
+    if (!hp) return set_error(SPAC_E_INVALID_ARG, "null problem");
+    spac_status rc = check_points_args(d_x0, d_y0, n, paths, h, d_stats);
+    if (rc) return rc;
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    if ((rc = no_pending(p))) return rc;
+    if (p->desc.precision != SPAC_FP64 && p->desc.precision != SPAC_FP32)
+        return set_error(SPAC_E_INVALID_ARG, "bad precision");
+    return solve_points_impl(p, d_x0, d_y0, n, paths, h, seed, d_stats, true);
+}
+
+spac_status spac_fk_query(spac_problem* hp, int32_t* done) {
+    if (!hp || !done) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    if (p->pending.kind == Problem::PENDING_NONE) {
+        *done = 1;
+        return SPAC_OK;
+    }
+    hipError_t e = hipStreamQuery(p->stream);
+    if (e == hipSuccess) *done = 1;
+    else if (e == hipErrorNotReady) *done = 0;
+    else return set_error(SPAC_E_HIP, "hipStreamQuery failed: %s", hipGetErrorString(e));
+    return SPAC_OK;
+}
+
+spac_status spac_fk_wait(spac_problem* hp) {
+    if (!hp) return set_error(SPAC_E_INVALID_ARG, "null problem");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    const Problem::Pending pd = p->pending;
+    p->pending = Problem::Pending();
+    if (pd.kind == Problem::PENDING_NONE) return SPAC_OK;
+    if (pd.kind == Problem::PENDING_SYSTEM) return finish_assemble_system(p, pd);
+    spac_status rc = p->finish_run(pd.blocks);
+    if (rc) return rc;
+    p->last.path_steps = -1;
+    return SPAC_OK;
 }
 
 spac_status spac_fk_solve_points_device_noalloc(spac_problem* hp, const double* d_x0, const double* d_y0, int32_t n,
@@ -1218,6 +1288,7 @@ spac_status spac_fk_solve_points_device_noalloc(spac_problem* hp, const double* 
     spac_status rc = check_points_args(d_x0, d_y0, n, paths, h, d_stats);
     if (rc) return rc;
     Problem* p = reinterpret_cast<Problem*>(hp);
+    if ((rc = no_pending(p))) return rc;
     if (p->desc.precision != SPAC_FP64 && p->desc.precision != SPAC_FP32)
         return set_error(SPAC_E_INVALID_ARG, "bad precision");
     if ((size_t)n > kNaSolves)
@@ -1261,6 +1332,7 @@ spac_status spac_fk_solve_points_device_noalloc(spac_problem* hp, const double* 
 spac_status spac_fk_solve_points(spac_problem* hp, const double* x0, const double* y0, int32_t n, uint32_t paths,
                                  double h, uint64_t seed, spac_point_stats* stats) {
     if (!hp) return set_error(SPAC_E_INVALID_ARG, "null problem");
+    if (spac_status pr = no_pending(reinterpret_cast<Problem*>(hp))) return pr;
     spac_status rc = check_points_args(x0, y0, n, paths, h, stats);
     if (rc) return rc;
     Problem* p = reinterpret_cast<Problem*>(hp);
@@ -1284,6 +1356,7 @@ spac_status spac_fk_solve_points(spac_problem* hp, const double* x0, const doubl
 spac_status spac_fk_simulate(spac_problem* hp, const spac_solve_batch* b, uint64_t seed, spac_path_records* rec,
                              spac_point_stats* stats) {
     if (!hp || !b) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (spac_status pr = no_pending(reinterpret_cast<Problem*>(hp))) return pr;
     if (b->n < 1 || !b->x0 || !b->y0 || !b->paths || !b->h || !b->stream)
         return set_error(SPAC_E_INVALID_ARG, "batch needs n >= 1 and x0, y0, paths, h, stream");
     Problem* p = reinterpret_cast<Problem*>(hp);

@@ -664,6 +664,7 @@ extern "C" spac_status spac_fk_assemble_rows(spac_problem* hp, const spac_interf
                                              const spac_node_batch* nb, uint64_t seed, spac_row_block* out) {
     if (!hp || !out || !out->B || !out->G || !out->G_col || !out->G_offset)
         return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (spac_status pr = no_pending(reinterpret_cast<Problem*>(hp))) return pr;
     spac_status rc = validate(t, nb);
     if (rc) return rc;
     Problem* p = reinterpret_cast<Problem*>(hp);
@@ -728,13 +729,13 @@ __global__ void k_system_b(int32_t n_rows, const int32_t* __restrict__ row_job_p
 }
 } // namespace spac
 
-extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_interface_table* t,
-                                               const spac_node_batch* nb, uint64_t seed, spac_system* hsys,
-                                               spac_point_stats* stats) {
-    if (!hp || !hsys) return set_error(SPAC_E_INVALID_ARG, "null argument");
+namespace spac {
+/* Checks and enqueues one assembly of the system's jobs (path kernel, reduction, B and G
+ * kernels, accumulation into the system); the call in flight is left in p->pending. */
+static spac_status enqueue_assemble_system(Problem* p, const spac_interface_table* t, const spac_node_batch* nb,
+                                           uint64_t seed, DeviceSystem* sys, spac_point_stats* stats) {
     spac_status rc = validate(t, nb);
     if (rc) return rc;
-    DeviceSystem* sys = reinterpret_cast<DeviceSystem*>(hsys);
     if (nb->n != sys->n_jobs)
         return set_error(SPAC_E_INVALID_ARG, "the node batch has %d jobs, the system's mesh %d", nb->n, sys->n_jobs);
     /* the batch must be the mesh's jobs: each job's G goes into the slots validated for it */
@@ -747,7 +748,6 @@ extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_inte
     /* and the table the mesh's: the system holds that table's device copy */
     if (table_signature(t) != sys->h_table_sig)
         return set_error(SPAC_E_INVALID_ARG, "the interface table is not the system mesh's");
-    Problem* p = reinterpret_cast<Problem*>(hp);
     if (p->device != sys->device) return set_error(SPAC_E_INVALID_ARG, "problem and system are on different devices");
     RowsSetup rs;
     if ((rc = rows_prepare(p, t, nb, &rs, sys))) return rc;
@@ -771,10 +771,25 @@ extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_inte
                        (const spac_point_stats*)rs.d_stats, d_flag);
     ROWS_HIP_CHECK(hipGetLastError());
     ROWS_HIP_CHECK(hipEventRecord(p->ev[3], s));
+    p->pending.kind = Problem::PENDING_SYSTEM;
+    p->pending.n = n;
+    p->pending.blocks = blocks;
+    p->pending.sys = sys;
+    p->pending.d_stats = rs.d_stats;
+    p->pending.host_stats = stats;
+    return SPAC_OK;
+}
+
+/* Waits for an enqueued system assembly: its status word, optional estimator records and
+ * timing. */
+spac_status finish_assemble_system(Problem* p, const Problem::Pending& pd) {
+    hipStream_t s = p->stream;
     int32_t worst = SPAC_OK;
-    ROWS_HIP_CHECK(hipMemcpyAsync(&worst, d_flag, 4, hipMemcpyDeviceToHost, s));
+    ROWS_HIP_CHECK(hipMemcpyAsync(&worst, pd.sys->flag.ptr, 4, hipMemcpyDeviceToHost, s));
     ROWS_HIP_CHECK(hipStreamSynchronize(s));
-    if (stats && (rc = p->copy_out(stats, rs.d_stats, (size_t)n * sizeof(spac_point_stats)))) return rc;
+    spac_status rc;
+    if (pd.host_stats && (rc = p->copy_out(pd.host_stats, pd.d_stats, (size_t)pd.n * sizeof(spac_point_stats))))
+        return rc;
     float ms_path = 0.f, ms_rest = 0.f, ms_all = 0.f;
     ROWS_HIP_CHECK(hipEventElapsedTime(&ms_path, p->ev[0], p->ev[1]));
     ROWS_HIP_CHECK(hipEventElapsedTime(&ms_rest, p->ev[1], p->ev[3]));
@@ -783,10 +798,43 @@ extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_inte
     p->last.reduce_kernel_ms = ms_rest; /* reduction, B and G kernels, accumulation */
     p->last.total_ms = ms_all;
     p->last.path_kernel_launches = 1;
-    p->last.grid_blocks = blocks;
+    p->last.grid_blocks = pd.blocks;
     p->last.path_steps = 0;
     if (worst == SPAC_E_NAN) return set_error(SPAC_E_NAN, "NaN reached a row estimator");
     if (worst == SPAC_E_MAX_STEPS) return set_error(SPAC_E_MAX_STEPS, "paths stopped by max_steps");
+    return SPAC_OK;
+}
+} // namespace spac
+
+extern "C" spac_status spac_fk_assemble_system(spac_problem* hp, const spac_interface_table* t,
+                                               const spac_node_batch* nb, uint64_t seed, spac_system* hsys,
+                                               spac_point_stats* stats) {
+    if (!hp || !hsys) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    spac_status rc = no_pending(p);
+    if (rc) return rc;
+    if ((rc = enqueue_assemble_system(p, t, nb, seed, reinterpret_cast<DeviceSystem*>(hsys), stats))) {
+        p->pending = Problem::Pending(); /* nothing is left pending by a refused call */
+        (void)hipStreamSynchronize(p->stream);
+        return rc;
+    }
+    const Problem::Pending pd = p->pending;
+    p->pending = Problem::Pending();
+    return finish_assemble_system(p, pd);
+}
+
+extern "C" spac_status spac_fk_assemble_system_async(spac_problem* hp, const spac_interface_table* t,
+                                                     const spac_node_batch* nb, uint64_t seed, spac_system* hsys,
+                                                     spac_point_stats* stats) {
+    if (!hp || !hsys) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    spac_status rc = no_pending(p);
+    if (rc) return rc;
+    if ((rc = enqueue_assemble_system(p, t, nb, seed, reinterpret_cast<DeviceSystem*>(hsys), stats))) {
+        p->pending = Problem::Pending();
+        (void)hipStreamSynchronize(p->stream);
+        return rc;
+    }
     return SPAC_OK;
 }
 
@@ -815,6 +863,7 @@ extern "C" spac_status spac_fk_plan_create(spac_problem* hp, const spac_interfac
     spac_status rc = validate(t, nb);
     if (rc) return rc;
     Problem* src = reinterpret_cast<Problem*>(hp);
+    if ((rc = no_pending(src))) return rc;
     spac_problem* hclone = nullptr;
     if ((rc = spac_fk_problem_create(&src->desc, &hclone))) return rc;
     RowsPlan* plan = new RowsPlan();
@@ -907,6 +956,7 @@ extern "C" spac_status spac_fk_autotune_rows(spac_problem* hp, const spac_interf
     spac_status rc = validate(t, nb);
     if (rc) return rc;
     Problem* p = reinterpret_cast<Problem*>(hp);
+    if ((rc = no_pending(p))) return rc;
     /* sample: every node, paths scaled by path_fraction (at least one) */
     const int n = nb->n;
     std::vector<uint32_t> paths(n);
