@@ -264,6 +264,26 @@ spac_status spac_fk_assemble_system(spac_problem* p, const spac_interface_table*
                                     const spac_node_batch* nodes, uint64_t seed, spac_system* sys,
                                     spac_point_stats* stats);
 
+/* Rows of nodes inside full interior circles (interface code 1) from the pseudospectral
+ * solver instead of paths: PseudoespectralCirculoSolver::Inicializa + Resuelve(BVP,
+ * nudos_interior, nudos_periferia) (Solvers/PseudoespectralCirculoSolver.cpp:2-260,
+ * BVP/Interpolador_PS.cpp:18-61), applied by the reference's drivers to every non-perimeter
+ * circle (main_test_serial.cpp:67-79).  Deterministic; host code.  coef_kind: a registered
+ * coefficient set whose sigma is sqrt(2) I and b is 0 on the circles (the operator is the
+ * Laplacian plus c), e.g. SPAC_COEF_FRANKE_POISSON.  Each circle needs an even node count;
+ * n_radio (odd) Chebyshev points span its diameter (the reference's N_radio, 43).  The rows
+ * have the layout of spac_fk_assemble_rows (G over the interface's nodes, columns their global
+ * indices); boundary nodes get B = g(x); stats, when given, are zero with status 0. */
+typedef struct {
+    int32_t circles;         /* interfaces solved */
+    int32_t rows;            /* interior-node rows */
+    int32_t factorizations;  /* operators factored (circles with equal grids and c share one) */
+    int32_t solves;          /* right-hand sides solved */
+    double seconds;          /* wall time of the call */
+} spac_ps_report;
+spac_status spac_ps_assemble_rows(int32_t coef_kind, const spac_interface_table* ifaces, const spac_node_batch* nodes,
+                                  int32_t n_radio, spac_row_block* out, spac_ps_report* report);
+
 /* Persistence and resume of row assemblies.  The reference persists only the finished
  * system and its solution (Escribe_GB_COO / Escribe_GB_CSR, Mesh/SistemaLineal.cpp:96-138;
  * solution_GMRES.txt, Solvers/SistemaLinealSolverGMRES_OMP.hpp:606); these keep the rows of

@@ -174,6 +174,11 @@ class ResumeReport(C.Structure):
     _fields_ = [("batches", C.c_int32), ("computed", C.c_int32), ("loaded", C.c_int32), ("rejected", C.c_int32)]
 
 
+class PsReport(C.Structure):
+    _fields_ = [("circles", C.c_int32), ("rows", C.c_int32), ("factorizations", C.c_int32), ("solves", C.c_int32),
+                ("seconds", C.c_double)]
+
+
 class Timing(C.Structure):
     _fields_ = [
         ("path_kernel_ms", C.c_double),
@@ -291,6 +296,8 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.spac_fk_assemble_rows_resumable.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
                                                     C.c_uint64, C.c_int32, C.c_char_p, C.POINTER(RowBlock),
                                                     C.POINTER(ResumeReport)]
+    lib.spac_ps_assemble_rows.argtypes = [C.c_int32, C.POINTER(InterfaceTable), C.POINTER(NodeBatch), C.c_int32,
+                                          C.POINTER(RowBlock), C.POINTER(PsReport)]
     lib.spac_rows_file_write.argtypes = [C.c_char_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_int32,
                                          C.POINTER(RowBlock)]
     lib.spac_rows_file_read.argtypes = [C.c_char_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_int32, C.c_int64,

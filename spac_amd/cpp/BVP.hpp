@@ -126,6 +126,8 @@ struct BoundaryEntry {
 };
 bool register_coefficient_set(const CoefSetEntry& e);
 bool register_boundary(const BoundaryEntry& e);
+/* The host functions registered for a device coefficient kind (false: none). */
+bool coefficient_set_of_kind(int kind, CoefSetEntry* out);
 /* SPAC_OK and the device coefficient kind, or SPAC_E_UNREGISTERED_PLUGIN. */
 int resolve_coefficients(const BVP& bvp, int* kind);
 /* SPAC_OK and the device boundary kind, or SPAC_E_UNREGISTERED_PLUGIN.  A

@@ -48,6 +48,16 @@ bool register_boundary(const BoundaryEntry& e) {
     return true;
 }
 
+bool coefficient_set_of_kind(int kind, CoefSetEntry* out) {
+    std::lock_guard<std::mutex> lk(reg_mutex());
+    for (const CoefSetEntry& e : coef_sets())
+        if (e.kind == kind) {
+            *out = e;
+            return true;
+        }
+    return false;
+}
+
 int resolve_coefficients(const BVP& bvp, int* kind) {
     std::lock_guard<std::mutex> lk(reg_mutex());
     for (const CoefSetEntry& e : coef_sets()) {

@@ -317,3 +317,19 @@ def write_times_csv(path, rows, append=False):
     secs = np.array([r[2] for r in rows], dtype=np.float64)
     abi.check(lib.spac_write_times_csv(str(path).encode(), n, names, abi.ptr(ranks, C.c_int32),
                                        abi.ptr(secs, C.c_double), 1 if append else 0), lib)
+
+
+def ps_assemble_rows(mesh, coef_kind=abi.SPAC_COEF_FRANKE_POISSON, n_radio=43, jobs=None):
+    """spac_ps_assemble_rows over a mesh's jobs (or a subset): rows of nodes inside full interior
+    circles from the pseudospectral solver.  Returns (rows, report)."""
+    lib = _setup(abi.load())
+    nb, keep = mesh.node_batch(0, 1.0, jobs)
+    rows = mesh.alloc_rows(jobs)
+    rb = mesh.row_block(rows)
+    rep = abi.PsReport()
+    abi.check(lib.spac_ps_assemble_rows(coef_kind, C.byref(mesh.table), C.byref(nb), n_radio, C.byref(rb),
+                                        C.byref(rep)), lib)
+    from .fk import stats_to_dicts
+
+    rows["stats"] = stats_to_dicts(rows["stats"])
+    return rows, rep
