@@ -395,6 +395,30 @@ spac_status spac_fk_autotune_rows(spac_problem* p, const spac_interface_table* i
                                   double path_fraction, spac_launch_config* tried, double* tried_ms, int32_t cap,
                                   spac_autotune_report* report);
 
+/* Tabulated coefficients: the device form of the reference's numeric coefficients
+ * (Coeficiente_Escalar_Numerico / _Vectorial_ / _Matricial_, BVP/Coeficiente_Escalar_Numerico.hpp:6-19,
+ * evaluated through an Interpolador over a TablaDeValores, BVP/Interpolador.hpp:7-19).  The
+ * table replaces the coefficients its mask names; the others stay the compiled plugin's. */
+enum {
+    SPAC_TAB_SIGMA = 1,     /* sigma = s(x) I */
+    SPAC_TAB_DRIFT = 2,     /* b */
+    SPAC_TAB_KILLING = 4,   /* c */
+    SPAC_TAB_SOURCE = 8,    /* f */
+    SPAC_TAB_DIRICHLET = 16, /* g (domain exits, boundary nodes) */
+    SPAC_TAB_SOLUTION = 32  /* u (the statistics' subdomain-exit score) */
+};
+typedef struct {
+    int32_t mask;            /* SPAC_TAB_* bits; 0 removes the table */
+    int32_t table_size;      /* T >= 2 */
+    double table_domain[4];  /* xmin, ymin, xmax, ymax */
+    const double* values;    /* host, T * T * 8 doubles: at node (i, j), (j T + i) * 8 + {0: s, 1: b_x,
+                                2: b_y, 3: c, 4: f, 5: g, 6: u, 7: unused}; copied */
+} spac_coef_tables;
+/* Install (or remove) tabulated coefficients on p.  Needs the base set SPAC_COEF_VARCOEF_FRANKE
+ * (any scalar-diffusion operator is a table of it) and absorbing sides; the FP64 kernel reads
+ * the table in FP64 (bilinear, the bits of the CPU restatement), the FP32 kernel a float copy. */
+spac_status spac_fk_set_coef_tables(spac_problem* p, const spac_coef_tables* tables);
+
 /* Control variate F in xi += Y F(X).dW (Solvers/FeynmanKacSolver.cpp:13). */
 enum {
     SPAC_CV_ANALYTIC = 0, /* the coefficient set's F plugin (reference behaviour; default) */

@@ -41,6 +41,7 @@
 #include "../spac_amd/csrc/spac_math.h"
 #include "../spac_amd/csrc/spac_rbf.h"
 #include "../spac_amd/csrc/spac_cv.h"
+#include "../spac_amd/csrc/spac_tab.h"
 
 typedef struct { double x, y; } vec2;
 typedef struct { double a00, a01, a10, a11; } mat2;
@@ -245,6 +246,44 @@ static void frontera_init(frontera_t* f, int kind, const double* params) {
     for (int i = 0; i < 4; ++i) f->params[i] = params ? params[i] : 0.0;
 }
 
+/* ------------------------------------------------- tabulated coefficients --
+ * The numeric coefficients of BVP/Coeficiente_*_Numerico.hpp: Evalua reads an interpolated
+ * table (spac_tab.h, shared with the path kernels) instead of a formula.  Process-wide, set
+ * with oracle_set_coef_tables; replaces, in the variable-coefficient set, the coefficients the
+ * mask names.  Pinned by: tests/test_coef_tables.py (bilinear exactness on linear data, a
+ * table of the compiled coefficients reproduces the compiled records within interpolation
+ * error). */
+static spac_coef_table g_tab = {NULL, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0};
+
+int oracle_set_coef_tables(int mask, const double* values, int t, const double* dom) {
+    g_tab.mask = values ? mask : 0;
+    g_tab.v = values;
+    g_tab.t = t;
+    if (values) {
+        g_tab.lox = dom[0];
+        g_tab.loy = dom[1];
+        g_tab.ix = (double)(t - 1) / (dom[2] - dom[0]);
+        g_tab.iy = (double)(t - 1) / (dom[3] - dom[1]);
+    }
+    return 0;
+}
+static mat2 tab_sigma(vec2 X) {
+    double s = spac_tab_at(&g_tab, X.x, X.y, SPAC_TABC_S);
+    mat2 m = {1.0 * s, 0.0 * s, 0.0 * s, 1.0 * s};
+    return m;
+}
+static vec2 tab_b(vec2 X) {
+    int32_t k00;
+    double ax, ay;
+    spac_tab_cell(&g_tab, X.x, X.y, &k00, &ax, &ay);
+    vec2 r = {spac_tab_eval(&g_tab, k00, ax, ay, SPAC_TABC_BX), spac_tab_eval(&g_tab, k00, ax, ay, SPAC_TABC_BY)};
+    return r;
+}
+static double tab_c(vec2 X) { return spac_tab_at(&g_tab, X.x, X.y, SPAC_TABC_C); }
+static double tab_f(vec2 X) { return spac_tab_at(&g_tab, X.x, X.y, SPAC_TABC_F); }
+static double tab_g(vec2 X) { return spac_tab_at(&g_tab, X.x, X.y, SPAC_TABC_G); }
+static double tab_u(vec2 X) { return spac_tab_at(&g_tab, X.x, X.y, SPAC_TABC_U); }
+
 static int bvp_init(bvp_t* bvp, const spac_problem_desc* d) {
     bvp->psi = escalarN_defecto;
     bvp->varphi = escalarN_defecto;
@@ -268,6 +307,15 @@ static int bvp_init(bvp_t* bvp, const spac_problem_desc* d) {
         break;
     default:
         return SPAC_E_UNREGISTERED_PLUGIN;
+    }
+    if (g_tab.mask) {
+        if (d->coef_kind != SPAC_COEF_VARCOEF_FRANKE || d->reflecting) return SPAC_E_INVALID_ARG;
+        if (g_tab.mask & SPAC_TAB_SIGMA) bvp->sigma = tab_sigma;
+        if (g_tab.mask & SPAC_TAB_DRIFT) bvp->b = tab_b;
+        if (g_tab.mask & SPAC_TAB_KILLING) bvp->c = tab_c;
+        if (g_tab.mask & SPAC_TAB_SOURCE) bvp->f = tab_f;
+        if (g_tab.mask & SPAC_TAB_DIRICHLET) bvp->g = tab_g;
+        if (g_tab.mask & SPAC_TAB_SOLUTION) bvp->u = tab_u;
     }
     if (d->reflecting & ~15) return SPAC_E_INVALID_ARG;
     if (d->reflecting && d->domain_kind != SPAC_BOUNDARY_RECTANGLE) return SPAC_E_INVALID_ARG;
@@ -713,6 +761,7 @@ int oracle_coef_eval(int coef_kind, int which, const double* x, const double* y,
         case 8: out[i] = franke_dudx(X); break;
         case 9: out[i] = franke_dudy(X); break;
         case 10: out[i] = franke_d2udx2(X) + franke_d2udy2(X); break;
+        case 14: out[i] = bvp.g(X); break;
         default: out[i] = 0.0; break;
         }
     }

@@ -51,6 +51,13 @@ SPAC_ORDER_COST = 1
 SPAC_CV_ANALYTIC = 0
 SPAC_CV_NONE = 1
 SPAC_CV_TABLE = 2
+SPAC_TAB_SIGMA = 1
+SPAC_TAB_DRIFT = 2
+SPAC_TAB_KILLING = 4
+SPAC_TAB_SOURCE = 8
+SPAC_TAB_DIRICHLET = 16
+SPAC_TAB_SOLUTION = 32
+SPAC_TAB_ALL = 63
 MLP_N_PARAMS = 8707
 
 GOBET_CONSTANT = -0.5826  # Solvers/FeynmanKacSolver.cpp:268,283
@@ -172,6 +179,11 @@ class RowBlock(C.Structure):
 
 class ResumeReport(C.Structure):
     _fields_ = [("batches", C.c_int32), ("computed", C.c_int32), ("loaded", C.c_int32), ("rejected", C.c_int32)]
+
+
+class CoefTables(C.Structure):
+    _fields_ = [("mask", C.c_int32), ("table_size", C.c_int32), ("table_domain", C.c_double * 4),
+                ("values", C.POINTER(C.c_double))]
 
 
 class PsReport(C.Structure):
@@ -304,6 +316,7 @@ def load(path: Path | None = None) -> C.CDLL:
                                         C.POINTER(RowBlock), C.POINTER(C.c_int32)]
     lib.spac_fk_set_launch.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
     lib.spac_fk_set_control_variate.argtypes = [C.c_void_p, C.POINTER(CvConfig)]
+    lib.spac_fk_set_coef_tables.argtypes = [C.c_void_p, C.POINTER(CoefTables)]
     lib.spac_fk_plan_create.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
                                         C.POINTER(C.c_void_p)]
     lib.spac_fk_plan_execute.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(RowBlock), C.POINTER(C.c_double)]

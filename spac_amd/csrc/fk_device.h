@@ -17,6 +17,7 @@
 
 #include "spac_cv.h"
 #include "spac_math.h"
+#include "spac_tab.h"
 
 namespace spac {
 
@@ -140,6 +141,53 @@ struct CoefVarFranke {
     }
     __device__ __forceinline__ static double g(double x, double y) { return franke_u(franke_terms(x, y)); }
     __device__ __forceinline__ static double u(double x, double y) { return franke_u(franke_terms(x, y)); }
+    __device__ __forceinline__ static void robin(double, double, double, double, double* psi, double* phi) {
+        *psi = 0.0;
+        *phi = 0.0;
+    }
+};
+
+/* A coefficient set with tabulated coefficients (spac_tab.h) over the compiled set Base: the
+ * coefficients in the table's mask come from the table, the others from Base evaluated as its
+ * own plugins would be (Base's f and F with Base's sigma).  Unlike the compiled sets it carries
+ * state (the table), so kernels call it through an instance. */
+template <class Base>
+struct CoefTab {
+    static constexpr bool kHasDrift = true;
+    static constexpr bool kHasKilling = true;
+    static constexpr bool kHasRobin = false;
+    static constexpr bool kConstSigma = false;
+    spac_coef_table t;
+    __device__ __forceinline__ double sigma(double x, double y) const {
+        return (t.mask & SPAC_TAB_SIGMA) ? spac_tab_at(&t, x, y, SPAC_TABC_S) : Base::sigma(x, y);
+    }
+    __device__ __forceinline__ StepCoef step_s(double x, double y, double s) const {
+        StepCoef c;
+        const int32_t fbc = SPAC_TAB_SOURCE | SPAC_TAB_DRIFT | SPAC_TAB_KILLING;
+        if ((t.mask & fbc) != fbc || t.need_F) {
+            c = Base::step_s(x, y, (t.mask & SPAC_TAB_SIGMA) ? Base::sigma(x, y) : s);
+        } else {
+            c.Fx = 0.0;
+            c.Fy = 0.0;
+        }
+        c.s = s;
+        int32_t k00;
+        double ax, ay;
+        spac_tab_cell(&t, x, y, &k00, &ax, &ay);
+        if (t.mask & SPAC_TAB_DRIFT) {
+            c.bx = spac_tab_eval(&t, k00, ax, ay, SPAC_TABC_BX);
+            c.by = spac_tab_eval(&t, k00, ax, ay, SPAC_TABC_BY);
+        }
+        if (t.mask & SPAC_TAB_KILLING) c.c = spac_tab_eval(&t, k00, ax, ay, SPAC_TABC_C);
+        if (t.mask & SPAC_TAB_SOURCE) c.f = spac_tab_eval(&t, k00, ax, ay, SPAC_TABC_F);
+        return c;
+    }
+    __device__ __forceinline__ double g(double x, double y) const {
+        return (t.mask & SPAC_TAB_DIRICHLET) ? spac_tab_at(&t, x, y, SPAC_TABC_G) : Base::g(x, y);
+    }
+    __device__ __forceinline__ double u(double x, double y) const {
+        return (t.mask & SPAC_TAB_SOLUTION) ? spac_tab_at(&t, x, y, SPAC_TABC_U) : Base::u(x, y);
+    }
     __device__ __forceinline__ static void robin(double, double, double, double, double* psi, double* phi) {
         *psi = 0.0;
         *phi = 0.0;
@@ -294,6 +342,64 @@ struct CoefVarFranke32 {
         c.f = -(((0.5f * (s * s)) * franke_lap32(t) + (c.bx * ux + c.by * uy)) + c.c * franke_u32(t));
         c.Fx = -(s * ux);
         c.Fy = -(s * uy);
+        return c;
+    }
+};
+
+/* The FP32 twin of CoefTab: a float copy of the table, bilinear in FP32. */
+struct CoefTable32 {
+    const float* v; /* T * T * 8 floats */
+    int32_t t, mask, need_F;
+    float lox, loy, ix, iy, top;
+};
+__device__ __forceinline__ void tab_cell32(const CoefTable32& c, float x, float y, int32_t* k00, float* ax, float* ay) {
+    float fx = (x - c.lox) * c.ix, fy = (y - c.loy) * c.iy;
+    fx = __builtin_fminf(__builtin_fmaxf(fx, 0.0f), c.top);
+    fy = __builtin_fminf(__builtin_fmaxf(fy, 0.0f), c.top);
+    int32_t i = min((int32_t)fx, c.t - 2), j = min((int32_t)fy, c.t - 2);
+    *ax = fx - (float)i;
+    *ay = fy - (float)j;
+    *k00 = j * c.t + i;
+}
+__device__ __forceinline__ float tab_eval32(const CoefTable32& c, int32_t k00, float ax, float ay, int comp) {
+    const float* p = c.v + (int64_t)k00 * SPAC_TAB_STRIDE + comp;
+    const int64_t row = (int64_t)c.t * SPAC_TAB_STRIDE;
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    return by * (bx * p[0] + ax * p[SPAC_TAB_STRIDE]) + ay * (bx * p[row] + ax * p[row + SPAC_TAB_STRIDE]);
+}
+template <class Base32>
+struct CoefTab32 {
+    static constexpr bool kHasDrift = true;
+    static constexpr bool kHasKilling = true;
+    static constexpr bool kConstSigma = false;
+    using F64 = CoefTab<typename Base32::F64>;
+    CoefTable32 t;
+    __device__ __forceinline__ float sigma(float x, float y) const {
+        if (!(t.mask & SPAC_TAB_SIGMA)) return Base32::sigma(x, y);
+        int32_t k00;
+        float ax, ay;
+        tab_cell32(t, x, y, &k00, &ax, &ay);
+        return tab_eval32(t, k00, ax, ay, SPAC_TABC_S);
+    }
+    __device__ __forceinline__ StepCoef32 step_s(float x, float y, float s) const {
+        StepCoef32 c;
+        const int32_t fbc = SPAC_TAB_SOURCE | SPAC_TAB_DRIFT | SPAC_TAB_KILLING;
+        if ((t.mask & fbc) != fbc || t.need_F) {
+            c = Base32::step_s(x, y, (t.mask & SPAC_TAB_SIGMA) ? Base32::sigma(x, y) : s);
+        } else {
+            c.Fx = 0.0f;
+            c.Fy = 0.0f;
+        }
+        c.s = s;
+        int32_t k00;
+        float ax, ay;
+        tab_cell32(t, x, y, &k00, &ax, &ay);
+        if (t.mask & SPAC_TAB_DRIFT) {
+            c.bx = tab_eval32(t, k00, ax, ay, SPAC_TABC_BX);
+            c.by = tab_eval32(t, k00, ax, ay, SPAC_TABC_BY);
+        }
+        if (t.mask & SPAC_TAB_KILLING) c.c = tab_eval32(t, k00, ax, ay, SPAC_TABC_C);
+        if (t.mask & SPAC_TAB_SOURCE) c.f = tab_eval32(t, k00, ax, ay, SPAC_TABC_F);
         return c;
     }
 };

@@ -60,6 +60,10 @@ struct RecordSet {
 
 struct DeviceSystem;
 
+/* the kernel set of a problem with tabulated coefficients (CoefTab over the variable-coefficient
+ * set), as opposed to the SPAC_COEF_* compiled sets */
+constexpr int kCoefTabulated = 1000;
+
 struct Problem {
     spac_problem_desc desc{};
     int device = 0;
@@ -84,11 +88,19 @@ struct Problem {
     int32_t cv_t = 0;
     double cv_dom[4] = {0.0, 0.0, 1.0, 1.0};
 
+    /* tabulated coefficients (spac_fk_set_coef_tables): T x T x 8 doubles and a float copy */
+    DeviceBuffer tab, tab32;
+    int32_t tab_mask = 0, tab_t = 0;
+    double tab_dom[4] = {0.0, 0.0, 1.0, 1.0};
+
     DeviceBuffer seed_dev; /* seed read by captured kernels (plans) */
     /* this device's address of the static scratch block (fk_kernels.hip), looked up when the
      * handle is created; nullptr if the lookup failed (scratch_status holds its status) */
     unsigned char* scratch = nullptr;
     spac_status scratch_status = SPAC_OK;
+
+    /* the coefficient kernels to launch: desc.coef_kind, or kCoefTabulated */
+    int coef_kernels() const { return tab_mask ? kCoefTabulated : desc.coef_kind; }
 
     /* the handle's own record buffers (valid after ensure_records) */
     RecordSet own_records() const;
@@ -144,7 +156,7 @@ struct Problem {
                                &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
                                &ssq, &ssig, &ssigf,
                                &queue, &stats, &ifc, &ifnode, &ifipsi, &ifoff, &rows_iface, &rows_goff,
-                               &rows_G, &rows_part, &rows_col, &cv_table, &cv_grad, &seed_dev};
+                               &rows_G, &rows_part, &rows_col, &cv_table, &cv_grad, &seed_dev, &tab, &tab32};
         for (DeviceBuffer* b : all) b->release();
     }
 };

@@ -88,6 +88,32 @@ struct PathArgs {
     /* tabulated control variate (CV_TABLE): FP64 and FP32 views of one table */
     spac_cv_table cv;
     CvTable32 cv32;
+    /* tabulated coefficients (CoefTab kernels): FP64 table and its float copy */
+    spac_coef_table tab;
+    CoefTable32 tab32;
+};
+
+/* A coefficient-set object for a kernel: the compiled sets are stateless, a tabulated set
+ * carries its table. */
+template <class C>
+struct CoefMaker {
+    __device__ __forceinline__ static C make(const spac_coef_table&, const CoefTable32&) { return C(); }
+};
+template <class B>
+struct CoefMaker<CoefTab<B>> {
+    __device__ __forceinline__ static CoefTab<B> make(const spac_coef_table& t, const CoefTable32&) {
+        CoefTab<B> c;
+        c.t = t;
+        return c;
+    }
+};
+template <class B>
+struct CoefMaker<CoefTab32<B>> {
+    __device__ __forceinline__ static CoefTab32<B> make(const spac_coef_table&, const CoefTable32& t) {
+        CoefTab32<B> c;
+        c.t = t;
+        return c;
+    }
 };
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
@@ -101,14 +127,14 @@ __global__ void zero_counter_kernel(int32_t* q) {
 /* u at a subdomain exit for the statistics' score (Solvers/FeynmanKacSolver.cpp:254):
  * the coefficient set's u, or u_hat from the control-variate table. */
 template <class Coef, int CV>
-__device__ __forceinline__ double exit_u(const PathArgs& a, double x, double y) {
+__device__ __forceinline__ double exit_u(const PathArgs& a, const Coef& coef, double x, double y) {
     if (CV == CV_TABLE) {
         int32_t k00;
         double ax, ay;
         spac_cv_cell(&a.cv, x, y, &k00, &ax, &ay);
         return spac_cv_eval(&a.cv, k00, ax, ay, 0);
     }
-    return Coef::u(x, y);
+    return coef.u(x, y);
 }
 
 /* The frontier Dentro tests (Solvers/FeynmanKacSolver.cpp:72-80): the domain
@@ -140,6 +166,7 @@ template <class Coef, int SUB, int CV, bool REFL>
 __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t seed = a.seed_dev ? *a.seed_dev : a.seed;
+    const Coef coef = CoefMaker<Coef>::make(a.tab, a.tab32);
     bool active = false;
     /* lane path state */
     int32_t solve = 0;
@@ -199,7 +226,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                 X = a.x0[solve];
                 Yc = a.y0[solve];
                 /* sigma(X0), evaluated once per solve before the launch */
-                sig = Coef::kConstSigma ? Coef::sigma(X, Yc) : a.sig0[solve];
+                sig = Coef::kConstSigma ? coef.sigma(X, Yc) : a.sig0[solve];
                 if constexpr (REFL) {
                     /* initial Dentro (:166): a start beyond a reflecting side is projected */
                     bool dom0;
@@ -208,7 +235,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                     rnx = h0.nx; rny = h0.ny; rd = h0.d; chi_p = 0.0;
                     if (rrefl && !(h0.d <= 0.0)) {
                         X = h0.ex; Yc = h0.ey; chi_p = h0.d; rd = 0.0;
-                        sig = Coef::sigma(X, Yc);
+                        sig = coef.sigma(X, Yc);
                     }
                 }
                 Yw = 1.0; Z = 0.0; xi = 0.0; t = 0.0;
@@ -228,7 +255,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
              * 2h) gives the local time chi = max(0, nu + d) and X' -= chi N; a point
              * still beyond a reflecting side is projected (chi += overshoot); Robin
              * data at the foot W = X - d N of X on the side. */
-            StepCoef cf = Coef::step_s(X, Yc, sig);
+            StepCoef cf = coef.step_s(X, Yc, sig);
             double n0, n1;
             spac_normal_pair(seed, stream, path, (uint32_t)nstep, &n0, &n1);
             double dWx = sqrth * n0, dWy = sqrth * n1;
@@ -259,7 +286,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                 chi = chi + ph.d;
             }
             double psi = 0.0, phi = 0.0;
-            if (Coef::kHasRobin) Coef::robin(X - rd * rnx, Yc - rd * rny, rnx, rny, &psi, &phi);
+            if (Coef::kHasRobin) coef.robin(X - rd * rnx, Yc - rd * rny, rnx, rny, &psi, &phi);
             if (Coef::kHasRobin) Z = Z + ((cf.f * Yw) * h + (psi * Yw) * chi);
             else Z = Z + (cf.f * Yw) * h;
             if (CV == CV_ANALYTIC) {
@@ -281,7 +308,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             nstep++;
         } else if (active) {
             /* Paso_Absorbente (Solvers/FeynmanKacSolver.cpp:7-18) */
-            StepCoef cf = Coef::step_s(X, Yc, sig);
+            StepCoef cf = coef.step_s(X, Yc, sig);
             double n0, n1;
             spac_normal_pair(seed, stream, path, (uint32_t)nstep, &n0, &n1);
             double dWx = sqrth * n0, dWy = sqrth * n1;
@@ -314,9 +341,9 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             BoundaryHit hit = nearest_boundary<SUB>(a, R, cx, cy, X, Yc, &use_dom);
             /* sigma at X inside the boundary, at the projection outside (one evaluation) */
             const double px = (hit.d < 0.0) ? X : hit.ex, py = (hit.d < 0.0) ? Yc : hit.ey;
-            double s = Coef::sigma(px, py);
+            double s = coef.sigma(px, py);
             /* a path that goes on is inside (d < gobet |N sigma| sqrt h <= 0): s = sigma(X) */
-            sig = a.sigma_fresh ? Coef::sigma(X, Yc) : s;
+            sig = a.sigma_fresh ? coef.sigma(X, Yc) : s;
             double r0 = hit.nx * s, r1 = hit.ny * s;
             double nrm = __builtin_sqrt(r0 * r0 + r1 * r1);
             bool inside = hit.d < a.gobet * nrm * sqrth;
@@ -334,7 +361,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                     out = SPAC_OUT_INSIDE;
                 }
                 /* the next step's sigma at the current (possibly projected) X */
-                sig = (px == X && py == Yc) ? s : Coef::sigma(X, Yc);
+                sig = (px == X && py == Yc) ? s : coef.sigma(X, Yc);
             } else {
                 if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
             }
@@ -360,7 +387,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
 template <class Coef32, int SUB, int CV>
 __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
 #pragma clang fp contract(fast) /* FP32 path: tolerance-checked, not bit-pinned */
-    using Coef = typename Coef32::F64;
+    const Coef32 coef32 = CoefMaker<Coef32>::make(a.tab, a.tab32);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t seed = a.seed_dev ? *a.seed_dev : a.seed;
     bool active = false;
@@ -419,7 +446,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
                 }
                 xf = (float)a.x0[solve];
                 yf = (float)a.y0[solve];
-                sigf = Coef32::kConstSigma ? Coef32::sigma(xf, yf) : a.sig0f[solve];
+                sigf = Coef32::kConstSigma ? coef32.sigma(xf, yf) : a.sig0f[solve];
                 Yw = 1.0; Z = 0.0; xi = 0.0;
                 nstep = 0;
                 active = true;
@@ -430,7 +457,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
         }
         if (__ballot(active) == 0ull) break;
         if (active) {
-            StepCoef32 cf = Coef32::step_s(xf, yf, sigf);
+            StepCoef32 cf = coef32.step_s(xf, yf, sigf);
             float n0, n1;
             normal_pair32(seed, stream, path, (uint32_t)nstep, &n0, &n1);
             const float dWx = sqrthf * n0, dWy = sqrthf * n1;
@@ -465,8 +492,8 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
                 if (!use_dom) { d = ds; nx = dx * ir; ny = dy * ir; ex = nx * Rf + cxf; ey = ny * Rf + cyf; }
             }
             const float pxf = (d < 0.0f) ? xf : ex, pyf = (d < 0.0f) ? yf : ey;
-            float sg = Coef32::sigma(pxf, pyf);
-            sigf = a.sigma_fresh ? Coef32::sigma(xf, yf) : sg;
+            float sg = coef32.sigma(pxf, pyf);
+            sigf = a.sigma_fresh ? coef32.sigma(xf, yf) : sg;
             float r0 = nx * sg, r1 = ny * sg;
             bool inside = d < gobetf * __builtin_sqrtf(r0 * r0 + r1 * r1) * sqrthf;
             int8_t out = inside ? (int8_t)SPAC_OUT_INSIDE : (use_dom ? (int8_t)SPAC_OUT_DOMAIN : (int8_t)SPAC_OUT_SUBDOMAIN);
@@ -494,12 +521,15 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
 /* sigma(X0) of every solve in FP64 and in FP32 (the FP32 kernel's rounding of X0). */
 template <class Coef, class Coef32>
 __global__ void __launch_bounds__(256) fk_solve_sigma_kernel(int32_t n, const double* x0, const double* y0, double* sig0,
-                                                             float* sig0f) {
+                                                             float* sig0f, const spac_coef_table tab,
+                                                             const CoefTable32 tab32) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    const Coef coef = CoefMaker<Coef>::make(tab, tab32);
+    const Coef32 coef32 = CoefMaker<Coef32>::make(tab, tab32);
     const double x = x0[k], y = y0[k];
-    sig0[k] = Coef::sigma(x, y);
-    sig0f[k] = Coef32::sigma((float)x, (float)y);
+    sig0[k] = coef.sigma(x, y);
+    sig0f[k] = coef32.sigma((float)x, (float)y);
 }
 
 /* g(X_tau) for domain exits and capped paths, u(X_tau) (or u_hat) for subdomain
@@ -509,9 +539,10 @@ __global__ void __launch_bounds__(256) fk_solve_sigma_kernel(int32_t n, const do
 template <class Coef, int CV>
 __global__ void __launch_bounds__(256) fk_exit_values_kernel(PathArgs a, int64_t total) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const Coef coef = CoefMaker<Coef>::make(a.tab, a.tab32);
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
         const double x = a.xt[g], y = a.yt[g];
-        a.gval[g] = (a.outcome[g] == SPAC_OUT_SUBDOMAIN) ? exit_u<Coef, CV>(a, x, y) : Coef::g(x, y);
+        a.gval[g] = (a.outcome[g] == SPAC_OUT_SUBDOMAIN) ? exit_u<Coef, CV>(a, coef, x, y) : coef.g(x, y);
     }
 }
 
@@ -647,6 +678,16 @@ static void launch_fp64(int sub, dim3 grid, dim3 block, hipStream_t s, const Pat
 template <int CV>
 static void launch_paths_cv(int coef, int precision, int sub, bool refl, dim3 grid, dim3 block, hipStream_t s,
                             const PathArgs& a) {
+    if (coef == kCoefTabulated) { /* absorbing sides only (checked when the table is installed) */
+        if (precision == SPAC_FP32) {
+            if (sub == SUB_CIRCLE)
+                hipLaunchKernelGGL((fk_path_kernel_fp32<CoefTab32<CoefVarFranke32>, SUB_CIRCLE, CV>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((fk_path_kernel_fp32<CoefTab32<CoefVarFranke32>, SUB_NONE, CV>), grid, block, 0, s, a);
+        } else {
+            launch_fp64<CoefTab<CoefVarFranke>, CV, false>(sub, grid, block, s, a);
+        }
+        return;
+    }
     if (precision == SPAC_FP32) {
         /* FRANKE_ROBIN differs from FRANKE_POISSON only on reflecting sides (FP64 only) */
         if (coef != SPAC_COEF_VARCOEF_FRANKE) {
@@ -854,7 +895,9 @@ static spac_status check_problem(const spac_problem_desc* d) {
 /* Queue reset and the persistent path kernel (no synchronisation). */
 template <int CV>
 static void launch_exit_values(int coef, dim3 grid, hipStream_t s, const PathArgs& a, int64_t total) {
-    if (coef == SPAC_COEF_VARCOEF_FRANKE)
+    if (coef == kCoefTabulated)
+        hipLaunchKernelGGL((fk_exit_values_kernel<CoefTab<CoefVarFranke>, CV>), grid, dim3(256), 0, s, a, total);
+    else if (coef == SPAC_COEF_VARCOEF_FRANKE)
         hipLaunchKernelGGL((fk_exit_values_kernel<CoefVarFranke, CV>), grid, dim3(256), 0, s, a, total);
     else
         hipLaunchKernelGGL((fk_exit_values_kernel<CoefFranke, CV>), grid, dim3(256), 0, s, a, total);
@@ -931,27 +974,51 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
         a.cv32.iy = (float)a.cv.iy;
         a.cv32.top = (float)(cv_t - 1);
     }
+    if (tab_mask) {
+        a.tab.v = (const double*)tab.ptr;
+        a.tab.t = tab_t;
+        a.tab.mask = tab_mask;
+        a.tab.lox = tab_dom[0];
+        a.tab.loy = tab_dom[1];
+        a.tab.ix = (double)(tab_t - 1) / (tab_dom[2] - tab_dom[0]);
+        a.tab.iy = (double)(tab_t - 1) / (tab_dom[3] - tab_dom[1]);
+        a.tab.need_F = cv_mode == SPAC_CV_ANALYTIC ? 1 : 0;
+        a.tab32.v = (const float*)tab32.ptr;
+        a.tab32.t = tab_t;
+        a.tab32.mask = tab_mask;
+        a.tab32.need_F = a.tab.need_F;
+        a.tab32.lox = (float)a.tab.lox;
+        a.tab32.loy = (float)a.tab.loy;
+        a.tab32.ix = (float)a.tab.ix;
+        a.tab32.iy = (float)a.tab.iy;
+        a.tab32.top = (float)(tab_t - 1);
+    }
+    const int coef = coef_kernels();
     int sub = (ss.circle || desc.subdomain_kind == SPAC_BOUNDARY_CIRCLE) ? SUB_CIRCLE : SUB_NONE;
     int block = launch.block_size;
     int blocks = num_cus * launch.blocks_per_cu;
     int64_t max_blocks = std::max<int64_t>(1, ((int64_t)ss.n_items + (block / 64) - 1) / (block / 64));
     if (blocks > max_blocks) blocks = (int)max_blocks;
     if (!ss.sqrth && !(ss.sqrth_uniform > 0.0)) return set_error(SPAC_E_INVALID_ARG, "solve set without sqrt(h)");
-    if (desc.coef_kind == SPAC_COEF_VARCOEF_FRANKE) {
+    if (coef == SPAC_COEF_VARCOEF_FRANKE || coef == kCoefTabulated) {
         if (!ss.sig0 || !ss.sig0f) return set_error(SPAC_E_INVALID_ARG, "solve set without sigma(X0) slots");
-        hipLaunchKernelGGL((fk_solve_sigma_kernel<CoefVarFranke, CoefVarFranke32>), dim3((ss.n + 255) / 256), dim3(256),
-                           0, stream, ss.n, ss.x0, ss.y0, ss.sig0, ss.sig0f);
+        if (coef == kCoefTabulated)
+            hipLaunchKernelGGL((fk_solve_sigma_kernel<CoefTab<CoefVarFranke>, CoefTab32<CoefVarFranke32>>),
+                               dim3((ss.n + 255) / 256), dim3(256), 0, stream, ss.n, ss.x0, ss.y0, ss.sig0, ss.sig0f,
+                               a.tab, a.tab32);
+        else
+            hipLaunchKernelGGL((fk_solve_sigma_kernel<CoefVarFranke, CoefVarFranke32>), dim3((ss.n + 255) / 256),
+                               dim3(256), 0, stream, ss.n, ss.x0, ss.y0, ss.sig0, ss.sig0f, a.tab, a.tab32);
     }
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
-    launch_paths(desc.coef_kind, desc.precision, sub, cv_mode, desc.reflecting != 0, dim3(blocks), dim3(block), stream,
-                 a);
+    launch_paths(coef, desc.precision, sub, cv_mode, desc.reflecting != 0, dim3(blocks), dim3(block), stream, a);
     SPAC_HIP_CHECK(hipGetLastError());
     if (after_path) SPAC_HIP_CHECK(hipEventRecord(after_path, stream));
     if (ss.total_paths > 0) {
         const int64_t want = (ss.total_paths + 255) / 256;
         const int gblocks = (int)std::min<int64_t>(want, (int64_t)num_cus * 16);
-        if (cv_mode == SPAC_CV_TABLE) launch_exit_values<CV_TABLE>(desc.coef_kind, dim3(gblocks), stream, a, ss.total_paths);
-        else launch_exit_values<CV_ANALYTIC>(desc.coef_kind, dim3(gblocks), stream, a, ss.total_paths);
+        if (cv_mode == SPAC_CV_TABLE) launch_exit_values<CV_TABLE>(coef, dim3(gblocks), stream, a, ss.total_paths);
+        else launch_exit_values<CV_ANALYTIC>(coef, dim3(gblocks), stream, a, ss.total_paths);
         SPAC_HIP_CHECK(hipGetLastError());
     }
     if (grid_blocks) *grid_blocks = blocks;
@@ -1184,6 +1251,41 @@ spac_status spac_fk_set_launch(spac_problem* hp, const spac_launch_config* cfg) 
 spac_status spac_fk_set_control_variate(spac_problem* hp, const spac_cv_config* cfg) {
     if (!hp || !cfg) return set_error(SPAC_E_INVALID_ARG, "null argument");
     return problem_set_cv(reinterpret_cast<Problem*>(hp), cfg, cfg->table, false);
+}
+
+spac_status spac_fk_set_coef_tables(spac_problem* hp, const spac_coef_tables* t) {
+    if (!hp || !t) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    spac_status rc = no_pending(p);
+    if (rc) return rc;
+    const int32_t all = SPAC_TAB_SIGMA | SPAC_TAB_DRIFT | SPAC_TAB_KILLING | SPAC_TAB_SOURCE | SPAC_TAB_DIRICHLET |
+                        SPAC_TAB_SOLUTION;
+    if (t->mask & ~all) return set_error(SPAC_E_INVALID_ARG, "table mask 0x%x has bits outside SPAC_TAB_*", t->mask);
+    if (t->mask == 0) {
+        p->tab_mask = 0;
+        return SPAC_OK;
+    }
+    if (p->desc.coef_kind != SPAC_COEF_VARCOEF_FRANKE)
+        return set_error(SPAC_E_INVALID_ARG, "tabulated coefficients need the base set SPAC_COEF_VARCOEF_FRANKE");
+    if (p->desc.reflecting) return set_error(SPAC_E_INVALID_ARG, "tabulated coefficients need absorbing sides");
+    if (t->table_size < 2 || t->table_size > 4096) return set_error(SPAC_E_INVALID_ARG, "table_size must be in [2, 4096]");
+    const double* d = t->table_domain;
+    if (!(d[2] > d[0]) || !(d[3] > d[1]))
+        return set_error(SPAC_E_INVALID_ARG, "table domain must have xmax > xmin and ymax > ymin");
+    if (!t->values) return set_error(SPAC_E_INVALID_ARG, "no table values");
+    const size_t n = (size_t)t->table_size * (size_t)t->table_size * SPAC_TAB_STRIDE;
+    for (size_t k = 0; k < n; ++k)
+        if (!std::isfinite(t->values[k])) return set_error(SPAC_E_NAN, "table value %zu is not finite", k);
+    std::vector<float> vf(n);
+    for (size_t k = 0; k < n; ++k) vf[k] = (float)t->values[k];
+    if ((rc = p->tab.ensure(n * 8)) || (rc = p->tab32.ensure(n * 4))) return rc;
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->tab.ptr, t->values, n * 8, hipMemcpyHostToDevice, p->stream));
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->tab32.ptr, vf.data(), n * 4, hipMemcpyHostToDevice, p->stream));
+    SPAC_HIP_CHECK(hipStreamSynchronize(p->stream));
+    p->tab_mask = t->mask;
+    p->tab_t = t->table_size;
+    for (int i = 0; i < 4; ++i) p->tab_dom[i] = d[i];
+    return SPAC_OK;
 }
 
 spac_status spac_fk_last_timing(spac_problem* hp, spac_timing* out) {

@@ -379,6 +379,15 @@ __global__ void fk_rows_b_kernel(int32_t n, const int32_t* frontera, const doubl
     if (k >= n) return;
     B[k] = frontera[k] ? Coef::g(x[k], y[k]) : stats[k].B;
 }
+/* the same with a tabulated g where the problem's table holds one */
+__global__ void fk_rows_b_tab_kernel(int32_t n, const int32_t* frontera, const double* x, const double* y,
+                                     const spac_point_stats* stats, double* B, const spac_coef_table tab) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    CoefTab<CoefVarFranke> coef;
+    coef.t = tab;
+    B[k] = frontera[k] ? coef.g(x[k], y[k]) : stats[k].B;
+}
 
 static spac_status validate(const spac_interface_table* t, const spac_node_batch* nb) {
     if (!t || !nb) return set_error(SPAC_E_INVALID_ARG, "null argument");
@@ -615,7 +624,19 @@ static spac_status rows_enqueue_bg(Problem* p, const RowsSetup& rs) {
     hipStream_t s = p->stream;
     const int n = rs.n;
     const int nblk = (n + 255) / 256;
-    if (p->desc.coef_kind != SPAC_COEF_VARCOEF_FRANKE) /* Franke g for both Franke sets */
+    if (p->tab_mask) {
+        spac_coef_table tab;
+        std::memset(&tab, 0, sizeof tab);
+        tab.v = (const double*)p->tab.ptr;
+        tab.t = p->tab_t;
+        tab.mask = p->tab_mask;
+        tab.lox = p->tab_dom[0];
+        tab.loy = p->tab_dom[1];
+        tab.ix = (double)(p->tab_t - 1) / (p->tab_dom[2] - p->tab_dom[0]);
+        tab.iy = (double)(p->tab_t - 1) / (p->tab_dom[3] - p->tab_dom[1]);
+        hipLaunchKernelGGL(fk_rows_b_tab_kernel, dim3(nblk), dim3(256), 0, s, n, rs.d_front, rs.ss.x0, rs.ss.y0,
+                           (const spac_point_stats*)rs.d_stats, rs.d_B, tab);
+    } else if (p->desc.coef_kind != SPAC_COEF_VARCOEF_FRANKE) /* Franke g for both Franke sets */
         hipLaunchKernelGGL(fk_rows_b_kernel<CoefFranke>, dim3(nblk), dim3(256), 0, s, n, rs.d_front, rs.ss.x0, rs.ss.y0,
                            (const spac_point_stats*)rs.d_stats, rs.d_B);
     else
@@ -883,6 +904,20 @@ extern "C" spac_status spac_fk_plan_create(spac_problem* hp, const spac_interfac
         p->cv_mode = src->cv_mode;
         p->cv_fallback = src->cv_fallback;
         p->cv_max_ratio = src->cv_max_ratio;
+    }
+    if (!rc && src->tab_mask) { /* the tabulated coefficients, copied device to device */
+        const size_t n = (size_t)src->tab_t * (size_t)src->tab_t * SPAC_TAB_STRIDE;
+        rc = p->tab.ensure(n * 8);
+        if (!rc) rc = p->tab32.ensure(n * 4);
+        if (!rc && (hipMemcpyAsync(p->tab.ptr, src->tab.ptr, n * 8, hipMemcpyDeviceToDevice, p->stream) != hipSuccess ||
+                    hipMemcpyAsync(p->tab32.ptr, src->tab32.ptr, n * 4, hipMemcpyDeviceToDevice, p->stream) != hipSuccess ||
+                    hipStreamSynchronize(p->stream) != hipSuccess))
+            rc = set_error(SPAC_E_HIP, "copying the coefficient tables into the plan failed");
+        if (!rc) {
+            p->tab_mask = src->tab_mask;
+            p->tab_t = src->tab_t;
+            for (int i = 0; i < 4; ++i) p->tab_dom[i] = src->tab_dom[i];
+        }
     }
     if (!rc) rc = rows_prepare(p, t, nb, &plan->rs);
     if (!rc) rc = p->seed_dev.ensure(64);

@@ -136,6 +136,21 @@ class FKProblem:
             cfg.table = abi.ptr(keep, C.c_float)
         abi.check(self.lib.spac_fk_set_control_variate(self.h, C.byref(cfg)), self.lib)
 
+    def set_coef_tables(self, mask, values=None, domain=None):
+        """spac_fk_set_coef_tables: values (T, T, 8) float64 indexed [j, i] = (s, b_x, b_y, c, f, g, u, 0)
+        at (xmin + i dx, ymin + j dy); the coefficients in mask (SPAC_TAB_*) come from the table.
+        mask 0 removes it."""
+        cfg = abi.CoefTables()
+        cfg.mask = mask
+        if mask:
+            keep = np.ascontiguousarray(values, dtype=np.float64)
+            assert keep.ndim == 3 and keep.shape[0] == keep.shape[1] and keep.shape[2] == 8
+            cfg.table_size = keep.shape[0]
+            for i, v in enumerate(domain):
+                cfg.table_domain[i] = v
+            cfg.values = abi.ptr(keep, C.c_double)
+        abi.check(self.lib.spac_fk_set_coef_tables(self.h, C.byref(cfg)), self.lib)
+
     def timing(self) -> abi.Timing:
         t = abi.Timing()
         abi.check(self.lib.spac_fk_last_timing(self.h, C.byref(t)), self.lib)

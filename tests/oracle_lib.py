@@ -222,3 +222,23 @@ def tabulate(fn, t, domain):
     out = np.zeros((t, t, 4), dtype=np.float32)
     out[..., 0], out[..., 1], out[..., 2] = u, ux, uy
     return out
+
+
+_coef_tables_keep = None
+
+
+def set_coef_tables(mask=0, values=None, domain=None):
+    """oracle_set_coef_tables: tabulated coefficients (values (T, T, 8)) of later calls
+    (process-wide; reset with set_coef_tables() when done)."""
+    global _coef_tables_keep
+    lib = load()
+    lib.oracle_set_coef_tables.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+    lib.oracle_set_coef_tables.restype = C.c_int
+    if not mask:
+        _coef_tables_keep = None
+        lib.oracle_set_coef_tables(0, None, 0, None)
+        return
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    d = np.ascontiguousarray(domain, dtype=np.float64)
+    _coef_tables_keep = (v, d)
+    lib.oracle_set_coef_tables(mask, _p(v), v.shape[0], _p(d))
