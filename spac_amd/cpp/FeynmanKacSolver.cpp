@@ -16,12 +16,103 @@ FeynmanKacSolver::~FeynmanKacSolver() {
     if (prob_) spac_fk_problem_destroy(prob_);
 }
 
+/* The interpolators of the numeric coefficients named by mask, in a fixed order. */
+static std::vector<InterpoladorPtr> table_key(const BVP& P, int mask) {
+    std::vector<InterpoladorPtr> k;
+    if (mask & SPAC_TAB_SIGMA)
+        k.insert(k.end(), {P.sigma.interpolador_11, P.sigma.interpolador_12, P.sigma.interpolador_21,
+                           P.sigma.interpolador_22});
+    if (mask & SPAC_TAB_DRIFT) k.insert(k.end(), {P.b.interpolador_x, P.b.interpolador_y});
+    if (mask & SPAC_TAB_KILLING) k.push_back(P.c.interpolador);
+    if (mask & SPAC_TAB_SOURCE) k.push_back(P.f.interpolador);
+    if (mask & SPAC_TAB_DIRICHLET) k.push_back(P.g.interpolador);
+    if (mask & SPAC_TAB_SOLUTION) k.push_back(P.u.interpolador);
+    return k;
+}
+
+int FeynmanKacSolver::sample_tables(const BVP& P, int mask, const double dom[4], bool* changed) {
+    *changed = false;
+    if (!mask) {
+        *changed = coef_mask_ != 0;
+        coef_mask_ = 0;
+        coef_tab_.clear();
+        coef_key_.clear();
+        return SPAC_OK;
+    }
+    if (T_tabla < 2 || T_tabla > 8192) return set_error(SPAC_E_INVALID_ARG, "T_tabla must be in [2, 8192]");
+    std::vector<InterpoladorPtr> key = table_key(P, mask);
+    bool same = mask == coef_mask_ && T_tabla == coef_T_ && key.size() == coef_key_.size() && !coef_tab_.empty();
+    for (int i = 0; same && i < 4; ++i) same = dom[i] == coef_dom_[i];
+    for (size_t i = 0; same && i < key.size(); ++i) same = key[i].get() == coef_key_[i].get();
+    if (same) return SPAC_OK;
+    const int T = (int)T_tabla;
+    std::vector<double> v((size_t)T * T * 8, 0.0);
+    const double dx = (dom[2] - dom[0]) / (double)(T - 1), dy = (dom[3] - dom[1]) / (double)(T - 1);
+    for (int j = 0; j < T; ++j)
+        for (int i = 0; i < T; ++i) {
+            /* the node positions of TablaDeValores::x(i), y(j) */
+            const Vec2 X(dom[0] + (double)i * dx, dom[1] + (double)j * dy);
+            double* o = &v[((size_t)j * T + i) * 8];
+            if (mask & SPAC_TAB_SIGMA) {
+                const Mat2 S = P.sigma.Evalua(X);
+                const double tol = 1e-12 * (1.0 + std::fabs(S(0, 0)));
+                if (!(std::fabs(S(0, 1)) <= tol && std::fabs(S(1, 0)) <= tol && std::fabs(S(1, 1) - S(0, 0)) <= tol))
+                    return set_error(SPAC_E_INVALID_ARG,
+                                     "numeric sigma must be s(x) I on the device (sigma at (%g, %g) is [%g %g; %g %g])",
+                                     X(0), X(1), S(0, 0), S(0, 1), S(1, 0), S(1, 1));
+                o[0] = S(0, 0);
+            }
+            if (mask & SPAC_TAB_DRIFT) {
+                const Vec2 B = P.b.Evalua(X);
+                o[1] = B(0);
+                o[2] = B(1);
+            }
+            if (mask & SPAC_TAB_KILLING) o[3] = P.c.Evalua(X);
+            if (mask & SPAC_TAB_SOURCE) o[4] = P.f.Evalua(X);
+            if (mask & SPAC_TAB_DIRICHLET) o[5] = P.g.Evalua(X);
+            if (mask & SPAC_TAB_SOLUTION) o[6] = P.u.Evalua(X);
+            for (int c = 0; c < 7; ++c)
+                if (!std::isfinite(o[c]))
+                    return set_error(SPAC_E_NAN, "numeric coefficient %d is not finite at (%g, %g)", c, X(0), X(1));
+        }
+    coef_tab_ = std::move(v);
+    coef_mask_ = mask;
+    coef_key_ = std::move(key);
+    coef_T_ = T_tabla;
+    for (int i = 0; i < 4; ++i) coef_dom_[i] = dom[i];
+    *changed = true;
+    return SPAC_OK;
+}
+
+int FeynmanKacSolver::apply_tables() {
+    if (!prob_) return SPAC_OK;
+    spac_coef_tables t;
+    std::memset(&t, 0, sizeof t);
+    t.mask = coef_mask_;
+    if (coef_mask_) {
+        t.table_size = (int32_t)coef_T_;
+        for (int i = 0; i < 4; ++i) t.table_domain[i] = coef_dom_[i];
+        t.values = coef_tab_.data();
+    }
+    const int rc = spac_fk_set_coef_tables(prob_, &t);
+    if (rc) { /* the device holds no tables: sample and install again on the next call */
+        coef_mask_ = 0;
+        coef_tab_.clear();
+        coef_key_.clear();
+        if (prob_) { spac_fk_problem_destroy(prob_); prob_ = nullptr; }
+    }
+    return rc;
+}
+
 int FeynmanKacSolver::ensure_problem(const BVP& P, bool with_subdomain, double rho) {
     spac_problem_desc d;
     std::memset(&d, 0, sizeof d);
-    int kind = 0;
-    if (resolve_coefficients(P, &kind) != SPAC_OK)
-        return set_error(SPAC_E_UNREGISTERED_PLUGIN, "BVP coefficients are not a registered device set");
+    int kind = 0, tab_mask = 0;
+    bool F_zero = false;
+    if (resolve_numeric_coefficients(P, &kind, &tab_mask, &F_zero) != SPAC_OK)
+        return set_error(SPAC_E_UNREGISTERED_PLUGIN,
+                         "BVP coefficients are neither a registered device set nor numeric coefficients over the "
+                         "variable-coefficient set");
     d.coef_kind = kind;
     int dk = 0;
     if (resolve_boundary(P.frontera_dominio, &dk) != SPAC_OK || dk != SPAC_BOUNDARY_RECTANGLE ||
@@ -51,17 +142,32 @@ int FeynmanKacSolver::ensure_problem(const BVP& P, bool with_subdomain, double r
     d.gobet_constant = -0.5826; /* Solvers/FeynmanKacSolver.cpp:268,283 */
     d.max_steps = (int32_t)max_steps;
     d.precision = SPAC_FP64;
-    if (prob_ && std::memcmp(&d, &desc_, sizeof d) == 0) return SPAC_OK;
+    if (tab_mask && mask)
+        return set_error(SPAC_E_INVALID_ARG, "numeric coefficients need absorbing sides (no reflecting side)");
+    if ((P.psi.Es_Numerico() || P.varphi.Es_Numerico()) && mask)
+        return set_error(SPAC_E_UNREGISTERED_PLUGIN, "numeric Robin data (psi, varphi) has no device form");
+    bool tables_changed = false;
+    int rc = sample_tables(P, tab_mask, d.domain_params, &tables_changed);
+    if (rc) return rc;
+    const bool cv_changed = F_zero != F_zero_;
+    F_zero_ = F_zero;
+    if (prob_ && std::memcmp(&d, &desc_, sizeof d) == 0) {
+        if (tables_changed && (rc = apply_tables())) return rc;
+        return cv_changed ? apply_cv() : SPAC_OK;
+    }
     if (prob_) { spac_fk_problem_destroy(prob_); prob_ = nullptr; }
-    int rc = spac_fk_problem_create(&d, &prob_);
+    rc = spac_fk_problem_create(&d, &prob_);
     if (rc) return rc;
     desc_ = d;
+    if ((rc = apply_tables())) return rc;
     return apply_cv();
 }
 
 int FeynmanKacSolver::apply_cv() {
     if (!prob_) return SPAC_OK; /* applied when the problem is created */
     spac_cv_config c = cv_;
+    /* the plugin control variate with F = 0 (the default) is xi = 0: plain Monte Carlo */
+    if (c.mode == SPAC_CV_ANALYTIC && F_zero_) c.mode = SPAC_CV_NONE;
     c.table = cv_table_.empty() ? nullptr : cv_table_.data();
     return spac_fk_set_control_variate(prob_, &c);
 }

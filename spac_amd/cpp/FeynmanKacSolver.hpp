@@ -72,6 +72,20 @@ public:
                     unsigned int numero_trayectorias, double discretizacion_temporal, const BVP& Problema,
                     double rho = 0.0);
 
+    /* Numeric coefficients (Coeficiente_*_Numerico: sigma = s I, b, c, f, g, u) run on the
+     * device as tables: each is sampled once, through its Evalua, at the nodes of a
+     * T_tabla x T_tabla grid over the domain rectangle, and read bilinearly by the path
+     * kernels.  The samples are kept while the BVP holds the same interpolator objects, the
+     * same T_tabla and the same rectangle; Retabula_Coeficientes() drops them (after an
+     * interpolator was changed in place).  The other coefficients must be those of the
+     * registered variable-coefficient set; F may also be left at its default (zero), which
+     * makes the plugin control variate xi = 0, i.e. plain Monte Carlo. */
+    unsigned int T_tabla = 257;
+    void Retabula_Coeficientes() { coef_key_.clear(); coef_tab_.clear(); }
+    /* The sampled table (T_tabla^2 x 8 doubles, the layout of spac_coef_tables) and its mask. */
+    const std::vector<double>& Tabla_Coeficientes() const { return coef_tab_; }
+    int Mascara_Coeficientes() const { return coef_mask_; }
+
 private:
     spac_problem* prob_ = nullptr;
     spac_problem_desc desc_{};
@@ -79,7 +93,15 @@ private:
     uint32_t calls_ = 0;
     spac_cv_config cv_{};           /* applied to every problem this solver creates */
     std::vector<float> cv_table_;
+    bool F_zero_ = false;           /* the BVP's F is the default zero: the plugin CV is plain MC */
+    std::vector<double> coef_tab_;  /* sampled numeric coefficients */
+    int coef_mask_ = 0;
+    std::vector<InterpoladorPtr> coef_key_; /* held, so no other object can take their addresses */
+    unsigned int coef_T_ = 0;
+    double coef_dom_[4] = {0.0, 0.0, 0.0, 0.0};
     int apply_cv();
+    int apply_tables();
+    int sample_tables(const BVP& P, int mask, const double dom[4], bool* changed);
     int ensure_problem(const BVP& P, bool with_subdomain, double rho);
     void copy_stats(const spac_point_stats& s);
 };

@@ -59,6 +59,11 @@ bool coefficient_set_of_kind(int kind, CoefSetEntry* out) {
 }
 
 int resolve_coefficients(const BVP& bvp, int* kind) {
+    /* a numeric coefficient has no compiled twin, whatever its function pointer says */
+    if (bvp.u.Es_Numerico() || bvp.g.Es_Numerico() || bvp.c.Es_Numerico() || bvp.f.Es_Numerico() ||
+        bvp.b.Es_Numerico() || bvp.F.Es_Numerico() || bvp.mu.Es_Numerico() || bvp.sigma.Es_Numerico() ||
+        bvp.psi.Es_Numerico() || bvp.varphi.Es_Numerico())
+        return SPAC_E_UNREGISTERED_PLUGIN;
     std::lock_guard<std::mutex> lk(reg_mutex());
     for (const CoefSetEntry& e : coef_sets()) {
         if (bvp.u.funcion == e.u && bvp.g.funcion == e.g && bvp.c.funcion == e.c && bvp.f.funcion == e.f &&
@@ -69,6 +74,30 @@ int resolve_coefficients(const BVP& bvp, int* kind) {
         }
     }
     return SPAC_E_UNREGISTERED_PLUGIN;
+}
+
+int resolve_numeric_coefficients(const BVP& bvp, int* kind, int* tab_mask, bool* F_zero) {
+    *tab_mask = 0;
+    *F_zero = false;
+    const int mask = (bvp.sigma.Es_Numerico() ? SPAC_TAB_SIGMA : 0) | (bvp.b.Es_Numerico() ? SPAC_TAB_DRIFT : 0) |
+                     (bvp.c.Es_Numerico() ? SPAC_TAB_KILLING : 0) | (bvp.f.Es_Numerico() ? SPAC_TAB_SOURCE : 0) |
+                     (bvp.g.Es_Numerico() ? SPAC_TAB_DIRICHLET : 0) | (bvp.u.Es_Numerico() ? SPAC_TAB_SOLUTION : 0);
+    if (bvp.F.Es_Numerico() || bvp.mu.Es_Numerico()) return SPAC_E_UNREGISTERED_PLUGIN;
+    if (!mask && !bvp.psi.Es_Numerico() && !bvp.varphi.Es_Numerico()) return resolve_coefficients(bvp, kind);
+    CoefSetEntry e;
+    if (!coefficient_set_of_kind(SPAC_COEF_VARCOEF_FRANKE, &e)) return SPAC_E_UNREGISTERED_PLUGIN;
+    const bool ok = (bvp.u.Es_Numerico() || bvp.u.funcion == e.u) && (bvp.g.Es_Numerico() || bvp.g.funcion == e.g) &&
+                    (bvp.c.Es_Numerico() || bvp.c.funcion == e.c) && (bvp.f.Es_Numerico() || bvp.f.funcion == e.f) &&
+                    (bvp.b.Es_Numerico() || bvp.b.funcion == e.b) &&
+                    (bvp.sigma.Es_Numerico() || bvp.sigma.funcion == e.sigma) &&
+                    (bvp.F.funcion == e.F || bvp.F.funcion == Vector_Defecto) && bvp.mu.funcion == e.mu &&
+                    (bvp.psi.Es_Numerico() || bvp.psi.funcion == e.psi) &&
+                    (bvp.varphi.Es_Numerico() || bvp.varphi.funcion == e.varphi);
+    if (!ok) return SPAC_E_UNREGISTERED_PLUGIN;
+    *kind = e.kind;
+    *tab_mask = mask;
+    *F_zero = bvp.F.funcion != e.F;
+    return SPAC_OK;
 }
 
 int resolve_boundary(const Frontera& fr, int* kind) {

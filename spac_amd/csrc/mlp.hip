@@ -830,6 +830,25 @@ static spac_status mlp_infer_dev(Mlp* m, const double* dx, const double* dy, int
     return SPAC_OK;
 }
 
+/* The executable graphs and events of one training call.  They are released on every
+ * return, error returns included, and only after the stream has drained, so that no graph
+ * is destroyed while a launch of it may still be queued. */
+struct TrainResources {
+    hipStream_t stream;
+    hipGraphExec_t gfull = nullptr, grest = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    explicit TrainResources(hipStream_t s) : stream(s) {}
+    TrainResources(const TrainResources&) = delete;
+    TrainResources& operator=(const TrainResources&) = delete;
+    ~TrainResources() {
+        (void)hipStreamSynchronize(stream);
+        if (gfull) (void)hipGraphExecDestroy(gfull);
+        if (grest) (void)hipGraphExecDestroy(grest);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 /* Capture `k` training steps into one executable graph. */
 static spac_status capture_steps(Mlp* m, int k, float lam, int t0, int steps, float lr0, float lr1,
                                  hipGraphExec_t* exec) {
@@ -953,33 +972,25 @@ spac_status spac_mlp_train(spac_mlp* hm, const double* x, const double* y, const
     constexpr int kPerGraph = 50;
     static_assert(kPerGraph % 2 == 0, "graph length must be even (step-counter parity)");
     const int full = cfg->steps / kPerGraph, rest = cfg->steps % kPerGraph;
-    hipGraphExec_t gfull = nullptr, grest = nullptr;
+    TrainResources r(m->stream);
     if (!cfg->eager) {
-        if (full && (rc = capture_steps(m, kPerGraph, lam, t0, cfg->steps, lr0, lr1, &gfull))) return rc;
-        if (rest && (rc = capture_steps(m, rest, lam, t0, cfg->steps, lr0, lr1, &grest))) {
-            if (gfull) (void)hipGraphExecDestroy(gfull);
-            return rc;
-        }
+        if (full && (rc = capture_steps(m, kPerGraph, lam, t0, cfg->steps, lr0, lr1, &r.gfull))) return rc;
+        if (rest && (rc = capture_steps(m, rest, lam, t0, cfg->steps, lr0, lr1, &r.grest))) return rc;
     }
-    hipEvent_t e0, e1;
-    MLP_HIP_CHECK(hipEventCreate(&e0));
-    MLP_HIP_CHECK(hipEventCreate(&e1));
-    MLP_HIP_CHECK(hipEventRecord(e0, m->stream));
+    MLP_HIP_CHECK(hipEventCreate(&r.e0));
+    MLP_HIP_CHECK(hipEventCreate(&r.e1));
+    MLP_HIP_CHECK(hipEventRecord(r.e0, m->stream));
     if (cfg->eager) {
         for (int i = 0; i < cfg->steps; ++i) enqueue_step(m, lam, true, t0, cfg->steps, lr0, lr1, t0 + i);
         MLP_HIP_CHECK(hipGetLastError());
     } else {
-        for (int i = 0; i < full; ++i) MLP_HIP_CHECK(hipGraphLaunch(gfull, m->stream));
-        if (rest) MLP_HIP_CHECK(hipGraphLaunch(grest, m->stream));
+        for (int i = 0; i < full; ++i) MLP_HIP_CHECK(hipGraphLaunch(r.gfull, m->stream));
+        if (rest) MLP_HIP_CHECK(hipGraphLaunch(r.grest, m->stream));
     }
-    MLP_HIP_CHECK(hipEventRecord(e1, m->stream));
-    MLP_HIP_CHECK(hipEventSynchronize(e1));
+    MLP_HIP_CHECK(hipEventRecord(r.e1, m->stream));
+    MLP_HIP_CHECK(hipEventSynchronize(r.e1));
     float ms = 0.f;
-    MLP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (gfull) (void)hipGraphExecDestroy(gfull);
-    if (grest) (void)hipGraphExecDestroy(grest);
+    MLP_HIP_CHECK(hipEventElapsedTime(&ms, r.e0, r.e1));
     m->adam_t += cfg->steps;
     if (rep) {
         rep->train_ms = ms;

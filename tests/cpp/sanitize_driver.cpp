@@ -1,7 +1,7 @@
 /*
  * sanitize_driver.cpp — runs the host code (mesh and interface tables, CSR pattern,
- * Actualiza_GB, host GMRES, text and row-file formats) and the CPU restatement of the
- * path solver under AddressSanitizer and UndefinedBehaviorSanitizer.  Built with g++/gcc
+ * Actualiza_GB, host GMRES, text and row-file formats, the pseudospectral rows, the RBF
+ * interpolator) and the CPU restatement of the path solver under AddressSanitizer and UndefinedBehaviorSanitizer.  Built with g++/gcc
  * only (no HIP): tests/test_sanitize.py compiles and runs it; any report aborts it.
  *
  * It follows the pipeline of the reference's serial driver (main_test_serial.cpp:11-83):
@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "Interpolador_RBF.hpp"
 #include "spac_fk.h"
 #include "spac_mesh.h"
 
@@ -196,6 +197,65 @@ static void run_points(int threads) {
     std::printf("points: %zu paths, %lld path-steps\n", tp, total);
 }
 
+static void run_pseudospectral_and_rbf() {
+    /* pseudospectral rows of every interior-circle node of a 4 x 4 mesh (44 nodes per circle),
+       twice (the second call reuses the factorisation), and an RBF fit of scattered data */
+    spac_mesh_desc md;
+    std::memset(&md, 0, sizeof md);
+    md.domain[2] = 1.25; md.domain[3] = 1.25;
+    md.n_side = 4; md.overlap = 1.0; md.theta0 = 0.1428; md.nodes_per_circle = 44;
+    md.c2_corner = 1.5; md.c2_side = 1.1; md.rbf_kind = SPAC_RBF_GAUSS_STENCIL;
+    md.gauss_eps = 3.0; md.stencil_size = 15; md.fk_all_circles = 1;
+    spac_mesh* m = nullptr;
+    CHECK(spac_mesh_build_square(&md, &m) == SPAC_OK, "ps mesh");
+    spac_mesh_info info;
+    CHECK(spac_mesh_get_info(m, &info) == SPAC_OK, "ps info");
+    spac_interface_table t;
+    CHECK(spac_mesh_interfaces(m, &t) == SPAC_OK, "ps interfaces");
+    const int nj = info.n_jobs;
+    std::vector<int32_t> row(nj), iface(nj), front(nj);
+    std::vector<double> x(nj), y(nj);
+    CHECK(spac_mesh_jobs(m, row.data(), x.data(), y.data(), iface.data(), front.data()) == SPAC_OK, "ps jobs");
+    std::vector<double> jx, jy;
+    std::vector<int32_t> ji, jf;
+    int64_t total = 0;
+    for (int k = 0; k < nj; ++k)
+        if (t.code[iface[k]] == 1 && !front[k]) {
+            jx.push_back(x[k]); jy.push_back(y[k]); ji.push_back(iface[k]); jf.push_back(0);
+            total += t.node_offset[iface[k] + 1] - t.node_offset[iface[k]];
+        }
+    const int n = (int)jx.size();
+    CHECK(n > 0, "interior-circle jobs");
+    std::vector<uint32_t> paths(n, 1), stream(n, 0);
+    std::vector<double> hh(n, 1e-3), B(n), G(total);
+    std::vector<int32_t> col(total);
+    std::vector<int64_t> goff(n + 1);
+    spac_node_batch nb;
+    std::memset(&nb, 0, sizeof nb);
+    nb.n = n; nb.x = jx.data(); nb.y = jy.data(); nb.iface = ji.data(); nb.frontera = jf.data();
+    nb.paths = paths.data(); nb.h = hh.data(); nb.stream = stream.data();
+    spac_row_block rb;
+    std::memset(&rb, 0, sizeof rb);
+    rb.B = B.data(); rb.G = G.data(); rb.G_col = col.data(); rb.G_offset = goff.data();
+    spac_ps_report rep;
+    for (int pass = 0; pass < 2; ++pass)
+        CHECK(spac_ps_assemble_rows(SPAC_COEF_FRANKE_POISSON, &t, &nb, 21, &rb, &rep) == SPAC_OK, "ps rows");
+    for (int k = 0; k < n; ++k) CHECK(std::isfinite(B[k]), "ps B %d", k);
+    CHECK(spac_mesh_destroy(m) == SPAC_OK, "ps destroy");
+    std::vector<spac::Vec2> cs;
+    std::vector<double> z;
+    for (int k = 0; k < 200; ++k) {
+        cs.emplace_back(std::fmod(0.618034 * k, 1.0), std::fmod(0.414214 * k + 0.2, 1.0));
+        z.push_back(std::sin(3.0 * cs.back()(0)) * cs.back()(1));
+    }
+    spac::Interpolador_RBF r;
+    CHECK(r.Inicializa(cs, z) == 0, "rbf fit");
+    double e = 0.0;
+    for (int k = 0; k < 200; ++k) e = std::fmax(e, std::fabs(r.Interpola(cs[k]) - z[k]));
+    CHECK(e < 1e-8 && std::isfinite(r.Deriva_t2(spac::Vec2(0.3, 0.4))), "rbf at centres %.2e", e);
+    std::printf("pseudospectral: %d rows, %d factorisations; rbf: 200 centres\n", n, rep.factorizations);
+}
+
 int main(int argc, char** argv) {
     const std::string dir = argc > 1 ? argv[1] : ".";
     const int threads = argc > 2 ? std::atoi(argv[2]) : 2;
@@ -203,6 +263,7 @@ int main(int argc, char** argv) {
     run_mesh(4, 24, SPAC_RBF_GAUSS_STENCIL, dir, 40, threads);
     run_mesh(4, 24, SPAC_RBF_IMQ_GLOBAL, dir, 40, threads);
     run_mesh(6, 20, SPAC_RBF_GAUSS_STENCIL, dir, 20, threads);
+    run_pseudospectral_and_rbf();
     std::printf("sanitized run ok\n");
     return 0;
 }

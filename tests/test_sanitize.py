@@ -2,7 +2,7 @@
 
 tests/cpp/sanitize_driver.cpp runs the mesh and interface-table builder (Malla, Interfaz,
 Calcula_Psi), the CSR pattern and Actualiza_GB, the host GMRES, the text formats and the row
-files, with every row assembled by the restatement (oracle/fk_oracle.c, OpenMP).  The
+files, the pseudospectral rows and the RBF interpolator, with every row assembled by the restatement (oracle/fk_oracle.c, OpenMP).  The
 sources are compiled with g++/gcc (no HIP code is involved) and any sanitizer report ends the
 run with a failure.
 """
@@ -16,7 +16,8 @@ import pytest
 ROOT = Path(__file__).resolve().parent.parent
 CPP = ROOT / "spac_amd" / "cpp"
 HOST_SOURCES = ["spac_error.cpp", "Malla.cpp", "Interfaz.cpp", "SistemaLineal.cpp", "BVPs.cpp", "mesh_capi.cpp",
-                "rows_io.cpp"]
+                "rows_io.cpp", "plugin_registry.cpp", "PseudoespectralCirculoSolver.cpp", "ps_capi.cpp",
+                "Interpolador_RBF.cpp"]
 SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-g", "-O1"]
 
 

@@ -18,8 +18,12 @@ DRIVER = Path(__file__).resolve().parent / "cpp" / "_bin" / "surface_driver"
 
 
 def run_driver(mode):
+    return run_driver_args(mode)
+
+
+def run_driver_args(*args):
     assert DRIVER.exists(), "build() compiles tests/cpp/surface_driver.cpp"
-    out = subprocess.run([str(DRIVER), mode], capture_output=True, text=True, timeout=300)
+    out = subprocess.run([str(DRIVER), *args], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     return json.loads(out.stdout)
 
@@ -112,3 +116,71 @@ def test_surface_gpu_reflecting_side(gpu):
     b = SolveBatchArrays([0.05], [0.5], 4000, 1e-3, [0])
     so = o.reduce(b, o.simulate(desc, b, seed=0))[0]
     assert r["phi"] == so["phi"] and r["phi_VR"] == so["phi_VR"] and r["steps"] == so["path_steps"]
+
+
+def test_surface_rbf_interpolator_and_numeric_classes():
+    """Interpolador_RBF (the reference's stub class, BVP/Interpolador_RBF.cpp:2-52, made real) and
+    the numeric coefficient classes (BVP/Coeficiente_*_Numerico.hpp), on the host.
+
+    Pins: interpolation at the centres; linear functions reproduced exactly with their
+    derivatives (the polynomial augmentation); analytic first and second derivatives, Cartesian
+    and polar, equal central differences of the interpolant; the error on Franke's function falls
+    as centres are added; repeated centres are refused."""
+    r = run_driver("rbf")
+    assert r["rc"] == 0
+    assert r["bad_rc"] == -1 and r["bad_nan"] == 1
+    assert r["at_centres"] < 1e-10
+    assert r["lin_err"] < 1e-10 and r["lin_d1"] < 1e-9 and r["lin_d2"] < 1e-8
+    assert r["fd_err"] < 1e-4 and r["polar_err"] < 1e-4
+    assert r["e1600"] < 0.6 * r["e400"] and r["e1600"] < 1e-2
+    # Es_Numerico: numeric scalar, a numeric scalar copied into a base-class field, matrix, vector;
+    # compiled coefficients are not numeric
+    assert r["numerico"] == [1, 1, 1, 1, 0, 0]
+    assert r["sliced_value"] == r["direct_value"]
+    s = r["direct_value"]
+    assert r["matrix"] == [s, 0.0, 0.0, s] and r["vector"] == [s, 0.0]
+    # numeric coefficients over the Franke-Poisson set have no device form; the exact-match
+    # resolver refuses any numeric coefficient; over the variable-coefficient set they resolve
+    # (sigma | c tabulated, F left at the default zero); a numeric F is refused
+    assert r["num_rc_poisson"] == abi.SPAC_E_UNREGISTERED_PLUGIN
+    assert r["plain_rc_numeric"] == abi.SPAC_E_UNREGISTERED_PLUGIN
+    assert r["num_rc"] == 0 and r["num_kind"] == abi.SPAC_COEF_VARCOEF_FRANKE
+    assert r["num_mask"] == abi.SPAC_TAB_SIGMA | abi.SPAC_TAB_KILLING and r["num_F_zero"] == 1
+    assert r["numF_rc"] == abi.SPAC_E_UNREGISTERED_PLUGIN
+
+
+@pytest.mark.gpu
+def test_surface_gpu_numeric_coefficients(gpu, tmp_path):
+    """FeynmanKacSolver with Coeficiente_*_Numerico for sigma, b, c, f, g, u (Interpolador_Tabla
+    of the variable-coefficient set on the solver's 513 x 513 grid): the sampled table, fed to
+    the restatement, gives the same estimators bit for bit; against the compiled set on the same
+    paths the estimators move by the interpolation error only; a numeric c from scattered RBF
+    data is sampled through Interpolador_RBF; the refusals."""
+    print("surface driver numeric", flush=True)
+    tab = tmp_path / "table.bin"
+    r = run_driver_args("numeric", str(tab))
+    assert r["rc"] == [0, 0, 0, 0, 0] and r["rbf_rc"] == 0
+    assert r["mask"] == abi.SPAC_TAB_ALL and r["same_table"] == 1
+    vals = np.fromfile(tab, dtype=np.float64).reshape(513, 513, 8)
+    dom = (0.0, 0.0, 1.0, 1.0)
+    desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=dom)
+    b = SolveBatchArrays([0.6], [0.7], 4000, 1e-3, [0])
+    o.set_cv(abi.SPAC_CV_NONE)
+    o.set_coef_tables(abi.SPAC_TAB_ALL, vals, dom)
+    try:
+        so = o.reduce(b, o.simulate(desc, b, seed=0))[0]
+    finally:
+        o.set_coef_tables()
+        o.set_cv()
+    n, c = r["numeric"], r["compiled"]
+    assert n["phi"] == so["phi"] and n["phiphi"] == so["phiphi"] and n["steps"] == so["path_steps"]
+    se = np.sqrt((c["phiphi"] - c["phi"] ** 2) / 4000)
+    assert abs(n["phi"] - c["phi"]) < 0.05 * se + 1e-4
+    assert abs(n["steps"] - c["steps"]) <= 0.001 * c["steps"]
+    assert r["rbf"]["tab_err"] == 0.0  # the table holds Interpolador_RBF's own values at the nodes
+    assert abs(r["rbf"]["phi"] - c["phi"]) < 4 * se
+    assert r["bad_sigma"] == abi.SPAC_E_INVALID_ARG
+    assert r["bad_F"] == abi.SPAC_E_UNREGISTERED_PLUGIN
+    assert r["bad_refl"] == abi.SPAC_E_INVALID_ARG
+    # F left at its default zero: the plugin control variate is xi = 0
+    assert r["F_zero"]["phi_VR"] == r["F_zero"]["phi"] and r["F_zero"]["xixi"] == 0.0
