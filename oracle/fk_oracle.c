@@ -29,6 +29,7 @@
  * Parity status of each function is stated beside it ("pinned by" = the test
  * that checks it independently of this file).
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -788,8 +789,12 @@ void oracle_boundary_eval(int kind, const double* params, const double* x, const
  *                       G_j = -(sum_i iPsi(i,j) S_i) / N
  *   IMQ (direct form, direct != 0, Mesh/Interfaz.cpp:253-261):
  *                       G_j = (sum_p sum_i -Y_p iPsi(i,j) phi(theta_p, theta_i)) / N
- *   Gaussian stencils:  G_j = -(sum_p Y_p H_j(theta_p)) / N over the paths whose
- *                       stencil holds column j.
+ *   Gaussian stencils (moment form, spac_rbf.h): fixed-point moments
+ *                       Phi[s][i] = sum_p Y_p phi_i(theta_p) by stencil start s, then
+ *                       G_c = -(sum_{r, s = c - r} sum_i iPsi_s(i, r) Phi[s][i]) / N
+ *   Gaussian stencils (direct form, direct != 0, Mesh/Interfaz.cpp:253-261):
+ *                       G_j = -(sum_p Y_p H_j(theta_p)) / N over the paths whose
+ *                       stencil holds column j, sixteen TwoSum partials.
  * Pinned by: tests/test_rows.py (cardinality of the interpolation tables,
  * consistency sum_j u(x_j) H_j(theta) ~ u(X) on the arcs, moment form = direct
  * form within cond(Psi) eps, G rows summing to -E[Y 1{circle exit}]). */
@@ -840,6 +845,47 @@ int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_tab
         int64_t np = paths[k];
         int stencil = t->rbf_kind == SPAC_RBF_GAUSS_STENCIL;
         int S = stencil ? (t->stencil_size < M ? t->stencil_size : M) : 0;
+        if (stencil && !direct) {
+            /* moment form (spac_rbf.h): fixed-point Phi[s][i], then G_c = -(sum over r in
+             * order, s = c - r, of the FMA chain over i of iPsi_s(i, r) Phi[s][i]) / N */
+            double ymax = 0.0;
+            for (int64_t p = 0; p < np; ++p) {
+                int64_t g = off[k] + p;
+                double ay = fabs(rec.Y[g]);
+                if (rec.outcome[g] == SPAC_OUT_SUBDOMAIN && ay <= DBL_MAX && ay > ymax) ymax = ay;
+            }
+            const int q = spac_moment_scale((double)np, ymax);
+            int64_t* mom = (int64_t*)calloc((size_t)M * S, sizeof(int64_t));
+            double* momd = (double*)malloc(sizeof(double) * (size_t)M * S);
+            for (int64_t p = 0; p < np; ++p) {
+                int64_t g = off[k] + p;
+                double Yp = rec.Y[g];
+                if (rec.outcome[g] != SPAC_OUT_SUBDOMAIN || !(fabs(Yp) <= DBL_MAX)) continue;
+                double theta = 0.0, phi[32];
+                spac_exit_angle(code, rec.x_tau[g] - cx, rec.y_tau[g] - cy, th[0], &theta);
+                int s0 = (code == 1) ? spac_circle_stencil(M, S, t->gauss_eps, theta, phi)
+                                     : spac_arc_stencil(th, M, S, t->gauss_eps, theta, phi);
+                for (int i = 0; i < S; ++i)
+                    mom[(size_t)s0 * S + i] = (int64_t)((uint64_t)mom[(size_t)s0 * S + i] +
+                                                        (uint64_t)spac_to_fixed(Yp * phi[i], q));
+            }
+            for (int i = 0; i < M * S; ++i) momd[i] = spac_from_fixed(mom[i], q);
+            double N = (double)np;
+            for (int c = 0; c < M; ++c) {
+                double acc = 0.0;
+                for (int r = 0; r < S; ++r) {
+                    int s = c - r;
+                    if (code == 1) { if (s < 0) s += M; }
+                    else if (s < 0 || s > M - S) continue;
+                    const double* blk = ipsi + ((code == 1) ? 0 : (int64_t)s * S * S);
+                    for (int i = 0; i < S; ++i) acc = SPAC_FMA(blk[i * S + r], momd[(size_t)s * S + i], acc);
+                }
+                Gk[c] = (np > 0) ? -acc / N : 0.0;
+            }
+            free(mom);
+            free(momd);
+            continue;
+        }
         /* Column sums in the fixed order of the GPU, paths in chunks of 256:
          * IMQ — partial w holds paths 64w..64w+63 of every chunk, four partials;
          * stencils — partial 4 (p % 256 / 64) + p % 4, sixteen partials; each
@@ -899,6 +945,11 @@ int oracle_assemble_rows(const spac_problem_desc* desc, const spac_interface_tab
     free(paths); free(circle); free(off);
     return rc;
 }
+
+/* fixed-point moment probes (spac_rbf.h) */
+int oracle_moment_scale(double n_paths, double ymax) { return spac_moment_scale(n_paths, ymax); }
+int64_t oracle_to_fixed(double v, int q) { return spac_to_fixed(v, q); }
+double oracle_from_fixed(int64_t n, int q) { return spac_from_fixed(n, q); }
 
 /* exit angle probe (spac_exit_angle) */
 int oracle_exit_angle(int code, double dx, double dy, double first, double* theta) {

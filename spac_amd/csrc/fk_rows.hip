@@ -10,14 +10,14 @@
  *      exits and (Z + xi) over circle exits, and the node statistics;
  *   3. boundary nodes (frontera) get B = g(x) with no paths (:346-347);
  *   4. the G kernels (one block per node) accumulate the row weights over the
- *      circle-exit paths.  Global IMQ interpolation uses the moment form
- *      S_i = sum_p Y_p phi(theta_p, theta_i), G_j = -sum_i iPsi(i,j) S_i / N;
- *      Gaussian stencils accumulate G_j += Y_p H_j(theta_p) directly for the
- *      stencil_size columns of each path's stencil.  Every column is a
- *      TwoSum-compensated sum in one fixed order (paths in chunks of 256;
- *      IMQ: wave w owns paths 64w..64w+63 of every chunk, four wave partials;
- *      stencils: sixteen partials by p / 64 and p % 4), so the result does not
- *      depend on the launch and equals the CPU restatement's.
+ *      circle-exit paths, both in moment form.  Global IMQ interpolation:
+ *      S_i = sum_p Y_p phi(theta_p, theta_i), G_j = -sum_i iPsi(i,j) S_i / N,
+ *      each S_i a TwoSum-compensated sum in one fixed order (paths in chunks of
+ *      256, wave w owns paths 64w..64w+63 of every chunk, four wave partials).
+ *      Gaussian stencils: fixed-point moments Phi[s][i] = sum Y_p phi_i by
+ *      stencil start (exact integer sums), then one S x S stencil inverse per
+ *      row (spac_rbf.h).  Neither depends on the launch, and both equal the
+ *      CPU restatement bit for bit.
  */
 #include <hip/hip_runtime.h>
 
@@ -171,205 +171,106 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_imq_kernel(RowsArgs a) {
     }
 }
 
-/* Gaussian stencil rows: per 256-path chunk every thread computes its path's exit angle,
- * stencil start s and the S Gaussian values phi_i into LDS.  Then each quarter-wave takes one
- * path at a time: lane r forms Y H_{s+r}(theta), H_{s+r} = sum_i iPsiS(i, r) phi_i with exact
- * FMAs, i in order (the stencil inverse is read a row of 16 contiguous values at a time, from
- * LDS for full circles and L1/L2 for arcs), and adds it into column s + r (mod M) of the
- * quarter-wave's own copy of the row.  Copy (w, q) holds the paths p of every chunk with
- * p / 64 = w and p % 4 = q, in path order; the 16 copies are merged in (w, q) order.
- * PAIR: rows of at most 128 paths, two per block (threads 128 h .. 128 h + 127 take row
- * 2 blockIdx + h), one chunk each.  Such a row only fills copies w = 0, 1, so its 8 copies
- * merged in order give the unpaired sum (the other 8 are zero), and half the block no longer
- * idles.  Every barrier is reached by the whole block: one chunk per half, no early return.
- * Dynamic LDS: kRowsBlock * wstride doubles of values, then 16 * mmax TwoSum accumulators. */
-constexpr int kCopies = kRowsWaves * 4;
-constexpr int kPairMaxPaths = kRowsBlock / 2;
-constexpr int kPhase2Group = 4; /* paired rows: paths of a quarter-wave whose weights are formed together */
-
-template <bool PAIR>
-__global__ void __launch_bounds__(kRowsBlock) fk_rows_g_stencil_kernel(RowsArgs a, int wstride, int mmax) {
-    extern __shared__ double s_dyn[];
-    constexpr int kRowThreads = PAIR ? kRowsBlock / 2 : kRowsBlock;
-    constexpr int kRowCopies = PAIR ? kCopies / 2 : kCopies;
+/* Gaussian stencil rows in moment form (spac_rbf.h), one block per row.
+ * Pass 1: the largest |Y| over the row's circle exits (a max: exact in any order) fixes the
+ * fixed-point scale q.  Pass 2: thread t takes the paths t, t + BLK, ...: exit angle, stencil
+ * start s and the S Gaussians phi_i, then S 64-bit integer adds of Y phi_i into the moments
+ * Phi[s][i] in LDS — integer sums, so they do not depend on the order of the adds.  Then the
+ * moments become doubles and each column is one FMA chain in the restatement's order,
+ *   G_c = -(sum over r = 0..S-1 with s = c - r, of sum over i = 0..S-1 of iPsi_s(i, r) Phi[s][i]) / N
+ * (s taken mod M on a full circle; on an arc only starts 0..M-S exist).  There are no
+ * per-path weight products (the S x S stencil inverse is applied once per row, not once per
+ * path) and no read-modify-write chains: a path costs its exit angle, four exps and S LDS
+ * atomics.  Non-finite Y values are left out (the row's estimator already reports them).
+ * Dynamic LDS: M_max * S 64-bit moments. */
+template <int BLK>
+__global__ void __launch_bounds__(BLK) fk_rows_g_moment_kernel(RowsArgs a) {
+    extern __shared__ unsigned long long s_mom[];
+    __shared__ double s_th[kMaxCols];
+    __shared__ double s_blk[kMaxStencil * kMaxStencil];
+    __shared__ double s_red[BLK / 64];
     const int tid = threadIdx.x;
-    const int half = PAIR ? (tid >> 7) : 0;
-    const int ltid = PAIR ? (tid & (kRowThreads - 1)) : tid; /* the thread's index within its row */
-    const int sbase = half * kRowThreads;                     /* the row's slots in s_v, s_s, s_y */
-    const int job = PAIR ? 2 * (int)blockIdx.x + half : (int)blockIdx.x;
-    const bool has_job = job < a.n_jobs; /* PAIR: the last block's second half may be empty */
-    const int jl = has_job ? job : a.n_jobs - 1;
-    const int64_t np = has_job ? a.paths[jl] : 0;
-    const int iface = a.job_iface[jl];
+    const int job = (int)blockIdx.x;
+    const int64_t np = a.paths[job];
+    const int iface = a.job_iface[job];
     const int nb = a.node_offset[iface];
     const int M = a.node_offset[iface + 1] - nb;
-    const double* th = a.theta + nb;
     const int code = a.code[iface];
+    const bool circle = code == 1;
     const double cx = a.cx[iface], cy = a.cy[iface];
     const int S = min(a.stencil_size, M);
     const double* ipsi = a.ipsi + (a.ipsi_offset[iface] >= 0 ? a.ipsi_offset[iface] : 0);
-    double* G = a.G + a.g_off[jl];
-
-    double* s_v = s_dyn; /* per path of the chunk: its S Gaussian values phi */
-    spac_ksum* s_acc =
-        reinterpret_cast<spac_ksum*>(s_dyn + (size_t)kRowsBlock * wstride) + (size_t)half * kRowCopies * mmax;
-    __shared__ double s_th_all[PAIR ? 2 : 1][kMaxCols];
-    __shared__ int s_s[kRowsBlock]; /* stencil start, -1: not a circle exit */
-    __shared__ double s_y[kRowsBlock];
-    __shared__ double s_blk_all[PAIR ? 2 : 1][kMaxStencil * kMaxStencil];
-    double* const s_th = s_th_all[half];
-    double* const s_blk = s_blk_all[half];
-
-    const int lane = tid & 63, wave = (tid >> 6) & (kRowThreads / 64 - 1);
-    for (int i = ltid; i < M; i += kRowThreads) s_th[i] = th[i];
-    const bool circle = code == 1;
-    if (circle)
-        for (int i = ltid; i < S * S; i += kRowThreads) s_blk[i] = ipsi[i];
-    for (int i = ltid; i < kRowCopies * M; i += kRowThreads) s_acc[i] = spac_ksum_zero();
-    s_s[tid] = -1; /* phase 2 may read entries of paths beyond the chunk: -1 or a valid start */
-    __syncthreads();
-    if (!PAIR && np == 0) {
-        for (int j = tid; j < M; j += kRowsBlock) G[j] = 0.0;
+    double* G = a.G + a.g_off[job];
+    if (np == 0) { /* uniform over the block: nobody reaches a barrier */
+        for (int c = tid; c < M; c += BLK) G[c] = 0.0;
         return;
     }
-    const int quarter = lane >> 4, r = lane & 15;
-    spac_ksum* my = s_acc + (size_t)(wave * 4 + quarter) * M;
-    const int64_t off = a.rec_off[jl];
-    /* the records of the next chunk are loaded while this one is processed (one
-     * round trip per chunk instead of two dependent ones on the critical path) */
-    int8_t n_out = 0;
-    double n_x = 0.0, n_y = 0.0, n_Y = 0.0;
-    if (ltid < np) {
-        const int64_t g = off + ltid;
-        n_out = a.outcome[g]; n_x = a.xt[g]; n_y = a.yt[g]; n_Y = a.Y[g];
+    for (int i = tid; i < M; i += BLK) s_th[i] = a.theta[nb + i];
+    if (circle)
+        for (int i = tid; i < S * S; i += BLK) s_blk[i] = ipsi[i];
+    for (int i = tid; i < M * S; i += BLK) s_mom[i] = 0ull;
+    const int64_t off = a.rec_off[job];
+    double ym = 0.0;
+    for (int64_t p = tid; p < np; p += BLK) {
+        const int64_t g = off + p;
+        const double ay = SPAC_FABS(a.Y[g]);
+        if (a.outcome[g] == SPAC_OUT_SUBDOMAIN && ay <= __DBL_MAX__ && ay > ym) ym = ay;
     }
-    /* PAIR: exactly one chunk for both halves (np <= 128), so the barriers stay uniform */
-    const int64_t nloop = PAIR ? 1 : np;
-    for (int64_t base = 0; base < nloop; base += kRowThreads) {
-        const int cnt = (int)((np - base < (int64_t)kRowThreads) ? (np - base) : kRowThreads);
-        const int8_t c_out = n_out;
-        const double c_x = n_x, c_y = n_y, c_Y = n_Y;
-        if (base + kRowThreads + ltid < np) {
-            const int64_t g = off + base + kRowThreads + ltid;
-            n_out = a.outcome[g]; n_x = a.xt[g]; n_y = a.yt[g]; n_Y = a.Y[g];
-        }
-        /* phase 1, one thread per path: exit angle, stencil start, Gaussians */
-        if (ltid < cnt) {
-            int s = -1;
-            if (c_out == SPAC_OUT_SUBDOMAIN) {
-                double theta = 0.0;
-                spac_exit_angle(code, c_x - cx, c_y - cy, s_th[0], &theta);
-                double phi[kMaxStencil];
-                s = circle ? spac_circle_stencil(M, S, a.gauss_eps, theta, phi)
-                           : spac_arc_stencil(s_th, M, S, a.gauss_eps, theta, phi);
-                const double Yp = c_Y;
-                if (!PAIR && circle) { /* the one stencil inverse, broadcast from LDS: Y w_r straight away */
-                    for (int rr = 0; rr < S; ++rr) {
-                        double w = 0.0;
-                        for (int i = 0; i < S; ++i) w = __builtin_fma(s_blk[i * S + rr], phi[i], w);
-                        s_v[tid * wstride + rr] = Yp * w;
-                    }
-                } else {
-                    for (int i = 0; i < S; ++i) s_v[tid * wstride + i] = phi[i];
-                }
-                s_y[tid] = Yp;
-            }
-            s_s[tid] = s;
-        }
-        __syncthreads();
-        /* phase 2, one quarter-wave per path, lane r = stencil column r: the weight
-         * w_r = sum_i iPsi_s[i][r] phi_i (the stencil's inverse read row by row, 16
-         * contiguous values per quarter-wave; for full circles the unpaired kernel formed Y w_r
-         * in phase 1 already, the paired one forms it here from the inverse in LDS — the same
-         * FMA chain either way).  The quarter-wave's 16 paths of the chunk
-         * are independent here, so their loads overlap: indices are clamped to valid
-         * entries and unused results dropped instead of branching.  Path p of the row
-         * sits in slot sbase + p. */
-        const int p0 = wave * 64 + quarter;
-        const int p1 = min(cnt, wave * 64 + 64);
-        const int rr = (r < S) ? r : 0;
-        if constexpr (PAIR) {
-            /* paired rows (short rows, many of them): full circles too form w_r here, from the
-               one inverse in LDS, which keeps phase 1 short.  Groups of kPhase2Group paths form
-               their weights together, then add them in path order. */
-#pragma unroll
-            for (int k0 = 0; k0 < 16; k0 += kPhase2Group) {
-                double yw[kPhase2Group];
-#pragma unroll
-                for (int u = 0; u < kPhase2Group; ++u) {
-                    const int p = p0 + 4 * (k0 + u);
-                    const int pc = sbase + ((p < p1) ? p : p0);
-                    const int s = s_s[pc];
-                    const int sc = (s >= 0) ? s : 0;
-                    const double* blk = circle ? s_blk : ipsi + (int64_t)sc * S * S;
-                    const double* v = s_v + pc * wstride;
-                    double w = 0.0;
-                    for (int i = 0; i < S; ++i) w = __builtin_fma(blk[i * S + rr], v[i], w);
-                    yw[u] = s_y[pc] * w;
-                }
-#pragma unroll
-                for (int u = 0; u < kPhase2Group; ++u) {
-                    const int p = p0 + 4 * (k0 + u);
-                    if (p < p1) {
-                        const int s = s_s[sbase + p];
-                        if (s >= 0 && r < S) {
-                            int c = s + r;
-                            if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
-                            my[c] = spac_ksum_add(my[c], yw[u]);
-                        }
-                    }
-                }
+    for (int o = 32; o > 0; o >>= 1) ym = fmax(ym, __shfl_xor(ym, o));
+    if ((tid & 63) == 0) s_red[tid >> 6] = ym;
+    __syncthreads();
+    ym = s_red[0];
+    for (int w = 1; w < BLK / 64; ++w) ym = fmax(ym, s_red[w]);
+    const int q = spac_moment_scale((double)np, ym);
+    for (int64_t p = tid; p < np; p += BLK) {
+        const int64_t g = off + p;
+        if (a.outcome[g] != SPAC_OUT_SUBDOMAIN) continue;
+        const double Yp = a.Y[g];
+        if (!(SPAC_FABS(Yp) <= __DBL_MAX__)) continue;
+        double theta = 0.0;
+        spac_exit_angle(code, a.xt[g] - cx, a.yt[g] - cy, s_th[0], &theta);
+        /* the Gaussians one at a time (the operations of spac_circle_stencil / spac_arc_stencil) */
+        if (circle) {
+            double ga, gb;
+            const int s = spac_circle_stencil_ab(M, S, a.gauss_eps, theta, &ga, &gb);
+            unsigned long long* row = s_mom + s * S;
+            spac_gauss_seq gs = spac_gauss_seq_start(ga, gb);
+            for (int i = 0; i < S; ++i) {
+                if (i > 0) spac_gauss_seq_next(&gs);
+                atomicAdd(row + i, (unsigned long long)spac_to_fixed(Yp * gs.p, q));
             }
         } else {
-            double yw[16];
-            if (circle) {
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const int p = p0 + 4 * k;
-                    const int pc = sbase + ((p < p1) ? p : p0);
-                    yw[k] = s_v[pc * wstride + rr];
-                }
-            } else
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int p = p0 + 4 * k;
-                const int pc = sbase + ((p < p1) ? p : p0);
-                const int s = s_s[pc];
-                const int sc = (s >= 0) ? s : 0;
-                const double* blk = circle ? s_blk : ipsi + (int64_t)sc * S * S;
-                const double* v = s_v + pc * wstride;
-                double w = 0.0;
-                for (int i = 0; i < S; ++i) w = __builtin_fma(blk[i * S + rr], v[i], w);
-                yw[k] = s_y[pc] * w;
-            }
-            /* then Y w_r into column s + r (mod M) of the quarter-wave's own copy, paths in order */
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int p = p0 + 4 * k;
-                if (p < p1) {
-                    const int s = s_s[sbase + p];
-                    if (s >= 0 && r < S) {
-                        int c = s + r;
-                        if (c >= M) c -= M; /* full circles wrap; arcs never reach M */
-                        my[c] = spac_ksum_add(my[c], yw[k]);
-                    }
-                }
-            }
+            double L;
+            const int s = spac_arc_stencil_start(s_th, M, S, theta, &L);
+            unsigned long long* row = s_mom + s * S;
+            for (int i = 0; i < S; ++i)
+                atomicAdd(row + i,
+                          (unsigned long long)spac_to_fixed(Yp * spac_gauss_phi((theta - s_th[s + i]) / L, a.gauss_eps), q));
         }
-        __syncthreads();
     }
-    if (!has_job) return;
-    if (PAIR && np == 0) {
-        for (int j = ltid; j < M; j += kRowThreads) G[j] = 0.0;
-        return;
-    }
+    __syncthreads();
+    for (int i = tid; i < M * S; i += BLK) s_mom[i] = spac_f64_as_u64(spac_from_fixed((int64_t)s_mom[i], q));
+    __syncthreads();
     const double N = (double)np;
-    for (int c = ltid; c < M; c += kRowThreads) {
-        spac_ksum t = s_acc[c];
-        for (int k = 1; k < kRowCopies; ++k) t = spac_ksum_merge(t, s_acc[(size_t)k * M + c]);
-        G[c] = -spac_ksum_value(t) / N;
+    for (int c = tid; c < M; c += BLK) {
+        double acc = 0.0;
+        for (int r = 0; r < S; ++r) {
+            int s = c - r;
+            if (circle) {
+                if (s < 0) s += M;
+            } else if (s < 0 || s > M - S) {
+                continue;
+            }
+            const double* blk = circle ? s_blk : ipsi + (int64_t)s * S * S;
+            const unsigned long long* mrow = s_mom + s * S;
+            for (int i = 0; i < S; ++i) acc = __builtin_fma(blk[i * S + r], spac_u64_as_f64(mrow[i]), acc);
+        }
+        G[c] = -acc / N;
     }
 }
+
+/* rows of at most this many paths take one wave per row */
+constexpr int64_t kMomentSmallRow = 1024;
 
 /* B for boundary nodes (g at the node) and the copy of every node's B. */
 template <class Coef>
@@ -430,8 +331,7 @@ struct RowsSetup {
     RowsArgs ra;
     int n = 0;
     int mmax = 1;     /* largest interface of the batch (dynamic LDS of the stencil kernel) */
-    int wstride = 1;  /* odd row stride of the per-path values */
-    bool pair_rows = false; /* every row has at most kPairMaxPaths paths: two rows per G block */
+    int g_block = 256; /* threads per row of the stencil kernel: 64 when every row is short */
     const int32_t* d_front = nullptr;
     double* d_B = nullptr;
     spac_point_stats* d_stats = nullptr;
@@ -440,7 +340,7 @@ struct RowsSetup {
 };
 
 static size_t stencil_lds_bytes(const RowsSetup& rs) {
-    return (size_t)kRowsBlock * rs.wstride * 8 + (size_t)kCopies * rs.mmax * sizeof(spac_ksum);
+    return (size_t)rs.mmax * (size_t)std::max(1, std::min(rs.ra.stencil_size, rs.mmax)) * 8;
 }
 
 } // namespace spac
@@ -533,18 +433,17 @@ static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const
         int f = nb->iface[k];
         rs->mmax = std::max(rs->mmax, t->node_offset[f + 1] - t->node_offset[f]);
     }
-    rs->wstride = std::min(t->stencil_size, rs->mmax) | 1;
+    rs->ra.stencil_size = t->stencil_size;
     if (t->rbf_kind == SPAC_RBF_GAUSS_STENCIL) {
         int lds_max = 0;
         ROWS_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
-        const size_t need = stencil_lds_bytes(*rs) + 6 * 1024; /* + the kernel's static LDS */
+        const size_t need = stencil_lds_bytes(*rs) + 5 * 1024; /* + the kernel's static LDS */
         if (need > (size_t)lds_max)
             return set_error(SPAC_E_INVALID_ARG, "interfaces of %d nodes need %zu B of LDS per block (device: %d)",
                              rs->mmax, need, lds_max);
         uint32_t pmax = 0;
         for (int k = 0; k < n; ++k) pmax = std::max(pmax, paths[k]);
-        /* the paired kernel's static LDS holds two rows' angles and stencil inverses */
-        rs->pair_rows = pmax <= (uint32_t)kPairMaxPaths && stencil_lds_bytes(*rs) + 12 * 1024 <= (size_t)lds_max;
+        rs->g_block = (int64_t)pmax <= kMomentSmallRow ? 64 : 256;
     }
     if (!sys) {
         rs->gcol.resize((size_t)rs->goff[n]);
@@ -645,12 +544,10 @@ static spac_status rows_enqueue_bg(Problem* p, const RowsSetup& rs) {
     ROWS_HIP_CHECK(hipGetLastError());
     if (rs.ra.rbf_kind == SPAC_RBF_GAUSS_STENCIL) {
         const size_t lds = stencil_lds_bytes(rs);
-        if (rs.pair_rows)
-            hipLaunchKernelGGL(fk_rows_g_stencil_kernel<true>, dim3((n + 1) / 2), dim3(kRowsBlock), lds, s, rs.ra,
-                               rs.wstride, rs.mmax);
+        if (rs.g_block == 64)
+            hipLaunchKernelGGL(fk_rows_g_moment_kernel<64>, dim3(n), dim3(64), lds, s, rs.ra);
         else
-            hipLaunchKernelGGL(fk_rows_g_stencil_kernel<false>, dim3(n), dim3(kRowsBlock), lds, s, rs.ra, rs.wstride,
-                               rs.mmax);
+            hipLaunchKernelGGL(fk_rows_g_moment_kernel<256>, dim3(n), dim3(256), lds, s, rs.ra);
     } else {
         hipLaunchKernelGGL(fk_rows_g_imq_kernel, dim3(n), dim3(kRowsBlock), 0, s, rs.ra);
     }

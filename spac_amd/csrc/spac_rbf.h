@@ -81,45 +81,94 @@ SPAC_HD int spac_arc_start(int k, int M, int S) {
     return s;
 }
 
-/* Arc stencil: start s and the S values phi_i for exit angle theta. */
-SPAC_HD int spac_arc_stencil(const double* th, int M, int S, double eps, double theta, double* phi) {
+/* Arc stencil start s and half-width L for exit angle theta; phi_i = spac_gauss_phi((theta -
+ * th[s + i]) / L, eps). */
+SPAC_HD int spac_arc_stencil_start(const double* th, int M, int S, double theta, double* L) {
     int k = spac_arc_nearest(th, M, theta);
     int s = spac_arc_start(k, M, S);
-    double L = (S > 1) ? (th[s + S - 1] - th[s]) / 2.0 : 1.0;
+    *L = (S > 1) ? (th[s + S - 1] - th[s]) / 2.0 : 1.0;
+    return s;
+}
+
+/* Arc stencil: start s and the S values phi_i for exit angle theta. */
+SPAC_HD int spac_arc_stencil(const double* th, int M, int S, double eps, double theta, double* phi) {
+    double L;
+    int s = spac_arc_stencil_start(th, M, S, theta, &L);
     for (int i = 0; i < S; ++i) phi[i] = spac_gauss_phi((theta - th[s + i]) / L, eps);
     return s;
 }
 
-/* Full-circle stencil (M equally spaced nodes, relative angle r in [0, 2pi)):
- * returns the index of the first stencil node (mod M). */
 /* phi_i = exp(-(a - i b)^2), i = 0..S-1, for equally spaced nodes: four exps
  * and the recurrence phi_{i+1} = phi_i (E F_i) with E = exp(2 a b) and
  * F_i = exp(-(2 i + 1) b^2) = F_{i-1} exp(-2 b^2).  Each product adds one
  * rounding, so phi_i carries at most ~2i ulp more than a direct exp. */
-SPAC_HD void spac_gauss_row(double a, double b, int S, double* phi) {
+typedef struct {
+    double p, E, F, G; /* p = the current phi_i */
+} spac_gauss_seq;
+SPAC_HD spac_gauss_seq spac_gauss_seq_start(double a, double b) {
+    spac_gauss_seq q;
     const double b2 = b * b;
-    double p = spac_exp(-(a * a));
-    const double E = spac_exp(2.0 * (a * b));
-    double F = spac_exp(-b2);
-    const double G = spac_exp(-2.0 * b2);
-    phi[0] = p;
+    q.p = spac_exp(-(a * a));
+    q.E = spac_exp(2.0 * (a * b));
+    q.F = spac_exp(-b2);
+    q.G = spac_exp(-2.0 * b2);
+    return q;
+}
+/* phi_i -> phi_{i+1} */
+SPAC_HD void spac_gauss_seq_next(spac_gauss_seq* q) {
+    q->p = q->p * (q->E * q->F);
+    q->F = q->F * q->G;
+}
+SPAC_HD void spac_gauss_row(double a, double b, int S, double* phi) {
+    spac_gauss_seq q = spac_gauss_seq_start(a, b);
+    phi[0] = q.p;
     for (int i = 1; i < S; ++i) {
-        p = p * (E * F);
-        phi[i] = p;
-        F = F * G;
+        spac_gauss_seq_next(&q);
+        phi[i] = q.p;
     }
 }
 
-SPAC_HD int spac_circle_stencil(int M, int S, double eps, double r, double* phi) {
+/* Fixed-point moments of Gaussian-stencil rows.  A row's G is linear in its paths:
+ *   G_{s+r} = -(1/N) sum_p Y_p H_{s_p+r}(theta_p),  H_{s+r} = sum_i iPsi_s(i, r) phi_i,
+ * so it is also -(1/N) sum_{s,r} sum_i iPsi_s(i, r) Phi[s][i] with the moments
+ *   Phi[s][i] = sum over the row's circle-exit paths with stencil start s of Y_p phi_i(theta_p).
+ * The moments are kept as 64-bit integers in units of 2^-q: each term Y_p phi_i is rounded
+ * once to the nearest multiple, and integer sums are exact, so a moment does not depend on
+ * the order its paths are added in.  q is the largest exponent with N max|Y| 2^q < 2^62, so
+ * no sum of N terms (phi <= 1) overflows; a term's rounding is at most 2^-63 N max|Y|. */
+SPAC_HD int spac_moment_scale(double n_paths, double ymax) {
+    const double b = n_paths * ymax;
+    if (!(b > 0.0) || !(b < 1e300)) return 0;
+    const int e = (int)((spac_f64_as_u64(b) >> 52) & 0x7ff) - 1023; /* b < 2^(e + 1) */
+    int q = 61 - e;
+    if (q > 1000) q = 1000;
+    if (q < -1000) q = -1000;
+    return q;
+}
+SPAC_HD int64_t spac_to_fixed(double v, int q) { return (int64_t)SPAC_RINT(v * spac_pow2i(q)); }
+SPAC_HD double spac_from_fixed(int64_t n, int q) { return (double)n * spac_pow2i(-q); }
+
+/* Full-circle stencil start s (mod M) and the (a, b) of its Gaussians phi_i = exp(-(a - i b)^2). */
+SPAC_HD int spac_circle_stencil_ab(int M, int S, double eps, double r, double* a, double* b) {
     double delta = SPAC_TWO_PI / (double)M;
     int m = S / 2;
     int k = (int)SPAC_RINT(r / delta);
     double L = (m > 0) ? (double)m * delta : 1.0;
     /* d_i / L = (r - (k - m) delta) / L - i delta / L */
     double d0 = r - (double)(k - m) * delta;
-    spac_gauss_row(eps * (d0 / L), eps * (delta / L), S, phi);
+    *a = eps * (d0 / L);
+    *b = eps * (delta / L);
     int s = (k - m) % M;
     if (s < 0) s += M;
+    return s;
+}
+
+/* Full-circle stencil (M equally spaced nodes, relative angle r in [0, 2pi)): returns the
+ * index of the first stencil node (mod M) and fills the S values phi_i. */
+SPAC_HD int spac_circle_stencil(int M, int S, double eps, double r, double* phi) {
+    double a, b;
+    int s = spac_circle_stencil_ab(M, S, eps, r, &a, &b);
+    spac_gauss_row(a, b, S, phi);
     return s;
 }
 

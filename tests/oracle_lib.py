@@ -182,6 +182,27 @@ def row_weights(mesh, f: int, theta: float) -> np.ndarray:
     return out
 
 
+def moment_scale(n_paths: float, ymax: float) -> int:
+    lib = load()
+    lib.oracle_moment_scale.argtypes = [C.c_double, C.c_double]
+    lib.oracle_moment_scale.restype = C.c_int
+    return lib.oracle_moment_scale(n_paths, ymax)
+
+
+def to_fixed(v: float, q: int) -> int:
+    lib = load()
+    lib.oracle_to_fixed.argtypes = [C.c_double, C.c_int]
+    lib.oracle_to_fixed.restype = C.c_int64
+    return lib.oracle_to_fixed(v, q)
+
+
+def from_fixed(n: int, q: int) -> float:
+    lib = load()
+    lib.oracle_from_fixed.argtypes = [C.c_int64, C.c_int]
+    lib.oracle_from_fixed.restype = C.c_double
+    return lib.oracle_from_fixed(n, q)
+
+
 def exit_angle(code: int, dx: float, dy: float, first: float) -> float:
     lib = load()
     lib.oracle_exit_angle.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]

@@ -68,6 +68,41 @@ def test_imq_moment_form_equals_direct_form():
     assert np.abs(G1 - G2).max() <= 1e-6 * max(1.0, np.abs(G2).max())
 
 
+def test_stencil_moment_form_equals_direct_form():
+    """Gaussian stencils: the fixed-point moment form (one stencil inverse per row) against the
+    per-path direct form G_j += Y H_j(theta) (Mesh/Interfaz.cpp:253-261, sixteen TwoSum partials),
+    on full circles and arcs, FP64 paths of the variable-coefficient operator.  Both compute
+    the same linear map; they differ by the reassociation of sum_i iPsi(i, r) phi_i
+    (|iPsi| up to ~4e3 here) and the 2^-63 N max|Y| rounding of each fixed-point term."""
+    m = Mesh(mesh_desc())
+    a = m.iface_arrays()
+    jobs = np.arange(0, m.info.n_jobs, 7)
+    assert (a["code"][m.job_iface[jobs]] == 1).any() and (a["code"][m.job_iface[jobs]] != 1).any()
+    desc = problem_desc(coef=VC, domain=(0.0, 0.0, 1.0, 1.0))
+    B1, G1, _ = o.assemble_rows(desc, m, 3000, 5e-4, seed=4, jobs=jobs)
+    B2, G2, _ = o.assemble_rows(desc, m, 3000, 5e-4, seed=4, jobs=jobs, direct=True)
+    assert np.array_equal(B1, B2)
+    M = a["node_offset"][1:] - a["node_offset"][:-1]
+    S = np.minimum(15, M)
+    n_ipsi = int((a["ipsi_offset"] + np.where(a["code"] == 1, 1, M - S + 1) * S * S).max())
+    scale = np.abs(np.ctypeslib.as_array(m.table.ipsi, (n_ipsi,))).max()
+    err = np.abs(G1 - G2).max()
+    assert err <= 64 * scale * 2.2e-16, (err, scale)
+    assert np.abs(G2).max() > 1e-3  # rows are not trivially zero
+
+
+def test_fixed_point_moment_helpers():
+    """spac_moment_scale keeps N max|Y| 2^q below 2^62 and is the largest such exponent;
+    to_fixed / from_fixed round a value to the nearest multiple of 2^-q."""
+    for n, ym in ((1, 1.0), (1000, 0.37), (100000, 1.0), (7, 3.1e5), (2 ** 20, 2.0 ** -30)):
+        q = o.moment_scale(float(n), ym)
+        assert n * ym * 2.0 ** q < 2.0 ** 62 <= n * ym * 2.0 ** (q + 1)
+        v = ym * 0.734
+        back = o.from_fixed(o.to_fixed(v, q), q)
+        assert abs(back - v) <= 2.0 ** (-q - 1)
+    assert o.moment_scale(100.0, 0.0) == 0
+
+
 def _rows_equal(gpu, B, G, st, jobs, mesh):
     assert np.array_equal(gpu["B"].view(np.uint64), B.view(np.uint64))
     bad = np.nonzero(gpu["G"].view(np.uint64) != G.view(np.uint64))[0]
@@ -100,16 +135,16 @@ def test_gpu_rows_bit_identical_stencil(gpu):
 
 @pytest.mark.gpu
 def test_gpu_rows_bit_identical_stencil_short_rows(gpu):
-    """Rows of at most 128 paths take the paired G kernel (two rows per block); 129 paths the
-    one-row kernel.  Both equal the oracle bit for bit, with boundary rows (no paths) among the
-    jobs and an odd job count (the last block's second half has no row)."""
-    say("rows parity: Gaussian stencils, rows of 100 / 128 / 129 paths")
+    """Batches whose rows have at most 1024 paths take the one-wave G kernel, longer ones the
+    four-wave kernel.  Both equal the oracle bit for bit, with boundary rows (no paths) among
+    the jobs, on a mesh of 111-node and one of 40-node circles."""
+    say("rows parity: Gaussian stencils, rows of 100 / 1024 / 1025 paths")
     desc = problem_desc(coef=VC, domain=(0.0, 0.0, 1.0, 1.0))
     for m, h, step in ((Mesh(mesh_desc()), 5e-4, 31), (Mesh(mesh_desc(n_side=16, nodes_per_circle=40)), 2e-4, 97)):
         jobs = np.concatenate([np.arange(0, m.info.n_jobs, step), np.nonzero(m.job_frontera)[0][:2]])
         if jobs.size % 2 == 0:
             jobs = jobs[:-1]
-        for paths in (100, 128, 129):
+        for paths in (100, 1024, 1025):
             B, G, st = o.assemble_rows(desc, m, paths, h, seed=13, jobs=jobs)
             with FKProblem(desc) as p:
                 rows = p.assemble_rows(m, paths, h, seed=13, jobs=jobs)
