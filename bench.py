@@ -469,6 +469,28 @@ def bench_ttr(args):
         r["path_steps"], r["fallback_rows"] = first["path_steps"], first["fallback_rows"]
     log(f"ttr learned: {Nl} paths per row, {np.mean([r['step_wall_ms'] for r in learned_runs]):.1f} ms end to end")
 
+    # ---- the learned control variate evaluated per step (SPAC_CV_NETWORK) against its table, same
+    # paths: variance ratios from the per-row estimators of one assembly each, and path-kernel times
+    def row_variances(prob, seed):
+        st = prob.assemble_system(mesh, system, Nl, h, seed=seed, stats=True)
+        t = prob.timing()
+        vp = float(np.sum(st["phiphi"] - st["phi"] ** 2))
+        vr = float(np.sum(st["phiphi_VR"] - st["phi_VR"] ** 2))
+        return vp, vr, t.path_kernel_ms
+
+    cvnet = {"paths_per_row": Nl, "seed": 95000}
+    with FKProblem(desc32) as net_prob:
+        net.attach(net_prob, fallback=True, network=True)
+        row_variances(net_prob, 94999)  # warm-up (code objects)
+        vp_t, vr_t, ms_t = row_variances(learned_prob, 95000)
+        vp_n, vr_n, ms_n = row_variances(net_prob, 95000)
+    cvnet.update({"variance_ratio_table": vp_t / vr_t, "variance_ratio_network": vp_n / vr_n,
+                  "path_kernel_ms_table": ms_t, "path_kernel_ms_network": ms_n,
+                  "note": "plain/control-variate variance summed over every row of one assembly at the learned "
+                          "count (same Philox paths); network = the MLP's gradient heads evaluated in FP32 at "
+                          "every step instead of read from the 256^2 table"})
+    log(f"ttr learned: table vs per-step network: {cvnet}")
+
     # ---- FP64 plain-MC reference run at the plain count (same seed as an FP32 run)
     with FKProblem(problem_desc(coef=VC, domain=UNIT_SQUARE)) as p64:
         p64.set_control_variate(abi.SPAC_CV_NONE)
@@ -548,6 +570,7 @@ def bench_ttr(args):
         S["end_to_end_ms_at_target"] = at_target(S, S["rms_last_set"])
         del S["_scaling_ms"], S["_fixed_ms"]
     res.update({"plain": P, "learned": L, "speedup": speed, "speedup_ci95": ci, "fp32_vs_fp64": fp,
+                "cv_table_vs_network": cvnet,
                 "row_variance_ratio_note": "plain/learned variance at equal paths = "
                                            f"{(P['rms_last_set'] ** 2 * P['paths_per_row']) / (L['rms_last_set'] ** 2 * L['paths_per_row']):.1f}"})
 
@@ -690,6 +713,7 @@ def main():
                          "note": "replica-mean solution against the exact Franke u: the bias of the time step and "
                                  "the RBF interpolation, shared by both methods and outside the statistical target"},
         "fp32_vs_fp64": res["fp32_vs_fp64"],
+        "cv_table_vs_network": res["cv_table_vs_network"],
         "roofline": roof,
         "time_to_rms": res,
     }

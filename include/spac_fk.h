@@ -423,7 +423,11 @@ spac_status spac_fk_set_coef_tables(spac_problem* p, const spac_coef_tables* tab
 enum {
     SPAC_CV_ANALYTIC = 0, /* the coefficient set's F plugin (reference behaviour; default) */
     SPAC_CV_NONE = 1,     /* F = 0: plain Monte Carlo, xi = 0 */
-    SPAC_CV_TABLE = 2     /* F = -sigma(X) grad u_hat(X), grad u_hat bilinear in a T x T table */
+    SPAC_CV_TABLE = 2,    /* F = -sigma(X) grad u_hat(X), grad u_hat bilinear in a T x T table */
+    SPAC_CV_NETWORK = 3   /* F = -sigma(X) grad u_hat(X), grad u_hat the learned network's gradient
+                             heads evaluated at X on every step (FP32 stepping, variable-coefficient
+                             set; installed by spac_mlp_attach_network only).  The per-step form of
+                             the learned control variate, to measure what the table changes. */
 };
 
 /* A tabulated u_hat (the device form of a TablaDeValores + Interpolador,

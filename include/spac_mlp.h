@@ -80,6 +80,15 @@ spac_status spac_mlp_read_table(spac_mlp* m, float* out4 /* t*t*4 */);
  * table is copied, m may be destroyed afterwards). */
 spac_status spac_mlp_attach(spac_mlp* m, spac_problem* p, int32_t fallback, double max_ratio);
 
+/* Attach the network itself (mode SPAC_CV_NETWORK): F = -sigma grad u_hat with the gradient heads
+ * evaluated in FP32 at every step of every path, instead of read from the table.  The per-step
+ * form of the learned control variate (Solvers/FeynmanKacSolver.cpp:13 with a learned F), kept
+ * to measure what the table changes: variance and throughput against spac_mlp_attach.  Needs a
+ * table (spac_mlp_build_table: the exit scores and the fallback read it), an FP32 problem of
+ * the compiled variable-coefficient set, absorbing sides and no coefficient tables.  The
+ * parameters and table are copied; m may be destroyed afterwards. */
+spac_status spac_mlp_attach_network(spac_mlp* m, spac_problem* p, int32_t fallback, double max_ratio);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,7 @@ SPAC_ORDER_COST = 1
 SPAC_CV_ANALYTIC = 0
 SPAC_CV_NONE = 1
 SPAC_CV_TABLE = 2
+SPAC_CV_NETWORK = 3
 SPAC_TAB_SIGMA = 1
 SPAC_TAB_DRIFT = 2
 SPAC_TAB_KILLING = 4
@@ -337,6 +338,7 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.spac_mlp_build_table.argtypes = [C.c_void_p, C.c_int32, dp]
     lib.spac_mlp_read_table.argtypes = [C.c_void_p, fp]
     lib.spac_mlp_attach.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double]
+    lib.spac_mlp_attach_network.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double]
     for header in sorted(HEADER.parent.glob("*.h")):
         for name in declared_symbols(header):
             getattr(lib, name).restype = C.c_int32 if name != "spac_last_error" else C.c_char_p

@@ -439,9 +439,12 @@ __device__ __forceinline__ float rect_distance32(const float* p, float x, float 
 }
 
 /* Tabulated control variate in FP32: bilinear d u_hat/dx, d u_hat/dy at (x, y),
- * the same cell rule as spac_cv_cell (clamped to the grid, NaN to the origin). */
+ * the same cell rule as spac_cv_cell (clamped to the grid, NaN to the origin).  The
+ * gradients are stored by cell: cell (i, j), i, j < T - 1, holds its four corners
+ * (x, y gradient at (i, j), (i + 1, j), (i, j + 1), (i + 1, j + 1)) in 32 bytes, so a lookup
+ * is two 16-byte loads from one cache line. */
 struct CvTable32 {
-    const float2* grad; /* (du/dx, du/dy) of the T x T table */
+    const float4* cells; /* (T - 1)^2 cells x 2 float4 */
     int32_t t;
     float lox, loy, ix, iy, top;
 };
@@ -451,10 +454,58 @@ __device__ __forceinline__ void cv_grad32(const CvTable32& c, float x, float y, 
     fy = __builtin_fminf(__builtin_fmaxf(fy, 0.0f), c.top);
     int32_t i = min((int32_t)fx, c.t - 2), j = min((int32_t)fy, c.t - 2);
     float ax = fx - (float)i, ay = fy - (float)j, bx = 1.0f - ax, by = 1.0f - ay;
-    const float2* p = c.grad + (j * c.t + i);
-    const float2 v00 = p[0], v10 = p[1], v01 = p[c.t], v11 = p[c.t + 1];
-    *gx = by * (bx * v00.x + ax * v10.x) + ay * (bx * v01.x + ax * v11.x);
-    *gy = by * (bx * v00.y + ax * v10.y) + ay * (bx * v01.y + ax * v11.y);
+    const float4* p = c.cells + 2 * (j * (c.t - 1) + i);
+    const float4 lo = p[0], hi = p[1]; /* lo: corners (i, j), (i + 1, j); hi: (i, j + 1), (i + 1, j + 1) */
+    *gx = by * (bx * lo.x + ax * lo.z) + ay * (bx * hi.x + ax * hi.z);
+    *gy = by * (bx * lo.y + ax * lo.w) + ay * (bx * hi.y + ax * hi.w);
+}
+
+/* The learned control variate evaluated directly (SPAC_CV_NETWORK): the MLP 2 -> 64 -> 64 ->
+ * 64 -> 3 of spac_mlp.h (tanh; parameters [W1 64x2, b1, W2 64x64, b2, W3 64x64, b3, W4 3x64,
+ * b4], W row-major [out][in]; input z = 2 (x - lo) s - 1 per coordinate), in FP32 on one lane,
+ * returning its gradient heads (outputs 1 and 2).  Layer 3 is accumulated while layer 2 is
+ * produced, so one lane holds 128 activations.  The parameters are the same for every lane,
+ * so they are read through the scalar cache. */
+constexpr int kNetH = 64;
+constexpr int kNetW1 = 0, kNetB1 = kNetW1 + 2 * kNetH, kNetW2 = kNetB1 + kNetH, kNetB2 = kNetW2 + kNetH * kNetH,
+              kNetW3 = kNetB2 + kNetH, kNetB3 = kNetW3 + kNetH * kNetH, kNetW4 = kNetB3 + kNetH,
+              kNetB4 = kNetW4 + 3 * kNetH, kNetParams = kNetB4 + 3;
+static_assert(kNetParams == 8707, "network parameter layout of spac_mlp.h");
+struct CvNet32 {
+    const float* P;
+    float lox, loy, sx, sy;
+};
+__device__ __forceinline__ float net_tanh(float x) {
+    /* tanh x = 1 - 2 / (e^{2x} + 1), the hardware 2^x and reciprocal (the inference kernel's form) */
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+__device__ __forceinline__ void net_grad32(const CvNet32& n, float x, float y, float* gx, float* gy) {
+    const float* __restrict__ P = n.P;
+    const float z0 = 2.0f * (x - n.lox) * n.sx - 1.0f, z1 = 2.0f * (y - n.loy) * n.sy - 1.0f;
+    float h1[kNetH], h3[kNetH];
+#pragma unroll
+    for (int f = 0; f < kNetH; ++f) {
+        h1[f] = net_tanh(__builtin_fmaf(P[kNetW1 + 2 * f], z0, __builtin_fmaf(P[kNetW1 + 2 * f + 1], z1, P[kNetB1 + f])));
+        h3[f] = P[kNetB3 + f];
+    }
+    for (int j = 0; j < kNetH; ++j) {
+        float a = P[kNetB2 + j];
+#pragma unroll
+        for (int k = 0; k < kNetH; ++k) a = __builtin_fmaf(P[kNetW2 + j * kNetH + k], h1[k], a);
+        const float h2 = net_tanh(a);
+#pragma unroll
+        for (int m = 0; m < kNetH; ++m) h3[m] = __builtin_fmaf(P[kNetW3 + m * kNetH + j], h2, h3[m]);
+    }
+    float ox = P[kNetB4 + 1], oy = P[kNetB4 + 2];
+#pragma unroll
+    for (int m = 0; m < kNetH; ++m) {
+        const float t = net_tanh(h3[m]);
+        ox = __builtin_fmaf(P[kNetW4 + kNetH + m], t, ox);
+        oy = __builtin_fmaf(P[kNetW4 + 2 * kNetH + m], t, oy);
+    }
+    *gx = ox;
+    *gy = oy;
 }
 
 #pragma clang fp contract(off)

@@ -81,7 +81,9 @@ struct Problem {
     DeviceBuffer ifc, ifnode, ifipsi, ifoff, rows_iface, rows_goff, rows_G, rows_part, rows_col;
     /* control variate (SPAC_CV_*) and its table (T x T x float4) */
     DeviceBuffer cv_table;
-    DeviceBuffer cv_grad; /* float2 [T * T]: the table's (du/dx, du/dy), read by the FP32 path kernel */
+    DeviceBuffer cv_net;  /* SPAC_CV_NETWORK: the network's parameters (FP32) */
+    float cv_net_map[4] = {0.0f, 0.0f, 1.0f, 1.0f}; /* its input map: lox, loy, sx, sy */
+    DeviceBuffer cv_grad; /* float4 [2 (T - 1)^2]: the table's (du/dx, du/dy) by cell, read by the FP32 path kernel */
     int32_t cv_mode = SPAC_CV_ANALYTIC;
     int32_t cv_fallback = 0;
     double cv_max_ratio = 1.0;
@@ -156,7 +158,7 @@ struct Problem {
                                &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
                                &ssq, &ssig, &ssigf,
                                &queue, &stats, &ifc, &ifnode, &ifipsi, &ifoff, &rows_iface, &rows_goff,
-                               &rows_G, &rows_part, &rows_col, &cv_table, &cv_grad, &seed_dev, &tab, &tab32};
+                               &rows_G, &rows_part, &rows_col, &cv_table, &cv_grad, &cv_net, &seed_dev, &tab, &tab32};
         for (DeviceBuffer* b : all) b->release();
     }
 };
@@ -167,6 +169,8 @@ spac_status no_pending(const Problem* p);
 /* Install a control variate; the table (cfg->table ignored) is copied from
  * src, a host pointer or (src_on_device) a device pointer. */
 spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* src, bool src_on_device);
+spac_status problem_set_cv_network(Problem* p, const spac_cv_config* cfg, const void* table_dev,
+                                   const float* params_dev, const float* map);
 
 /* A mesh's interface system held on the device: the CSR pattern, the values right after
  * SistemaLineal::Inicializa_CSR (diagonal = containing circles, 1e-23 elsewhere), each job's

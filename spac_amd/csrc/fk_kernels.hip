@@ -43,7 +43,7 @@ namespace spac {
 
 /* ------------------------------------------------------ kernel args -- */
 enum { SUB_NONE = 0, SUB_CIRCLE = 2 };
-enum { CV_ANALYTIC = SPAC_CV_ANALYTIC, CV_NONE = SPAC_CV_NONE, CV_TABLE = SPAC_CV_TABLE };
+enum { CV_ANALYTIC = SPAC_CV_ANALYTIC, CV_NONE = SPAC_CV_NONE, CV_TABLE = SPAC_CV_TABLE, CV_NETWORK = SPAC_CV_NETWORK };
 
 struct PathArgs {
     double dom[4];
@@ -88,6 +88,8 @@ struct PathArgs {
     /* tabulated control variate (CV_TABLE): FP64 and FP32 views of one table */
     spac_cv_table cv;
     CvTable32 cv32;
+    /* the network itself (CV_NETWORK, FP32 kernel; cv and cv32 hold its table for the exit scores) */
+    CvNet32 cvnet;
     /* tabulated coefficients (CoefTab kernels): FP64 table and its float copy */
     spac_coef_table tab;
     CoefTable32 tab32;
@@ -465,9 +467,10 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
             Z += (double)((cf.f * Ywf) * hf);
             if (CV == CV_ANALYTIC) {
                 xi += (double)(Ywf * (cf.Fx * dWx + cf.Fy * dWy));
-            } else if (CV == CV_TABLE) {
+            } else if (CV == CV_TABLE || CV == CV_NETWORK) {
                 float gx, gy;
-                cv_grad32(a.cv32, xf, yf, &gx, &gy);
+                if (CV == CV_NETWORK) net_grad32(a.cvnet, xf, yf, &gx, &gy);
+                else cv_grad32(a.cv32, xf, yf, &gx, &gy);
                 xi += (double)(Ywf * (-cf.s) * (gx * dWx + gy * dWy));
             }
             if (Coef32::kHasKilling) Yw += (double)(cf.c * hf) * Yw;
@@ -711,6 +714,12 @@ static void launch_paths_cv(int coef, int precision, int sub, bool refl, dim3 gr
 
 static void launch_paths(int coef, int precision, int sub, int cv, bool refl, dim3 grid, dim3 block, hipStream_t s,
                          const PathArgs& a) {
+    if (cv == CV_NETWORK) { /* FP32, the compiled variable-coefficient set (checked at installation) */
+        if (sub == SUB_CIRCLE)
+            hipLaunchKernelGGL((fk_path_kernel_fp32<CoefVarFranke32, SUB_CIRCLE, CV_NETWORK>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fk_path_kernel_fp32<CoefVarFranke32, SUB_NONE, CV_NETWORK>), grid, block, 0, s, a);
+        return;
+    }
     if (cv == CV_NONE) launch_paths_cv<CV_NONE>(coef, precision, sub, refl, grid, block, s, a);
     else if (cv == CV_TABLE) launch_paths_cv<CV_TABLE>(coef, precision, sub, refl, grid, block, s, a);
     else launch_paths_cv<CV_ANALYTIC>(coef, precision, sub, refl, grid, block, s, a);
@@ -959,20 +968,27 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
     a.Y = rs.Y; a.Z = rs.Z; a.xi = rs.xi;
     a.tau = rs.tau; a.gval = rs.gval;
     a.steps = rs.steps; a.outcome = rs.outcome;
-    if (cv_mode == SPAC_CV_TABLE) {
+    if (cv_mode == SPAC_CV_TABLE || cv_mode == SPAC_CV_NETWORK) {
         a.cv.table = (const float*)cv_table.ptr;
         a.cv.t = cv_t;
         a.cv.lox = cv_dom[0];
         a.cv.loy = cv_dom[1];
         a.cv.ix = (double)(cv_t - 1) / (cv_dom[2] - cv_dom[0]);
         a.cv.iy = (double)(cv_t - 1) / (cv_dom[3] - cv_dom[1]);
-        a.cv32.grad = (const float2*)cv_grad.ptr;
+        a.cv32.cells = (const float4*)cv_grad.ptr;
         a.cv32.t = cv_t;
         a.cv32.lox = (float)a.cv.lox;
         a.cv32.loy = (float)a.cv.loy;
         a.cv32.ix = (float)a.cv.ix;
         a.cv32.iy = (float)a.cv.iy;
         a.cv32.top = (float)(cv_t - 1);
+    }
+    if (cv_mode == SPAC_CV_NETWORK) {
+        a.cvnet.P = (const float*)cv_net.ptr;
+        a.cvnet.lox = cv_net_map[0];
+        a.cvnet.loy = cv_net_map[1];
+        a.cvnet.sx = cv_net_map[2];
+        a.cvnet.sy = cv_net_map[3];
     }
     if (tab_mask) {
         a.tab.v = (const double*)tab.ptr;
@@ -1011,13 +1027,16 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
                                dim3(256), 0, stream, ss.n, ss.x0, ss.y0, ss.sig0, ss.sig0f, a.tab, a.tab32);
     }
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
+    if (cv_mode == SPAC_CV_NETWORK && (desc.precision != SPAC_FP32 || coef != SPAC_COEF_VARCOEF_FRANKE))
+        return set_error(SPAC_E_INVALID_ARG, "SPAC_CV_NETWORK needs FP32 stepping and the compiled variable-coefficient set");
     launch_paths(coef, desc.precision, sub, cv_mode, desc.reflecting != 0, dim3(blocks), dim3(block), stream, a);
     SPAC_HIP_CHECK(hipGetLastError());
     if (after_path) SPAC_HIP_CHECK(hipEventRecord(after_path, stream));
     if (ss.total_paths > 0) {
         const int64_t want = (ss.total_paths + 255) / 256;
         const int gblocks = (int)std::min<int64_t>(want, (int64_t)num_cus * 16);
-        if (cv_mode == SPAC_CV_TABLE) launch_exit_values<CV_TABLE>(coef, dim3(gblocks), stream, a, ss.total_paths);
+        if (cv_mode == SPAC_CV_TABLE || cv_mode == SPAC_CV_NETWORK)
+            launch_exit_values<CV_TABLE>(coef, dim3(gblocks), stream, a, ss.total_paths);
         else launch_exit_values<CV_ANALYTIC>(coef, dim3(gblocks), stream, a, ss.total_paths);
         SPAC_HIP_CHECK(hipGetLastError());
     }
@@ -1040,7 +1059,7 @@ spac_status Problem::enqueue_reduce(const SolveSet& ss, spac_point_stats* d_stat
     r.tau = rs.tau; r.gval = rs.gval;
     r.steps = rs.steps; r.outcome = rs.outcome;
     r.stats = d_stats;
-    r.cv_fallback = (cv_mode == SPAC_CV_TABLE) ? cv_fallback : 0;
+    r.cv_fallback = (cv_mode == SPAC_CV_TABLE || cv_mode == SPAC_CV_NETWORK) ? cv_fallback : 0;
     r.cv_max_ratio = cv_max_ratio;
     hipLaunchKernelGGL(fk_reduce_kernel, dim3(ss.n), dim3(64), 0, stream, r);
     SPAC_HIP_CHECK(hipGetLastError());
@@ -1084,14 +1103,22 @@ spac_status Problem::finish_run(int blocks) {
     return SPAC_OK;
 }
 
-/* (du/dx, du/dy) channels of a T x T float4 control-variate table */
-__global__ void k_cv_gradients(int n, const float4* __restrict__ tab, float2* __restrict__ grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) grad[i] = make_float2(tab[i].y, tab[i].z);
+/* The (du/dx, du/dy) channels of a T x T float4 control-variate table, by cell (the layout of
+ * CvTable32): cell (i, j) gets the corners (i, j), (i + 1, j), (i, j + 1), (i + 1, j + 1). */
+__global__ void k_cv_gradient_cells(int t, const float4* __restrict__ tab, float4* __restrict__ cells) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = (t - 1) * (t - 1);
+    if (k >= n) return;
+    const int i = k % (t - 1), j = k / (t - 1);
+    const float4 a = tab[j * t + i], b = tab[j * t + i + 1], c = tab[(j + 1) * t + i], d = tab[(j + 1) * t + i + 1];
+    cells[2 * k] = make_float4(a.y, a.z, b.y, b.z);
+    cells[2 * k + 1] = make_float4(c.y, c.z, d.y, d.z);
 }
 
 spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* src, bool src_on_device) {
     if (spac_status rc = no_pending(p)) return rc;
+    if (cfg->mode == SPAC_CV_NETWORK)
+        return set_error(SPAC_E_INVALID_ARG, "SPAC_CV_NETWORK is installed by spac_mlp_attach_network");
     if (cfg->mode != SPAC_CV_ANALYTIC && cfg->mode != SPAC_CV_NONE && cfg->mode != SPAC_CV_TABLE)
         return set_error(SPAC_E_INVALID_ARG, "unknown control-variate mode %d", cfg->mode);
     if (cfg->fallback && !(cfg->max_ratio > 0.0))
@@ -1111,11 +1138,11 @@ spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* sr
         if ((rc = p->cv_table.ensure(bytes))) return rc;
         SPAC_HIP_CHECK(hipMemcpyAsync(p->cv_table.ptr, src, bytes,
                                       src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
-        /* the FP32 kernel reads only the two gradient channels: half the bytes per lookup */
-        const int tt = cfg->table_size * cfg->table_size;
-        if ((rc = p->cv_grad.ensure((size_t)tt * 8))) return rc;
-        hipLaunchKernelGGL(k_cv_gradients, dim3((tt + 255) / 256), dim3(256), 0, p->stream, tt,
-                           (const float4*)p->cv_table.ptr, (float2*)p->cv_grad.ptr);
+        /* the FP32 kernel reads only the two gradient channels, by cell: one line per lookup */
+        const int nc = (cfg->table_size - 1) * (cfg->table_size - 1);
+        if ((rc = p->cv_grad.ensure((size_t)nc * 32))) return rc;
+        hipLaunchKernelGGL(k_cv_gradient_cells, dim3((nc + 255) / 256), dim3(256), 0, p->stream, cfg->table_size,
+                           (const float4*)p->cv_table.ptr, (float4*)p->cv_grad.ptr);
         SPAC_HIP_CHECK(hipGetLastError());
         SPAC_HIP_CHECK(hipStreamSynchronize(p->stream));
         p->cv_t = cfg->table_size;
@@ -1124,6 +1151,29 @@ spac_status problem_set_cv(Problem* p, const spac_cv_config* cfg, const void* sr
     p->cv_mode = cfg->mode;
     p->cv_fallback = cfg->fallback ? 1 : 0;
     p->cv_max_ratio = cfg->max_ratio > 0.0 ? cfg->max_ratio : 1.0;
+    return SPAC_OK;
+}
+
+/* SPAC_CV_NETWORK: the network's table as in SPAC_CV_TABLE (the exit scores and the fallback
+ * read it), plus a device copy of the network's parameters and its input map (lox, loy, sx, sy)
+ * for the per-step evaluation in the FP32 kernel. */
+spac_status problem_set_cv_network(Problem* p, const spac_cv_config* cfg, const void* table_dev,
+                                   const float* params_dev, const float* map) {
+    if (spac_status rc = no_pending(p)) return rc;
+    if (p->desc.precision != SPAC_FP32)
+        return set_error(SPAC_E_INVALID_ARG, "SPAC_CV_NETWORK runs in the FP32 path kernel: the problem must be SPAC_FP32");
+    if (p->desc.coef_kind != SPAC_COEF_VARCOEF_FRANKE || p->tab_mask || p->desc.reflecting)
+        return set_error(SPAC_E_INVALID_ARG, "SPAC_CV_NETWORK needs the compiled variable-coefficient set, absorbing "
+                                             "sides and no coefficient tables");
+    spac_cv_config t = *cfg;
+    t.mode = SPAC_CV_TABLE;
+    spac_status rc = problem_set_cv(p, &t, table_dev, true);
+    if (rc) return rc;
+    if ((rc = p->cv_net.ensure((size_t)kNetParams * 4))) return rc;
+    SPAC_HIP_CHECK(hipMemcpyAsync(p->cv_net.ptr, params_dev, (size_t)kNetParams * 4, hipMemcpyDeviceToDevice, p->stream));
+    SPAC_HIP_CHECK(hipStreamSynchronize(p->stream));
+    for (int i = 0; i < 4; ++i) p->cv_net_map[i] = map[i];
+    p->cv_mode = SPAC_CV_NETWORK;
     return SPAC_OK;
 }
 

@@ -788,7 +788,7 @@ extern "C" spac_status spac_fk_plan_create(spac_problem* hp, const spac_interfac
     plan->p = reinterpret_cast<Problem*>(hclone);
     Problem* p = plan->p;
     p->launch = src->launch;
-    if (src->cv_mode == SPAC_CV_TABLE) {
+    if (src->cv_mode == SPAC_CV_TABLE || src->cv_mode == SPAC_CV_NETWORK) {
         spac_cv_config cfg;
         std::memset(&cfg, 0, sizeof cfg);
         cfg.mode = SPAC_CV_TABLE;
@@ -796,7 +796,9 @@ extern "C" spac_status spac_fk_plan_create(spac_problem* hp, const spac_interfac
         cfg.max_ratio = src->cv_max_ratio;
         cfg.table_size = src->cv_t;
         for (int i = 0; i < 4; ++i) cfg.table_domain[i] = src->cv_dom[i];
-        rc = problem_set_cv(p, &cfg, src->cv_table.ptr, true);
+        rc = src->cv_mode == SPAC_CV_NETWORK
+                 ? problem_set_cv_network(p, &cfg, src->cv_table.ptr, (const float*)src->cv_net.ptr, src->cv_net_map)
+                 : problem_set_cv(p, &cfg, src->cv_table.ptr, true);
     } else {
         p->cv_mode = src->cv_mode;
         p->cv_fallback = src->cv_fallback;

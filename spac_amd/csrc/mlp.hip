@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/spac_mlp.h"
+#include "fk_device.h"
 #include "fk_internal.h"
 
 namespace spac {
@@ -45,6 +46,8 @@ constexpr int kH = 64;
 constexpr int oW1 = 0, ob1 = oW1 + kH * 2, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, oW3 = ob2 + kH,
               ob3 = oW3 + kH * kH, oW4 = ob3 + kH, ob4 = oW4 + 3 * kH, kNParams = ob4 + 3;
 static_assert(kNParams == SPAC_MLP_N_PARAMS, "parameter count");
+static_assert(kNParams == kNetParams && oW2 == kNetW2 && ob3 == kNetB3 && oW4 == kNetW4 && ob4 == kNetB4,
+              "the per-step network evaluation (fk_device.h) reads this layout");
 /* row stride of the per-block gradient rows, padded to a multiple of 4 floats (16-byte rows;
  * the pad float is never written nor read) */
 constexpr int kPartStride = (kNParams + 3) & ~3;
@@ -1077,6 +1080,22 @@ spac_status spac_mlp_attach(spac_mlp* hm, spac_problem* hp, int32_t fallback, do
     cfg.table_size = m->table_t;
     for (int i = 0; i < 4; ++i) cfg.table_domain[i] = m->desc.domain[i];
     return problem_set_cv(reinterpret_cast<Problem*>(hp), &cfg, m->table.ptr, true);
+}
+
+spac_status spac_mlp_attach_network(spac_mlp* hm, spac_problem* hp, int32_t fallback, double max_ratio) {
+    Mlp* m = reinterpret_cast<Mlp*>(hm);
+    if (!m || !hp) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    if (!m->table_t) return set_error(SPAC_E_INVALID_ARG, "the MLP has no table (spac_mlp_build_table)");
+    MLP_HIP_CHECK(hipStreamSynchronize(m->stream));
+    spac_cv_config cfg;
+    std::memset(&cfg, 0, sizeof cfg);
+    cfg.mode = SPAC_CV_TABLE;
+    cfg.fallback = fallback;
+    cfg.max_ratio = max_ratio;
+    cfg.table_size = m->table_t;
+    for (int i = 0; i < 4; ++i) cfg.table_domain[i] = m->desc.domain[i];
+    const float map[4] = {m->lox, m->loy, m->sx, m->sy};
+    return problem_set_cv_network(reinterpret_cast<Problem*>(hp), &cfg, m->table.ptr, (const float*)m->params.ptr, map);
 }
 
 } /* extern "C" */

@@ -134,18 +134,28 @@ SPAC_HD void spac_gauss_row(double a, double b, int S, double* phi) {
  *   Phi[s][i] = sum over the row's circle-exit paths with stencil start s of Y_p phi_i(theta_p).
  * The moments are kept as 64-bit integers in units of 2^-q: each term Y_p phi_i is rounded
  * once to the nearest multiple, and integer sums are exact, so a moment does not depend on
- * the order its paths are added in.  q is the largest exponent with N max|Y| 2^q < 2^62, so
- * no sum of N terms (phi <= 1) overflows; a term's rounding is at most 2^-63 N max|Y|. */
+ * the order its paths are added in.  q is the largest exponent with N max|Y| 2^q < 2^62 (no
+ * sum of N terms, phi <= 1, overflows) and max|Y| 2^q < 2^50 (every term is below 2^50, so
+ * adding 1.5 2^52 rounds it to the nearest integer, ties to even, and leaves that integer in
+ * the low bits: a conversion of four instructions).  A term's rounding is at most 2^-q-1,
+ * i.e. below 2^-63 N max|Y| and 2^-51 max|Y|. */
+SPAC_HD int spac_exponent_above(double b) { /* e with b < 2^(e + 1), b > 0 finite */
+    return (int)((spac_f64_as_u64(b) >> 52) & 0x7ff) - 1023;
+}
 SPAC_HD int spac_moment_scale(double n_paths, double ymax) {
     const double b = n_paths * ymax;
     if (!(b > 0.0) || !(b < 1e300)) return 0;
-    const int e = (int)((spac_f64_as_u64(b) >> 52) & 0x7ff) - 1023; /* b < 2^(e + 1) */
-    int q = 61 - e;
+    int q = 61 - spac_exponent_above(b);
+    const int qy = 49 - spac_exponent_above(ymax);
+    if (qy < q) q = qy;
     if (q > 1000) q = 1000;
     if (q < -1000) q = -1000;
     return q;
 }
-SPAC_HD int64_t spac_to_fixed(double v, int q) { return (int64_t)SPAC_RINT(v * spac_pow2i(q)); }
+#define SPAC_FIXED_MAGIC 6755399441055744.0 /* 1.5 2^52 */
+SPAC_HD int64_t spac_to_fixed(double v, int q) {
+    return (int64_t)(spac_f64_as_u64(v * spac_pow2i(q) + SPAC_FIXED_MAGIC) - spac_f64_as_u64(SPAC_FIXED_MAGIC));
+}
 SPAC_HD double spac_from_fixed(int64_t n, int q) { return (double)n * spac_pow2i(-q); }
 
 /* Full-circle stencil start s (mod M) and the (a, b) of its Gaussians phi_i = exp(-(a - i b)^2). */

@@ -105,5 +105,8 @@ class MLP:
         abi.check(self.lib.spac_mlp_read_table(self.h, abi.ptr(out, C.c_float)), self.lib)
         return out
 
-    def attach(self, problem, fallback=True, max_ratio=1.0):
-        abi.check(self.lib.spac_mlp_attach(self.h, problem.h, 1 if fallback else 0, max_ratio), self.lib)
+    def attach(self, problem, fallback=True, max_ratio=1.0, network=False):
+        """spac_mlp_attach (the table, SPAC_CV_TABLE) or, with network=True,
+        spac_mlp_attach_network (the network evaluated at every step, SPAC_CV_NETWORK)."""
+        fn = self.lib.spac_mlp_attach_network if network else self.lib.spac_mlp_attach
+        abi.check(fn(self.h, problem.h, 1 if fallback else 0, max_ratio), self.lib)

@@ -178,3 +178,65 @@ def test_learned_cv_gain(gpu):
     Bl = np.array([s["B"] for s in res["learned"]])
     se = np.sqrt(var_plain / 4000)
     assert np.all(np.abs(Bl - Bp) < 5 * se + 1e-3)
+
+
+def test_network_cv_per_step_against_table(gpu):
+    """SPAC_CV_NETWORK (the trained network's gradient heads evaluated at every step) against
+    its table (SPAC_CV_TABLE) on the same Philox paths: the plain score phi and the path-step
+    counts are identical (the control variate does not move a path), xi is a martingale (mean
+    0 within 4 standard errors), and the variances of phi_VR agree within 10 %: the table's
+    interpolation changes the control variate by less than the network's own error.  A plan of
+    a network-mode problem replays the eager assembly bit for bit; FP64 problems and the
+    generic entry point refuse the mode."""
+    from spac_amd.mlp import MLP
+    from spac_amd.workloads import LEARNED_CV, train_learned_cv
+    from spac_amd.fk import RowsPlan
+    from spac_amd.mesh import Mesh, mesh_desc
+
+    say("network CV: train")
+    cfg = {k: v for k, v in LEARNED_CV.items() if k != "seed"}
+    desc32 = problem_desc(coef=VC, domain=DOM, precision=abi.SPAC_FP32)
+    rng = np.random.default_rng(8)
+    n = 48
+    cx, cy = rng.uniform(0.25, 0.75, n), rng.uniform(0.25, 0.75, n)
+    r = rng.uniform(0.0, 0.05, n)
+    a = rng.uniform(0, 2 * np.pi, n)
+    b = SolveBatchArrays(cx + r * np.cos(a), cy + r * np.sin(a), 2000, 1e-4, np.arange(n),
+                         circle=np.stack([np.full(n, 0.08), cx, cy], 1))
+    res = {}
+    with MLP(DOM, seed=0) as m, FKProblem(desc32) as dp:
+        train_learned_cv(m, **cfg, eager=False, data_problem=dp)
+        for mode in ("table", "network"):
+            with FKProblem(desc32) as p:
+                m.attach(p, fallback=False, network=(mode == "network"))
+                res[mode] = p.simulate(b, seed=4, want_records=True)
+        with FKProblem(problem_desc(coef=VC, domain=DOM)) as p64:
+            with pytest.raises(abi.SpacError) as e:
+                m.attach(p64, network=True)
+            assert e.value.code == abi.SPAC_E_INVALID_ARG
+        mesh = Mesh(mesh_desc())
+        jobs = np.arange(0, mesh.info.n_jobs, 41)
+        with FKProblem(desc32) as p:
+            m.attach(p, fallback=True, network=True)
+            rows = p.assemble_rows(mesh, 300, 5e-4, seed=9, jobs=jobs)
+            with RowsPlan(p, mesh, 300, 5e-4, jobs=jobs) as plan:
+                again = plan.execute(seed=9)
+        assert np.array_equal(rows["B"].view(np.uint64), again["B"].view(np.uint64))
+        assert np.array_equal(rows["G"].view(np.uint64), again["G"].view(np.uint64))
+    with FKProblem(desc32) as p:
+        with pytest.raises(abi.SpacError):
+            p.set_control_variate(abi.SPAC_CV_NETWORK, np.zeros((4, 4, 4), np.float32), DOM)
+    (rt, st), (rn, sn) = res["table"], res["network"]
+    assert np.array_equal(np.asarray(rt["steps"]), np.asarray(rn["steps"]))
+    vt = np.array([s["phiphi_VR"] - s["phi_VR"] ** 2 for s in st])
+    vn = np.array([s["phiphi_VR"] - s["phi_VR"] ** 2 for s in sn])
+    vp = np.array([s["phiphi"] - s["phi"] ** 2 for s in sn])
+    for s1, s2 in zip(st, sn):
+        assert s1["phi"] == s2["phi"]
+    xi = np.asarray(rn["xi"])
+    se = xi.std() / np.sqrt(xi.size)
+    say(f"variance plain/table {vp.sum() / vt.sum():.1f}, plain/network {vp.sum() / vn.sum():.1f}; "
+        f"mean xi {xi.mean():.2e} (SE {se:.1e})")
+    assert abs(xi.mean()) < 4 * se
+    assert abs(vn.sum() / vt.sum() - 1.0) < 0.1
+    assert vp.sum() > 5 * vn.sum()

@@ -92,14 +92,19 @@ def test_stencil_moment_form_equals_direct_form():
 
 
 def test_fixed_point_moment_helpers():
-    """spac_moment_scale keeps N max|Y| 2^q below 2^62 and is the largest such exponent;
-    to_fixed / from_fixed round a value to the nearest multiple of 2^-q."""
-    for n, ym in ((1, 1.0), (1000, 0.37), (100000, 1.0), (7, 3.1e5), (2 ** 20, 2.0 ** -30)):
+    """spac_moment_scale keeps N max|Y| 2^q below 2^62 and max|Y| 2^q below 2^50, and is the
+    largest such exponent; to_fixed rounds to the nearest multiple of 2^-q with ties to even
+    (numpy's rint), from_fixed maps back exactly."""
+    for n, ym in ((1, 1.0), (1000, 0.37), (100000, 1.0), (7, 3.1e5), (2 ** 20, 2.0 ** -30), (2 ** 13, 1.0)):
         q = o.moment_scale(float(n), ym)
-        assert n * ym * 2.0 ** q < 2.0 ** 62 <= n * ym * 2.0 ** (q + 1)
-        v = ym * 0.734
-        back = o.from_fixed(o.to_fixed(v, q), q)
-        assert abs(back - v) <= 2.0 ** (-q - 1)
+        assert n * ym * 2.0 ** q < 2.0 ** 62 and ym * 2.0 ** q < 2.0 ** 50
+        assert n * ym * 2.0 ** (q + 1) >= 2.0 ** 62 or ym * 2.0 ** (q + 1) >= 2.0 ** 50
+        for v in (ym * 0.734, -ym * 0.9999, ym * 1e-9, 0.0):
+            t = o.to_fixed(v, q)
+            assert t == int(np.rint(v * 2.0 ** q))
+            assert o.from_fixed(t, q) == t * 2.0 ** (-q)
+        for half in (2.5, 3.5, -2.5):  # ties go to even
+            assert o.to_fixed(half * 2.0 ** (-q), q) == int(np.rint(half))
     assert o.moment_scale(100.0, 0.0) == 0
 
 
