@@ -118,6 +118,16 @@ struct CoefMaker<CoefTab32<B>> {
     }
 };
 
+/* Lanes of a wave whose exits (and refills) are handled together.  The FP32 kernel's exit
+ * branch projects the exit point in FP64 (tens of instructions under a divergent branch that
+ * would otherwise run on almost every wave-step), so it waits for a batch; the FP64 kernel's
+ * exit branch is only the record stores, and waiting would only idle lanes. */
+#ifndef SPAC_FP32_LANE_BATCH
+#define SPAC_FP32_LANE_BATCH 8
+#endif
+constexpr int kLaneBatch32 = SPAC_FP32_LANE_BATCH;
+constexpr int kLaneBatch64 = 1;
+
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
@@ -181,6 +191,9 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
      * step reflects, and local time carried from a projection */
     double rd = 0.0, rnx = 0.0, rny = 0.0, chi_p = 0.0;
     bool rrefl = false;
+    bool done = false;   /* stopped, its record not yet stored (batched exits) */
+    int8_t done_out = 0;
+    double done_ex = 0.0, done_ey = 0.0;
     uint32_t stream = 0;
     int64_t rec = 0;
     /* wave-uniform pool */
@@ -189,7 +202,27 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     bool drained = false;
 
     while (true) {
-        unsigned long long idle = __ballot(!active);
+        /* Exits and refills, in batches of kLaneBatch64 lanes (one: at once) or when no lane
+           steps any more; the structure is shared with the FP32 kernel. */
+        {
+            const unsigned long long stopped = __ballot(done);
+            if (stopped != 0ull && (__popcll(stopped) >= kLaneBatch64 || __ballot(active) == 0ull)) {
+                if (done) {
+                    int64_t g = rec + (int64_t)path;
+                    a.xt[g] = done_ex;
+                    a.yt[g] = done_ey;
+                    a.Y[g] = Yw;
+                    a.Z[g] = Z;
+                    a.xi[g] = xi;
+                    a.tau[g] = t;
+                    a.steps[g] = nstep; /* gval: fk_exit_values_kernel, lane-dense after the launch */
+                    a.outcome[g] = done_out;
+                    done = false;
+                }
+            }
+        }
+        unsigned long long idle = __ballot(!active && !done);
+        if (__popcll(idle) < kLaneBatch64 && __ballot(active) != 0ull) idle = 0ull; /* refill later, together */
         while (idle != 0ull && !drained) {
             if (pool_next >= pool_end) {
                 int32_t item = 0;
@@ -211,7 +244,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             uint32_t avail = pool_end - pool_next;
             uint32_t nidle = (uint32_t)__popcll(idle);
             uint32_t rank = lane_rank(idle);
-            if (!active && rank < avail) {
+            if (!active && !done && rank < avail) {
                 solve = pool_solve;
                 path = pool_next + rank;
                 h = a.h ? a.h[solve] : a.h_uniform;
@@ -246,9 +279,9 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             }
             uint32_t take = min(avail, nidle);
             pool_next += take;
-            idle = __ballot(!active);
+            idle = __ballot(!active && !done);
         }
-        if (__ballot(active) == 0ull) break;
+        if (__ballot(active) == 0ull && __ballot(done) == 0ull) break;
         bool reflect_step = false;
         if constexpr (REFL) reflect_step = rrefl;
         if (active && reflect_step) {
@@ -367,16 +400,11 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             } else {
                 if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
             }
-            if (out != SPAC_OUT_INSIDE) {
-                int64_t g = rec + (int64_t)path;
-                a.xt[g] = hit.ex;
-                a.yt[g] = hit.ey;
-                a.Y[g] = Yw;
-                a.Z[g] = Z;
-                a.xi[g] = xi;
-                a.tau[g] = t;
-                a.steps[g] = nstep; /* gval: fk_exit_values_kernel, lane-dense after the launch */
-                a.outcome[g] = out;
+            if (out != SPAC_OUT_INSIDE) { /* the record is stored with the wave's next batch of exits */
+                done = true;
+                done_out = out;
+                done_ex = hit.ex;
+                done_ey = hit.ey;
                 active = false;
             }
         }
@@ -393,6 +421,8 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t seed = a.seed_dev ? *a.seed_dev : a.seed;
     bool active = false;
+    bool done = false;   /* stopped, its record not yet stored (batched exits) */
+    int8_t done_out = 0; /* the stopped path's outcome */
     int32_t solve = 0;
     uint32_t path = 0;
     int32_t nstep = 0;
@@ -408,7 +438,33 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
     const float gobetf = (float)a.gobet;
 
     while (true) {
-        unsigned long long idle = __ballot(!active);
+        /* Exits and refills in batches: a stopped lane keeps its final state in registers until
+           kLaneBatch32 lanes of the wave have stopped (or no lane steps any more); then all of them
+           project and store their records in one pass of the divergent branch, and the idle
+           lanes take new paths together.  A path's record depends only on its own state, so
+           batching changes no result. */
+        {
+            const unsigned long long stopped = __ballot(done);
+            if (stopped != 0ull && (__popcll(stopped) >= kLaneBatch32 || __ballot(active) == 0ull)) {
+                if (done) {
+                    /* exit projection in FP64 onto the boundary the FP32 test chose */
+                    BoundaryHit hit = (done_out == SPAC_OUT_SUBDOMAIN) ? circle_distance(R, cx, cy, (double)xf, (double)yf)
+                                                                       : rectangle_distance(a.dom, (double)xf, (double)yf);
+                    int64_t g = rec + (int64_t)path;
+                    a.xt[g] = hit.ex;
+                    a.yt[g] = hit.ey;
+                    a.Y[g] = Yw;
+                    a.Z[g] = Z;
+                    a.xi[g] = xi;
+                    a.tau[g] = (double)nstep * h;
+                    a.steps[g] = nstep;
+                    a.outcome[g] = done_out;
+                    done = false;
+                }
+            }
+        }
+        unsigned long long idle = __ballot(!active && !done);
+        if (__popcll(idle) < kLaneBatch32 && __ballot(active) != 0ull) idle = 0ull; /* refill later, together */
         while (idle != 0ull && !drained) {
             if (pool_next >= pool_end) {
                 int32_t item = 0;
@@ -430,7 +486,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
             uint32_t avail = pool_end - pool_next;
             uint32_t nidle = (uint32_t)__popcll(idle);
             uint32_t rank = lane_rank(idle);
-            if (!active && rank < avail) {
+            if (!active && !done && rank < avail) {
                 solve = pool_solve;
                 path = pool_next + rank;
                 h = a.h ? a.h[solve] : a.h_uniform;
@@ -455,9 +511,9 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
             }
             uint32_t take = min(avail, nidle);
             pool_next += take;
-            idle = __ballot(!active);
+            idle = __ballot(!active && !done);
         }
-        if (__ballot(active) == 0ull) break;
+        if (__ballot(active) == 0ull && __ballot(done) == 0ull) break;
         if (active) {
             StepCoef32 cf = coef32.step_s(xf, yf, sigf);
             float n0, n1;
@@ -501,19 +557,9 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
             bool inside = d < gobetf * __builtin_sqrtf(r0 * r0 + r1 * r1) * sqrthf;
             int8_t out = inside ? (int8_t)SPAC_OUT_INSIDE : (use_dom ? (int8_t)SPAC_OUT_DOMAIN : (int8_t)SPAC_OUT_SUBDOMAIN);
             if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
-            if (out != SPAC_OUT_INSIDE) {
-                /* exit projection in FP64 onto the boundary the FP32 test chose */
-                BoundaryHit hit = (out == SPAC_OUT_SUBDOMAIN) ? circle_distance(R, cx, cy, (double)xf, (double)yf)
-                                                              : rectangle_distance(a.dom, (double)xf, (double)yf);
-                int64_t g = rec + (int64_t)path;
-                a.xt[g] = hit.ex;
-                a.yt[g] = hit.ey;
-                a.Y[g] = Yw;
-                a.Z[g] = Z;
-                a.xi[g] = xi;
-                a.tau[g] = (double)nstep * h;
-                a.steps[g] = nstep;
-                a.outcome[g] = out;
+            if (out != SPAC_OUT_INSIDE) { /* the record is stored with the wave's next batch of exits */
+                done = true;
+                done_out = out;
                 active = false;
             }
         }
