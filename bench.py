@@ -50,10 +50,10 @@ PATH_KERNEL_TRAFFIC_BYTES = {
     "rows": (7177.5 * 2.0 + 7598712.3) * 1024,
 }
 # The learned-CV path kernel of the full-scale system, per path of the launch: measured at 72
-# paths per row (13,458,240 paths; FETCH_SIZE 95,095 KiB x 2.0, WRITE_SIZE 3,404,908 KiB x 1.0,
-# mean of the last four dispatches); its launches in the bench run at the calibrated count,
-# so the bytes per launch are this figure times the launch's paths.
-LEARNED_PATH_KERNEL_BYTES_PER_PATH = (95094.4 * 2.0 + 3404908.5) * 1024 / (186920 * 72)
+# paths per row (13,458,240 paths; FETCH_SIZE 186,282.5 KiB x 2.0, WRITE_SIZE 3,169,083.3 KiB
+# x 1.0, mean of the last four dispatches); its launches in the bench run at the calibrated
+# count, so the bytes per launch are this figure times the launch's paths.
+LEARNED_PATH_KERNEL_BYTES_PER_PATH = (186282.5 * 2.0 + 3169083.3) * 1024 / (186920 * 72)
 RECORD_BYTES_PER_PATH = 6 * 8 + 4 + 1  # x_tau, y_tau, Y, Z, xi, tau; steps; outcome
 COMPUTE_UNIT_64 = "VALU FP64 (no matrix-core work on this path; FP64 vector peak = dense FP64 MFMA peak = 78.6 TFLOP/s)"
 COMPUTE_UNIT_32 = "VALU FP32 (no matrix-core work in the path kernel; MI355X FP32 vector peak 157.3 TFLOP/s)"

@@ -202,9 +202,10 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     bool drained = false;
 
     while (true) {
-        /* Exits and refills, in batches of kLaneBatch64 lanes (one: at once) or when no lane
-           steps any more; the structure is shared with the FP32 kernel. */
-        {
+        /* Exits and refills in batches of kLaneBatch64 lanes or when no lane steps any more (the
+           structure of the FP32 kernel); with batches of one a record is stored at its stop and
+           none of this is compiled. */
+        if constexpr (kLaneBatch64 > 1) {
             const unsigned long long stopped = __ballot(done);
             if (stopped != 0ull && (__popcll(stopped) >= kLaneBatch64 || __ballot(active) == 0ull)) {
                 if (done) {
@@ -222,7 +223,8 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             }
         }
         unsigned long long idle = __ballot(!active && !done);
-        if (__popcll(idle) < kLaneBatch64 && __ballot(active) != 0ull) idle = 0ull; /* refill later, together */
+        if constexpr (kLaneBatch64 > 1)
+            if (__popcll(idle) < kLaneBatch64 && __ballot(active) != 0ull) idle = 0ull; /* refill later, together */
         while (idle != 0ull && !drained) {
             if (pool_next >= pool_end) {
                 int32_t item = 0;
@@ -400,11 +402,23 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             } else {
                 if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
             }
-            if (out != SPAC_OUT_INSIDE) { /* the record is stored with the wave's next batch of exits */
-                done = true;
-                done_out = out;
-                done_ex = hit.ex;
-                done_ey = hit.ey;
+            if (out != SPAC_OUT_INSIDE) {
+                if constexpr (kLaneBatch64 > 1) { /* stored with the wave's next batch of exits */
+                    done = true;
+                    done_out = out;
+                    done_ex = hit.ex;
+                    done_ey = hit.ey;
+                } else {
+                    int64_t g = rec + (int64_t)path;
+                    a.xt[g] = hit.ex;
+                    a.yt[g] = hit.ey;
+                    a.Y[g] = Yw;
+                    a.Z[g] = Z;
+                    a.xi[g] = xi;
+                    a.tau[g] = t;
+                    a.steps[g] = nstep; /* gval: fk_exit_values_kernel, lane-dense after the launch */
+                    a.outcome[g] = out;
+                }
                 active = false;
             }
         }
