@@ -367,9 +367,8 @@ def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep,
 
     A probe set at probe_paths, then two sets, each at the count where the fit over every set
     so far gives 0.9 x target, then further sets (at most max_extra) while the last set's own
-    RMS is above the target, each at the count the last set's RMS scales to (RMS ~ N^-1/2) with
-    5 % margin.  The last set (its replicas and timings) is what the bootstrap and the
-    projection use."""
+    RMS is above the target, each at the larger of that fit's count and 1.05 x the last count.
+    The last set (its replicas and timings) is what the bootstrap and the projection use."""
     sets, hist = [], []
     paths, reps, it = probe_paths, probe_rep, 0
     while True:
@@ -390,7 +389,8 @@ def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep,
         if it <= 1:
             paths = _fit_paths(sets, 0.9 * target)[0]
         else:
-            paths = int(np.ceil(paths * 1.05 * (rms / target) ** 2))
+            # the fit over every set so far (one set's RMS scatters by ~12 %), and at least 5 % more
+            paths = max(_fit_paths(sets, 0.9 * target)[0], int(np.ceil(paths * 1.05)))
         reps, it = rep, it + 1
     return {"paths": int(paths), "U": U, "recs": recs, "rms": rms, "history": hist,
             "reached_target": bool(rms <= target)}
@@ -529,7 +529,7 @@ def bench_ttr(args):
             for k in ("training_data_ms", "train_ms", "train_wall_ms", "table_ms", "train_loss"):
                 s[k] = float(np.mean([r[k] for r in runs]))
             s["training_data"] = runs[0]["training_data"]
-            s["train"] = {"steps": cfg["steps"], "lr": list(cfg["lr"]), "beta2": cfg["beta2"],
+            s["train"] = {"points": cfg["points_side"] ** 2, "steps": cfg["steps"], "lr": list(cfg["lr"]), "beta2": cfg["beta2"],
                           "grad_weight": cfg["grad_weight"], "table": cfg["table"]}
         # the part of the wall time that scales with paths per row (assembly), and the rest
         s["_scaling_ms"] = s["assemble_wall_ms"]
