@@ -6,8 +6,9 @@ Default (--workload ttr): the headline line, time to a statistical RMS error of 
 solved interface values on the full-scale system (64 x 64 circles, 99,604 nodes, 186,920
 rows), the learned control variate end to end against plain Monte Carlo on the same GPU.
 One step = one learned-CV solve from scratch: training data, training, table, assembly of
-every row at the calibrated path count into the device-resident system, and GMRES from a
-zero start.  Inputs (mesh, interface tables, job arrays) are resident on the device before
+every row at the calibrated path count into the device-resident system, and GMRES from the
+network's prediction at the nodes (plain MC: from zero).  Inputs (mesh, interface tables, job
+arrays) are resident on the device before
 the timed region.  Beside it: plain MC at its own calibrated count, bootstrap confidence
 intervals of the speed-ups, path-steps/s, the FP32-against-FP64 difference, the roofline of
 the dominant kernel (HIP events on its stream, measured inside the library), the PDDSparse
@@ -307,15 +308,17 @@ GMRES_TOL = 1e-8
 COARSE_GRID = 12
 
 
-def _solve_replica(prob, mesh, system, paths, h, seed, stats=False):
-    """One assembly of every row into the device-resident system and its GPU GMRES solve from a
-    zero start; returns (u, record).  All calls are synchronous, so the host clock brackets the
-    device work."""
+def _solve_replica(prob, mesh, system, paths, h, seed, stats=False, guess=None):
+    """One assembly of every row into the device-resident system and its GPU GMRES solve, from a
+    zero start or from guess() (the learned method: its network's u at the nodes, evaluated inside
+    the timed solve); returns (u, record).  All calls are synchronous, so the host clock brackets
+    the device work."""
     t0 = time.perf_counter()
-    st = prob.assemble_system(mesh, system, paths, h, seed=seed, stats=stats)
+    st = prob.assemble_system_uniform(system, paths, h, seed=seed, stats=stats)
     t1 = time.perf_counter()
     t = prob.timing()
-    u, info = system.solve(tol=GMRES_TOL, max_iter=20000, restart=200)
+    u0 = guess() if guess is not None else None
+    u, info = system.solve(tol=GMRES_TOL, max_iter=20000, restart=200, u0=u0)
     t2 = time.perf_counter()
     if not info.converged:
         raise RuntimeError(f"GMRES did not converge (seed {seed}): residual {info.residual:.3e}")
@@ -362,7 +365,7 @@ def _fit_paths(sets, aim):
     return int(np.ceil((a + np.sqrt(a * a + 4.0 * b * T2)) / (2.0 * T2))), float(a), float(b)
 
 
-def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep, seed0, max_extra=2):
+def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep, seed0, max_extra=2, guess=None):
     """Path count per row at which this method's own replica set measures RMS <= target.
 
     A probe set at probe_paths, then two sets, each at the count where the fit over every set
@@ -374,7 +377,7 @@ def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep,
     while True:
         us, recs = [], []
         for k in range(reps):
-            u, r = _solve_replica(prob, mesh, system, paths, h, seed0 + 1000 * it + k, stats=(k == 0))
+            u, r = _solve_replica(prob, mesh, system, paths, h, seed0 + 1000 * it + k, stats=(k == 0), guess=guess)
             us.append(u)
             recs.append(r)
         U = np.array(us)
@@ -439,9 +442,13 @@ def bench_ttr(args):
     theta0 = net.params
     train_learned_cv(net, **cfg, eager=args.train_eager, data_problem=data_prob)
     net.attach(learned_prob, fallback=True)
-    _solve_replica(learned_prob, mesh, system, 16, h, 1)  # warm-up
+
+    def guess():
+        return net.predict_system(system)
+
+    _solve_replica(learned_prob, mesh, system, 16, h, 1, guess=guess)  # warm-up
     cal_l = _calibrate("learned", learned_prob, mesh, system, h, target, args.ttr_probe_paths, args.ttr_replicas,
-                       args.ttr_refine_replicas, 200)
+                       args.ttr_refine_replicas, 200, guess=guess)
     Nl = cal_l["paths"]
 
     def learned_step(seed, stats=False):
@@ -450,7 +457,7 @@ def bench_ttr(args):
         setup = train_learned_cv(net, **cfg, eager=args.train_eager, data_problem=data_prob)
         net.attach(learned_prob, fallback=True)
         t1 = time.perf_counter()
-        u, r = _solve_replica(learned_prob, mesh, system, Nl, h, seed, stats=stats)
+        u, r = _solve_replica(learned_prob, mesh, system, Nl, h, seed, stats=stats, guess=guess)
         t2 = time.perf_counter()
         r.update(setup)
         r["setup_wall_ms"] = (t1 - t0) * 1e3
@@ -472,7 +479,7 @@ def bench_ttr(args):
     # ---- the learned control variate evaluated per step (SPAC_CV_NETWORK) against its table, same
     # paths: variance ratios from the per-row estimators of one assembly each, and path-kernel times
     def row_variances(prob, seed):
-        st = prob.assemble_system(mesh, system, Nl, h, seed=seed, stats=True)
+        st = prob.assemble_system_uniform(system, Nl, h, seed=seed, stats=True)
         t = prob.timing()
         vp = float(np.sum(st["phiphi"] - st["phi"] ** 2))
         vr = float(np.sum(st["phiphi_VR"] - st["phi_VR"] ** 2))
@@ -699,7 +706,8 @@ def main():
                                "25 nodes per circle), 99,604 nodes / 186,920 rows, sigma=(1+0.5 sin(pi x) cos(pi y)) I, "
                                "b=0.3(y,-x), c=-1-x^2, manufactured Franke u, Gaussian RBF eps=3 on 15-node stencils; "
                                "learned control variate (MLP 2-64-64-64-3 trained on the GPU) + FP32 stepping with "
-                               "FP64 sums + two-level FP64 GMRES to 1e-8; one step = one solve end to end",
+                               "FP64 sums + two-level FP64 GMRES to 1e-8 (learned: from the network's u at the "
+                               "nodes; plain: from zero); one step = one solve end to end",
                    "target_rms": 1e-3, "rms": "statistical RMS of the solved node values over independent replicas",
                    "h": res["mesh"]["h"], "nodes": res["mesh"]["nodes"], "rows": res["mesh"]["rows"],
                    "paths_per_row_learned": L["paths_per_row"], "paths_per_row_plain": P["paths_per_row"],

@@ -264,6 +264,20 @@ spac_status spac_fk_assemble_system(spac_problem* p, const spac_interface_table*
                                     const spac_node_batch* nodes, uint64_t seed, spac_system* sys,
                                     spac_point_stats* stats);
 
+/* spac_fk_assemble_system for the batch of every job of the system's mesh at one path count
+ * and step: `paths` paths of step h for every non-boundary job, RNG stream = job index (the
+ * node batch spac_mesh_jobs describes).  This is the reference's use of the solver, one N and
+ * one h for every node of the mesh (FeynmanKacSolver::Solve_OMP_Analytic(Nudo&, ..., N, h),
+ * Solvers/FeynmanKacSolver.cpp:233-260, as driven by main_test_mpi.cpp over every node).
+ * The solve table and work items are built on the device from the mesh's job arrays, which
+ * the system holds since spac_mesh_system_create, so nothing goes up from the host per call.
+ * Results equal spac_fk_assemble_system with that node batch bit for bit.  The _async form
+ * returns once the kernels are enqueued, as spac_fk_assemble_system_async (spac_fk_wait). */
+spac_status spac_fk_assemble_system_uniform(spac_problem* p, spac_system* sys, uint32_t paths, double h,
+                                            uint64_t seed, spac_point_stats* stats);
+spac_status spac_fk_assemble_system_uniform_async(spac_problem* p, spac_system* sys, uint32_t paths, double h,
+                                                  uint64_t seed, spac_point_stats* stats);
+
 /* Rows of nodes inside full interior circles (interface code 1) from the pseudospectral
  * solver instead of paths: PseudoespectralCirculoSolver::Inicializa + Resuelve(BVP,
  * nudos_interior, nudos_periferia) (Solvers/PseudoespectralCirculoSolver.cpp:2-260,

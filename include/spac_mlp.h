@@ -80,6 +80,14 @@ spac_status spac_mlp_read_table(spac_mlp* m, float* out4 /* t*t*4 */);
  * table is copied, m may be destroyed afterwards). */
 spac_status spac_mlp_attach(spac_mlp* m, spac_problem* p, int32_t fallback, double max_ratio);
 
+/* The network's u head at every node of a device-resident system (spac_mesh_system_create),
+ * evaluated on the device by the BF16-MFMA inference: u[i] = u_hat(x_i, y_i) for the system's
+ * n_rows nodes.  The learned method's initial guess for the interface solve
+ * (spac_gmres_solve_system): the reference's GMRES starts from zero
+ * (Solvers/SistemaLinealSolverGMRES_OMP.hpp:519-577); a guess near the solution only changes
+ * how many iterations reach the same tolerance. */
+spac_status spac_mlp_predict_system(spac_mlp* m, const spac_system* sys, double* u);
+
 /* Attach the network itself (mode SPAC_CV_NETWORK): F = -sigma grad u_hat with the gradient heads
  * evaluated in FP32 at every step of every path, instead of read from the table.  The per-step
  * form of the learned control variate (Solvers/FeynmanKacSolver.cpp:13 with a learned F), kept

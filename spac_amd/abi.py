@@ -301,6 +301,9 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.spac_fk_assemble_system.argtypes = [C.c_void_p, C.POINTER(InterfaceTable), C.POINTER(NodeBatch),
                                             C.c_uint64, C.c_void_p, C.POINTER(PointStats)]
     lib.spac_fk_assemble_system_async.argtypes = lib.spac_fk_assemble_system.argtypes
+    lib.spac_fk_assemble_system_uniform.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_double, C.c_uint64,
+                                                    C.POINTER(PointStats)]
+    lib.spac_fk_assemble_system_uniform_async.argtypes = lib.spac_fk_assemble_system_uniform.argtypes
     lib.spac_fk_solve_points_device_async.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32,
                                                       C.c_double, C.c_uint64, C.c_void_p]
     lib.spac_fk_wait.argtypes = [C.c_void_p]
@@ -338,6 +341,7 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.spac_mlp_build_table.argtypes = [C.c_void_p, C.c_int32, dp]
     lib.spac_mlp_read_table.argtypes = [C.c_void_p, fp]
     lib.spac_mlp_attach.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double]
+    lib.spac_mlp_predict_system.argtypes = [C.c_void_p, C.c_void_p, dp]
     lib.spac_mlp_attach_network.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double]
     for header in sorted(HEADER.parent.glob("*.h")):
         for name in declared_symbols(header):

@@ -99,6 +99,26 @@ spac_status spac_mesh_system_create(const spac_mesh* hm, spac_system** out) {
         SYS_HIP_CHECK(hipMemcpy(d->row_jobs.ptr, rjobs.data(), (size_t)nj * 4, hipMemcpyHostToDevice));
     }
     SYS_HIP_CHECK(hipMemcpy(d->row_job_ptr.ptr, rptr.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    /* each row's diagonal slot (the GMRES Jacobi scaling reads it); a pattern with two diagonal
+       entries in a row keeps none and GMRES scans the rows instead */
+    {
+        std::vector<int32_t> ds(n, -1);
+        bool unique = true;
+        for (int r = 0; r < n && unique; ++r)
+            for (int j = s.G_i[r]; j < s.G_i[r + 1]; ++j)
+                if (s.G_j[j] == r) {
+                    if (ds[r] >= 0) unique = false;
+                    ds[r] = j;
+                }
+        if (unique) {
+            if ((rc = d->dslot.ensure((size_t)n * 4 + 4))) {
+                d->release();
+                delete d;
+                return rc;
+            }
+            SYS_HIP_CHECK(hipMemcpy(d->dslot.ptr, ds.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+        }
+    }
     /* the interface table and the job arrays the row kernels read */
     spac_interface_table tab;
     spac_mesh_interfaces(hm, &tab);
@@ -118,6 +138,59 @@ spac_status spac_mesh_system_create(const spac_mesh* hm, spac_system** out) {
     }
     if (nj > 0) SYS_HIP_CHECK(hipMemcpy(d->t_jobs.ptr, jarr.data(), jarr.size() * 4, hipMemcpyHostToDevice));
     SYS_HIP_CHECK(hipMemcpy(d->t_goff.ptr, goff.data(), goff.size() * 8, hipMemcpyHostToDevice));
+    d->n_ifaces = tab.n_interfaces;
+    d->rbf_kind = tab.rbf_kind;
+    d->stencil_size = tab.stencil_size;
+    d->gauss_eps = tab.gauss_eps;
+    d->g_total = goff[nj];
+    d->mmax = 1;
+    for (int k = 0; k < nj; ++k) d->mmax = std::max(d->mmax, job_m[k]);
+    /* the fixed part of the every-job batch's solve table (the node batch spac_mesh_jobs gives) */
+    {
+        std::vector<double> jx(nj), jy(nj), xy(2 * (size_t)nj), circ(3 * (size_t)nj);
+        std::vector<int32_t> jr(nj), ji(nj), jf(nj), live((size_t)nj + 1, 0);
+        if ((rc = spac_mesh_jobs(hm, jr.data(), jx.data(), jy.data(), ji.data(), jf.data()))) {
+            d->release();
+            delete d;
+            return rc;
+        }
+        for (int k = 0; k < nj; ++k) {
+            xy[k] = jx[k];
+            xy[nj + k] = jy[k];
+            const int f = job_iface[k];
+            circ[3 * (size_t)k] = tab.radius[f];
+            circ[3 * (size_t)k + 1] = tab.cx[f];
+            circ[3 * (size_t)k + 2] = tab.cy[f];
+            live[k + 1] = live[k] + (job_front[k] ? 0 : 1);
+        }
+        d->n_live = live[nj];
+        if ((rc = d->u_xy.ensure(xy.size() * 8 + 8)) || (rc = d->u_circle.ensure(circ.size() * 8 + 8)) ||
+            (rc = d->u_live.ensure(live.size() * 4))) {
+            d->release();
+            delete d;
+            return rc;
+        }
+        if (nj > 0) {
+            SYS_HIP_CHECK(hipMemcpy(d->u_xy.ptr, xy.data(), xy.size() * 8, hipMemcpyHostToDevice));
+            SYS_HIP_CHECK(hipMemcpy(d->u_circle.ptr, circ.data(), circ.size() * 8, hipMemcpyHostToDevice));
+        }
+        SYS_HIP_CHECK(hipMemcpy(d->u_live.ptr, live.data(), live.size() * 4, hipMemcpyHostToDevice));
+    }
+    /* the rows' nodes (spac_mlp_predict_system evaluates the network there) */
+    {
+        std::vector<double> nx(n), ny(n), nxy(2 * (size_t)n);
+        std::vector<int32_t> nf(n);
+        if ((rc = spac_mesh_nodes(hm, nx.data(), ny.data(), nf.data())) || (rc = d->node_xy.ensure(nxy.size() * 8 + 8))) {
+            d->release();
+            delete d;
+            return rc;
+        }
+        for (int i = 0; i < n; ++i) {
+            nxy[i] = nx[i];
+            nxy[n + i] = ny[i];
+        }
+        if (n > 0) SYS_HIP_CHECK(hipMemcpy(d->node_xy.ptr, nxy.data(), nxy.size() * 8, hipMemcpyHostToDevice));
+    }
     d->h_job_row = std::move(job_row);
     d->h_job_iface = std::move(job_iface);
     d->h_job_m = std::move(job_m);
@@ -176,14 +249,51 @@ spac_status spac_mesh_system_set_coarse(const spac_mesh* hm, spac_system* hs, in
     for (int a = 0; a < k; ++a) ptr[a + 1] += ptr[a];
     std::vector<int32_t> cur(ptr.begin(), ptr.end() - 1);
     for (int i = 0; i < n; ++i) rows[cur[agg[i]]++] = i;
+    /* the aggregates each block of kCoarseRowBlock rows spans (the prolongation forms one coarse
+       value per aggregate of its block) */
+    const int nblk = (n + kCoarseRowBlock - 1) / kCoarseRowBlock;
+    std::vector<int32_t> bptr(nblk + 1, 0), bagg;
+    std::vector<uint8_t> rslot(n, 0);
+    int blk_max = 0;
+    for (int b = 0; b < nblk; ++b) {
+        const int first = (int)bagg.size();
+        for (int i = b * kCoarseRowBlock; i < std::min(n, (b + 1) * kCoarseRowBlock); ++i) {
+            int q = first;
+            while (q < (int)bagg.size() && bagg[q] != agg[i]) ++q;
+            if (q == (int)bagg.size()) bagg.push_back(agg[i]);
+            rslot[i] = (uint8_t)std::min(q - first, 255);
+        }
+        bptr[b + 1] = (int32_t)bagg.size();
+        blk_max = std::max(blk_max, bptr[b + 1] - bptr[b]);
+    }
+    /* each aggregate's entries of the block lists, in block order */
+    std::vector<int32_t> aptr(k + 1, 0), aent(bagg.size());
+    for (size_t e = 0; e < bagg.size(); ++e) aptr[bagg[e] + 1]++;
+    for (int a = 0; a < k; ++a) aptr[a + 1] += aptr[a];
+    {
+        std::vector<int32_t> fill(aptr.begin(), aptr.end() - 1);
+        for (size_t e = 0; e < bagg.size(); ++e) aent[fill[bagg[e]]++] = (int32_t)e;
+    }
+    if (bagg.empty()) bagg.push_back(0);
+    if (aent.empty()) aent.push_back(0);
     if ((rc = d->c_agg.ensure((size_t)n * 4)) || (rc = d->c_ptr.ensure((size_t)(k + 1) * 4)) ||
         (rc = d->c_rows.ensure((size_t)n * 4)) || (rc = d->c_efix.ensure((size_t)k * k * 8)) ||
-        (rc = d->c_einv.ensure((size_t)k * k * 8)) || (rc = d->c_vec.ensure((size_t)2 * k * 8)))
+        (rc = d->c_einv.ensure((size_t)k * k * 8)) || (rc = d->c_work.ensure((size_t)2 * k * k * 8 + 64)) ||
+        (rc = d->c_vec.ensure((size_t)2 * k * 8)) || (rc = d->c_bptr.ensure((size_t)(nblk + 1) * 4)) ||
+        (rc = d->c_bagg.ensure(bagg.size() * 4)) || (rc = d->c_rslot.ensure((size_t)n + 4)) ||
+        (rc = d->c_part.ensure(bagg.size() * 8)) || (rc = d->c_aptr.ensure((size_t)(k + 1) * 4)) ||
+        (rc = d->c_aent.ensure(aent.size() * 4)))
         return rc;
     if (hipMemcpy(d->c_agg.ptr, agg.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d->c_ptr.ptr, ptr.data(), (size_t)(k + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d->c_rows.ptr, rows.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(d->c_rows.ptr, rows.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d->c_bptr.ptr, bptr.data(), (size_t)(nblk + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d->c_bagg.ptr, bagg.data(), bagg.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d->c_rslot.ptr, rslot.data(), (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d->c_aptr.ptr, aptr.data(), (size_t)(k + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d->c_aent.ptr, aent.data(), aent.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
         return set_error(SPAC_E_HIP, "uploading the coarse space failed");
+    d->c_blk_max = blk_max;
     d->coarse_k = k;
     return SPAC_OK;
 }

@@ -125,6 +125,10 @@ struct Problem {
     spac_status ensure_solve_scratch(int32_t n, SolveSet* ss);
     spac_status upload_solves(int32_t n, const double* x0, const double* y0, const uint32_t* paths, const double* h,
                               const uint32_t* strm, const double* circle, SolveSet* out);
+    /* The solve table upload_solves would build for every job of a device system's mesh at one
+     * (paths, h) (RNG stream = job index), built on the device from the system's resident job
+     * arrays: nothing goes up from the host (fk_rows.hip). */
+    spac_status device_solves_uniform(const DeviceSystem* sys, uint32_t paths, double h, SolveSet* out);
     /* Enqueue without synchronising (capturable): per-solve sigma(X0), queue
      * reset, path kernel (after_path recorded behind it when given), then the
      * exit values g(X_tau) / u(X_tau) of every record.  seed_dev != nullptr:
@@ -163,6 +167,10 @@ struct Problem {
     }
 };
 
+/* Paths per work item of a launch over total_paths paths: the launch's chunk, enlarged for very
+ * large batches so that the queue holds at most ~2^20 items (fk_kernels.hip). */
+uint32_t work_item_paths(uint32_t chunk, int64_t total_paths);
+
 /* SPAC_OK when no asynchronous call is pending on p, else an error naming spac_fk_wait. */
 spac_status no_pending(const Problem* p);
 
@@ -176,8 +184,13 @@ spac_status problem_set_cv_network(Problem* p, const spac_cv_config* cfg, const 
  * SistemaLineal::Inicializa_CSR (diagonal = containing circles, 1e-23 elsewhere), each job's
  * first slot in its row, and its rows' jobs in job order.  Rows assembled on the device are
  * accumulated here and solved here (spac_fk_assemble_system, spac_gmres_solve_system). */
-/* largest coarse space of the system GMRES: its Galerkin matrix is inverted on the host per solve */
+/* largest coarse space of the system GMRES: its Galerkin matrix is inverted on the device per
+ * solve, by one workgroup */
 constexpr int kMaxCoarse = 576;
+/* rows per block of the coarse prolongation, and the most aggregates one block's rows may span
+ * (coarse spaces beyond it prolong row by row) */
+constexpr int kCoarseRowBlock = 256;
+constexpr int kCoarseBlockAggs = 32;
 
 struct DeviceSystem {
     int32_t n_rows = 0, n_jobs = 0;
@@ -185,6 +198,7 @@ struct DeviceSystem {
     int device = 0;
     DeviceBuffer Ip, J, init, vals, B; /* int32 [n_rows + 1], int32 [nnz], f64 [nnz] x 2, f64 [n_rows] */
     DeviceBuffer job_slot;             /* int64 [n_jobs]: first CSR slot of the job's G, -1: none */
+    DeviceBuffer dslot;                /* int32 [n_rows]: each row's diagonal slot, -1: none */
     DeviceBuffer row_job_ptr, row_jobs; /* int32 [n_rows + 1], int32 [n_jobs]: jobs of a row, in order */
     std::vector<int32_t> h_job_row;    /* host copy: row of every job */
     std::vector<int32_t> h_job_iface, h_job_m, h_job_frontera; /* interface, G length, boundary flag */
@@ -193,6 +207,17 @@ struct DeviceSystem {
     /* the mesh's interface table (iface_table_upload layout), its jobs' interface and boundary
      * flag (int32 [n_jobs] each, one buffer) and G offsets (int64 [n_jobs + 1]) */
     DeviceBuffer t_ifc, t_ifoff, t_ifnode, t_ifipsi, t_jobs, t_goff;
+    int32_t n_ifaces = 0, rbf_kind = 0, stencil_size = 0; /* of that table */
+    double gauss_eps = 0.0;
+    int32_t mmax = 1;                       /* its largest interface (nodes) */
+    int64_t g_total = 0;                    /* G values of all jobs (t_goff[n_jobs]) */
+    /* the solve table of the batch of every job at one (paths, h), which
+     * spac_fk_assemble_system_uniform completes on the device: each job's node (x, then y;
+     * f64 [n_jobs] each), its circle (f64 [3 n_jobs]: radius, cx, cy) and the number of live
+     * (non-boundary) jobs before it (int32 [n_jobs + 1]) */
+    DeviceBuffer u_xy, u_circle, u_live;
+    int32_t n_live = 0;
+    DeviceBuffer node_xy; /* the rows' nodes: x, then y (f64 [n_rows] each) */
     DeviceBuffer flag; /* int32: worst estimator status of the last assembly */
     /* optional coarse space of the GMRES preconditioner (spac_mesh_system_set_coarse):
      * coarse_k aggregates of rows, piecewise constant */
@@ -201,12 +226,30 @@ struct DeviceSystem {
     DeviceBuffer c_ptr, c_rows;  /* int32 [k + 1], [n_rows]: rows of each aggregate, ascending */
     DeviceBuffer c_efix;         /* int64 [k * k]: Z^T A Z in 2^-40 fixed point */
     DeviceBuffer c_einv;         /* f64 [k * k]: its inverse */
+    DeviceBuffer c_work;         /* f64 [k * 2k]: [E | I] reduced in place by the device inversion, then int32
+                                    flag (singular) */
     DeviceBuffer c_vec;          /* f64 [2 k]: aggregate sums, coarse correction */
+    /* the aggregates of each kCoarseRowBlock-row block of rows, in order of first appearance
+       (int32 [blocks + 1] offsets, int32 entries), and each row's index in its block's list
+       (uint8 [n_rows]); c_blk_max: the longest list (<= kCoarseBlockAggs) */
+    DeviceBuffer c_bptr, c_bagg, c_rslot;
+    int32_t c_blk_max = 0;
+    /* per-block partial sums of a vector over each block's aggregates (f64, one per entry of
+       c_bagg), and for each aggregate its entries in block order (int32 [k + 1] offsets, int32
+       entries): Z^T x is summed from these */
+    DeviceBuffer c_part, c_aptr, c_aent;
+    /* GMRES work state of the last solve (buffers, stream, events, pinned columns), kept for the
+       next solve of the same size (gmres.hip); gm_free releases it */
+    mutable void* gm = nullptr;
+    mutable void (*gm_free)(void*) = nullptr;
     void release() {
-        for (DeviceBuffer* b : {&Ip, &J, &init, &vals, &B, &job_slot, &row_job_ptr, &row_jobs, &t_ifc, &t_ifoff,
-                                &t_ifnode, &t_ifipsi, &t_jobs, &t_goff, &flag, &c_agg, &c_ptr, &c_rows, &c_efix,
-                                &c_einv, &c_vec})
+        for (DeviceBuffer* b : {&Ip, &J, &init, &vals, &B, &job_slot, &dslot, &row_job_ptr, &row_jobs, &t_ifc, &t_ifoff,
+                                &t_ifnode, &t_ifipsi, &t_jobs, &t_goff, &u_xy, &u_circle, &u_live, &node_xy, &flag, &c_agg,
+                                &c_ptr, &c_rows, &c_efix, &c_einv, &c_work, &c_vec, &c_bptr, &c_bagg, &c_rslot,
+                                &c_part, &c_aptr, &c_aent})
             b->release();
+        if (gm && gm_free) gm_free(gm);
+        gm = nullptr;
     }
 };
 

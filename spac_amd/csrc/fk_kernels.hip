@@ -821,7 +821,7 @@ spac_status Problem::ensure_records(int64_t total_paths) {
 /* Paths per work item: the launch's chunk size, enlarged for very large batches so
  * that the queue holds at most ~2^20 items (every item costs one atomic on a single
  * counter; 1.6e7 of them cost ~120 ms).  Results do not depend on it. */
-static uint32_t work_item_paths(uint32_t chunk, int64_t total_paths) {
+uint32_t work_item_paths(uint32_t chunk, int64_t total_paths) {
     const int64_t kMaxItems = (int64_t)1 << 20;
     const int64_t c = (total_paths + kMaxItems - 1) / kMaxItems;
     return (uint32_t)std::max<int64_t>((int64_t)chunk, c);

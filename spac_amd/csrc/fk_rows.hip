@@ -408,6 +408,10 @@ std::vector<char> spac::table_signature(const spac_interface_table* t) {
 
 namespace spac {
 
+static void fill_rows_args(Problem* p, RowsSetup* rs, int K, int rbf_kind, int stencil_size, double gauss_eps,
+                           const DeviceBuffer& ifc, const DeviceBuffer& ifoff, const DeviceBuffer& ifnode,
+                           const DeviceBuffer& ifipsi, const DeviceBuffer& jobs, const DeviceBuffer& goff);
+
 /* Upload the node batch, interface table and work list into p's buffers and
  * allocate the records, so that rows_enqueue only launches kernels.  With sys (a device
  * system of the batch's mesh), the interface table and the job arrays are the copies the
@@ -456,7 +460,6 @@ static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const
     spac_status rc;
     if ((rc = p->upload_solves(n, nb->x, nb->y, paths.data(), nb->h, nb->stream, circle.data(), &rs->ss))) return rc;
     const int K = t->n_interfaces;
-    const size_t szK = (size_t)K;
     if ((rc = p->rows_G.ensure((size_t)std::max<int64_t>(rs->goff[n], 1) * 8))) return rc;
     if ((rc = p->rows_part.ensure((size_t)n * 8))) return rc; /* B */
     if ((rc = p->stats.ensure((size_t)n * sizeof(spac_point_stats)))) return rc;
@@ -487,14 +490,28 @@ static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const
         jobs = &p->rows_iface;
         goff = &p->rows_goff;
     }
+    fill_rows_args(p, rs, K, t->rbf_kind, t->stencil_size, t->gauss_eps, *ifc, *ifoff, *ifnode, *ifipsi, *jobs, *goff);
+    return SPAC_OK;
+}
+
+/* The row kernels' arguments (RowsArgs) over the interface table in (ifc, ifoff, ifnode, ifipsi)
+ * of iface_table_upload's layout, the job arrays `jobs` (interface, then boundary flag; int32
+ * [n] each) and G offsets `goff` (int64 [n + 1]), the solve table rs->ss and p's records. */
+static void fill_rows_args(Problem* p, RowsSetup* rs, int K, int rbf_kind, int stencil_size, double gauss_eps,
+                           const DeviceBuffer& ifcb, const DeviceBuffer& ifoffb, const DeviceBuffer& ifnodeb,
+                           const DeviceBuffer& ifipsib, const DeviceBuffer& jobsb, const DeviceBuffer& goffb) {
+    const DeviceBuffer *ifc = &ifcb, *ifoff = &ifoffb, *ifnode = &ifnodeb, *ifipsi = &ifipsib, *jobs = &jobsb,
+                       *goff = &goffb;
+    const int n = rs->n;
+    const size_t szK = (size_t)K;
     const size_t noff_bytes = (((szK + 1) * 4 + 7) / 8) * 8;
     const double* d_cx = (const double*)ifc->ptr;
     RowsArgs& ra = rs->ra;
     std::memset(&ra, 0, sizeof ra);
     ra.n_jobs = n;
-    ra.rbf_kind = t->rbf_kind;
-    ra.stencil_size = t->stencil_size;
-    ra.gauss_eps = t->gauss_eps;
+    ra.rbf_kind = rbf_kind;
+    ra.stencil_size = stencil_size;
+    ra.gauss_eps = gauss_eps;
     ra.job_iface = (const int32_t*)jobs->ptr;
     ra.paths = rs->ss.paths;
     ra.rec_off = rs->ss.rec_off;
@@ -515,6 +532,111 @@ static spac_status rows_prepare(Problem* p, const spac_interface_table* t, const
     rs->d_front = (const int32_t*)jobs->ptr + n;
     rs->d_B = (double*)p->rows_part.ptr;
     rs->d_stats = (spac_point_stats*)p->stats.ptr;
+}
+
+/* The solve table of every job at one (paths, h), one thread per job: what upload_solves
+ * builds on the host for that node batch (RNG stream = job index, boundary jobs without paths
+ * or work items, live jobs' records and items in job order), from the system's resident job
+ * arrays.  Job k's items are those of live job live[k]: (k, c chunk, min((c + 1) chunk, paths)). */
+__global__ void k_uniform_solves(int32_t n, const double* __restrict__ jxy, const double* __restrict__ jcirc,
+                                 const int32_t* __restrict__ live, const int32_t* __restrict__ front, uint32_t paths,
+                                 double h, double sqrth, uint32_t chunk, int32_t cps, double* __restrict__ x0,
+                                 double* __restrict__ y0, double* __restrict__ hs, uint32_t* __restrict__ strm,
+                                 uint32_t* __restrict__ ps, int64_t* __restrict__ off, double* __restrict__ circ,
+                                 double* __restrict__ sq, int4* __restrict__ items) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    const int64_t lv = live[k];
+    off[k] = lv * (int64_t)paths;
+    if (k == n) return;
+    x0[k] = jxy[k];
+    y0[k] = jxy[n + k];
+    hs[k] = h;
+    sq[k] = sqrth;
+    strm[k] = (uint32_t)k;
+    circ[3 * (int64_t)k] = jcirc[3 * (int64_t)k];
+    circ[3 * (int64_t)k + 1] = jcirc[3 * (int64_t)k + 1];
+    circ[3 * (int64_t)k + 2] = jcirc[3 * (int64_t)k + 2];
+    if (front[k]) {
+        ps[k] = 0u;
+        return;
+    }
+    ps[k] = paths;
+    for (int32_t c = 0; c < cps; ++c) {
+        const uint32_t p0 = (uint32_t)c * chunk;
+        items[lv * cps + c] = make_int4(k, (int)p0, (int)min(p0 + chunk, paths), 0);
+    }
+}
+
+spac_status Problem::device_solves_uniform(const DeviceSystem* sys, uint32_t paths, double h, SolveSet* out) {
+    if (paths < 1) return set_error(SPAC_E_INVALID_ARG, "paths must be >= 1");
+    if (!(h > 0.0) || !(h < 1e30)) return set_error(SPAC_E_INVALID_ARG, "bad h");
+    const int32_t n = sys->n_jobs;
+    const int64_t total = (int64_t)sys->n_live * (int64_t)paths;
+    const uint32_t chunk = work_item_paths((uint32_t)launch.chunk_paths, total);
+    const int64_t cps = ((int64_t)paths + chunk - 1) / chunk;
+    const int64_t n_items = cps * (int64_t)sys->n_live;
+    if (n_items > 0x7fffffff) return set_error(SPAC_E_INVALID_ARG, "too many work items");
+    const size_t nb = (size_t)std::max(n, 1);
+    spac_status rc;
+    if ((rc = sx.ensure(nb * 8)) || (rc = sy.ensure(nb * 8)) || (rc = sh.ensure(nb * 8)) ||
+        (rc = sstream.ensure(nb * 4)) || (rc = spaths.ensure(nb * 4)) || (rc = soff.ensure((nb + 1) * 8)) ||
+        (rc = scircle.ensure(nb * 24)) || (rc = sitems.ensure((size_t)std::max<int64_t>(n_items, 1) * sizeof(int4))) ||
+        (rc = ssq.ensure(nb * 8)))
+        return rc;
+    const int32_t* jobs = (const int32_t*)sys->t_jobs.ptr;
+    hipLaunchKernelGGL(k_uniform_solves, dim3((n + 1 + 255) / 256), dim3(256), 0, stream, n,
+                       (const double*)sys->u_xy.ptr, (const double*)sys->u_circle.ptr, (const int32_t*)sys->u_live.ptr,
+                       jobs + n, paths, h, std::sqrt(h) /* correctly rounded, as upload_solves' */, chunk, (int32_t)cps,
+                       (double*)sx.ptr, (double*)sy.ptr, (double*)sh.ptr, (uint32_t*)sstream.ptr, (uint32_t*)spaths.ptr,
+                       (int64_t*)soff.ptr, (double*)scircle.ptr, (double*)ssq.ptr, (int4*)sitems.ptr);
+    ROWS_HIP_CHECK(hipGetLastError());
+    SolveSet ss;
+    ss.n = n;
+    ss.x0 = (const double*)sx.ptr;
+    ss.y0 = (const double*)sy.ptr;
+    ss.h = (const double*)sh.ptr;
+    ss.stream = (const uint32_t*)sstream.ptr;
+    ss.paths = (const uint32_t*)spaths.ptr;
+    ss.rec_off = (const int64_t*)soff.ptr;
+    ss.circle = (const double*)scircle.ptr;
+    ss.items = (const int4*)sitems.ptr;
+    ss.n_items = (int32_t)n_items;
+    ss.total_paths = total;
+    ss.chunk = chunk;
+    ss.sqrth = (const double*)ssq.ptr;
+    if ((rc = ensure_solve_scratch(n, &ss))) return rc;
+    *out = ss;
+    return SPAC_OK;
+}
+
+/* rows_prepare for the batch of every job of sys's mesh at one (paths, h): the solve table is
+ * built on the device (Problem::device_solves_uniform) and the row kernels read the system's
+ * interface table and job arrays, so the host touches nothing per job. */
+static spac_status rows_prepare_uniform(Problem* p, const DeviceSystem* sys, uint32_t paths, double h,
+                                        RowsSetup* rs) {
+    const int n = sys->n_jobs;
+    rs->n = n;
+    rs->mmax = sys->mmax;
+    if (sys->rbf_kind == SPAC_RBF_GAUSS_STENCIL) {
+        rs->ra.stencil_size = sys->stencil_size;
+        int lds_max = 0;
+        ROWS_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
+        const size_t need = stencil_lds_bytes(*rs) + 5 * 1024; /* + the kernel's static LDS */
+        if (need > (size_t)lds_max)
+            return set_error(SPAC_E_INVALID_ARG, "interfaces of %d nodes need %zu B of LDS per block (device: %d)",
+                             rs->mmax, need, lds_max);
+        rs->g_block = (int64_t)paths <= kMomentSmallRow ? 64 : 256;
+    }
+    spac_status rc;
+    if ((rc = p->device_solves_uniform(sys, paths, h, &rs->ss))) return rc;
+    if ((rc = p->rows_G.ensure((size_t)std::max<int64_t>(sys->g_total, 1) * 8))) return rc;
+    if ((rc = p->rows_part.ensure((size_t)n * 8))) return rc; /* B */
+    if ((rc = p->stats.ensure((size_t)n * sizeof(spac_point_stats)))) return rc;
+    if ((rc = p->ensure_records(rs->ss.total_paths))) return rc;
+    if ((rc = p->queue.ensure(64))) return rc;
+    fill_rows_args(p, rs, sys->n_ifaces, sys->rbf_kind, sys->stencil_size, sys->gauss_eps, sys->t_ifc, sys->t_ifoff,
+                   sys->t_ifnode, sys->t_ifipsi, sys->t_jobs, sys->t_goff);
     return SPAC_OK;
 }
 
@@ -650,6 +772,9 @@ __global__ void k_system_b(int32_t n_rows, const int32_t* __restrict__ row_job_p
 namespace spac {
 /* Checks and enqueues one assembly of the system's jobs (path kernel, reduction, B and G
  * kernels, accumulation into the system); the call in flight is left in p->pending. */
+static spac_status enqueue_system_kernels(Problem* p, const RowsSetup& rs, uint64_t seed, DeviceSystem* sys,
+                                          spac_point_stats* stats);
+
 static spac_status enqueue_assemble_system(Problem* p, const spac_interface_table* t, const spac_node_batch* nb,
                                            uint64_t seed, DeviceSystem* sys, spac_point_stats* stats) {
     spac_status rc = validate(t, nb);
@@ -669,6 +794,28 @@ static spac_status enqueue_assemble_system(Problem* p, const spac_interface_tabl
     if (p->device != sys->device) return set_error(SPAC_E_INVALID_ARG, "problem and system are on different devices");
     RowsSetup rs;
     if ((rc = rows_prepare(p, t, nb, &rs, sys))) return rc;
+    return enqueue_system_kernels(p, rs, seed, sys, stats);
+}
+
+/* Every job of sys's mesh at one (paths, h): the same work with the solve table built on the
+ * device. */
+static spac_status enqueue_assemble_system_uniform(Problem* p, DeviceSystem* sys, uint32_t paths, double h,
+                                                   uint64_t seed, spac_point_stats* stats) {
+    if (p->device != sys->device) return set_error(SPAC_E_INVALID_ARG, "problem and system are on different devices");
+    int dev = -1;
+    ROWS_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != sys->device) return set_error(SPAC_E_INVALID_ARG, "the system's device is not the current device");
+    RowsSetup rs;
+    spac_status rc;
+    if ((rc = rows_prepare_uniform(p, sys, paths, h, &rs))) return rc;
+    return enqueue_system_kernels(p, rs, seed, sys, stats);
+}
+
+/* The kernels of one system assembly over a prepared batch (path kernel, reduction, B and G,
+ * accumulation into the system); the call in flight is left in p->pending. */
+static spac_status enqueue_system_kernels(Problem* p, const RowsSetup& rs, uint64_t seed, DeviceSystem* sys,
+                                          spac_point_stats* stats) {
+    spac_status rc;
     hipStream_t s = p->stream;
     int blocks = 0;
     ROWS_HIP_CHECK(hipEventRecord(p->ev[0], s));
@@ -749,6 +896,36 @@ extern "C" spac_status spac_fk_assemble_system_async(spac_problem* hp, const spa
     spac_status rc = no_pending(p);
     if (rc) return rc;
     if ((rc = enqueue_assemble_system(p, t, nb, seed, reinterpret_cast<DeviceSystem*>(hsys), stats))) {
+        p->pending = Problem::Pending();
+        (void)hipStreamSynchronize(p->stream);
+        return rc;
+    }
+    return SPAC_OK;
+}
+
+extern "C" spac_status spac_fk_assemble_system_uniform(spac_problem* hp, spac_system* hsys, uint32_t paths, double h,
+                                                       uint64_t seed, spac_point_stats* stats) {
+    if (!hp || !hsys) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    spac_status rc = no_pending(p);
+    if (rc) return rc;
+    if ((rc = enqueue_assemble_system_uniform(p, reinterpret_cast<DeviceSystem*>(hsys), paths, h, seed, stats))) {
+        p->pending = Problem::Pending();
+        (void)hipStreamSynchronize(p->stream);
+        return rc;
+    }
+    const Problem::Pending pd = p->pending;
+    p->pending = Problem::Pending();
+    return finish_assemble_system(p, pd);
+}
+
+extern "C" spac_status spac_fk_assemble_system_uniform_async(spac_problem* hp, spac_system* hsys, uint32_t paths,
+                                                             double h, uint64_t seed, spac_point_stats* stats) {
+    if (!hp || !hsys) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    Problem* p = reinterpret_cast<Problem*>(hp);
+    spac_status rc = no_pending(p);
+    if (rc) return rc;
+    if ((rc = enqueue_assemble_system_uniform(p, reinterpret_cast<DeviceSystem*>(hsys), paths, h, seed, stats))) {
         p->pending = Problem::Pending();
         (void)hipStreamSynchronize(p->stream);
         return rc;

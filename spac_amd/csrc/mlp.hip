@@ -673,6 +673,12 @@ __global__ void __launch_bounds__(256) k_mlp_infer_mfma(InferArgs a) {
     }
 }
 
+/* u[i] = the u head of the inference output (u, gx, gy, 0) of point i */
+__global__ void k_take_u(int n, const float4* __restrict__ out4, double* __restrict__ u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = (double)out4[i].x;
+}
+
 __global__ void k_to_bf16(const float* in, short* out, int n) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) out[k] = f2bf(in[k]);
@@ -682,7 +688,9 @@ __global__ void k_to_bf16(const float* in, short* out, int n) {
 struct Mlp {
     spac_mlp_desc desc{};
     hipStream_t stream = nullptr;
-    DeviceBuffer params, grads, am, av, wbf, act, pts, tgt, table, part, ctr;
+    DeviceBuffer params, grads, am, av, wbf, act, pts, tgt, table, part, ctr, pred;
+    double* pinned = nullptr; /* pinned host staging of spac_mlp_predict_system's result */
+    size_t pinned_cap = 0;
     int n = 0; /* prepared batch size */
     int adam_t = 0;
     int table_t = 0;
@@ -917,8 +925,9 @@ spac_status spac_mlp_destroy(spac_mlp* hm) {
     if (!m) return SPAC_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (DeviceBuffer* b : {&m->params, &m->grads, &m->am, &m->av, &m->wbf, &m->act, &m->pts, &m->tgt, &m->table,
-                            &m->part, &m->ctr})
+                            &m->part, &m->ctr, &m->pred})
         b->release();
+    if (m->pinned) (void)hipHostFree(m->pinned);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return SPAC_OK;
@@ -1065,6 +1074,38 @@ spac_status spac_mlp_read_table(spac_mlp* hm, float* out4) {
     Mlp* m = reinterpret_cast<Mlp*>(hm);
     if (!m || !out4 || !m->table_t) return set_error(SPAC_E_INVALID_ARG, "no table");
     MLP_HIP_CHECK(hipMemcpy(out4, m->table.ptr, (size_t)m->table_t * m->table_t * 16, hipMemcpyDeviceToHost));
+    return SPAC_OK;
+}
+
+spac_status spac_mlp_predict_system(spac_mlp* hm, const spac_system* hs, double* u) {
+    Mlp* m = reinterpret_cast<Mlp*>(hm);
+    const DeviceSystem* sys = reinterpret_cast<const DeviceSystem*>(hs);
+    if (!m || !sys || !u) return set_error(SPAC_E_INVALID_ARG, "null argument");
+    int dev = -1;
+    MLP_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != sys->device) return set_error(SPAC_E_INVALID_ARG, "the system lives on device %d; make it current first",
+                                             sys->device);
+    const int n = sys->n_rows;
+    if (n < 1) return SPAC_OK;
+    spac_status rc;
+    if ((rc = m->pred.ensure((size_t)n * (16 + 8)))) return rc;
+    if (m->pinned_cap < (size_t)n * 8) {
+        double* old = m->pinned;
+        m->pinned = nullptr;
+        m->pinned_cap = 0;
+        if (old) MLP_HIP_CHECK(hipHostFree(old));
+        MLP_HIP_CHECK(hipHostMalloc((void**)&m->pinned, (size_t)n * 8, hipHostMallocDefault));
+        m->pinned_cap = (size_t)n * 8;
+    }
+    const double* nx = (const double*)sys->node_xy.ptr;
+    float* out4 = (float*)m->pred.ptr;
+    double* ud = (double*)(out4 + 4 * (size_t)n);
+    if ((rc = mlp_infer_dev(m, nx, nx + n, n, 0, out4))) return rc;
+    hipLaunchKernelGGL(k_take_u, dim3((n + 255) / 256), dim3(256), 0, m->stream, n, (const float4*)out4, ud);
+    MLP_HIP_CHECK(hipGetLastError());
+    MLP_HIP_CHECK(hipMemcpyAsync(m->pinned, ud, (size_t)n * 8, hipMemcpyDeviceToHost, m->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(m->stream));
+    std::memcpy(u, m->pinned, (size_t)n * 8);
     return SPAC_OK;
 }
 

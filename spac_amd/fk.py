@@ -214,6 +214,16 @@ class FKProblem:
                   self.lib)
         return np.ctypeslib.as_array(st) if stats else None
 
+    def assemble_system_uniform(self, system, paths, h, seed=0, stats=True):
+        """spac_fk_assemble_system_uniform: assemble_system over every job of the system's mesh at
+        scalar (paths, h), with the solve table built on the device (no per-call upload).  Same
+        results as assemble_system(system.mesh, system, paths, h, seed) bit for bit."""
+        n = system.mesh.info.n_jobs
+        st = (abi.PointStats * n)() if stats else None
+        abi.check(self.lib.spac_fk_assemble_system_uniform(self.h, system.h, int(paths), float(h), seed, st),
+                  self.lib)
+        return np.ctypeslib.as_array(st) if stats else None
+
     def autotune_rows(self, mesh, paths, h, path_fraction=0.125, jobs=None):
         """spac_fk_autotune_rows: installs the fastest launch configuration; returns the report and
         the measured candidates [(block_size, blocks_per_cu, chunk_paths, item_order, ms), ...]."""

@@ -105,6 +105,13 @@ class MLP:
         abi.check(self.lib.spac_mlp_read_table(self.h, abi.ptr(out, C.c_float)), self.lib)
         return out
 
+    def predict_system(self, system):
+        """spac_mlp_predict_system: the u head at the system's nodes (BF16-MFMA inference on the
+        device), the learned initial guess of its GMRES solve."""
+        u = np.zeros(system.mesh.info.n_nodes)
+        abi.check(self.lib.spac_mlp_predict_system(self.h, system.h, abi.ptr(u, dp)), self.lib)
+        return u
+
     def attach(self, problem, fallback=True, max_ratio=1.0, network=False):
         """spac_mlp_attach (the table, SPAC_CV_TABLE) or, with network=True,
         spac_mlp_attach_network (the network evaluated at every step, SPAC_CV_NETWORK)."""

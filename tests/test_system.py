@@ -35,6 +35,8 @@ def test_system_calls_without_handle_fail_cleanly():
     assert lib.spac_mesh_system_read(None, None, None, None, None) == abi.SPAC_E_INVALID_ARG
     assert lib.spac_gmres_solve_system(None, None, 1e-8, 10, 10, None) == abi.SPAC_E_INVALID_ARG
     assert lib.spac_fk_assemble_system(None, None, None, 0, None, None) == abi.SPAC_E_INVALID_ARG
+    assert lib.spac_fk_assemble_system_uniform(None, None, 10, 1e-3, 0, None) == abi.SPAC_E_INVALID_ARG
+    assert lib.spac_fk_assemble_system_uniform_async(None, None, 10, 1e-3, 0, None) == abi.SPAC_E_INVALID_ARG
 
 
 def test_system_create_without_device_is_a_status():
@@ -72,6 +74,37 @@ def test_gpu_system_equals_host_accumulation(gpu, precision):
                     assert np.array_equal(ref, st[f]), f
                 t = p.timing()
                 assert t.total_ms > 0 and t.path_kernel_ms > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [abi.SPAC_FP64, abi.SPAC_FP32])
+def test_gpu_system_uniform_equals_node_batch(gpu, precision):
+    """The every-job batch built on the device gives the host-built node batch's system and
+    estimators bit for bit, on two meshes and two calls in a row (the second with fewer paths,
+    so the device solve table shrinks)."""
+    for desc, paths, h in ((mesh_desc(), 300, 5e-4), (mesh_desc(n_side=16, nodes_per_circle=40), 130, 2e-4)):
+        m = Mesh(desc)
+        say(f"device system uniform: precision {precision}, {m.info.n_jobs} jobs")
+        with FKProblem(problem_desc(coef=VC, domain=DOM, precision=precision)) as p, DeviceSystem(m) as s:
+            for seed, np_ in ((5, paths), (6, paths // 3)):
+                st = p.assemble_system(m, s, np_, h, seed=seed)
+                ref = s.read()
+                st_u = p.assemble_system_uniform(s, np_, h, seed=seed)
+                got = s.read()
+                for a, b in zip(ref, got):
+                    assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+                for f in st.dtype.names:
+                    a, b = np.ascontiguousarray(st[f]), np.ascontiguousarray(st_u[f])
+                    assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f
+                live = m.job_frontera == 0
+                assert (st_u["paths"][live] == np_).all() and (st_u["paths"][~live] == 0).all()
+        with FKProblem(problem_desc(coef=VC, domain=DOM, precision=precision)) as p, DeviceSystem(m) as s:
+            with pytest.raises(abi.SpacError) as e:
+                p.assemble_system_uniform(s, 0, h)
+            assert e.value.code == abi.SPAC_E_INVALID_ARG
+            with pytest.raises(abi.SpacError) as e:
+                p.assemble_system_uniform(s, 10, -1.0)
+            assert e.value.code == abi.SPAC_E_INVALID_ARG
 
 
 @pytest.mark.gpu
@@ -176,3 +209,24 @@ def test_gpu_system_two_level_gmres(gpu):
         u_o, info_o = s.solve(tol=1e-10, max_iter=5000, restart=100)
         assert info_o.iterations == info_j.iterations
         assert np.array_equal(u_o.view(np.int64), u_j.view(np.int64))
+
+
+@pytest.mark.gpu
+def test_gpu_mlp_predict_system_is_the_inference_at_the_nodes(gpu):
+    """spac_mlp_predict_system gives the BF16-MFMA inference's u head at every node of the
+    system, bit for bit, and GMRES from that guess reaches the tolerance and the zero-start
+    solution."""
+    from spac_amd.mlp import MLP
+
+    m = Mesh(mesh_desc())
+    say(f"predict system: {m.info.n_nodes} nodes")
+    with MLP(DOM, seed=0) as net, DeviceSystem(m) as s, \
+            FKProblem(problem_desc(coef=VC, domain=DOM, precision=abi.SPAC_FP32)) as p:
+        u_hat = net.predict_system(s)
+        ref = net.infer(m.node_x, m.node_y)[:, 0].astype(np.float64)
+        assert np.array_equal(u_hat, ref)
+        p.assemble_system_uniform(s, 500, 5e-4, seed=2, stats=False)
+        u0, i0 = s.solve(tol=1e-10, max_iter=3000, restart=60)
+        u1, i1 = s.solve(tol=1e-10, max_iter=3000, restart=60, u0=u_hat)
+        assert i0.converged and i1.converged
+        assert np.abs(u1 - u0).max() < 1e-7 * np.abs(u0).max()
