@@ -435,10 +435,10 @@ def bench_ttr(args):
 
     # ---- learned control variate: calibrate with the trained network (training is deterministic,
     # so every timed step retrains exactly this network), then time the whole method per step
-    cfg = {k: v for k, v in LEARNED_CV.items() if k != "seed"}
+    cfg = {k: v for k, v in LEARNED_CV.items() if k not in ("seed", "first_layer_scale")}
     learned_prob = FKProblem(desc32)
     data_prob = FKProblem(desc32)  # the training data's point solves
-    net = MLP(UNIT_SQUARE, seed=LEARNED_CV["seed"])
+    net = MLP(UNIT_SQUARE, seed=LEARNED_CV["seed"], first_layer_scale=LEARNED_CV["first_layer_scale"])
     theta0 = net.params
     train_learned_cv(net, **cfg, eager=args.train_eager, data_problem=data_prob)
     net.attach(learned_prob, fallback=True)

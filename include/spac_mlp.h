@@ -28,8 +28,11 @@ extern "C" {
 typedef struct spac_mlp spac_mlp;
 
 typedef struct {
-    double domain[4]; /* xmin, ymin, xmax, ymax: inputs are scaled to [-1, 1]^2 over it */
-    uint64_t seed;    /* Glorot-uniform initialisation */
+    double domain[4];         /* xmin, ymin, xmax, ymax: inputs are scaled to [-1, 1]^2 over it */
+    uint64_t seed;            /* of the initialisation: Glorot-uniform weights, zero biases */
+    double first_layer_scale; /* > 0: the first layer's weights and biases uniform in (-s, s)
+                                 instead (tanh features of varied slope and offset across the
+                                 domain from the first step); 0: Glorot, zero biases */
 } spac_mlp_desc;
 
 /* Adam on the full batch (beta1 0.9, beta2 as given, 0: 0.999); learning rate

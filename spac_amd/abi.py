@@ -235,7 +235,7 @@ class CvConfig(C.Structure):
 
 
 class MlpDesc(C.Structure):
-    _fields_ = [("domain", C.c_double * 4), ("seed", C.c_uint64)]
+    _fields_ = [("domain", C.c_double * 4), ("seed", C.c_uint64), ("first_layer_scale", C.c_double)]
 
 
 class MlpTrainCfg(C.Structure):

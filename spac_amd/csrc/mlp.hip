@@ -885,6 +885,8 @@ spac_status spac_mlp_create(const spac_mlp_desc* d, spac_mlp** out) {
     *out = nullptr;
     if (!(d->domain[2] > d->domain[0]) || !(d->domain[3] > d->domain[1]))
         return set_error(SPAC_E_INVALID_ARG, "MLP domain must have xmax > xmin and ymax > ymin");
+    if (!(d->first_layer_scale >= 0.0) || !(d->first_layer_scale < 1e3))
+        return set_error(SPAC_E_INVALID_ARG, "first_layer_scale must be in [0, 1000)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(SPAC_E_NO_DEVICE, "no HIP device visible");
     Mlp* m = new Mlp();
@@ -900,18 +902,23 @@ spac_status spac_mlp_create(const spac_mlp_desc* d, spac_mlp** out) {
         spac_mlp_destroy(reinterpret_cast<spac_mlp*>(m));
         return rc;
     }
-    /* Glorot-uniform weights, zero biases */
+    /* Glorot-uniform weights, zero biases; with first_layer_scale s > 0, W1 and b1 uniform in
+       (-s, s): the Glorot bound of W1 (0.30) keeps every first-layer tanh nearly linear over the
+       scaled domain, and features of varied slope and offset fit the solution's bumps in fewer
+       steps */
     std::vector<float> p(kNParams, 0.0f);
     std::mt19937_64 g(d->seed);
-    auto fill = [&](int off, int fo, int fi) {
-        float lim = std::sqrt(6.0f / (float)(fo + fi));
+    auto fill = [&](int off, int n, float lim) {
         std::uniform_real_distribution<float> U(-lim, lim);
-        for (int k = 0; k < fo * fi; ++k) p[off + k] = U(g);
+        for (int k = 0; k < n; ++k) p[off + k] = U(g);
     };
-    fill(oW1, kH, 2);
-    fill(oW2, kH, kH);
-    fill(oW3, kH, kH);
-    fill(oW4, 3, kH);
+    auto glorot = [](int fo, int fi) { return std::sqrt(6.0f / (float)(fo + fi)); };
+    const float s1 = (float)d->first_layer_scale;
+    fill(oW1, kH * 2, s1 > 0.0f ? s1 : glorot(kH, 2));
+    if (s1 > 0.0f) fill(ob1, kH, s1);
+    fill(oW2, kH * kH, glorot(kH, kH));
+    fill(oW3, kH * kH, glorot(kH, kH));
+    fill(oW4, 3 * kH, glorot(3, kH));
     MLP_HIP_CHECK(hipMemcpy(m->params.ptr, p.data(), kNParams * 4, hipMemcpyHostToDevice));
     MLP_HIP_CHECK(hipMemset(m->am.ptr, 0, kNParams * 4));
     MLP_HIP_CHECK(hipMemset(m->av.ptr, 0, kNParams * 4));

@@ -17,12 +17,13 @@ def _d(a):
 class MLP:
     """Owns one spac_mlp handle: parameters, Adam state, training buffers and the table."""
 
-    def __init__(self, domain, seed=0):
+    def __init__(self, domain, seed=0, first_layer_scale=0.0):
         self.lib = abi.load()
         d = abi.MlpDesc()
         for i, v in enumerate(domain):
             d.domain[i] = v
         d.seed = seed
+        d.first_layer_scale = first_layer_scale
         h = C.c_void_p()
         abi.check(self.lib.spac_mlp_create(C.byref(d), C.byref(h)), self.lib)
         self.h = h

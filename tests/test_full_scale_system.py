@@ -79,9 +79,9 @@ def test_full_scale_fp32_and_learned_against_fp64(gpu, system):
 
     from spac_amd.mlp import MLP
 
-    with MLP(UNIT_SQUARE, seed=LEARNED_CV["seed"]) as net, \
+    with MLP(UNIT_SQUARE, seed=LEARNED_CV["seed"], first_layer_scale=LEARNED_CV["first_layer_scale"]) as net, \
             FKProblem(problem_desc(coef=VC, domain=UNIT_SQUARE, precision=abi.SPAC_FP32)) as pl:
-        cfg = {k: v for k, v in LEARNED_CV.items() if k != "seed"}
+        cfg = {k: v for k, v in LEARNED_CV.items() if k not in ("seed", "first_layer_scale")}
         setup = train_learned_cv(net, **cfg)
         say(f"learned CV setup: {setup}")
         net.attach(pl, fallback=True)

@@ -194,7 +194,7 @@ def test_network_cv_per_step_against_table(gpu):
     from spac_amd.mesh import Mesh, mesh_desc
 
     say("network CV: train")
-    cfg = {k: v for k, v in LEARNED_CV.items() if k != "seed"}
+    cfg = {k: v for k, v in LEARNED_CV.items() if k not in ("seed", "first_layer_scale")}
     desc32 = problem_desc(coef=VC, domain=DOM, precision=abi.SPAC_FP32)
     rng = np.random.default_rng(8)
     n = 48
@@ -204,7 +204,7 @@ def test_network_cv_per_step_against_table(gpu):
     b = SolveBatchArrays(cx + r * np.cos(a), cy + r * np.sin(a), 2000, 1e-4, np.arange(n),
                          circle=np.stack([np.full(n, 0.08), cx, cy], 1))
     res = {}
-    with MLP(DOM, seed=0) as m, FKProblem(desc32) as dp:
+    with MLP(DOM, seed=0, first_layer_scale=LEARNED_CV["first_layer_scale"]) as m, FKProblem(desc32) as dp:
         train_learned_cv(m, **cfg, eager=False, data_problem=dp)
         for mode in ("table", "network"):
             with FKProblem(desc32) as p:

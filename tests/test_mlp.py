@@ -241,3 +241,37 @@ def test_training_beta2_option(gpu):
         assert rep["final_loss"] < 0.05 * l0
         with pytest.raises(abi.SpacError):
             m.train(x, y, u, steps=10, lr=1e-3, lr_final=1e-4, grad_weight=0.03, beta2=1.5)
+
+
+def test_first_layer_scale_rejected_out_of_range():
+    """A negative or non-finite first-layer scale is an argument error, checked before the device."""
+    import ctypes as C
+    from spac_amd import abi
+    from spac_amd.mlp import MLP  # noqa: F401  (binds the MLP entry points)
+
+    lib = abi.load()
+    for bad in (-1.0, float("nan"), 1e6):
+        d = abi.MlpDesc()
+        d.domain[:] = DOM
+        d.seed = 0
+        d.first_layer_scale = bad
+        h = C.c_void_p()
+        assert lib.spac_mlp_create(C.byref(d), C.byref(h)) == abi.SPAC_E_INVALID_ARG
+        assert "first_layer_scale" in abi.last_error(lib)
+
+
+@pytest.mark.gpu
+def test_first_layer_scale_initialisation(gpu):
+    """first_layer_scale s: W1 and b1 uniform in (-s, s), the deeper layers Glorot with zero
+    biases; s = 0: Glorot W1 and zero b1 throughout."""
+    from spac_amd.mlp import MLP
+
+    W1, b1, W2, b2, W3, b3, W4, b4 = range(8)
+    with MLP(DOM, seed=3, first_layer_scale=2.0) as m:
+        p = mo.unpack(m.params)
+    assert np.abs(p[W1]).max() < 2.0 and np.abs(p[W1]).max() > 1.5
+    assert np.abs(p[b1]).max() < 2.0 and np.abs(p[b1]).max() > 1.5
+    assert np.abs(p[W2]).max() <= np.sqrt(6.0 / 128) and not p[b2].any() and not p[b4].any()
+    with MLP(DOM, seed=3) as m:
+        q = mo.unpack(m.params)
+    assert np.abs(q[W1]).max() <= np.sqrt(6.0 / 66) and not q[b1].any()
