@@ -80,10 +80,10 @@ static vec2 vector_defecto(vec2 x) { vec2 r = {x.x * 0.0, x.y * 0.0}; return r; 
  * BVPs/Franke_Poisson.hpp:4-73.  pow(a, 2) is written a*a (pow(a,2) is
  * exactly a*a under correct rounding).  Pinned by: tests/test_oracle.py
  * (values and derivatives against finite differences of u; f = -lap u). */
-static double fr1(vec2 X) { double a = 9.0 * X.x - 2.0, b = 9.0 * X.y - 2.0; return spac_exp(-0.25 * (a * a + b * b)); }
-static double fr2(vec2 X) { double a = 9.0 * X.x + 1.0; return spac_exp((-1.0 / 49.0) * (a * a) - 0.1 * (9.0 * X.y + 1.0)); }
-static double fr3(vec2 X) { double a = 9.0 * X.x - 7.0, b = 9.0 * X.y - 3.0; return spac_exp(-0.25 * (a * a + b * b)); }
-static double fr4(vec2 X) { double a = 9.0 * X.x - 4.0, b = 9.0 * X.y - 7.0; return spac_exp(-1.0 * (a * a + b * b)); }
+static double fr1(vec2 X) { double a = 9.0 * X.x - 2.0, b = 9.0 * X.y - 2.0; return spac_exp_finite(-0.25 * (a * a + b * b)); }
+static double fr2(vec2 X) { double a = 9.0 * X.x + 1.0; return spac_exp_finite((-1.0 / 49.0) * (a * a) - 0.1 * (9.0 * X.y + 1.0)); }
+static double fr3(vec2 X) { double a = 9.0 * X.x - 7.0, b = 9.0 * X.y - 3.0; return spac_exp_finite(-0.25 * (a * a + b * b)); }
+static double fr4(vec2 X) { double a = 9.0 * X.x - 4.0, b = 9.0 * X.y - 7.0; return spac_exp_finite(-1.0 * (a * a + b * b)); }
 
 static double franke_u(vec2 X) {
     return 0.75 * fr1(X) + 0.75 * fr2(X) + 0.5 * fr3(X) - 0.2 * fr4(X);
@@ -221,8 +221,9 @@ static double circulo(const double* p, vec2 posicion, vec2* exitpoint, vec2* nor
     normal->y = posicion.y - p[2];
     double r = sqrt(normal->x * normal->x + normal->y * normal->y);
     r = (5E-4 < r) ? r : 5E-4; /* std::max(5E-4, r) */
-    normal->x = normal->x / r;
-    normal->y = normal->y / r;
+    const double ir = 1.0 / r;   /* (X - C) / r, as one reciprocal and two products */
+    normal->x = normal->x * ir;
+    normal->y = normal->y * ir;
     exitpoint->x = normal->x * p[0] + p[1];
     exitpoint->y = normal->y * p[0] + p[2];
     return r - p[0];
@@ -361,19 +362,17 @@ static int dentro(const bvp_t* bvp, vec2 X, double sqrth, double gobet, vec2* Np
         s = (d < 0.0) ? bvp->sigma(X) : bvp->sigma(*Npro);
         double r0 = N->x * s.a00 + N->y * s.a10;
         double r1 = N->x * s.a01 + N->y * s.a11;
-        double nrm = sqrt(r0 * r0 + r1 * r1);
         if (bvp->reflecting && side_reflects(bvp->reflecting, *N)) {
             *refl = 1;
             status = (d <= 0.0) ? SPAC_OUT_INSIDE : SPAC_OUT_REFLECTED;
         } else {
-            status = (d < gobet * nrm * sqrth) ? SPAC_OUT_INSIDE : SPAC_OUT_DOMAIN;
+            status = spac_gobet_inside(d, gobet * sqrth, r0 * r0 + r1 * r1) ? SPAC_OUT_INSIDE : SPAC_OUT_DOMAIN;
         }
     } else {
         s = (d < 0.0) ? bvp->sigma(X) : bvp->sigma(*Npro);
         double r0 = N->x * s.a00 + N->y * s.a10;
         double r1 = N->x * s.a01 + N->y * s.a11;
-        double nrm = sqrt(r0 * r0 + r1 * r1);
-        status = (d < gobet * nrm * sqrth) ? SPAC_OUT_INSIDE : SPAC_OUT_SUBDOMAIN;
+        status = spac_gobet_inside(d, gobet * sqrth, r0 * r0 + r1 * r1) ? SPAC_OUT_INSIDE : SPAC_OUT_SUBDOMAIN;
     }
     *distancia = d;
     return status;
@@ -718,6 +717,8 @@ void oracle_math_eval(int fn, const double* x, const double* y, double* out, int
         case 4: out[i] = spac_atan2(x[i], y[i]); break;
         case 5: out[i] = sqrt(x[i]); break;
         case 6: out[i] = x[i] / y[i]; break;
+        case 9: out[i] = spac_exp_finite(x[i]); break;
+        case 10: out[i] = spac_log_normal(x[i]); break;
         default: out[i] = 0.0; break;
         }
     }

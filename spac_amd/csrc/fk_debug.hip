@@ -31,6 +31,8 @@ __global__ void debug_math_kernel(int32_t fn, const double* x, const double* y, 
         r = n0;
         break;
     }
+    case 9: r = spac_exp_finite(a); break;
+    case 10: r = spac_log_normal(a); break;
     default: r = 0.0; break;
     }
     out[i] = r;

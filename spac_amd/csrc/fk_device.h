@@ -42,10 +42,10 @@ __device__ __forceinline__ FrankeTerms franke_terms(double x, double y) {
     t.a2 = 9.0 * x + 1.0;
     t.a3 = 9.0 * x - 7.0; t.b3 = 9.0 * y - 3.0;
     t.a4 = 9.0 * x - 4.0; t.b4 = 9.0 * y - 7.0;
-    t.F1 = spac_exp(-0.25 * (t.a1 * t.a1 + t.b1 * t.b1));
-    t.F2 = spac_exp((-1.0 / 49.0) * (t.a2 * t.a2) - 0.1 * (9.0 * y + 1.0));
-    t.F3 = spac_exp(-0.25 * (t.a3 * t.a3 + t.b3 * t.b3));
-    t.F4 = spac_exp(-1.0 * (t.a4 * t.a4 + t.b4 * t.b4));
+    t.F1 = spac_exp_finite(-0.25 * (t.a1 * t.a1 + t.b1 * t.b1));
+    t.F2 = spac_exp_finite((-1.0 / 49.0) * (t.a2 * t.a2) - 0.1 * (9.0 * y + 1.0));
+    t.F3 = spac_exp_finite(-0.25 * (t.a3 * t.a3 + t.b3 * t.b3));
+    t.F4 = spac_exp_finite(-1.0 * (t.a4 * t.a4 + t.b4 * t.b4));
     return t;
 }
 __device__ __forceinline__ double franke_u(const FrankeTerms& t) {
@@ -243,8 +243,9 @@ __device__ __forceinline__ BoundaryHit circle_distance(double R, double cx, doub
     double nx = x - cx, ny = y - cy;
     double rr = __builtin_sqrt(nx * nx + ny * ny);
     rr = (5E-4 < rr) ? rr : 5E-4;
-    nx = nx / rr;
-    ny = ny / rr;
+    const double ir = 1.0 / rr;
+    nx = nx * ir;
+    ny = ny * ir;
     r.nx = nx; r.ny = ny;
     r.ex = nx * R + cx;
     r.ey = ny * R + cy;

@@ -382,8 +382,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             /* a path that goes on is inside (d < gobet |N sigma| sqrt h <= 0): s = sigma(X) */
             sig = a.sigma_fresh ? coef.sigma(X, Yc) : s;
             double r0 = hit.nx * s, r1 = hit.ny * s;
-            double nrm = __builtin_sqrt(r0 * r0 + r1 * r1);
-            bool inside = hit.d < a.gobet * nrm * sqrth;
+            bool inside = spac_gobet_inside(hit.d, a.gobet * sqrth, r0 * r0 + r1 * r1);
             int8_t out = inside ? (int8_t)SPAC_OUT_INSIDE : (use_dom ? (int8_t)SPAC_OUT_DOMAIN : (int8_t)SPAC_OUT_SUBDOMAIN);
             if constexpr (REFL) {
                 /* a reflecting side keeps the path; beyond it, reflejada: project */
@@ -568,7 +567,9 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
             float sg = coef32.sigma(pxf, pyf);
             sigf = a.sigma_fresh ? coef32.sigma(xf, yf) : sg;
             float r0 = nx * sg, r1 = ny * sg;
-            bool inside = d < gobetf * __builtin_sqrtf(r0 * r0 + r1 * r1) * sqrthf;
+            /* d < g |N sigma| sqrt h without the square root when g <= 0 (both sides <= 0) */
+            const float kf = gobetf * sqrthf, n2 = r0 * r0 + r1 * r1;
+            bool inside = (kf <= 0.0f) ? (d < 0.0f && d * d > (kf * kf) * n2) : (d < kf * __builtin_sqrtf(n2));
             int8_t out = inside ? (int8_t)SPAC_OUT_INSIDE : (use_dom ? (int8_t)SPAC_OUT_DOMAIN : (int8_t)SPAC_OUT_SUBDOMAIN);
             if (inside && nstep >= a.max_steps) out = SPAC_OUT_MAX_STEPS;
             if (out != SPAC_OUT_INSIDE) { /* the record is stored with the wave's next batch of exits */

@@ -21,6 +21,8 @@
  *           log m = 2 atanh f as an odd series in f (12 terms).
  *   sincospi : exact reduction a = q/2 + r, |r| <= 1/4, Taylor series of
  *           sin(pi r) and cos(pi r), quadrant rotation by q mod 4.
+ *   sinpi, cospi : exact reduction a = n + r, |r| <= 1/2, one Taylor series of
+ *           sin(pi t) to t^23 each (cos(pi r) = sin(pi (1/2 - |r|))).
  *   atan2 : octant reduction to t in [0,1], t > tan(pi/8) mapped by
  *           atan t = pi/4 + atan((t-1)/(t+1)), one half-angle step
  *           atan u = 2 atan(u / (1 + sqrt(1+u^2))), odd Taylor series.
@@ -72,6 +74,46 @@ SPAC_HD double spac_pow2i(int k) {
 }
 
 /* ----------------------------------------------------------------- exp -- */
+/* 2^k p for p in [0.7, 1.5] and k in [-1076, 1025]: correctly rounded, also into the subnormal
+ * range (one rounding). */
+SPAC_HD double spac_scale2(double p, int k) {
+#if defined(__HIPCC__)
+    return __builtin_ldexp(p, k);
+#else
+    return ldexp(p, k);
+#endif
+}
+
+/* e^x for x < 709.78 (finite results): spac_exp without its NaN and overflow selects, the same
+ * bits for every such x (the same reduction and polynomial; 2^k applied in one correctly rounded
+ * step instead of two exact-then-rounded products, which round to the same value).  Larger x
+ * give +inf, as from spac_exp. */
+SPAC_HD double spac_exp_finite(double x) {
+    const double LOG2E = 1.4426950408889634;
+    const double LN2_HI = 0.6931471803691238;     /* 32 significant bits */
+    const double LN2_LO = 1.9082149292705877e-10; /* ln2 - LN2_HI */
+    const double xc = (x < -746.0) ? -746.0 : ((x > 710.0) ? 710.0 : x);
+    const double kd = SPAC_RINT(xc * LOG2E);
+    double r = SPAC_FMA(-kd, LN2_HI, xc);
+    r = SPAC_FMA(-kd, LN2_LO, r);
+    double p = 1.6059043836821613e-10;
+    p = SPAC_FMA(p, r, 2.08767569878681e-09);
+    p = SPAC_FMA(p, r, 2.505210838544172e-08);
+    p = SPAC_FMA(p, r, 2.755731922398589e-07);
+    p = SPAC_FMA(p, r, 2.7557319223985893e-06);
+    p = SPAC_FMA(p, r, 2.48015873015873e-05);
+    p = SPAC_FMA(p, r, 0.0001984126984126984);
+    p = SPAC_FMA(p, r, 0.001388888888888889);
+    p = SPAC_FMA(p, r, 0.008333333333333333);
+    p = SPAC_FMA(p, r, 0.041666666666666664);
+    p = SPAC_FMA(p, r, 0.16666666666666666);
+    p = SPAC_FMA(p, r, 0.5);
+    p = SPAC_FMA(p, r, 1.0);
+    p = SPAC_FMA(p, r, 1.0);
+    const double res = spac_scale2(p, (int)kd);
+    return (x < -745.1332191019412) ? 0.0 : res;
+}
+
 SPAC_HD double spac_exp(double x) {
     const double LOG2E = 1.4426950408889634;
     const double LN2_HI = 0.6931471803691238;     /* 32 significant bits */
@@ -109,6 +151,36 @@ SPAC_HD double spac_exp(double x) {
 }
 
 /* ----------------------------------------------------------------- log -- */
+/* Natural log for normal x > 0 (x >= 2^-1022, finite): spac_log without its subnormal, zero and
+ * NaN cases, the same bits for every such x (the uniforms of the normal and exponential draws
+ * are at least 2^-53). */
+SPAC_HD double spac_log_normal(double x) {
+    const double LN2_HI = 0.6931471803691238;
+    const double LN2_LO = 1.9082149292705877e-10;
+    uint64_t bits = spac_f64_as_u64(x);
+    int e = (int)((bits >> 52) & 0x7ff) - 1023;
+    double m = spac_u64_as_f64((bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL); /* [1,2) */
+    const int hi = m > 1.4142135623730951;
+    m = hi ? m * 0.5 : m;
+    e += hi;
+    double f = (m - 1.0) / (m + 1.0);
+    double s = f * f;
+    double p = 0.08695652173913043;
+    p = SPAC_FMA(p, s, 0.09523809523809523);
+    p = SPAC_FMA(p, s, 0.10526315789473684);
+    p = SPAC_FMA(p, s, 0.11764705882352941);
+    p = SPAC_FMA(p, s, 0.13333333333333333);
+    p = SPAC_FMA(p, s, 0.15384615384615385);
+    p = SPAC_FMA(p, s, 0.18181818181818182);
+    p = SPAC_FMA(p, s, 0.2222222222222222);
+    p = SPAC_FMA(p, s, 0.2857142857142857);
+    p = SPAC_FMA(p, s, 0.4);
+    p = SPAC_FMA(p, s, 0.6666666666666666);
+    double logm = SPAC_FMA(f * s, p, 2.0 * f);
+    double ed = (double)e;
+    return SPAC_FMA(ed, LN2_HI, SPAC_FMA(ed, LN2_LO, logm));
+}
+
 /* Natural log for finite x > 0 (the only domain the solver uses). */
 SPAC_HD double spac_log(double x) {
     const double LN2_HI = 0.6931471803691238;
@@ -179,8 +251,45 @@ SPAC_HD void spac_sincospi(double a, double* s_out, double* c_out) {
     *s_out = s;
     *c_out = c;
 }
-SPAC_HD double spac_sinpi(double a) { double s, c; spac_sincospi(a, &s, &c); return s; }
-SPAC_HD double spac_cospi(double a) { double s, c; spac_sincospi(a, &s, &c); return c; }
+/* sin(pi t) for |t| <= 1/2: Taylor series to t^23 (truncation below 6e-21 at |t| = 1/2),
+ * coefficients (-1)^k pi^(2k+1) / (2k+1)! rounded from exact rational arithmetic. */
+SPAC_HD double spac_sinpi_half(double t) {
+    const double t2 = t * t;
+    double p = -1.0518471716932065e-11;
+    p = SPAC_FMA(p, t2, 5.392664662608129e-10);
+    p = SPAC_FMA(p, t2, -2.2948428997269873e-08);
+    p = SPAC_FMA(p, t2, 7.952054001475513e-07);
+    p = SPAC_FMA(p, t2, -2.1915353447830217e-05);
+    p = SPAC_FMA(p, t2, 0.00046630280576761255);
+    p = SPAC_FMA(p, t2, -0.0073704309457143504);
+    p = SPAC_FMA(p, t2, 0.08214588661112823);
+    p = SPAC_FMA(p, t2, -0.5992645293207921);
+    p = SPAC_FMA(p, t2, 2.5501640398773455);
+    p = SPAC_FMA(p, t2, -5.16771278004997);
+    /* pi t as hi + lo exactly (an FMA gives the product's rounding error), plus pi's own tail:
+       near |t| = 1/2 the rounding of pi t alone is an ulp of the result */
+    const double hi = t * 3.141592653589793;
+    const double lo = SPAC_FMA(t, 3.141592653589793, -hi);
+    const double s = hi + SPAC_FMA(t * t2, p, SPAC_FMA(t, 1.2246467991473532e-16, lo));
+    return (SPAC_FABS(t) == 0.5) ? SPAC_COPYSIGN(1.0, t) : s; /* sin(pi / 2) = 1 exactly */
+}
+/* sin(pi a), cos(pi a) for |a| < 2^51, one polynomial each: a = n + r with n = rint(a),
+ * |r| <= 1/2 (exact); sin(pi a) = (-1)^n sin(pi r) and cos(pi a) = (-1)^n sin(pi (1/2 - |r|)). */
+/* n odd, for an integer-valued n with |n| < 2^53 */
+SPAC_HD int spac_odd(double n) {
+    const double h = 0.5 * n;
+    return h != SPAC_RINT(h);
+}
+SPAC_HD double spac_sinpi(double a) {
+    const double n = SPAC_RINT(a);
+    const double s = spac_sinpi_half(a - n);
+    return spac_odd(n) ? -s : s;
+}
+SPAC_HD double spac_cospi(double a) {
+    const double n = SPAC_RINT(a);
+    const double s = spac_sinpi_half(0.5 - SPAC_FABS(a - n));
+    return spac_odd(n) ? -s : s;
+}
 
 /* --------------------------------------------------------------- atan2 -- */
 /* atan(u) for |u| <= 0.4143 */
@@ -272,7 +381,7 @@ SPAC_HD void spac_normal_pair(uint64_t seed, uint32_t stream, uint32_t path, uin
                                       (uint32_t)seed, stream);
     double u1 = 1.0 - spac_u01_53(r.v[0], r.v[1]); /* (0, 1] */
     double u2 = spac_u01_53(r.v[2], r.v[3]);       /* [0, 1) */
-    double rad = SPAC_SQRT(-2.0 * spac_log(u1));
+    double rad = SPAC_SQRT(-2.0 * spac_log_normal(u1));
     double s, c;
     spac_sincospi(2.0 * u2, &s, &c);
     *n0 = rad * c;
@@ -288,7 +397,16 @@ SPAC_HD double spac_exponential(uint64_t seed, uint32_t stream, uint32_t path, u
     spac_u32x4 r = spac_philox4x32_10(step, path, (uint32_t)(seed >> 32), SPAC_TAG_REFLECT,
                                       (uint32_t)seed, stream);
     double u = 1.0 - spac_u01_53(r.v[0], r.v[1]); /* (0, 1] */
-    return -spac_log(u);
+    return -spac_log_normal(u);
+}
+
+/* ---------------------------------------------------------- stop test -- */
+/* The Gobet–Menozzi stop test of Dentro (Solvers/FeynmanKacSolver.cpp:84,96), "inside" when
+ * d < g |N^T sigma| sqrt(h), from k = g sqrt(h) and nrm2 = |N^T sigma|^2.  With k <= 0 (the
+ * reference's g = -0.5826) both sides of d < k |.| are non-positive, so the test is d < 0 and
+ * d^2 > k^2 nrm2, with no square root. */
+SPAC_HD int spac_gobet_inside(double d, double k, double nrm2) {
+    return (k <= 0.0) ? (d < 0.0 && d * d > (k * k) * nrm2) : (d < k * SPAC_SQRT(nrm2));
 }
 
 /* -------------------------------------------- compensated accumulation -- */

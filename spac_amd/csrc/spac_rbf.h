@@ -57,7 +57,7 @@ SPAC_HD double spac_imq(double theta, double theta_j, double c2) {
 
 SPAC_HD double spac_gauss_phi(double t, double eps) {
     double e = eps * t;
-    return spac_exp(-(e * e));
+    return spac_exp_finite(-(e * e));
 }
 
 /* Nearest node on an arc of sorted angles th[0..M-1] (ties: lower index). */
@@ -108,10 +108,10 @@ typedef struct {
 SPAC_HD spac_gauss_seq spac_gauss_seq_start(double a, double b) {
     spac_gauss_seq q;
     const double b2 = b * b;
-    q.p = spac_exp(-(a * a));
-    q.E = spac_exp(2.0 * (a * b));
-    q.F = spac_exp(-b2);
-    q.G = spac_exp(-2.0 * b2);
+    q.p = spac_exp_finite(-(a * a));
+    q.E = spac_exp_finite(2.0 * (a * b));
+    q.F = spac_exp_finite(-b2);
+    q.G = spac_exp_finite(-2.0 * b2);
     return q;
 }
 /* phi_i -> phi_{i+1} */

@@ -40,6 +40,20 @@ def test_log_ulp():
     assert o.math_eval(1, np.array([1.0]))[0] == 0.0
 
 
+def test_exp_finite_and_log_normal_equal_the_full_functions():
+    """spac_exp_finite (no NaN / overflow selects, one-step 2^k scaling) gives spac_exp's bits for
+    every x < 709.78, subnormal results and underflow to zero included; spac_log_normal gives
+    spac_log's bits for every normal x > 0."""
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(-760, 709.78, 400000), rng.uniform(-746, -700, 100000),
+                        np.array([0.0, -0.0, -708.4, -745.1332191019412, -745.14, -746.0, -800.0, 709.7])])
+    assert np.array_equal(o.math_eval(9, x).view(np.uint64), o.math_eval(0, x).view(np.uint64))
+    assert np.isinf(o.math_eval(9, np.array([709.8, 800.0, 1e300]))).all()
+    u = np.concatenate([rng.uniform(2.3e-308, 1.0, 200000), 10 ** rng.uniform(-307, 300, 100000),
+                        2.0 ** -np.arange(0, 1022, dtype=np.float64), np.array([1.0, 2.0, 0.5])])
+    assert np.array_equal(o.math_eval(10, u).view(np.uint64), o.math_eval(1, u).view(np.uint64))
+
+
 def _sincospi_ref(x):
     pi = np.longdouble("3.14159265358979323846264338327950288")
     xl = x.astype(np.longdouble)
@@ -61,6 +75,10 @@ def test_sincospi_ulp():
     # exact values at quarter points
     q = np.array([0.0, 0.5, 1.0, 1.5])
     assert np.array_equal(o.math_eval(2, q), [0.0, 1.0, 0.0, -1.0])
+    assert np.array_equal(o.math_eval(3, q), [1.0, 0.0, -1.0, 0.0])
+    # odd / even symmetry: sinpi(-a) = -sinpi(a), cospi(-a) = cospi(a)
+    assert np.array_equal(o.math_eval(2, -x), -o.math_eval(2, x))
+    assert np.array_equal(o.math_eval(3, -x), o.math_eval(3, x))
 
 
 def test_atan2_ulp():
