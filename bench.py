@@ -48,7 +48,7 @@ def log(msg):
 # profiles/pddsparse_rows_fp64_4x4_1952rows_20k.md, profiles/learned_cv_time_to_rms_64x64_fp32.md.
 PATH_KERNEL_TRAFFIC_BYTES = {
     "points": (85.5 * 2.0 + 507894.5) * 1024,
-    "rows": (7177.5 * 2.0 + 7598712.3) * 1024,
+    "rows": (25955.9 * 2.0 + 8102934.9) * 1024,
 }
 # The learned-CV path kernel of the full-scale system, per path of the launch: measured at 72
 # paths per row (13,458,240 paths; FETCH_SIZE 186,282.5 KiB x 2.0, WRITE_SIZE 3,169,083.3 KiB

@@ -109,6 +109,18 @@ static double franke_d2udy2(vec2 X) {
            + (243.0 / 400.0) * fr2(X) - 2.25 * (9.0 - 4.5 * (c * c)) * fr3(X)
            + 3.6 * (9.0 - 18.0 * (d * d)) * fr4(X);
 }
+/* u_xx + u_yy as the device evaluates it: per Gaussian the sum of the u_xx and u_yy factors
+ * above, linear in the squared distance of its exponent (F1: 15.1875 s1 - 60.75; F2:
+ * (243/2401) q2 + 243/400 - 243/98, constants rounded once from the exact rationals). */
+static double franke_lap(vec2 X) {
+    double a1 = 9.0 * X.x - 2.0, b1 = 9.0 * X.y - 2.0, a2 = 9.0 * X.x + 1.0;
+    double a3 = 9.0 * X.x - 7.0, b3 = 9.0 * X.y - 3.0, a4 = 9.0 * X.x - 4.0, b4 = 9.0 * X.y - 7.0;
+    double A1 = fma(15.1875, a1 * a1 + b1 * b1, -60.75);
+    double A2 = fma(0.10120783007080383, a2 * a2, -1.872091836734694);
+    double A3 = fma(10.125, a3 * a3 + b3 * b3, -40.5);
+    double A4 = fma(-64.8, a4 * a4 + b4 * b4, 64.8);
+    return fma(A1, fr1(X), fma(A2, fr2(X), fma(A3, fr3(X), A4 * fr4(X))));
+}
 static double franke_c(vec2 X) { (void)X; return -0.0; }
 /* Franke_b builds an uninitialised Vector2d times 0.0 (BVPs/Franke_Poisson.hpp:51-54);
  * restated as the intended zero vector. */
@@ -119,7 +131,7 @@ static mat2 franke_sigma(vec2 X) {
     mat2 m = {1.0 * s, 0.0 * s, 0.0 * s, 1.0 * s};
     return m;
 }
-static double franke_f(vec2 X) { return -franke_d2udx2(X) - franke_d2udy2(X); }
+static double franke_f(vec2 X) { return -franke_lap(X); }
 static vec2 franke_gradu(vec2 X) { vec2 r = {franke_dudx(X), franke_dudy(X)}; return r; }
 /* F = -sigma^T grad u (BVPs/Franke_Poisson.hpp:67-73) */
 static vec2 franke_F(vec2 X) {
@@ -156,7 +168,7 @@ static vec2 vc_b(vec2 X) { vec2 r = {0.3 * X.y, -0.3 * X.x}; return r; }
 static double vc_c(vec2 X) { return -1.0 - X.x * X.x; }
 static double vc_f(vec2 X) {
     double s = vc_s(X);
-    double lap = franke_d2udx2(X) + franke_d2udy2(X);
+    double lap = franke_lap(X);
     vec2 b = vc_b(X);
     double adv = b.x * franke_dudx(X) + b.y * franke_dudy(X);
     double reac = vc_c(X) * franke_u(X);
@@ -719,6 +731,7 @@ void oracle_math_eval(int fn, const double* x, const double* y, double* out, int
         case 6: out[i] = x[i] / y[i]; break;
         case 9: out[i] = spac_exp_finite(x[i]); break;
         case 10: out[i] = spac_log_normal(x[i]); break;
+        case 11: out[i] = spac_gobet_inside(x[i], y[i], 1.7) ? 1.0 : 0.0; break; /* d, k; |N^T sigma|^2 = 1.7 */
         default: out[i] = 0.0; break;
         }
     }
@@ -738,8 +751,8 @@ void oracle_exponentials(uint64_t seed, uint32_t stream, uint32_t path, int32_t 
 }
 
 /* Coefficient probes for the plugin pins: which = 0 u, 1 f, 2 F.x, 3 F.y,
- * 4 sigma00, 5 b.x, 6 b.y, 7 c, 8 dudx, 9 dudy, 10 d2udx2 + d2udy2,
- * 11 psi(X, (-1, 0)), 12 psi(X, (0, 1)), 13 varphi */
+ * 4 sigma00, 5 b.x, 6 b.y, 7 c, 8 dudx, 9 dudy, 10 the Laplacian as the solver forms it,
+ * 11 psi(X, (-1, 0)), 12 psi(X, (0, 1)), 13 varphi, 14 g, 15 d2udx2 + d2udy2 term by term */
 int oracle_coef_eval(int coef_kind, int which, const double* x, const double* y, double* out, int64_t n) {
     spac_problem_desc d;
     memset(&d, 0, sizeof d);
@@ -762,7 +775,8 @@ int oracle_coef_eval(int coef_kind, int which, const double* x, const double* y,
         case 7: out[i] = bvp.c(X); break;
         case 8: out[i] = franke_dudx(X); break;
         case 9: out[i] = franke_dudy(X); break;
-        case 10: out[i] = franke_d2udx2(X) + franke_d2udy2(X); break;
+        case 10: out[i] = franke_lap(X); break;
+        case 15: out[i] = franke_d2udx2(X) + franke_d2udy2(X); break;
         case 14: out[i] = bvp.g(X); break;
         default: out[i] = 0.0; break;
         }

@@ -33,6 +33,7 @@ __global__ void debug_math_kernel(int32_t fn, const double* x, const double* y, 
     }
     case 9: r = spac_exp_finite(a); break;
     case 10: r = spac_log_normal(a); break;
+    case 11: r = spac_gobet_inside(a, b, 1.7) ? 1.0 : 0.0; break;
     default: r = 0.0; break;
     }
     out[i] = r;

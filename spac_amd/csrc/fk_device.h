@@ -33,6 +33,7 @@ struct StepCoef {
 
 struct FrankeTerms {
     double a1, b1, a2, a3, b3, a4, b4;
+    double s1, q2, s3, s4; /* the squared distances in the exponents */
     double F1, F2, F3, F4;
 };
 
@@ -42,10 +43,14 @@ __device__ __forceinline__ FrankeTerms franke_terms(double x, double y) {
     t.a2 = 9.0 * x + 1.0;
     t.a3 = 9.0 * x - 7.0; t.b3 = 9.0 * y - 3.0;
     t.a4 = 9.0 * x - 4.0; t.b4 = 9.0 * y - 7.0;
-    t.F1 = spac_exp_finite(-0.25 * (t.a1 * t.a1 + t.b1 * t.b1));
-    t.F2 = spac_exp_finite((-1.0 / 49.0) * (t.a2 * t.a2) - 0.1 * (9.0 * y + 1.0));
-    t.F3 = spac_exp_finite(-0.25 * (t.a3 * t.a3 + t.b3 * t.b3));
-    t.F4 = spac_exp_finite(-1.0 * (t.a4 * t.a4 + t.b4 * t.b4));
+    t.s1 = t.a1 * t.a1 + t.b1 * t.b1;
+    t.q2 = t.a2 * t.a2;
+    t.s3 = t.a3 * t.a3 + t.b3 * t.b3;
+    t.s4 = t.a4 * t.a4 + t.b4 * t.b4;
+    t.F1 = spac_exp_finite(-0.25 * t.s1);
+    t.F2 = spac_exp_finite((-1.0 / 49.0) * t.q2 - 0.1 * (9.0 * y + 1.0));
+    t.F3 = spac_exp_finite(-0.25 * t.s3);
+    t.F4 = spac_exp_finite(-1.0 * t.s4);
     return t;
 }
 __device__ __forceinline__ double franke_u(const FrankeTerms& t) {
@@ -57,16 +62,16 @@ __device__ __forceinline__ double franke_ux(const FrankeTerms& t) {
 __device__ __forceinline__ double franke_uy(const FrankeTerms& t) {
     return -3.375 * t.b1 * t.F1 - 0.675 * t.F2 - 2.25 * t.b3 * t.F3 + 3.6 * t.b4 * t.F4;
 }
-__device__ __forceinline__ double franke_uxx(const FrankeTerms& t) {
-    return -3.375 * (9.0 - 4.5 * (t.a1 * t.a1)) * t.F1
-           - (27.0 / 98.0) * (9.0 - (18.0 / 49.0) * (t.a2 * t.a2)) * t.F2
-           - 2.25 * (9.0 - 4.5 * (t.a3 * t.a3)) * t.F3
-           + 3.6 * (9.0 - 18.0 * (t.a4 * t.a4)) * t.F4;
-}
-__device__ __forceinline__ double franke_uyy(const FrankeTerms& t) {
-    return -3.375 * (9.0 - 4.5 * (t.b1 * t.b1)) * t.F1
-           + (243.0 / 400.0) * t.F2 - 2.25 * (9.0 - 4.5 * (t.b3 * t.b3)) * t.F3
-           + 3.6 * (9.0 - 18.0 * (t.b4 * t.b4)) * t.F4;
+/* u_xx + u_yy with one factor per Gaussian, linear in its squared distance: the sum of the
+ * reference's u_xx and u_yy terms (BVPs/Franke_Poisson.hpp), e.g. for F1
+ * -3.375 (9 - 4.5 a1^2) - 3.375 (9 - 4.5 b1^2) = 15.1875 s1 - 60.75; for F2
+ * (243/2401) q2 + (243/400 - 243/98), the constants rounded once from the exact rationals. */
+__device__ __forceinline__ double franke_lap(const FrankeTerms& t) {
+    const double A1 = SPAC_FMA(15.1875, t.s1, -60.75);
+    const double A2 = SPAC_FMA(0.10120783007080383, t.q2, -1.872091836734694);
+    const double A3 = SPAC_FMA(10.125, t.s3, -40.5);
+    const double A4 = SPAC_FMA(-64.8, t.s4, 64.8);
+    return SPAC_FMA(A1, t.F1, SPAC_FMA(A2, t.F2, SPAC_FMA(A3, t.F3, A4 * t.F4)));
 }
 
 /* Franke Poisson (BVPs/Franke_Poisson.hpp): sigma = 1.41421356237 I, b = 0,
@@ -84,7 +89,7 @@ struct CoefFranke {
         StepCoef c;
         c.s = kSigma;
         c.bx = 0.0; c.by = 0.0; c.c = 0.0;
-        c.f = -franke_uxx(t) - franke_uyy(t);
+        c.f = -franke_lap(t);
         c.Fx = -(kSigma * franke_ux(t));
         c.Fy = -(kSigma * franke_uy(t));
         return c;
@@ -130,7 +135,7 @@ struct CoefVarFranke {
         c.by = -0.3 * x;
         c.c = -1.0 - x * x;
         double ux = franke_ux(t), uy = franke_uy(t);
-        double lap = franke_uxx(t) + franke_uyy(t);
+        double lap = franke_lap(t);
         double adv = c.bx * ux + c.by * uy;
         double reac = c.c * franke_u(t);
         double diff = (0.5 * (s * s)) * lap;
@@ -294,13 +299,13 @@ __device__ __forceinline__ float franke_ux32(const FrankeTerms32& t) {
 __device__ __forceinline__ float franke_uy32(const FrankeTerms32& t) {
     return -3.375f * t.b1 * t.F1 - 0.675f * t.F2 - 2.25f * t.b3 * t.F3 + 3.6f * t.b4 * t.F4;
 }
+/* the factors of franke_lap in FP32 */
 __device__ __forceinline__ float franke_lap32(const FrankeTerms32& t) {
-    float uxx = -3.375f * (9.0f - 4.5f * (t.a1 * t.a1)) * t.F1 -
-                (27.0f / 98.0f) * (9.0f - (18.0f / 49.0f) * (t.a2 * t.a2)) * t.F2 -
-                2.25f * (9.0f - 4.5f * (t.a3 * t.a3)) * t.F3 + 3.6f * (9.0f - 18.0f * (t.a4 * t.a4)) * t.F4;
-    float uyy = -3.375f * (9.0f - 4.5f * (t.b1 * t.b1)) * t.F1 + (243.0f / 400.0f) * t.F2 -
-                2.25f * (9.0f - 4.5f * (t.b3 * t.b3)) * t.F3 + 3.6f * (9.0f - 18.0f * (t.b4 * t.b4)) * t.F4;
-    return uxx + uyy;
+    const float A1 = 15.1875f * (t.a1 * t.a1 + t.b1 * t.b1) - 60.75f;
+    const float A2 = 0.10120783f * (t.a2 * t.a2) - 1.8720918f;
+    const float A3 = 10.125f * (t.a3 * t.a3 + t.b3 * t.b3) - 40.5f;
+    const float A4 = 64.8f - 64.8f * (t.a4 * t.a4 + t.b4 * t.b4);
+    return A1 * t.F1 + A2 * t.F2 + A3 * t.F3 + A4 * t.F4;
 }
 __device__ __forceinline__ float franke_u32(const FrankeTerms32& t) {
     return 0.75f * t.F1 + 0.75f * t.F2 + 0.5f * t.F3 - 0.2f * t.F4;

@@ -13,16 +13,19 @@
  * therefore produce the same bits, which is what lets per-path exit data be
  * compared exactly between the HIP kernels and the CPU restatement.
  *
- * Methods (all from the mathematics; constants derived with exact rational
- * arithmetic):
+ * Methods (all from the mathematics).  The polynomials are Chebyshev fits of
+ * the remainder after each function's exact leading terms, the shortest whose
+ * approximation error is below 0.02 ulp (derive_coefficients.py, mpmath at 60
+ * digits, printed as the literals below); atan's series are Taylor, with
+ * coefficients rounded from exact rationals:
  *   exp   : Cody–Waite reduction x = k ln2 + r with a 32-bit ln2 head,
- *           Taylor polynomial of degree 13 on |r| <= ln2/2, scaling by 2^k.
+ *           e^r = 1 + r + r^2 Q(r) on |r| <= ln2/2 (Q of degree 10), scaling by 2^k.
  *   log   : x = 2^e m, m in [sqrt(2)/2, sqrt(2)), f = (m-1)/(m+1),
- *           log m = 2 atanh f as an odd series in f (12 terms).
- *   sincospi : exact reduction a = q/2 + r, |r| <= 1/4, Taylor series of
- *           sin(pi r) and cos(pi r), quadrant rotation by q mod 4.
- *   sinpi, cospi : exact reduction a = n + r, |r| <= 1/2, one Taylor series of
- *           sin(pi t) to t^23 each (cos(pi r) = sin(pi (1/2 - |r|))).
+ *           log m = 2f + f^3 R(f^2) (R of degree 7).
+ *   sincospi : exact reduction a = q/2 + r, |r| <= 1/4, sin(pi r) = pi r + r^3 S(r^2)
+ *           and cos(pi r) = 1 + r^2 C(r^2) (degree 6 each), quadrant rotation by q mod 4.
+ *   sinpi, cospi : exact reduction a = n + r, |r| <= 1/2, one polynomial each,
+ *           sin(pi t) = pi t + t^3 P(t^2) (P of degree 7; cos(pi r) = sin(pi (1/2 - |r|))).
  *   atan2 : octant reduction to t in [0,1], t > tan(pi/8) mapped by
  *           atan t = pi/4 + atan((t-1)/(t+1)), one half-angle step
  *           atan u = 2 atan(u / (1 + sqrt(1+u^2))), odd Taylor series.
@@ -56,6 +59,22 @@
 #define SPAC_PI 3.141592653589793
 #define SPAC_PI_2 1.5707963267948966
 #define SPAC_PI_4 0.7853981633974483
+
+/* SPAC_FMAK(p, x, c): a Horner step p x + c whose constant c the compiler keeps in a VGPR across
+ * the path loop (the log and sincospi coefficients: the loop's scalar registers are full).  With
+ * the builtin it picks the two-address v_fmac_f64 and copies c into the destination first, one
+ * extra v_mov_b64 per step; the three-address v_fma_f64 written directly reads c in place.  Same
+ * operation, same bits. */
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double spac_fmak(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+#define SPAC_FMAK(a, b, c) spac_fmak((a), (b), (c))
+#else
+#define SPAC_FMAK(a, b, c) SPAC_FMA(a, b, c)
+#endif
 
 /* ---------------------------------------------------------------- bits -- */
 SPAC_HD double spac_u64_as_f64(uint64_t u) {
@@ -96,17 +115,16 @@ SPAC_HD double spac_exp_finite(double x) {
     const double kd = SPAC_RINT(xc * LOG2E);
     double r = SPAC_FMA(-kd, LN2_HI, xc);
     r = SPAC_FMA(-kd, LN2_LO, r);
-    double p = 1.6059043836821613e-10;
-    p = SPAC_FMA(p, r, 2.08767569878681e-09);
-    p = SPAC_FMA(p, r, 2.505210838544172e-08);
-    p = SPAC_FMA(p, r, 2.755731922398589e-07);
-    p = SPAC_FMA(p, r, 2.7557319223985893e-06);
-    p = SPAC_FMA(p, r, 2.48015873015873e-05);
-    p = SPAC_FMA(p, r, 0.0001984126984126984);
-    p = SPAC_FMA(p, r, 0.001388888888888889);
-    p = SPAC_FMA(p, r, 0.008333333333333333);
-    p = SPAC_FMA(p, r, 0.041666666666666664);
-    p = SPAC_FMA(p, r, 0.16666666666666666);
+    double p = 2.0914679452495193e-09;
+    p = SPAC_FMA(p, r, 2.5105206480259365e-08);
+    p = SPAC_FMA(p, r, 2.755727366116628e-07);
+    p = SPAC_FMA(p, r, 2.755725542549107e-06);
+    p = SPAC_FMA(p, r, 2.480158732553351e-05);
+    p = SPAC_FMA(p, r, 0.00019841269874800694);
+    p = SPAC_FMA(p, r, 0.0013888888888883752);
+    p = SPAC_FMA(p, r, 0.008333333333326141);
+    p = SPAC_FMA(p, r, 0.04166666666666667);
+    p = SPAC_FMA(p, r, 0.1666666666666667);
     p = SPAC_FMA(p, r, 0.5);
     p = SPAC_FMA(p, r, 1.0);
     p = SPAC_FMA(p, r, 1.0);
@@ -123,18 +141,17 @@ SPAC_HD double spac_exp(double x) {
     double kd = SPAC_RINT(xc * LOG2E);
     double r = SPAC_FMA(-kd, LN2_HI, xc); /* exact: |k| <= 1076 needs 11 bits */
     r = SPAC_FMA(-kd, LN2_LO, r);
-    /* e^r = 1 + r + r^2/2! + ... + r^13/13!, |r| <= 0.3466 */
-    double p = 1.6059043836821613e-10;
-    p = SPAC_FMA(p, r, 2.08767569878681e-09);
-    p = SPAC_FMA(p, r, 2.505210838544172e-08);
-    p = SPAC_FMA(p, r, 2.755731922398589e-07);
-    p = SPAC_FMA(p, r, 2.7557319223985893e-06);
-    p = SPAC_FMA(p, r, 2.48015873015873e-05);
-    p = SPAC_FMA(p, r, 0.0001984126984126984);
-    p = SPAC_FMA(p, r, 0.001388888888888889);
-    p = SPAC_FMA(p, r, 0.008333333333333333);
-    p = SPAC_FMA(p, r, 0.041666666666666664);
-    p = SPAC_FMA(p, r, 0.16666666666666666);
+    /* e^r = 1 + r + r^2 Q(r), |r| <= 0.3466, Q of degree 10 */
+    double p = 2.0914679452495193e-09;
+    p = SPAC_FMA(p, r, 2.5105206480259365e-08);
+    p = SPAC_FMA(p, r, 2.755727366116628e-07);
+    p = SPAC_FMA(p, r, 2.755725542549107e-06);
+    p = SPAC_FMA(p, r, 2.480158732553351e-05);
+    p = SPAC_FMA(p, r, 0.00019841269874800694);
+    p = SPAC_FMA(p, r, 0.0013888888888883752);
+    p = SPAC_FMA(p, r, 0.008333333333326141);
+    p = SPAC_FMA(p, r, 0.04166666666666667);
+    p = SPAC_FMA(p, r, 0.1666666666666667);
     p = SPAC_FMA(p, r, 0.5);
     p = SPAC_FMA(p, r, 1.0);
     p = SPAC_FMA(p, r, 1.0);
@@ -165,17 +182,14 @@ SPAC_HD double spac_log_normal(double x) {
     e += hi;
     double f = (m - 1.0) / (m + 1.0);
     double s = f * f;
-    double p = 0.08695652173913043;
-    p = SPAC_FMA(p, s, 0.09523809523809523);
-    p = SPAC_FMA(p, s, 0.10526315789473684);
-    p = SPAC_FMA(p, s, 0.11764705882352941);
-    p = SPAC_FMA(p, s, 0.13333333333333333);
-    p = SPAC_FMA(p, s, 0.15384615384615385);
-    p = SPAC_FMA(p, s, 0.18181818181818182);
-    p = SPAC_FMA(p, s, 0.2222222222222222);
-    p = SPAC_FMA(p, s, 0.2857142857142857);
-    p = SPAC_FMA(p, s, 0.4);
-    p = SPAC_FMA(p, s, 0.6666666666666666);
+    double p = 0.13086626149248715;
+    p = SPAC_FMAK(p, s, 0.1326877313852219);
+    p = SPAC_FMAK(p, s, 0.15386239702819687);
+    p = SPAC_FMAK(p, s, 0.18181795640132814);
+    p = SPAC_FMAK(p, s, 0.2222222239171392);
+    p = SPAC_FMAK(p, s, 0.28571428570803614);
+    p = SPAC_FMAK(p, s, 0.4000000000000088);
+    p = SPAC_FMAK(p, s, 0.6666666666666666);
     double logm = SPAC_FMA(f * s, p, 2.0 * f);
     double ed = (double)e;
     return SPAC_FMA(ed, LN2_HI, SPAC_FMA(ed, LN2_LO, logm));
@@ -196,17 +210,14 @@ SPAC_HD double spac_log(double x) {
     e += hi + (sub ? -54 : 0);
     double f = (m - 1.0) / (m + 1.0);
     double s = f * f;
-    /* log m = 2f (1 + s/3 + s^2/5 + ... + s^11/23) */
-    double p = 0.08695652173913043;
-    p = SPAC_FMA(p, s, 0.09523809523809523);
-    p = SPAC_FMA(p, s, 0.10526315789473684);
-    p = SPAC_FMA(p, s, 0.11764705882352941);
-    p = SPAC_FMA(p, s, 0.13333333333333333);
-    p = SPAC_FMA(p, s, 0.15384615384615385);
-    p = SPAC_FMA(p, s, 0.18181818181818182);
-    p = SPAC_FMA(p, s, 0.2222222222222222);
-    p = SPAC_FMA(p, s, 0.2857142857142857);
-    p = SPAC_FMA(p, s, 0.4);
+    /* log m = 2f + f s R(s), R of degree 7 */
+    double p = 0.13086626149248715;
+    p = SPAC_FMA(p, s, 0.1326877313852219);
+    p = SPAC_FMA(p, s, 0.15386239702819687);
+    p = SPAC_FMA(p, s, 0.18181795640132814);
+    p = SPAC_FMA(p, s, 0.2222222239171392);
+    p = SPAC_FMA(p, s, 0.28571428570803614);
+    p = SPAC_FMA(p, s, 0.4000000000000088);
     p = SPAC_FMA(p, s, 0.6666666666666666);
     double logm = SPAC_FMA(f * s, p, 2.0 * f);
     double ed = (double)e;
@@ -221,25 +232,21 @@ SPAC_HD void spac_sincospi(double a, double* s_out, double* c_out) {
     double q = SPAC_RINT(a * 2.0);
     double r = a - 0.5 * q; /* exact, |r| <= 1/4 */
     double r2 = r * r;
-    double sp = -2.2948428997269873e-08;
-    sp = SPAC_FMA(sp, r2, 7.952054001475513e-07);
-    sp = SPAC_FMA(sp, r2, -2.1915353447830217e-05);
-    sp = SPAC_FMA(sp, r2, 0.00046630280576761255);
-    sp = SPAC_FMA(sp, r2, -0.0073704309457143504);
-    sp = SPAC_FMA(sp, r2, 0.08214588661112823);
-    sp = SPAC_FMA(sp, r2, -0.5992645293207921);
-    sp = SPAC_FMA(sp, r2, 2.5501640398773455);
-    sp = SPAC_FMA(sp, r2, -5.16771278004997);
+    double sp = -2.1742067342478005e-05;
+    sp = SPAC_FMAK(sp, r2, 0.0004662879326147481);
+    sp = SPAC_FMAK(sp, r2, -0.0073704303122948235);
+    sp = SPAC_FMAK(sp, r2, 0.08214588659727676);
+    sp = SPAC_FMAK(sp, r2, -0.5992645293206478);
+    sp = SPAC_FMAK(sp, r2, 2.550164039877345);
+    sp = SPAC_FMAK(sp, r2, -5.16771278004997);
     double sr = SPAC_FMA(r * r2, sp, r * 3.141592653589793);
-    double cp = -1.3878952462213771e-07;
-    cp = SPAC_FMA(cp, r2, 4.303069587032947e-06);
-    cp = SPAC_FMA(cp, r2, -0.0001046381049248457);
-    cp = SPAC_FMA(cp, r2, 0.0019295743094039231);
-    cp = SPAC_FMA(cp, r2, -0.02580689139001406);
-    cp = SPAC_FMA(cp, r2, 0.2353306303588932);
-    cp = SPAC_FMA(cp, r2, -1.3352627688545895);
-    cp = SPAC_FMA(cp, r2, 4.0587121264167685);
-    cp = SPAC_FMA(cp, r2, -4.934802200544679);
+    double cp = -0.00010370082971594627;
+    cp = SPAC_FMAK(cp, r2, 0.001929493874894941);
+    cp = SPAC_FMAK(cp, r2, -0.025806887964701878);
+    cp = SPAC_FMAK(cp, r2, 0.23533063028399223);
+    cp = SPAC_FMAK(cp, r2, -1.3352627688538095);
+    cp = SPAC_FMAK(cp, r2, 4.058712126416765);
+    cp = SPAC_FMAK(cp, r2, -4.934802200544679);
     double cr = SPAC_FMA(cp, r2, 1.0);
     int64_t qi = (int64_t)q;
     int quad = (int)(qi & 3);
@@ -251,20 +258,16 @@ SPAC_HD void spac_sincospi(double a, double* s_out, double* c_out) {
     *s_out = s;
     *c_out = c;
 }
-/* sin(pi t) for |t| <= 1/2: Taylor series to t^23 (truncation below 6e-21 at |t| = 1/2),
- * coefficients (-1)^k pi^(2k+1) / (2k+1)! rounded from exact rational arithmetic. */
+/* sin(pi t) for |t| <= 1/2: pi t + t^3 P(t^2), P of degree 7 (approximation error 0.012 ulp). */
 SPAC_HD double spac_sinpi_half(double t) {
     const double t2 = t * t;
-    double p = -1.0518471716932065e-11;
-    p = SPAC_FMA(p, t2, 5.392664662608129e-10);
-    p = SPAC_FMA(p, t2, -2.2948428997269873e-08);
-    p = SPAC_FMA(p, t2, 7.952054001475513e-07);
-    p = SPAC_FMA(p, t2, -2.1915353447830217e-05);
-    p = SPAC_FMA(p, t2, 0.00046630280576761255);
-    p = SPAC_FMA(p, t2, -0.0073704309457143504);
-    p = SPAC_FMA(p, t2, 0.08214588661112823);
-    p = SPAC_FMA(p, t2, -0.5992645293207921);
-    p = SPAC_FMA(p, t2, 2.5501640398773455);
+    double p = 7.725743030789876e-07;
+    p = SPAC_FMA(p, t2, -2.1906201655131958e-05);
+    p = SPAC_FMA(p, t2, 0.00046630087411496363);
+    p = SPAC_FMA(p, t2, -0.007370430719634586);
+    p = SPAC_FMA(p, t2, 0.08214588659675232);
+    p = SPAC_FMA(p, t2, -0.599264529320343);
+    p = SPAC_FMA(p, t2, 2.55016403987734);
     p = SPAC_FMA(p, t2, -5.16771278004997);
     /* pi t as hi + lo exactly (an FMA gives the product's rounding error), plus pi's own tail:
        near |t| = 1/2 the rounding of pi t alone is an ulp of the result */
@@ -402,11 +405,13 @@ SPAC_HD double spac_exponential(uint64_t seed, uint32_t stream, uint32_t path, u
 
 /* ---------------------------------------------------------- stop test -- */
 /* The Gobet–Menozzi stop test of Dentro (Solvers/FeynmanKacSolver.cpp:84,96), "inside" when
- * d < g |N^T sigma| sqrt(h), from k = g sqrt(h) and nrm2 = |N^T sigma|^2.  With k <= 0 (the
- * reference's g = -0.5826) both sides of d < k |.| are non-positive, so the test is d < 0 and
- * d^2 > k^2 nrm2, with no square root. */
+ * d < g |N^T sigma| sqrt(h), from k = g sqrt(h) and nrm2 = |N^T sigma|^2, compared in squares
+ * (no square root).  With k <= 0 (the reference's g = -0.5826) both sides of d < k |.| are
+ * non-positive: d < 0 and d^2 > k^2 nrm2.  With k > 0 the right side is non-negative: d < 0, or
+ * d >= 0 and d^2 < k^2 nrm2. */
 SPAC_HD int spac_gobet_inside(double d, double k, double nrm2) {
-    return (k <= 0.0) ? (d < 0.0 && d * d > (k * k) * nrm2) : (d < k * SPAC_SQRT(nrm2));
+    const double d2 = d * d, r2 = (k * k) * nrm2;
+    return (k <= 0.0) ? (d < 0.0 && d2 > r2) : (d < 0.0 || d2 < r2);
 }
 
 /* -------------------------------------------- compensated accumulation -- */

@@ -40,7 +40,8 @@ def _debug_math(fn, x, y):
 
 
 @pytest.mark.parametrize("fn,lo,hi", [(0, -745, 709), (1, 1e-300, 10.0), (2, -4, 4), (3, -4, 4), (4, -3, 3),
-                                      (5, 0, 1e6), (6, -5, 5), (9, -760, 712), (10, 2.3e-308, 10.0)])
+                                      (5, 0, 1e6), (6, -5, 5), (9, -760, 712), (10, 2.3e-308, 10.0),
+                                      (11, -0.1, 0.1)])
 def test_device_math_bits_equal_cpu(gpu, fn, lo, hi):
     say(f"device math fn={fn}")
     rng = np.random.default_rng(fn)

@@ -119,3 +119,21 @@ def test_normals_pairs_independent(seed):
     z = o.normals(seed, 5, 9, 20000).reshape(-1, 2)
     r = np.corrcoef(z[:, 0], z[:, 1])[0, 1]
     assert abs(r) < 4.0 / np.sqrt(z.shape[0])
+
+
+def test_gobet_test_in_squares_equals_square_root_form():
+    """Dentro's stop test d < k |N^T sigma| compared in squares, for both signs of k, against
+    the square-root form wherever the two sides are not within rounding of each other."""
+    rng = np.random.default_rng(11)
+    n = 200000
+    k = rng.uniform(-0.05, 0.05, n)
+    d = rng.uniform(-0.1, 0.1, n)
+    d[:2000] = 0.0
+    k[2000:4000] = 0.0
+    rhs = k * np.sqrt(1.7)
+    want = d < rhs
+    got = o.math_eval(11, d, k) == 1.0
+    far = np.abs(d - rhs) > 1e-12 * (np.abs(d) + np.abs(rhs)) + 1e-300
+    far |= (d == 0.0) | (k == 0.0)
+    assert far.mean() > 0.99
+    assert np.array_equal(got[far], want[far])

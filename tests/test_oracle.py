@@ -64,6 +64,16 @@ def test_franke_values_and_derivatives():
     assert np.allclose(o.coef_eval(FR, 4, x, y), s)
 
 
+def test_franke_laplacian_combined_equals_term_by_term():
+    """The solver's Laplacian (one factor per Gaussian, linear in the squared distance) against
+    the reference's u_xx + u_yy evaluated term by term: equal up to rounding of O(100) terms."""
+    rng = np.random.default_rng(5)
+    x, y = rng.uniform(-1, 1.2, 20000), rng.uniform(-1, 1.2, 20000)
+    comb, terms = o.coef_eval(FR, 10, x, y), o.coef_eval(FR, 15, x, y)
+    assert np.abs(comb - terms).max() <= 2e-13
+    assert np.abs(comb - terms).max() > 0  # a different rounding, not the same expression
+
+
 def test_varcoef_operator_manufactured():
     rng = np.random.default_rng(1)
     x, y = rng.uniform(0, 1, 300), rng.uniform(0, 1, 300)
