@@ -47,14 +47,15 @@ def log(msg):
 # Commands and figures: profiles/fk_path_kernel_fp64_franke_points_256x10k.md,
 # profiles/pddsparse_rows_fp64_4x4_1952rows_20k.md, profiles/learned_cv_time_to_rms_64x64_fp32.md.
 PATH_KERNEL_TRAFFIC_BYTES = {
-    "points": (85.5 * 2.0 + 507894.5) * 1024,
+    "points": (87.8 * 2.0 + 553664.1) * 1024,
     "rows": (25955.9 * 2.0 + 8102934.9) * 1024,
 }
-# The learned-CV path kernel of the full-scale system, per path of the launch: measured at 72
-# paths per row (13,458,240 paths; FETCH_SIZE 186,282.5 KiB x 2.0, WRITE_SIZE 3,169,083.3 KiB
-# x 1.0, mean of the last four dispatches); its launches in the bench run at the calibrated
-# count, so the bytes per launch are this figure times the launch's paths.
-LEARNED_PATH_KERNEL_BYTES_PER_PATH = (186282.5 * 2.0 + 3169083.3) * 1024 / (186920 * 72)
+# The learned-CV path kernel of the full-scale system, per path of the launch: measured at 94
+# paths per row (17,570,480 paths; FETCH_SIZE 185,228.3 KiB x 2.0, WRITE_SIZE 4,170,522.8 KiB
+# x 1.0, mean of two dispatches of `--workload assemble --paths-per-row 94`); its launches in
+# the bench run at the calibrated count, so the bytes per launch are this figure times the
+# launch's paths.
+LEARNED_PATH_KERNEL_BYTES_PER_PATH = (185228.3 * 2.0 + 4170522.8) * 1024 / (186920 * 94)
 RECORD_BYTES_PER_PATH = 6 * 8 + 4 + 1  # x_tau, y_tau, Y, Z, xi, tau; steps; outcome
 COMPUTE_UNIT_64 = "VALU FP64 (no matrix-core work on this path; FP64 vector peak = dense FP64 MFMA peak = 78.6 TFLOP/s)"
 COMPUTE_UNIT_32 = "VALU FP32 (no matrix-core work in the path kernel; MI355X FP32 vector peak 157.3 TFLOP/s)"
@@ -600,6 +601,58 @@ def bench_ttr(args):
     return res, roof, mesh
 
 
+def bench_assemble(args):
+    """Assemblies of the full-scale system at a fixed path count, for kernel profiles of the FP32
+    path kernel: the learned control variate trained by the recipe (CV_TABLE with the fallback),
+    or plain MC with --plain; --warmup untimed and --steps timed assemblies of every row into the
+    device-resident system at --paths-per-row paths and the workload's h."""
+    from spac_amd.mlp import MLP
+
+    mesh, h = full_scale()
+    system = DeviceSystem(mesh)
+    prob = FKProblem(problem_desc(coef=VC, domain=UNIT_SQUARE, precision=abi.SPAC_FP32))
+    net = None
+    if args.plain:
+        prob.set_control_variate(abi.SPAC_CV_NONE)
+    else:
+        cfg = {k: v for k, v in LEARNED_CV.items() if k not in ("seed", "first_layer_scale")}
+        net = MLP(UNIT_SQUARE, seed=LEARNED_CV["seed"], first_layer_scale=LEARNED_CV["first_layer_scale"])
+        train_learned_cv(net, **cfg, eager=args.train_eager)
+        net.attach(prob, fallback=True)
+    N = args.paths_per_row
+    st = prob.assemble_system_uniform(system, N, h, seed=1, stats=True)
+    if not (st["status"] == 0).all():
+        raise RuntimeError(f"assembly status {int(st['status'].min())}")
+    path_steps = int(st["path_steps"].astype(np.int64).sum())
+    for k in range(args.warmup):
+        prob.assemble_system_uniform(system, N, h, seed=2 + k)
+    wall, dev, kern = [], [], []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        prob.assemble_system_uniform(system, N, h, seed=100 + k)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        t = prob.timing()
+        dev.append(t.total_ms)
+        kern.append(t.path_kernel_ms)
+    fps = FLOPS_PER_STEP["varcoef_circle_fp32_plain" if args.plain else "varcoef_circle_fp32_table"]
+    kms = float(np.mean(kern))
+    achieved = path_steps * fps / (kms * 1e-3) / 1e12
+    out = {"workload": "full-scale system assembly, FP32 stepping: 64x64 circles, 186,920 rows, "
+                       f"{N} paths per row, h={h:.4g}, " + ("plain MC" if args.plain else "learned CV (table)"),
+           "paths_per_row": N, "path_steps_per_assembly": path_steps, "assembly_wall_ms": float(np.mean(wall)),
+           "assembly_device_ms": float(np.mean(dev)), "path_kernel_ms": kms,
+           "path_steps_per_s_path_kernel": path_steps / (kms * 1e-3),
+           "roofline": {"bound": "mfma", "compute_unit": COMPUTE_UNIT_32, "achieved": achieved,
+                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                        "flops_per_path_step": fps},
+           "fallback_rows": int(st["cv_fallback"].sum())}
+    prob.close()
+    if net is not None:
+        net.close()
+    system.close()
+    return out
+
+
 def cpu_baseline_ttr(mesh, h, plain, budget_s):
     """The CPU restatement (FP64, plain Monte Carlo, OpenMP over paths) on a sample of the
     full-scale rows at the plain GPU count; its time to RMS 1e-3 is projected from the measured
@@ -653,8 +706,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed steps (ttr: learned-CV solves end to end)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="ttr", choices=["ttr", "rows", "points", "batched", "train"],
-                    help="ttr (default: the headline line) | rows | points | batched | train")
+    ap.add_argument("--workload", default="ttr", choices=["ttr", "rows", "points", "batched", "train", "assemble"],
+                    help="ttr (default: the headline line) | rows | points | batched | train | assemble")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extras", action="store_true", help="ttr: skip the rows and batched workloads")
@@ -664,6 +717,8 @@ def main():
     ap.add_argument("--ttr-refine-replicas", type=int, default=32, help="replicas of each later set")
     ap.add_argument("--bootstrap", type=int, default=4000)
     ap.add_argument("--train-steps", type=int, default=LEARNED_CV["steps"])
+    ap.add_argument("--paths-per-row", type=int, default=90, help="assemble: paths per row")
+    ap.add_argument("--plain", action="store_true", help="assemble: plain MC instead of the learned CV")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run plain device copies of this many MiB (counter calibration for profiles)")
     ap.add_argument("--train-eager", action="store_true",
@@ -678,6 +733,9 @@ def main():
         copy_calibration(args.copy_mib)
     if args.workload == "batched":
         print(json.dumps({"batched": bench_batched(steps=args.steps)}), flush=True)
+        return
+    if args.workload == "assemble":
+        print(json.dumps({"assemble": bench_assemble(args)}), flush=True)
         return
     if args.workload == "train":
         print(json.dumps({"train": bench_train(steps=args.train_steps, eager=args.train_eager)}), flush=True)
