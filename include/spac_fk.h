@@ -363,7 +363,13 @@ spac_status spac_fk_last_timing(spac_problem* p, spac_timing* out);
 /* Work-item order of the persistent kernel's queue. */
 enum {
     SPAC_ORDER_NODE = 0, /* items in node order */
-    SPAC_ORDER_COST = 1  /* items of the nodes with the smallest h (most steps per path) first */
+    SPAC_ORDER_COST = 1, /* items of the nodes with the smallest h (most steps per path) first */
+    SPAC_ORDER_TAIL = 2, /* items of the solves whose exit times have the longest tail first: the
+                            key R^2 / (sigma(X0)^2 h), R the solve's circle radius (1 without a
+                            circle), bucketed by eighth octaves on the device */
+    SPAC_ORDER_AUTO = 3  /* a new problem's default: SPAC_ORDER_TAIL for launches of at most 256
+                            paths per lane (where the longest paths set the end of the launch),
+                            SPAC_ORDER_NODE above */
 };
 
 /* Launch shape and refill schedule of the persistent path kernel (0 = library
@@ -372,7 +378,7 @@ typedef struct {
     int32_t block_size;       /* threads per block (multiple of 64) */
     int32_t blocks_per_cu;    /* persistent blocks per compute unit */
     int32_t chunk_paths;      /* paths per work-queue item */
-    int32_t item_order;       /* SPAC_ORDER_* */
+    int32_t item_order;       /* SPAC_ORDER_* (0 is SPAC_ORDER_NODE; a new problem starts with SPAC_ORDER_AUTO) */
 } spac_launch_config;
 spac_status spac_fk_set_launch(spac_problem* p, const spac_launch_config* cfg);
 

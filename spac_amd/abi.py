@@ -48,6 +48,8 @@ SPAC_RBF_IMQ_GLOBAL = 0
 SPAC_RBF_GAUSS_STENCIL = 1
 SPAC_ORDER_NODE = 0
 SPAC_ORDER_COST = 1
+SPAC_ORDER_TAIL = 2
+SPAC_ORDER_AUTO = 3
 SPAC_CV_ANALYTIC = 0
 SPAC_CV_NONE = 1
 SPAC_CV_TABLE = 2

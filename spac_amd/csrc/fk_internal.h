@@ -42,6 +42,8 @@ struct SolveSet {
     const int64_t* rec_off = nullptr;
     const double* circle = nullptr;
     const int4* items = nullptr;
+    int4* items_tail = nullptr;    /* the items in tail-first order (SPAC_ORDER_TAIL), or nullptr */
+    int32_t* tail_hist = nullptr;  /* kTailBlocks x kTailBuckets counts, then offsets */
     int32_t n_items = 0;
     int32_t chunks_per_solve = 0;
     uint32_t chunk = 0;            /* paths per work item (see work_item_paths) */
@@ -77,6 +79,7 @@ struct Problem {
     /* solve table, work queue, outputs */
     DeviceBuffer sx, sy, sh, sstream, spaths, soff, scircle, sitems, queue, stats;
     DeviceBuffer ssq, ssig, ssigf; /* per solve sqrt(h) and sigma(X0) */
+    DeviceBuffer sitems_tail, stail_hist; /* the tail-first item order */
     /* interface tables and row outputs */
     DeviceBuffer ifc, ifnode, ifipsi, ifoff, rows_iface, rows_goff, rows_G, rows_part, rows_col;
     /* control variate (SPAC_CV_*) and its table (T x T x float4) */
@@ -160,12 +163,22 @@ struct Problem {
         stage_cap = 0;
         DeviceBuffer* all[] = {&rec_xt, &rec_yt, &rec_Y, &rec_Z, &rec_xi, &rec_tau, &rec_gval, &rec_steps,
                                &rec_outcome, &sx, &sy, &sh, &sstream, &spaths, &soff, &scircle, &sitems,
-                               &ssq, &ssig, &ssigf,
+                               &ssq, &ssig, &ssigf, &sitems_tail, &stail_hist,
                                &queue, &stats, &ifc, &ifnode, &ifipsi, &ifoff, &rows_iface, &rows_goff,
                                &rows_G, &rows_part, &rows_col, &cv_table, &cv_grad, &cv_net, &seed_dev, &tab, &tab32};
         for (DeviceBuffer* b : all) b->release();
     }
 };
+
+/* The tail-first item order: a counting sort of the work items into kTailBuckets buckets of
+ * log2(R^2 / (sigma(X0)^2 h)) in eighth octaves, over kTailBlocks slices of the item list. */
+constexpr int kTailBuckets = 256;
+constexpr int kTailBlocks = 64;
+constexpr int64_t kTailAutoPathsPerLane = 256; /* SPAC_ORDER_AUTO sorts launches up to this */
+
+/* Most work items one launch may queue: the queue counter runs past the last item by up to a
+ * claim per wave (at most 2^18), and it is an int32. */
+constexpr int64_t kMaxWorkItems = 0x7ff00000;
 
 /* Paths per work item of a launch over total_paths paths: the launch's chunk, enlarged for very
  * large batches so that the queue holds at most ~2^20 items (fk_kernels.hip). */

@@ -128,12 +128,113 @@ struct CoefMaker<CoefTab32<B>> {
 constexpr int kLaneBatch32 = SPAC_FP32_LANE_BATCH;
 constexpr int kLaneBatch64 = 1;
 
+/* A wave claims a run of work items with one atomic: several while many items remain (fewer
+ * atomics on the one queue counter), one near the end so that the last items stay balanced
+ * across waves.  Returns the first item (>= n_items: the queue is drained) and sets *end. */
+constexpr int32_t kMaxClaim = 4;
+__device__ __forceinline__ int32_t claim_items(const PathArgs& a, uint32_t lane, int32_t* end) {
+    const int32_t waves = (int32_t)(gridDim.x * (blockDim.x >> 6));
+    const int32_t left = a.n_items - *end; /* from the wave's last claim: a lower bound of what was left */
+    const int32_t want = max(1, min(kMaxClaim, left / (4 * waves)));
+    int32_t item = 0;
+    if (lane == 0) item = atomicAdd(a.queue, want);
+    item = __shfl(item, 0);
+    *end = min(item + want, a.n_items);
+    return item;
+}
+
+/* Paths that have taken many steps are the likely last ones of the launch (exit times have an
+ * exponential tail): a wave holding one issues ahead of the others on its SIMD, so the longest
+ * paths advance at close to their own latency while the bulk of the work fills the rest.  Used
+ * by the FP32 kernel, whose launches at a few tens of paths per row end on such paths; the FP64
+ * kernel measured no difference with it. */
+constexpr int32_t kOldPathSteps = 384;
+__device__ __forceinline__ void wave_priority_by_age(bool active, int32_t nstep) {
+    if (__ballot(active && nstep >= kOldPathSteps) != 0ull) __builtin_amdgcn_s_setprio(3);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 __global__ void zero_counter_kernel(int32_t* q) {
     if (threadIdx.x == 0) q[0] = 0;
+}
+
+/* Tail-first work order (SPAC_ORDER_TAIL).  The exit time of a path started at X0 in a region of
+ * size R has the tail P(tau > t) ~ exp(-lambda t), 1/lambda proportional to R^2 / sigma^2: the
+ * last paths to finish a launch come from the solves with the largest R^2 / (sigma(X0)^2 h) (in
+ * steps), and the launch ends sooner when those start first.  A counting sort of the items by
+ * that key, bucketed in eighth octaves, in three kernels without global atomics: histograms of
+ * contiguous item slices (one block each), bucket offsets (largest key first, then slice
+ * order), and the scatter through LDS cursors.  Only the queue order changes: records sit at
+ * fixed slots and the random stream is keyed by (solve, path), so results do not depend on it. */
+struct TailKey {
+    const double* sig0;    /* sigma(X0) per solve, or nullptr: 1 */
+    const double* h;       /* per solve, or nullptr: h_uniform */
+    double h_uniform;
+    const double* circle;  /* (R, cx, cy) per solve, or nullptr: R_uniform */
+    double R_uniform;
+};
+
+__device__ __forceinline__ int tail_bucket(const TailKey& k, int32_t solve) {
+    const double s = k.sig0 ? k.sig0[solve] : 1.0;
+    const double h = k.h ? k.h[solve] : k.h_uniform;
+    const double R = k.circle ? k.circle[3 * (int64_t)solve] : k.R_uniform;
+    const float key = (float)((R * R) / (s * s * h));
+    const float b = (key > 1.0f) ? __builtin_amdgcn_logf(key) * 8.0f : 0.0f; /* log2 */
+    return (int)__builtin_fminf(b, (float)(kTailBuckets - 1));
+}
+
+__device__ __forceinline__ void tail_slice(int32_t n, int32_t* i0, int32_t* i1) {
+    const int32_t per = (n + kTailBlocks - 1) / kTailBlocks;
+    *i0 = min(n, (int32_t)blockIdx.x * per);
+    *i1 = min(n, *i0 + per);
+}
+
+__global__ void __launch_bounds__(kTailBuckets) k_tail_hist(int32_t n, const int4* __restrict__ items, TailKey k,
+                                                             int32_t* __restrict__ hist) {
+    __shared__ int32_t cnt[kTailBuckets];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int32_t i0, i1;
+    tail_slice(n, &i0, &i1);
+    for (int32_t i = i0 + (int32_t)threadIdx.x; i < i1; i += kTailBuckets) atomicAdd(&cnt[tail_bucket(k, items[i].x)], 1);
+    __syncthreads();
+    hist[blockIdx.x * kTailBuckets + threadIdx.x] = cnt[threadIdx.x];
+}
+
+/* one block: hist[slice][bucket] becomes the first output slot of that slice's items of that
+ * bucket; buckets from the largest key down, slices in order within a bucket */
+__global__ void __launch_bounds__(kTailBuckets) k_tail_offsets(int32_t* __restrict__ hist) {
+    __shared__ int32_t tot[kTailBuckets];
+    const int b = threadIdx.x;
+    int32_t t = 0;
+    for (int s = 0; s < kTailBlocks; ++s) t += hist[s * kTailBuckets + b];
+    tot[b] = t;
+    __syncthreads();
+    int32_t base = 0;
+    for (int c = kTailBuckets - 1; c > b; --c) base += tot[c];
+    for (int s = 0; s < kTailBlocks; ++s) {
+        const int32_t v = hist[s * kTailBuckets + b];
+        hist[s * kTailBuckets + b] = base;
+        base += v;
+    }
+}
+
+__global__ void __launch_bounds__(kTailBuckets) k_tail_scatter(int32_t n, const int4* __restrict__ items, TailKey k,
+                                                                const int32_t* __restrict__ offs,
+                                                                int4* __restrict__ out) {
+    __shared__ int32_t cur[kTailBuckets];
+    cur[threadIdx.x] = offs[blockIdx.x * kTailBuckets + threadIdx.x];
+    __syncthreads();
+    int32_t i0, i1;
+    tail_slice(n, &i0, &i1);
+    for (int32_t i = i0 + (int32_t)threadIdx.x; i < i1; i += kTailBuckets) {
+        const int4 it = items[i];
+        out[atomicAdd(&cur[tail_bucket(k, it.x)], 1)] = it;
+    }
 }
 
 /* u at a subdomain exit for the statistics' score (Solvers/FeynmanKacSolver.cpp:254):
@@ -199,6 +300,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     /* wave-uniform pool */
     int32_t pool_solve = 0;
     uint32_t pool_next = 0, pool_end = 0;
+    int32_t pool_item = 0, pool_item_end = 0; /* the wave's claimed run of items */
     bool drained = false;
 
     while (true) {
@@ -227,10 +329,12 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             if (__popcll(idle) < kLaneBatch64 && __ballot(active) != 0ull) idle = 0ull; /* refill later, together */
         while (idle != 0ull && !drained) {
             if (pool_next >= pool_end) {
-                int32_t item = 0;
-                if (lane == 0) item = atomicAdd(a.queue, 1);
-                item = __shfl(item, 0);
-                if (item >= a.n_items) { drained = true; break; }
+                if (pool_item >= pool_item_end) {
+                    const int32_t item0 = claim_items(a, lane, &pool_item_end);
+                    if (item0 >= a.n_items) { drained = true; break; }
+                    pool_item = item0;
+                }
+                const int32_t item = pool_item++;
                 if (a.items) {
                     int4 it = a.items[item];
                     pool_solve = it.x;
@@ -446,6 +550,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
     int64_t rec = 0;
     int32_t pool_solve = 0;
     uint32_t pool_next = 0, pool_end = 0;
+    int32_t pool_item = 0, pool_item_end = 0; /* the wave's claimed run of items */
     bool drained = false;
     float domf[4] = {(float)a.dom[0], (float)a.dom[1], (float)a.dom[2], (float)a.dom[3]};
     const float gobetf = (float)a.gobet;
@@ -480,10 +585,12 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
         if (__popcll(idle) < kLaneBatch32 && __ballot(active) != 0ull) idle = 0ull; /* refill later, together */
         while (idle != 0ull && !drained) {
             if (pool_next >= pool_end) {
-                int32_t item = 0;
-                if (lane == 0) item = atomicAdd(a.queue, 1);
-                item = __shfl(item, 0);
-                if (item >= a.n_items) { drained = true; break; }
+                if (pool_item >= pool_item_end) {
+                    const int32_t item0 = claim_items(a, lane, &pool_item_end);
+                    if (item0 >= a.n_items) { drained = true; break; }
+                    pool_item = item0;
+                }
+                const int32_t item = pool_item++;
                 if (a.items) {
                     int4 it = a.items[item];
                     pool_solve = it.x;
@@ -527,6 +634,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
             idle = __ballot(!active && !done);
         }
         if (__ballot(active) == 0ull && __ballot(done) == 0ull) break;
+        wave_priority_by_age(active, nstep);
         if (active) {
             StepCoef32 cf = coef32.step_s(xf, yf, sigf);
             float n0, n1;
@@ -868,7 +976,7 @@ spac_status Problem::upload_solves(int32_t n, const double* x0, const double* y0
             return set_error(SPAC_E_INVALID_ARG, "solve %d has a circle of radius <= 0", k);
         n_items += (paths[k] + chunk - 1) / chunk;
     }
-    if (n_items > 0x7fffffffu) return set_error(SPAC_E_INVALID_ARG, "too many work items");
+    if (n_items > (size_t)kMaxWorkItems) return set_error(SPAC_E_INVALID_ARG, "too many work items");
     /* every array is built in (or copied to) one pinned staging block and goes up by DMA */
     const size_t nb = (size_t)n;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -979,6 +1087,12 @@ spac_status Problem::ensure_solve_scratch(int32_t n, SolveSet* ss) {
     if ((rc = ssigf.ensure((size_t)std::max(n, 1) * 4))) return rc;
     ss->sig0 = (double*)ssig.ptr;
     ss->sig0f = (float*)ssigf.ptr;
+    if (ss->items && ss->n_items > 1) { /* room for the tail-first order, whatever the launch order */
+        if ((rc = sitems_tail.ensure((size_t)ss->n_items * sizeof(int4)))) return rc;
+        if ((rc = stail_hist.ensure((size_t)kTailBlocks * kTailBuckets * 4))) return rc;
+        ss->items_tail = (int4*)sitems_tail.ptr;
+        ss->tail_hist = (int32_t*)stail_hist.ptr;
+    }
     return SPAC_OK;
 }
 
@@ -1086,6 +1200,26 @@ spac_status Problem::enqueue_paths(const SolveSet& ss, uint64_t seed, const uint
         else
             hipLaunchKernelGGL((fk_solve_sigma_kernel<CoefVarFranke, CoefVarFranke32>), dim3((ss.n + 255) / 256),
                                dim3(256), 0, stream, ss.n, ss.x0, ss.y0, ss.sig0, ss.sig0f, a.tab, a.tab32);
+    }
+    /* tail-first order: asked for, or by default where the longest paths set the end of the
+       launch (few paths per lane; measured: learned CV at 64 x 64 circles, 33 paths per lane,
+       9.3 -> 7.8 ms; an uneven batch of ~2,000 paths per lane 92 -> 99 ms, so node order there) */
+    const bool tail_order = launch.item_order == SPAC_ORDER_TAIL ||
+                            (launch.item_order == SPAC_ORDER_AUTO &&
+                             ss.total_paths <= kTailAutoPathsPerLane * (int64_t)blocks * (int64_t)block);
+    if (tail_order && ss.items && ss.items_tail && ss.tail_hist && ss.n_items > 1) {
+        TailKey k;
+        k.sig0 = (coef == SPAC_COEF_VARCOEF_FRANKE || coef == kCoefTabulated) ? ss.sig0 : nullptr;
+        k.h = ss.h;
+        k.h_uniform = ss.h_uniform;
+        k.circle = ss.circle;
+        k.R_uniform = (sub == SUB_CIRCLE && !ss.circle) ? desc.subdomain_params[0] : 1.0;
+        hipLaunchKernelGGL(k_tail_hist, dim3(kTailBlocks), dim3(kTailBuckets), 0, stream, ss.n_items, ss.items, k,
+                           ss.tail_hist);
+        hipLaunchKernelGGL(k_tail_offsets, dim3(1), dim3(kTailBuckets), 0, stream, ss.tail_hist);
+        hipLaunchKernelGGL(k_tail_scatter, dim3(kTailBlocks), dim3(kTailBuckets), 0, stream, ss.n_items, ss.items, k,
+                           (const int32_t*)ss.tail_hist, ss.items_tail);
+        a.items = ss.items_tail;
     }
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
     if (cv_mode == SPAC_CV_NETWORK && (desc.precision != SPAC_FP32 || coef != SPAC_COEF_VARCOEF_FRANKE))
@@ -1318,6 +1452,7 @@ spac_status spac_fk_problem_create(const spac_problem_desc* desc, spac_problem**
     p->launch.block_size = 256;
     p->launch.blocks_per_cu = 8;
     p->launch.chunk_paths = 64; /* paths per work item: smaller items balance the tail of heavy nodes */
+    p->launch.item_order = SPAC_ORDER_AUTO;
     /* set-up of the no-allocation entry point on this (current) device */
     p->scratch_status = noalloc_scratch_address(p->device, &p->scratch);
     *out = reinterpret_cast<spac_problem*>(p);
@@ -1353,8 +1488,8 @@ spac_status spac_fk_set_launch(spac_problem* hp, const spac_launch_config* cfg) 
         if (cfg->chunk_paths < 1) return set_error(SPAC_E_INVALID_ARG, "chunk_paths must be >= 1");
         p->launch.chunk_paths = cfg->chunk_paths;
     }
-    if (cfg->item_order != SPAC_ORDER_NODE && cfg->item_order != SPAC_ORDER_COST)
-        return set_error(SPAC_E_INVALID_ARG, "item_order must be SPAC_ORDER_NODE or SPAC_ORDER_COST");
+    if (cfg->item_order < SPAC_ORDER_NODE || cfg->item_order > SPAC_ORDER_AUTO)
+        return set_error(SPAC_E_INVALID_ARG, "item_order must be one of SPAC_ORDER_NODE, _COST, _TAIL, _AUTO");
     p->launch.item_order = cfg->item_order;
     return SPAC_OK;
 }
@@ -1418,7 +1553,7 @@ static spac_status solve_points_impl(Problem* p, const double* d_x0, const doubl
     ss.chunk = work_item_paths((uint32_t)p->launch.chunk_paths, ss.total_paths);
     ss.chunks_per_solve = (int32_t)((paths + ss.chunk - 1) / ss.chunk);
     int64_t items = (int64_t)ss.chunks_per_solve * n;
-    if (items > 0x7fffffff) return set_error(SPAC_E_INVALID_ARG, "too many work items (%lld)", (long long)items);
+    if (items > kMaxWorkItems) return set_error(SPAC_E_INVALID_ARG, "too many work items (%lld)", (long long)items);
     ss.n_items = (int32_t)items;
     spac_status rc = p->ensure_solve_scratch(n, &ss);
     if (rc) return rc;

@@ -576,7 +576,7 @@ spac_status Problem::device_solves_uniform(const DeviceSystem* sys, uint32_t pat
     const uint32_t chunk = work_item_paths((uint32_t)launch.chunk_paths, total);
     const int64_t cps = ((int64_t)paths + chunk - 1) / chunk;
     const int64_t n_items = cps * (int64_t)sys->n_live;
-    if (n_items > 0x7fffffff) return set_error(SPAC_E_INVALID_ARG, "too many work items");
+    if (n_items > kMaxWorkItems) return set_error(SPAC_E_INVALID_ARG, "too many work items");
     const size_t nb = (size_t)std::max(n, 1);
     spac_status rc;
     if ((rc = sx.ensure(nb * 8)) || (rc = sy.ensure(nb * 8)) || (rc = sh.ensure(nb * 8)) ||

@@ -115,7 +115,7 @@ class FKProblem:
         except Exception:
             pass
 
-    def set_launch(self, block_size=0, blocks_per_cu=0, chunk_paths=0, item_order=abi.SPAC_ORDER_NODE):
+    def set_launch(self, block_size=0, blocks_per_cu=0, chunk_paths=0, item_order=abi.SPAC_ORDER_AUTO):
         cfg = abi.LaunchConfig(block_size, blocks_per_cu, chunk_paths, item_order)
         abi.check(self.lib.spac_fk_set_launch(self.h, C.byref(cfg)), self.lib)
 

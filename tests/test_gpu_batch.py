@@ -89,7 +89,8 @@ def test_item_order_and_launch_shape_do_not_change_results(gpu, batch):
     with FKProblem(desc) as p:
         ref = p.assemble_rows(m, paths, h, seed=7)
         for cfg in ((64, 2, 32, abi.SPAC_ORDER_COST), (128, 16, 1024, abi.SPAC_ORDER_NODE),
-                    (256, 4, 100, abi.SPAC_ORDER_COST)):
+                    (256, 4, 100, abi.SPAC_ORDER_COST), (64, 4, 16, abi.SPAC_ORDER_TAIL),
+                    (256, 8, 64, abi.SPAC_ORDER_NODE)):
             say(f"launch {cfg}")
             p.set_launch(*cfg)
             _same_rows(p.assemble_rows(m, paths, h, seed=7), ref)

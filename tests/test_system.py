@@ -108,6 +108,28 @@ def test_gpu_system_uniform_equals_node_batch(gpu, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [abi.SPAC_FP64, abi.SPAC_FP32])
+def test_gpu_system_uniform_item_orders_equal(gpu, precision):
+    """The tail-first queue order (items of the solves with the largest R^2 / (sigma(X0)^2 h)
+    first, sorted on the device; the default at this size) against node order, against explicit
+    tail order with small items (many more queue entries to sort) and against cost order: the same
+    system and estimators bit for bit."""
+    m = Mesh(mesh_desc(n_side=16, nodes_per_circle=40))
+    say(f"item orders: precision {precision}, {m.info.n_jobs} jobs")
+    with FKProblem(problem_desc(coef=VC, domain=DOM, precision=precision)) as p, DeviceSystem(m) as s:
+        st = p.assemble_system_uniform(s, 150, 2e-4, seed=3)
+        ref = s.read()
+        for cfg in ((0, 0, 0, abi.SPAC_ORDER_NODE), (0, 0, 16, abi.SPAC_ORDER_TAIL), (64, 2, 0, abi.SPAC_ORDER_COST)):
+            p.set_launch(*cfg)
+            st2 = p.assemble_system_uniform(s, 150, 2e-4, seed=3)
+            for a, b in zip(ref, s.read()):
+                assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), cfg
+            for f in st.dtype.names:
+                a, b = np.ascontiguousarray(st[f]), np.ascontiguousarray(st2[f])
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (cfg, f)
+
+
+@pytest.mark.gpu
 def test_gpu_system_solve_equals_gmres_on_copied_csr(gpu):
     m = Mesh(mesh_desc())
     say(f"device system solve: {m.info.n_jobs} jobs")
