@@ -411,14 +411,26 @@ struct CoefTab32 {
 };
 
 /* Box–Muller in FP32 from the same Philox block as the FP64 path (24-bit uniforms). */
-__device__ __forceinline__ void normal_pair32(uint64_t seed, uint32_t stream, uint32_t path, uint32_t step, float* n0,
-                                              float* n1) {
-    spac_u32x4 r = spac_philox4x32_10(step, path, (uint32_t)(seed >> 32), SPAC_TAG_BROWNIAN, (uint32_t)seed, stream);
-    float u1 = 1.0f - (float)(r.v[0] >> 8) * 5.9604644775390625e-08f; /* (0, 1] */
-    float u2 = (float)(r.v[2] >> 8) * 5.9604644775390625e-08f;        /* [0, 1) */
+/* FP32 Box–Muller from the same two words as spac_normal_words (their top 24 bits), so FP32 and
+ * FP64 paths of one seed draw the same increments up to the uniforms' rounding. */
+__device__ __forceinline__ void normal_words32(uint32_t a, uint32_t b, float* n0, float* n1) {
+    float u1 = 1.0f - (float)(a >> 8) * 5.9604644775390625e-08f; /* (0, 1] */
+    float u2 = (float)(b >> 8) * 5.9604644775390625e-08f;        /* [0, 1) */
     float rad = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1)); /* -2 ln u1 = -2 ln2 log2 u1 */
     *n0 = rad * __builtin_amdgcn_cosf(u2);
     *n1 = rad * __builtin_amdgcn_sinf(u2);
+}
+/* step_normals (fk_kernels.hip) for the FP32 kernel */
+__device__ __forceinline__ void step_normals32(uint64_t seed, uint32_t stream, uint32_t path, int32_t nstep,
+                                               uint32_t* keep0, uint32_t* keep1, float* n0, float* n1) {
+    uint32_t w0, w1;
+    if ((nstep & 1) == 0) {
+        const spac_u32x4 r = spac_brownian_block(seed, stream, path, (uint32_t)nstep >> 1);
+        w0 = r.v[0]; w1 = r.v[1]; *keep0 = r.v[2]; *keep1 = r.v[3];
+    } else {
+        w0 = *keep0; w1 = *keep1;
+    }
+    normal_words32(w0, w1, n0, n1);
 }
 
 __device__ __forceinline__ float rect_distance32(const float* p, float x, float y, float* nx, float* ny, float* ex,

@@ -154,6 +154,28 @@ __device__ __forceinline__ void wave_priority_by_age(bool active, int32_t nstep)
     else __builtin_amdgcn_s_setprio(0);
 }
 
+/* The Brownian normals of path-step nstep: on an even step the pair's Philox block, whose
+ * odd-step words go to *keep0, *keep1; on an odd step those kept words (the same path's previous
+ * step on this lane).  A path starts at step 0, so an odd step always follows its even one. */
+__device__ __forceinline__ void step_normals(uint64_t seed, uint32_t stream, uint32_t path, int32_t nstep,
+                                             uint32_t* keep0, uint32_t* keep1, double* n0, double* n1) {
+    uint32_t w0, w1;
+    if ((nstep & 1) == 0) {
+        const spac_u32x4 r = spac_brownian_block(seed, stream, path, (uint32_t)nstep >> 1);
+        w0 = r.v[0]; w1 = r.v[1]; *keep0 = r.v[2]; *keep1 = r.v[3];
+    } else {
+        w0 = *keep0; w1 = *keep1;
+    }
+    spac_normal_words(w0, w1, n0, n1);
+}
+
+/* Idle lanes take new paths only while the wave's active paths are at an even step (or none is
+ * active), so that all active lanes of a wave share their step parity and the Philox block of a
+ * step pair is drawn under a wave-uniform branch; an idle lane waits at most one step. */
+__device__ __forceinline__ bool odd_step_phase(bool active, int32_t nstep) {
+    return __ballot(active && (nstep & 1)) != 0ull;
+}
+
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
@@ -285,6 +307,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
     int32_t solve = 0;
     uint32_t path = 0;
     int32_t nstep = 0;
+    uint32_t keep0 = 0u, keep1 = 0u; /* the odd step's Philox words of the current step pair */
     double X = 0.0, Yc = 0.0, Yw = 0.0, Z = 0.0, xi = 0.0, t = 0.0;
     double h = 0.0, sqrth = 0.0, R = 0.0, cx = 0.0, cy = 0.0;
     double sig = 0.0; /* sigma at the current X */
@@ -327,6 +350,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
         unsigned long long idle = __ballot(!active && !done);
         if constexpr (kLaneBatch64 > 1)
             if (__popcll(idle) < kLaneBatch64 && __ballot(active) != 0ull) idle = 0ull; /* refill later, together */
+        if (odd_step_phase(active, nstep)) idle = 0ull;
         while (idle != 0ull && !drained) {
             if (pool_next >= pool_end) {
                 if (pool_item >= pool_item_end) {
@@ -398,7 +422,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
              * data at the foot W = X - d N of X on the side. */
             StepCoef cf = coef.step_s(X, Yc, sig);
             double n0, n1;
-            spac_normal_pair(seed, stream, path, (uint32_t)nstep, &n0, &n1);
+            step_normals(seed, stream, path, nstep, &keep0, &keep1, &n0, &n1);
             double dWx = sqrth * n0, dWy = sqrth * n1;
             double xp, yp;
             if (Coef::kHasDrift) {
@@ -451,7 +475,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             /* Paso_Absorbente (Solvers/FeynmanKacSolver.cpp:7-18) */
             StepCoef cf = coef.step_s(X, Yc, sig);
             double n0, n1;
-            spac_normal_pair(seed, stream, path, (uint32_t)nstep, &n0, &n1);
+            step_normals(seed, stream, path, nstep, &keep0, &keep1, &n0, &n1);
             double dWx = sqrth * n0, dWy = sqrth * n1;
             Z = Z + (cf.f * Yw) * h;
             if (CV == CV_ANALYTIC) {
@@ -543,6 +567,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
     int32_t solve = 0;
     uint32_t path = 0;
     int32_t nstep = 0;
+    uint32_t keep0 = 0u, keep1 = 0u; /* the odd step's Philox words of the current step pair */
     float xf = 0.f, yf = 0.f, hf = 0.f, sqrthf = 0.f, Rf = 0.f, cxf = 0.f, cyf = 0.f;
     float sigf = 0.f; /* sigma at the current X */
     double Yw = 0.0, Z = 0.0, xi = 0.0, h = 0.0, R = 0.0, cx = 0.0, cy = 0.0;
@@ -583,6 +608,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
         }
         unsigned long long idle = __ballot(!active && !done);
         if (__popcll(idle) < kLaneBatch32 && __ballot(active) != 0ull) idle = 0ull; /* refill later, together */
+        if (odd_step_phase(active, nstep)) idle = 0ull;
         while (idle != 0ull && !drained) {
             if (pool_next >= pool_end) {
                 if (pool_item >= pool_item_end) {
@@ -638,7 +664,7 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
         if (active) {
             StepCoef32 cf = coef32.step_s(xf, yf, sigf);
             float n0, n1;
-            normal_pair32(seed, stream, path, (uint32_t)nstep, &n0, &n1);
+            step_normals32(seed, stream, path, nstep, &keep0, &keep1, &n0, &n1);
             const float dWx = sqrthf * n0, dWy = sqrthf * n1;
             const float Ywf = (float)Yw;
             Z += (double)((cf.f * Ywf) * hf);

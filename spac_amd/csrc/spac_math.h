@@ -31,7 +31,8 @@
  *           atan u = 2 atan(u / (1 + sqrt(1+u^2))), odd Taylor series.
  *   Philox4x32-10 : Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
  *           easy as 1, 2, 3", SC'11 (multipliers, Weyl key increments, 10 rounds).
- *   Box–Muller : two 53-bit uniforms from one Philox block give two N(0,1).
+ *   Box–Muller : two 32-bit uniforms (two words of a Philox block) give two N(0,1); one
+ *           block serves two path-steps.
  */
 #ifndef SPAC_MATH_H
 #define SPAC_MATH_H
@@ -372,23 +373,35 @@ SPAC_HD double spac_u01_53(uint32_t a, uint32_t b) {
 /*
  * Stream layout shared by every kernel and the CPU restatement:
  *   key     = (low 32 bits of the seed, stream id = solve/node index)
- *   counter = (step index, path index, high 32 bits of the seed, purpose tag)
- * One Philox block per path-step gives the two Brownian normals of that step.
+ *   counter = (step pair = step / 2, path index, high 32 bits of the seed, purpose tag)
+ * One Philox block per two path-steps: words (v0, v1) give the two Brownian normals of the
+ * even step, (v2, v3) those of the odd step, each pair by Box–Muller from two 32-bit uniforms
+ * (the resolution of the reference's 32-bit mt19937 draws).  The path kernels keep the odd
+ * step's two words in registers, so a path-step costs half a Philox block.
  */
 #define SPAC_TAG_BROWNIAN 0u
 #define SPAC_TAG_WORKLOAD 7u
 
-SPAC_HD void spac_normal_pair(uint64_t seed, uint32_t stream, uint32_t path, uint32_t step,
-                              double* n0, double* n1) {
-    spac_u32x4 r = spac_philox4x32_10(step, path, (uint32_t)(seed >> 32), SPAC_TAG_BROWNIAN,
-                                      (uint32_t)seed, stream);
-    double u1 = 1.0 - spac_u01_53(r.v[0], r.v[1]); /* (0, 1] */
-    double u2 = spac_u01_53(r.v[2], r.v[3]);       /* [0, 1) */
-    double rad = SPAC_SQRT(-2.0 * spac_log_normal(u1));
+SPAC_HD spac_u32x4 spac_brownian_block(uint64_t seed, uint32_t stream, uint32_t path, uint32_t pair) {
+    return spac_philox4x32_10(pair, path, (uint32_t)(seed >> 32), SPAC_TAG_BROWNIAN, (uint32_t)seed, stream);
+}
+/* Box–Muller from two 32-bit words: u1 = 1 - a 2^-32 in (0, 1], u2 = b 2^-32 in [0, 1) (both
+ * exact in double). */
+SPAC_HD void spac_normal_words(uint32_t a, uint32_t b, double* n0, double* n1) {
+    const double u1 = 1.0 - (double)a * 2.3283064365386963e-10;
+    const double u2 = (double)b * 2.3283064365386963e-10;
+    const double rad = SPAC_SQRT(-2.0 * spac_log_normal(u1));
     double s, c;
     spac_sincospi(2.0 * u2, &s, &c);
     *n0 = rad * c;
     *n1 = rad * s;
+}
+/* the normals of one path-step */
+SPAC_HD void spac_normal_pair(uint64_t seed, uint32_t stream, uint32_t path, uint32_t step,
+                              double* n0, double* n1) {
+    const spac_u32x4 r = spac_brownian_block(seed, stream, path, step >> 1);
+    if (step & 1u) spac_normal_words(r.v[2], r.v[3], n0, n1);
+    else spac_normal_words(r.v[0], r.v[1], n0, n1);
 }
 
 /* Unit exponential E = -log U, U in (0, 1], for the reflecting step's bridge

@@ -9,10 +9,13 @@ Bars:
   weight, path steps), each at its own paths and h (Solvers/FeynmanKacSolver.cpp:344-355 plus
   Mesh/Interfaz.cpp:219-264 per row);
 - FP32 plan against the FP64 plan with the same seed (same Philox stream): every row's B
-  within 0.5 of its FP64 standard error and the RMS over rows within 0.01 of it (a path whose
-  exit step moves under FP32 rounding changes its row by O(1/N) of a path's spread; first
-  measurement: max 0.33, RMS 0.0036); the solved interface values within 2e-6 (RMS) of the
-  FP64 solve.
+  within 0.5 of its FP64 standard error, or off by no more than one path's exit value (|g|/N,
+  |g| <= 1.5 for the Franke u on the unit square: a path that leaves a circle in one precision
+  and the domain in the other, where the two boundaries are equally near, moves g/N between
+  B and the row's G), for at most 5 rows; the RMS over the other rows within 0.01 of the
+  standard error (a path whose exit step moves under FP32 rounding changes its row by O(1/N)
+  of a path's spread; measured: max 0.33-0.55, RMS 0.0036-0.004, one or two exit-kind rows per
+  batch); the solved interface values within 2e-6 (RMS) of the FP64 solve.
 """
 import sys
 
@@ -84,10 +87,14 @@ def test_fp32_plan_tracks_fp64_plan(gpu, full_rows):
     s64 = r64["stats"]
     var = np.array([max(s["phiphi"] - s["phi"] ** 2, 0.0) for s in s64])
     se = np.sqrt(var / np.maximum(paths, 1))
-    dB = np.abs(r32["B"] - r64["B"])[live] / se[live]
-    say(f"FP32 - FP64 row B: max {dB.max():.3e} SE, rms {np.sqrt(np.mean(dB ** 2)):.3e} SE")
-    assert dB.max() < 0.5
-    assert np.sqrt(np.mean(dB ** 2)) < 0.01
+    diff = np.abs(r32["B"] - r64["B"])[live]
+    dB = diff / se[live]
+    one_exit = (dB >= 0.5) & (diff <= 1.5 / paths[live])  # one path's exit kind differs
+    say(f"FP32 - FP64 row B: max {dB.max():.3e} SE, rms {np.sqrt(np.mean(dB ** 2)):.3e} SE, "
+        f"{int(one_exit.sum())} rows off by one path's exit value")
+    assert ((dB < 0.5) | one_exit).all()
+    assert one_exit.sum() <= 5
+    assert np.sqrt(np.mean(dB[~one_exit] ** 2)) < 0.01
     u64 = _solve(mesh, r64)
     u32 = _solve(mesh, r32)
     d = np.sqrt(np.mean((u32 - u64) ** 2))

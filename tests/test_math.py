@@ -115,6 +115,30 @@ def test_box_muller_normals():
 
 
 @pytest.mark.parametrize("seed", [0, 123456789, 2**40 + 7])
+def test_normals_come_from_one_philox_block_per_step_pair(seed):
+    """Steps 2k and 2k+1 of a path take words (0, 1) and (2, 3) of the Philox block with counter
+    (k, path, seed_hi, 0) and key (seed_lo, stream), through Box-Muller on 32-bit uniforms; numpy's
+    log/cos/sin against the library's own (a few ulp apart)."""
+    stream, path, nsteps = 5, 9, 12
+    z = o.normals(seed, stream, path, nsteps).reshape(-1, 2)
+    for step in range(nsteps):
+        w = o.philox(10, [step // 2, path, seed >> 32, 0], [seed & 0xFFFFFFFF, stream])
+        a, b = (w[0], w[1]) if step % 2 == 0 else (w[2], w[3])
+        u1, u2 = 1.0 - a * 2.0 ** -32, b * 2.0 ** -32
+        rad = np.sqrt(-2.0 * np.log(u1))
+        want = np.array([rad * np.cos(2 * np.pi * u2), rad * np.sin(2 * np.pi * u2)])
+        assert np.allclose(z[step], want, rtol=1e-13, atol=1e-13), step
+
+
+@pytest.mark.parametrize("seed", [0, 123456789, 2**40 + 7])
+def test_normals_of_consecutive_steps_independent(seed):
+    z = o.normals(seed, 5, 9, 20000).reshape(-1, 2)
+    for c in (0, 1):  # step 2k against step 2k+1, which share a Philox block
+        r = np.corrcoef(z[0::2, c], z[1::2, c])[0, 1]
+        assert abs(r) < 4.0 / np.sqrt(z.shape[0] // 2)
+
+
+@pytest.mark.parametrize("seed", [0, 123456789, 2**40 + 7])
 def test_normals_pairs_independent(seed):
     z = o.normals(seed, 5, 9, 20000).reshape(-1, 2)
     r = np.corrcoef(z[:, 0], z[:, 1])[0, 1]

@@ -220,9 +220,29 @@ def test_estimator_within_3_sigma(franke_run):
         assert abs(s["phi"] - u[k]) < 3 * se + 5e-3, (k, s["phi"], u[k], se)
         var_vr = s["phiphi_VR"] - s["phi_VR"] ** 2
         se_vr = math.sqrt(var_vr / s["paths"])
-        # CV estimator: much smaller variance; bias of the Euler/Gobet scheme dominates
+        # CV estimator: much smaller variance; the bias of the Euler/Gobet scheme dominates, up
+        # to 0.020 at h = 1e-3 (grid point 170, mean over 8 seeds; it is O(h): see below)
         assert var_vr < 0.25 * var
-        assert abs(s["phi_VR"] - u[k]) < 3 * se_vr + 5e-3
+        assert abs(s["phi_VR"] - u[k]) < 3 * se_vr + 0.025
+
+
+def test_estimator_bias_is_first_order_in_h():
+    """At the grid point of the largest control-variate error (170) the error is the scheme's:
+    it falls by about 4 when h falls by 4 (mean over 4 seeds of 4000 paths)."""
+    gx, gy = grid_points()
+    idx = np.array([170])
+    u = franke_published(gx[idx], gy[idx])[0]
+
+    def bias(h):
+        e = []
+        for seed in range(4):
+            b = SolveBatchArrays(gx[idx], gy[idx], 4000, h, idx)
+            e.append(o.reduce(b, o.simulate(problem_desc(), b, seed=seed))[0]["phi_VR"] - u)
+        return float(np.mean(e))
+
+    b1, b4 = bias(1e-3), bias(2.5e-4)
+    assert 0.012 < b1 < 0.03
+    assert 2.5 < b1 / b4 < 6.0
 
 
 def test_exit_time_against_closed_form(franke_run):
