@@ -370,7 +370,7 @@ def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep,
     """Path count per row at which this method's own replica set measures RMS <= target.
 
     A probe set at probe_paths, then two sets, each at the count where the fit over every set
-    so far gives 0.9 x target, then further sets (at most max_extra) while the last set's own
+    so far gives 0.95 x target, then further sets (at most max_extra) while the last set's own
     RMS is above the target, each at the larger of that fit's count and 1.05 x the last count.
     The last set (its replicas and timings) is what the bootstrap and the projection use."""
     sets, hist = [], []
@@ -391,10 +391,10 @@ def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep,
         if it >= 2 + max_extra:
             break
         if it <= 1:
-            paths = _fit_paths(sets, 0.9 * target)[0]
+            paths = _fit_paths(sets, 0.95 * target)[0]
         else:
             # the fit over every set so far (one set's RMS scatters by ~12 %), and at least 5 % more
-            paths = max(_fit_paths(sets, 0.9 * target)[0], int(np.ceil(paths * 1.05)))
+            paths = max(_fit_paths(sets, 0.95 * target)[0], int(np.ceil(paths * 1.05)))
         reps, it = rep, it + 1
     return {"paths": int(paths), "U": U, "recs": recs, "rms": rms, "history": hist,
             "reached_target": bool(rms <= target)}

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstddef>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -175,6 +176,19 @@ __device__ __forceinline__ void step_normals(uint64_t seed, uint32_t stream, uin
 __device__ __forceinline__ bool odd_step_phase(bool active, int32_t nstep) {
     return __ballot(active && (nstep & 1)) != 0ull;
 }
+
+/* A kernel argument read where it is used.  The kernarg segment pointer passes through an empty
+ * asm, so the scalar load stays at the use and the value does not occupy scalar registers across
+ * the path loop (its scalar registers are full: a value held there is spilled to VGPR lanes and
+ * costs two VALU instructions per use).  PathArgs is the kernels' only argument, at offset 0. */
+template <class T>
+__device__ __forceinline__ T late_arg(size_t offset) {
+    const __attribute__((address_space(4))) char* k =
+        (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(k));
+    return *(const __attribute__((address_space(4))) T*)(k + offset);
+}
+#define SPAC_LATE_ARG(field) late_arg<decltype(PathArgs::field)>(offsetof(PathArgs, field))
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -377,21 +391,26 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
             if (!active && !done && rank < avail) {
                 solve = pool_solve;
                 path = pool_next + rank;
-                h = a.h ? a.h[solve] : a.h_uniform;
-                sqrth = a.sqrth ? a.sqrth[solve] : a.sqrth_uniform;
-                stream = a.stream ? a.stream[solve] : (uint32_t)solve;
-                rec = a.rec_off ? a.rec_off[solve] : (int64_t)solve * (int64_t)a.paths_uniform;
+                const double* ah = SPAC_LATE_ARG(h);
+                const double* asq = SPAC_LATE_ARG(sqrth);
+                const uint32_t* ast = SPAC_LATE_ARG(stream);
+                const int64_t* aoff = SPAC_LATE_ARG(rec_off);
+                h = ah ? ah[solve] : SPAC_LATE_ARG(h_uniform);
+                sqrth = asq ? asq[solve] : SPAC_LATE_ARG(sqrth_uniform);
+                stream = ast ? ast[solve] : (uint32_t)solve;
+                rec = aoff ? aoff[solve] : (int64_t)solve * (int64_t)SPAC_LATE_ARG(paths_uniform);
                 if (SUB == SUB_CIRCLE) {
-                    if (a.circle) {
-                        R = a.circle[3 * solve]; cx = a.circle[3 * solve + 1]; cy = a.circle[3 * solve + 2];
+                    const double* ac = SPAC_LATE_ARG(circle);
+                    if (ac) {
+                        R = ac[3 * solve]; cx = ac[3 * solve + 1]; cy = ac[3 * solve + 2];
                     } else {
                         R = a.circle_uniform[0]; cx = a.circle_uniform[1]; cy = a.circle_uniform[2];
                     }
                 }
-                X = a.x0[solve];
-                Yc = a.y0[solve];
+                X = SPAC_LATE_ARG(x0)[solve];
+                Yc = SPAC_LATE_ARG(y0)[solve];
                 /* sigma(X0), evaluated once per solve before the launch */
-                sig = Coef::kConstSigma ? coef.sigma(X, Yc) : a.sig0[solve];
+                sig = Coef::kConstSigma ? coef.sigma(X, Yc) : SPAC_LATE_ARG(sig0)[solve];
                 if constexpr (REFL) {
                     /* initial Dentro (:166): a start beyond a reflecting side is projected */
                     bool dom0;
@@ -537,14 +556,14 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp64(PathArgs a) {
                     done_ey = hit.ey;
                 } else {
                     int64_t g = rec + (int64_t)path;
-                    a.xt[g] = hit.ex;
-                    a.yt[g] = hit.ey;
-                    a.Y[g] = Yw;
-                    a.Z[g] = Z;
-                    a.xi[g] = xi;
-                    a.tau[g] = t;
-                    a.steps[g] = nstep; /* gval: fk_exit_values_kernel, lane-dense after the launch */
-                    a.outcome[g] = out;
+                    SPAC_LATE_ARG(xt)[g] = hit.ex;
+                    SPAC_LATE_ARG(yt)[g] = hit.ey;
+                    SPAC_LATE_ARG(Y)[g] = Yw;
+                    SPAC_LATE_ARG(Z)[g] = Z;
+                    SPAC_LATE_ARG(xi)[g] = xi;
+                    SPAC_LATE_ARG(tau)[g] = t;
+                    SPAC_LATE_ARG(steps)[g] = nstep; /* gval: fk_exit_values_kernel, lane-dense after the launch */
+                    SPAC_LATE_ARG(outcome)[g] = out;
                 }
                 active = false;
             }
@@ -594,14 +613,14 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
                     BoundaryHit hit = (done_out == SPAC_OUT_SUBDOMAIN) ? circle_distance(R, cx, cy, (double)xf, (double)yf)
                                                                        : rectangle_distance(a.dom, (double)xf, (double)yf);
                     int64_t g = rec + (int64_t)path;
-                    a.xt[g] = hit.ex;
-                    a.yt[g] = hit.ey;
-                    a.Y[g] = Yw;
-                    a.Z[g] = Z;
-                    a.xi[g] = xi;
-                    a.tau[g] = (double)nstep * h;
-                    a.steps[g] = nstep;
-                    a.outcome[g] = done_out;
+                    SPAC_LATE_ARG(xt)[g] = hit.ex;
+                    SPAC_LATE_ARG(yt)[g] = hit.ey;
+                    SPAC_LATE_ARG(Y)[g] = Yw;
+                    SPAC_LATE_ARG(Z)[g] = Z;
+                    SPAC_LATE_ARG(xi)[g] = xi;
+                    SPAC_LATE_ARG(tau)[g] = (double)nstep * h;
+                    SPAC_LATE_ARG(steps)[g] = nstep;
+                    SPAC_LATE_ARG(outcome)[g] = done_out;
                     done = false;
                 }
             }
@@ -635,22 +654,26 @@ __global__ void __launch_bounds__(256) fk_path_kernel_fp32(PathArgs a) {
             if (!active && !done && rank < avail) {
                 solve = pool_solve;
                 path = pool_next + rank;
-                h = a.h ? a.h[solve] : a.h_uniform;
+                const double* ah = SPAC_LATE_ARG(h);
+                const uint32_t* ast = SPAC_LATE_ARG(stream);
+                const int64_t* aoff = SPAC_LATE_ARG(rec_off);
+                h = ah ? ah[solve] : SPAC_LATE_ARG(h_uniform);
                 hf = (float)h;
                 sqrthf = __builtin_sqrtf(hf);
-                stream = a.stream ? a.stream[solve] : (uint32_t)solve;
-                rec = a.rec_off ? a.rec_off[solve] : (int64_t)solve * (int64_t)a.paths_uniform;
+                stream = ast ? ast[solve] : (uint32_t)solve;
+                rec = aoff ? aoff[solve] : (int64_t)solve * (int64_t)SPAC_LATE_ARG(paths_uniform);
                 if (SUB == SUB_CIRCLE) {
-                    if (a.circle) {
-                        R = a.circle[3 * solve]; cx = a.circle[3 * solve + 1]; cy = a.circle[3 * solve + 2];
+                    const double* ac = SPAC_LATE_ARG(circle);
+                    if (ac) {
+                        R = ac[3 * solve]; cx = ac[3 * solve + 1]; cy = ac[3 * solve + 2];
                     } else {
                         R = a.circle_uniform[0]; cx = a.circle_uniform[1]; cy = a.circle_uniform[2];
                     }
                     Rf = (float)R; cxf = (float)cx; cyf = (float)cy;
                 }
-                xf = (float)a.x0[solve];
-                yf = (float)a.y0[solve];
-                sigf = Coef32::kConstSigma ? coef32.sigma(xf, yf) : a.sig0f[solve];
+                xf = (float)SPAC_LATE_ARG(x0)[solve];
+                yf = (float)SPAC_LATE_ARG(y0)[solve];
+                sigf = Coef32::kConstSigma ? coef32.sigma(xf, yf) : SPAC_LATE_ARG(sig0f)[solve];
                 Yw = 1.0; Z = 0.0; xi = 0.0;
                 nstep = 0;
                 active = true;
