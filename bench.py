@@ -47,15 +47,15 @@ def log(msg):
 # Commands and figures: profiles/fk_path_kernel_fp64_franke_points_256x10k.md,
 # profiles/pddsparse_rows_fp64_4x4_1952rows_20k.md, profiles/learned_cv_time_to_rms_64x64_fp32.md.
 PATH_KERNEL_TRAFFIC_BYTES = {
-    "points": (87.8 * 2.0 + 553664.1) * 1024,
-    "rows": (25955.9 * 2.0 + 8102934.9) * 1024,
+    "points": (92.7 * 2.0 + 564166.6) * 1024,
+    "rows": (24920.6 * 2.0 + 7580208.8) * 1024,
 }
 # The learned-CV path kernel of the full-scale system, per path of the launch: measured at 94
-# paths per row (17,570,480 paths; FETCH_SIZE 185,228.3 KiB x 2.0, WRITE_SIZE 4,170,522.8 KiB
+# paths per row (17,570,480 paths; FETCH_SIZE 240,166.6 KiB x 2.0, WRITE_SIZE 4,360,928.6 KiB
 # x 1.0, mean of two dispatches of `--workload assemble --paths-per-row 94`); its launches in
 # the bench run at the calibrated count, so the bytes per launch are this figure times the
 # launch's paths.
-LEARNED_PATH_KERNEL_BYTES_PER_PATH = (185228.3 * 2.0 + 4170522.8) * 1024 / (186920 * 94)
+LEARNED_PATH_KERNEL_BYTES_PER_PATH = (240166.6 * 2.0 + 4360928.6) * 1024 / (186920 * 94)
 RECORD_BYTES_PER_PATH = 6 * 8 + 4 + 1  # x_tau, y_tau, Y, Z, xi, tau; steps; outcome
 COMPUTE_UNIT_64 = "VALU FP64 (no matrix-core work on this path; FP64 vector peak = dense FP64 MFMA peak = 78.6 TFLOP/s)"
 COMPUTE_UNIT_32 = "VALU FP32 (no matrix-core work in the path kernel; MI355X FP32 vector peak 157.3 TFLOP/s)"
@@ -393,7 +393,7 @@ def _calibrate(name, prob, mesh, system, h, target, probe_paths, probe_rep, rep,
         if it <= 1:
             paths = _fit_paths(sets, 0.95 * target)[0]
         else:
-            # the fit over every set so far (one set's RMS scatters by ~12 %), and at least 5 % more
+            # the fit over every set so far (one set's RMS scatters by ~10 % at 48 replicas), and at least 5 % more
             paths = max(_fit_paths(sets, 0.95 * target)[0], int(np.ceil(paths * 1.05)))
         reps, it = rep, it + 1
     return {"paths": int(paths), "U": U, "recs": recs, "rms": rms, "history": hist,
@@ -713,8 +713,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="ttr: skip the rows and batched workloads")
     ap.add_argument("--plain-steps", type=int, default=2, help="ttr: timed plain-MC solves at its count")
     ap.add_argument("--ttr-probe-paths", type=int, default=256)
-    ap.add_argument("--ttr-replicas", type=int, default=8, help="replicas of the probe set")
-    ap.add_argument("--ttr-refine-replicas", type=int, default=32, help="replicas of each later set")
+    ap.add_argument("--ttr-replicas", type=int, default=16, help="replicas of the probe set")
+    ap.add_argument("--ttr-refine-replicas", type=int, default=48, help="replicas of each later set")
     ap.add_argument("--bootstrap", type=int, default=4000)
     ap.add_argument("--train-steps", type=int, default=LEARNED_CV["steps"])
     ap.add_argument("--paths-per-row", type=int, default=90, help="assemble: paths per row")
