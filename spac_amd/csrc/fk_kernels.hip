@@ -1589,6 +1589,16 @@ spac_status spac_fk_last_timing(spac_problem* hp, spac_timing* out) {
     return SPAC_OK;
 }
 
+/* The uniform point solve's work items as a list (solve, p0, p1), so that the tail-first order
+ * can sort them (the list is what the queue would hand out without one). */
+__global__ void k_point_items(int32_t n, int32_t cps, uint32_t chunk, uint32_t paths, int4* __restrict__ items) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * cps) return;
+    const int32_t k = (int32_t)(i / cps), c = (int32_t)(i % cps);
+    const uint32_t p0 = (uint32_t)c * chunk;
+    items[i] = make_int4(k, (int)p0, (int)min(p0 + chunk, paths), 0);
+}
+
 static spac_status solve_points_impl(Problem* p, const double* d_x0, const double* d_y0, int32_t n, uint32_t paths,
                                      double h, uint64_t seed, spac_point_stats* d_stats, bool async = false) {
     SolveSet ss;
@@ -1604,8 +1614,20 @@ static spac_status solve_points_impl(Problem* p, const double* d_x0, const doubl
     int64_t items = (int64_t)ss.chunks_per_solve * n;
     if (items > kMaxWorkItems) return set_error(SPAC_E_INVALID_ARG, "too many work items (%lld)", (long long)items);
     ss.n_items = (int32_t)items;
-    spac_status rc = p->ensure_solve_scratch(n, &ss);
-    if (rc) return rc;
+    spac_status rc;
+    /* with a variable sigma the solves' exit-time tails differ: list the items so that the
+       tail-first order can put the slow ones first (a constant sigma and one h give every item
+       the same key, and the implicit order stays) */
+    const int coef = p->coef_kernels();
+    if ((coef == SPAC_COEF_VARCOEF_FRANKE || coef == kCoefTabulated) && items > 1 &&
+        (p->launch.item_order == SPAC_ORDER_TAIL || p->launch.item_order == SPAC_ORDER_AUTO)) {
+        if ((rc = p->sitems.ensure((size_t)items * sizeof(int4)))) return rc;
+        hipLaunchKernelGGL(k_point_items, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, p->stream, n,
+                           ss.chunks_per_solve, ss.chunk, paths, (int4*)p->sitems.ptr);
+        SPAC_HIP_CHECK(hipGetLastError());
+        ss.items = (const int4*)p->sitems.ptr;
+    }
+    if ((rc = p->ensure_solve_scratch(n, &ss))) return rc;
     if (async) {
         int blocks = 0;
         if ((rc = p->launch_run(ss, seed, d_stats, 0, nullptr, &blocks))) return rc;

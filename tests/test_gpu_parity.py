@@ -137,6 +137,25 @@ def test_point_solve_matches_batch_and_oracle(gpu):
     _assert_stats_equal(st, so)
 
 
+@pytest.mark.parametrize("precision", [abi.SPAC_FP64, abi.SPAC_FP32])
+def test_point_solve_item_orders_equal(gpu, precision):
+    """Point solves of the variable-coefficient operator list their work items so that the
+    tail-first order can sort them by R^2 / (sigma(X0)^2 h): the estimators are the same bits in
+    tail-first (default), node and cost order and with small items."""
+    say(f"point solve item orders, precision {precision}")
+    rng = np.random.default_rng(4)
+    x, y = rng.uniform(0.05, 0.95, 300), rng.uniform(0.05, 0.95, 300)
+    desc = problem_desc(coef=abi.SPAC_COEF_VARCOEF_FRANKE, domain=(0.0, 0.0, 1.0, 1.0), precision=precision)
+    with FKProblem(desc) as p:
+        ref, _ = p.solve_points(x, y, 700, 2e-3, seed=9, as_array=True)
+        for cfg in ((0, 0, 0, abi.SPAC_ORDER_NODE), (0, 0, 16, abi.SPAC_ORDER_TAIL), (0, 0, 0, abi.SPAC_ORDER_COST)):
+            p.set_launch(*cfg)
+            got, _ = p.solve_points(x, y, 700, 2e-3, seed=9, as_array=True)
+            for f in ref.dtype.names:
+                a, b = np.ascontiguousarray(ref[f]), np.ascontiguousarray(got[f])
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (cfg, f)
+
+
 def test_max_steps_and_status(gpu):
     say("max_steps cap")
     desc = problem_desc(max_steps=5)
