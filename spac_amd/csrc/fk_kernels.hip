@@ -787,6 +787,8 @@ struct ReduceArgs {
 };
 
 #define NSUMS 10
+/* paths whose records a lane of the reduction loads together */
+constexpr int kReducePaths = 4;
 enum { S_SCORE, S_SCORE2, S_XI, S_XI2, S_VR, S_VR2, S_XISCORE, S_RNG, S_TAU, S_B };
 
 __device__ __forceinline__ spac_ksum shfl_down_ksum(spac_ksum v, int off) {
@@ -808,29 +810,45 @@ __global__ void __launch_bounds__(64) fk_reduce_kernel(ReduceArgs a) {
 #pragma unroll
     for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_zero();
     int64_t my_steps = 0, my_max = 0, my_dom = 0;
-    for (int64_t p = lane; p < np; p += 64) {
-        const int64_t g = off + p;
-        const double Y = a.Y[g], Z = a.Z[g], xi = a.xi[g], gv = a.gval[g];
-        const int32_t st = a.steps[g];
-        const int8_t oc = a.outcome[g];
-        const double score = Z + Y * gv;
-        const double vr = score + xi;
-        double v[NSUMS];
-        v[S_SCORE] = score;
-        v[S_SCORE2] = score * score;
-        v[S_XI] = xi;
-        v[S_XI2] = xi * xi;
-        v[S_VR] = vr;
-        v[S_VR2] = vr * vr;
-        v[S_XISCORE] = xi * score;
-        v[S_RNG] = 2.0 * (double)st;
-        v[S_TAU] = a.tau[g];
-        v[S_B] = a.node_mode ? ((oc == SPAC_OUT_SUBDOMAIN) ? (Z + xi) : vr) : 0.0;
+    /* kReducePaths of the lane's paths are loaded before any of them is summed (one round
+       trip to memory per group); past the solve's end the lane reloads its first path */
+    for (int64_t p0 = lane; p0 < np; p0 += 64 * kReducePaths) {
+        double Yv[kReducePaths], Zv[kReducePaths], xv[kReducePaths], gvv[kReducePaths], tv[kReducePaths];
+        int32_t sv[kReducePaths];
+        int8_t ov[kReducePaths];
 #pragma unroll
-        for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_add(tot[k], v[k]);
-        my_steps += st;
-        my_max += (oc == SPAC_OUT_MAX_STEPS);
-        my_dom += (oc != SPAC_OUT_SUBDOMAIN);
+        for (int u = 0; u < kReducePaths; ++u) {
+            const int64_t p = p0 + 64 * u;
+            const int64_t g = off + (p < np ? p : p0);
+            Yv[u] = a.Y[g]; Zv[u] = a.Z[g]; xv[u] = a.xi[g]; gvv[u] = a.gval[g]; tv[u] = a.tau[g];
+            sv[u] = a.steps[g];
+            ov[u] = a.outcome[g];
+        }
+#pragma unroll
+        for (int u = 0; u < kReducePaths; ++u) {
+            if (p0 + 64 * u >= np) break;
+            const double Y = Yv[u], Z = Zv[u], xi = xv[u], gv = gvv[u];
+            const int32_t st = sv[u];
+            const int8_t oc = ov[u];
+            const double score = Z + Y * gv;
+            const double vr = score + xi;
+            double v[NSUMS];
+            v[S_SCORE] = score;
+            v[S_SCORE2] = score * score;
+            v[S_XI] = xi;
+            v[S_XI2] = xi * xi;
+            v[S_VR] = vr;
+            v[S_VR2] = vr * vr;
+            v[S_XISCORE] = xi * score;
+            v[S_RNG] = 2.0 * (double)st;
+            v[S_TAU] = tv[u];
+            v[S_B] = a.node_mode ? ((oc == SPAC_OUT_SUBDOMAIN) ? (Z + xi) : vr) : 0.0;
+#pragma unroll
+            for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_add(tot[k], v[k]);
+            my_steps += st;
+            my_max += (oc == SPAC_OUT_MAX_STEPS);
+            my_dom += (oc != SPAC_OUT_SUBDOMAIN);
+        }
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1)

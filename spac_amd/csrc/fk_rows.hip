@@ -171,6 +171,32 @@ __global__ void __launch_bounds__(kRowsBlock) fk_rows_g_imq_kernel(RowsArgs a) {
     }
 }
 
+/* the stencil size whose column products are unrolled at compile time */
+constexpr int kMomFastStencil = 15;
+
+/* The columns of a full circle's row from its moments (both in LDS; the circle's one S x S
+ * stencil inverse blk), each one FMA chain: G_c = -(sum over r, s = c - r mod M, of sum over i
+ * of blk[i][r] Phi[s][i]) / N.  SS > 0 fixes S at compile time, so that every LDS address is
+ * a base plus an immediate offset; SS = 0 takes the runtime S. */
+template <int SS>
+__device__ __forceinline__ void moment_columns_circle(int M, int S_rt, const double* __restrict__ blk,
+                                                      const unsigned long long* __restrict__ mom, double N,
+                                                      double* __restrict__ G) {
+    const int S = SS > 0 ? SS : S_rt;
+    for (int c = (int)threadIdx.x; c < M; c += (int)blockDim.x) {
+        double acc = 0.0;
+#pragma unroll 1
+        for (int r = 0; r < S; ++r) {
+            int s = c - r;
+            if (s < 0) s += M;
+            const unsigned long long* mrow = mom + s * S;
+#pragma unroll
+            for (int i = 0; i < S; ++i) acc = __builtin_fma(blk[i * S + r], spac_u64_as_f64(mrow[i]), acc);
+        }
+        G[c] = -acc / N;
+    }
+}
+
 /* Gaussian stencil rows in moment form (spac_rbf.h), one block per row.
  * Pass 1: the largest |Y| over the row's circle exits (a max: exact in any order) fixes the
  * fixed-point scale q.  Pass 2: thread t takes the paths t, t + BLK, ...: exit angle, stencil
@@ -252,20 +278,22 @@ __global__ void __launch_bounds__(BLK) fk_rows_g_moment_kernel(RowsArgs a) {
     for (int i = tid; i < M * S; i += BLK) s_mom[i] = spac_f64_as_u64(spac_from_fixed((int64_t)s_mom[i], q));
     __syncthreads();
     const double N = (double)np;
-    for (int c = tid; c < M; c += BLK) {
-        double acc = 0.0;
-        for (int r = 0; r < S; ++r) {
-            int s = c - r;
-            if (circle) {
-                if (s < 0) s += M;
-            } else if (s < 0 || s > M - S) {
-                continue;
+    if (circle && S == kMomFastStencil) {
+        moment_columns_circle<kMomFastStencil>(M, S, s_blk, s_mom, N, G);
+    } else if (circle) {
+        moment_columns_circle<0>(M, S, s_blk, s_mom, N, G);
+    } else {
+        for (int c = tid; c < M; c += BLK) {
+            double acc = 0.0;
+            for (int r = 0; r < S; ++r) {
+                const int s = c - r;
+                if (s < 0 || s > M - S) continue;
+                const double* blk = ipsi + (int64_t)s * S * S;
+                const unsigned long long* mrow = s_mom + s * S;
+                for (int i = 0; i < S; ++i) acc = __builtin_fma(blk[i * S + r], spac_u64_as_f64(mrow[i]), acc);
             }
-            const double* blk = circle ? s_blk : ipsi + (int64_t)s * S * S;
-            const unsigned long long* mrow = s_mom + s * S;
-            for (int i = 0; i < S; ++i) acc = __builtin_fma(blk[i * S + r], spac_u64_as_f64(mrow[i]), acc);
+            G[c] = -acc / N;
         }
-        G[c] = -acc / N;
     }
 }
 
