@@ -791,13 +791,6 @@ struct ReduceArgs {
 constexpr int kReducePaths = 4;
 enum { S_SCORE, S_SCORE2, S_XI, S_XI2, S_VR, S_VR2, S_XISCORE, S_RNG, S_TAU, S_B };
 
-__device__ __forceinline__ spac_ksum shfl_down_ksum(spac_ksum v, int off) {
-    spac_ksum r;
-    r.s = __shfl_down(v.s, off, 64);
-    r.c = __shfl_down(v.c, off, 64);
-    return r;
-}
-
 __global__ void __launch_bounds__(64) fk_reduce_kernel(ReduceArgs a) {
     const int solve = blockIdx.x;
     const int lane = threadIdx.x;
@@ -850,10 +843,37 @@ __global__ void __launch_bounds__(64) fk_reduce_kernel(ReduceArgs a) {
             my_dom += (oc != SPAC_OUT_SUBDOMAIN);
         }
     }
+    /* The fold (lane l += lane l + off, off = 32, ..., 1) of every sum, through LDS: a level's
+       NSUMS x off merges are spread over the wave's lanes instead of every lane merging every
+       sum at every level.  The same pairs are merged in the same order, so the totals are those
+       of a shuffle tree; a merge reads slots l and l + off and writes slot l, and no other merge
+       of its level touches either. */
+    __shared__ double s_fold[2][NSUMS][64];
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
+    for (int k = 0; k < NSUMS; ++k) {
+        s_fold[0][k][lane] = tot[k].s;
+        s_fold[1][k][lane] = tot[k].c;
+    }
+    __syncthreads();
 #pragma unroll
-        for (int k = 0; k < NSUMS; ++k) tot[k] = spac_ksum_merge(tot[k], shfl_down_ksum(tot[k], o));
+    for (int lg = 5; lg >= 0; --lg) {
+        const int off = 1 << lg;
+        for (int m = lane; m < NSUMS * off; m += 64) {
+            const int k = m >> lg, l = m & (off - 1);
+            spac_ksum x, y;
+            x.s = s_fold[0][k][l]; x.c = s_fold[1][k][l];
+            y.s = s_fold[0][k][l + off]; y.c = s_fold[1][k][l + off];
+            const spac_ksum r = spac_ksum_merge(x, y);
+            s_fold[0][k][l] = r.s;
+            s_fold[1][k][l] = r.c;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NSUMS; ++k) {
+        tot[k].s = s_fold[0][k][0];
+        tot[k].c = s_fold[1][k][0];
+    }
     /* exact integer totals */
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
